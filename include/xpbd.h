@@ -1,0 +1,169 @@
+/*
+ * xpbd.h — C ABI of the MI355X-native XPBD muscle / soft-tissue substep.
+ *
+ * Drop-in for the substep of XpbdEngine under MuscleSim / MultiGridSim in
+ * chunleili/rlmuscleV2.  Every entry point is plain C: opaque context, plain
+ * pointers and sizes, int32 status (0 = ok, see XPBD_E*), the message of the
+ * last failure from xpbd_last_error().  No exceptions cross the boundary.
+ *
+ * All host arrays are in the caller's vertex / tet / constraint order; the
+ * library keeps its own device layout (vertices grouped into patches, tets
+ * replicated per patch) and converts on upload / download.
+ *
+ * Each declaration names the reference interface it replaces.
+ */
+#ifndef XPBD_H
+#define XPBD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XPBD_OK           0
+#define XPBD_EINVAL       1   /* bad argument / size / state order            */
+#define XPBD_EUNSUPPORTED 2   /* constraint type or mode not on this path      */
+#define XPBD_EOOM         3   /* device allocation failed                      */
+#define XPBD_EHIP         4   /* HIP runtime error                             */
+#define XPBD_ELAYOUT      5   /* mesh does not fit the patch layout limits     */
+#define XPBD_ENOTREADY    6   /* xpbd_finalize() not called yet                */
+
+/* constraint type tags — src/muscle/defs.py:20-36 (same integer values) */
+#define XPBD_CT_ATTACH       1650556
+#define XPBD_CT_PINTOTARGET  771624311
+#define XPBD_CT_TETVOLUME    (-215389979)
+#define XPBD_CT_TETARAPSVD   (-92199132)
+#define XPBD_CT_TETARAPANISO (-92199133)
+#define XPBD_CT_TETSNH       (-503462311)
+
+typedef struct xpbd_ctx xpbd_ctx;
+
+/* Engine scalars.  Reference: XpbdEngine.integrate / update_velocities
+ * (src/muscle/engine.py:798-829), MuscleSim.__init__ (src/muscle/musclesim.py:72-82),
+ * MultiGridSim.__init__ (src/muscle/mg/mg.py:95-187). */
+typedef struct {
+    float dt;                     /* substep dt = delta_t / nsubsteps            */
+    float gravity[3];             /* _gravity_vec, or (0, cfg.gravity, 0)        */
+    float veldamping;             /* cfg.veldamping                              */
+    int32_t quasi_static;         /* cfg.quasi_static                            */
+    float contraction_ratio;      /* aniso_arap active shortening ratio          */
+    float fiber_stiffness_scale;  /* aniso_arap stiffness scale                  */
+    int32_t has_compress;         /* HAS_compressstiffness                       */
+    float exp_damp, const_damp;   /* Gaia velocity damping (1, 0 = off)          */
+} xpbd_params;
+
+/* Layout / launch tuning, filled by the autotuner or by hand.  0 = default. */
+typedef struct {
+    int32_t patch_verts;          /* owned vertices per patch = threads per block (64..256) */
+    int32_t reserved[7];
+} xpbd_tuning;
+
+typedef struct {
+    int32_t nv, nt, n_elems, n_patches;
+    int32_t n_patch_tets;         /* sum over patches of replicated tets          */
+    int32_t max_table;            /* largest patch vertex table (owned + halo)    */
+    int32_t max_patch_tets;
+    int32_t feature_mask;         /* bit0 arap, bit1 volume, bit2 snh, bit3 aniso, bit4 damping, bit5 attach, bit6 pins */
+    double  bytes_per_iter_alg;   /* algorithmic bytes of one Jacobi iteration   */
+    double  bytes_per_iter_impl;  /* bytes the projection kernel actually moves  */
+} xpbd_stats;
+
+/* ---- lifetime --------------------------------------------------------- */
+/* replaces wp.init()/device selection (src/muscle/musclesim.py:44-86) */
+int32_t xpbd_create(int32_t device, xpbd_ctx **out);
+void    xpbd_destroy(xpbd_ctx *ctx);
+const char *xpbd_last_error(const xpbd_ctx *ctx);
+/* stream: a hipStream_t (NULL = the context's own stream) */
+int32_t xpbd_set_stream(xpbd_ctx *ctx, void *stream);
+
+/* ---- problem definition ---------------------------------------------- */
+/* MuscleSim._allocate_fields/_init_fields + precompute_rest_kernel
+ * (src/muscle/musclesim.py:331-352,490-543; src/muscle/engine.py:199-224):
+ * rest positions and tets; computes B = inv(Dm), |V|, lumped mass with density. */
+int32_t xpbd_set_mesh(xpbd_ctx *ctx, int32_t nv, const float *pos0, int32_t nt,
+                      const int32_t *tets, float density);
+/* read back what xpbd_set_mesh computed (any pointer may be NULL) */
+int32_t xpbd_get_rest(xpbd_ctx *ctx, float *rest_volume, float *rest_matrix, float *mass);
+/* MuscleSim mass / stopped fields (engine.py:431-464 reads them) */
+int32_t xpbd_set_mass(xpbd_ctx *ctx, const float *mass);
+int32_t xpbd_set_stopped(xpbd_ctx *ctx, const int32_t *stopped);
+int32_t xpbd_set_force(xpbd_ctx *ctx, const float *force);
+/* v_fiber_dir (nv*3) and per-tet tendonmask (nt); musclesim.py:490-512 */
+int32_t xpbd_set_fiber(xpbd_ctx *ctx, const float *v_fiber_dir, const float *tendonmask);
+/* per-tet activation (nt); MultiGridSim.set_fiber_activation (mg.py:1098-1111) */
+int32_t xpbd_set_activation(xpbd_ctx *ctx, const float *activation);
+/* the SoA constraint table of XpbdEngine._upload_cons (engine.py:1179-1211);
+ * pts n*4, restvector n*4, restdir n*3, others n. */
+int32_t xpbd_upload_constraints(xpbd_ctx *ctx, int32_t n, const int32_t *type,
+                                const int32_t *pts, const float *stiffness,
+                                const float *compressionstiffness, const float *dampingratio,
+                                const int32_t *tetid, const float *restlength,
+                                const float *restvector, const float *restdir);
+/* ATTACH targets refreshed per frame, in table order of the ATTACH range
+ * (MultiGridSim._update_belly_attach_targets, mg.py:725-734); xyz n*3 */
+int32_t xpbd_set_attach_targets(xpbd_ctx *ctx, int32_t n, const float *xyz);
+int32_t xpbd_set_params(xpbd_ctx *ctx, const xpbd_params *p);
+/* kinematic pins: vertex ids + active mask (MultiGridSim pts / pin_active,
+ * mg.py:486-535; pintotarget_overwrite_masked_kernel mg/kernel.py:212-226) */
+int32_t xpbd_set_pins(xpbd_ctx *ctx, int32_t n, const int32_t *vids, const int32_t *active);
+/* per-pin target = target_pos + rest_offset (n*3), per frame (mg.py:1233-1261) */
+int32_t xpbd_set_pin_targets(xpbd_ctx *ctx, int32_t n, const float *xyz);
+/* bone push-out on trilinear SDF grids; the push rule of
+ * solve_mesh_contact_pos_kernel (src/muscle/collision/sdf/kernel.py:123-170).
+ * grids: nbones*res^3 f32, x fastest; origins nbones*3 (bone-local); spacing nbones;
+ * coll_w: per-vertex collision inverse mass (0 = skip), humanV2.py:361-366 */
+int32_t xpbd_set_sdf(xpbd_ctx *ctx, int32_t nbones, int32_t res, const float *grids,
+                     const float *origins, const float *spacing, float contact_offset,
+                     float max_push, const float *coll_w);
+/* which bones each vertex tests: vertex_group[nv] selects the bone range
+ * [group_bone_off[g], group_bone_off[g+1]) (default: one group, all bones).
+ * Batched instances each test only their own bone (no reference counterpart:
+ * the reference has one skeleton collider, humanV2.py:319-418). */
+int32_t xpbd_set_sdf_groups(xpbd_ctx *ctx, int32_t ngroups, const int32_t *group_bone_off,
+                            const int32_t *vertex_group);
+/* per-bone rigid transforms, row-major [R|t] 3x4 each (AnimatedBoneMeshCollider.set_frame,
+ * src/example/humanV2.py:386-398) */
+int32_t xpbd_set_bone_transforms(xpbd_ctx *ctx, int32_t nbones, const float *xforms);
+/* build the device layout; call after the definition calls above */
+int32_t xpbd_finalize(xpbd_ctx *ctx, const xpbd_tuning *tuning);
+int32_t xpbd_get_stats(xpbd_ctx *ctx, xpbd_stats *out);
+
+/* ---- state ----------------------------------------------------------- */
+int32_t xpbd_set_state(xpbd_ctx *ctx, const float *pos, const float *pprev, const float *vel);
+int32_t xpbd_get_state(xpbd_ctx *ctx, float *pos, float *pprev, float *vel);
+
+/* ---- phases (XpbdEngine methods, engine.py:798-1134) ------------------ */
+int32_t xpbd_integrate(xpbd_ctx *ctx);                /* integrate() + apply_pintotarget()  */
+int32_t xpbd_clear(xpbd_ctx *ctx);                    /* clear()                             */
+int32_t xpbd_solve_constraints(xpbd_ctx *ctx);        /* solve_constraints(run_extras=False) */
+int32_t xpbd_apply_dP(xpbd_ctx *ctx);                 /* apply_dP()                          */
+int32_t xpbd_apply_pins(xpbd_ctx *ctx);               /* apply_pintotarget()                 */
+int32_t xpbd_iterate(xpbd_ctx *ctx, int32_t n);       /* n x (clear, dispatch, apply_dP, pin), fused */
+int32_t xpbd_collide(xpbd_ctx *ctx);                  /* collision_solvers: bone SDF push    */
+int32_t xpbd_update_velocities(xpbd_ctx *ctx);        /* update_velocities()                 */
+/* MultiGridSim.substep (mg.py:1322-1384): integrate, pin, iters x iterate, collide, velocity */
+int32_t xpbd_substep(xpbd_ctx *ctx, int32_t iters);
+/* MultiGridSim.step_frame substep loop (mg.py:1409-1418); use_graph=1 replays a
+ * captured hipGraph of the whole frame (re-captured when iters/nsub change) */
+int32_t xpbd_step_frame(xpbd_ctx *ctx, int32_t nsub, int32_t iters, int32_t use_graph);
+int32_t xpbd_synchronize(xpbd_ctx *ctx);
+
+/* ---- diagnostics ------------------------------------------------------ */
+/* per-type residual sums: out[0] sum ||sigma-1||^2 over ARAP tets, out[1] count,
+ * out[2] sum (V-V0)^2 over volume tets, out[3] count, out[4] sum C_D^2 + C_H^2 (SNH),
+ * out[5] count, out[6] sum C^2 (aniso), out[7] count.
+ * arap_residual_accum_kernel (src/muscle/mg/kernel.py:87-118) generalised per type. */
+int32_t xpbd_residuals(xpbd_ctx *ctx, double *out8);
+/* count_inverted_tets_kernel (src/muscle/engine.py:410-428) */
+int32_t xpbd_count_inverted(xpbd_ctx *ctx, int32_t *out);
+/* SDF cells of the last collide pass (nv*2: bone, linear cell; -1 = none) */
+int32_t xpbd_get_sdf_cells(xpbd_ctx *ctx, int32_t *cells);
+/* event timing of projection launches: enable=1 starts accumulating */
+int32_t xpbd_kernel_timing(xpbd_ctx *ctx, int32_t enable);
+int32_t xpbd_get_kernel_timing(xpbd_ctx *ctx, double *total_ms, int32_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1729 @@
+// MI355X (gfx950) XPBD muscle substep: kernels + C ABI (include/xpbd.h).
+//
+// Hot loop: one launch per Jacobi iteration, one workgroup per patch of
+// vertices.  The workgroup stages the patch's vertex table in LDS, evaluates
+// every tet element that touches an owned vertex (ARAP-SVD / volume / SNH /
+// aniso fused per tet), parks the corner corrections in LDS and sums each
+// owned vertex's corrections in a fixed slot order (deterministic; no
+// atomics), then applies dP/dPw, attach and pins and writes the new position
+// into the other half of a ping-pong pair.  Same semantics as the reference
+// per-type kernels + update_dP atomics + apply_dP + pin overwrite
+// (src/muscle/constraint/solve_kernels.py, src/muscle/engine.py:689-698,
+// src/muscle/mg/kernel.py:212-226).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/xpbd.h"
+#include "xpbd_kernels.h"
+#include "xpbd_layout.h"
+
+using namespace xpbd;
+
+// ---------------------------------------------------------------------------
+// small FP32 vector helpers
+// ---------------------------------------------------------------------------
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 mk3(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 xyz(float4 a) { return V3{a.x, a.y, a.z}; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a) { return V3{-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ V3 operator*(float s, V3 a) { return V3{s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ V3 operator/(V3 a, float s) { return V3{a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float det3(const float *m) {
+    return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+           m[2] * (m[3] * m[7] - m[4] * m[6]);
+}
+// gradient of corner k: G * B[k,:]^T  (solve_kernels.py:1269-1281)
+__device__ __forceinline__ V3 grad_row(const float *G, const float *B, int k) {
+    return V3{G[0] * B[3 * k] + G[1] * B[3 * k + 1] + G[2] * B[3 * k + 2],
+              G[3] * B[3 * k] + G[4] * B[3 * k + 1] + G[5] * B[3 * k + 2],
+              G[6] * B[3 * k] + G[7] * B[3 * k + 1] + G[8] * B[3 * k + 2]};
+}
+
+// ---------------------------------------------------------------------------
+// Closest proper rotation R of F (the rotation of the sign-fixed SVD,
+// R = U V^T, solve_kernels.py:85-132).  ARAP-SVD's C = ||sigma - 1|| equals
+// ||F - R||_F and dC/dF = (F - R) / C, so only R is needed.
+//
+// det F > 0: Higham's scaled Newton iteration X <- (g X + X^{-T}/g)/2 with
+// Frobenius-norm scaling (N. J. Higham, "Computing the polar decomposition —
+// with applications", SIAM J. Sci. Stat. Comput. 7, 1986), quadratically
+// convergent; 2 steps near rest.  Otherwise (inverted / degenerate, or no
+// convergence) a one-sided Jacobi SVD gives R = U diag(1,1,det U det V) V^T.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool polar_newton(const float *F, float *X) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) X[i] = F[i];
+    bool scale = true;
+    for (int it = 0; it < 20; ++it) {
+        const float C0 = X[4] * X[8] - X[5] * X[7], C1 = X[5] * X[6] - X[3] * X[8],
+                    C2 = X[3] * X[7] - X[4] * X[6];
+        const float C3 = X[2] * X[7] - X[1] * X[8], C4 = X[0] * X[8] - X[2] * X[6],
+                    C5 = X[1] * X[6] - X[0] * X[7];
+        const float C6 = X[1] * X[5] - X[2] * X[4], C7 = X[2] * X[3] - X[0] * X[5],
+                    C8 = X[0] * X[4] - X[1] * X[3];
+        const float det = X[0] * C0 + X[1] * C1 + X[2] * C2;
+        if (!(det > 1.0e-30f)) return false;
+        const float idet = 1.0f / det;
+        float g = 1.0f;
+        if (scale) {
+            float nx = 0.f;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) nx += X[i] * X[i];
+            const float nc = C0 * C0 + C1 * C1 + C2 * C2 + C3 * C3 + C4 * C4 + C5 * C5 + C6 * C6 +
+                             C7 * C7 + C8 * C8;
+            g = sqrtf(sqrtf(nc / nx) * idet);
+        }
+        const float a = 0.5f * g, b = 0.5f * idet / g;
+        const float C[9] = {C0, C1, C2, C3, C4, C5, C6, C7, C8};
+        float delta = 0.f;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            const float xn = a * X[i] + b * C[i];
+            const float d = xn - X[i];
+            delta += d * d;
+            X[i] = xn;
+        }
+        if (delta < 1.0e-4f) scale = false;
+        if (delta < 1.0e-10f) return true;
+    }
+    return false;
+}
+
+struct M9 {
+    float m[9];
+};
+
+// by-value in/out keeps the caller's matrices in registers (no address taken)
+__device__ __noinline__ M9 rotation_from_svd(M9 Fm) {
+    const float *F = Fm.m;
+    M9 Rm;
+    float *R = Rm.m;
+    float A[9], W[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    for (int i = 0; i < 9; ++i) A[i] = F[i];
+    const int P[3] = {0, 0, 1}, Q[3] = {1, 2, 2};
+    for (int sweep = 0; sweep < 12; ++sweep) {
+        bool rotated = false;
+        for (int r = 0; r < 3; ++r) {
+            const int p = P[r], q = Q[r];
+            float al = 0, be = 0, ga = 0;
+            for (int i = 0; i < 3; ++i) {
+                al += A[3 * i + p] * A[3 * i + p];
+                be += A[3 * i + q] * A[3 * i + q];
+                ga += A[3 * i + p] * A[3 * i + q];
+            }
+            if (ga == 0.0f || fabsf(ga) <= 1.0e-7f * sqrtf(al * be)) continue;
+            rotated = true;
+            const float zeta = (be - al) / (2.0f * ga);
+            const float t = (zeta >= 0.f ? 1.0f : -1.0f) / (fabsf(zeta) + sqrtf(1.0f + zeta * zeta));
+            const float c = 1.0f / sqrtf(1.0f + t * t), s = c * t;
+            for (int i = 0; i < 3; ++i) {
+                const float ap = A[3 * i + p], aq = A[3 * i + q];
+                A[3 * i + p] = c * ap - s * aq;
+                A[3 * i + q] = s * ap + c * aq;
+                const float vp = W[3 * i + p], vq = W[3 * i + q];
+                W[3 * i + p] = c * vp - s * vq;
+                W[3 * i + q] = s * vp + c * vq;
+            }
+        }
+        if (!rotated) break;
+    }
+    float sg[3];
+    for (int j = 0; j < 3; ++j) sg[j] = sqrtf(A[j] * A[j] + A[3 + j] * A[3 + j] + A[6 + j] * A[6 + j]);
+    int o[3] = {0, 1, 2};
+    for (int a = 0; a < 3; ++a)
+        for (int b = a + 1; b < 3; ++b)
+            if (sg[o[b]] > sg[o[a]]) { int t = o[a]; o[a] = o[b]; o[b] = t; }
+    float U[9], V[9];
+    const float smax = sg[o[0]];
+    int known = 0;
+    for (int j = 0; j < 3; ++j) {
+        for (int i = 0; i < 3; ++i) V[3 * i + j] = W[3 * i + o[j]];
+        if (sg[o[j]] > 1.0e-30f && sg[o[j]] > 1.0e-6f * smax) {
+            for (int i = 0; i < 3; ++i) U[3 * i + j] = A[3 * i + o[j]] / sg[o[j]];
+            known = j + 1;
+        }
+    }
+    if (known == 0) { for (int i = 0; i < 9; ++i) U[i] = (i % 4 == 0) ? 1.f : 0.f; known = 1; }
+    if (known == 1) {
+        const float a0 = U[0], a1 = U[3], a2 = U[6];
+        float e0 = 0, e1 = 0, e2 = 0;
+        if (fabsf(a0) <= fabsf(a1) && fabsf(a0) <= fabsf(a2)) e0 = 1;
+        else if (fabsf(a1) <= fabsf(a2)) e1 = 1;
+        else e2 = 1;
+        float b0 = a1 * e2 - a2 * e1, b1 = a2 * e0 - a0 * e2, b2 = a0 * e1 - a1 * e0;
+        const float nb = sqrtf(b0 * b0 + b1 * b1 + b2 * b2);
+        U[1] = b0 / nb; U[4] = b1 / nb; U[7] = b2 / nb;
+        known = 2;
+    }
+    if (known == 2) {
+        U[2] = U[3] * U[7] - U[6] * U[4];
+        U[5] = U[6] * U[1] - U[0] * U[7];
+        U[8] = U[0] * U[4] - U[3] * U[1];
+    }
+    const float s3 = (det3(U) * det3(V) < 0.f) ? -1.f : 1.f;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            R[3 * i + j] = U[3 * i] * V[3 * j] + U[3 * i + 1] * V[3 * j + 1] + s3 * U[3 * i + 2] * V[3 * j + 2];
+    return Rm;
+}
+
+__device__ __forceinline__ void closest_rotation(const float *F, float *R) {
+    if (!(det3(F) > 1.0e-20f) || !polar_newton(F, R)) {
+        M9 f;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) f.m[i] = F[i];
+        const M9 r = rotation_from_svd(f);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) R[i] = r.m[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// per-element projection (all constraint types of one tet, Jacobi, lambda = 0)
+// ---------------------------------------------------------------------------
+struct Corr {
+    V3 d[4];
+    float cnt;
+};
+
+template <int FEAT, int DAMP>
+__device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const float4 *sx,
+                                             const float4 *spp, Corr &out) {
+    const ushort4 li = a.lidx[g];
+    const uint32_t meta = a.meta[g];
+    const uint32_t mask = (meta >> 16) & 15u;
+    const ClassParams cp = a.cls[meta & 0xffffu];
+    const int lv[4] = {li.x, li.y, li.z, li.w};
+    const float4 X0 = sx[lv[0]], X1 = sx[lv[1]], X2 = sx[lv[2]], X3 = sx[lv[3]];
+    const V3 P[4] = {xyz(X0), xyz(X1), xyz(X2), xyz(X3)};
+    const float w[4] = {X0.w, X1.w, X2.w, X3.w};
+    const V3 c0 = P[0] - P[3], c1 = P[1] - P[3], c2 = P[2] - P[3];
+    const float dt = a.sp.dt;
+    for (int i = 0; i < 4; ++i) out.d[i] = mk3(0.f, 0.f, 0.f);
+    out.cnt = 0.f;
+
+    float B[9], F[9];
+    constexpr bool NEED_F = (FEAT & (F_ARAP | F_SNH | F_ANISO)) != 0;
+    if (NEED_F && (mask & (M_ARAP | M_SNH | M_ANISO))) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) B[q] = a.Bp[(int64_t)q * a.nptet + g];
+        const float Ds[9] = {c0.x, c1.x, c2.x, c0.y, c1.y, c2.y, c0.z, c1.z, c2.z};
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                F[3 * i + j] = Ds[3 * i] * B[j] + Ds[3 * i + 1] * B[3 + j] + Ds[3 * i + 2] * B[6 + j];
+    }
+
+    // ---- ARAP-SVD: solve_tetarap_svd_kernel, solve_kernels.py:1194-1307 ----
+    if ((FEAT & F_ARAP) && (mask & M_ARAP)) {
+        float R[9];
+        closest_rotation(F, R);
+        float G[9], c2s = 0.f;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            G[i] = F[i] - R[i];
+            c2s += G[i] * G[i];
+        }
+        if (!(c2s < 1.0e-12f)) {
+            const float C = sqrtf(c2s), invC = 1.0f / C;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) G[i] *= invC;
+            V3 gr[4];
+            gr[0] = grad_row(G, B, 0);
+            gr[1] = grad_row(G, B, 1);
+            gr[2] = grad_row(G, B, 2);
+            gr[3] = -gr[0] - gr[1] - gr[2];
+            const float wsum = w[0] * dot(gr[0], gr[0]) + w[1] * dot(gr[1], gr[1]) +
+                               w[2] * dot(gr[2], gr[2]) + w[3] * dot(gr[3], gr[3]);
+            if (wsum != 0.0f) {
+                const float at = (1.0f / cp.k_arap) / (dt * dt);
+                const float dL = -C / (wsum + at);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) out.d[i] = out.d[i] + (dL * w[i]) * gr[i];
+                out.cnt += 1.0f;
+            }
+        }
+    }
+
+    // ---- volume: tet_volume_update_xpbd_fn, solve_kernels.py:187-281 ----
+    if ((FEAT & F_VOL) && (mask & M_VOL)) {
+        V3 gr[4];
+        gr[0] = cross(c1, c2) / 6.0f;
+        gr[1] = cross(c2, c0) / 6.0f;
+        gr[2] = cross(c0, c1) / 6.0f;
+        gr[3] = -(gr[0] + gr[1] + gr[2]);
+        const float wsum = w[0] * dot(gr[0], gr[0]) + w[1] * dot(gr[1], gr[1]) +
+                           w[2] * dot(gr[2], gr[2]) + w[3] * dot(gr[3], gr[3]);
+        if (wsum > 1e-9f) {
+            const float vol = dot(cross(c0, c1), c2) / 6.0f;
+            const float rest = a.rvol[g];
+            const bool comp = a.sp.has_compress && (vol < rest);
+            const float k = comp ? cp.kcomp_vol : cp.k_vol;
+            if (k != 0.0f) {
+                const float alpha = (1.0f / k) / (dt * dt);
+                const float C = vol - rest;
+                float dsum = 0.f, gamma = 1.f;
+                if (DAMP && cp.damp_vol > 0.0f) {
+                    const float beta = k * cp.damp_vol * dt * dt;
+                    gamma = alpha * beta / dt;
+                    for (int i = 0; i < 4; ++i) {
+                        const float4 pp = spp[lv[i]];
+                        dsum += dot(gr[i], P[i] - xyz(pp));
+                    }
+                    dsum = dsum * gamma;
+                    gamma = gamma + 1.0f;
+                }
+                const float dl = (-C - dsum) / (gamma * wsum + alpha);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) out.d[i] = out.d[i] + (dl * w[i]) * gr[i];
+                out.cnt += 1.0f;
+            }
+        }
+    }
+
+    // ---- SNH: tet_snh_update_xpbd_fn, solve_kernels.py:1429-1584 ----
+    if ((FEAT & F_SNH) && (mask & M_SNH)) {
+        float Ic = 0.f;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Ic += F[i] * F[i];
+        const float r_s = sqrtf(fmaxf(Ic, 1.0e-12f));
+        if (!(r_s < 1.0e-6f)) {
+            const float dC = r_s - 1.7320508075688772f;
+            const float inv_rs = 1.0f / r_s;
+            V3 dg[4];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                dg[k] = V3{B[3 * k] * (F[0] * inv_rs) + B[3 * k + 1] * (F[1] * inv_rs) + B[3 * k + 2] * (F[2] * inv_rs),
+                           B[3 * k] * (F[3] * inv_rs) + B[3 * k + 1] * (F[4] * inv_rs) + B[3 * k + 2] * (F[5] * inv_rs),
+                           B[3 * k] * (F[6] * inv_rs) + B[3 * k + 1] * (F[7] * inv_rs) + B[3 * k + 2] * (F[8] * inv_rs)};
+            }
+            dg[3] = -dg[0] - dg[1] - dg[2];
+            const float dsw = w[0] * dot(dg[0], dg[0]) + w[1] * dot(dg[1], dg[1]) +
+                              w[2] * dot(dg[2], dg[2]) + w[3] * dot(dg[3], dg[3]);
+            if (dsw != 0.0f) {
+                const float mu = cp.mu, lam = cp.lam;
+                const float hC = det3(F) - (1.0f + mu / lam);
+                const float restv = a.rsnh[g];
+                const float inv_detDm = 1.0f / fmaxf(fabsf(restv) * 6.0f, 1.0e-20f);
+                V3 hg[4];
+                hg[0] = inv_detDm * cross(c1, c2);
+                hg[1] = inv_detDm * cross(c2, c0);
+                hg[2] = inv_detDm * cross(c0, c1);
+                hg[3] = -hg[0] - hg[1] - hg[2];
+                const float hsw = w[0] * dot(hg[0], hg[0]) + w[1] * dot(hg[1], hg[1]) +
+                                  w[2] * dot(hg[2], hg[2]) + w[3] * dot(hg[3], hg[3]);
+                if (hsw != 0.0f) {
+                    const float dh = w[0] * dot(dg[0], hg[0]) + w[1] * dot(dg[1], hg[1]) +
+                                     w[2] * dot(dg[2], hg[2]) + w[3] * dot(dg[3], hg[3]);
+                    const float abs_vol = fmaxf(fabsf(restv), 1.0e-20f);
+                    const float dalpha = 1.0f / (mu * abs_vol * dt * dt);
+                    const float halpha = 1.0f / (lam * abs_vol * dt * dt);
+                    float ddamp = 0.f, dgam = 1.f, hdamp = 0.f, hgam = 1.f;
+                    if (DAMP && cp.damp_snh > 0.0f) {
+                        const float kd = cp.damp_snh;
+                        const float dbeta = mu * abs_vol * kd * dt * dt;
+                        const float hbeta = lam * abs_vol * kd * dt * dt;
+                        dgam = dalpha * dbeta / dt;
+                        hgam = halpha * hbeta / dt;
+                        for (int i = 0; i < 4; ++i) {
+                            const V3 dv = P[i] - xyz(spp[lv[i]]);
+                            ddamp += dot(dg[i], dv);
+                            hdamp += dot(hg[i], dv);
+                        }
+                        ddamp = ddamp * dgam;
+                        hdamp = hdamp * hgam;
+                        dgam = dgam + 1.0f;
+                        hgam = hgam + 1.0f;
+                    }
+                    const float Axx = dgam * dsw + dalpha;
+                    const float Ayy = hgam * hsw + halpha;
+                    const float detA = Axx * Ayy - dh * dh;
+                    if (!(detA < 1.0e-8f)) {
+                        const float inv_det = 1.0f / detA;
+                        const float db = -dC - ddamp;
+                        const float hb = -hC - hdamp;
+                        const float ddL = inv_det * (Ayy * db - dh * hb);
+                        const float hdL = inv_det * (Axx * hb - dh * db);
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                            out.d[i] = out.d[i] + w[i] * (ddL * dg[i] + hdL * hg[i]);
+                        out.cnt += 1.0f;
+                    }
+                }
+            }
+        }
+    }
+
+    // ---- aniso_arap: solve_tet_aniso_arap_kernel, solve_kernels.py:1310-1426 ----
+    if ((FEAT & F_ANISO) && (mask & M_ANISO)) {
+        const float4 fb = a.fib[g];  // f0 (normalised at build), fiberscale
+        const V3 f0 = xyz(fb);
+        const V3 d = V3{F[0] * f0.x + F[1] * f0.y + F[2] * f0.z, F[3] * f0.x + F[4] * f0.y + F[5] * f0.z,
+                        F[6] * f0.x + F[7] * f0.y + F[8] * f0.z};
+        const float s = sqrtf(dot(d, d));
+        const float detF = det3(F);
+        if (!(s < 1.0e-9f) && !(detF <= 0.0f || s > 1.5f)) {
+            const float diff = s - fb.w;  // SI4 = +1 since det F > 0
+            const float C = 0.5f * diff * diff;
+            const float gs = diff / s;
+            float G[9];
+            const float dv[3] = {d.x, d.y, d.z}, fv[3] = {f0.x, f0.y, f0.z};
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) G[3 * i + j] = gs * (dv[i] * fv[j]);
+            V3 gr[4];
+            gr[0] = grad_row(G, B, 0);
+            gr[1] = grad_row(G, B, 1);
+            gr[2] = grad_row(G, B, 2);
+            gr[3] = -gr[0] - gr[1] - gr[2];
+            const float wsum = w[0] * dot(gr[0], gr[0]) + w[1] * dot(gr[1], gr[1]) +
+                               w[2] * dot(gr[2], gr[2]) + w[3] * dot(gr[3], gr[3]);
+            if (!(wsum < 1.0e-20f)) {
+                const float sv = cp.k_aniso * a.sp.fiber_scale;
+                const float at = (1.0f / sv) / (dt * dt);
+                const float dL = -C / (wsum + at);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) out.d[i] = out.d[i] + (dL * w[i]) * gr[i];
+                out.cnt += 1.0f;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// attach (per owned vertex): attach_bilateral_update_fn, solve_kernels.py:462-526
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void attach_vertex(const IterArgs &a, int v, float4 x, float4 pp,
+                                              float &ax, float &ay, float &az, float &aw) {
+    const float4 a0 = a.att0[v];
+    if (!(a0.w > 0.0f)) return;
+    const float wv = x.w;
+    if (wv == 0.0f) return;
+    const float4 a1 = a.att1[v];
+    V3 n = xyz(x) - xyz(a0);
+    const float d = sqrtf(dot(n, n));
+    if (!(d >= 1e-6f)) return;
+    const bool comp = a.sp.has_compress && (d < a1.x);
+    const float kv = comp ? a1.y : a0.w;
+    if (kv == 0.0f) return;
+    const float dt = a.sp.dt;
+    const float alpha = (1.0f / kv) / (dt * dt);
+    const float C = d - a1.x;
+    n = n / d;
+    float dsum = 0.f, gamma = 1.f;
+    if (a1.z > 0.0f) {
+        const float beta = kv * a1.z * dt * dt;
+        gamma = alpha * beta / dt;
+        dsum = gamma * dot(n, xyz(x) - xyz(pp));
+        gamma = gamma + 1.0f;
+    }
+    const float dL = (-C - dsum) / (gamma * wv + alpha);
+    const V3 dp = V3{n.x * (-dL), n.y * (-dL), n.z * (-dL)};
+    const V3 c = (-wv) * dp;
+    ax += c.x;
+    ay += c.y;
+    az += c.z;
+    aw += 1.0f;
+}
+
+// ---------------------------------------------------------------------------
+// one Jacobi iteration over all patches
+// ---------------------------------------------------------------------------
+template <int BS, int FEAT, int DAMP, int MODE>
+__global__ __launch_bounds__(BS) void k_iterate(IterArgs a, const float4 *__restrict__ xin,
+                                                float4 *__restrict__ xout,
+                                                const float4 *__restrict__ pprev) {
+    extern __shared__ float4 smem[];
+    float4 *sx = smem;
+    float4 *spp = smem + a.max_table;
+    float4 *sc = smem + (DAMP ? 2 : 1) * a.max_table;
+    const PatchDesc P = a.patch[blockIdx.x];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < P.nown; i += BS) {
+        sx[i] = xin[P.vbeg + i];
+        if (DAMP) spp[i] = pprev[P.vbeg + i];
+    }
+    for (int i = tid; i < P.nhalo; i += BS) {
+        const int gv = a.halo[P.hbeg + i];
+        sx[P.nown + i] = xin[gv];
+        if (DAMP) spp[P.nown + i] = pprev[gv];
+    }
+    __syncthreads();
+    float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
+    int cur = 0, end = 0;
+    if (tid < P.nown) {
+        cur = a.vslot_off[P.vbeg + tid];
+        end = a.vslot_off[P.vbeg + tid + 1];
+    }
+    for (int base = 0; base < P.ntet; base += BS) {
+        const int t = base + tid;
+        if (t < P.ntet) {
+            Corr c;
+            project_elem<FEAT, DAMP>(a, (int64_t)P.tbeg + t, sx, spp, c);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sc[tid * 4 + k] = make_float4(c.d[k].x, c.d[k].y, c.d[k].z, c.cnt);
+        }
+        __syncthreads();
+        if (tid < P.nown) {
+            const int lim = (base + BS) * 4;
+            while (cur < end) {
+                const int s = a.vslot[cur];
+                if (s >= lim) break;
+                const float4 q = sc[s - base * 4];
+                ax += q.x;
+                ay += q.y;
+                az += q.z;
+                aw += q.w;
+                ++cur;
+            }
+        }
+        __syncthreads();
+    }
+    if (tid < P.nown) {
+        const int v = P.vbeg + tid;
+        float4 x = sx[tid];
+        if (a.att0) attach_vertex(a, v, x, DAMP ? spp[tid] : make_float4(0, 0, 0, 0), ax, ay, az, aw);
+        if (MODE == MODE_ACCUM) {
+            float4 o = a.dpw[v];
+            o.x += ax;
+            o.y += ay;
+            o.z += az;
+            o.w += aw;
+            a.dpw[v] = o;
+        } else {
+            if (aw > 1e-9f) {  // apply_dP_kernel, engine.py:689-698
+                x.x = x.x + ax / aw;
+                x.y = x.y + ay / aw;
+                x.z = x.z + az / aw;
+            }
+            if (a.pin) {  // pintotarget overwrite, mg/kernel.py:212-226
+                const float4 p = a.pin[v];
+                if (p.w != 0.0f) { x.x = p.x; x.y = p.y; x.z = p.z; }
+            }
+            xout[v] = x;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// per-vertex kernels
+// ---------------------------------------------------------------------------
+// integrate_kernel (engine.py:431-464) + apply_pintotarget (mg.py:1337)
+__global__ void k_integrate(int nv, float4 *X, float4 *PP, float4 *VEL, const float4 *FRC,
+                            const uint32_t *flags, const float4 *pin, SimParams sp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    float4 x = X[i], pp = PP[i], v = VEL[i];
+    if (flags[i] & 1u) {
+        pp.x = x.x; pp.y = x.y; pp.z = x.z;
+        v.x = v.y = v.z = 0.f;
+    } else {
+        V3 ex = V3{sp.gx, sp.gy, sp.gz};
+        const float m = v.w;
+        if (FRC && m > 1.0e-12f) ex = ex + xyz(FRC[i]) / m;
+        pp.x = x.x; pp.y = x.y; pp.z = x.z;
+        if (sp.quasi_static) {
+            const float dt2 = sp.dt * sp.dt;
+            x.x = x.x + dt2 * ex.x; x.y = x.y + dt2 * ex.y; x.z = x.z + dt2 * ex.z;
+            v.x = v.y = v.z = 0.f;
+        } else {
+            const float dmp = 1.0f - sp.veldamping;
+            V3 vv = dmp * xyz(v);
+            vv = vv + sp.dt * ex;
+            v.x = vv.x; v.y = vv.y; v.z = vv.z;
+            x.x = x.x + sp.dt * vv.x; x.y = x.y + sp.dt * vv.y; x.z = x.z + sp.dt * vv.z;
+        }
+    }
+    if (pin) {
+        const float4 p = pin[i];
+        if (p.w != 0.0f) { x.x = p.x; x.y = p.y; x.z = p.z; }
+    }
+    X[i] = x;
+    PP[i] = pp;
+    VEL[i] = v;
+}
+
+// SDF lookup on one bone grid; every FP32 step rounded as written so the
+// cell index and the push match the CPU oracle bit for bit.
+struct SdfHit {
+    float phi;
+    V3 n;
+    int cell;
+    int ok;
+};
+
+__device__ SdfHit sdf_eval(const float *grid, int res, const float *bn, V3 x) {
+#pragma clang fp contract(off)
+    const float inv_h = bn[15];
+    SdfHit r;
+    r.ok = 0; r.phi = 0.f; r.n = mk3(0, 0, 0); r.cell = -1;
+    const float *X = bn;            // 3x4 row-major [R|t]
+    const float *origin = bn + 12;
+    const V3 q = x - V3{X[3], X[7], X[11]};
+    const V3 y = V3{X[0] * q.x + X[4] * q.y + X[8] * q.z, X[1] * q.x + X[5] * q.y + X[9] * q.z,
+                    X[2] * q.x + X[6] * q.y + X[10] * q.z};
+    const float gx = (y.x - origin[0]) * inv_h, gy = (y.y - origin[1]) * inv_h,
+                gz = (y.z - origin[2]) * inv_h;
+    const float lim = (float)(res - 1);
+    if (!(gx >= 0.0f && gy >= 0.0f && gz >= 0.0f && gx <= lim && gy <= lim && gz <= lim)) return r;
+    int ix = (int)floorf(gx), iy = (int)floorf(gy), iz = (int)floorf(gz);
+    ix = min(ix, res - 2); iy = min(iy, res - 2); iz = min(iz, res - 2);
+    const float fx = gx - (float)ix, fy = gy - (float)iy, fz = gz - (float)iz;
+    const float *g0 = grid + ((size_t)iz * res + iy) * res + ix;
+    const size_t sy = (size_t)res, sz = (size_t)res * res;
+    const float c000 = g0[0], c100 = g0[1], c010 = g0[sy], c110 = g0[sy + 1];
+    const float c001 = g0[sz], c101 = g0[sz + 1], c011 = g0[sz + sy], c111 = g0[sz + sy + 1];
+    const float ux = 1.0f - fx, uy = 1.0f - fy, uz = 1.0f - fz;
+    const float c00 = c000 * ux + c100 * fx, c10 = c010 * ux + c110 * fx;
+    const float c01 = c001 * ux + c101 * fx, c11 = c011 * ux + c111 * fx;
+    const float c0 = c00 * uy + c10 * fy, c1 = c01 * uy + c11 * fy;
+    r.phi = c0 * uz + c1 * fz;
+    const float dx = ((c100 - c000) * uy + (c110 - c010) * fy) * uz + ((c101 - c001) * uy + (c111 - c011) * fy) * fz;
+    const float dy = ((c010 - c000) * ux + (c110 - c100) * fx) * uz + ((c011 - c001) * ux + (c111 - c101) * fx) * fz;
+    const float dz = c1 - c0;
+    const V3 gw = V3{X[0] * dx + X[1] * dy + X[2] * dz, X[4] * dx + X[5] * dy + X[6] * dz,
+                     X[8] * dx + X[9] * dy + X[10] * dz};
+    const float gl = sqrtf(gw.x * gw.x + gw.y * gw.y + gw.z * gw.z);
+    if (!(gl > 1e-20f)) return r;
+    r.n = V3{gw.x / gl, gw.y / gl, gw.z / gl};
+    r.cell = (iz * res + iy) * res + ix;
+    r.ok = 1;
+    return r;
+}
+
+// post phase: bone SDF push (collision_solvers, mg.py:1370-1371) then
+// update_velocities_kernel (engine.py:582-605); result lands in X (buffer A)
+__global__ void k_post(int nv, const float4 *Xc, float4 *X, const float4 *PP, float4 *VEL,
+                       SdfParams sd, int32_t *cells, SimParams sp, int do_sdf, int do_vel) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    float4 x = Xc[i];
+    const float4 pp = PP[i];
+    if (do_sdf) {
+#pragma clang fp contract(off)
+        int bb = -1, bc = -1;
+        if (pp.w != 0.0f) {
+            const V3 p = xyz(x);
+            SdfHit best;
+            best.ok = 0; best.phi = 0.f; best.n = mk3(0, 0, 0); best.cell = -1;
+            const size_t gsz = (size_t)sd.res * sd.res * sd.res;
+            const int grp = sd.vgroup[i];
+            const int b1 = sd.goff[grp + 1];
+            for (int b = sd.goff[grp]; b < b1; ++b) {
+                const SdfHit h = sdf_eval(sd.grids + (size_t)b * gsz, sd.res, sd.bone + 16 * b, p);
+                if (h.ok && (!best.ok || h.phi < best.phi)) { best = h; bb = b; }
+            }
+            if (best.ok) {
+                bc = best.cell;
+                const float err = best.phi - sd.contact_offset;
+                if (err < 0.0f && (sd.max_push <= 0.0f || -err <= sd.max_push)) {
+                    x.x = p.x - err * best.n.x;
+                    x.y = p.y - err * best.n.y;
+                    x.z = p.z - err * best.n.z;
+                }
+            } else {
+                bb = -1;
+            }
+        }
+        if (cells) { cells[2 * i] = bb; cells[2 * i + 1] = bc; }
+    }
+    if (do_vel && !sp.quasi_static) {
+        float4 v = VEL[i];
+        V3 vv = V3{(x.x - pp.x) / sp.dt, (x.y - pp.y) / sp.dt, (x.z - pp.z) / sp.dt};
+        if (sp.const_damp != 0.0f || sp.exp_damp != 1.0f) {
+            const float vm = sqrtf(dot(vv, vv));
+            if (vm > 1.0e-6f) {
+                float vn = vm * sp.exp_damp - sp.const_damp;
+                if (vn < 1.0e-6f) vn = 0.0f;
+                vv = (vn / vm) * vv;
+            } else {
+                vv = mk3(0, 0, 0);
+            }
+        }
+        v.x = vv.x; v.y = vv.y; v.z = vv.z;
+        VEL[i] = v;
+    }
+    X[i] = x;
+}
+
+__global__ void k_apply_dpw(int nv, float4 *X, const float4 *dpw, const float4 *pin, int do_pin) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    float4 x = X[i];
+    const float4 d = dpw[i];
+    if (d.w > 1e-9f) { x.x = x.x + d.x / d.w; x.y = x.y + d.y / d.w; x.z = x.z + d.z / d.w; }
+    if (do_pin && pin) {
+        const float4 p = pin[i];
+        if (p.w != 0.0f) { x.x = p.x; x.y = p.y; x.z = p.z; }
+    }
+    X[i] = x;
+}
+
+__global__ void k_pins(int nv, float4 *X, const float4 *pin) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    const float4 p = pin[i];
+    if (p.w != 0.0f) { float4 x = X[i]; x.x = p.x; x.y = p.y; x.z = p.z; X[i] = x; }
+}
+
+__global__ void k_zero_i32(int n, int32_t *p) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+__global__ void k_zero4(int64_t n, float4 *p) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// fiberscale = 1 - belly * activation[tetid] * contraction_ratio (solve_kernels.py:1388-1389)
+__global__ void k_refresh_fiber(int64_t nptet, const float4 *fstat, const int32_t *tetid,
+                                const float *act, float cr, float4 *fib) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nptet) return;
+    float4 f = fstat[i];
+    f.w = 1.0f - f.w * act[tetid[i]] * cr;
+    fib[i] = f;
+}
+
+// residual sums per type over primary elements: out per block 8 doubles
+template <int BS>
+__global__ __launch_bounds__(BS) void k_residual(IterArgs a, const float4 *__restrict__ X,
+                                                 const int32_t *__restrict__ pvert,
+                                                 double *partial) {
+    __shared__ double red[BS];
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t g = (int64_t)blockIdx.x * BS + threadIdx.x; g < a.nptet; g += (int64_t)gridDim.x * BS) {
+        const uint32_t meta = a.meta[g];
+        if (!((meta >> 20) & 1u)) continue;
+        const uint32_t mask = (meta >> 16) & 15u;
+        const ClassParams cp = a.cls[meta & 0xffffu];
+        const int32_t *pv = pvert + 4 * g;
+        const V3 P0 = xyz(X[pv[0]]), P1 = xyz(X[pv[1]]), P2 = xyz(X[pv[2]]), P3 = xyz(X[pv[3]]);
+        const V3 c0 = P0 - P3, c1 = P1 - P3, c2 = P2 - P3;
+        float B[9], F[9];
+        for (int q = 0; q < 9; ++q) B[q] = a.Bp[(int64_t)q * a.nptet + g];
+        const float Ds[9] = {c0.x, c1.x, c2.x, c0.y, c1.y, c2.y, c0.z, c1.z, c2.z};
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                F[3 * i + j] = Ds[3 * i] * B[j] + Ds[3 * i + 1] * B[3 + j] + Ds[3 * i + 2] * B[6 + j];
+        if (mask & M_ARAP) {
+            float R[9];
+            closest_rotation(F, R);
+            float s = 0.f;
+            for (int i = 0; i < 9; ++i) s += (F[i] - R[i]) * (F[i] - R[i]);
+            acc[0] += s;
+            acc[1] += 1.0;
+        }
+        if (mask & M_VOL) {
+            const float vol = dot(cross(c0, c1), c2) / 6.0f;
+            const double dv = (double)vol - (double)a.rvol[g];
+            acc[2] += dv * dv;
+            acc[3] += 1.0;
+        }
+        if (mask & M_SNH) {
+            float Ic = 0.f;
+            for (int i = 0; i < 9; ++i) Ic += F[i] * F[i];
+            const double cd = (double)sqrtf(fmaxf(Ic, 1e-12f)) - 1.7320508075688772;
+            const double ch = (double)det3(F) - (1.0 + (double)cp.mu / (double)cp.lam);
+            acc[4] += cd * cd + ch * ch;
+            acc[5] += 1.0;
+        }
+        if (mask & M_ANISO) {
+            const float4 fb = a.fib[g];
+            const V3 d = V3{F[0] * fb.x + F[1] * fb.y + F[2] * fb.z, F[3] * fb.x + F[4] * fb.y + F[5] * fb.z,
+                            F[6] * fb.x + F[7] * fb.y + F[8] * fb.z};
+            const double s = sqrt((double)dot(d, d)) - (double)fb.w;
+            const double C = 0.5 * s * s;
+            acc[6] += C * C;
+            acc[7] += 1.0;
+        }
+    }
+    for (int k = 0; k < 8; ++k) {
+        red[threadIdx.x] = acc[k];
+        __syncthreads();
+        for (int s = BS / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partial[(size_t)blockIdx.x * 8 + k] = red[0];
+        __syncthreads();
+    }
+}
+
+__global__ void k_count_inverted(int nt, const int4 *tets, const float *B, const float4 *X,
+                                 int *count) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int inv = 0;
+    if (c < nt) {
+        const int4 t = tets[c];
+        const V3 P3 = xyz(X[t.w]);
+        const V3 c0 = xyz(X[t.x]) - P3, c1 = xyz(X[t.y]) - P3, c2 = xyz(X[t.z]) - P3;
+        const float *b = B + 9 * (size_t)c;
+        const float Ds[9] = {c0.x, c1.x, c2.x, c0.y, c1.y, c2.y, c0.z, c1.z, c2.z};
+        float F[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                F[3 * i + j] = Ds[3 * i] * b[j] + Ds[3 * i + 1] * b[3 + j] + Ds[3 * i + 2] * b[6 + j];
+        inv = det3(F) <= 0.0f ? 1 : 0;
+    }
+    // integer reduction is order-free
+    if (inv) atomicAdd(count, 1);
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+namespace {
+
+struct DBuf {
+    void *p = nullptr;
+    size_t n = 0;
+};
+
+struct EvPair {
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct xpbd_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr, stream = nullptr;
+    std::string err;
+    // host definition
+    int nv = 0, nt = 0;
+    std::vector<float> pos0, rest_volume, rest_matrix, mass, force, fiber, tendonmask, activation;
+    std::vector<int32_t> tets, stopped;
+    bool has_force = false;
+    // constraint table copy
+    int ncons = 0;
+    std::vector<int32_t> c_type, c_pts, c_tetid;
+    std::vector<float> c_stiff, c_comp, c_damp, c_rest, c_rvec, c_rdir;
+    SimParams sp{};
+    bool params_set = false;
+    // pins
+    std::vector<int32_t> pin_vids, pin_active;
+    std::vector<float> pin_targets;
+    // sdf
+    int nbones = 0, res = 0;
+    std::vector<float> sdf_grids, sdf_origin, sdf_invh, coll_w, bone_xf;
+    std::vector<int32_t> vgroup, goff;
+    float sdf_offset = 0.f, sdf_maxpush = -1.f;
+    // layout
+    xpbd::Layout L;
+    bool ready = false;
+    int P = 256;
+    int feature = 0;
+    // device buffers
+    std::vector<DBuf> bufs;
+    float4 *X[2] = {nullptr, nullptr}, *PPv = nullptr, *VEL = nullptr, *FRC = nullptr, *PIN = nullptr,
+           *ATT0 = nullptr, *ATT1 = nullptr, *DPW = nullptr, *FIB = nullptr, *FIBS = nullptr;
+    uint32_t *FLAGS = nullptr;
+    PatchDesc *d_patch = nullptr;
+    int32_t *d_halo = nullptr, *d_vslot_off = nullptr, *d_ptet_tetid = nullptr, *d_pvert = nullptr;
+    ushort4 *d_lidx = nullptr;
+    uint32_t *d_meta = nullptr;
+    uint16_t *d_vslot = nullptr;
+    float *d_Bp = nullptr, *d_rvol = nullptr, *d_rsnh = nullptr, *d_act = nullptr;
+    ClassParams *d_cls = nullptr;
+    int4 *d_tets = nullptr;
+    float *d_B = nullptr;
+    float *d_grids = nullptr, *d_bone = nullptr;
+    int32_t *d_vgroup = nullptr, *d_goff = nullptr;
+    int32_t *d_cells = nullptr, *d_count = nullptr;
+    double *d_partial = nullptr;
+    int n_partial = 0;
+    bool pins_on = false;
+    // graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int g_nsub = -1, g_iters = -1;
+    // timing
+    bool timing = false;
+    std::vector<EvPair> evs;
+    double timed_ms = 0.0;
+    int timed_n = 0;
+};
+
+namespace {
+
+int fail(xpbd_ctx *c, int code, const std::string &m) {
+    if (c) c->err = m;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                       \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail(ctx, XPBD_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+template <typename T>
+int dalloc(xpbd_ctx *c, T **p, size_t count) {
+    size_t bytes = std::max<size_t>(count * sizeof(T), 16);
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return fail(c, XPBD_EOOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->bufs.push_back(DBuf{q, bytes});
+    *p = (T *)q;
+    return XPBD_OK;
+}
+
+template <typename T>
+int dupload(xpbd_ctx *c, T **p, const T *src, size_t count) {
+    int r = dalloc(c, p, count);
+    if (r) return r;
+    if (count) HIPCHK(c, hipMemcpy(*p, src, count * sizeof(T), hipMemcpyHostToDevice));
+    return XPBD_OK;
+}
+
+void free_device(xpbd_ctx *c) {
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+    c->g_nsub = c->g_iters = -1;
+    for (auto &b : c->bufs) (void)hipFree(b.p);
+    c->bufs.clear();
+    c->ready = false;
+}
+
+void invalidate_graph(xpbd_ctx *c) {
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+    c->g_nsub = c->g_iters = -1;
+}
+
+inline float inv_mass_of(const xpbd_ctx *c, int old) {
+    if (!c->stopped.empty() && c->stopped[old] != 0) return 0.0f;
+    float m = c->mass[old];
+    return m > 0.0f ? 1.0f / m : 0.0f;
+}
+
+// upload the per-vertex W (inverse mass) / mass / flags after mass or stopped change
+int upload_vertex_static(xpbd_ctx *c) {
+    const int nv = c->nv;
+    std::vector<float4> x(nv), v(nv), pp(nv);
+    HIPCHK(c, hipMemcpyAsync(x.data(), c->X[0], nv * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(v.data(), c->VEL, nv * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(pp.data(), c->PPv, nv * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<uint32_t> fl(nv);
+    for (int n = 0; n < nv; ++n) {
+        const int o = c->L.perm[n];
+        x[n].w = inv_mass_of(c, o);
+        v[n].w = c->mass[o];
+        fl[n] = (!c->stopped.empty() && c->stopped[o]) ? 1u : 0u;
+        pp[n].w = c->coll_w.empty() ? 0.0f : c->coll_w[o];
+    }
+    HIPCHK(c, hipMemcpy(c->X[0], x.data(), nv * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->X[1], x.data(), nv * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->VEL, v.data(), nv * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->PPv, pp.data(), nv * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->FLAGS, fl.data(), nv * sizeof(uint32_t), hipMemcpyHostToDevice));
+    return XPBD_OK;
+}
+
+int upload_pins(xpbd_ctx *c) {
+    const int nv = c->nv;
+    std::vector<float4> pin(nv, make_float4(0, 0, 0, 0));
+    bool any = false;
+    for (int n = 0; n < nv; ++n) {
+        const float *q = &c->L.pin_from_cons[4 * (size_t)n];
+        if (q[3] != 0.0f) { pin[n] = make_float4(q[0], q[1], q[2], 1.0f); any = true; }
+    }
+    const int np = (int)c->pin_vids.size();
+    for (int i = 0; i < np; ++i) {
+        if (!c->pin_active[i]) continue;
+        const int n = c->L.iperm[c->pin_vids[i]];
+        const float *t = c->pin_targets.empty() ? nullptr : &c->pin_targets[3 * (size_t)i];
+        if (!t) continue;
+        pin[n] = make_float4(t[0], t[1], t[2], 1.0f);
+        any = true;
+    }
+    c->pins_on = any;
+    HIPCHK(c, hipMemcpyAsync(c->PIN, pin.data(), nv * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return XPBD_OK;
+}
+
+IterArgs make_args(xpbd_ctx *c, float4 *dpw) {
+    IterArgs a;
+    a.patch = c->d_patch;
+    a.halo = c->d_halo;
+    a.lidx = c->d_lidx;
+    a.meta = c->d_meta;
+    a.Bp = c->d_Bp;
+    a.rvol = c->d_rvol;
+    a.rsnh = c->d_rsnh;
+    a.fib = c->FIB;
+    a.cls = c->d_cls;
+    a.vslot_off = c->d_vslot_off;
+    a.vslot = c->d_vslot;
+    a.pin = c->pins_on ? c->PIN : nullptr;
+    a.att0 = (c->feature & F_ATTACH) ? c->ATT0 : nullptr;
+    a.att1 = c->ATT1;
+    a.dpw = dpw;
+    a.nptet = (int64_t)c->L.meta.size();
+    a.max_table = c->L.max_table;
+    a.sp = c->sp;
+    return a;
+}
+
+// template dispatch of the iteration kernel
+typedef void (*IterKernel)(IterArgs, const float4 *, float4 *, const float4 *);
+
+template <int BS, int FEAT>
+IterKernel pick_damp(int damp, int mode) {
+    if (mode == MODE_ACCUM) return damp ? k_iterate<BS, FEAT, 1, MODE_ACCUM> : k_iterate<BS, FEAT, 0, MODE_ACCUM>;
+    return damp ? k_iterate<BS, FEAT, 1, MODE_FUSED> : k_iterate<BS, FEAT, 0, MODE_FUSED>;
+}
+
+template <int BS>
+IterKernel pick_feat(int feat, int damp, int mode) {
+    constexpr int AV = F_ARAP | F_VOL, AVA = F_ARAP | F_VOL | F_ANISO, SA = F_SNH | F_ANISO,
+                  ALL = F_ARAP | F_VOL | F_SNH | F_ANISO;
+    const int f = feat & ALL;
+    if (mode == MODE_ACCUM) return pick_damp<BS, ALL>(damp, mode);
+    if ((f & ~AV) == 0) return pick_damp<BS, AV>(damp, mode);
+    if ((f & ~SA) == 0) return pick_damp<BS, SA>(damp, mode);
+    if ((f & ~AVA) == 0) return pick_damp<BS, AVA>(damp, mode);
+    return pick_damp<BS, ALL>(damp, mode);
+}
+
+IterKernel pick_iter(int P, int feat, int mode) {
+    const int damp = (feat & F_DAMP) ? 1 : 0;
+    switch (P) {
+        case 64: return pick_feat<64>(feat, damp, mode);
+        case 128: return pick_feat<128>(feat, damp, mode);
+        default: return pick_feat<256>(feat, damp, mode);
+    }
+}
+
+size_t iter_lds(const xpbd_ctx *c) {
+    const int damp = (c->feature & F_DAMP) ? 2 : 1;
+    return ((size_t)damp * c->L.max_table + (size_t)c->P * 4) * sizeof(float4);
+}
+
+int launch_iter(xpbd_ctx *c, const float4 *xin, float4 *xout, int mode) {
+    IterKernel k = pick_iter(c->P, c->feature, mode);
+    IterArgs a = make_args(c, c->DPW);
+    const size_t lds = iter_lds(c);
+    EvPair ev{};
+    const bool tm = c->timing;
+    if (tm) {
+        HIPCHK(c, hipEventCreate(&ev.a));
+        HIPCHK(c, hipEventCreate(&ev.b));
+        HIPCHK(c, hipEventRecord(ev.a, c->stream));
+    }
+    const float4 *pp = c->PPv;
+    void *args[] = {(void *)&a, (void *)&xin, (void *)&xout, (void *)&pp};
+    HIPCHK(c, hipLaunchKernel((const void *)k, dim3((unsigned)c->L.patches.size()), dim3(c->P), args, lds,
+                              c->stream));
+    if (tm) {
+        HIPCHK(c, hipEventRecord(ev.b, c->stream));
+        c->evs.push_back(ev);
+    }
+    return XPBD_OK;
+}
+
+inline dim3 vgrid(int n) { return dim3((unsigned)((n + 255) / 256)); }
+
+int launch_integrate(xpbd_ctx *c) {
+    hipLaunchKernelGGL(k_integrate, vgrid(c->nv), dim3(256), 0, c->stream, c->nv, c->X[0], c->PPv, c->VEL,
+                       c->has_force ? (const float4 *)c->FRC : nullptr, (const uint32_t *)c->FLAGS,
+                       c->pins_on ? (const float4 *)c->PIN : nullptr, c->sp);
+    HIPCHK(c, hipGetLastError());
+    return XPBD_OK;
+}
+
+int launch_post(xpbd_ctx *c, const float4 *xc, int do_sdf, int do_vel) {
+    SdfParams sd{};
+    sd.nbones = c->nbones;
+    sd.res = c->res;
+    sd.contact_offset = c->sdf_offset;
+    sd.max_push = c->sdf_maxpush;
+    sd.grids = c->d_grids;
+    sd.bone = c->d_bone;
+    sd.vgroup = c->d_vgroup;
+    sd.goff = c->d_goff;
+    const int sdf_on = (do_sdf && c->nbones > 0 && c->d_grids) ? 1 : 0;
+    hipLaunchKernelGGL(k_post, vgrid(c->nv), dim3(256), 0, c->stream, c->nv, xc, c->X[0],
+                       (const float4 *)c->PPv, c->VEL, sd, sdf_on ? c->d_cells : nullptr, c->sp, sdf_on,
+                       do_vel);
+    HIPCHK(c, hipGetLastError());
+    return XPBD_OK;
+}
+
+// one substep, positions start and end in X[0]
+int enqueue_substep(xpbd_ctx *c, int iters) {
+    int r = launch_integrate(c);
+    if (r) return r;
+    for (int it = 0; it < iters; ++it) {
+        r = launch_iter(c, c->X[it & 1], c->X[(it + 1) & 1], MODE_FUSED);
+        if (r) return r;
+    }
+    return launch_post(c, c->X[iters & 1], 1, 1);
+}
+
+int check_ready(xpbd_ctx *c) {
+    if (!c) return XPBD_EINVAL;
+    if (!c->ready) return fail(c, XPBD_ENOTREADY, "call xpbd_finalize() first");
+    return XPBD_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int32_t xpbd_create(int32_t device, xpbd_ctx **out) {
+    if (!out) return XPBD_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return XPBD_EHIP;
+    if (device < 0 || device >= n) return XPBD_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return XPBD_EHIP;
+    xpbd_ctx *c = new xpbd_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return XPBD_EHIP;
+    }
+    c->stream = c->own_stream;
+    c->sp.dt = 1.0f / 240.0f;
+    c->sp.exp_damp = 1.0f;
+    c->sp.fiber_scale = 1.0f;
+    *out = c;
+    return XPBD_OK;
+}
+
+void xpbd_destroy(xpbd_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &e : c->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+    free_device(c);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    delete c;
+}
+
+const char *xpbd_last_error(const xpbd_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int32_t xpbd_set_stream(xpbd_ctx *c, void *stream) {
+    if (!c) return XPBD_EINVAL;
+    c->stream = stream ? (hipStream_t)stream : c->own_stream;
+    invalidate_graph(c);
+    return XPBD_OK;
+}
+
+int32_t xpbd_set_mesh(xpbd_ctx *c, int32_t nv, const float *pos0, int32_t nt, const int32_t *tets,
+                      float density) {
+    if (!c || nv <= 0 || nt < 0 || !pos0 || (nt > 0 && !tets)) return fail(c, XPBD_EINVAL, "bad mesh");
+    for (int64_t i = 0; i < 4 * (int64_t)nt; ++i)
+        if (tets[i] < 0 || tets[i] >= nv) return fail(c, XPBD_EINVAL, "tet index out of range");
+    free_device(c);
+    c->nv = nv;
+    c->nt = nt;
+    c->pos0.assign(pos0, pos0 + 3 * (size_t)nv);
+    c->tets.assign(tets, tets + 4 * (size_t)nt);
+    c->rest_volume.assign(nt, 0.f);
+    c->rest_matrix.assign(9 * (size_t)nt, 0.f);
+    c->mass.assign(nv, 0.f);
+    xpbd::precompute_rest(nv, pos0, nt, tets, density, c->rest_volume.data(), c->rest_matrix.data(),
+                          c->mass.data());
+    c->stopped.assign(nv, 0);
+    c->force.clear();
+    c->has_force = false;
+    c->fiber.clear();
+    c->tendonmask.assign(nt, 0.f);
+    c->activation.assign(nt, 0.f);
+    c->coll_w.clear();
+    return XPBD_OK;
+}
+
+int32_t xpbd_get_rest(xpbd_ctx *c, float *rv, float *rm, float *mass) {
+    if (!c || c->nv == 0) return fail(c, XPBD_EINVAL, "no mesh");
+    if (rv) std::memcpy(rv, c->rest_volume.data(), c->rest_volume.size() * sizeof(float));
+    if (rm) std::memcpy(rm, c->rest_matrix.data(), c->rest_matrix.size() * sizeof(float));
+    if (mass) std::memcpy(mass, c->mass.data(), c->mass.size() * sizeof(float));
+    return XPBD_OK;
+}
+
+int32_t xpbd_set_mass(xpbd_ctx *c, const float *mass) {
+    if (!c || !mass || c->nv == 0) return fail(c, XPBD_EINVAL, "bad mass");
+    c->mass.assign(mass, mass + c->nv);
+    return c->ready ? upload_vertex_static(c) : XPBD_OK;
+}
+
+int32_t xpbd_set_stopped(xpbd_ctx *c, const int32_t *st) {
+    if (!c || !st || c->nv == 0) return fail(c, XPBD_EINVAL, "bad stopped");
+    c->stopped.assign(st, st + c->nv);
+    return c->ready ? upload_vertex_static(c) : XPBD_OK;
+}
+
+int32_t xpbd_set_force(xpbd_ctx *c, const float *f) {
+    if (!c || c->nv == 0) return fail(c, XPBD_EINVAL, "bad force");
+    if (!f) { c->force.clear(); c->has_force = false; }
+    else { c->force.assign(f, f + 3 * (size_t)c->nv); c->has_force = true; }
+    if (c->ready) {
+        invalidate_graph(c);
+        std::vector<float4> fr(c->nv, make_float4(0, 0, 0, 0));
+        if (c->has_force)
+            for (int n = 0; n < c->nv; ++n) {
+                const int o = c->L.perm[n];
+                fr[n] = make_float4(c->force[3 * o], c->force[3 * o + 1], c->force[3 * o + 2], 0.f);
+            }
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpy(c->FRC, fr.data(), c->nv * sizeof(float4), hipMemcpyHostToDevice));
+    }
+    return XPBD_OK;
+}
+
+int32_t xpbd_set_fiber(xpbd_ctx *c, const float *vf, const float *tm) {
+    if (!c || c->nv == 0) return fail(c, XPBD_EINVAL, "no mesh");
+    if (c->ready) return fail(c, XPBD_EINVAL, "fiber data must be set before xpbd_finalize");
+    if (vf) c->fiber.assign(vf, vf + 3 * (size_t)c->nv);
+    if (tm) c->tendonmask.assign(tm, tm + c->nt);
+    return XPBD_OK;
+}
+
+int32_t xpbd_set_activation(xpbd_ctx *c, const float *act) {
+    if (!c || !act || c->nt == 0) return fail(c, XPBD_EINVAL, "bad activation");
+    c->activation.assign(act, act + c->nt);
+    if (c->ready && (c->feature & F_ANISO)) {
+        HIPCHK(c, hipMemcpyAsync(c->d_act, c->activation.data(), c->nt * sizeof(float),
+                                 hipMemcpyHostToDevice, c->stream));
+        const int64_t n = (int64_t)c->L.meta.size();
+        hipLaunchKernelGGL(k_refresh_fiber, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n,
+                           (const float4 *)c->FIBS, (const int32_t *)c->d_ptet_tetid, (const float *)c->d_act,
+                           c->sp.contraction_ratio, c->FIB);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return XPBD_OK;
+}
+
+int32_t xpbd_upload_constraints(xpbd_ctx *c, int32_t n, const int32_t *type, const int32_t *pts,
+                                const float *stiffness, const float *comp, const float *damp,
+                                const int32_t *tetid, const float *restlength, const float *restvector,
+                                const float *restdir) {
+    if (!c || n < 0) return fail(c, XPBD_EINVAL, "bad constraint table");
+    if (n > 0 && (!type || !pts || !stiffness || !comp || !damp || !tetid || !restlength || !restvector ||
+                  !restdir))
+        return fail(c, XPBD_EINVAL, "null constraint field");
+    if (c->ready) free_device(c);
+    c->ncons = n;
+    c->c_type.assign(type, type + n);
+    c->c_pts.assign(pts, pts + 4 * (size_t)n);
+    c->c_stiff.assign(stiffness, stiffness + n);
+    c->c_comp.assign(comp, comp + n);
+    c->c_damp.assign(damp, damp + n);
+    c->c_tetid.assign(tetid, tetid + n);
+    c->c_rest.assign(restlength, restlength + n);
+    c->c_rvec.assign(restvector, restvector + 4 * (size_t)n);
+    c->c_rdir.assign(restdir, restdir + 3 * (size_t)n);
+    return XPBD_OK;
+}
+
+int32_t xpbd_set_attach_targets(xpbd_ctx *c, int32_t n, const float *xyz) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (n != (int)c->L.attach_vertex.size() || (n > 0 && !xyz))
+        return fail(c, XPBD_EINVAL, "attach target count mismatch");
+    for (int i = 0; i < n; ++i) {
+        float *a0 = &c->L.att0[4 * (size_t)c->L.attach_vertex[i]];
+        a0[0] = xyz[3 * i];
+        a0[1] = xyz[3 * i + 1];
+        a0[2] = xyz[3 * i + 2];
+    }
+    HIPCHK(c, hipMemcpyAsync(c->ATT0, c->L.att0.data(), c->L.att0.size() * sizeof(float), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return XPBD_OK;
+}
+
+int32_t xpbd_set_params(xpbd_ctx *c, const xpbd_params *p) {
+    if (!c || !p) return fail(c, XPBD_EINVAL, "null params");
+    if (!(p->dt > 0.0f)) return fail(c, XPBD_EINVAL, "dt must be > 0");
+    SimParams s{};
+    s.dt = p->dt;
+    s.inv_dt2 = 1.0f / (p->dt * p->dt);
+    s.gx = p->gravity[0];
+    s.gy = p->gravity[1];
+    s.gz = p->gravity[2];
+    s.veldamping = p->veldamping;
+    s.quasi_static = p->quasi_static;
+    s.contraction_ratio = p->contraction_ratio;
+    s.fiber_scale = p->fiber_stiffness_scale;
+    s.has_compress = p->has_compress;
+    s.exp_damp = p->exp_damp;
+    s.const_damp = p->const_damp;
+    const bool cr_changed = s.contraction_ratio != c->sp.contraction_ratio;
+    c->sp = s;
+    c->params_set = true;
+    invalidate_graph(c);
+    if (c->ready && cr_changed && (c->feature & F_ANISO)) return xpbd_set_activation(c, c->activation.data());
+    return XPBD_OK;
+}
+
+int32_t xpbd_set_pins(xpbd_ctx *c, int32_t n, const int32_t *vids, const int32_t *active) {
+    if (!c || n < 0 || (n > 0 && (!vids || !active))) return fail(c, XPBD_EINVAL, "bad pins");
+    for (int i = 0; i < n; ++i)
+        if (vids[i] < 0 || vids[i] >= c->nv) return fail(c, XPBD_EINVAL, "pin vertex out of range");
+    c->pin_vids.assign(vids, vids + n);
+    c->pin_active.assign(active, active + n);
+    c->pin_targets.clear();
+    invalidate_graph(c);
+    return c->ready ? upload_pins(c) : XPBD_OK;
+}
+
+int32_t xpbd_set_pin_targets(xpbd_ctx *c, int32_t n, const float *xyz) {
+    if (!c || n != (int)c->pin_vids.size() || (n > 0 && !xyz)) return fail(c, XPBD_EINVAL, "pin target count");
+    const bool was_on = c->pins_on;
+    c->pin_targets.assign(xyz, xyz + 3 * (size_t)n);
+    if (!c->ready) return XPBD_OK;
+    int r = upload_pins(c);
+    if (r) return r;
+    if (was_on != c->pins_on) invalidate_graph(c);
+    return XPBD_OK;
+}
+
+int32_t xpbd_set_sdf(xpbd_ctx *c, int32_t nbones, int32_t res, const float *grids, const float *origins,
+                     const float *spacing, float contact_offset, float max_push, const float *coll_w) {
+    if (!c || c->nv == 0) return fail(c, XPBD_EINVAL, "no mesh");
+    if (nbones < 0 || (nbones > 0 && (res < 2 || !grids || !origins || !spacing)))
+        return fail(c, XPBD_EINVAL, "bad sdf grid");
+    c->sdf_invh.assign(nbones, 0.f);
+    for (int b = 0; b < nbones; ++b) {
+        if (!(spacing[b] > 0.0f)) return fail(c, XPBD_EINVAL, "sdf spacing must be > 0");
+        c->sdf_invh[b] = 1.0f / spacing[b];
+    }
+    c->nbones = nbones;
+    c->res = res;
+    c->sdf_grids.assign(grids, grids + (size_t)nbones * res * res * res);
+    c->sdf_origin.assign(origins, origins + 3 * (size_t)nbones);
+    c->sdf_offset = contact_offset;
+    c->sdf_maxpush = max_push;
+    if (coll_w) c->coll_w.assign(coll_w, coll_w + c->nv);
+    else c->coll_w.assign(c->nv, 1.0f);
+    c->goff.clear();
+    c->vgroup.clear();
+    if ((int)c->bone_xf.size() != 12 * nbones) {
+        c->bone_xf.assign(12 * (size_t)nbones, 0.f);
+        for (int b = 0; b < nbones; ++b) {
+            c->bone_xf[12 * b + 0] = c->bone_xf[12 * b + 5] = c->bone_xf[12 * b + 10] = 1.f;
+        }
+    }
+    if (c->ready) free_device(c);
+    return XPBD_OK;
+}
+
+int32_t xpbd_set_sdf_groups(xpbd_ctx *c, int32_t ngroups, const int32_t *goff, const int32_t *vgroup) {
+    if (!c || c->nv == 0 || ngroups <= 0 || !goff || !vgroup) return fail(c, XPBD_EINVAL, "bad sdf groups");
+    if (goff[0] != 0 || goff[ngroups] != c->nbones) return fail(c, XPBD_EINVAL, "group offsets must span [0, nbones]");
+    for (int g = 0; g < ngroups; ++g)
+        if (goff[g + 1] < goff[g]) return fail(c, XPBD_EINVAL, "group offsets must be non-decreasing");
+    for (int i = 0; i < c->nv; ++i)
+        if (vgroup[i] < 0 || vgroup[i] >= ngroups) return fail(c, XPBD_EINVAL, "vertex group out of range");
+    c->goff.assign(goff, goff + ngroups + 1);
+    c->vgroup.assign(vgroup, vgroup + c->nv);
+    if (c->ready) free_device(c);
+    return XPBD_OK;
+}
+
+int32_t xpbd_set_bone_transforms(xpbd_ctx *c, int32_t nbones, const float *xf) {
+    if (!c || nbones != c->nbones || (nbones > 0 && !xf)) return fail(c, XPBD_EINVAL, "bone count mismatch");
+    c->bone_xf.assign(xf, xf + 12 * (size_t)nbones);
+    if (c->ready && nbones > 0) {
+        std::vector<float> bn(16 * (size_t)nbones, 0.f);
+        for (int b = 0; b < nbones; ++b) {
+            std::memcpy(&bn[16 * b], &c->bone_xf[12 * b], 12 * sizeof(float));
+            std::memcpy(&bn[16 * b + 12], &c->sdf_origin[3 * b], 3 * sizeof(float));
+            bn[16 * b + 15] = c->sdf_invh[b];
+        }
+        HIPCHK(c, hipMemcpyAsync(c->d_bone, bn.data(), bn.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return XPBD_OK;
+}
+
+int32_t xpbd_finalize(xpbd_ctx *c, const xpbd_tuning *t) {
+    if (!c || c->nv == 0) return fail(c, XPBD_EINVAL, "no mesh");
+    if (!c->params_set) return fail(c, XPBD_EINVAL, "call xpbd_set_params() first");
+    HIPCHK(c, hipSetDevice(c->device));
+    free_device(c);
+    int P = (t && t->patch_verts > 0) ? t->patch_verts : 256;
+    if (P != 64 && P != 128 && P != 256) return fail(c, XPBD_EINVAL, "patch_verts must be 64, 128 or 256");
+    c->P = P;
+    xpbd::HostInput in;
+    in.nv = c->nv;
+    in.nt = c->nt;
+    in.pos0 = c->pos0.data();
+    in.tets = c->tets.data();
+    in.rest_matrix = c->rest_matrix.data();
+    in.v_fiber_dir = c->fiber.empty() ? nullptr : c->fiber.data();
+    in.tendonmask = c->tendonmask.data();
+    in.ncons = c->ncons;
+    in.type = c->c_type.data();
+    in.pts = c->c_pts.data();
+    in.tetid = c->c_tetid.data();
+    in.stiffness = c->c_stiff.data();
+    in.compressionstiffness = c->c_comp.data();
+    in.dampingratio = c->c_damp.data();
+    in.restlength = c->c_rest.data();
+    in.restvector = c->c_rvec.data();
+    in.restdir = c->c_rdir.data();
+    in.patch_verts = P;
+    c->L = xpbd::Layout();
+    int code = 0;
+    std::string m = xpbd::build_layout(in, c->L, &code);
+    if (code) return fail(c, code, m);
+    c->feature = c->L.feature;
+    const size_t lds = iter_lds(c);
+    int maxlds = 0;
+    HIPCHK(c, hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+    if (lds > (size_t)maxlds)
+        return fail(c, XPBD_ELAYOUT, "patch vertex table needs " + std::to_string(lds) + " B of LDS; limit " +
+                                         std::to_string(maxlds));
+    if (lds > 65536) {
+        IterKernel k = pick_iter(c->P, c->feature, MODE_FUSED);
+        HIPCHK(c, hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        IterKernel k2 = pick_iter(c->P, c->feature, MODE_ACCUM);
+        HIPCHK(c, hipFuncSetAttribute((const void *)k2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
+    const xpbd::Layout &L = c->L;
+    const int nv = c->nv;
+    const size_t nptet = L.meta.size();
+    int r;
+#define UP(dst, vec) do { r = dupload(c, &dst, (vec).data(), (vec).size()); if (r) return r; } while (0)
+    for (int b = 0; b < 2; ++b) if ((r = dalloc(c, &c->X[b], nv))) return r;
+    if ((r = dalloc(c, &c->PPv, nv))) return r;
+    if ((r = dalloc(c, &c->VEL, nv))) return r;
+    if ((r = dalloc(c, &c->FRC, nv))) return r;
+    if ((r = dalloc(c, &c->PIN, nv))) return r;
+    if ((r = dalloc(c, &c->DPW, nv))) return r;
+    if ((r = dalloc(c, &c->FLAGS, nv))) return r;
+    r = dupload(c, &c->ATT0, (const float4 *)L.att0.data(), (size_t)nv); if (r) return r;
+    r = dupload(c, &c->ATT1, (const float4 *)L.att1.data(), (size_t)nv); if (r) return r;
+    UP(c->d_patch, L.patches);
+    UP(c->d_halo, L.halo);
+    r = dupload(c, &c->d_lidx, (const ushort4 *)L.lidx.data(), nptet); if (r) return r;
+    UP(c->d_meta, L.meta);
+    UP(c->d_Bp, L.Bp);
+    UP(c->d_rvol, L.rvol);
+    UP(c->d_rsnh, L.rsnh);
+    r = dupload(c, &c->FIBS, (const float4 *)L.fib.data(), nptet); if (r) return r;
+    r = dupload(c, &c->FIB, (const float4 *)L.fib.data(), nptet); if (r) return r;
+    UP(c->d_ptet_tetid, L.ptet_tetid);
+    UP(c->d_cls, L.classes);
+    UP(c->d_vslot_off, L.vslot_off);
+    UP(c->d_vslot, L.vslot);
+    UP(c->d_act, c->activation);
+    // per-ptet global vertex ids (residual kernel)
+    {
+        std::vector<int32_t> pv(4 * nptet);
+        for (const auto &P_ : L.patches)
+            for (int j = 0; j < P_.ntet; ++j) {
+                const size_t g = (size_t)P_.tbeg + j;
+                for (int k = 0; k < 4; ++k) {
+                    const int li = L.lidx[4 * g + k];
+                    pv[4 * g + k] = li < P_.nown ? P_.vbeg + li : L.halo[P_.hbeg + li - P_.nown];
+                }
+            }
+        UP(c->d_pvert, pv);
+    }
+    {
+        std::vector<int4> tt(c->nt);
+        for (int i = 0; i < c->nt; ++i)
+            tt[i] = make_int4(L.iperm[c->tets[4 * i]], L.iperm[c->tets[4 * i + 1]], L.iperm[c->tets[4 * i + 2]],
+                              L.iperm[c->tets[4 * i + 3]]);
+        UP(c->d_tets, tt);
+        UP(c->d_B, c->rest_matrix);
+    }
+    if (c->nbones > 0) {
+        UP(c->d_grids, c->sdf_grids);
+        if (c->goff.empty()) { c->goff = {0, c->nbones}; c->vgroup.assign(nv, 0); }
+        std::vector<int32_t> vg(nv);
+        for (int n = 0; n < nv; ++n) vg[n] = c->vgroup[L.perm[n]];
+        UP(c->d_vgroup, vg);
+        UP(c->d_goff, c->goff);
+        if ((r = dalloc(c, &c->d_bone, 16 * (size_t)c->nbones))) return r;
+    }
+    if ((r = dalloc(c, &c->d_cells, 2 * (size_t)nv))) return r;
+    if ((r = dalloc(c, &c->d_count, 1))) return r;
+    c->n_partial = 512;
+    if ((r = dalloc(c, &c->d_partial, 8 * (size_t)c->n_partial))) return r;
+#undef UP
+    // vertex state from rest pose
+    {
+        std::vector<float4> x(nv), z(nv, make_float4(0, 0, 0, 0));
+        for (int n = 0; n < nv; ++n) {
+            const int o = L.perm[n];
+            x[n] = make_float4(c->pos0[3 * o], c->pos0[3 * o + 1], c->pos0[3 * o + 2], 0.f);
+        }
+        HIPCHK(c, hipMemcpy(c->X[0], x.data(), nv * sizeof(float4), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->PPv, x.data(), nv * sizeof(float4), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->VEL, z.data(), nv * sizeof(float4), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->FRC, z.data(), nv * sizeof(float4), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->DPW, z.data(), nv * sizeof(float4), hipMemcpyHostToDevice));
+    }
+    c->ready = true;
+    if ((r = upload_vertex_static(c))) return r;
+    if ((r = upload_pins(c))) return r;
+    if (c->has_force && (r = xpbd_set_force(c, c->force.data()))) return r;
+    if ((r = xpbd_set_bone_transforms(c, c->nbones, c->bone_xf.data()))) return r;
+    if (c->feature & F_ANISO)
+        if ((r = xpbd_set_activation(c, c->activation.data()))) return r;
+    return XPBD_OK;
+}
+
+int32_t xpbd_get_stats(xpbd_ctx *c, xpbd_stats *o) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!o) return fail(c, XPBD_EINVAL, "null stats");
+    o->nv = c->nv;
+    o->nt = c->nt;
+    o->n_elems = c->L.n_elems;
+    o->n_patches = (int)c->L.patches.size();
+    o->n_patch_tets = (int)c->L.meta.size();
+    o->max_table = c->L.max_table;
+    o->max_patch_tets = c->L.max_ptets;
+    o->feature_mask = c->feature;
+    o->bytes_per_iter_alg = c->L.bytes_alg;
+    o->bytes_per_iter_impl = c->L.bytes_impl;
+    return XPBD_OK;
+}
+
+int32_t xpbd_set_state(xpbd_ctx *c, const float *pos, const float *pprev, const float *vel) {
+    int r = check_ready(c);
+    if (r) return r;
+    const int nv = c->nv;
+    std::vector<float4> x(nv), pp(nv), v(nv);
+    HIPCHK(c, hipMemcpyAsync(x.data(), c->X[0], nv * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(pp.data(), c->PPv, nv * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(v.data(), c->VEL, nv * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int n = 0; n < nv; ++n) {
+        const int o = c->L.perm[n];
+        if (pos) { x[n].x = pos[3 * o]; x[n].y = pos[3 * o + 1]; x[n].z = pos[3 * o + 2]; }
+        if (pprev) { pp[n].x = pprev[3 * o]; pp[n].y = pprev[3 * o + 1]; pp[n].z = pprev[3 * o + 2]; }
+        if (vel) { v[n].x = vel[3 * o]; v[n].y = vel[3 * o + 1]; v[n].z = vel[3 * o + 2]; }
+    }
+    HIPCHK(c, hipMemcpy(c->X[0], x.data(), nv * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->PPv, pp.data(), nv * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->VEL, v.data(), nv * sizeof(float4), hipMemcpyHostToDevice));
+    return XPBD_OK;
+}
+
+int32_t xpbd_get_state(xpbd_ctx *c, float *pos, float *pprev, float *vel) {
+    int r = check_ready(c);
+    if (r) return r;
+    const int nv = c->nv;
+    std::vector<float4> x(nv), pp(nv), v(nv);
+    HIPCHK(c, hipMemcpyAsync(x.data(), c->X[0], nv * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(pp.data(), c->PPv, nv * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(v.data(), c->VEL, nv * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int n = 0; n < nv; ++n) {
+        const int o = c->L.perm[n];
+        if (pos) { pos[3 * o] = x[n].x; pos[3 * o + 1] = x[n].y; pos[3 * o + 2] = x[n].z; }
+        if (pprev) { pprev[3 * o] = pp[n].x; pprev[3 * o + 1] = pp[n].y; pprev[3 * o + 2] = pp[n].z; }
+        if (vel) { vel[3 * o] = v[n].x; vel[3 * o + 1] = v[n].y; vel[3 * o + 2] = v[n].z; }
+    }
+    return XPBD_OK;
+}
+
+int32_t xpbd_integrate(xpbd_ctx *c) {
+    int r = check_ready(c);
+    return r ? r : launch_integrate(c);
+}
+
+int32_t xpbd_clear(xpbd_ctx *c) {
+    int r = check_ready(c);
+    if (r) return r;
+    hipLaunchKernelGGL(k_zero4, vgrid(c->nv), dim3(256), 0, c->stream, (int64_t)c->nv, c->DPW);
+    HIPCHK(c, hipGetLastError());
+    return XPBD_OK;
+}
+
+int32_t xpbd_solve_constraints(xpbd_ctx *c) {
+    int r = check_ready(c);
+    return r ? r : launch_iter(c, c->X[0], c->X[1], MODE_ACCUM);
+}
+
+int32_t xpbd_apply_dP(xpbd_ctx *c) {
+    int r = check_ready(c);
+    if (r) return r;
+    hipLaunchKernelGGL(k_apply_dpw, vgrid(c->nv), dim3(256), 0, c->stream, c->nv, c->X[0],
+                       (const float4 *)c->DPW, (const float4 *)c->PIN, 0);
+    HIPCHK(c, hipGetLastError());
+    return XPBD_OK;
+}
+
+int32_t xpbd_apply_pins(xpbd_ctx *c) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!c->pins_on) return XPBD_OK;
+    hipLaunchKernelGGL(k_pins, vgrid(c->nv), dim3(256), 0, c->stream, c->nv, c->X[0], (const float4 *)c->PIN);
+    HIPCHK(c, hipGetLastError());
+    return XPBD_OK;
+}
+
+int32_t xpbd_iterate(xpbd_ctx *c, int32_t n) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (n < 0) return fail(c, XPBD_EINVAL, "negative iteration count");
+    for (int it = 0; it < n; ++it)
+        if ((r = launch_iter(c, c->X[it & 1], c->X[(it + 1) & 1], MODE_FUSED))) return r;
+    if (n & 1) {
+        HIPCHK(c, hipMemcpyAsync(c->X[0], c->X[1], c->nv * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+    }
+    return XPBD_OK;
+}
+
+int32_t xpbd_collide(xpbd_ctx *c) {
+    int r = check_ready(c);
+    return r ? r : launch_post(c, c->X[0], 1, 0);
+}
+
+int32_t xpbd_update_velocities(xpbd_ctx *c) {
+    int r = check_ready(c);
+    return r ? r : launch_post(c, c->X[0], 0, 1);
+}
+
+int32_t xpbd_substep(xpbd_ctx *c, int32_t iters) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (iters < 0) return fail(c, XPBD_EINVAL, "negative iteration count");
+    return enqueue_substep(c, iters);
+}
+
+int32_t xpbd_step_frame(xpbd_ctx *c, int32_t nsub, int32_t iters, int32_t use_graph) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (nsub < 0 || iters < 0) return fail(c, XPBD_EINVAL, "negative count");
+    if (!use_graph || c->timing) {
+        for (int s = 0; s < nsub; ++s)
+            if ((r = enqueue_substep(c, iters))) return r;
+        return XPBD_OK;
+    }
+    if (!c->gexec || c->g_nsub != nsub || c->g_iters != iters) {
+        invalidate_graph(c);
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (int s = 0; s < nsub && r == XPBD_OK; ++s) r = enqueue_substep(c, iters);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (r) { if (g) (void)hipGraphDestroy(g); return r; }
+        if (e != hipSuccess) return fail(c, XPBD_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        c->graph = g;
+        HIPCHK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+        c->g_nsub = nsub;
+        c->g_iters = iters;
+    }
+    HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    return XPBD_OK;
+}
+
+int32_t xpbd_synchronize(xpbd_ctx *c) {
+    if (!c) return XPBD_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return XPBD_OK;
+}
+
+int32_t xpbd_residuals(xpbd_ctx *c, double *out8) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!out8) return fail(c, XPBD_EINVAL, "null output");
+    IterArgs a = make_args(c, c->DPW);
+    hipLaunchKernelGGL(k_residual<256>, dim3(c->n_partial), dim3(256), 0, c->stream, a, (const float4 *)c->X[0],
+                       (const int32_t *)c->d_pvert, c->d_partial);
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> h(8 * (size_t)c->n_partial);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->d_partial, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 8; ++k) out8[k] = 0.0;
+    for (int b = 0; b < c->n_partial; ++b)
+        for (int k = 0; k < 8; ++k) out8[k] += h[8 * (size_t)b + k];
+    return XPBD_OK;
+}
+
+int32_t xpbd_count_inverted(xpbd_ctx *c, int32_t *out) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!out) return fail(c, XPBD_EINVAL, "null output");
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, c->stream, 1, c->d_count);
+    HIPCHK(c, hipGetLastError());
+    if (c->nt > 0) {
+        hipLaunchKernelGGL(k_count_inverted, vgrid(c->nt), dim3(256), 0, c->stream, c->nt, (const int4 *)c->d_tets,
+                           (const float *)c->d_B, (const float4 *)c->X[0], c->d_count);
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipMemcpyAsync(out, c->d_count, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return XPBD_OK;
+}
+
+int32_t xpbd_get_sdf_cells(xpbd_ctx *c, int32_t *cells) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!cells) return fail(c, XPBD_EINVAL, "null output");
+    std::vector<int32_t> h(2 * (size_t)c->nv);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->d_cells, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int n = 0; n < c->nv; ++n) {
+        const int o = c->L.perm[n];
+        cells[2 * o] = h[2 * n];
+        cells[2 * o + 1] = h[2 * n + 1];
+    }
+    return XPBD_OK;
+}
+
+int32_t xpbd_kernel_timing(xpbd_ctx *c, int32_t enable) {
+    if (!c) return XPBD_EINVAL;
+    if (enable) {
+        for (auto &e : c->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+        c->evs.clear();
+        c->timed_ms = 0.0;
+        c->timed_n = 0;
+    }
+    c->timing = enable != 0;
+    return XPBD_OK;
+}
+
+int32_t xpbd_get_kernel_timing(xpbd_ctx *c, double *total_ms, int32_t *launches) {
+    if (!c) return XPBD_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (auto &e : c->evs) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, e.a, e.b));
+        c->timed_ms += ms;
+        c->timed_n += 1;
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    c->evs.clear();
+    if (total_ms) *total_ms = c->timed_ms;
+    if (launches) *launches = c->timed_n;
+    return XPBD_OK;
+}
+
+}  // extern "C"

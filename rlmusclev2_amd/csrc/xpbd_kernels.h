@@ -1,0 +1,51 @@
+// Device kernels of the XPBD substep (gfx950).  Included by xpbd.hip only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "xpbd_layout.h"
+
+namespace xpbd {
+
+struct SimParams {
+    float dt, inv_dt2;
+    float gx, gy, gz;
+    float veldamping;
+    int quasi_static;
+    float contraction_ratio, fiber_scale;
+    int has_compress;
+    float exp_damp, const_damp;
+};
+
+struct SdfParams {
+    int nbones, res;
+    float contact_offset, max_push;
+    const float *grids;      // nbones * res^3
+    const float *bone;       // per bone: 12 xform + 3 origin + inv spacing = 16 floats
+    const int32_t *vgroup;   // per vertex (new order): bone group
+    const int32_t *goff;     // ngroups + 1 bone offsets
+};
+
+struct IterArgs {
+    const PatchDesc *patch;
+    const int32_t *halo;
+    const ushort4 *lidx;
+    const uint32_t *meta;
+    const float *Bp;         // 9 planes of nptet
+    const float *rvol, *rsnh;
+    const float4 *fib;       // f0.xyz, fiberscale (per ptet, refreshed from activation)
+    const ClassParams *cls;
+    const int32_t *vslot_off;
+    const uint16_t *vslot;
+    const float4 *pin;       // target.xyz, active
+    const float4 *att0, *att1;
+    float4 *dpw;             // accumulate mode output
+    int64_t nptet;
+    int max_table;
+    SimParams sp;
+};
+
+enum { MODE_FUSED = 0, MODE_ACCUM = 1 };
+
+}  // namespace xpbd

@@ -1,0 +1,438 @@
+// Host-side layout builder for the patch-owned Jacobi projection.
+#include "xpbd_layout.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <unordered_map>
+
+#pragma clang fp contract(off)
+
+namespace xpbd {
+
+namespace {
+
+constexpr int32_t CT_ATTACH = 1650556;
+constexpr int32_t CT_PINTOTARGET = 771624311;
+constexpr int32_t CT_TETVOLUME = -215389979;
+constexpr int32_t CT_TETARAPSVD = -92199132;
+constexpr int32_t CT_TETARAPANISO = -92199133;
+constexpr int32_t CT_TETSNH = -503462311;
+
+inline uint64_t spread21(uint64_t v) {
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+}
+
+struct Elem {
+    int32_t p[4];
+    int32_t tetid;
+    uint32_t mask = 0;
+    ClassParams cp{};
+    float rvol = 0.f, rsnh = 0.f;
+    float f0[3] = {0, 0, 0};
+    float belly = 0.f;
+};
+
+struct KeyHash {
+    size_t operator()(const std::array<int32_t, 5> &k) const {
+        uint64_t h = 1469598103934665603ULL;
+        for (int32_t v : k) { h ^= (uint32_t)v; h *= 1099511628211ULL; }
+        return (size_t)h;
+    }
+};
+
+}  // namespace
+
+void precompute_rest(int nv, const float *pos0, int nt, const int32_t *tets, float density,
+                     float *rest_volume, float *rest_matrix, float *mass) {
+    std::fill(mass, mass + nv, 0.0f);
+    for (int c = 0; c < nt; ++c) {
+        const int32_t *t = tets + 4 * c;
+        const float *p3 = pos0 + 3 * t[3];
+        float col[3][3];
+        for (int k = 0; k < 3; ++k)
+            for (int a = 0; a < 3; ++a) col[k][a] = pos0[3 * t[k] + a] - p3[a];
+        // Dm row-major: Dm[a][k] = col[k][a]
+        float m[9] = {col[0][0], col[1][0], col[2][0], col[0][1], col[1][1],
+                      col[2][1], col[0][2], col[1][2], col[2][2]};
+        float d = m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+                  m[2] * (m[3] * m[7] - m[4] * m[6]);
+        float vol = std::fabs(d) / 6.0f;
+        rest_volume[c] = vol;
+        float mc = vol * density / 4.0f;
+        for (int i = 0; i < 4; ++i) mass[t[i]] += mc;
+        float *o = rest_matrix + 9 * c;
+        if (d == 0.0f) {
+            std::memset(o, 0, 9 * sizeof(float));
+            continue;
+        }
+        float inv = 1.0f / d;
+        o[0] = (m[4] * m[8] - m[5] * m[7]) * inv;
+        o[1] = (m[2] * m[7] - m[1] * m[8]) * inv;
+        o[2] = (m[1] * m[5] - m[2] * m[4]) * inv;
+        o[3] = (m[5] * m[6] - m[3] * m[8]) * inv;
+        o[4] = (m[0] * m[8] - m[2] * m[6]) * inv;
+        o[5] = (m[2] * m[3] - m[0] * m[5]) * inv;
+        o[6] = (m[3] * m[7] - m[4] * m[6]) * inv;
+        o[7] = (m[1] * m[6] - m[0] * m[7]) * inv;
+        o[8] = (m[0] * m[4] - m[1] * m[3]) * inv;
+    }
+}
+
+std::string build_layout(const HostInput &in, Layout &L, int *code) {
+    *code = 0;
+    const int nv = in.nv;
+    const int P = in.patch_verts;
+    if (P < 32 || P > 256 || (P & 31)) { *code = 1; return "patch_verts must be a multiple of 32 in [32,256]"; }
+
+    // ---- 1. constraints -> fused tet elements ---------------------------
+    std::vector<Elem> elems;
+    std::unordered_map<std::array<int32_t, 5>, std::vector<int32_t>, KeyHash> by_key;
+    L.att0.assign(4 * (size_t)nv, 0.f);
+    L.att1.assign(4 * (size_t)nv, 0.f);
+    L.pin_from_cons.assign(4 * (size_t)nv, 0.f);
+    std::vector<int32_t> attach_old;  // attach #i -> old vertex
+    std::vector<uint8_t> has_att(nv, 0);
+    int feature = 0;
+    for (int c = 0; c < in.ncons; ++c) {
+        const int32_t t = in.type[c];
+        const int32_t *pt = in.pts + 4 * c;
+        const float k = in.stiffness[c];
+        if (t == CT_ATTACH) {
+            int v = pt[0];
+            if (v < 0 || v >= nv) { *code = 1; return "ATTACH vertex out of range"; }
+            attach_old.push_back(v);
+            if (has_att[v]) { *code = 2; return "more than one ATTACH constraint on a vertex"; }
+            has_att[v] = 1;
+            float kc = in.compressionstiffness[c];
+            kc = kc >= 0.0f ? kc : k;
+            float *a0 = &L.att0[4 * (size_t)v], *a1 = &L.att1[4 * (size_t)v];
+            a0[0] = in.restvector[4 * c];
+            a0[1] = in.restvector[4 * c + 1];
+            a0[2] = in.restvector[4 * c + 2];
+            a0[3] = k;  // k <= 0 -> skipped in the kernel (solve_attach_kernel early return)
+            a1[0] = in.restlength[c];
+            a1[1] = kc;
+            a1[2] = in.dampingratio[c];
+            if (k > 0.0f) {
+                feature |= F_ATTACH;
+                if (in.dampingratio[c] > 0.0f) feature |= F_DAMP;
+            }
+            continue;
+        }
+        if (t == CT_PINTOTARGET) {
+            int v = pt[0];
+            if (v < 0) continue;
+            if (v >= nv) { *code = 1; return "PINTOTARGET vertex out of range"; }
+            float *q = &L.pin_from_cons[4 * (size_t)v];
+            q[0] = in.restvector[4 * c] + in.restdir[3 * c];
+            q[1] = in.restvector[4 * c + 1] + in.restdir[3 * c + 1];
+            q[2] = in.restvector[4 * c + 2] + in.restdir[3 * c + 2];
+            q[3] = 1.0f;
+            feature |= F_PINS;
+            continue;
+        }
+        uint32_t bit;
+        if (t == CT_TETARAPSVD) bit = M_ARAP;
+        else if (t == CT_TETVOLUME) bit = M_VOL;
+        else if (t == CT_TETSNH) bit = M_SNH;
+        else if (t == CT_TETARAPANISO) bit = M_ANISO;
+        else { *code = 2; return "constraint type " + std::to_string(t) + " is not on the HIP path"; }
+        for (int i = 0; i < 4; ++i)
+            if (pt[i] < 0 || pt[i] >= nv) { *code = 1; return "tet constraint vertex out of range"; }
+        int32_t tid = in.tetid[c];
+        if ((bit & (M_ARAP | M_SNH | M_ANISO)) && (tid < 0 || tid >= in.nt)) {
+            *code = 1;
+            return "tet constraint tetid out of range";
+        }
+        // static early-outs of the reference become "constraint absent"
+        if (k <= 0.0f) continue;  // every tet kernel returns when c_stiffness <= 0
+        float f0[3] = {0, 0, 0};
+        if (bit == M_ANISO) {
+            if (!in.v_fiber_dir) { *code = 1; return "aniso_arap needs v_fiber_dir"; }
+            // f0 = (sum of 4 vertex fiber dirs)/4, normalised (solve_kernels.py:1345-1350)
+            float s[3];
+            for (int a = 0; a < 3; ++a) {
+                s[a] = in.v_fiber_dir[3 * pt[0] + a] + in.v_fiber_dir[3 * pt[1] + a];
+                s[a] = s[a] + in.v_fiber_dir[3 * pt[2] + a];
+                s[a] = s[a] + in.v_fiber_dir[3 * pt[3] + a];
+                s[a] = s[a] / 4.0f;
+            }
+            float fn = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+            if (fn < 1.0e-9f) continue;
+            for (int a = 0; a < 3; ++a) f0[a] = s[a] / fn;
+        }
+        // one element per vertex 4-tuple; volume does not read rest_matrix, so it
+        // joins the element of any tetid
+        int target = -1;
+        auto place = [&](std::vector<int32_t> &lst) {
+            for (int e : lst)
+                if (!(elems[e].mask & bit) &&
+                    (bit == M_VOL || elems[e].tetid < 0 || elems[e].tetid == tid)) {
+                    target = e;
+                    return;
+                }
+        };
+        std::array<int32_t, 5> kk{pt[0], pt[1], pt[2], pt[3], 0};
+        auto it = by_key.find(kk);
+        if (it != by_key.end()) place(it->second);
+        if (target < 0) {
+            Elem e;
+            for (int i = 0; i < 4; ++i) e.p[i] = pt[i];
+            e.tetid = (bit == M_VOL) ? -1 : tid;
+            target = (int)elems.size();
+            elems.push_back(e);
+            by_key[kk].push_back(target);
+        }
+        Elem &e = elems[target];
+        e.mask |= bit;
+        if (bit != M_VOL) e.tetid = tid;
+        const float kd = in.dampingratio[c];
+        if (bit == M_ARAP) {
+            e.cp.k_arap = k;
+        } else if (bit == M_VOL) {
+            float kc = in.compressionstiffness[c];
+            e.cp.k_vol = k;
+            e.cp.kcomp_vol = kc >= 0.0f ? kc : k;
+            e.cp.damp_vol = kd;
+            e.rvol = in.restlength[c];
+            if (kd > 0.0f) feature |= F_DAMP;
+        } else if (bit == M_SNH) {
+            e.cp.mu = k;
+            e.cp.lam = in.restdir[3 * c];
+            e.cp.damp_snh = kd;
+            e.rsnh = in.restlength[c];
+            if (kd > 0.0f) feature |= F_DAMP;
+        } else {
+            e.cp.k_aniso = k;
+            for (int a = 0; a < 3; ++a) e.f0[a] = f0[a];
+            e.belly = 1.0f - (in.tendonmask ? in.tendonmask[tid] : 0.0f);
+        }
+        feature |= (int)bit;
+    }
+    for (auto &e : elems)
+        if (e.tetid < 0) e.tetid = 0;  // volume-only element: tetid unused
+    L.feature = feature;
+    L.n_elems = (int)elems.size();
+
+    // ---- 2. material classes ----------------------------------------------
+    std::vector<uint32_t> ecls(elems.size());
+    {
+        std::unordered_map<std::string, uint32_t> seen;
+        for (size_t i = 0; i < elems.size(); ++i) {
+            std::string key((const char *)&elems[i].cp, sizeof(ClassParams));
+            auto f = seen.find(key);
+            if (f == seen.end()) {
+                uint32_t id = (uint32_t)L.classes.size();
+                if (id > 0xffff) { *code = 5; return "too many distinct material classes"; }
+                L.classes.push_back(elems[i].cp);
+                seen.emplace(key, id);
+                ecls[i] = id;
+            } else {
+                ecls[i] = f->second;
+            }
+        }
+        if (L.classes.empty()) L.classes.push_back(ClassParams{});
+    }
+
+    // ---- 3. vertex renumbering: Morton order of rest positions -----------
+    float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+    for (int v = 0; v < nv; ++v)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], in.pos0[3 * v + a]);
+            hi[a] = std::max(hi[a], in.pos0[3 * v + a]);
+        }
+    double ext = 0;
+    for (int a = 0; a < 3; ++a) ext = std::max(ext, (double)hi[a] - lo[a]);
+    if (!(ext > 0)) ext = 1;
+    auto morton = [&](const float *x) {
+        uint64_t c = 0;
+        for (int a = 0; a < 3; ++a) {
+            double q = ((double)x[a] - lo[a]) / ext * 2097151.0;
+            uint64_t u = (uint64_t)std::min(std::max(q, 0.0), 2097151.0);
+            c |= spread21(u) << a;
+        }
+        return c;
+    };
+    std::vector<uint64_t> vcode(nv);
+    for (int v = 0; v < nv; ++v) vcode[v] = morton(in.pos0 + 3 * v);
+    L.perm.resize(nv);
+    std::iota(L.perm.begin(), L.perm.end(), 0);
+    std::stable_sort(L.perm.begin(), L.perm.end(),
+                     [&](int a, int b) { return vcode[a] < vcode[b]; });
+    L.iperm.assign(nv, 0);
+    for (int i = 0; i < nv; ++i) L.iperm[L.perm[i]] = i;
+    const int npatch = (nv + P - 1) / P;
+
+    // ---- 4. element order: Morton of centroid -----------------------------
+    std::vector<int32_t> eord(elems.size());
+    std::iota(eord.begin(), eord.end(), 0);
+    {
+        std::vector<uint64_t> ecode(elems.size());
+        for (size_t i = 0; i < elems.size(); ++i) {
+            float c[3] = {0, 0, 0};
+            for (int k = 0; k < 4; ++k)
+                for (int a = 0; a < 3; ++a) c[a] += 0.25f * in.pos0[3 * elems[i].p[k] + a];
+            ecode[i] = morton(c);
+        }
+        std::stable_sort(eord.begin(), eord.end(),
+                         [&](int a, int b) { return ecode[a] < ecode[b]; });
+    }
+
+    // ---- 5. (patch, element) entries ----------------------------------------
+    std::vector<int32_t> pcount(npatch + 1, 0);
+    auto patches_of = [&](const Elem &e, int *out) {
+        int n = 0;
+        for (int k = 0; k < 4; ++k) {
+            int p = L.iperm[e.p[k]] / P;
+            bool dup = false;
+            for (int j = 0; j < n; ++j) dup |= (out[j] == p);
+            if (!dup) out[n++] = p;
+        }
+        return n;
+    };
+    for (int e : eord) {
+        int ps[4];
+        int n = patches_of(elems[e], ps);
+        for (int j = 0; j < n; ++j) pcount[ps[j] + 1]++;
+    }
+    for (int p = 0; p < npatch; ++p) pcount[p + 1] += pcount[p];
+    const size_t nptet = (size_t)pcount[npatch];
+    if (nptet > (size_t)INT32_MAX / 4) { *code = 5; return "too many replicated tets"; }
+    std::vector<int32_t> pe(nptet);
+    {
+        std::vector<int32_t> fill(pcount.begin(), pcount.end() - 1);
+        for (int e : eord) {
+            int ps[4];
+            int n = patches_of(elems[e], ps);
+            for (int j = 0; j < n; ++j) pe[fill[ps[j]]++] = e;
+        }
+    }
+
+    // ---- 6. per patch tables ------------------------------------------------
+    L.patches.resize(npatch);
+    L.lidx.assign(4 * nptet, 0);
+    L.meta.assign(nptet, 0);
+    L.Bp.assign(9 * nptet, 0.f);
+    L.rvol.assign(nptet, 0.f);
+    L.rsnh.assign(nptet, 0.f);
+    L.fib.assign(4 * nptet, 0.f);
+    L.ptet_tetid.assign(nptet, 0);
+    L.vslot_off.assign(nv + 1, 0);
+    L.halo.clear();
+    std::vector<std::vector<uint16_t>> vs_tmp;  // reused per patch
+    int max_table = 0, max_ptets = 0;
+    std::vector<int32_t> hl;
+    for (int p = 0; p < npatch; ++p) {
+        const int vbeg = p * P, nown = std::min(P, nv - vbeg);
+        const int tb = pcount[p], nt_p = pcount[p + 1] - tb;
+        hl.clear();
+        for (int j = tb; j < tb + nt_p; ++j)
+            for (int k = 0; k < 4; ++k) {
+                int v = L.iperm[elems[pe[j]].p[k]];
+                if (v < vbeg || v >= vbeg + nown) hl.push_back(v);
+            }
+        std::sort(hl.begin(), hl.end());
+        hl.erase(std::unique(hl.begin(), hl.end()), hl.end());
+        const int table = nown + (int)hl.size();
+        if (table > 65535) { *code = 5; return "patch vertex table exceeds 65535"; }
+        if ((size_t)nt_p * 4 > 65535) { *code = 5; return "patch has more than 16383 tets"; }
+        max_table = std::max(max_table, table);
+        max_ptets = std::max(max_ptets, nt_p);
+        L.patches[p] = PatchDesc{vbeg, nown, (int32_t)L.halo.size(), (int32_t)hl.size(), tb, nt_p, 0, 0};
+        L.halo.insert(L.halo.end(), hl.begin(), hl.end());
+        // slot counts per owned vertex
+        std::vector<int32_t> cnt(nown, 0);
+        for (int j = 0; j < nt_p; ++j) {
+            const Elem &e = elems[pe[tb + j]];
+            const size_t g = (size_t)tb + j;
+            for (int k = 0; k < 4; ++k) {
+                int v = L.iperm[e.p[k]];
+                int li;
+                if (v >= vbeg && v < vbeg + nown) {
+                    li = v - vbeg;
+                    cnt[li]++;
+                } else {
+                    li = nown + (int)(std::lower_bound(hl.begin(), hl.end(), v) - hl.begin());
+                }
+                L.lidx[4 * g + k] = (uint16_t)li;
+            }
+            bool primary = (L.iperm[e.p[0]] >= vbeg && L.iperm[e.p[0]] < vbeg + nown);
+            L.meta[g] = ecls[pe[tb + j]] | (e.mask << 16) | ((primary ? 1u : 0u) << 20);
+            const float *B = in.rest_matrix + 9 * (size_t)e.tetid;
+            for (int q = 0; q < 9; ++q) L.Bp[(size_t)q * nptet + g] = B[q];
+            L.rvol[g] = e.rvol;
+            L.rsnh[g] = e.rsnh;
+            L.fib[4 * g + 0] = e.f0[0];
+            L.fib[4 * g + 1] = e.f0[1];
+            L.fib[4 * g + 2] = e.f0[2];
+            L.fib[4 * g + 3] = e.belly;
+            L.ptet_tetid[g] = e.tetid;
+        }
+        for (int i = 0; i < nown; ++i) L.vslot_off[vbeg + i + 1] = cnt[i];
+    }
+    for (int v = 0; v < nv; ++v) L.vslot_off[v + 1] += L.vslot_off[v];
+    L.vslot.assign((size_t)L.vslot_off[nv], 0);
+    {
+        std::vector<int32_t> fill(L.vslot_off.begin(), L.vslot_off.end() - 1);
+        for (int p = 0; p < npatch; ++p) {
+            const PatchDesc &d = L.patches[p];
+            for (int j = 0; j < d.ntet; ++j) {
+                const size_t g = (size_t)d.tbeg + j;
+                for (int k = 0; k < 4; ++k) {
+                    int li = L.lidx[4 * g + k];
+                    if (li < d.nown) L.vslot[fill[d.vbeg + li]++] = (uint16_t)(j * 4 + k);
+                }
+            }
+        }
+    }
+    L.max_table = max_table;
+    L.max_ptets = max_ptets;
+
+    // attach / pin records to new order
+    {
+        std::vector<float> a0(L.att0.size()), a1(L.att1.size()), pc(L.pin_from_cons.size());
+        for (int v = 0; v < nv; ++v) {
+            int n = L.iperm[v];
+            std::memcpy(&a0[4 * (size_t)n], &L.att0[4 * (size_t)v], 16);
+            std::memcpy(&a1[4 * (size_t)n], &L.att1[4 * (size_t)v], 16);
+            std::memcpy(&pc[4 * (size_t)n], &L.pin_from_cons[4 * (size_t)v], 16);
+        }
+        L.att0.swap(a0);
+        L.att1.swap(a1);
+        L.pin_from_cons.swap(pc);
+        L.attach_vertex.resize(attach_old.size());
+        for (size_t i = 0; i < attach_old.size(); ++i) L.attach_vertex[i] = L.iperm[attach_old[i]];
+    }
+
+    // ---- 7. byte accounting ---------------------------------------------------
+    // algorithmic: each tet element's static data read once, each vertex read+written once
+    double btet = 16.0;
+    if (feature & (F_ARAP | F_SNH | F_ANISO)) btet += 36.0;
+    if (feature & (F_VOL | F_SNH)) btet += 4.0;
+    if (feature & F_ANISO) btet += 20.0;
+    L.bytes_alg = btet * (double)elems.size() + 28.0 * nv;
+    double bpt = 8.0 + 4.0;  // lidx + meta
+    if (feature & (F_ARAP | F_SNH | F_ANISO)) bpt += 36.0;
+    if (feature & F_VOL) bpt += 4.0;
+    if (feature & F_SNH) bpt += 4.0;
+    if (feature & F_ANISO) bpt += 16.0;
+    double tab = (double)nv + (double)L.halo.size();
+    double per_table = (feature & F_DAMP) ? 32.0 : 16.0;
+    L.bytes_impl = bpt * (double)nptet + per_table * tab + 4.0 * L.halo.size() +
+                   2.0 * L.vslot.size() + 4.0 * nv /*vslot_off*/ + 16.0 * nv /*write*/ +
+                   32.0 * npatch;
+    if (feature & F_PINS) L.bytes_impl += 16.0 * nv;
+    if (feature & F_ATTACH) L.bytes_impl += 32.0 * nv;
+    return std::string();
+}
+
+}  // namespace xpbd

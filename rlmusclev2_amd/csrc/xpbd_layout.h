@@ -1,0 +1,83 @@
+// Host-side layout builder: constraint table -> fused tet elements -> patches.
+//
+// A patch owns a contiguous run of (renumbered) vertices and holds a copy of
+// every tet element that touches one of them.  One workgroup processes one
+// patch per Jacobi iteration: it stages the patch's vertex table (owned + halo)
+// in LDS, evaluates its tets, and sums the corner corrections of each owned
+// vertex in a fixed slot order — a deterministic segmented reduction that
+// replaces the reference's float atomics (update_dP, solve_kernels.py:52-56).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace xpbd {
+
+enum : uint32_t {
+    M_ARAP = 1u,   // TETARAPSVD
+    M_VOL = 2u,    // TETVOLUME
+    M_SNH = 4u,    // TETSNH
+    M_ANISO = 8u,  // TETARAPANISO
+};
+enum : int32_t {
+    F_ARAP = 1, F_VOL = 2, F_SNH = 4, F_ANISO = 8, F_DAMP = 16, F_ATTACH = 32, F_PINS = 64,
+};
+
+struct PatchDesc {
+    int32_t vbeg, nown, hbeg, nhalo, tbeg, ntet, pad0, pad1;
+};
+
+struct ClassParams {
+    float k_arap, k_vol, kcomp_vol, damp_vol, mu, lam, damp_snh, k_aniso;
+};
+
+struct HostInput {
+    int nv = 0, nt = 0;
+    const float *pos0 = nullptr;       // nv*3
+    const int32_t *tets = nullptr;     // nt*4
+    const float *rest_matrix = nullptr;  // nt*9
+    const float *v_fiber_dir = nullptr;  // nv*3 (may be null)
+    const float *tendonmask = nullptr;   // nt   (may be null)
+    // constraint table (SoA, reference engine.py:1179-1211)
+    int ncons = 0;
+    const int32_t *type = nullptr, *pts = nullptr, *tetid = nullptr;
+    const float *stiffness = nullptr, *compressionstiffness = nullptr, *dampingratio = nullptr,
+                *restlength = nullptr, *restvector = nullptr, *restdir = nullptr;
+    int patch_verts = 256;
+};
+
+struct Layout {
+    // vertex renumbering
+    std::vector<int32_t> perm;   // new -> old
+    std::vector<int32_t> iperm;  // old -> new
+    std::vector<PatchDesc> patches;
+    std::vector<int32_t> halo;               // new vertex ids
+    // replicated tets ("ptets"), patch-major
+    std::vector<uint16_t> lidx;              // 4 per ptet
+    std::vector<uint32_t> meta;              // class | mask<<16 | primary<<20
+    std::vector<float> Bp;                   // 9 planes of nptet
+    std::vector<float> rvol, rsnh;           // per ptet
+    std::vector<float> fib;                  // 4 per ptet: f0.xyz, belly
+    std::vector<int32_t> ptet_tetid;         // per ptet (activation index)
+    std::vector<int32_t> vslot_off;          // nv+1 (new order)
+    std::vector<uint16_t> vslot;             // slot = local tet*4 + corner
+    std::vector<ClassParams> classes;
+    // attach records in new vertex order
+    std::vector<float> att0, att1;           // 4 per vertex
+    std::vector<int32_t> attach_vertex;      // attach constraint #i (table order) -> new vertex
+    std::vector<float> pin_from_cons;        // 4 per vertex (PINTOTARGET constraints)
+    int32_t feature = 0;
+    int32_t max_table = 0, max_ptets = 0, n_elems = 0;
+    double bytes_alg = 0.0, bytes_impl = 0.0;
+};
+
+// returns empty string on success, else the error message; *code gets the status
+std::string build_layout(const HostInput &in, Layout &out, int *code);
+
+// FP32 rest precompute in the reference's operation order (engine.py:199-224):
+// Dm columns p_k - p3, |det|/6, mass += V*density/4 (tet order), B = inv(Dm).
+void precompute_rest(int nv, const float *pos0, int nt, const int32_t *tets, float density,
+                     float *rest_volume, float *rest_matrix, float *mass);
+
+}  // namespace xpbd

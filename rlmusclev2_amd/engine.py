@@ -1,0 +1,250 @@
+"""XpbdEngine on the HIP substep library.
+
+Same primitive names and semantics as the reference ``XpbdEngine``
+(src/muscle/engine.py:785-1134): ``integrate``, ``clear``,
+``solve_constraints``, ``apply_dP``, ``apply_pintotarget``, ``solve_extras``,
+``post_smooth``, ``update_velocities``, ``run_program``.  Each maps to one C-ABI
+call (include/xpbd.h).  The hot path is ``substep`` / ``step_frame``: the whole
+substep (integrate + pin, ``iters`` fused Jacobi iterations, bone push-out,
+velocity update) runs as native launches, and a frame replays one captured
+hipGraph.
+
+Only Jacobi mode exists on the GPU (the reference also forces Jacobi there,
+musclesim.py:72-73); ``use_jacobi=False`` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from .capi import XpbdError, f32, i32, ptr
+from .scene import Scene
+
+
+class XpbdEngine:
+    """Device state + phase primitives for one mesh (or one packed batch)."""
+
+    def __init__(self, scene: Scene, device: int = 0, patch_verts: int = 256):
+        self.scene = scene
+        self.ctx = capi.Context(device)
+        self.n_verts = scene.nv
+        self.NV = scene.nv
+        self.NT = scene.nt
+        self.dt = float(scene.params.dt)
+        self.use_jacobi = True
+        self.collision_solvers = []        # extra Python post-phase hooks (break graph capture)
+        self.collision_solvers_iter = []   # per-iteration Python hooks
+        self.constraint_solvers = []       # MuscleSim extras
+        self._keep = {}
+        self._upload(scene, patch_verts)
+
+    # -- setup -----------------------------------------------------------
+    def _upload(self, s: Scene, patch_verts: int):
+        c = self.ctx
+        k = self._keep
+        k["pos0"] = f32(s.pos0, (-1, 3))
+        k["tets"] = i32(s.tets, (-1, 4))
+        c.call("xpbd_set_mesh", s.nv, ptr(k["pos0"]), s.nt, ptr(k["tets"]), C.c_float(s.density))
+        if s.mass is not None:
+            k["mass"] = f32(s.mass)
+            c.call("xpbd_set_mass", ptr(k["mass"]))
+        if s.stopped is not None:
+            k["stopped"] = i32(s.stopped)
+            c.call("xpbd_set_stopped", ptr(k["stopped"]))
+        if s.v_fiber_dir is not None or s.tendonmask is not None:
+            k["vf"] = None if s.v_fiber_dir is None else f32(s.v_fiber_dir, (-1, 3))
+            k["tm"] = None if s.tendonmask is None else f32(s.tendonmask)
+            c.call("xpbd_set_fiber", ptr(k["vf"]), ptr(k["tm"]))
+        t = s.cons.contiguous()
+        c.call("xpbd_upload_constraints", t.n, ptr(t.type), ptr(t.pts), ptr(t.stiffness),
+               ptr(t.compressionstiffness), ptr(t.dampingratio), ptr(t.tetid),
+               ptr(t.restlength), ptr(t.restvector), ptr(t.restdir))
+        self.set_params(s.params)
+        if s.pin_vids is not None:
+            k["pin_vids"] = i32(s.pin_vids)
+            k["pin_active"] = i32(s.pin_active if s.pin_active is not None
+                                  else np.ones(len(s.pin_vids), np.int32))
+            c.call("xpbd_set_pins", len(k["pin_vids"]), ptr(k["pin_vids"]), ptr(k["pin_active"]))
+        if s.sdf is not None:
+            sd = s.sdf
+            k["grids"] = f32(sd.grids)
+            k["origins"] = f32(sd.origins, (-1, 3))
+            k["spacing"] = f32(sd.spacing)
+            k["coll_w"] = None if sd.coll_w is None else f32(sd.coll_w)
+            c.call("xpbd_set_sdf", sd.nbones, sd.res, ptr(k["grids"]), ptr(k["origins"]),
+                   ptr(k["spacing"]), C.c_float(sd.contact_offset), C.c_float(sd.max_push),
+                   ptr(k["coll_w"]))
+            if sd.vertex_group is not None:
+                k["goff"] = i32(sd.group_bone_off)
+                k["vgroup"] = i32(sd.vertex_group)
+                c.call("xpbd_set_sdf_groups", len(k["goff"]) - 1, ptr(k["goff"]), ptr(k["vgroup"]))
+        tun = capi.XpbdTuning()
+        tun.patch_verts = int(patch_verts)
+        c.call("xpbd_finalize", C.byref(tun))
+        if s.pin_vids is not None:
+            tgt = f32(s.pos0, (-1, 3))[k["pin_vids"]]
+            self.set_pin_targets(tgt)
+
+    def set_params(self, p):
+        q = capi.XpbdParams()
+        q.dt = float(p.dt)
+        for i in range(3):
+            q.gravity[i] = float(p.gravity[i])
+        q.veldamping = float(p.veldamping)
+        q.quasi_static = int(bool(p.quasi_static))
+        q.contraction_ratio = float(p.contraction_ratio)
+        q.fiber_stiffness_scale = float(p.fiber_stiffness_scale)
+        q.has_compress = int(bool(p.has_compress))
+        q.exp_damp = float(p.exp_damp)
+        q.const_damp = float(p.const_damp)
+        self.ctx.call("xpbd_set_params", C.byref(q))
+        self.dt = float(p.dt)
+
+    def stats(self) -> dict:
+        st = capi.XpbdStats()
+        self.ctx.call("xpbd_get_stats", C.byref(st))
+        return {f: getattr(st, f) for f, _ in capi.XpbdStats._fields_}
+
+    # -- per-frame inputs ---------------------------------------------------
+    def set_pin_targets(self, xyz):
+        a = f32(xyz, (-1, 3))
+        self._keep["pin_tgt"] = a
+        self.ctx.call("xpbd_set_pin_targets", a.shape[0], ptr(a))
+
+    def set_fiber_activation(self, act):
+        """MultiGridSim.set_fiber_activation (mg.py:1098-1111): per-tet activation."""
+        a = f32(act)
+        if a.shape[0] != self.NT:
+            raise ValueError(f"activation has {a.shape[0]} entries, expected NT={self.NT}")
+        self._keep["act"] = a
+        self.ctx.call("xpbd_set_activation", ptr(a))
+
+    def set_bone_transforms(self, xforms):
+        x = f32(xforms, (-1, 12))
+        self._keep["xf"] = x
+        self.ctx.call("xpbd_set_bone_transforms", x.shape[0], ptr(x))
+
+    def set_attach_targets(self, xyz):
+        a = f32(xyz, (-1, 3))
+        self._keep["att"] = a
+        self.ctx.call("xpbd_set_attach_targets", a.shape[0], ptr(a))
+
+    # -- XpbdEngine primitives (engine.py:798-1134) -------------------------
+    def integrate(self):
+        self.ctx.call("xpbd_integrate")
+
+    def clear(self):
+        self.ctx.call("xpbd_clear")
+
+    def solve_constraints(self, run_extras: bool = True):
+        if not self.use_jacobi:
+            raise XpbdError(capi.XPBD_EUNSUPPORTED, "Gauss-Seidel mode is not on the GPU path")
+        self.ctx.call("xpbd_solve_constraints")
+        if run_extras:
+            for fn in self.constraint_solvers:
+                fn()
+
+    def apply_dP(self):
+        self.ctx.call("xpbd_apply_dP")
+
+    def apply_pintotarget(self):
+        self.ctx.call("xpbd_apply_pins")
+
+    def solve_extras(self):
+        for fn in self.constraint_solvers:
+            fn()
+
+    def post_smooth(self, n_iters: int = 0):
+        """post_smooth (engine.py:1072-1100): n Jacobi passes of clear/solve/apply/pin."""
+        if n_iters > 0:
+            self.ctx.call("xpbd_iterate", int(n_iters))
+
+    def iterate(self, n: int):
+        """n fused Jacobi iterations: clear, dispatch, apply_dP, pin (mg.py:1341-1360)."""
+        self.ctx.call("xpbd_iterate", int(n))
+
+    def collide(self):
+        self.ctx.call("xpbd_collide")
+
+    def update_velocities(self):
+        self.ctx.call("xpbd_update_velocities")
+
+    def native_substep(self, iters: int):
+        self.ctx.call("xpbd_substep", int(iters))
+
+    def native_frame(self, nsub: int, iters: int, use_graph: bool = True):
+        self.ctx.call("xpbd_step_frame", int(nsub), int(iters), int(bool(use_graph)))
+
+    def synchronize(self):
+        self.ctx.call("xpbd_synchronize")
+
+    # -- state ------------------------------------------------------------
+    def get_state(self):
+        n = self.NV
+        pos = np.empty((n, 3), np.float32)
+        pp = np.empty((n, 3), np.float32)
+        vel = np.empty((n, 3), np.float32)
+        self.ctx.call("xpbd_get_state", ptr(pos), ptr(pp), ptr(vel))
+        return pos, pp, vel
+
+    def positions_np(self) -> np.ndarray:
+        n = self.NV
+        pos = np.empty((n, 3), np.float32)
+        self.ctx.call("xpbd_get_state", ptr(pos), None, None)
+        return pos
+
+    def velocities_np(self) -> np.ndarray:
+        n = self.NV
+        vel = np.empty((n, 3), np.float32)
+        self.ctx.call("xpbd_get_state", None, None, ptr(vel))
+        return vel
+
+    def set_state(self, pos=None, pprev=None, vel=None):
+        a = None if pos is None else f32(pos, (-1, 3))
+        b = None if pprev is None else f32(pprev, (-1, 3))
+        v = None if vel is None else f32(vel, (-1, 3))
+        self.ctx.call("xpbd_set_state", ptr(a), ptr(b), ptr(v))
+
+    def get_rest(self):
+        rv = np.empty(self.NT, np.float32)
+        rm = np.empty((self.NT, 3, 3), np.float32)
+        m = np.empty(self.NV, np.float32)
+        self.ctx.call("xpbd_get_rest", ptr(rv), ptr(rm), ptr(m))
+        return rv, rm, m
+
+    # -- diagnostics --------------------------------------------------------
+    def residuals(self) -> dict:
+        """Per-type residual RMS (arap_residual_accum_kernel, mg/kernel.py:87-118,
+        generalised per constraint type)."""
+        out = np.zeros(8, np.float64)
+        self.ctx.call("xpbd_residuals", ptr(out))
+        r = {}
+        for i, name in enumerate(("arap", "volume", "snh", "aniso")):
+            cnt = out[2 * i + 1]
+            if cnt > 0:
+                r[name] = float(np.sqrt(out[2 * i] / cnt))
+        return r
+
+    def count_inverted(self) -> int:
+        out = np.zeros(1, np.int32)
+        self.ctx.call("xpbd_count_inverted", ptr(out))
+        return int(out[0])
+
+    def sdf_cells(self) -> np.ndarray:
+        out = np.empty((self.NV, 2), np.int32)
+        self.ctx.call("xpbd_get_sdf_cells", ptr(out))
+        return out
+
+    def kernel_timing(self, enable: bool):
+        self.ctx.call("xpbd_kernel_timing", int(bool(enable)))
+
+    def get_kernel_timing(self):
+        ms = np.zeros(1, np.float64)
+        n = np.zeros(1, np.int32)
+        self.ctx.call("xpbd_get_kernel_timing", ptr(ms), ptr(n))
+        return float(ms[0]), int(n[0])
+
+    def close(self):
+        self.ctx.close()

@@ -1,0 +1,73 @@
+"""Problem description shared by the HIP engine and its checker.
+
+A ``Scene`` carries exactly the fields the reference sim objects hand to the
+substep kernels: rest mesh, mass/density, the constraint table, fibre data,
+engine scalars, kinematic pins and the bone SDF colliders
+(src/muscle/musclesim.py:331-352, src/muscle/mg/mg.py:95-576).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .builder import ConsTable
+
+
+@dataclass
+class SimParams:
+    dt: float                                   # substep dt
+    gravity: tuple = (0.0, 0.0, 0.0)
+    veldamping: float = 0.0
+    quasi_static: bool = False
+    contraction_ratio: float = 0.4
+    fiber_stiffness_scale: float = 1.0
+    has_compress: bool = False
+    exp_damp: float = 1.0
+    const_damp: float = 0.0
+
+
+@dataclass
+class SdfColliders:
+    """Bone SDF grids (bone-local), rigid transforms and collision weights."""
+    grids: np.ndarray          # (nbones, res, res, res) f32, index [b, z, y, x]
+    origins: np.ndarray        # (nbones, 3) bone-local grid origin
+    spacing: np.ndarray        # (nbones,) grid spacing
+    contact_offset: float = 1e-5
+    max_push: float = -1.0
+    coll_w: np.ndarray | None = None          # (nv,) 0 = never pushed
+    vertex_group: np.ndarray | None = None    # (nv,) bone group per vertex
+    group_bone_off: np.ndarray | None = None  # (ngroups+1,)
+
+    @property
+    def nbones(self) -> int:
+        return int(self.grids.shape[0])
+
+    @property
+    def res(self) -> int:
+        return int(self.grids.shape[1])
+
+
+@dataclass
+class Scene:
+    pos0: np.ndarray
+    tets: np.ndarray
+    density: float
+    cons: ConsTable
+    params: SimParams
+    v_fiber_dir: np.ndarray | None = None
+    tendonmask: np.ndarray | None = None       # per tet
+    stopped: np.ndarray | None = None
+    mass: np.ndarray | None = None             # overrides density-lumped mass
+    pin_vids: np.ndarray | None = None
+    pin_active: np.ndarray | None = None
+    sdf: SdfColliders | None = None
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def nv(self) -> int:
+        return int(self.pos0.shape[0])
+
+    @property
+    def nt(self) -> int:
+        return int(self.tets.shape[0])

@@ -1,0 +1,245 @@
+"""Sim programs on top of the HIP XpbdEngine: MultiGridSim and MuscleSim.
+
+``MultiGridSim`` follows the JSON-driven pin-to-target program of the
+reference (src/muscle/mg/mg.py): config semantics (:95-400), constraint set
+(:379-399), density from total mass (:736-740), pins (:486-535), substep order
+(``build_program`` :1322-1378) and ``step_frame`` (:1409-1418).  Meshes come
+from a ``synthetic`` config block (the reference's .geo/USD assets are not
+available) or from arrays passed in.
+
+``MuscleSim.from_procedural`` follows src/muscle/musclesim.py:181-287: one
+constraint iteration per substep, pins as stopped vertices.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import builder, workloads
+from .builder import ConsTable
+from .config import find_constraint
+from .engine import XpbdEngine
+from .scene import Scene, SdfColliders, SimParams
+
+
+def mg_constraint_configs(cfg: dict) -> list:
+    """The fine-level constraint set of MultiGridSim (mg.py:103-131, 379-399)."""
+    arap = find_constraint(cfg, "arap")
+    if arap is None:
+        raise ValueError("constraints[] must define an 'arap' entry (the muscle ARAP-SVD shape match)")
+    mu = float(arap["stiffness"])
+    arap_damp = float(arap.get("dampingratio", 0.0))
+    snh = find_constraint(cfg, "snh")
+    vol = find_constraint(cfg, "volume")
+    fib = find_constraint(cfg, "fiber")
+    out = []
+    if snh is not None:
+        out.append(dict(type="snh", name="snh", mu=float(snh.get("mu", snh.get("stiffness", mu))),
+                        lam=float(snh.get("lam", 1.0e8)),
+                        dampingratio=float(snh.get("dampingratio", arap_damp))))
+    else:
+        out.append(dict(type="tetarapsvd", name="arap", stiffness=mu, dampingratio=arap_damp))
+    if vol is not None and snh is None:
+        out.append(dict(type="volume", name="volume", stiffness=float(vol["stiffness"]),
+                        dampingratio=float(vol.get("dampingratio", 0.0))))
+    if fib is not None:
+        model = str(fib.get("fiber_model", "aniso_arap"))
+        if model != "aniso_arap":
+            raise NotImplementedError(f"fiber_model={model!r}: only aniso_arap is on the HIP path")
+        out.append(dict(type="aniso_arap", name="fiber", stiffness=float(fib.get("stiffness", 1.0e8)),
+                        dampingratio=float(fib.get("dampingratio", 0.0))))
+    return out
+
+
+def build_workload(cfg: dict) -> workloads.Workload:
+    syn = cfg.get("synthetic")
+    if not syn:
+        raise ValueError("config has no 'synthetic' block and no mesh arrays were given")
+    name = syn["workload"]
+    seed = int(syn.get("seed", 0))
+    if name == "cantilever":
+        return workloads.cantilever(seed)
+    if name == "humanv2_scale":
+        return workloads.humanv2_scale(seed, target_tets=int(syn.get("muscle_tets", 468_000)),
+                                       fascia_tets=int(syn.get("fascia_tets", 67_000)),
+                                       sdf_res=int(syn.get("sdf_res", 128)))
+    if name == "batch":
+        return workloads.batch(seed, n_inst=int(syn.get("instances", 512)), lo=int(syn.get("min_tets", 2000)),
+                               hi=int(syn.get("max_tets", 40000)), sdf_res=int(syn.get("sdf_res", 64)))
+    raise ValueError(f"unknown synthetic workload {name!r}")
+
+
+def scene_from_config(cfg: dict, w: workloads.Workload) -> Scene:
+    """Scene for the MultiGridSim program on workload ``w``."""
+    pos0 = w.pos0.astype(np.float32)
+    tets = w.tets.astype(np.int32)
+    ccfg = mg_constraint_configs(cfg)
+    fib = find_constraint(cfg, "fiber")
+    cons = builder.build_from_configs(pos0, tets, ccfg)
+    volumes, valid = builder.tet_rest_data(pos0, tets)
+    if cfg.get("density", -1.0) and float(cfg.get("density", -1.0)) > 0:
+        density = float(cfg["density"])
+    else:  # _rest_density (mg.py:736-740)
+        density = float(cfg["total_mass"]) / float(max(volumes[valid].sum(), 1e-30))
+    dt = float(cfg["delta_t"]) / int(cfg["nsubsteps"])
+    params = SimParams(dt=dt, gravity=tuple(float(g) for g in cfg.get("gravity", [0.0, 0.0, 0.0])),
+                       veldamping=float(cfg.get("veldamping", 0.0)),
+                       quasi_static=bool(cfg.get("quasi_static", False)),
+                       contraction_ratio=float((fib or {}).get("contraction_ratio", 0.4)),
+                       fiber_stiffness_scale=1.0)
+    tm = None
+    if w.v_tendonmask is not None:
+        tm = builder.tet_tendonmask(tets, w.v_tendonmask)
+    vf = w.v_fiber_dir if fib is not None else None
+    # kinematic pins (muscleToBonePin); synthetic meshes carry the tendon-tip set
+    pin_vids = pin_active = None
+    pin = find_constraint(cfg, "muscleToBonePin")
+    if pin is not None and w.pin_vids is not None and w.stopped is None:
+        pin_vids = w.pin_vids
+        region = str(pin.get("region", "tips")).lower()
+        pin_active = np.ones(len(pin_vids), np.int32) if region != "none" else np.zeros(len(pin_vids), np.int32)
+    sdf = None
+    coll = cfg.get("demo", {}).get("collision", {})
+    if coll.get("enabled", False) and w.bones:
+        grids, origins, spacing = workloads.bake_bone_sdfs(w, int(cfg.get("synthetic", {}).get("sdf_res", 64)))
+        m = np.ones(w.nv, np.float32)
+        if w.is_fascia is not None:
+            m[w.is_fascia] = 0.0   # fascia excluded from bone collision (humanV2.py:361-366)
+        vg = goff = None
+        if w.vertex_group is not None and w.name == "batch":
+            vg = w.vertex_group.astype(np.int32)
+            goff = np.arange(len(w.bones) + 1, dtype=np.int32)
+        sdf = SdfColliders(grids=grids, origins=origins, spacing=spacing,
+                           contact_offset=float(coll.get("contact_offset", 1e-5)),
+                           max_push=float(coll.get("max_push", -1.0)), coll_w=m,
+                           vertex_group=vg, group_bone_off=goff)
+    return Scene(pos0=pos0, tets=tets, density=density, cons=cons, params=params, v_fiber_dir=vf,
+                 tendonmask=tm, stopped=w.stopped, pin_vids=pin_vids, pin_active=pin_active, sdf=sdf,
+                 meta=dict(workload=w.name, constraint_configs=ccfg))
+
+
+class MultiGridSim:
+    """Pin-to-target muscle sim on the HIP substep (mg.py:85-1434, single level)."""
+
+    def __init__(self, cfg: dict, workload: workloads.Workload | None = None, device: int = 0,
+                 patch_verts: int = 256, engine_cls=XpbdEngine):
+        if cfg.get("use_multigrid", False):
+            raise NotImplementedError("use_multigrid: the two-level V-cycle is not on the HIP path yet")
+        if not cfg.get("use_jacobi", True):
+            raise NotImplementedError("use_jacobi=false: Gauss-Seidel is not on the GPU path")
+        self.cfg = cfg
+        self.workload = workload if workload is not None else build_workload(cfg)
+        self.scene = scene_from_config(cfg, self.workload)
+        self.nsubsteps = int(cfg["nsubsteps"])
+        self.fine_iters = int(cfg["fine_iterations"])
+        self.dt = self.scene.params.dt
+        self.start_frame = int(cfg.get("start_frame", 1))
+        self.engine = engine_cls(self.scene, device=device, patch_verts=patch_verts) \
+            if engine_cls is XpbdEngine else engine_cls(self.scene)
+        self.NV = self.scene.nv
+        self.NT = self.scene.nt
+        self.use_graph = True
+        self.collision_solvers = []
+        self.collision_solvers_iter = []
+        self._frame = None
+
+    # kinematic drive ---------------------------------------------------------
+    def load_frame_targets(self, frame: int) -> None:
+        """Per-frame bone pose -> pin targets and SDF transforms (mg.py:1233-1249)."""
+        w = self.workload
+        if self.scene.pin_vids is not None and w.bones:
+            self.engine.set_pin_targets(workloads.pin_targets(w, frame))
+        if self.scene.sdf is not None:
+            self.engine.set_bone_transforms(workloads.bone_xforms(w, frame))
+        self._frame = frame
+
+    def set_fiber_activation(self, acti) -> None:
+        if find_constraint(self.cfg, "fiber") is None:
+            raise RuntimeError("set_fiber_activation requires a 'fiber' constraint")
+        self.engine.set_fiber_activation(acti)
+
+    def apply_pintotarget(self) -> None:
+        self.engine.apply_pintotarget()
+
+    # program -------------------------------------------------------------------
+    def substep(self) -> None:
+        """integrate -> pin -> fine_iters x (clear, dispatch, apply_dP, pin) ->
+        collision_solvers -> update_velocities (mg.py:1322-1384)."""
+        if self.collision_solvers or self.collision_solvers_iter:
+            e = self.engine
+            e.integrate()
+            for it in range(self.fine_iters):
+                e.iterate(1)
+                for fn in self.collision_solvers_iter:
+                    fn(it)
+            e.collide()
+            for fn in self.collision_solvers:
+                fn()
+            e.update_velocities()
+        else:
+            self.engine.native_substep(self.fine_iters)
+
+    def step_frame(self, frame: int) -> None:
+        """Kinematic step once, then nsubsteps substeps (mg.py:1409-1418)."""
+        self.load_frame_targets(frame)
+        self.engine.apply_pintotarget()
+        if self.collision_solvers or self.collision_solvers_iter:
+            for _ in range(self.nsubsteps):
+                self.substep()
+        else:
+            self.engine.native_frame(self.nsubsteps, self.fine_iters, self.use_graph)
+
+    def positions_np(self) -> np.ndarray:
+        return self.engine.positions_np()
+
+    def count_inverted(self) -> int:
+        return self.engine.count_inverted()
+
+    def residuals(self) -> dict:
+        return self.engine.residuals()
+
+
+class MuscleSim:
+    """MuscleSim program on the HIP substep: 1 Jacobi iteration per substep."""
+
+    def __init__(self, scene: Scene, num_substeps: int, device: int = 0):
+        self.scene = scene
+        self.num_substeps = int(num_substeps)
+        self.engine = XpbdEngine(scene, device=device)
+        self.n_verts = scene.nv
+        self.dt = scene.params.dt
+        self.step_cnt = 0
+        self.post_smooth_iters = 0
+
+    @classmethod
+    def from_procedural(cls, vertices, tets, fiber_dirs_per_tet, bone_targets=None, *,
+                        constraint_configs, dts, num_substeps=1, device=0, density=1060.0,
+                        veldamping=0.02, gravity=0.0, contraction_ratio=0.0,
+                        fiber_stiffness_scale=1.0, stopped=None, **_unused):
+        """musclesim.py:181-287 (gravity is the y component, as the reference engine)."""
+        v = np.asarray(vertices, np.float32)
+        t = np.asarray(tets, np.int32)
+        vf = builder.vertex_fiber_from_tets(v.shape[0], t, fiber_dirs_per_tet)
+        cons = builder.build_from_configs(v, t, [c for c in constraint_configs
+                                                 if str(c["type"]).lower() != "attach"])
+        params = SimParams(dt=float(dts) / num_substeps, gravity=(0.0, float(gravity), 0.0),
+                           veldamping=float(veldamping), contraction_ratio=float(contraction_ratio),
+                           fiber_stiffness_scale=float(fiber_stiffness_scale))
+        scene = Scene(pos0=v, tets=t, density=float(density), cons=cons, params=params,
+                      v_fiber_dir=vf, tendonmask=np.zeros(t.shape[0], np.float32),
+                      stopped=None if stopped is None else np.asarray(stopped, np.int32))
+        return cls(scene, num_substeps, device=device)
+
+    def step(self):
+        """integrate_phase, solve_phase, finalize_phase per substep (musclesim.py:107-142)."""
+        for _ in range(self.num_substeps):
+            self.engine.native_substep(1)
+            if self.post_smooth_iters:
+                self.engine.post_smooth(self.post_smooth_iters)
+        self.step_cnt += 1
+
+    def reset(self):
+        p0 = self.scene.pos0
+        self.engine.set_state(p0, p0, np.zeros_like(p0))
+
+    def positions_np(self):
+        return self.engine.positions_np()
