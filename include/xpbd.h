@@ -149,6 +149,30 @@ int32_t xpbd_substep(xpbd_ctx *ctx, int32_t iters);
 int32_t xpbd_step_frame(xpbd_ctx *ctx, int32_t nsub, int32_t iters, int32_t use_graph);
 int32_t xpbd_synchronize(xpbd_ctx *ctx);
 
+/* ---- tri-tri DCD (UdcfDcdCollider, src/muscle/collision/udcf/solver.py:146-457) --- */
+typedef struct {
+    float thickness;          /* broadphase sphere slack (UdcfParams.thickness)        */
+    float multiplier;         /* collisionConstraintMultiplier                          */
+    float friction_static, friction_dynamic;
+    float offset;             /* narrow-phase accept threshold (1e-6)                   */
+    float normal_cull;        /* >= 1 disables the co-oriented self-pair cull           */
+    float cell_size;          /* 0 = 2 x mean rest edge (never below 2 r_max + thickness) */
+    int32_t allow_self;       /* movable-vs-movable pairs                               */
+    int32_t iters;            /* DCD passes per substep (humanV2 "iters", default 2)    */
+    int32_t rebuild_substeps; /* 1 = broadphase every substep, 0 = once per frame       */
+    int64_t max_pairs;        /* pair / contact capacity (default 1 << 22)              */
+} xpbd_dcd_params;
+/* surface triangles (ntris*3 vertex ids) and optional group (0 movable, 1 static);
+ * enables the DCD post pass before the bone push (humanV2.py:820-866) */
+int32_t xpbd_set_dcd(xpbd_ctx *ctx, int32_t ntris, const int32_t *tris, const int32_t *group,
+                     const xpbd_dcd_params *p);
+/* one DCD pass on the current positions (UdcfDcdCollider.solve, solver.py:365-440) */
+int32_t xpbd_dcd_solve(xpbd_ctx *ctx, int32_t rebuild);
+/* the (i<j)-sorted candidate pair list of the last broadphase (int32 pairs, 2 per entry) */
+int32_t xpbd_dcd_pairs(xpbd_ctx *ctx, int32_t *out, int64_t cap, int64_t *n_total);
+/* UdcfDcdCollider.stats (solver.py:442-457): raw pair count, contacts of the last pass, cell */
+int32_t xpbd_dcd_stats(xpbd_ctx *ctx, int64_t *n_pairs, int64_t *n_contacts, float *cell);
+
 /* ---- diagnostics ------------------------------------------------------ */
 /* per-type residual sums: out[0] sum ||sigma-1||^2 over ARAP tets, out[1] count,
  * out[2] sum (V-V0)^2 over volume tets, out[3] count, out[4] sum C_D^2 + C_H^2 (SNH),

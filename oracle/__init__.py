@@ -335,3 +335,90 @@ def residuals_numpy(cons, pos, rest_matrix, v_fiber, tendonmask, activation, par
             C_ = 0.5 * (np.linalg.norm(d, axis=1) - fs) ** 2
             out["aniso"] = float(np.sqrt(np.mean(C_ * C_)))
     return out
+
+
+# ---------------------------------------------------------------------------
+# tri-tri DCD (src/muscle/collision/udcf/kernel.py)
+# ---------------------------------------------------------------------------
+class OrcContact(C.Structure):
+    _fields_ = [("cur", C.c_int * 3), ("closest", C.c_int * 3), ("C", C.c_float), ("C1", C.c_float),
+                ("C2", C.c_float), ("n", C.c_float * 3), ("b1", C.c_float * 3), ("b2", C.c_float * 3),
+                ("valid", C.c_int)]
+
+
+def _dcd_lib():
+    lib = load()
+    if not getattr(lib, "_dcd_bound", False):
+        lib.orc_dcd_tri_data.argtypes = [C.c_int, _P, _P, _P, _P, _P]
+        lib.orc_dcd_pairs_bruteforce.argtypes = [C.c_int, _P, _P, _P, _P, _P, C.c_float, C.c_int, C.c_float,
+                                                 C.c_long, _P, _P]
+        lib.orc_dcd_pairs_bruteforce.restype = C.c_long
+        lib.orc_dcd_narrow.argtypes = [C.c_long, _P, _P, _P, _P, C.c_float, _P]
+        lib.orc_dcd_respond.argtypes = [C.c_long, _P, _P, _P, _P, C.c_float, C.c_float, C.c_float, _P, _P]
+        lib.orc_dcd_apply.argtypes = [C.c_int, _P, _P, _P, C.c_int]
+        lib._dcd_bound = True
+    return lib
+
+
+def dcd_tri_data(tris, x):
+    lib = _dcd_lib()
+    tris = np.ascontiguousarray(tris, np.int32)
+    x = np.ascontiguousarray(x, np.float32)
+    m = tris.shape[0]
+    c = np.zeros((m, 3), np.float32)
+    r = np.zeros(m, np.float32)
+    n = np.zeros((m, 3), np.float32)
+    lib.orc_dcd_tri_data(m, _p(tris), _p(x), _p(c), _p(r), _p(n))
+    return c, r, n
+
+
+def dcd_pairs(tris, x, group=None, allow_self=True, thickness=0.0, normal_cull=1.0, max_pairs=1 << 22):
+    """Sorted (i<j) candidate pairs by exhaustive enumeration of the emit rules."""
+    lib = _dcd_lib()
+    tris = np.ascontiguousarray(tris, np.int32)
+    c, r, n = dcd_tri_data(tris, x)
+    g = None if group is None else np.ascontiguousarray(group, np.int32)
+    pa = np.zeros(max_pairs, np.int32)
+    pb = np.zeros(max_pairs, np.int32)
+    k = lib.orc_dcd_pairs_bruteforce(tris.shape[0], _p(tris), _p(c), _p(r), _p(n), _p(g),
+                                     C.c_float(normal_cull), int(allow_self), C.c_float(thickness),
+                                     max_pairs, _p(pa), _p(pb))
+    k = min(int(k), max_pairs)
+    return np.stack([pa[:k], pb[:k]], axis=1)
+
+
+def dcd_narrow(pairs, tris, x, offset_eps=1e-6):
+    lib = _dcd_lib()
+    pairs = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
+    pa = np.ascontiguousarray(pairs[:, 0])
+    pb = np.ascontiguousarray(pairs[:, 1])
+    out = (OrcContact * max(1, len(pairs)))()
+    tris = np.ascontiguousarray(tris, np.int32)
+    x = np.ascontiguousarray(x, np.float32)
+    lib.orc_dcd_narrow(len(pairs), _p(pa), _p(pb), _p(tris), _p(x), C.c_float(offset_eps), out)
+    return out, len(pairs)
+
+
+def dcd_respond_apply(contacts, ncontacts, x, x_prev, inv_mass, multiplier, mu_s, mu_d, use_avg=True):
+    lib = _dcd_lib()
+    x = np.ascontiguousarray(x, np.float32).copy()
+    xp = np.ascontiguousarray(x_prev, np.float32)
+    im = np.ascontiguousarray(inv_mass, np.float32)
+    dP = np.zeros_like(x)
+    dPw = np.zeros(x.shape[0], np.float32)
+    lib.orc_dcd_respond(ncontacts, contacts, _p(x), _p(xp), _p(im), C.c_float(multiplier), C.c_float(mu_s),
+                        C.c_float(mu_d), _p(dP), _p(dPw))
+    lib.orc_dcd_apply(x.shape[0], _p(x), _p(dP), _p(dPw), int(use_avg))
+    return x
+
+
+def make_contact(cur, closest, Cval, n, b1, b2):
+    c = (OrcContact * 1)()
+    c[0].cur[:] = list(cur)
+    c[0].closest[:] = list(closest)
+    c[0].C = Cval
+    c[0].n[:] = list(n)
+    c[0].b1[:] = list(b1)
+    c[0].b2[:] = list(b2)
+    c[0].valid = 1
+    return c

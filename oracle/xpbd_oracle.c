@@ -813,3 +813,290 @@ int orc_count_inverted(int nt, const int *tets, const float *pos, const float *r
     }
     return cnt;
 }
+
+/* ================================================================== */
+/* tri-tri DCD: src/muscle/collision/udcf/kernel.py                    */
+/* ================================================================== */
+
+/* centroid_radius_kernel :47-63, face_normal_kernel :66-84 */
+void orc_dcd_tri_data(int m, const int *tris, const float *x, float *centroids, float *radii,
+                      float *normals) {
+    for (int i = 0; i < m; ++i) {
+        v3 a = ld3(x, tris[3 * i]), b = ld3(x, tris[3 * i + 1]), c = ld3(x, tris[3 * i + 2]);
+        v3 ce = dvs(add(add(a, b), c), 3.0f);
+        float r0 = length(sub(a, ce)), r1 = length(sub(b, ce)), r2 = length(sub(c, ce));
+        float r = r0 > r1 ? r0 : r1;
+        r = r > r2 ? r : r2;
+        st3(centroids, i, ce);
+        radii[i] = r;
+        if (normals) {
+            v3 n = cross(sub(b, a), sub(c, a));
+            float ln = length(n);
+            st3(normals, i, ln > 0.0f ? dvs(n, ln) : mk(0, 0, 0));
+        }
+    }
+}
+
+/* broadphase_find_pairs_kernel :851-915 emit rules, enumerated exhaustively */
+long orc_dcd_pairs_bruteforce(int m, const int *tris, const float *centroids, const float *radii,
+                              const float *normals, const int *group, float normal_cull,
+                              int allow_self, float thickness, long max_pairs, int *pair_a,
+                              int *pair_b) {
+    long cnt = 0;
+    for (int i = 0; i < m; ++i) {
+        v3 ci = ld3(centroids, i);
+        float ri = radii[i];
+        int gi = group ? group[i] : 0;
+        for (int j = i + 1; j < m; ++j) {
+            int gj = group ? group[j] : 0;
+            if (gi == 1 && gj == 1) continue;
+            if (gi == 0 && gj == 0) {
+                if (!allow_self) continue;
+                const int *a = tris + 3 * i, *b = tris + 3 * j;
+                int share = 0;
+                for (int u = 0; u < 3; ++u)
+                    for (int v = 0; v < 3; ++v) share |= (a[u] == b[v]);
+                if (share) continue;
+                if (normals && dot(ld3(normals, i), ld3(normals, j)) > normal_cull) continue;
+            }
+            if (length(sub(ci, ld3(centroids, j))) > ri + radii[j] + thickness) continue;
+            if (cnt < max_pairs) { pair_a[cnt] = i; pair_b[cnt] = j; }
+            ++cnt;
+        }
+    }
+    return cnt;
+}
+
+/* geometry helpers of narrowPhase (kernel.py:165-309) */
+static v3 pick3(v3 a, v3 b, v3 c, int i) { return i == 0 ? a : (i == 1 ? b : c); }
+
+static int inside_tri(v3 p1, v3 p2, v3 p3, v3 p) {
+    v3 c1 = cross(sub(p2, p1), sub(p, p2));
+    v3 c2 = cross(sub(p3, p2), sub(p, p3));
+    v3 c3 = cross(sub(p1, p3), sub(p, p1));
+    return (dot(c1, c2) > 0.0f && dot(c1, c3) > 0.0f) ? 1 : 0;
+}
+
+static v3 bary(v3 A, v3 B, v3 C, v3 P) {
+    v3 v0 = sub(B, A), v1 = sub(C, A), v2 = sub(P, A);
+    float d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1), d20 = dot(v2, v0), d21 = dot(v2, v1);
+    float denom = d00 * d11 - d01 * d01;
+    if (denom == 0.0f) return mk(1.0f / 3.0f, 1.0f / 3.0f, 1.0f / 3.0f);
+    float v = (d11 * d20 - d01 * d21) / denom;
+    float w = (d00 * d21 - d01 * d20) / denom;
+    return mk(1.0f - v - w, v, w);
+}
+
+static int line_isect(v3 p1, v3 p2, v3 p3, v3 p4, v3 v1, v3 v2, v3 *out) {
+    v3 d1 = sub(p2, p1), d2 = sub(p4, p3);
+    v3 nrm = cross(d1, d2);
+    if (length(nrm) < 1.0e-6f) return 0;
+    float denom = dot(nrm, nrm);
+    if (denom < 1.0e-6f) return 0;
+    v3 v = sub(p3, p1);
+    float t1 = dot(cross(v, d2), nrm) / denom;
+    float t2 = dot(cross(v, d1), nrm) / denom;
+    if (t2 >= 0.0f && t2 <= 1.0f && t1 >= 0.0f && t1 <= 1.0f) {
+        *out = add(v1, scl(t1, sub(v2, v1)));
+        return 1;
+    }
+    return 0;
+}
+
+static int find_isect(v3 N, v3 p0, v3 p, v3 dir, v3 *out) {
+    float denom = dot(N, dir);
+    if (denom == 0.0f) return 0;
+    float t = dot(N, sub(p0, p)) / denom;
+    *out = add(p, scl(t, dir));
+    return 1;
+}
+
+#define UDCF_THRESH 1.0e-5f
+
+static int gen_t(v3 N, v3 v0, v3 v1, v3 v2, float d, v3 D, float *lo, float *hi) {
+    float d0 = dot(N, v0) + d, d1 = dot(N, v1) + d, d2 = dot(N, v2) + d;
+    if (d0 <= UDCF_THRESH && d1 <= UDCF_THRESH && d2 <= UDCF_THRESH) return 0;
+    if (d0 >= -UDCF_THRESH && d1 >= -UDCF_THRESH && d2 >= -UDCF_THRESH) return 0;
+    v3 s0 = v0, s1 = v1, s2 = v2;
+    float e0 = d0, e1 = d1, e2 = d2;
+    if ((d0 <= 0.0f && d1 >= 0.0f && d2 >= 0.0f) || (d0 >= 0.0f && d1 <= 0.0f && d2 <= 0.0f)) {
+        s0 = v1; s1 = v0; e0 = d1; e1 = d0;
+    } else if ((d0 >= 0.0f && d1 >= 0.0f && d2 <= 0.0f) || (d0 <= 0.0f && d1 <= 0.0f && d2 >= 0.0f)) {
+        s1 = v2; s2 = v1; e1 = d2; e2 = d1;
+    }
+    float p0 = dot(D, s0), p1 = dot(D, s1), p2 = dot(D, s2);
+    float tA = p0 + (p1 - p0) * fabsf(e0 / (e0 - e1));
+    float tB = p2 + (p1 - p2) * fabsf(e2 / (e2 - e1));
+    if (tA > tB) { *lo = tB; *hi = tA; } else { *lo = tA; *hi = tB; }
+    return 1;
+}
+
+/* narrow_kernel :317-474 */
+void orc_dcd_narrow(long npairs, const int *pair_a, const int *pair_b, const int *tris,
+                    const float *x, float offset_eps, orc_contact_t *out) {
+    for (long k = 0; k < npairs; ++k) {
+        orc_contact_t *o = out + k;
+        memset(o, 0, sizeof(*o));
+        int ta = pair_a[k], tb = pair_b[k];
+        v3 q00 = ld3(x, tris[3 * ta]), q01 = ld3(x, tris[3 * ta + 1]), q02 = ld3(x, tris[3 * ta + 2]);
+        v3 q10 = ld3(x, tris[3 * tb]), q11 = ld3(x, tris[3 * tb + 1]), q12 = ld3(x, tris[3 * tb + 2]);
+        v3 n1f = cross(sub(q01, q00), sub(q02, q00)), n2f = cross(sub(q11, q10), sub(q12, q10));
+        float l1 = length(n1f), l2 = length(n2f);
+        if (l1 == 0.0f || l2 == 0.0f) continue;
+        v3 N1 = dvs(n1f, l1), N2 = dvs(n2f, l2);
+        float d1 = -dot(N1, q00), d2 = -dot(N2, q10);
+        v3 cp = cross(N1, N2);
+        float lcp = length(cp);
+        if (lcp < 1.0e-8f) continue;
+        v3 D = dvs(cp, lcp);
+        float t1, t2, t3, t4;
+        if (!gen_t(N2, q00, q01, q02, d2, D, &t1, &t2)) continue;
+        if (!gen_t(N1, q10, q11, q12, d1, D, &t3, &t4)) continue;
+        if (!(t2 >= t3 && t4 >= t1)) continue;
+        float C1 = 0.0f, C2 = 0.0f;
+        v3 b1 = mk(0, 0, 0), b2 = b1, b3 = b1, b4 = b1;
+        for (int j = 0; j < 3; ++j) {
+            v3 q0j = pick3(q00, q01, q02, j), q1j = pick3(q10, q11, q12, j);
+            v3 P0j = sub(q0j, scl(dot(N2, sub(q0j, q10)), N2));
+            if (inside_tri(q10, q11, q12, P0j)) {
+                float cand = dot(N2, q0j) + d2;
+                if (C1 < -cand) {
+                    C1 = -cand;
+                    b1 = bary(q00, q01, q02, q0j);
+                    b2 = bary(q10, q11, q12, sub(q0j, scl(cand, N2)));
+                }
+            }
+            v3 Q0j;
+            if (find_isect(N1, q00, q1j, N2, &Q0j) && inside_tri(q00, q01, q02, Q0j)) {
+                float cand = dot(N2, Q0j) + d2;
+                if (C1 < -cand) {
+                    C1 = -cand;
+                    b1 = bary(q00, q01, q02, Q0j);
+                    b2 = bary(q10, q11, q12, sub(Q0j, scl(cand, N2)));
+                }
+            }
+            v3 P1j = sub(q1j, scl(dot(N1, sub(q1j, q00)), N1));
+            if (inside_tri(q00, q01, q02, P1j)) {
+                float cand = dot(N1, q1j) + d1;
+                if (C2 < -cand) {
+                    C2 = -cand;
+                    b3 = bary(q10, q11, q12, q1j);
+                    b4 = bary(q00, q01, q02, sub(q1j, scl(cand, N1)));
+                }
+            }
+            v3 Q1j;
+            if (find_isect(N2, q10, q0j, N1, &Q1j) && inside_tri(q10, q11, q12, Q1j)) {
+                float cand = dot(N1, Q1j) + d1;
+                if (C2 < -cand) {
+                    C2 = -cand;
+                    b3 = bary(q10, q11, q12, Q1j);
+                    b4 = bary(q00, q01, q02, sub(Q1j, scl(cand, N1)));
+                }
+            }
+            int jn = (j + 1) % 3;
+            v3 q0jn = pick3(q00, q01, q02, jn), q1jn = pick3(q10, q11, q12, jn);
+            v3 P0jn = sub(q0jn, scl(dot(N2, sub(q0jn, q10)), N2));
+            v3 P1jn = sub(q1jn, scl(dot(N1, sub(q1jn, q00)), N1));
+            for (int kk = 0; kk < 3; ++kk) {
+                int kn = (kk + 1) % 3;
+                v3 q1k = pick3(q10, q11, q12, kk), q1kn = pick3(q10, q11, q12, kn);
+                v3 hit;
+                if (line_isect(P0j, P0jn, q1k, q1kn, q0j, q0jn, &hit)) {
+                    float cand = dot(N2, hit) + d2;
+                    if (C1 < -cand) {
+                        C1 = -cand;
+                        b1 = bary(q00, q01, q02, hit);
+                        b2 = bary(q10, q11, q12, sub(hit, scl(cand, N2)));
+                    }
+                }
+                v3 q0k = pick3(q00, q01, q02, kk), q0kn = pick3(q00, q01, q02, kn);
+                if (line_isect(P1j, P1jn, q0k, q0kn, q1j, q1jn, &hit)) {
+                    float cand = dot(N1, hit) + d1;
+                    if (C2 < -cand) {
+                        C2 = -cand;
+                        b3 = bary(q10, q11, q12, hit);
+                        b4 = bary(q00, q01, q02, sub(hit, scl(cand, N1)));
+                    }
+                }
+            }
+        }
+        if (C1 <= offset_eps || C2 <= offset_eps) continue;
+        o->valid = 1;
+        o->C1 = C1;
+        o->C2 = C2;
+        if (C1 < C2) {
+            for (int j = 0; j < 3; ++j) { o->cur[j] = tris[3 * ta + j]; o->closest[j] = tris[3 * tb + j]; }
+            o->C = C1;
+            o->n[0] = N2.x; o->n[1] = N2.y; o->n[2] = N2.z;
+            o->b1[0] = b1.x; o->b1[1] = b1.y; o->b1[2] = b1.z;
+            o->b2[0] = b2.x; o->b2[1] = b2.y; o->b2[2] = b2.z;
+        } else {
+            for (int j = 0; j < 3; ++j) { o->cur[j] = tris[3 * tb + j]; o->closest[j] = tris[3 * ta + j]; }
+            o->C = C2;
+            o->n[0] = N1.x; o->n[1] = N1.y; o->n[2] = N1.z;
+            o->b1[0] = b3.x; o->b1[1] = b3.y; o->b1[2] = b3.z;
+            o->b2[0] = b4.x; o->b2[1] = b4.y; o->b2[2] = b4.z;
+        }
+    }
+}
+
+/* respond_kernel :714-799 (serial accumulation, contacts in pair order) */
+void orc_dcd_respond(long ncontacts, const orc_contact_t *cs, const float *x, const float *x_prev,
+                     const float *inv_mass, float multiplier, float mu_s, float mu_d, float *dP,
+                     float *dPw) {
+    for (long k = 0; k < ncontacts; ++k) {
+        const orc_contact_t *c = cs + k;
+        if (!c->valid) continue;
+        float pen = c->C;
+        float C = pen * multiplier;
+        if (C == 0.0f) continue;
+        v3 nrm = mk(c->n[0], c->n[1], c->n[2]);
+        const int *cu = c->cur, *cl = c->closest;
+        int solid_cur = (inv_mass[cu[0]] == 0.0f || inv_mass[cu[1]] == 0.0f || inv_mass[cu[2]] == 0.0f);
+        int solid_clo = (inv_mass[cl[0]] == 0.0f || inv_mass[cl[1]] == 0.0f || inv_mass[cl[2]] == 0.0f);
+        if (!solid_cur && !solid_clo) C = C * 0.5f;
+        v3 push_clo = solid_clo ? mk(0, 0, 0) : scl(-C, nrm);
+        v3 push_cur = solid_cur ? mk(0, 0, 0) : scl(C, nrm);
+        v3 v0 = sub(add(ld3(x, cl[0]), push_clo), ld3(x_prev, cl[0]));
+        v3 v1 = sub(add(ld3(x, cl[1]), push_clo), ld3(x_prev, cl[1]));
+        v3 v2 = sub(add(ld3(x, cl[2]), push_clo), ld3(x_prev, cl[2]));
+        v3 v3_ = sub(add(ld3(x, cu[0]), push_cur), ld3(x_prev, cu[0]));
+        v3 v4 = sub(add(ld3(x, cu[1]), push_cur), ld3(x_prev, cu[1]));
+        v3 v5 = sub(add(ld3(x, cu[2]), push_cur), ld3(x_prev, cu[2]));
+        v3 g = dvs(nrm, length(nrm));
+        v3 vclo = add(add(scl(c->b1[0], v0), scl(c->b1[1], v1)), scl(c->b1[2], v2));
+        v3 perp_clo = sub(vclo, scl(dot(vclo, g), g));
+        v3 vcur = add(add(scl(c->b2[0], v3_), scl(c->b2[1], v4)), scl(c->b2[2], v5));
+        v3 perp_cur = sub(vcur, scl(dot(vcur, g), g));
+        v3 diff = sub(perp_cur, perp_clo);
+        float dn = length(diff);
+        float static_norm = pen * mu_s;
+        if (dn > static_norm && dn > 0.0f) diff = scl(pen * mu_d / dn, diff);
+        v3 diffV = scl(0.5f, diff);
+        v3 diffT = sub(diff, diffV);
+        v3 dcur = sub(push_cur, diffV), dclo = add(push_clo, diffT);
+        for (int j = 0; j < 3; ++j) {
+            if (inv_mass[cu[j]] > 0.0f) {
+                dP[3 * cu[j]] += dcur.x; dP[3 * cu[j] + 1] += dcur.y; dP[3 * cu[j] + 2] += dcur.z;
+                dPw[cu[j]] += 1.0f;
+            }
+        }
+        for (int j = 0; j < 3; ++j) {
+            if (inv_mass[cl[j]] > 0.0f) {
+                dP[3 * cl[j]] += dclo.x; dP[3 * cl[j] + 1] += dclo.y; dP[3 * cl[j] + 2] += dclo.z;
+                dPw[cl[j]] += 1.0f;
+            }
+        }
+    }
+}
+
+/* apply_kernel :802-815 */
+void orc_dcd_apply(int n, float *pos, const float *dP, const float *dPw, int use_avg) {
+    for (int i = 0; i < n; ++i) {
+        float w = dPw[i];
+        if (w > 0.0f) {
+            v3 d = ld3(dP, i);
+            st3(pos, i, add(ld3(pos, i), use_avg ? dvs(d, w) : d));
+        }
+    }
+}

@@ -97,3 +97,32 @@ int  orc_count_inverted(int nt, const int *tets, const float *pos, const float *
 }
 #endif
 #endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* ---- tri-tri DCD (src/muscle/collision/udcf/kernel.py) ---------------- */
+typedef struct {
+    int cur[3], closest[3];
+    float C, C1, C2;
+    float n[3], b1[3], b2[3];
+    int valid;
+} orc_contact_t;
+
+void orc_dcd_tri_data(int m, const int *tris, const float *x, float *centroids, float *radii,
+                      float *normals);
+/* all pairs i<j passing the broadphase emit rules, in (i, j) order; returns the
+ * number found (entries beyond max_pairs are counted but not stored) */
+long orc_dcd_pairs_bruteforce(int m, const int *tris, const float *centroids, const float *radii,
+                              const float *normals, const int *group, float normal_cull,
+                              int allow_self, float thickness, long max_pairs, int *pair_a,
+                              int *pair_b);
+void orc_dcd_narrow(long npairs, const int *pair_a, const int *pair_b, const int *tris,
+                    const float *x, float offset_eps, orc_contact_t *out);
+void orc_dcd_respond(long ncontacts, const orc_contact_t *c, const float *x, const float *x_prev,
+                     const float *inv_mass, float multiplier, float mu_s, float mu_d, float *dP,
+                     float *dPw);
+void orc_dcd_apply(int n, float *pos, const float *dP, const float *dPw, int use_avg);
+#ifdef __cplusplus
+}
+#endif

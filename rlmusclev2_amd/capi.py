@@ -41,6 +41,13 @@ class XpbdTuning(C.Structure):
     _fields_ = [("patch_verts", C.c_int32), ("reserved", C.c_int32 * 7)]
 
 
+class XpbdDcdParams(C.Structure):
+    _fields_ = [("thickness", C.c_float), ("multiplier", C.c_float), ("friction_static", C.c_float),
+                ("friction_dynamic", C.c_float), ("offset", C.c_float), ("normal_cull", C.c_float),
+                ("cell_size", C.c_float), ("allow_self", C.c_int32), ("iters", C.c_int32),
+                ("rebuild_substeps", C.c_int32), ("max_pairs", C.c_int64)]
+
+
 class XpbdStats(C.Structure):
     _fields_ = [("nv", C.c_int32), ("nt", C.c_int32), ("n_elems", C.c_int32),
                 ("n_patches", C.c_int32), ("n_patch_tets", C.c_int32), ("max_table", C.c_int32),
@@ -90,6 +97,10 @@ SIGNATURES = {
     "xpbd_residuals": [_P, _P],
     "xpbd_count_inverted": [_P, _P],
     "xpbd_get_sdf_cells": [_P, _P],
+    "xpbd_set_dcd": [_P, _I, _P, _P, C.POINTER(XpbdDcdParams)],
+    "xpbd_dcd_solve": [_P, _I],
+    "xpbd_dcd_pairs": [_P, _P, C.c_int64, _P],
+    "xpbd_dcd_stats": [_P, _P, _P, _P],
     "xpbd_kernel_timing": [_P, _I],
     "xpbd_get_kernel_timing": [_P, _P, _P],
 }

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/xpbd.h"
+#include "xpbd_dcd.h"
 #include "xpbd_kernels.h"
 #include "xpbd_layout.h"
 
@@ -787,6 +788,423 @@ __global__ void k_count_inverted(int nt, const int4 *tets, const float *B, const
     if (inv) atomicAdd(count, 1);
 }
 
+
+// ---------------------------------------------------------------------------
+// tri-tri DCD kernels (semantics: src/muscle/collision/udcf/kernel.py)
+// ---------------------------------------------------------------------------
+#define DCD_THRESH 1.0e-5f
+constexpr float DCD_FIX = 4294967296.0f;  // 2^32 fixed-point scale (metres)
+
+__device__ __forceinline__ uint32_t dcd_hash(int x, int y, int z, int bits) {
+    const uint32_t h = ((uint32_t)x * 73856093u) ^ ((uint32_t)y * 19349663u) ^ ((uint32_t)z * 83492791u);
+    return h & ((1u << bits) - 1u);
+}
+
+// centroid_radius_kernel (:47-63) + face_normal_kernel (:66-84) + max radius
+__global__ void k_dcd_tridata(DcdDev d, const float4 *X) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.ntris) return;
+    const int4 t = d.tris[i];
+    const V3 a = xyz(X[t.x]), b = xyz(X[t.y]), c = xyz(X[t.z]);
+    const V3 s = (a + b) + c;
+    const V3 ce = V3{s.x / 3.0f, s.y / 3.0f, s.z / 3.0f};
+    const V3 e0 = a - ce, e1 = b - ce, e2 = c - ce;
+    const float r0 = sqrtf(dot(e0, e0)), r1 = sqrtf(dot(e1, e1)), r2 = sqrtf(dot(e2, e2));
+    const float r = fmaxf(fmaxf(r0, r1), r2);
+    d.cr[i] = make_float4(ce.x, ce.y, ce.z, r);
+    const V3 n = cross(b - a, c - a);
+    const float ln = sqrtf(dot(n, n));
+    d.nrm[i] = ln > 0.0f ? make_float4(n.x / ln, n.y / ln, n.z / ln, 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+    atomicMax(d.rmax_bits, __float_as_uint(r));  // r >= 0: uint order == float order
+}
+
+__global__ void k_dcd_cellsize(DcdDev d) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const float rmax = __uint_as_float(*d.rmax_bits);
+    // 1% margin keeps float rounding of c/cell from separating a valid pair by two cells
+    const float cell = 1.01f * fmaxf(fmaxf(d.mean_edge2, 2.0f * rmax + d.thickness), 1.0e-4f);
+    d.scal[0] = cell;
+    d.scal[1] = 1.0f / cell;
+    d.scal[2] = rmax;
+}
+
+__global__ void k_dcd_keys(DcdDev d) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.ntris) return;
+    const float inv = d.scal[1];
+    const float4 c = d.cr[i];
+    const int x = (int)floorf(c.x * inv), y = (int)floorf(c.y * inv), z = (int)floorf(c.z * inv);
+    const uint32_t h = dcd_hash(x, y, z, d.tbits);
+    d.cell[i] = make_int4(x, y, z, (int)h);
+    atomicAdd(&d.bucket_cnt[h], 1u);
+}
+
+__global__ void k_dcd_scatter(DcdDev d) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.ntris) return;
+    const uint32_t h = (uint32_t)d.cell[i].w;
+    const uint32_t k = atomicAdd(&d.bucket_fill[h], 1u);
+    d.sorted[d.bucket_off[h] + k] = i;
+}
+
+// emit rules of broadphase_find_pairs_kernel (:851-915) for candidate j of i
+__device__ __forceinline__ bool dcd_emit(const DcdDev &d, int i, int j, const int4 ti, float4 ci, float4 ni, int gi) {
+#pragma clang fp contract(off)
+    const int gj = d.group ? d.group[j] : 0;
+    if (gi == 1 && gj == 1) return false;
+    if (gi == 0 && gj == 0) {
+        if (!d.allow_self) return false;
+        const int4 tj = d.tris[j];
+        if (ti.x == tj.x || ti.x == tj.y || ti.x == tj.z || ti.y == tj.x || ti.y == tj.y || ti.y == tj.z ||
+            ti.z == tj.x || ti.z == tj.y || ti.z == tj.z)
+            return false;
+        if (d.normal_cull < 1.0f) {
+            const float4 nj = d.nrm[j];
+            if (ni.x * nj.x + ni.y * nj.y + ni.z * nj.z > d.normal_cull) return false;
+        }
+    }
+    const float4 cj = d.cr[j];
+    const V3 dd = V3{ci.x - cj.x, ci.y - cj.y, ci.z - cj.z};
+    return !(sqrtf(dot(dd, dd)) > ci.w + cj.w + d.thickness);
+}
+
+template <int FILL>
+__global__ void k_dcd_pairs(DcdDev d) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.ntris) return;
+    const int4 ci4 = d.cell[i];
+    const int4 ti = d.tris[i];
+    const float4 ci = d.cr[i];
+    const float4 ni = d.nrm[i];
+    const int gi = d.group ? d.group[i] : 0;
+    uint32_t cnt = 0;
+    uint64_t base = 0;
+    if (FILL) base = d.pair_off[i];
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int x = ci4.x + dx, y = ci4.y + dy, z = ci4.z + dz;
+                const uint32_t h = dcd_hash(x, y, z, d.tbits);
+                const uint32_t e = d.bucket_off[h + 1];
+                for (uint32_t q = d.bucket_off[h]; q < e; ++q) {
+                    const int j = d.sorted[q];
+                    if (j <= i) continue;
+                    const int4 cj = d.cell[j];
+                    if (cj.x != x || cj.y != y || cj.z != z) continue;  // visit j only from its own cell
+                    if (!dcd_emit(d, i, j, ti, ci, ni, gi)) continue;
+                    if (FILL) {
+                        const uint64_t o = base + cnt;
+                        if ((int64_t)o < d.max_pairs) d.pairs[o] = make_int2(i, j);
+                    }
+                    ++cnt;
+                }
+            }
+    if (!FILL) {
+        d.pair_cnt[i] = cnt;
+        return;
+    }
+    // sort this triangle's segment by j (insertion sort; segments are short)
+    const int64_t lo = (int64_t)base;
+    const int64_t hi = min((int64_t)base + (int64_t)cnt, d.max_pairs);
+    for (int64_t a = lo + 1; a < hi; ++a) {
+        const int2 v = d.pairs[a];
+        int64_t b = a - 1;
+        while (b >= lo && d.pairs[b].y > v.y) {
+            d.pairs[b + 1] = d.pairs[b];
+            --b;
+        }
+        d.pairs[b + 1] = v;
+    }
+}
+
+__device__ __forceinline__ V3 dcd_pick(V3 a, V3 b, V3 c, int i) { return i == 0 ? a : (i == 1 ? b : c); }
+
+__device__ __forceinline__ int dcd_inside(V3 p1, V3 p2, V3 p3, V3 p) {
+#pragma clang fp contract(off)
+    const V3 c1 = cross(p2 - p1, p - p2), c2 = cross(p3 - p2, p - p3), c3 = cross(p1 - p3, p - p1);
+    return (dot(c1, c2) > 0.0f && dot(c1, c3) > 0.0f) ? 1 : 0;
+}
+
+__device__ __forceinline__ V3 dcd_bary(V3 A, V3 B, V3 C, V3 P) {
+#pragma clang fp contract(off)
+    const V3 v0 = B - A, v1 = C - A, v2 = P - A;
+    const float d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1), d20 = dot(v2, v0), d21 = dot(v2, v1);
+    const float den = d00 * d11 - d01 * d01;
+    if (den == 0.0f) return V3{1.0f / 3.0f, 1.0f / 3.0f, 1.0f / 3.0f};
+    const float v = (d11 * d20 - d01 * d21) / den;
+    const float w = (d00 * d21 - d01 * d20) / den;
+    return V3{1.0f - v - w, v, w};
+}
+
+__device__ __forceinline__ int dcd_line(V3 p1, V3 p2, V3 p3, V3 p4, V3 v1, V3 v2, V3 &out) {
+#pragma clang fp contract(off)
+    const V3 d1 = p2 - p1, d2 = p4 - p3;
+    const V3 nr = cross(d1, d2);
+    if (sqrtf(dot(nr, nr)) < 1.0e-6f) return 0;
+    const float den = dot(nr, nr);
+    if (den < 1.0e-6f) return 0;
+    const V3 v = p3 - p1;
+    const float t1 = dot(cross(v, d2), nr) / den;
+    const float t2 = dot(cross(v, d1), nr) / den;
+    if (t2 >= 0.0f && t2 <= 1.0f && t1 >= 0.0f && t1 <= 1.0f) {
+        out = v1 + t1 * (v2 - v1);
+        return 1;
+    }
+    return 0;
+}
+
+__device__ __forceinline__ int dcd_gen_t(V3 N, V3 v0, V3 v1, V3 v2, float d, V3 D, float &lo, float &hi) {
+#pragma clang fp contract(off)
+    const float d0 = dot(N, v0) + d, d1 = dot(N, v1) + d, d2 = dot(N, v2) + d;
+    if (d0 <= DCD_THRESH && d1 <= DCD_THRESH && d2 <= DCD_THRESH) return 0;
+    if (d0 >= -DCD_THRESH && d1 >= -DCD_THRESH && d2 >= -DCD_THRESH) return 0;
+    V3 s0 = v0, s1 = v1, s2 = v2;
+    float e0 = d0, e1 = d1, e2 = d2;
+    if ((d0 <= 0.0f && d1 >= 0.0f && d2 >= 0.0f) || (d0 >= 0.0f && d1 <= 0.0f && d2 <= 0.0f)) {
+        s0 = v1; s1 = v0; e0 = d1; e1 = d0;
+    } else if ((d0 >= 0.0f && d1 >= 0.0f && d2 <= 0.0f) || (d0 <= 0.0f && d1 <= 0.0f && d2 >= 0.0f)) {
+        s1 = v2; s2 = v1; e1 = d2; e2 = d1;
+    }
+    const float p0 = dot(D, s0), p1 = dot(D, s1), p2 = dot(D, s2);
+    const float tA = p0 + (p1 - p0) * fabsf(e0 / (e0 - e1));
+    const float tB = p2 + (p1 - p2) * fabsf(e2 / (e2 - e1));
+    if (tA > tB) { lo = tB; hi = tA; } else { lo = tA; hi = tB; }
+    return 1;
+}
+
+// narrow_kernel (:317-474); the contact stays in its pair slot
+__global__ void k_dcd_narrow(DcdDev d, const float4 *X) {
+#pragma clang fp contract(off)
+    const int64_t total = min((int64_t)d.pair_off[d.ntris], d.max_pairs);
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        DcdContact o;
+        o.valid = 0;
+        const int2 pr = d.pairs[k];
+        const int4 A = d.tris[pr.x], Bt = d.tris[pr.y];
+        const V3 q00 = xyz(X[A.x]), q01 = xyz(X[A.y]), q02 = xyz(X[A.z]);
+        const V3 q10 = xyz(X[Bt.x]), q11 = xyz(X[Bt.y]), q12 = xyz(X[Bt.z]);
+        const V3 n1f = cross(q01 - q00, q02 - q00), n2f = cross(q11 - q10, q12 - q10);
+        const float l1 = sqrtf(dot(n1f, n1f)), l2 = sqrtf(dot(n2f, n2f));
+        bool ok = !(l1 == 0.0f || l2 == 0.0f);
+        V3 N1{}, N2{}, D{};
+        float d1 = 0.f, d2 = 0.f;
+        if (ok) {
+            N1 = n1f / l1;
+            N2 = n2f / l2;
+            d1 = -dot(N1, q00);
+            d2 = -dot(N2, q10);
+            const V3 cp = cross(N1, N2);
+            const float lcp = sqrtf(dot(cp, cp));
+            ok = !(lcp < 1.0e-8f);
+            if (ok) D = cp / lcp;
+        }
+        float t1, t2, t3, t4;
+        if (ok) ok = dcd_gen_t(N2, q00, q01, q02, d2, D, t1, t2) && dcd_gen_t(N1, q10, q11, q12, d1, D, t3, t4);
+        if (ok) ok = (t2 >= t3 && t4 >= t1);
+        if (ok) {
+            float C1 = 0.f, C2 = 0.f;
+            V3 b1{0, 0, 0}, b2{0, 0, 0}, b3{0, 0, 0}, b4{0, 0, 0};
+            for (int j = 0; j < 3; ++j) {
+                const V3 q0j = dcd_pick(q00, q01, q02, j), q1j = dcd_pick(q10, q11, q12, j);
+                const V3 P0j = q0j - dot(N2, q0j - q10) * N2;
+                if (dcd_inside(q10, q11, q12, P0j)) {
+                    const float cand = dot(N2, q0j) + d2;
+                    if (C1 < -cand) { C1 = -cand; b1 = dcd_bary(q00, q01, q02, q0j); b2 = dcd_bary(q10, q11, q12, q0j - cand * N2); }
+                }
+                const float dq = dot(N1, N2);
+                if (dq != 0.0f) {
+                    const V3 Q0j = q1j + (dot(N1, q00 - q1j) / dq) * N2;
+                    if (dcd_inside(q00, q01, q02, Q0j)) {
+                        const float cand = dot(N2, Q0j) + d2;
+                        if (C1 < -cand) { C1 = -cand; b1 = dcd_bary(q00, q01, q02, Q0j); b2 = dcd_bary(q10, q11, q12, Q0j - cand * N2); }
+                    }
+                }
+                const V3 P1j = q1j - dot(N1, q1j - q00) * N1;
+                if (dcd_inside(q00, q01, q02, P1j)) {
+                    const float cand = dot(N1, q1j) + d1;
+                    if (C2 < -cand) { C2 = -cand; b3 = dcd_bary(q10, q11, q12, q1j); b4 = dcd_bary(q00, q01, q02, q1j - cand * N1); }
+                }
+                const float dq2 = dot(N2, N1);
+                if (dq2 != 0.0f) {
+                    const V3 Q1j = q0j + (dot(N2, q10 - q0j) / dq2) * N1;
+                    if (dcd_inside(q10, q11, q12, Q1j)) {
+                        const float cand = dot(N1, Q1j) + d1;
+                        if (C2 < -cand) { C2 = -cand; b3 = dcd_bary(q10, q11, q12, Q1j); b4 = dcd_bary(q00, q01, q02, Q1j - cand * N1); }
+                    }
+                }
+                const int jn = (j + 1) % 3;
+                const V3 q0jn = dcd_pick(q00, q01, q02, jn), q1jn = dcd_pick(q10, q11, q12, jn);
+                const V3 P0jn = q0jn - dot(N2, q0jn - q10) * N2;
+                const V3 P1jn = q1jn - dot(N1, q1jn - q00) * N1;
+                for (int kk = 0; kk < 3; ++kk) {
+                    const int kn = (kk + 1) % 3;
+                    V3 hit;
+                    if (dcd_line(P0j, P0jn, dcd_pick(q10, q11, q12, kk), dcd_pick(q10, q11, q12, kn), q0j, q0jn, hit)) {
+                        const float cand = dot(N2, hit) + d2;
+                        if (C1 < -cand) { C1 = -cand; b1 = dcd_bary(q00, q01, q02, hit); b2 = dcd_bary(q10, q11, q12, hit - cand * N2); }
+                    }
+                    if (dcd_line(P1j, P1jn, dcd_pick(q00, q01, q02, kk), dcd_pick(q00, q01, q02, kn), q1j, q1jn, hit)) {
+                        const float cand = dot(N1, hit) + d1;
+                        if (C2 < -cand) { C2 = -cand; b3 = dcd_bary(q10, q11, q12, hit); b4 = dcd_bary(q00, q01, q02, hit - cand * N1); }
+                    }
+                }
+            }
+            if (!(C1 <= d.offset_eps || C2 <= d.offset_eps)) {
+                o.valid = 1;
+                o.C1 = C1;
+                o.C2 = C2;
+                const bool first = C1 < C2;
+                const int4 cu = first ? A : Bt, cl = first ? Bt : A;
+                o.cur[0] = cu.x; o.cur[1] = cu.y; o.cur[2] = cu.z;
+                o.closest[0] = cl.x; o.closest[1] = cl.y; o.closest[2] = cl.z;
+                o.C = first ? C1 : C2;
+                const V3 n = first ? N2 : N1, ba = first ? b1 : b3, bb = first ? b2 : b4;
+                o.n[0] = n.x; o.n[1] = n.y; o.n[2] = n.z;
+                o.b1[0] = ba.x; o.b1[1] = ba.y; o.b1[2] = ba.z;
+                o.b2[0] = bb.x; o.b2[1] = bb.y; o.b2[2] = bb.z;
+                atomicAdd(d.ncontact, 1u);
+            }
+        }
+        d.contacts[k] = o;
+    }
+}
+
+__device__ __forceinline__ void dcd_add(const DcdDev &d, int v, V3 c) {
+    atomicAdd(&d.acc[3 * v + 0], (unsigned long long)__float2ll_rn(c.x * DCD_FIX));
+    atomicAdd(&d.acc[3 * v + 1], (unsigned long long)__float2ll_rn(c.y * DCD_FIX));
+    atomicAdd(&d.acc[3 * v + 2], (unsigned long long)__float2ll_rn(c.z * DCD_FIX));
+    atomicAdd(&d.acc_n[v], 1u);
+}
+
+// respond_kernel (:714-799)
+__global__ void k_dcd_respond(DcdDev d, const float4 *X) {
+#pragma clang fp contract(off)
+    const int64_t total = min((int64_t)d.pair_off[d.ntris], d.max_pairs);
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const DcdContact &c = d.contacts[k];
+        if (!c.valid) continue;
+        const float pen = c.C;
+        float C = pen * d.multiplier;
+        if (C == 0.0f) continue;
+        const V3 nrm = V3{c.n[0], c.n[1], c.n[2]};
+        const int cu[3] = {c.cur[0], c.cur[1], c.cur[2]}, cl[3] = {c.closest[0], c.closest[1], c.closest[2]};
+        float wcu[3], wcl[3];
+        for (int j = 0; j < 3; ++j) { wcu[j] = X[cu[j]].w; wcl[j] = X[cl[j]].w; }
+        const bool solid_cur = wcu[0] == 0.0f || wcu[1] == 0.0f || wcu[2] == 0.0f;
+        const bool solid_clo = wcl[0] == 0.0f || wcl[1] == 0.0f || wcl[2] == 0.0f;
+        if (!solid_cur && !solid_clo) C = C * 0.5f;
+        const V3 push_clo = solid_clo ? V3{0, 0, 0} : (-C) * nrm;
+        const V3 push_cur = solid_cur ? V3{0, 0, 0} : C * nrm;
+        V3 vcl[3], vcu[3];
+        for (int j = 0; j < 3; ++j) {
+            vcl[j] = (xyz(X[cl[j]]) + push_clo) - xyz(d.xprev[cl[j]]);
+            vcu[j] = (xyz(X[cu[j]]) + push_cur) - xyz(d.xprev[cu[j]]);
+        }
+        const V3 g = nrm / sqrtf(dot(nrm, nrm));
+        const V3 vclo = (c.b1[0] * vcl[0] + c.b1[1] * vcl[1]) + c.b1[2] * vcl[2];
+        const V3 perp_clo = vclo - dot(vclo, g) * g;
+        const V3 vcur = (c.b2[0] * vcu[0] + c.b2[1] * vcu[1]) + c.b2[2] * vcu[2];
+        const V3 perp_cur = vcur - dot(vcur, g) * g;
+        V3 diff = perp_cur - perp_clo;
+        const float dn = sqrtf(dot(diff, diff));
+        if (dn > pen * d.mu_s && dn > 0.0f) diff = ((pen * d.mu_d) / dn) * diff;
+        const V3 diffV = 0.5f * diff;
+        const V3 diffT = diff - diffV;
+        const V3 dcur = push_cur - diffV, dclo = push_clo + diffT;
+        for (int j = 0; j < 3; ++j) {
+            if (wcu[j] > 0.0f) dcd_add(d, cu[j], dcur);
+            if (wcl[j] > 0.0f) dcd_add(d, cl[j], dclo);
+        }
+    }
+}
+
+// apply_kernel (:802-815), averaged; clears the accumulators for the next pass
+__global__ void k_dcd_apply(DcdDev d, float4 *X, int nv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    const uint32_t n = d.acc_n[i];
+    if (n > 0) {
+        const float w = (float)n;
+        float4 x = X[i];
+        const float inv = 1.0f / DCD_FIX;
+        x.x = x.x + ((float)(long long)d.acc[3 * i + 0] * inv) / w;
+        x.y = x.y + ((float)(long long)d.acc[3 * i + 1] * inv) / w;
+        x.z = x.z + ((float)(long long)d.acc[3 * i + 2] * inv) / w;
+        X[i] = x;
+        d.acc[3 * i + 0] = d.acc[3 * i + 1] = d.acc[3 * i + 2] = 0ull;
+        d.acc_n[i] = 0u;
+    }
+}
+
+__global__ void k_dcd_snapshot(float4 *dst, const float4 *src, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+__global__ void k_zero_u32(uint32_t *p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// exclusive scan of n uint32 -> out[0..n], three phases (block sums, scan of sums, add)
+constexpr int SCAN_BLOCK = 1024;
+__global__ __launch_bounds__(256) void k_scan_blocks(const uint32_t *in, uint32_t *out, uint32_t *bsum, int64_t n) {
+    __shared__ uint32_t s[256];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_BLOCK;
+    uint32_t v[4];
+    uint32_t loc = 0;
+    for (int k = 0; k < 4; ++k) {
+        const int64_t idx = base + threadIdx.x * 4 + k;
+        v[k] = idx < n ? in[idx] : 0u;
+        loc += v[k];
+    }
+    s[threadIdx.x] = loc;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t t = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t run = s[threadIdx.x] - loc;
+    for (int k = 0; k < 4; ++k) {
+        const int64_t idx = base + threadIdx.x * 4 + k;
+        if (idx < n) out[idx] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 255) bsum[blockIdx.x] = s[255];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_sums(uint32_t *bsum, int nb, uint32_t *total_out) {
+    __shared__ uint32_t s[1024];
+    uint32_t carry = 0;
+    for (int b0 = 0; b0 < nb; b0 += 1024) {
+        const int i = b0 + threadIdx.x;
+        const uint32_t v = i < nb ? bsum[i] : 0u;
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const uint32_t t = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0u;
+            __syncthreads();
+            s[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < nb) bsum[i] = carry + s[threadIdx.x] - v;
+        const uint32_t tot = s[1023];
+        __syncthreads();
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total_out = carry;
+}
+
+__global__ void k_scan_add(uint32_t *out, const uint32_t *bsum, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += bsum[i / SCAN_BLOCK];
+}
+
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
@@ -851,6 +1269,15 @@ struct xpbd_ctx {
     double *d_partial = nullptr;
     int n_partial = 0;
     bool pins_on = false;
+    // tri-tri DCD
+    bool dcd_on = false;
+    int dcd_iters = 2, dcd_rebuild_substeps = 0;
+    bool dcd_fresh = true;
+    std::vector<int32_t> dcd_tris, dcd_group;
+    xpbd_dcd_params dcd_p{};
+    DcdDev dcd{};
+    uint32_t *d_scan_bsum = nullptr;
+    int n_scan_bsum = 0;
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -1071,14 +1498,135 @@ int launch_post(xpbd_ctx *c, const float4 *xc, int do_sdf, int do_vel) {
     return XPBD_OK;
 }
 
+
+int launch_scan(xpbd_ctx *c, const uint32_t *in, uint32_t *out, int64_t n, uint32_t *total) {
+    const int nb = (int)((n + SCAN_BLOCK - 1) / SCAN_BLOCK);
+    if (nb > c->n_scan_bsum) return fail(c, XPBD_EINVAL, "scan scratch too small");
+    if (nb > 0) {
+        hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(256), 0, c->stream, in, out, c->d_scan_bsum, n);
+        HIPCHK(c, hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, c->stream, c->d_scan_bsum, nb, total);
+    HIPCHK(c, hipGetLastError());
+    if (nb > 0) {
+        hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, out,
+                           (const uint32_t *)c->d_scan_bsum, n);
+        HIPCHK(c, hipGetLastError());
+    }
+    return XPBD_OK;
+}
+
+// one DCD pass on positions X (in place); rebuild = broadphase first
+int launch_dcd(xpbd_ctx *c, float4 *X, bool rebuild) {
+    DcdDev &d = c->dcd;
+    const int m = d.ntris;
+    const int64_t T = (int64_t)1 << d.tbits;
+    int r;
+    auto g1 = [](int64_t n) { return dim3((unsigned)((n + 255) / 256)); };
+    if (rebuild) {
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, d.rmax_bits, (int64_t)1);
+        hipLaunchKernelGGL(k_zero_u32, g1(T), dim3(256), 0, c->stream, d.bucket_cnt, T);
+        hipLaunchKernelGGL(k_zero_u32, g1(T), dim3(256), 0, c->stream, d.bucket_fill, T);
+        hipLaunchKernelGGL(k_dcd_snapshot, g1(c->nv), dim3(256), 0, c->stream, d.xprev, (const float4 *)X, c->nv);
+        hipLaunchKernelGGL(k_dcd_tridata, g1(m), dim3(256), 0, c->stream, d, (const float4 *)X);
+        hipLaunchKernelGGL(k_dcd_cellsize, dim3(1), dim3(64), 0, c->stream, d);
+        hipLaunchKernelGGL(k_dcd_keys, g1(m), dim3(256), 0, c->stream, d);
+        HIPCHK(c, hipGetLastError());
+        if ((r = launch_scan(c, d.bucket_cnt, d.bucket_off, T, d.bucket_off + T))) return r;
+        hipLaunchKernelGGL(k_dcd_scatter, g1(m), dim3(256), 0, c->stream, d);
+        hipLaunchKernelGGL(k_dcd_pairs<0>, g1(m), dim3(256), 0, c->stream, d);
+        HIPCHK(c, hipGetLastError());
+        if ((r = launch_scan(c, d.pair_cnt, d.pair_off, m, d.pair_off + m))) return r;
+        hipLaunchKernelGGL(k_dcd_pairs<1>, g1(m), dim3(256), 0, c->stream, d);
+        HIPCHK(c, hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, d.ncontact, (int64_t)1);
+    const unsigned gp = 2048;
+    hipLaunchKernelGGL(k_dcd_narrow, dim3(gp), dim3(256), 0, c->stream, d, (const float4 *)X);
+    hipLaunchKernelGGL(k_dcd_respond, dim3(gp), dim3(256), 0, c->stream, d, (const float4 *)X);
+    hipLaunchKernelGGL(k_dcd_apply, g1(c->nv), dim3(256), 0, c->stream, d, X, c->nv);
+    HIPCHK(c, hipGetLastError());
+    return XPBD_OK;
+}
+
+int alloc_dcd(xpbd_ctx *c) {
+    DcdDev &d = c->dcd;
+    d = DcdDev{};
+    const int m = (int)(c->dcd_tris.size() / 3);
+    d.ntris = m;
+    int bits = 10;
+    while (((int64_t)1 << bits) < 2 * (int64_t)m && bits < 30) ++bits;
+    d.tbits = bits;
+    const int64_t T = (int64_t)1 << bits;
+    d.max_pairs = c->dcd_p.max_pairs > 0 ? c->dcd_p.max_pairs : ((int64_t)1 << 22);
+    d.thickness = c->dcd_p.thickness;
+    d.multiplier = c->dcd_p.multiplier;
+    d.mu_s = c->dcd_p.friction_static;
+    d.mu_d = c->dcd_p.friction_dynamic;
+    d.offset_eps = c->dcd_p.offset;
+    d.normal_cull = c->dcd_p.normal_cull;
+    d.allow_self = c->dcd_p.allow_self;
+    // 2 x mean rest edge length of the surface triangles
+    double es = 0.0;
+    for (int i = 0; i < m; ++i)
+        for (int k = 0; k < 3; ++k) {
+            const int a = c->dcd_tris[3 * i + k], b = c->dcd_tris[3 * i + (k + 1) % 3];
+            double q = 0;
+            for (int e = 0; e < 3; ++e) {
+                const double t = (double)c->pos0[3 * a + e] - c->pos0[3 * b + e];
+                q += t * t;
+            }
+            es += std::sqrt(q);
+        }
+    d.mean_edge2 = c->dcd_p.cell_size > 0.0f ? c->dcd_p.cell_size : (float)(m > 0 ? 2.0 * es / (3.0 * m) : 0.0);
+    std::vector<int4> tt(m);
+    for (int i = 0; i < m; ++i)
+        tt[i] = make_int4(c->L.iperm[c->dcd_tris[3 * i]], c->L.iperm[c->dcd_tris[3 * i + 1]],
+                          c->L.iperm[c->dcd_tris[3 * i + 2]], 0);
+    int r;
+    int4 *dt = nullptr;
+    if ((r = dupload(c, &dt, tt.data(), tt.size()))) return r;
+    d.tris = dt;
+    if (!c->dcd_group.empty()) {
+        int32_t *g = nullptr;
+        if ((r = dupload(c, &g, c->dcd_group.data(), c->dcd_group.size()))) return r;
+        d.group = g;
+    }
+    if ((r = dalloc(c, &d.cr, m)) || (r = dalloc(c, &d.nrm, m)) || (r = dalloc(c, &d.cell, m)) ||
+        (r = dalloc(c, &d.bucket_cnt, T)) || (r = dalloc(c, &d.bucket_off, T + 1)) ||
+        (r = dalloc(c, &d.bucket_fill, T)) || (r = dalloc(c, &d.sorted, m)) ||
+        (r = dalloc(c, &d.pair_cnt, (size_t)m + 1)) || (r = dalloc(c, &d.pair_off, (size_t)m + 1)) ||
+        (r = dalloc(c, &d.pairs, (size_t)d.max_pairs)) || (r = dalloc(c, &d.contacts, (size_t)d.max_pairs)) ||
+        (r = dalloc(c, &d.acc, 3 * (size_t)c->nv)) || (r = dalloc(c, &d.acc_n, (size_t)c->nv)) ||
+        (r = dalloc(c, &d.xprev, (size_t)c->nv)) || (r = dalloc(c, &d.scal, 4)) ||
+        (r = dalloc(c, &d.rmax_bits, 1)) || (r = dalloc(c, &d.ncontact, 1)))
+        return r;
+    HIPCHK(c, hipMemset(d.acc, 0, 3 * (size_t)c->nv * sizeof(unsigned long long)));
+    HIPCHK(c, hipMemset(d.acc_n, 0, (size_t)c->nv * sizeof(uint32_t)));
+    HIPCHK(c, hipMemset(d.pair_off, 0, ((size_t)m + 1) * sizeof(uint32_t)));
+    const int64_t need = std::max<int64_t>(T, (int64_t)m + 1);
+    const int nb = (int)((need + SCAN_BLOCK - 1) / SCAN_BLOCK) + 1;
+    if (nb > c->n_scan_bsum) {
+        if ((r = dalloc(c, &c->d_scan_bsum, (size_t)nb))) return r;
+        c->n_scan_bsum = nb;
+    }
+    c->dcd_on = m > 0;
+    c->dcd_fresh = true;
+    return XPBD_OK;
+}
+
 // one substep, positions start and end in X[0]
-int enqueue_substep(xpbd_ctx *c, int iters) {
+int enqueue_substep(xpbd_ctx *c, int iters, bool dcd_rebuild) {
     int r = launch_integrate(c);
     if (r) return r;
     for (int it = 0; it < iters; ++it) {
         r = launch_iter(c, c->X[it & 1], c->X[(it + 1) & 1], MODE_FUSED);
         if (r) return r;
     }
+    // post phase: self DCD passes, then bone push-out and velocities (mg.py:1362-1376)
+    if (c->dcd_on)
+        for (int k = 0; k < c->dcd_iters; ++k)
+            if ((r = launch_dcd(c, c->X[iters & 1], k == 0 && (dcd_rebuild || c->dcd_rebuild_substeps)))) return r;
     return launch_post(c, c->X[iters & 1], 1, 1);
 }
 
@@ -1487,6 +2035,7 @@ int32_t xpbd_finalize(xpbd_ctx *c, const xpbd_tuning *t) {
         HIPCHK(c, hipMemcpy(c->DPW, z.data(), nv * sizeof(float4), hipMemcpyHostToDevice));
     }
     c->ready = true;
+    if (!c->dcd_tris.empty() && (r = alloc_dcd(c))) return r;
     if ((r = upload_vertex_static(c))) return r;
     if ((r = upload_pins(c))) return r;
     if (c->has_force && (r = xpbd_set_force(c, c->force.data()))) return r;
@@ -1614,22 +2163,26 @@ int32_t xpbd_substep(xpbd_ctx *c, int32_t iters) {
     int r = check_ready(c);
     if (r) return r;
     if (iters < 0) return fail(c, XPBD_EINVAL, "negative iteration count");
-    return enqueue_substep(c, iters);
+    r = enqueue_substep(c, iters, c->dcd_fresh);
+    c->dcd_fresh = false;
+    return r;
 }
 
 int32_t xpbd_step_frame(xpbd_ctx *c, int32_t nsub, int32_t iters, int32_t use_graph) {
     int r = check_ready(c);
     if (r) return r;
     if (nsub < 0 || iters < 0) return fail(c, XPBD_EINVAL, "negative count");
+    // the DCD broadphase is rebuilt once per frame (humanV2.py:853-865)
+    c->dcd_fresh = false;
     if (!use_graph || c->timing) {
         for (int s = 0; s < nsub; ++s)
-            if ((r = enqueue_substep(c, iters))) return r;
+            if ((r = enqueue_substep(c, iters, s == 0))) return r;
         return XPBD_OK;
     }
     if (!c->gexec || c->g_nsub != nsub || c->g_iters != iters) {
         invalidate_graph(c);
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        for (int s = 0; s < nsub && r == XPBD_OK; ++s) r = enqueue_substep(c, iters);
+        for (int s = 0; s < nsub && r == XPBD_OK; ++s) r = enqueue_substep(c, iters, s == 0);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (r) { if (g) (void)hipGraphDestroy(g); return r; }
@@ -1694,6 +2247,66 @@ int32_t xpbd_get_sdf_cells(xpbd_ctx *c, int32_t *cells) {
         cells[2 * o] = h[2 * n];
         cells[2 * o + 1] = h[2 * n + 1];
     }
+    return XPBD_OK;
+}
+
+
+int32_t xpbd_set_dcd(xpbd_ctx *c, int32_t ntris, const int32_t *tris, const int32_t *group,
+                     const xpbd_dcd_params *p) {
+    if (!c || c->nv == 0 || ntris < 0 || (ntris > 0 && (!tris || !p))) return fail(c, XPBD_EINVAL, "bad dcd");
+    for (int64_t i = 0; i < 3 * (int64_t)ntris; ++i)
+        if (tris[i] < 0 || tris[i] >= c->nv) return fail(c, XPBD_EINVAL, "dcd triangle vertex out of range");
+    if (group)
+        for (int i = 0; i < ntris; ++i)
+            if (group[i] != 0 && group[i] != 1) return fail(c, XPBD_EINVAL, "dcd group must be 0 or 1");
+    c->dcd_tris.assign(tris, tris + 3 * (size_t)ntris);
+    if (group) c->dcd_group.assign(group, group + ntris);
+    else c->dcd_group.clear();
+    if (p) {
+        c->dcd_p = *p;
+        c->dcd_iters = p->iters > 0 ? p->iters : 2;
+        c->dcd_rebuild_substeps = p->rebuild_substeps;
+    }
+    if (c->ready) free_device(c);
+    return XPBD_OK;
+}
+
+int32_t xpbd_dcd_solve(xpbd_ctx *c, int32_t rebuild) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!c->dcd_on) return fail(c, XPBD_EINVAL, "no DCD surface set");
+    return launch_dcd(c, c->X[0], rebuild != 0);
+}
+
+int32_t xpbd_dcd_pairs(xpbd_ctx *c, int32_t *out, int64_t cap, int64_t *n_total) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!c->dcd_on) return fail(c, XPBD_EINVAL, "no DCD surface set");
+    uint32_t tot = 0;
+    HIPCHK(c, hipMemcpyAsync(&tot, c->dcd.pair_off + c->dcd.ntris, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (n_total) *n_total = tot;
+    const int64_t n = std::min<int64_t>(std::min<int64_t>(tot, c->dcd.max_pairs), cap);
+    if (out && n > 0) {
+        HIPCHK(c, hipMemcpyAsync(out, c->dcd.pairs, (size_t)n * sizeof(int2), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return XPBD_OK;
+}
+
+int32_t xpbd_dcd_stats(xpbd_ctx *c, int64_t *n_pairs, int64_t *n_contacts, float *cell) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!c->dcd_on) return fail(c, XPBD_EINVAL, "no DCD surface set");
+    uint32_t tot = 0, nc = 0;
+    float sc[4];
+    HIPCHK(c, hipMemcpyAsync(&tot, c->dcd.pair_off + c->dcd.ntris, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&nc, c->dcd.ncontact, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(sc, c->dcd.scal, sizeof(sc), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (n_pairs) *n_pairs = tot;
+    if (n_contacts) *n_contacts = nc;
+    if (cell) *cell = sc[0];
     return XPBD_OK;
 }
 

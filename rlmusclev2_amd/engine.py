@@ -80,12 +80,43 @@ class XpbdEngine:
                 k["goff"] = i32(sd.group_bone_off)
                 k["vgroup"] = i32(sd.vertex_group)
                 c.call("xpbd_set_sdf_groups", len(k["goff"]) - 1, ptr(k["goff"]), ptr(k["vgroup"]))
+        if s.dcd is not None:
+            self._set_dcd(s.dcd)
         tun = capi.XpbdTuning()
         tun.patch_verts = int(patch_verts)
         c.call("xpbd_finalize", C.byref(tun))
         if s.pin_vids is not None:
             tgt = f32(s.pos0, (-1, 3))[k["pin_vids"]]
             self.set_pin_targets(tgt)
+
+    def _set_dcd(self, spec):
+        from .collision import DcdParams
+        p = spec.params or DcdParams()
+        q = capi.XpbdDcdParams()
+        for f, _ in capi.XpbdDcdParams._fields_:
+            setattr(q, f, type(getattr(q, f))(getattr(p, f)))  # float or int field
+        k = self._keep
+        k["dcd_tris"] = i32(spec.tris, (-1, 3))
+        k["dcd_group"] = None if spec.group is None else i32(spec.group)
+        self.ctx.call("xpbd_set_dcd", k["dcd_tris"].shape[0], ptr(k["dcd_tris"]), ptr(k["dcd_group"]),
+                      C.byref(q))
+
+    def dcd_solve(self, rebuild: bool):
+        self.ctx.call("xpbd_dcd_solve", int(bool(rebuild)))
+
+    def dcd_pairs(self) -> np.ndarray:
+        n = np.zeros(1, np.int64)
+        self.ctx.call("xpbd_dcd_pairs", None, 0, ptr(n))
+        out = np.zeros((int(n[0]), 2), np.int32)
+        self.ctx.call("xpbd_dcd_pairs", ptr(out), int(n[0]), ptr(n))
+        return out
+
+    def dcd_stats(self) -> dict:
+        a = np.zeros(1, np.int64)
+        b = np.zeros(1, np.int64)
+        cell = np.zeros(1, np.float32)
+        self.ctx.call("xpbd_dcd_stats", ptr(a), ptr(b), ptr(cell))
+        return {"n_pairs": int(a[0]), "n_contacts": int(b[0]), "cell": float(cell[0])}
 
     def set_params(self, p):
         q = capi.XpbdParams()

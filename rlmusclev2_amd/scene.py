@@ -62,6 +62,7 @@ class Scene:
     pin_vids: np.ndarray | None = None
     pin_active: np.ndarray | None = None
     sdf: SdfColliders | None = None
+    dcd: object | None = None                  # collision.DcdSpec
     meta: dict = field(default_factory=dict)
 
     @property

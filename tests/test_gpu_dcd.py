@@ -1,0 +1,83 @@
+"""HIP tri-tri DCD vs the oracle: bit-exact sorted pair lists, one DCD pass
+(narrow + respond + averaged apply) within float tolerance, and the reference's
+penetration-resolution scenario."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from rlmusclev2_amd import sims, workloads
+from rlmusclev2_amd.collision import DcdParams, DcdSpec, surface_tris
+from rlmusclev2_amd.engine import XpbdEngine
+
+pytestmark = pytest.mark.gpu
+CFG = Path(__file__).resolve().parents[1] / "rlmusclev2_amd" / "configs"
+
+
+def _scene(seed=2, thickness=1e-3, cell=0.0):
+    cfg = json.loads((CFG / "humanv2_scale.json").read_text())
+    w = workloads.humanv2_scale(seed, n_bones=1, per_bone=6, target_tets=15000, fascia_tets=2000, sdf_res=32)
+    cfg["synthetic"]["sdf_res"] = 32
+    sc = sims.scene_from_config(cfg, w)
+    tris = surface_tris(w.tets)
+    sc.dcd = DcdSpec(tris=tris, params=DcdParams(thickness=thickness, cell_size=cell))
+    return sc, tris
+
+
+@pytest.mark.parametrize("cell", [0.0, 0.05])
+def test_pairs_bit_exact_vs_bruteforce(cell):
+    sc, tris = _scene(cell=cell)
+    eng = XpbdEngine(sc)
+    rng = np.random.default_rng(0)
+    x = sc.pos0 + rng.normal(scale=2e-3, size=sc.pos0.shape).astype(np.float32)
+    eng.set_state(x, x, np.zeros_like(x))
+    eng.dcd_solve(True)
+    got = eng.dcd_pairs()
+    ref = oracle.dcd_pairs(tris, x, allow_self=True, thickness=1e-3)
+    print("pairs", len(got), len(ref), eng.dcd_stats())
+    assert len(ref) > 1000
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_dcd_pass_matches_oracle():
+    sc, tris = _scene()
+    eng = XpbdEngine(sc)
+    x = sc.pos0.copy()
+    eng.set_state(x, x, np.zeros_like(x))
+    eng.dcd_solve(True)
+    after = eng.positions_np()
+    pairs = oracle.dcd_pairs(tris, x, allow_self=True, thickness=1e-3)
+    c, n = oracle.dcd_narrow(pairs, tris, x)
+    nvalid = sum(c[i].valid for i in range(n))
+    _, _, mass = oracle.precompute_rest(sc.pos0, sc.tets, sc.density)
+    im = np.where(mass > 0, 1.0 / np.maximum(mass, 1e-30), 0).astype(np.float32)
+    ref = oracle.dcd_respond_apply(c, n, x, x, im, 0.3, 0.5, 0.4)
+    st = eng.dcd_stats()
+    print("contacts", nvalid, st)
+    assert st["n_contacts"] == nvalid
+    assert nvalid > 0          # overlapping spindles in this workload
+    np.testing.assert_allclose(after, ref, atol=2e-6)
+
+
+def test_penetration_resolves_on_gpu():
+    """movable tilted triangle through a static one (tests/test_udcf_dcd.py:289-332)."""
+    from rlmusclev2_amd.builder import ConsTable
+    from rlmusclev2_amd.scene import Scene, SimParams
+    mov = np.array([[-0.5, -0.02, -0.3], [0.5, 0.04, -0.3], [0.0, -0.02, 0.6]], np.float32)
+    sta = np.array([[-1, 0, -1], [1, 0, -1], [0, 0, 1.5]], np.float32)
+    x = np.concatenate([mov, sta])
+    tris = np.array([[0, 1, 2], [3, 4, 5]], np.int32)
+    sc = Scene(pos0=x, tets=np.zeros((0, 4), np.int32), density=1.0, cons=ConsTable.empty(),
+               params=SimParams(dt=1 / 240), mass=np.array([1, 1, 1, 0, 0, 0], np.float32),
+               dcd=DcdSpec(tris=tris, group=np.array([0, 1], np.int32),
+                           params=DcdParams(thickness=0.0, multiplier=1.01)))
+    eng = XpbdEngine(sc)
+    depth0 = -float(x[:3, 1].min())
+    for it in range(8):
+        eng.dcd_solve(it == 0)
+    p = eng.positions_np()
+    assert eng.dcd_stats()["n_pairs"] == 1
+    assert -float(p[:3, 1].min()) < 0.5 * depth0
+    np.testing.assert_array_equal(p[3:], sta)

@@ -62,3 +62,104 @@ def test_small_humanv2_pins_sdf_matches_oracle(patch_verts):
     for k in rr:
         assert abs(rs[k] - rr[k]) <= 0.01 * rr[k] + 1e-12, (k, rs[k], rr[k])
     assert sim.count_inverted() == ref.count_inverted()
+
+
+def test_snh_aniso_activation_matches_oracle():
+    """SNH + aniso_arap with an activation ramp (c3 constraint set, small mesh)."""
+    cfg = _cfg("humanv2_snh_aniso.json")
+    w = workloads.humanv2_scale(3, n_bones=1, per_bone=4, target_tets=8000, fascia_tets=1500, sdf_res=32)
+    cfg["synthetic"]["sdf_res"] = 32
+    sim = sims.MultiGridSim(cfg, workload=w)
+    ref = sims.MultiGridSim(cfg, workload=w, engine_cls=oracle.OracleSim)
+    for f in range(1, 6):
+        a = np.full(w.nt, min(1.0, f / 5.0), np.float32)
+        sim.set_fiber_activation(a)
+        ref.set_fiber_activation(a)
+        sim.step_frame(f)
+        ref.step_frame(f)
+    p, q = sim.positions_np(), ref.positions_np()
+    tol = 1e-4 * _bbox_diag(w.pos0)
+    err = np.abs(p - q).max()
+    print("snh+aniso max |dx|", err, "tol", tol)
+    assert err <= tol
+    rs, rr = sim.residuals(), ref.residuals()
+    assert set(rr) == {"snh", "aniso"}
+    for k in rr:
+        assert abs(rs[k] - rr[k]) <= 0.01 * rr[k] + 1e-12, (k, rs[k], rr[k])
+
+
+def test_batch_instances_match_oracle():
+    """uneven instances in one launch, each with its own bone SDF group."""
+    cfg = _cfg("batch512.json")
+    w = workloads.batch(4, n_inst=6, lo=500, hi=3000, sdf_res=24)
+    cfg["synthetic"]["sdf_res"] = 24
+    sim = sims.MultiGridSim(cfg, workload=w, patch_verts=128)
+    ref = sims.MultiGridSim(cfg, workload=w, engine_cls=oracle.OracleSim)
+    rng = np.random.default_rng(0)
+    inst_act = rng.uniform(0, 1, w.info["n_inst"]).astype(np.float32)
+    for f in range(1, 4):
+        inst_act = np.clip(inst_act + rng.normal(0, 0.1, inst_act.shape), 0, 1).astype(np.float32)
+        a = inst_act[w.tet_instance]
+        sim.set_fiber_activation(a)
+        ref.set_fiber_activation(a)
+        sim.step_frame(f)
+        ref.step_frame(f)
+    p, q = sim.positions_np(), ref.positions_np()
+    tol = 1e-4 * _bbox_diag(w.pos0[w.vertex_group == 0])
+    err = np.abs(p - q).max()
+    print("batch max |dx|", err, "tol", tol)
+    assert err <= tol
+
+
+def test_deterministic_and_graph_equals_stream():
+    cfg = _cfg("humanv2_scale.json")
+    w = workloads.humanv2_scale(1, n_bones=1, per_bone=4, target_tets=6000, fascia_tets=1000, sdf_res=32)
+    cfg["synthetic"]["sdf_res"] = 32
+    outs = []
+    for use_graph in (True, True, False):
+        sim = sims.MultiGridSim(cfg, workload=w)
+        sim.use_graph = use_graph
+        for f in range(1, 4):
+            sim.step_frame(f)
+        outs.append(sim.positions_np())
+    np.testing.assert_array_equal(outs[0], outs[1])   # run-to-run bitwise identical
+    np.testing.assert_array_equal(outs[0], outs[2])   # graph replay == stream launches
+
+
+def test_phase_api_equals_fused_iteration():
+    """clear / solve_constraints / apply_dP / apply_pintotarget == one fused iteration."""
+    cfg = _cfg("humanv2_scale.json")
+    w = workloads.humanv2_scale(1, n_bones=1, per_bone=4, target_tets=6000, fascia_tets=1000, sdf_res=32)
+    cfg["synthetic"]["sdf_res"] = 32
+    a = sims.MultiGridSim(cfg, workload=w)
+    b = sims.MultiGridSim(cfg, workload=w)
+    for s in (a, b):
+        s.load_frame_targets(3)
+        s.engine.apply_pintotarget()
+        s.engine.integrate()
+    a.engine.iterate(1)
+    e = b.engine
+    e.clear()
+    e.solve_constraints()
+    e.apply_dP()
+    e.apply_pintotarget()
+    np.testing.assert_allclose(a.positions_np(), b.positions_np(), atol=1e-7)
+
+
+def test_sdf_cells_and_push_bit_exact():
+    """same positions in -> identical SDF cell indices and pushed positions."""
+    cfg = _cfg("humanv2_scale.json")
+    w = workloads.humanv2_scale(1, n_bones=2, per_bone=3, target_tets=6000, fascia_tets=1000, sdf_res=40)
+    cfg["synthetic"]["sdf_res"] = 40
+    sim = sims.MultiGridSim(cfg, workload=w)
+    ref = sims.MultiGridSim(cfg, workload=w, engine_cls=oracle.OracleSim)
+    rng = np.random.default_rng(1)
+    x = (w.pos0 + rng.normal(scale=5e-3, size=w.pos0.shape)).astype(np.float32)
+    for s in (sim, ref):
+        s.load_frame_targets(7)
+        s.engine.set_state(x, x, np.zeros_like(x))
+        s.engine.collide()
+    cg, cr = sim.engine.sdf_cells(), ref.engine.sdf_cells()
+    assert (cr[:, 0] >= 0).sum() > 100
+    np.testing.assert_array_equal(cg, cr)
+    np.testing.assert_array_equal(sim.positions_np(), ref.positions_np())
