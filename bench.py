@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--patch-verts", type=int, default=0, help="0 = autotune")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB (HBM counter calibration)")
     return ap.parse_args()
 
 
@@ -87,7 +89,17 @@ def main():
                                    "(instances or mesh partitions per GPU), --gpus must be 1"}))
         return 2
     import numpy as np
-    import torch  # noqa: F401  (device memory is owned by the HIP library; torch only confirms the GPU)
+    import torch  # device memory is owned by the HIP library; torch only runs the copy calibration
+
+    if args.calibrate_copy_mb:
+        n = args.calibrate_copy_mb * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_mib": args.calibrate_copy_mb, "bytes_read": 4 * n, "bytes_written": 4 * n}))
+        return 0
 
     from rlmusclev2_amd import autotune, sims
 
