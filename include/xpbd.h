@@ -55,8 +55,10 @@ typedef struct {
 
 /* Layout / launch tuning, filled by the autotuner or by hand.  0 = default. */
 typedef struct {
-    int32_t patch_verts;          /* owned vertices per patch = threads per block (64..256) */
-    int32_t reserved[7];
+    int32_t patch_verts;          /* patch strategy: owned vertices per patch = threads per block (64..256) */
+    int32_t strategy;             /* 0 = patch (LDS-staged, one launch per iteration), 1 = flat
+                                     (tet pass + vertex pass, two launches per iteration)        */
+    int32_t reserved[6];
 } xpbd_tuning;
 
 typedef struct {
@@ -172,6 +174,30 @@ int32_t xpbd_dcd_solve(xpbd_ctx *ctx, int32_t rebuild);
 int32_t xpbd_dcd_pairs(xpbd_ctx *ctx, int32_t *out, int64_t cap, int64_t *n_total);
 /* UdcfDcdCollider.stats (solver.py:442-457): raw pair count, contacts of the last pass, cell */
 int32_t xpbd_dcd_stats(xpbd_ctx *ctx, int64_t *n_pairs, int64_t *n_contacts, float *cell);
+
+/* ---- learned warm start: per-tet MLP (no reference counterpart) ------- */
+/* 24 features (22 used) -> 64 -> 64 -> 4 outputs (3 used), ReLU hidden layers; layers 1-2
+ * on BF16 MFMA with FP32 accumulate.  All pointers are device pointers. */
+#define XPBD_MLP_IN 24
+#define XPBD_MLP_HIDDEN 64
+#define XPBD_MLP_OUT 4
+typedef struct {
+    const uint16_t *W1;   /* bf16 bits [HIDDEN][IN]     */
+    const float *b1;      /* [HIDDEN]                   */
+    const uint16_t *W2;   /* bf16 bits [HIDDEN][HIDDEN] */
+    const float *b2;      /* [HIDDEN]                   */
+    const float *W3;      /* fp32 [OUT][HIDDEN]         */
+    const float *b3;      /* [OUT]                      */
+} xpbd_mlp_dev;
+/* Y (n x OUT); H1, H2 (n x HIDDEN, post-ReLU) written when non-NULL; stream = hipStream_t */
+int32_t xpbd_mlp_forward(const xpbd_mlp_dev *net, const float *X, int64_t n, float *Y, float *H1,
+                         float *H2, void *stream);
+/* number of floats of the gradient vector: W1, b1, W2, b2, W3, b3 flattened in that order */
+int32_t xpbd_mlp_grad_size(void);
+/* d loss / d params from dY (n x OUT); scratch >= ceil(n/32) * grad_size floats */
+int32_t xpbd_mlp_backward(const xpbd_mlp_dev *net, const float *X, const float *H1, const float *H2,
+                          const float *dY, int64_t n, float *grad, float *scratch, int64_t scratch_floats,
+                          void *stream);
 
 /* ---- diagnostics ------------------------------------------------------ */
 /* per-type residual sums: out[0] sum ||sigma-1||^2 over ARAP tets, out[1] count,

@@ -1,11 +1,13 @@
 """Launch-shape autotuner for the fused Jacobi projection.
 
-The tunable is the patch size (owned vertices per patch = threads per
-workgroup, 64/128/256): it trades workgroup count and occupancy against tet
-replication across patch borders and halo loads.  Each candidate layout is
-built for the actual mesh, a few iterations are timed with HIP events around
-the projection launches, and the fastest is kept.  Results are cached per mesh
-signature in ``rlmusclev2_amd/_tune/cache.json``.
+Candidates: the "patch" strategy (LDS-staged vertex table, one launch per
+iteration) at 64/128/256 owned vertices per workgroup, which trades workgroup
+count and occupancy against tet replication across patch borders, and the
+"flat" strategy (one thread per tet, then one per vertex; two launches per
+iteration, no replication).  Each candidate layout is built for the actual
+mesh, a few iterations are timed with HIP events around the projection
+launches, and the fastest is kept.  Results are cached per mesh signature in
+``rlmusclev2_amd/_tune/cache.json``.
 """
 from __future__ import annotations
 
@@ -16,7 +18,7 @@ from pathlib import Path
 import numpy as np
 
 CACHE = Path(__file__).resolve().parent / "_tune" / "cache.json"
-CANDIDATES = (64, 128, 256)
+CANDIDATES = ((1, 256), (0, 64), (0, 128), (0, 256))   # (strategy, patch_verts)
 
 
 def signature(cfg: dict, w) -> str:
@@ -36,30 +38,33 @@ def _load():
 def time_candidates(cfg: dict, w, candidates=CANDIDATES, iters: int = 20) -> dict:
     from .sims import MultiGridSim
     out = {}
-    for pv in candidates:
-        sim = MultiGridSim(cfg, workload=w, patch_verts=pv)
+    for strat, pv in candidates:
+        sim = MultiGridSim(cfg, workload=w, patch_verts=pv, strategy=strat)
         sim.engine.iterate(2)
         sim.engine.synchronize()
         sim.engine.kernel_timing(True)
         sim.engine.iterate(iters)
         ms, n = sim.engine.get_kernel_timing()
         sim.engine.kernel_timing(False)
-        out[pv] = ms * 1e3 / max(n, 1)
+        out[(strat, pv)] = ms * 1e3 / max(n, 1)
         sim.engine.close()
     return out
 
 
-def pick_patch_verts(cfg: dict, w, use_cache: bool = True) -> int:
+def pick(cfg: dict, w, use_cache: bool = True):
+    """(strategy, patch_verts, {candidate: us per iteration})"""
     key = signature(cfg, w)
     cache = _load()
     if use_cache and key in cache:
-        return int(cache[key]["patch_verts"])
+        e = cache[key]
+        return int(e["strategy"]), int(e["patch_verts"]), e["us_per_iteration"]
     times = time_candidates(cfg, w)
     best = min(times, key=times.get)
-    cache[key] = {"patch_verts": int(best), "us_per_iteration": {str(k): v for k, v in times.items()}}
+    table = {f"{'flat' if k[0] else 'patch'}{k[1] if not k[0] else ''}": round(v, 3) for k, v in times.items()}
+    cache[key] = {"strategy": int(best[0]), "patch_verts": int(best[1]), "us_per_iteration": table}
     try:
         CACHE.parent.mkdir(parents=True, exist_ok=True)
         CACHE.write_text(json.dumps(cache, indent=1))
     except OSError:
         pass
-    return int(best)
+    return int(best[0]), int(best[1]), table

@@ -38,7 +38,7 @@ class XpbdParams(C.Structure):
 
 
 class XpbdTuning(C.Structure):
-    _fields_ = [("patch_verts", C.c_int32), ("reserved", C.c_int32 * 7)]
+    _fields_ = [("patch_verts", C.c_int32), ("strategy", C.c_int32), ("reserved", C.c_int32 * 6)]
 
 
 class XpbdDcdParams(C.Structure):
@@ -46,6 +46,11 @@ class XpbdDcdParams(C.Structure):
                 ("friction_dynamic", C.c_float), ("offset", C.c_float), ("normal_cull", C.c_float),
                 ("cell_size", C.c_float), ("allow_self", C.c_int32), ("iters", C.c_int32),
                 ("rebuild_substeps", C.c_int32), ("max_pairs", C.c_int64)]
+
+
+class XpbdMlpDev(C.Structure):
+    _fields_ = [("W1", C.c_void_p), ("b1", C.c_void_p), ("W2", C.c_void_p), ("b2", C.c_void_p),
+                ("W3", C.c_void_p), ("b3", C.c_void_p)]
 
 
 class XpbdStats(C.Structure):
@@ -101,6 +106,9 @@ SIGNATURES = {
     "xpbd_dcd_solve": [_P, _I],
     "xpbd_dcd_pairs": [_P, _P, C.c_int64, _P],
     "xpbd_dcd_stats": [_P, _P, _P, _P],
+    "xpbd_mlp_forward": [C.POINTER(XpbdMlpDev), _P, C.c_int64, _P, _P, _P, _P],
+    "xpbd_mlp_grad_size": [],
+    "xpbd_mlp_backward": [C.POINTER(XpbdMlpDev), _P, _P, _P, _P, C.c_int64, _P, _P, C.c_int64, _P],
     "xpbd_kernel_timing": [_P, _I],
     "xpbd_get_kernel_timing": [_P, _P, _P],
 }

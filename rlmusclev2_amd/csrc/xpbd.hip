@@ -78,17 +78,19 @@ __device__ __forceinline__ bool polar_newton(const float *F, float *X) {
                     C8 = X[0] * X[4] - X[1] * X[3];
         const float det = X[0] * C0 + X[1] * C1 + X[2] * C2;
         if (!(det > 1.0e-30f)) return false;
-        const float idet = 1.0f / det;
-        float g = 1.0f;
+        const float idet = __builtin_amdgcn_rcpf(det);
+        float g = 1.0f, ig = 1.0f;
         if (scale) {
             float nx = 0.f;
 #pragma unroll
             for (int i = 0; i < 9; ++i) nx += X[i] * X[i];
             const float nc = C0 * C0 + C1 * C1 + C2 * C2 + C3 * C3 + C4 * C4 + C5 * C5 + C6 * C6 +
                              C7 * C7 + C8 * C8;
-            g = sqrtf(sqrtf(nc / nx) * idet);
+            // g = (||X^-1|| / ||X||)^(1/2) = (nc / nx)^(1/4) / sqrt(det)
+            g = __builtin_amdgcn_sqrtf(__builtin_amdgcn_sqrtf(nc * __builtin_amdgcn_rcpf(nx)) * idet);
+            ig = __builtin_amdgcn_rcpf(g);
         }
-        const float a = 0.5f * g, b = 0.5f * idet / g;
+        const float a = 0.5f * g, b = 0.5f * idet * ig;
         const float C[9] = {C0, C1, C2, C3, C4, C5, C6, C7, C8};
         float delta = 0.f;
 #pragma unroll
@@ -203,12 +205,11 @@ struct Corr {
 
 template <int FEAT, int DAMP>
 __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const float4 *sx,
-                                             const float4 *spp, Corr &out) {
-    const ushort4 li = a.lidx[g];
+                                             const float4 *spp, const int lv[4], Corr &out) {
     const uint32_t meta = a.meta[g];
     const uint32_t mask = (meta >> 16) & 15u;
-    const ClassParams cp = a.cls[meta & 0xffffu];
-    const int lv[4] = {li.x, li.y, li.z, li.w};
+    const DevClass dc = a.cls[meta & 0xffffu];
+    const ClassParams &cp = dc.p;
     const float4 X0 = sx[lv[0]], X1 = sx[lv[1]], X2 = sx[lv[2]], X3 = sx[lv[3]];
     const V3 P[4] = {xyz(X0), xyz(X1), xyz(X2), xyz(X3)};
     const float w[4] = {X0.w, X1.w, X2.w, X3.w};
@@ -241,7 +242,7 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
             c2s += G[i] * G[i];
         }
         if (!(c2s < 1.0e-12f)) {
-            const float C = sqrtf(c2s), invC = 1.0f / C;
+            const float C = __builtin_amdgcn_sqrtf(c2s), invC = __builtin_amdgcn_rcpf(C);
 #pragma unroll
             for (int i = 0; i < 9; ++i) G[i] *= invC;
             V3 gr[4];
@@ -252,8 +253,7 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
             const float wsum = w[0] * dot(gr[0], gr[0]) + w[1] * dot(gr[1], gr[1]) +
                                w[2] * dot(gr[2], gr[2]) + w[3] * dot(gr[3], gr[3]);
             if (wsum != 0.0f) {
-                const float at = (1.0f / cp.k_arap) / (dt * dt);
-                const float dL = -C / (wsum + at);
+                const float dL = -C * __builtin_amdgcn_rcpf(wsum + dc.at_arap);
 #pragma unroll
                 for (int i = 0; i < 4; ++i) out.d[i] = out.d[i] + (dL * w[i]) * gr[i];
                 out.cnt += 1.0f;
@@ -263,25 +263,25 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
 
     // ---- volume: tet_volume_update_xpbd_fn, solve_kernels.py:187-281 ----
     if ((FEAT & F_VOL) && (mask & M_VOL)) {
+        constexpr float SIXTH = 1.0f / 6.0f;
         V3 gr[4];
-        gr[0] = cross(c1, c2) / 6.0f;
-        gr[1] = cross(c2, c0) / 6.0f;
-        gr[2] = cross(c0, c1) / 6.0f;
+        gr[0] = SIXTH * cross(c1, c2);
+        gr[1] = SIXTH * cross(c2, c0);
+        gr[2] = SIXTH * cross(c0, c1);
         gr[3] = -(gr[0] + gr[1] + gr[2]);
         const float wsum = w[0] * dot(gr[0], gr[0]) + w[1] * dot(gr[1], gr[1]) +
                            w[2] * dot(gr[2], gr[2]) + w[3] * dot(gr[3], gr[3]);
         if (wsum > 1e-9f) {
-            const float vol = dot(cross(c0, c1), c2) / 6.0f;
+            const float vol = SIXTH * dot(cross(c0, c1), c2);
             const float rest = a.rvol[g];
             const bool comp = a.sp.has_compress && (vol < rest);
             const float k = comp ? cp.kcomp_vol : cp.k_vol;
             if (k != 0.0f) {
-                const float alpha = (1.0f / k) / (dt * dt);
+                const float alpha = comp ? dc.a_comp : dc.a_vol;
                 const float C = vol - rest;
                 float dsum = 0.f, gamma = 1.f;
                 if (DAMP && cp.damp_vol > 0.0f) {
-                    const float beta = k * cp.damp_vol * dt * dt;
-                    gamma = alpha * beta / dt;
+                    gamma = comp ? dc.g_comp : dc.g_vol;
                     for (int i = 0; i < 4; ++i) {
                         const float4 pp = spp[lv[i]];
                         dsum += dot(gr[i], P[i] - xyz(pp));
@@ -289,7 +289,7 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
                     dsum = dsum * gamma;
                     gamma = gamma + 1.0f;
                 }
-                const float dl = (-C - dsum) / (gamma * wsum + alpha);
+                const float dl = (-C - dsum) * __builtin_amdgcn_rcpf(gamma * wsum + alpha);
 #pragma unroll
                 for (int i = 0; i < 4; ++i) out.d[i] = out.d[i] + (dl * w[i]) * gr[i];
                 out.cnt += 1.0f;
@@ -396,9 +396,7 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
             const float wsum = w[0] * dot(gr[0], gr[0]) + w[1] * dot(gr[1], gr[1]) +
                                w[2] * dot(gr[2], gr[2]) + w[3] * dot(gr[3], gr[3]);
             if (!(wsum < 1.0e-20f)) {
-                const float sv = cp.k_aniso * a.sp.fiber_scale;
-                const float at = (1.0f / sv) / (dt * dt);
-                const float dL = -C / (wsum + at);
+                const float dL = -C * __builtin_amdgcn_rcpf(wsum + dc.at_aniso);
 #pragma unroll
                 for (int i = 0; i < 4; ++i) out.d[i] = out.d[i] + (dL * w[i]) * gr[i];
                 out.cnt += 1.0f;
@@ -476,7 +474,9 @@ __global__ __launch_bounds__(BS) void k_iterate(IterArgs a, const float4 *__rest
         const int t = base + tid;
         if (t < P.ntet) {
             Corr c;
-            project_elem<FEAT, DAMP>(a, (int64_t)P.tbeg + t, sx, spp, c);
+            const ushort4 li = a.lidx[(int64_t)P.tbeg + t];
+            const int lv[4] = {li.x, li.y, li.z, li.w};
+            project_elem<FEAT, DAMP>(a, (int64_t)P.tbeg + t, sx, spp, lv, c);
 #pragma unroll
             for (int k = 0; k < 4; ++k) sc[tid * 4 + k] = make_float4(c.d[k].x, c.d[k].y, c.d[k].z, c.cnt);
         }
@@ -520,6 +520,66 @@ __global__ __launch_bounds__(BS) void k_iterate(IterArgs a, const float4 *__rest
             xout[v] = x;
         }
     }
+}
+
+
+// ---------------------------------------------------------------------------
+// "flat" strategy: one thread per tet element (each element once, Morton
+// order) writes its four corner corrections; one thread per vertex then sums
+// its corners in fixed CSR order, applies dP/dPw, attach and pins.
+// ---------------------------------------------------------------------------
+template <int FEAT, int DAMP>
+__global__ __launch_bounds__(256) void k_flat_tets(IterArgs a, const float4 *__restrict__ xin,
+                                                   const float4 *__restrict__ pprev,
+                                                   float4 *__restrict__ corr) {
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= a.nptet) return;
+    const int4 gi = a.gidx[g];
+    const int lv[4] = {gi.x, gi.y, gi.z, gi.w};
+    Corr c;
+    project_elem<FEAT, DAMP>(a, g, xin, pprev, lv, c);
+    float4 *o = corr + 4 * g;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o[k] = make_float4(c.d[k].x, c.d[k].y, c.d[k].z, c.cnt);
+}
+
+template <int DAMP, int MODE>
+__global__ __launch_bounds__(256) void k_flat_verts(IterArgs a, const float4 *__restrict__ xin,
+                                                    float4 *__restrict__ xout,
+                                                    const float4 *__restrict__ pprev,
+                                                    const float4 *__restrict__ corr, int nv) {
+    const int v = blockIdx.x * 256 + threadIdx.x;
+    if (v >= nv) return;
+    float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
+    const int e = a.vslot_off[v + 1];
+    for (int s = a.vslot_off[v]; s < e; ++s) {
+        const float4 q = corr[a.vslot32[s]];
+        ax += q.x;
+        ay += q.y;
+        az += q.z;
+        aw += q.w;
+    }
+    float4 x = xin[v];
+    if (a.att0) attach_vertex(a, v, x, DAMP ? pprev[v] : make_float4(0, 0, 0, 0), ax, ay, az, aw);
+    if (MODE == MODE_ACCUM) {
+        float4 o = a.dpw[v];
+        o.x += ax;
+        o.y += ay;
+        o.z += az;
+        o.w += aw;
+        a.dpw[v] = o;
+        return;
+    }
+    if (aw > 1e-9f) {
+        x.x = x.x + ax / aw;
+        x.y = x.y + ay / aw;
+        x.z = x.z + az / aw;
+    }
+    if (a.pin) {
+        const float4 p = a.pin[v];
+        if (p.w != 0.0f) { x.x = p.x; x.y = p.y; x.z = p.z; }
+    }
+    xout[v] = x;
 }
 
 // ---------------------------------------------------------------------------
@@ -714,7 +774,7 @@ __global__ __launch_bounds__(BS) void k_residual(IterArgs a, const float4 *__res
         const uint32_t meta = a.meta[g];
         if (!((meta >> 20) & 1u)) continue;
         const uint32_t mask = (meta >> 16) & 15u;
-        const ClassParams cp = a.cls[meta & 0xffffu];
+        const ClassParams cp = a.cls[meta & 0xffffu].p;
         const int32_t *pv = pvert + 4 * g;
         const V3 P0 = xyz(X[pv[0]]), P1 = xyz(X[pv[1]]), P2 = xyz(X[pv[2]]), P3 = xyz(X[pv[3]]);
         const V3 c0 = P0 - P3, c1 = P1 - P3, c2 = P2 - P3;
@@ -1248,6 +1308,7 @@ struct xpbd_ctx {
     xpbd::Layout L;
     bool ready = false;
     int P = 256;
+    bool flat = false;
     int feature = 0;
     // device buffers
     std::vector<DBuf> bufs;
@@ -1259,8 +1320,11 @@ struct xpbd_ctx {
     ushort4 *d_lidx = nullptr;
     uint32_t *d_meta = nullptr;
     uint16_t *d_vslot = nullptr;
+    int4 *d_gidx = nullptr;
+    int32_t *d_vslot32 = nullptr;
+    float4 *CORR = nullptr;
     float *d_Bp = nullptr, *d_rvol = nullptr, *d_rsnh = nullptr, *d_act = nullptr;
-    ClassParams *d_cls = nullptr;
+    DevClass *d_cls = nullptr;
     int4 *d_tets = nullptr;
     float *d_B = nullptr;
     float *d_grids = nullptr, *d_bone = nullptr;
@@ -1390,6 +1454,26 @@ int upload_pins(xpbd_ctx *c) {
     return XPBD_OK;
 }
 
+std::vector<DevClass> dev_classes(const std::vector<ClassParams> &cls, const SimParams &sp) {
+    std::vector<DevClass> out(cls.size());
+    const float dt = sp.dt;
+    for (size_t i = 0; i < cls.size(); ++i) {
+        const ClassParams &p = cls[i];
+        DevClass &d = out[i];
+        d = DevClass{};
+        d.p = p;
+        d.at_arap = p.k_arap > 0.0f ? (1.0f / p.k_arap) / (dt * dt) : 0.0f;
+        d.a_vol = p.k_vol != 0.0f ? (1.0f / p.k_vol) / (dt * dt) : 0.0f;
+        d.a_comp = p.kcomp_vol != 0.0f ? (1.0f / p.kcomp_vol) / (dt * dt) : 0.0f;
+        // tet_volume_update_xpbd_fn damping: beta = k*ratio*dt^2, gamma = alpha*beta/dt
+        d.g_vol = d.a_vol * (p.k_vol * p.damp_vol * dt * dt) / dt;
+        d.g_comp = d.a_comp * (p.kcomp_vol * p.damp_vol * dt * dt) / dt;
+        const float sv = p.k_aniso * sp.fiber_scale;
+        d.at_aniso = sv > 0.0f ? (1.0f / sv) / (dt * dt) : 0.0f;
+    }
+    return out;
+}
+
 IterArgs make_args(xpbd_ctx *c, float4 *dpw) {
     IterArgs a;
     a.patch = c->d_patch;
@@ -1403,6 +1487,8 @@ IterArgs make_args(xpbd_ctx *c, float4 *dpw) {
     a.cls = c->d_cls;
     a.vslot_off = c->d_vslot_off;
     a.vslot = c->d_vslot;
+    a.gidx = c->d_gidx;
+    a.vslot32 = c->d_vslot32;
     a.pin = c->pins_on ? c->PIN : nullptr;
     a.att0 = (c->feature & F_ATTACH) ? c->ATT0 : nullptr;
     a.att1 = c->ATT1;
@@ -1443,15 +1529,38 @@ IterKernel pick_iter(int P, int feat, int mode) {
     }
 }
 
+typedef void (*FlatTetKernel)(IterArgs, const float4 *, const float4 *, float4 *);
+typedef void (*FlatVertKernel)(IterArgs, const float4 *, float4 *, const float4 *, const float4 *, int);
+
+template <int FEAT>
+FlatTetKernel pick_flat_damp(int damp) {
+    return damp ? k_flat_tets<FEAT, 1> : k_flat_tets<FEAT, 0>;
+}
+
+FlatTetKernel pick_flat_tets(int feat) {
+    constexpr int AV = F_ARAP | F_VOL, AVA = F_ARAP | F_VOL | F_ANISO, SA = F_SNH | F_ANISO,
+                  ALL = F_ARAP | F_VOL | F_SNH | F_ANISO;
+    const int f = feat & ALL;
+    const int damp = (feat & F_DAMP) ? 1 : 0;
+    if ((f & ~AV) == 0) return pick_flat_damp<AV>(damp);
+    if ((f & ~SA) == 0) return pick_flat_damp<SA>(damp);
+    if ((f & ~AVA) == 0) return pick_flat_damp<AVA>(damp);
+    return pick_flat_damp<ALL>(damp);
+}
+
+FlatVertKernel pick_flat_verts(int feat, int mode) {
+    const int damp = (feat & F_DAMP) ? 1 : 0;
+    if (mode == MODE_ACCUM) return damp ? k_flat_verts<1, MODE_ACCUM> : k_flat_verts<0, MODE_ACCUM>;
+    return damp ? k_flat_verts<1, MODE_FUSED> : k_flat_verts<0, MODE_FUSED>;
+}
+
 size_t iter_lds(const xpbd_ctx *c) {
     const int damp = (c->feature & F_DAMP) ? 2 : 1;
     return ((size_t)damp * c->L.max_table + (size_t)c->P * 4) * sizeof(float4);
 }
 
 int launch_iter(xpbd_ctx *c, const float4 *xin, float4 *xout, int mode) {
-    IterKernel k = pick_iter(c->P, c->feature, mode);
     IterArgs a = make_args(c, c->DPW);
-    const size_t lds = iter_lds(c);
     EvPair ev{};
     const bool tm = c->timing;
     if (tm) {
@@ -1460,9 +1569,27 @@ int launch_iter(xpbd_ctx *c, const float4 *xin, float4 *xout, int mode) {
         HIPCHK(c, hipEventRecord(ev.a, c->stream));
     }
     const float4 *pp = c->PPv;
-    void *args[] = {(void *)&a, (void *)&xin, (void *)&xout, (void *)&pp};
-    HIPCHK(c, hipLaunchKernel((const void *)k, dim3((unsigned)c->L.patches.size()), dim3(c->P), args, lds,
-                              c->stream));
+    if (c->flat) {
+        FlatTetKernel kt = pick_flat_tets(c->feature);
+        FlatVertKernel kv = pick_flat_verts(c->feature, mode);
+        const int64_t ne = (int64_t)c->L.meta.size();
+        float4 *corr = c->CORR;
+        void *at[] = {(void *)&a, (void *)&xin, (void *)&pp, (void *)&corr};
+        if (ne > 0)
+            HIPCHK(c, hipLaunchKernel((const void *)kt, dim3((unsigned)((ne + 255) / 256)), dim3(256), at, 0,
+                                      c->stream));
+        const float4 *cc = corr;
+        int nv = c->nv;
+        void *av[] = {(void *)&a, (void *)&xin, (void *)&xout, (void *)&pp, (void *)&cc, (void *)&nv};
+        HIPCHK(c, hipLaunchKernel((const void *)kv, dim3((unsigned)((nv + 255) / 256)), dim3(256), av, 0,
+                                  c->stream));
+    } else {
+        IterKernel k = pick_iter(c->P, c->feature, mode);
+        const size_t lds = iter_lds(c);
+        void *args[] = {(void *)&a, (void *)&xin, (void *)&xout, (void *)&pp};
+        HIPCHK(c, hipLaunchKernel((const void *)k, dim3((unsigned)c->L.patches.size()), dim3(c->P), args, lds,
+                                  c->stream));
+    }
     if (tm) {
         HIPCHK(c, hipEventRecord(ev.b, c->stream));
         c->evs.push_back(ev);
@@ -1829,6 +1956,11 @@ int32_t xpbd_set_params(xpbd_ctx *c, const xpbd_params *p) {
     c->sp = s;
     c->params_set = true;
     invalidate_graph(c);
+    if (c->ready && c->d_cls) {
+        std::vector<DevClass> dcl = dev_classes(c->L.classes, c->sp);
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpy(c->d_cls, dcl.data(), dcl.size() * sizeof(DevClass), hipMemcpyHostToDevice));
+    }
     if (c->ready && cr_changed && (c->feature & F_ANISO)) return xpbd_set_activation(c, c->activation.data());
     return XPBD_OK;
 }
@@ -1922,6 +2054,9 @@ int32_t xpbd_finalize(xpbd_ctx *c, const xpbd_tuning *t) {
     int P = (t && t->patch_verts > 0) ? t->patch_verts : 256;
     if (P != 64 && P != 128 && P != 256) return fail(c, XPBD_EINVAL, "patch_verts must be 64, 128 or 256");
     c->P = P;
+    const int strategy = t ? t->strategy : 0;
+    if (strategy != 0 && strategy != 1) return fail(c, XPBD_EINVAL, "strategy must be 0 (patch) or 1 (flat)");
+    c->flat = strategy == 1;
     xpbd::HostInput in;
     in.nv = c->nv;
     in.nt = c->nt;
@@ -1941,12 +2076,13 @@ int32_t xpbd_finalize(xpbd_ctx *c, const xpbd_tuning *t) {
     in.restvector = c->c_rvec.data();
     in.restdir = c->c_rdir.data();
     in.patch_verts = P;
+    in.flat = c->flat;
     c->L = xpbd::Layout();
     int code = 0;
     std::string m = xpbd::build_layout(in, c->L, &code);
     if (code) return fail(c, code, m);
     c->feature = c->L.feature;
-    const size_t lds = iter_lds(c);
+    const size_t lds = c->flat ? 0 : iter_lds(c);
     int maxlds = 0;
     HIPCHK(c, hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
     if (lds > (size_t)maxlds)
@@ -1974,7 +2110,13 @@ int32_t xpbd_finalize(xpbd_ctx *c, const xpbd_tuning *t) {
     r = dupload(c, &c->ATT1, (const float4 *)L.att1.data(), (size_t)nv); if (r) return r;
     UP(c->d_patch, L.patches);
     UP(c->d_halo, L.halo);
-    r = dupload(c, &c->d_lidx, (const ushort4 *)L.lidx.data(), nptet); if (r) return r;
+    if (c->flat) {
+        r = dupload(c, &c->d_gidx, (const int4 *)L.gidx.data(), nptet); if (r) return r;
+        UP(c->d_vslot32, L.vslot32);
+        if ((r = dalloc(c, &c->CORR, 4 * std::max<size_t>(nptet, 1)))) return r;
+    } else {
+        r = dupload(c, &c->d_lidx, (const ushort4 *)L.lidx.data(), nptet); if (r) return r;
+    }
     UP(c->d_meta, L.meta);
     UP(c->d_Bp, L.Bp);
     UP(c->d_rvol, L.rvol);
@@ -1982,12 +2124,17 @@ int32_t xpbd_finalize(xpbd_ctx *c, const xpbd_tuning *t) {
     r = dupload(c, &c->FIBS, (const float4 *)L.fib.data(), nptet); if (r) return r;
     r = dupload(c, &c->FIB, (const float4 *)L.fib.data(), nptet); if (r) return r;
     UP(c->d_ptet_tetid, L.ptet_tetid);
-    UP(c->d_cls, L.classes);
+    {
+        std::vector<DevClass> dcl = dev_classes(c->L.classes, c->sp);
+        UP(c->d_cls, dcl);
+    }
     UP(c->d_vslot_off, L.vslot_off);
     UP(c->d_vslot, L.vslot);
     UP(c->d_act, c->activation);
     // per-ptet global vertex ids (residual kernel)
-    {
+    if (c->flat) {
+        UP(c->d_pvert, L.gidx);
+    } else {
         std::vector<int32_t> pv(4 * nptet);
         for (const auto &P_ : L.patches)
             for (int j = 0; j < P_.ntet; ++j) {

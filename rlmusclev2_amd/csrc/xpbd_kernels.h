@@ -18,6 +18,18 @@ struct SimParams {
     float exp_damp, const_damp;
 };
 
+// per-class constants that depend on dt, folded on the host once per set_params
+struct DevClass {
+    ClassParams p;
+    float at_arap;   // (1/k_arap)/dt^2
+    float a_vol;     // (1/k_vol)/dt^2
+    float a_comp;    // (1/kcomp)/dt^2
+    float g_vol;     // damping gamma of the volume constraint (before +1)
+    float g_comp;
+    float at_aniso;  // (1/(k_aniso*fiber_scale))/dt^2
+    float pad0, pad1;
+};
+
 struct SdfParams {
     int nbones, res;
     float contact_offset, max_push;
@@ -35,9 +47,11 @@ struct IterArgs {
     const float *Bp;         // 9 planes of nptet
     const float *rvol, *rsnh;
     const float4 *fib;       // f0.xyz, fiberscale (per ptet, refreshed from activation)
-    const ClassParams *cls;
+    const DevClass *cls;
     const int32_t *vslot_off;
     const uint16_t *vslot;
+    const int4 *gidx;        // flat strategy: element vertex ids
+    const int32_t *vslot32;  // flat strategy: corner slot = 4*element + k
     const float4 *pin;       // target.xyz, active
     const float4 *att0, *att1;
     float4 *dpw;             // accumulate mode output

@@ -87,6 +87,95 @@ void precompute_rest(int nv, const float *pos0, int nt, const int32_t *tets, flo
     }
 }
 
+
+namespace {
+// attach / pin records of the host arrays, renumbered to the new vertex order
+void renumber_vertex_records(Layout &L, const std::vector<int32_t> &attach_old, int nv) {
+    std::vector<float> a0(L.att0.size()), a1(L.att1.size()), pc(L.pin_from_cons.size());
+    for (int v = 0; v < nv; ++v) {
+        int n = L.iperm[v];
+        std::memcpy(&a0[4 * (size_t)n], &L.att0[4 * (size_t)v], 16);
+        std::memcpy(&a1[4 * (size_t)n], &L.att1[4 * (size_t)v], 16);
+        std::memcpy(&pc[4 * (size_t)n], &L.pin_from_cons[4 * (size_t)v], 16);
+    }
+    L.att0.swap(a0);
+    L.att1.swap(a1);
+    L.pin_from_cons.swap(pc);
+    L.attach_vertex.resize(attach_old.size());
+    for (size_t i = 0; i < attach_old.size(); ++i) L.attach_vertex[i] = L.iperm[attach_old[i]];
+}
+
+double elem_static_bytes(int feature) {
+    double b = 4.0;  // meta
+    if (feature & (F_ARAP | F_SNH | F_ANISO)) b += 36.0;
+    if (feature & F_VOL) b += 4.0;
+    if (feature & F_SNH) b += 4.0;
+    if (feature & F_ANISO) b += 16.0;
+    return b;
+}
+
+double alg_bytes(int feature, double n_elems, double nv) {
+    double btet = 16.0;
+    if (feature & (F_ARAP | F_SNH | F_ANISO)) btet += 36.0;
+    if (feature & (F_VOL | F_SNH)) btet += 4.0;
+    if (feature & F_ANISO) btet += 20.0;
+    return btet * n_elems + 28.0 * nv;
+}
+}  // namespace
+
+std::string build_flat(const HostInput &in, Layout &L, const std::vector<Elem> &elems,
+                       const std::vector<uint32_t> &ecls, const std::vector<int32_t> &eord, int feature,
+                       int *code) {
+    const int nv = in.nv;
+    const size_t ne = eord.size();
+    L.patches.clear();
+    L.halo.clear();
+    L.lidx.clear();
+    L.gidx.assign(4 * ne, 0);
+    L.meta.assign(ne, 0);
+    L.Bp.assign(9 * ne, 0.f);
+    L.rvol.assign(ne, 0.f);
+    L.rsnh.assign(ne, 0.f);
+    L.fib.assign(4 * ne, 0.f);
+    L.ptet_tetid.assign(ne, 0);
+    L.vslot_off.assign(nv + 1, 0);
+    for (size_t g = 0; g < ne; ++g) {
+        const Elem &e = elems[eord[g]];
+        for (int k = 0; k < 4; ++k) {
+            const int v = L.iperm[e.p[k]];
+            L.gidx[4 * g + k] = v;
+            L.vslot_off[v + 1]++;
+        }
+        L.meta[g] = ecls[eord[g]] | (e.mask << 16) | (1u << 20);
+        const float *B = in.rest_matrix + 9 * (size_t)e.tetid;
+        for (int q = 0; q < 9; ++q) L.Bp[(size_t)q * ne + g] = B[q];
+        L.rvol[g] = e.rvol;
+        L.rsnh[g] = e.rsnh;
+        L.fib[4 * g + 0] = e.f0[0];
+        L.fib[4 * g + 1] = e.f0[1];
+        L.fib[4 * g + 2] = e.f0[2];
+        L.fib[4 * g + 3] = e.belly;
+        L.ptet_tetid[g] = e.tetid;
+    }
+    for (int v = 0; v < nv; ++v) L.vslot_off[v + 1] += L.vslot_off[v];
+    L.vslot32.assign((size_t)L.vslot_off[nv], 0);
+    std::vector<int32_t> fill(L.vslot_off.begin(), L.vslot_off.end() - 1);
+    for (size_t g = 0; g < ne; ++g)
+        for (int k = 0; k < 4; ++k) L.vslot32[fill[L.gidx[4 * g + k]]++] = (int32_t)(4 * g + k);
+    L.max_table = 0;
+    L.max_ptets = (int)ne;
+    L.bytes_alg = alg_bytes(feature, (double)ne, nv);
+    const double damp = (feature & F_DAMP) ? 2.0 : 1.0;
+    // tets: static data + index + 4 vertex gathers + 4 corner writes; vertices: offsets,
+    // corner slots + gathers, position read and write
+    L.bytes_impl = (elem_static_bytes(feature) + 16.0 + 64.0 * damp + 64.0) * (double)ne +
+                   (4.0 + 16.0 * damp + 16.0) * nv + 20.0 * (double)L.vslot32.size();
+    if (feature & F_PINS) L.bytes_impl += 16.0 * nv;
+    if (feature & F_ATTACH) L.bytes_impl += 32.0 * nv;
+    (void)code;
+    return std::string();
+}
+
 std::string build_layout(const HostInput &in, Layout &L, int *code) {
     *code = 0;
     const int nv = in.nv;
@@ -287,6 +376,11 @@ std::string build_layout(const HostInput &in, Layout &L, int *code) {
                          [&](int a, int b) { return ecode[a] < ecode[b]; });
     }
 
+    if (in.flat) {
+        renumber_vertex_records(L, attach_old, nv);
+        return build_flat(in, L, elems, ecls, eord, feature, code);
+    }
+
     // ---- 5. (patch, element) entries ----------------------------------------
     std::vector<int32_t> pcount(npatch + 1, 0);
     auto patches_of = [&](const Elem &e, int *out) {
@@ -397,29 +491,11 @@ std::string build_layout(const HostInput &in, Layout &L, int *code) {
     L.max_table = max_table;
     L.max_ptets = max_ptets;
 
-    // attach / pin records to new order
-    {
-        std::vector<float> a0(L.att0.size()), a1(L.att1.size()), pc(L.pin_from_cons.size());
-        for (int v = 0; v < nv; ++v) {
-            int n = L.iperm[v];
-            std::memcpy(&a0[4 * (size_t)n], &L.att0[4 * (size_t)v], 16);
-            std::memcpy(&a1[4 * (size_t)n], &L.att1[4 * (size_t)v], 16);
-            std::memcpy(&pc[4 * (size_t)n], &L.pin_from_cons[4 * (size_t)v], 16);
-        }
-        L.att0.swap(a0);
-        L.att1.swap(a1);
-        L.pin_from_cons.swap(pc);
-        L.attach_vertex.resize(attach_old.size());
-        for (size_t i = 0; i < attach_old.size(); ++i) L.attach_vertex[i] = L.iperm[attach_old[i]];
-    }
+    renumber_vertex_records(L, attach_old, nv);
 
     // ---- 7. byte accounting ---------------------------------------------------
     // algorithmic: each tet element's static data read once, each vertex read+written once
-    double btet = 16.0;
-    if (feature & (F_ARAP | F_SNH | F_ANISO)) btet += 36.0;
-    if (feature & (F_VOL | F_SNH)) btet += 4.0;
-    if (feature & F_ANISO) btet += 20.0;
-    L.bytes_alg = btet * (double)elems.size() + 28.0 * nv;
+    L.bytes_alg = alg_bytes(feature, (double)elems.size(), nv);
     double bpt = 8.0 + 4.0;  // lidx + meta
     if (feature & (F_ARAP | F_SNH | F_ANISO)) bpt += 36.0;
     if (feature & F_VOL) bpt += 4.0;

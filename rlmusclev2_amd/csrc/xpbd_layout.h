@@ -45,6 +45,7 @@ struct HostInput {
     const float *stiffness = nullptr, *compressionstiffness = nullptr, *dampingratio = nullptr,
                 *restlength = nullptr, *restvector = nullptr, *restdir = nullptr;
     int patch_verts = 256;
+    bool flat = false;   // one copy per element + per-vertex corner CSR (no patches)
 };
 
 struct Layout {
@@ -62,6 +63,8 @@ struct Layout {
     std::vector<int32_t> ptet_tetid;         // per ptet (activation index)
     std::vector<int32_t> vslot_off;          // nv+1 (new order)
     std::vector<uint16_t> vslot;             // slot = local tet*4 + corner
+    std::vector<int32_t> gidx;               // flat: 4 new vertex ids per element
+    std::vector<int32_t> vslot32;            // flat: corner slots 4*element+k per vertex
     std::vector<ClassParams> classes;
     // attach records in new vertex order
     std::vector<float> att0, att1;           // 4 per vertex

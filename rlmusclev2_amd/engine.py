@@ -26,7 +26,7 @@ from .scene import Scene
 class XpbdEngine:
     """Device state + phase primitives for one mesh (or one packed batch)."""
 
-    def __init__(self, scene: Scene, device: int = 0, patch_verts: int = 256):
+    def __init__(self, scene: Scene, device: int = 0, patch_verts: int = 256, strategy: int = 0):
         self.scene = scene
         self.ctx = capi.Context(device)
         self.n_verts = scene.nv
@@ -38,6 +38,7 @@ class XpbdEngine:
         self.collision_solvers_iter = []   # per-iteration Python hooks
         self.constraint_solvers = []       # MuscleSim extras
         self._keep = {}
+        self.strategy = int(strategy)
         self._upload(scene, patch_verts)
 
     # -- setup -----------------------------------------------------------
@@ -84,6 +85,7 @@ class XpbdEngine:
             self._set_dcd(s.dcd)
         tun = capi.XpbdTuning()
         tun.patch_verts = int(patch_verts)
+        tun.strategy = self.strategy
         c.call("xpbd_finalize", C.byref(tun))
         if s.pin_vids is not None:
             tgt = f32(s.pos0, (-1, 3))[k["pin_vids"]]

@@ -22,8 +22,8 @@ def _bbox_diag(p):
     return float(np.linalg.norm(p.max(0) - p.min(0)))
 
 
-def _run_pair(cfg, w, frames, patch_verts=256):
-    sim = sims.MultiGridSim(cfg, workload=w, patch_verts=patch_verts)
+def _run_pair(cfg, w, frames, patch_verts=256, strategy=0):
+    sim = sims.MultiGridSim(cfg, workload=w, patch_verts=patch_verts, strategy=strategy)
     ref = sims.MultiGridSim(cfg, workload=w, engine_cls=oracle.OracleSim)
     for f in range(1, frames + 1):
         print(f"frame {f}", flush=True)
@@ -47,12 +47,12 @@ def test_cantilever_10_frames_matches_oracle():
         assert abs(rs[k] - rr[k]) <= 0.01 * rr[k] + 1e-12, (k, rs[k], rr[k])
 
 
-@pytest.mark.parametrize("patch_verts", [64, 256])
-def test_small_humanv2_pins_sdf_matches_oracle(patch_verts):
+@pytest.mark.parametrize("strategy,patch_verts", [(0, 64), (0, 256), (1, 256)])
+def test_small_humanv2_pins_sdf_matches_oracle(strategy, patch_verts):
     cfg = _cfg("humanv2_scale.json")
     w = workloads.humanv2_scale(1, n_bones=2, per_bone=4, target_tets=12000, fascia_tets=3000, sdf_res=48)
     cfg["synthetic"]["sdf_res"] = 48
-    sim, ref = _run_pair(cfg, w, 10, patch_verts)
+    sim, ref = _run_pair(cfg, w, 10, patch_verts, strategy)
     p, q = sim.positions_np(), ref.positions_np()
     tol = 1e-4 * _bbox_diag(w.pos0)
     err = np.abs(p - q).max()
@@ -64,12 +64,13 @@ def test_small_humanv2_pins_sdf_matches_oracle(patch_verts):
     assert sim.count_inverted() == ref.count_inverted()
 
 
-def test_snh_aniso_activation_matches_oracle():
+@pytest.mark.parametrize("strategy", [0, 1])
+def test_snh_aniso_activation_matches_oracle(strategy):
     """SNH + aniso_arap with an activation ramp (c3 constraint set, small mesh)."""
     cfg = _cfg("humanv2_snh_aniso.json")
     w = workloads.humanv2_scale(3, n_bones=1, per_bone=4, target_tets=8000, fascia_tets=1500, sdf_res=32)
     cfg["synthetic"]["sdf_res"] = 32
-    sim = sims.MultiGridSim(cfg, workload=w)
+    sim = sims.MultiGridSim(cfg, workload=w, strategy=strategy)
     ref = sims.MultiGridSim(cfg, workload=w, engine_cls=oracle.OracleSim)
     for f in range(1, 6):
         a = np.full(w.nt, min(1.0, f / 5.0), np.float32)
@@ -116,14 +117,16 @@ def test_deterministic_and_graph_equals_stream():
     w = workloads.humanv2_scale(1, n_bones=1, per_bone=4, target_tets=6000, fascia_tets=1000, sdf_res=32)
     cfg["synthetic"]["sdf_res"] = 32
     outs = []
-    for use_graph in (True, True, False):
-        sim = sims.MultiGridSim(cfg, workload=w)
+    for use_graph, strat in ((True, 0), (True, 0), (False, 0), (True, 1), (True, 1)):
+        sim = sims.MultiGridSim(cfg, workload=w, strategy=strat)
         sim.use_graph = use_graph
         for f in range(1, 4):
             sim.step_frame(f)
         outs.append(sim.positions_np())
     np.testing.assert_array_equal(outs[0], outs[1])   # run-to-run bitwise identical
     np.testing.assert_array_equal(outs[0], outs[2])   # graph replay == stream launches
+    np.testing.assert_array_equal(outs[3], outs[4])   # flat strategy: run-to-run identical
+    np.testing.assert_allclose(outs[0], outs[3], atol=1e-5)
 
 
 def test_phase_api_equals_fused_iteration():
@@ -132,7 +135,7 @@ def test_phase_api_equals_fused_iteration():
     w = workloads.humanv2_scale(1, n_bones=1, per_bone=4, target_tets=6000, fascia_tets=1000, sdf_res=32)
     cfg["synthetic"]["sdf_res"] = 32
     a = sims.MultiGridSim(cfg, workload=w)
-    b = sims.MultiGridSim(cfg, workload=w)
+    b = sims.MultiGridSim(cfg, workload=w, strategy=1)
     for s in (a, b):
         s.load_frame_targets(3)
         s.engine.apply_pintotarget()
