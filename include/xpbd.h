@@ -199,6 +199,28 @@ int32_t xpbd_mlp_backward(const xpbd_mlp_dev *net, const float *X, const float *
                           const float *dY, int64_t n, float *grad, float *scratch, int64_t scratch_floats,
                           void *stream);
 
+/* multiplier modes (flat strategy): 0 = plain zero-lambda Jacobi (the reference GPU path,
+ * solve_kernels.py:271-281 with clear() every iteration, mg.py:1348); 1 = lambda carried across
+ * the iterations of a substep from 0; 2 = learned warm start: after integrate the network
+ * predicts lambda0 per tet, the first iteration applies it as an impulse along the constraint
+ * gradients, the rest carry lambda; 3 = the same with lambda0 = the previous substep's final
+ * multipliers (no network) */
+int32_t xpbd_set_lambda_mode(xpbd_ctx *ctx, int32_t mode);
+/* network weights (host fp32; W1, W2 stored as bf16), feature standardisation (IN each) and
+ * per-output scales (OUT) */
+int32_t xpbd_set_mlp(xpbd_ctx *ctx, const float *W1, const float *b1, const float *W2, const float *b2,
+                     const float *W3, const float *b3, const float *feat_mean, const float *feat_invstd,
+                     const float *out_scale);
+/* start-of-substep multiplier set-up of the current mode (xpbd_substep calls it after integrate) */
+int32_t xpbd_warm_start(xpbd_ctx *ctx);
+/* per-element features (n_elems x IN, device pointer; normalized = standardised) */
+int32_t xpbd_features(xpbd_ctx *ctx, float *dev_out, int32_t normalized);
+/* per-element multipliers (n_elems x 4 floats, device pointers) */
+int32_t xpbd_get_lambda(xpbd_ctx *ctx, float *dev_out);
+int32_t xpbd_set_lambda(xpbd_ctx *ctx, const float *dev_in);
+/* save (1) / restore (0) positions, previous positions and velocities on the device */
+int32_t xpbd_snapshot(xpbd_ctx *ctx, int32_t save);
+
 /* ---- diagnostics ------------------------------------------------------ */
 /* per-type residual sums: out[0] sum ||sigma-1||^2 over ARAP tets, out[1] count,
  * out[2] sum (V-V0)^2 over volume tets, out[3] count, out[4] sum C_D^2 + C_H^2 (SNH),

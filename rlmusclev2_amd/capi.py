@@ -109,6 +109,13 @@ SIGNATURES = {
     "xpbd_mlp_forward": [C.POINTER(XpbdMlpDev), _P, C.c_int64, _P, _P, _P, _P],
     "xpbd_mlp_grad_size": [],
     "xpbd_mlp_backward": [C.POINTER(XpbdMlpDev), _P, _P, _P, _P, C.c_int64, _P, _P, C.c_int64, _P],
+    "xpbd_set_lambda_mode": [_P, _I],
+    "xpbd_set_mlp": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "xpbd_warm_start": [_P],
+    "xpbd_features": [_P, _P, _I],
+    "xpbd_get_lambda": [_P, _P],
+    "xpbd_set_lambda": [_P, _P],
+    "xpbd_snapshot": [_P, _I],
     "xpbd_kernel_timing": [_P, _I],
     "xpbd_get_kernel_timing": [_P, _P, _P],
 }

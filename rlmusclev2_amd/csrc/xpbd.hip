@@ -203,9 +203,15 @@ struct Corr {
     float cnt;
 };
 
-template <int FEAT, int DAMP>
+// LAM = 0: plain zero-lambda Jacobi (the reference GPU path); 1: lambda carried
+// across the iterations of a substep (a.lam read and updated in place, one
+// thread per element); 2: warm-start impulse, the step applies dlambda = lambda0
+// from a.lam along each constraint gradient and keeps lambda = lambda0.
+template <int FEAT, int DAMP, int LAM = 0>
 __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const float4 *sx,
                                              const float4 *spp, const int lv[4], Corr &out) {
+    const float4 lam = LAM ? a.lam[g] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 nl = lam;
     const uint32_t meta = a.meta[g];
     const uint32_t mask = (meta >> 16) & 15u;
     const DevClass dc = a.cls[meta & 0xffffu];
@@ -253,7 +259,8 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
             const float wsum = w[0] * dot(gr[0], gr[0]) + w[1] * dot(gr[1], gr[1]) +
                                w[2] * dot(gr[2], gr[2]) + w[3] * dot(gr[3], gr[3]);
             if (wsum != 0.0f) {
-                const float dL = -C * __builtin_amdgcn_rcpf(wsum + dc.at_arap);
+                const float dL = LAM == 2 ? lam.x : -(C + dc.at_arap * lam.x) * __builtin_amdgcn_rcpf(wsum + dc.at_arap);
+                if (LAM == 1) nl.x = lam.x + dL;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) out.d[i] = out.d[i] + (dL * w[i]) * gr[i];
                 out.cnt += 1.0f;
@@ -289,7 +296,8 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
                     dsum = dsum * gamma;
                     gamma = gamma + 1.0f;
                 }
-                const float dl = (-C - dsum) * __builtin_amdgcn_rcpf(gamma * wsum + alpha);
+                const float dl = LAM == 2 ? lam.y : (-C - alpha * lam.y - dsum) * __builtin_amdgcn_rcpf(gamma * wsum + alpha);
+                if (LAM == 1) nl.y = lam.y + dl;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) out.d[i] = out.d[i] + (dl * w[i]) * gr[i];
                 out.cnt += 1.0f;
@@ -317,8 +325,8 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
             const float dsw = w[0] * dot(dg[0], dg[0]) + w[1] * dot(dg[1], dg[1]) +
                               w[2] * dot(dg[2], dg[2]) + w[3] * dot(dg[3], dg[3]);
             if (dsw != 0.0f) {
-                const float mu = cp.mu, lam = cp.lam;
-                const float hC = det3(F) - (1.0f + mu / lam);
+                const float mu = cp.mu, lamS = cp.lam;
+                const float hC = det3(F) - (1.0f + mu / lamS);
                 const float restv = a.rsnh[g];
                 const float inv_detDm = 1.0f / fmaxf(fabsf(restv) * 6.0f, 1.0e-20f);
                 V3 hg[4];
@@ -333,12 +341,12 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
                                      w[2] * dot(dg[2], hg[2]) + w[3] * dot(dg[3], hg[3]);
                     const float abs_vol = fmaxf(fabsf(restv), 1.0e-20f);
                     const float dalpha = 1.0f / (mu * abs_vol * dt * dt);
-                    const float halpha = 1.0f / (lam * abs_vol * dt * dt);
+                    const float halpha = 1.0f / (lamS * abs_vol * dt * dt);
                     float ddamp = 0.f, dgam = 1.f, hdamp = 0.f, hgam = 1.f;
                     if (DAMP && cp.damp_snh > 0.0f) {
                         const float kd = cp.damp_snh;
                         const float dbeta = mu * abs_vol * kd * dt * dt;
-                        const float hbeta = lam * abs_vol * kd * dt * dt;
+                        const float hbeta = lamS * abs_vol * kd * dt * dt;
                         dgam = dalpha * dbeta / dt;
                         hgam = halpha * hbeta / dt;
                         for (int i = 0; i < 4; ++i) {
@@ -356,10 +364,11 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
                     const float detA = Axx * Ayy - dh * dh;
                     if (!(detA < 1.0e-8f)) {
                         const float inv_det = 1.0f / detA;
-                        const float db = -dC - ddamp;
-                        const float hb = -hC - hdamp;
-                        const float ddL = inv_det * (Ayy * db - dh * hb);
-                        const float hdL = inv_det * (Axx * hb - dh * db);
+                        const float db = -dC - dalpha * lam.x - ddamp;
+                        const float hb = -hC - halpha * lam.y - hdamp;
+                        const float ddL = LAM == 2 ? lam.x : inv_det * (Ayy * db - dh * hb);
+                        const float hdL = LAM == 2 ? lam.y : inv_det * (Axx * hb - dh * db);
+                        if (LAM == 1) { nl.x = lam.x + ddL; nl.y = lam.y + hdL; }
 #pragma unroll
                         for (int i = 0; i < 4; ++i)
                             out.d[i] = out.d[i] + w[i] * (ddL * dg[i] + hdL * hg[i]);
@@ -396,13 +405,15 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
             const float wsum = w[0] * dot(gr[0], gr[0]) + w[1] * dot(gr[1], gr[1]) +
                                w[2] * dot(gr[2], gr[2]) + w[3] * dot(gr[3], gr[3]);
             if (!(wsum < 1.0e-20f)) {
-                const float dL = -C * __builtin_amdgcn_rcpf(wsum + dc.at_aniso);
+                const float dL = LAM == 2 ? lam.z : -(C + dc.at_aniso * lam.z) * __builtin_amdgcn_rcpf(wsum + dc.at_aniso);
+                if (LAM == 1) nl.z = lam.z + dL;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) out.d[i] = out.d[i] + (dL * w[i]) * gr[i];
                 out.cnt += 1.0f;
             }
         }
     }
+    if (LAM == 1) a.lam[g] = nl;
 }
 
 // ---------------------------------------------------------------------------
@@ -525,10 +536,12 @@ __global__ __launch_bounds__(BS) void k_iterate(IterArgs a, const float4 *__rest
 
 // ---------------------------------------------------------------------------
 // "flat" strategy: one thread per tet element (each element once, Morton
-// order) writes its four corner corrections; one thread per vertex then sums
-// its corners in fixed CSR order, applies dP/dPw, attach and pins.
+// order) writes its four corner corrections straight into a vertex-major
+// corner array; one thread per vertex then sums its contiguous run of corners
+// (ascending element order, so the sum order is fixed), applies dP/dPw,
+// attach and pins.
 // ---------------------------------------------------------------------------
-template <int FEAT, int DAMP>
+template <int FEAT, int DAMP, int LAM = 0>
 __global__ __launch_bounds__(256) void k_flat_tets(IterArgs a, const float4 *__restrict__ xin,
                                                    const float4 *__restrict__ pprev,
                                                    float4 *__restrict__ corr) {
@@ -537,10 +550,12 @@ __global__ __launch_bounds__(256) void k_flat_tets(IterArgs a, const float4 *__r
     const int4 gi = a.gidx[g];
     const int lv[4] = {gi.x, gi.y, gi.z, gi.w};
     Corr c;
-    project_elem<FEAT, DAMP>(a, g, xin, pprev, lv, c);
-    float4 *o = corr + 4 * g;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) o[k] = make_float4(c.d[k].x, c.d[k].y, c.d[k].z, c.cnt);
+    project_elem<FEAT, DAMP, LAM>(a, g, xin, pprev, lv, c);
+    const int4 so = a.slot_of[g];
+    corr[so.x] = make_float4(c.d[0].x, c.d[0].y, c.d[0].z, c.cnt);
+    corr[so.y] = make_float4(c.d[1].x, c.d[1].y, c.d[1].z, c.cnt);
+    corr[so.z] = make_float4(c.d[2].x, c.d[2].y, c.d[2].z, c.cnt);
+    corr[so.w] = make_float4(c.d[3].x, c.d[3].y, c.d[3].z, c.cnt);
 }
 
 template <int DAMP, int MODE>
@@ -553,7 +568,7 @@ __global__ __launch_bounds__(256) void k_flat_verts(IterArgs a, const float4 *__
     float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
     const int e = a.vslot_off[v + 1];
     for (int s = a.vslot_off[v]; s < e; ++s) {
-        const float4 q = corr[a.vslot32[s]];
+        const float4 q = corr[s];
         ax += q.x;
         ay += q.y;
         az += q.z;
@@ -580,6 +595,55 @@ __global__ __launch_bounds__(256) void k_flat_verts(IterArgs a, const float4 *__
         if (p.w != 0.0f) { x.x = p.x; x.y = p.y; x.z = p.z; }
     }
     xout[v] = x;
+}
+
+
+// per-element input features of the warm-start network (22 used of 24):
+// F - I (9), fibre direction (3), fibre target length (1), previous-substep
+// multipliers divided by the output scales (3), log10 rest volume, det F - 1,
+// |F|^2 - 3, log10 mean inverse mass, log10 ARAP and volume compliances.
+__global__ void k_features(IterArgs a, const float4 *__restrict__ X, float *__restrict__ out,
+                           const float *__restrict__ fmean, const float *__restrict__ finv,
+                           const float *__restrict__ oscale, int normalize) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.nptet) return;
+    const int4 gi = a.gidx[g];
+    const uint32_t meta = a.meta[g];
+    const uint32_t mask = (meta >> 16) & 15u;
+    const DevClass dc = a.cls[meta & 0xffffu];
+    const float4 X0 = X[gi.x], X1 = X[gi.y], X2 = X[gi.z], X3 = X[gi.w];
+    const V3 c0 = xyz(X0) - xyz(X3), c1 = xyz(X1) - xyz(X3), c2 = xyz(X2) - xyz(X3);
+    float B[9], F[9];
+    for (int q = 0; q < 9; ++q) B[q] = a.Bp[(int64_t)q * a.nptet + g];
+    const float Ds[9] = {c0.x, c1.x, c2.x, c0.y, c1.y, c2.y, c0.z, c1.z, c2.z};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) F[3 * i + j] = Ds[3 * i] * B[j] + Ds[3 * i + 1] * B[3 + j] + Ds[3 * i + 2] * B[6 + j];
+    float f[24];
+    for (int i = 0; i < 9; ++i) f[i] = F[i] - ((i % 4 == 0) ? 1.0f : 0.0f);
+    const float4 fb = (mask & M_ANISO) ? a.fib[g] : make_float4(0.f, 0.f, 0.f, 1.f);
+    f[9] = fb.x; f[10] = fb.y; f[11] = fb.z; f[12] = fb.w;
+    const float4 lp = a.lam ? a.lam[g] : make_float4(0.f, 0.f, 0.f, 0.f);
+    f[13] = lp.x / oscale[0]; f[14] = lp.y / oscale[1]; f[15] = lp.z / oscale[2];
+    const float detB = det3(B);
+    f[16] = log10f(1.0f / (6.0f * fabsf(detB) + 1e-30f) + 1e-30f);
+    float n2 = 0.f;
+    for (int i = 0; i < 9; ++i) n2 += F[i] * F[i];
+    f[17] = det3(F) - 1.0f;
+    f[18] = n2 - 3.0f;
+    f[19] = log10f(0.25f * (X0.w + X1.w + X2.w + X3.w) + 1e-30f);
+    f[20] = log10f(dc.at_arap + 1e-30f);
+    f[21] = log10f(dc.a_vol + 1e-30f);
+    f[22] = 0.f;
+    f[23] = 0.f;
+    float *o = out + g * 24;
+    for (int i = 0; i < 24; ++i) o[i] = normalize ? (f[i] - fmean[i]) * finv[i] : f[i];
+}
+
+// lambda0 = network output * per-type output scale
+__global__ void k_lam0(int64_t n, const float *__restrict__ Y, const float *__restrict__ oscale, float4 *lam) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    lam[g] = make_float4(Y[4 * g] * oscale[0], Y[4 * g + 1] * oscale[1], Y[4 * g + 2] * oscale[2], 0.f);
 }
 
 // ---------------------------------------------------------------------------
@@ -1333,6 +1397,16 @@ struct xpbd_ctx {
     double *d_partial = nullptr;
     int n_partial = 0;
     bool pins_on = false;
+    // learned warm start
+    int lam_mode = 0;             // 0 plain, 1 carry, 2 network warm start, 3 previous-substep warm start
+    bool impulse_pending = false;
+    float4 *LAMB = nullptr;
+    float *FEAT = nullptr, *MLPY = nullptr;
+    uint16_t *mW1 = nullptr, *mW2 = nullptr;
+    float *mb1 = nullptr, *mb2 = nullptr, *mW3 = nullptr, *mb3 = nullptr;
+    float *fmean = nullptr, *finv = nullptr, *oscale = nullptr;
+    bool mlp_ready = false;
+    float4 *S_X = nullptr, *S_PP = nullptr, *S_VEL = nullptr;
     // tri-tri DCD
     bool dcd_on = false;
     int dcd_iters = 2, dcd_rebuild_substeps = 0;
@@ -1488,11 +1562,12 @@ IterArgs make_args(xpbd_ctx *c, float4 *dpw) {
     a.vslot_off = c->d_vslot_off;
     a.vslot = c->d_vslot;
     a.gidx = c->d_gidx;
-    a.vslot32 = c->d_vslot32;
+    a.slot_of = (const int4 *)c->d_vslot32;
     a.pin = c->pins_on ? c->PIN : nullptr;
     a.att0 = (c->feature & F_ATTACH) ? c->ATT0 : nullptr;
     a.att1 = c->ATT1;
     a.dpw = dpw;
+    a.lam = c->LAMB;
     a.nptet = (int64_t)c->L.meta.size();
     a.max_table = c->L.max_table;
     a.sp = c->sp;
@@ -1532,20 +1607,25 @@ IterKernel pick_iter(int P, int feat, int mode) {
 typedef void (*FlatTetKernel)(IterArgs, const float4 *, const float4 *, float4 *);
 typedef void (*FlatVertKernel)(IterArgs, const float4 *, float4 *, const float4 *, const float4 *, int);
 
-template <int FEAT>
-FlatTetKernel pick_flat_damp(int damp) {
-    return damp ? k_flat_tets<FEAT, 1> : k_flat_tets<FEAT, 0>;
+template <int FEAT, int DAMP>
+FlatTetKernel pick_flat_lam(int lam) {
+    return lam == 2 ? k_flat_tets<FEAT, DAMP, 2> : (lam == 1 ? k_flat_tets<FEAT, DAMP, 1> : k_flat_tets<FEAT, DAMP, 0>);
 }
 
-FlatTetKernel pick_flat_tets(int feat) {
+template <int FEAT>
+FlatTetKernel pick_flat_damp(int damp, int lam) {
+    return damp ? pick_flat_lam<FEAT, 1>(lam) : pick_flat_lam<FEAT, 0>(lam);
+}
+
+FlatTetKernel pick_flat_tets(int feat, int lam) {
     constexpr int AV = F_ARAP | F_VOL, AVA = F_ARAP | F_VOL | F_ANISO, SA = F_SNH | F_ANISO,
                   ALL = F_ARAP | F_VOL | F_SNH | F_ANISO;
     const int f = feat & ALL;
     const int damp = (feat & F_DAMP) ? 1 : 0;
-    if ((f & ~AV) == 0) return pick_flat_damp<AV>(damp);
-    if ((f & ~SA) == 0) return pick_flat_damp<SA>(damp);
-    if ((f & ~AVA) == 0) return pick_flat_damp<AVA>(damp);
-    return pick_flat_damp<ALL>(damp);
+    if ((f & ~AV) == 0) return pick_flat_damp<AV>(damp, lam);
+    if ((f & ~SA) == 0) return pick_flat_damp<SA>(damp, lam);
+    if ((f & ~AVA) == 0) return pick_flat_damp<AVA>(damp, lam);
+    return pick_flat_damp<ALL>(damp, lam);
 }
 
 FlatVertKernel pick_flat_verts(int feat, int mode) {
@@ -1559,7 +1639,7 @@ size_t iter_lds(const xpbd_ctx *c) {
     return ((size_t)damp * c->L.max_table + (size_t)c->P * 4) * sizeof(float4);
 }
 
-int launch_iter(xpbd_ctx *c, const float4 *xin, float4 *xout, int mode) {
+int launch_iter(xpbd_ctx *c, const float4 *xin, float4 *xout, int mode, int lam = 0) {
     IterArgs a = make_args(c, c->DPW);
     EvPair ev{};
     const bool tm = c->timing;
@@ -1570,7 +1650,7 @@ int launch_iter(xpbd_ctx *c, const float4 *xin, float4 *xout, int mode) {
     }
     const float4 *pp = c->PPv;
     if (c->flat) {
-        FlatTetKernel kt = pick_flat_tets(c->feature);
+        FlatTetKernel kt = pick_flat_tets(c->feature, lam);
         FlatVertKernel kv = pick_flat_verts(c->feature, mode);
         const int64_t ne = (int64_t)c->L.meta.size();
         float4 *corr = c->CORR;
@@ -1742,12 +1822,48 @@ int alloc_dcd(xpbd_ctx *c) {
     return XPBD_OK;
 }
 
+// start-of-substep multiplier set-up for the lambda-carrying modes
+int launch_warm_start(xpbd_ctx *c) {
+    const int64_t ne = (int64_t)c->L.meta.size();
+    auto g1 = [](int64_t n) { return dim3((unsigned)((n + 255) / 256)); };
+    c->impulse_pending = false;
+    if (c->lam_mode == 1) {
+        hipLaunchKernelGGL(k_zero4, g1(ne), dim3(256), 0, c->stream, ne, c->LAMB);
+    } else if (c->lam_mode == 2) {
+        if (!c->mlp_ready) return fail(c, XPBD_EINVAL, "lambda mode 2 needs xpbd_set_mlp()");
+        IterArgs a = make_args(c, c->DPW);
+        hipLaunchKernelGGL(k_features, g1(ne), dim3(256), 0, c->stream, a, (const float4 *)c->X[0], c->FEAT,
+                           (const float *)c->fmean, (const float *)c->finv, (const float *)c->oscale, 1);
+        HIPCHK(c, hipGetLastError());
+        xpbd_mlp_dev net{c->mW1, c->mb1, c->mW2, c->mb2, c->mW3, c->mb3};
+        int r = xpbd_mlp_forward(&net, c->FEAT, ne, c->MLPY, nullptr, nullptr, c->stream);
+        if (r) return fail(c, r, "xpbd_mlp_forward failed");
+        hipLaunchKernelGGL(k_lam0, g1(ne), dim3(256), 0, c->stream, ne, (const float *)c->MLPY,
+                           (const float *)c->oscale, c->LAMB);
+        c->impulse_pending = true;
+    } else if (c->lam_mode == 3) {
+        c->impulse_pending = true;   // lambda0 = the multipliers the previous substep ended with
+    }
+    HIPCHK(c, hipGetLastError());
+    return XPBD_OK;
+}
+
+int lam_for_iteration(xpbd_ctx *c) {
+    if (c->lam_mode == 0) return 0;
+    if (c->impulse_pending) {
+        c->impulse_pending = false;
+        return 2;
+    }
+    return 1;
+}
+
 // one substep, positions start and end in X[0]
 int enqueue_substep(xpbd_ctx *c, int iters, bool dcd_rebuild) {
     int r = launch_integrate(c);
     if (r) return r;
+    if (c->lam_mode && (r = launch_warm_start(c))) return r;
     for (int it = 0; it < iters; ++it) {
-        r = launch_iter(c, c->X[it & 1], c->X[(it + 1) & 1], MODE_FUSED);
+        r = launch_iter(c, c->X[it & 1], c->X[(it + 1) & 1], MODE_FUSED, lam_for_iteration(c));
         if (r) return r;
     }
     // post phase: self DCD passes, then bone push-out and velocities (mg.py:1362-1376)
@@ -2114,6 +2230,23 @@ int32_t xpbd_finalize(xpbd_ctx *c, const xpbd_tuning *t) {
         r = dupload(c, &c->d_gidx, (const int4 *)L.gidx.data(), nptet); if (r) return r;
         UP(c->d_vslot32, L.vslot32);
         if ((r = dalloc(c, &c->CORR, 4 * std::max<size_t>(nptet, 1)))) return r;
+        if ((r = dalloc(c, &c->LAMB, std::max<size_t>(nptet, 1)))) return r;
+        HIPCHK(c, hipMemset(c->LAMB, 0, std::max<size_t>(nptet, 1) * sizeof(float4)));
+        if ((r = dalloc(c, &c->FEAT, 24 * std::max<size_t>(nptet, 1)))) return r;
+        if ((r = dalloc(c, &c->MLPY, 4 * std::max<size_t>(nptet, 1)))) return r;
+        if ((r = dalloc(c, &c->mW1, (size_t)XPBD_MLP_HIDDEN * XPBD_MLP_IN)) ||
+            (r = dalloc(c, &c->mW2, (size_t)XPBD_MLP_HIDDEN * XPBD_MLP_HIDDEN)) ||
+            (r = dalloc(c, &c->mb1, XPBD_MLP_HIDDEN)) || (r = dalloc(c, &c->mb2, XPBD_MLP_HIDDEN)) ||
+            (r = dalloc(c, &c->mW3, (size_t)XPBD_MLP_OUT * XPBD_MLP_HIDDEN)) || (r = dalloc(c, &c->mb3, XPBD_MLP_OUT)) ||
+            (r = dalloc(c, &c->fmean, 24)) || (r = dalloc(c, &c->finv, 24)) || (r = dalloc(c, &c->oscale, 4)))
+            return r;
+        {
+            std::vector<float> zero(24, 0.f), one(24, 1.f);
+            HIPCHK(c, hipMemcpy(c->fmean, zero.data(), 24 * sizeof(float), hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(c->finv, one.data(), 24 * sizeof(float), hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(c->oscale, one.data(), 4 * sizeof(float), hipMemcpyHostToDevice));
+        }
+        c->mlp_ready = false;
     } else {
         r = dupload(c, &c->d_lidx, (const ushort4 *)L.lidx.data(), nptet); if (r) return r;
     }
@@ -2164,6 +2297,7 @@ int32_t xpbd_finalize(xpbd_ctx *c, const xpbd_tuning *t) {
         if ((r = dalloc(c, &c->d_bone, 16 * (size_t)c->nbones))) return r;
     }
     if ((r = dalloc(c, &c->d_cells, 2 * (size_t)nv))) return r;
+    if ((r = dalloc(c, &c->S_X, nv)) || (r = dalloc(c, &c->S_PP, nv)) || (r = dalloc(c, &c->S_VEL, nv))) return r;
     if ((r = dalloc(c, &c->d_count, 1))) return r;
     c->n_partial = 512;
     if ((r = dalloc(c, &c->d_partial, 8 * (size_t)c->n_partial))) return r;
@@ -2289,7 +2423,7 @@ int32_t xpbd_iterate(xpbd_ctx *c, int32_t n) {
     if (r) return r;
     if (n < 0) return fail(c, XPBD_EINVAL, "negative iteration count");
     for (int it = 0; it < n; ++it)
-        if ((r = launch_iter(c, c->X[it & 1], c->X[(it + 1) & 1], MODE_FUSED))) return r;
+        if ((r = launch_iter(c, c->X[it & 1], c->X[(it + 1) & 1], MODE_FUSED, lam_for_iteration(c)))) return r;
     if (n & 1) {
         HIPCHK(c, hipMemcpyAsync(c->X[0], c->X[1], c->nv * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
     }
@@ -2454,6 +2588,106 @@ int32_t xpbd_dcd_stats(xpbd_ctx *c, int64_t *n_pairs, int64_t *n_contacts, float
     if (n_pairs) *n_pairs = tot;
     if (n_contacts) *n_contacts = nc;
     if (cell) *cell = sc[0];
+    return XPBD_OK;
+}
+
+
+static uint16_t f2bf(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u = u + (((u >> 16) & 1u) + 0x7FFFu);
+    return (uint16_t)(u >> 16);
+}
+
+int32_t xpbd_set_mlp(xpbd_ctx *c, const float *W1, const float *b1, const float *W2, const float *b2,
+                     const float *W3, const float *b3, const float *feat_mean, const float *feat_invstd,
+                     const float *out_scale) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!c->flat) return fail(c, XPBD_EUNSUPPORTED, "the learned warm start runs on the flat strategy");
+    if (!W1 || !b1 || !W2 || !b2 || !W3 || !b3 || !feat_mean || !feat_invstd || !out_scale)
+        return fail(c, XPBD_EINVAL, "null MLP parameter");
+    const int NH = XPBD_MLP_HIDDEN, NI = XPBD_MLP_IN, NO = XPBD_MLP_OUT;
+    std::vector<uint16_t> w1(NH * NI), w2(NH * NH);
+    for (int i = 0; i < NH * NI; ++i) w1[i] = f2bf(W1[i]);
+    for (int i = 0; i < NH * NH; ++i) w2[i] = f2bf(W2[i]);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->mW1, w1.data(), w1.size() * 2, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->mW2, w2.data(), w2.size() * 2, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->mb1, b1, NH * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->mb2, b2, NH * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->mW3, W3, NO * NH * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->mb3, b3, NO * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->fmean, feat_mean, 24 * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->finv, feat_invstd, 24 * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->oscale, out_scale, 4 * 4, hipMemcpyHostToDevice));
+    c->mlp_ready = true;
+    return XPBD_OK;
+}
+
+int32_t xpbd_set_lambda_mode(xpbd_ctx *c, int32_t mode) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (mode < 0 || mode > 3) return fail(c, XPBD_EINVAL, "lambda mode must be 0..3");
+    if (mode && !c->flat) return fail(c, XPBD_EUNSUPPORTED, "lambda modes run on the flat strategy");
+    if (mode && (c->feature & F_ARAP) && (c->feature & F_SNH))
+        return fail(c, XPBD_EUNSUPPORTED, "lambda modes need ARAP-SVD and SNH on separate meshes");
+    if (mode == 2 && !c->mlp_ready) return fail(c, XPBD_EINVAL, "lambda mode 2 needs xpbd_set_mlp()");
+    if (mode != c->lam_mode) invalidate_graph(c);
+    c->lam_mode = mode;
+    c->impulse_pending = false;
+    return XPBD_OK;
+}
+
+int32_t xpbd_warm_start(xpbd_ctx *c) {
+    int r = check_ready(c);
+    if (r) return r;
+    return c->lam_mode ? launch_warm_start(c) : XPBD_OK;
+}
+
+int32_t xpbd_features(xpbd_ctx *c, float *dev_out, int32_t normalized) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!c->flat) return fail(c, XPBD_EUNSUPPORTED, "features run on the flat strategy");
+    if (!dev_out) return fail(c, XPBD_EINVAL, "null output");
+    const int64_t ne = (int64_t)c->L.meta.size();
+    IterArgs a = make_args(c, c->DPW);
+    hipLaunchKernelGGL(k_features, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, c->stream, a,
+                       (const float4 *)c->X[0], dev_out, (const float *)c->fmean, (const float *)c->finv,
+                       (const float *)c->oscale, normalized ? 1 : 0);
+    HIPCHK(c, hipGetLastError());
+    return XPBD_OK;
+}
+
+int32_t xpbd_get_lambda(xpbd_ctx *c, float *dev_out) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!c->flat || !dev_out) return fail(c, XPBD_EINVAL, "flat strategy and an output buffer required");
+    HIPCHK(c, hipMemcpyAsync(dev_out, c->LAMB, c->L.meta.size() * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+    return XPBD_OK;
+}
+
+int32_t xpbd_set_lambda(xpbd_ctx *c, const float *dev_in) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!c->flat || !dev_in) return fail(c, XPBD_EINVAL, "flat strategy and an input buffer required");
+    HIPCHK(c, hipMemcpyAsync(c->LAMB, dev_in, c->L.meta.size() * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+    return XPBD_OK;
+}
+
+int32_t xpbd_snapshot(xpbd_ctx *c, int32_t save) {
+    int r = check_ready(c);
+    if (r) return r;
+    const size_t b = (size_t)c->nv * sizeof(float4);
+    if (save) {
+        HIPCHK(c, hipMemcpyAsync(c->S_X, c->X[0], b, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->S_PP, c->PPv, b, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->S_VEL, c->VEL, b, hipMemcpyDeviceToDevice, c->stream));
+    } else {
+        HIPCHK(c, hipMemcpyAsync(c->X[0], c->S_X, b, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->PPv, c->S_PP, b, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->VEL, c->S_VEL, b, hipMemcpyDeviceToDevice, c->stream));
+    }
     return XPBD_OK;
 }
 

@@ -51,10 +51,11 @@ struct IterArgs {
     const int32_t *vslot_off;
     const uint16_t *vslot;
     const int4 *gidx;        // flat strategy: element vertex ids
-    const int32_t *vslot32;  // flat strategy: corner slot = 4*element + k
+    const int4 *slot_of;     // flat strategy: vertex-major corner position of each element corner
     const float4 *pin;       // target.xyz, active
     const float4 *att0, *att1;
     float4 *dpw;             // accumulate mode output
+    float4 *lam;             // per element multipliers (x: arap or snh-D, y: volume or snh-H, z: aniso)
     int64_t nptet;
     int max_table;
     SimParams sp;

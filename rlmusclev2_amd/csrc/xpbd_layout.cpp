@@ -158,18 +158,20 @@ std::string build_flat(const HostInput &in, Layout &L, const std::vector<Elem> &
         L.ptet_tetid[g] = e.tetid;
     }
     for (int v = 0; v < nv; ++v) L.vslot_off[v + 1] += L.vslot_off[v];
-    L.vslot32.assign((size_t)L.vslot_off[nv], 0);
+    // corner (g, k) is written straight to its place in the vertex-major corner array,
+    // so each vertex later reads its corners as one contiguous run (ascending g)
+    L.vslot32.assign(4 * ne, 0);
     std::vector<int32_t> fill(L.vslot_off.begin(), L.vslot_off.end() - 1);
     for (size_t g = 0; g < ne; ++g)
-        for (int k = 0; k < 4; ++k) L.vslot32[fill[L.gidx[4 * g + k]]++] = (int32_t)(4 * g + k);
+        for (int k = 0; k < 4; ++k) L.vslot32[4 * g + k] = fill[L.gidx[4 * g + k]]++;
     L.max_table = 0;
     L.max_ptets = (int)ne;
     L.bytes_alg = alg_bytes(feature, (double)ne, nv);
     const double damp = (feature & F_DAMP) ? 2.0 : 1.0;
     // tets: static data + index + 4 vertex gathers + 4 corner writes; vertices: offsets,
     // corner slots + gathers, position read and write
-    L.bytes_impl = (elem_static_bytes(feature) + 16.0 + 64.0 * damp + 64.0) * (double)ne +
-                   (4.0 + 16.0 * damp + 16.0) * nv + 20.0 * (double)L.vslot32.size();
+    L.bytes_impl = (elem_static_bytes(feature) + 32.0 + 64.0 * damp + 64.0) * (double)ne +
+                   (4.0 + 16.0 * damp + 16.0) * nv + 16.0 * (double)L.vslot32.size();
     if (feature & F_PINS) L.bytes_impl += 16.0 * nv;
     if (feature & F_ATTACH) L.bytes_impl += 32.0 * nv;
     (void)code;

@@ -64,7 +64,7 @@ struct Layout {
     std::vector<int32_t> vslot_off;          // nv+1 (new order)
     std::vector<uint16_t> vslot;             // slot = local tet*4 + corner
     std::vector<int32_t> gidx;               // flat: 4 new vertex ids per element
-    std::vector<int32_t> vslot32;            // flat: corner slots 4*element+k per vertex
+    std::vector<int32_t> vslot32;            // flat: vertex-major position of corner (element, k)
     std::vector<ClassParams> classes;
     // attach records in new vertex order
     std::vector<float> att0, att1;           // 4 per vertex

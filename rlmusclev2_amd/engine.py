@@ -270,6 +270,32 @@ class XpbdEngine:
         self.ctx.call("xpbd_get_sdf_cells", ptr(out))
         return out
 
+    # -- learned warm start ----------------------------------------------------
+    def set_lambda_mode(self, mode: int):
+        self.ctx.call("xpbd_set_lambda_mode", int(mode))
+
+    def warm_start(self):
+        self.ctx.call("xpbd_warm_start")
+
+    def features(self, out, normalized: bool = False):
+        """per-element features into a (n_elems, 24) float32 CUDA tensor"""
+        self.ctx.call("xpbd_features", C.c_void_p(out.data_ptr()), int(bool(normalized)))
+
+    def get_lambda(self, out):
+        self.ctx.call("xpbd_get_lambda", C.c_void_p(out.data_ptr()))
+
+    def set_lambda(self, t):
+        self.ctx.call("xpbd_set_lambda", C.c_void_p(t.data_ptr()))
+
+    def snapshot(self, save: bool):
+        self.ctx.call("xpbd_snapshot", int(bool(save)))
+
+    def set_mlp(self, p: dict, feat_mean, feat_invstd, out_scale):
+        k = self._keep
+        k["mlp"] = [np.ascontiguousarray(p[n], np.float32) for n in ("W1", "b1", "W2", "b2", "W3", "b3")]
+        k["mlp_norm"] = [np.ascontiguousarray(a, np.float32) for a in (feat_mean, feat_invstd, out_scale)]
+        self.ctx.call("xpbd_set_mlp", *[ptr(a) for a in k["mlp"]], *[ptr(a) for a in k["mlp_norm"]])
+
     def kernel_timing(self, enable: bool):
         self.ctx.call("xpbd_kernel_timing", int(bool(enable)))
 

@@ -1,0 +1,269 @@
+"""Learned warm start for the Jacobi XPBD substep (no reference counterpart).
+
+The reference GPU path is "plain zero-lambda XPBD": every Jacobi iteration
+starts from lambda = 0 (solve_kernels.py:271-281, clear() each iteration at
+mg.py:1348).  Here a small per-tet MLP predicts, from local state at the start
+of a substep (deformation gradient, fibre data, activation, the previous
+substep's multipliers, rest volume, compliances), the multipliers that a
+converged lambda-carrying solve of that substep reaches.  The first iteration
+applies the prediction as an impulse along the constraint gradients; the
+remaining iterations carry lambda.  Plain XPBD is both the judge (its
+10-iteration residual is the target) and the per-frame fallback.
+
+Training data comes from plain-solver trajectories with randomized activations
+and bone motions; targets are the multipliers of a 40-iteration
+lambda-carrying solve from each sampled substep start.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import workloads
+from .mlp import NIN, NOUT, Mlp
+
+MODE_PLAIN, MODE_CARRY, MODE_NET, MODE_PREV = 0, 1, 2, 3
+
+
+def _dptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def randomize_motion(w, rng, amp_deg=(4.0, 12.0), freq=(0.3, 0.8)):
+    """New bone phases/amplitudes/frequencies (|dtheta/dt| <= 30 deg/s kept by clipping)."""
+    for bn in w.bones:
+        f = float(rng.uniform(*freq))
+        amp = np.deg2rad(rng.uniform(*amp_deg))
+        amp = min(amp, np.deg2rad(30.0) / (2 * np.pi * f))
+        bn.update(amp=amp, freq=f, phase=float(rng.uniform(0, 2 * np.pi)))
+
+
+class ActivationWalk:
+    """Per-muscle (or per-instance) activation random walk clipped to [0, 1]."""
+
+    def __init__(self, tet_group, rng, step=0.08):
+        self.g = np.asarray(tet_group)
+        self.n = int(self.g.max()) + 1 if self.g.size else 1
+        self.rng = rng
+        self.step = step
+        self.a = rng.uniform(0, 1, self.n).astype(np.float32)
+
+    def next(self):
+        self.a = np.clip(self.a + self.rng.normal(0, self.step, self.n), 0, 1).astype(np.float32)
+        return self.a[self.g]
+
+
+def tet_groups(w):
+    if w.tet_instance is not None:
+        return w.tet_instance
+    mid = w.info.get("muscle_id")
+    if mid is None:
+        return np.zeros(w.nt, np.int32)
+    return mid[w.tets[:, 0]]
+
+
+@dataclass
+class Dataset:
+    X: object            # torch (N, 24) raw features
+    Y: object            # torch (N, 4) converged multipliers
+    meta: dict = field(default_factory=dict)
+
+
+def collect(sim, n_frames: int, seed: int = 0, samples_per_substep: int = 8192, conv_iters: int = 40,
+            substep_stride: int = 2) -> Dataset:
+    """Plain-solver trajectory with randomized activations and bone motion; at
+    every ``substep_stride``-th substep record (features, converged multipliers)."""
+    import torch
+    e = sim.engine
+    w = sim.workload
+    rng = np.random.default_rng(seed)
+    randomize_motion(w, rng)
+    walk = ActivationWalk(tet_groups(w), rng)
+    ne = e.stats()["n_elems"]
+    feat = torch.empty((ne, NIN), dtype=torch.float32, device="cuda")
+    lam = torch.zeros((ne, 4), dtype=torch.float32, device="cuda")
+    lam_prev = torch.zeros((ne, 4), dtype=torch.float32, device="cuda")
+    Xs, Ys = [], []
+    has_fib = sim.scene.v_fiber_dir is not None
+    t0 = time.perf_counter()
+    sub = 0
+    for f in range(1, n_frames + 1):
+        if has_fib:
+            e.set_fiber_activation(walk.next())
+        sim.load_frame_targets(f)
+        e.apply_pintotarget()
+        for s in range(sim.nsubsteps):
+            e.set_lambda_mode(MODE_PLAIN)
+            e.integrate()
+            if sub % substep_stride == 0:
+                e.set_lambda(lam_prev)
+                e.features(feat, normalized=False)
+                e.snapshot(True)
+                e.set_lambda_mode(MODE_CARRY)
+                e.warm_start()
+                e.iterate(conv_iters)
+                e.get_lambda(lam)
+                e.snapshot(False)
+                e.set_lambda_mode(MODE_PLAIN)
+                idx = torch.randint(0, ne, (min(samples_per_substep, ne),), device="cuda")
+                Xs.append(feat[idx].clone())
+                Ys.append(lam[idx].clone())
+                lam_prev.copy_(lam)
+            e.iterate(sim.fine_iters)
+            e.collide()
+            e.update_velocities()
+            sub += 1
+    torch.cuda.synchronize()
+    return Dataset(torch.cat(Xs), torch.cat(Ys),
+                   dict(frames=n_frames, samples=int(sum(x.shape[0] for x in Xs)),
+                        seconds=time.perf_counter() - t0, conv_iters=conv_iters))
+
+
+@dataclass
+class Normalizer:
+    mean: np.ndarray
+    invstd: np.ndarray
+    oscale: np.ndarray
+
+
+def fit_normalizer(ds: Dataset) -> Normalizer:
+    import torch
+    X = ds.X.double()
+    mean = X.mean(0)
+    std = X.std(0)
+    invstd = torch.where(std > 1e-6, 1.0 / std, torch.zeros_like(std))
+    Y = ds.Y.double()
+    osc = Y.abs().mean(0)
+    osc = torch.where(osc > 1e-12, osc, torch.ones_like(osc))
+    return Normalizer(mean.float().cpu().numpy(), invstd.float().cpu().numpy(), osc.float().cpu().numpy())
+
+
+def train(ds: Dataset, norm: Normalizer, epochs: int = 30, batch: int = 16384, lr: float = 2e-3,
+          seed: int = 0, log=None) -> tuple[Mlp, list]:
+    """MSE on normalized multipliers; forward/backward are the HIP MFMA kernels."""
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    mean = torch.tensor(norm.mean, device="cuda")
+    inv = torch.tensor(norm.invstd, device="cuda")
+    osc = torch.tensor(norm.oscale, device="cuda")
+    X = ((ds.X - mean) * inv).contiguous()
+    Y = (ds.Y / osc).contiguous()
+    n = X.shape[0]
+    nval = max(1024, n // 20)
+    perm = torch.randperm(n, generator=g, device="cuda")
+    val, tr = perm[:nval], perm[nval:]
+    net = Mlp(seed=seed)
+    hist = []
+    steps_per_epoch = max(1, tr.numel() // batch)
+    total = epochs * steps_per_epoch
+    step = 0
+    for ep in range(epochs):
+        order = tr[torch.randperm(tr.numel(), generator=g, device="cuda")]
+        for b in range(steps_per_epoch):
+            idx = order[b * batch:(b + 1) * batch]
+            xb, yb = X[idx].contiguous(), Y[idx].contiguous()
+            yp, h1, h2 = net.forward(xb, keep=True)
+            dy = ((yp - yb) / xb.shape[0]).contiguous()
+            lr_t = lr * 0.5 * (1 + np.cos(np.pi * step / total))
+            net.adam_step(net.backward(xb, h1, h2, dy), lr=lr_t)
+            step += 1
+        yv = net.forward(X[val].contiguous())
+        vl = float(((yv - Y[val]) ** 2).mean())
+        base = float((Y[val] ** 2).mean())
+        hist.append({"epoch": ep, "val_mse": vl, "zero_mse": base})
+        if log:
+            log(hist[-1])
+    return net, hist
+
+
+def install(engine, net: Mlp, norm: Normalizer):
+    p = net.numpy_params()
+    arr = {k: np.ascontiguousarray(v, np.float32) for k, v in p.items()}
+    mean = np.ascontiguousarray(norm.mean, np.float32)
+    inv = np.ascontiguousarray(norm.invstd, np.float32)
+    osc = np.ascontiguousarray(norm.oscale, np.float32)
+    engine.set_mlp(arr, mean, inv, osc)
+
+
+def save(path, net: Mlp, norm: Normalizer, meta: dict | None = None):
+    p = net.numpy_params()
+    np.savez(path, **{k: v.astype(np.float32) for k, v in p.items()}, feat_mean=norm.mean,
+             feat_invstd=norm.invstd, out_scale=norm.oscale, meta=np.array(str(meta or {})))
+
+
+def load(path):
+    z = np.load(path, allow_pickle=False)
+    p = {k: z[k] for k in ("W1", "b1", "W2", "b2", "W3", "b3")}
+    return Mlp(p), Normalizer(z["feat_mean"], z["feat_invstd"], z["out_scale"])
+
+
+def _res(e):
+    r = e.residuals()
+    return r
+
+
+def iterations_to_reach(sim, mode: int, target: dict, kmax: int = 10):
+    """From the current (integrated) state: run ``mode`` one iteration at a time
+    and return the first K whose every per-type residual is <= target, else None.
+    State is restored afterwards."""
+    e = sim.engine
+    e.snapshot(True)
+    e.set_lambda_mode(mode)
+    e.warm_start()
+    hit = None
+    for k in range(1, kmax + 1):
+        e.iterate(1)
+        r = _res(e)
+        if all(r[t] <= target[t] for t in target):
+            hit = k
+            break
+    e.snapshot(False)
+    e.set_lambda_mode(MODE_PLAIN)
+    return hit
+
+
+def compare(sim, n_frames: int, seed: int = 1, plain_iters: int = 10, modes=(MODE_CARRY, MODE_PREV, MODE_NET)):
+    """Per substep of a fresh randomized trajectory: the plain 10-iteration
+    residual is the target; record the iterations each mode needs to reach it."""
+    import torch
+    e = sim.engine
+    w = sim.workload
+    rng = np.random.default_rng(seed)
+    randomize_motion(w, rng)
+    walk = ActivationWalk(tet_groups(w), rng)
+    has_fib = sim.scene.v_fiber_dir is not None
+    ne = e.stats()["n_elems"]
+    lam_prev = torch.zeros((ne, 4), dtype=torch.float32, device="cuda")
+    lam = torch.zeros_like(lam_prev)
+    out = {m: [] for m in modes}
+    for f in range(1, n_frames + 1):
+        if has_fib:
+            e.set_fiber_activation(walk.next())
+        sim.load_frame_targets(f)
+        e.apply_pintotarget()
+        for s in range(sim.nsubsteps):
+            e.set_lambda_mode(MODE_PLAIN)
+            e.integrate()
+            e.snapshot(True)
+            e.iterate(plain_iters)
+            target = _res(e)
+            e.snapshot(False)
+            for m in modes:
+                e.set_lambda(lam_prev)
+                out[m].append(iterations_to_reach(sim, m, target, kmax=plain_iters))
+            # advance the trajectory with the converged multipliers for the next substep's features
+            e.set_lambda_mode(MODE_CARRY)
+            e.warm_start()
+            e.snapshot(True)
+            e.iterate(40)
+            e.get_lambda(lam)
+            e.snapshot(False)
+            lam_prev.copy_(lam)
+            e.set_lambda_mode(MODE_PLAIN)
+            e.iterate(plain_iters)
+            e.collide()
+            e.update_velocities()
+    return out
