@@ -38,7 +38,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="humanv2", choices=sorted(WORKLOADS))
     ap.add_argument("--patch-verts", type=int, default=0, help="0 = autotune strategy and patch size")
-    ap.add_argument("--strategy", type=int, default=-1, help="-1 = autotune, 0 = patch, 1 = flat")
+    ap.add_argument("--strategy", type=int, default=-1, help="-1 = autotune, 0 = patch, 1 = flat, 2 = block")
+    ap.add_argument("--block-tets", type=int, default=1024, help="block strategy: tets per workgroup")
+    ap.add_argument("--block-threads", type=int, default=256, help="block strategy: threads per workgroup")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
@@ -108,10 +110,12 @@ def main():
     w = sims.build_workload(cfg)
     tune_table = None
     if args.patch_verts or args.strategy >= 0:
-        strat, pv = max(args.strategy, 0), (args.patch_verts or 256)
+        tuning = dict(strategy=max(args.strategy, 0), patch_verts=args.patch_verts or 256,
+                      block_tets=args.block_tets, block_threads=args.block_threads)
     else:
-        strat, pv, tune_table = autotune.pick(cfg, w)
-    sim = sims.MultiGridSim(cfg, workload=w, patch_verts=pv, strategy=strat)
+        tuning, tune_table = autotune.pick(cfg, w)
+    strat = tuning["strategy"]
+    sim = sims.MultiGridSim(cfg, workload=w, **tuning)
     st = sim.engine.stats()
     nsub, iters = sim.nsubsteps, sim.fine_iters
 
@@ -166,16 +170,17 @@ def main():
                         f"{len(w.pin_vids) if w.pin_vids is not None else 0} pinned verts, "
                         f"{len(w.bones)} bone SDFs",
             "tets": w.nt, "verts": w.nv, "substeps": nsub, "iterations": iters,
-            "strategy": "flat" if strat else "patch", "patch_verts": pv, "patches": st["n_patches"],
-            "replicated_tets": st["n_patch_tets"], "autotune_us_per_iteration": tune_table,
+            "strategy": ("patch", "flat", "block")[strat], "tuning": tuning, "patches_or_blocks": st["n_patches"],
+            "autotune_us_per_iteration": tune_table,
         },
         "tet_iterations_per_s": round(tet_it, 1),
         "env_substeps_per_s": round(n_inst * nsub / (ms_frame * 1e-3), 1),
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": ("k_flat_tets + k_flat_verts (one Jacobi iteration)" if strat
-                       else "k_iterate (fused Jacobi projection)"), "avg_launch_us": round(avg_us, 3),
+            "kernel": ("k_iterate (fused Jacobi projection)", "k_flat_tets + k_flat_verts (one Jacobi iteration)",
+                       "k_block_tets + k_block_shared (one Jacobi iteration)")[strat],
+            "avg_launch_us": round(avg_us, 3),
             "launches_timed": k_n, "alg_bytes_per_launch": alg,
             "impl_bytes_per_launch": st["bytes_per_iter_impl"],
         },

@@ -57,12 +57,15 @@ typedef struct {
 typedef struct {
     int32_t patch_verts;          /* patch strategy: owned vertices per patch = threads per block (64..256) */
     int32_t strategy;             /* 0 = patch (LDS-staged, one launch per iteration), 1 = flat
-                                     (tet pass + vertex pass, two launches per iteration)        */
-    int32_t reserved[6];
+                                     (tet pass + vertex pass, two launches per iteration), 2 = block
+                                     (tets in blocks reduced in LDS + shared-vertex pass)        */
+    int32_t block_tets;           /* block strategy: tets per block (64..8192, default 1024)     */
+    int32_t block_threads;        /* block strategy: threads per block (64..512, default 256)    */
+    int32_t reserved[4];
 } xpbd_tuning;
 
 typedef struct {
-    int32_t nv, nt, n_elems, n_patches;
+    int32_t nv, nt, n_elems, n_patches;  /* n_patches: patches, or blocks (block strategy) */
     int32_t n_patch_tets;         /* sum over patches of replicated tets          */
     int32_t max_table;            /* largest patch vertex table (owned + halo)    */
     int32_t max_patch_tets;
@@ -202,9 +205,10 @@ int32_t xpbd_mlp_backward(const xpbd_mlp_dev *net, const float *X, const float *
 /* multiplier modes (flat strategy): 0 = plain zero-lambda Jacobi (the reference GPU path,
  * solve_kernels.py:271-281 with clear() every iteration, mg.py:1348); 1 = lambda carried across
  * the iterations of a substep from 0; 2 = learned warm start: after integrate the network
- * predicts lambda0 per tet, the first iteration applies it as an impulse along the constraint
- * gradients, the rest carry lambda; 3 = the same with lambda0 = the previous substep's final
- * multipliers (no network) */
+ * predicts, per tet, the change of the multipliers since the previous substep, lambda0 =
+ * previous final multipliers + prediction; the first iteration applies lambda0 as an impulse
+ * along the constraint gradients, the rest carry lambda; 3 = the same with lambda0 = the
+ * previous substep's final multipliers (no network).  No reference counterpart. */
 int32_t xpbd_set_lambda_mode(xpbd_ctx *ctx, int32_t mode);
 /* network weights (host fp32; W1, W2 stored as bf16), feature standardisation (IN each) and
  * per-output scales (OUT) */

@@ -2,11 +2,14 @@
 
 Candidates: the "patch" strategy (LDS-staged vertex table, one launch per
 iteration) at 64/128/256 owned vertices per workgroup, which trades workgroup
-count and occupancy against tet replication across patch borders, and the
-"flat" strategy (one thread per tet, then one per vertex; two launches per
-iteration, no replication).  Each candidate layout is built for the actual
-mesh, a few iterations are timed with HIP events around the projection
-launches, and the fastest is kept.  Results are cached per mesh signature in
+count and occupancy against tet replication across patch borders; the "flat"
+strategy (one thread per tet, then one per vertex; two launches per
+iteration, no replication, every corner correction goes through memory); and
+the "block" strategy (consecutive tets per workgroup, corner sums in LDS,
+only vertices shared between blocks go through memory) at several block
+sizes and thread counts.  Each candidate layout is built for the actual mesh,
+a few iterations are timed with HIP events around the projection launches,
+and the fastest is kept.  Results are cached per mesh signature in
 ``rlmusclev2_amd/_tune/cache.json``.
 """
 from __future__ import annotations
@@ -17,8 +20,22 @@ from pathlib import Path
 
 import numpy as np
 
+from .capi import XpbdError
+
 CACHE = Path(__file__).resolve().parent / "_tune" / "cache.json"
-CANDIDATES = ((1, 256), (0, 64), (0, 128), (0, 256))   # (strategy, patch_verts)
+# (strategy, patch_verts, block_tets, block_threads)
+CANDIDATES = ((1, 256, 0, 0), (0, 64, 0, 0), (0, 128, 0, 0), (0, 256, 0, 0),
+              (2, 256, 512, 256), (2, 256, 1024, 256), (2, 256, 1024, 512), (2, 256, 2048, 512))
+
+
+def name(c) -> str:
+    strat, pv, tb, bs = c
+    return "flat" if strat == 1 else (f"patch{pv}" if strat == 0 else f"block{tb}x{bs}")
+
+
+def as_kwargs(c) -> dict:
+    strat, pv, tb, bs = c
+    return dict(strategy=strat, patch_verts=pv, block_tets=tb or 1024, block_threads=bs or 256)
 
 
 def signature(cfg: dict, w) -> str:
@@ -38,33 +55,36 @@ def _load():
 def time_candidates(cfg: dict, w, candidates=CANDIDATES, iters: int = 20) -> dict:
     from .sims import MultiGridSim
     out = {}
-    for strat, pv in candidates:
-        sim = MultiGridSim(cfg, workload=w, patch_verts=pv, strategy=strat)
+    for c in candidates:
+        try:
+            sim = MultiGridSim(cfg, workload=w, **as_kwargs(c))
+        except XpbdError:      # e.g. the layout needs more LDS than a workgroup can have
+            continue
         sim.engine.iterate(2)
         sim.engine.synchronize()
         sim.engine.kernel_timing(True)
         sim.engine.iterate(iters)
         ms, n = sim.engine.get_kernel_timing()
         sim.engine.kernel_timing(False)
-        out[(strat, pv)] = ms * 1e3 / max(n, 1)
+        out[c] = ms * 1e3 / max(n, 1)
         sim.engine.close()
     return out
 
 
 def pick(cfg: dict, w, use_cache: bool = True):
-    """(strategy, patch_verts, {candidate: us per iteration})"""
+    """(MultiGridSim tuning kwargs, {candidate name: us per iteration})"""
     key = signature(cfg, w)
     cache = _load()
-    if use_cache and key in cache:
+    if use_cache and key in cache and "tuning" in cache[key]:
         e = cache[key]
-        return int(e["strategy"]), int(e["patch_verts"]), e["us_per_iteration"]
+        return dict(e["tuning"]), e["us_per_iteration"]
     times = time_candidates(cfg, w)
     best = min(times, key=times.get)
-    table = {f"{'flat' if k[0] else 'patch'}{k[1] if not k[0] else ''}": round(v, 3) for k, v in times.items()}
-    cache[key] = {"strategy": int(best[0]), "patch_verts": int(best[1]), "us_per_iteration": table}
+    table = {name(k): round(v, 3) for k, v in times.items()}
+    cache[key] = {"tuning": as_kwargs(best), "us_per_iteration": table}
     try:
         CACHE.parent.mkdir(parents=True, exist_ok=True)
         CACHE.write_text(json.dumps(cache, indent=1))
     except OSError:
         pass
-    return int(best[0]), int(best[1]), table
+    return as_kwargs(best), table

@@ -38,7 +38,8 @@ class XpbdParams(C.Structure):
 
 
 class XpbdTuning(C.Structure):
-    _fields_ = [("patch_verts", C.c_int32), ("strategy", C.c_int32), ("reserved", C.c_int32 * 6)]
+    _fields_ = [("patch_verts", C.c_int32), ("strategy", C.c_int32), ("block_tets", C.c_int32),
+                ("block_threads", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 class XpbdDcdParams(C.Structure):

@@ -558,24 +558,13 @@ __global__ __launch_bounds__(256) void k_flat_tets(IterArgs a, const float4 *__r
     corr[so.w] = make_float4(c.d[3].x, c.d[3].y, c.d[3].z, c.cnt);
 }
 
+// finish one vertex: attach, then either accumulate (MODE_ACCUM) or apply
+// dP/dPw (apply_dP_kernel, engine.py:689-698) and the pin overwrite
+// (pintotarget_overwrite_masked_kernel, mg/kernel.py:212-226)
 template <int DAMP, int MODE>
-__global__ __launch_bounds__(256) void k_flat_verts(IterArgs a, const float4 *__restrict__ xin,
-                                                    float4 *__restrict__ xout,
-                                                    const float4 *__restrict__ pprev,
-                                                    const float4 *__restrict__ corr, int nv) {
-    const int v = blockIdx.x * 256 + threadIdx.x;
-    if (v >= nv) return;
-    float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
-    const int e = a.vslot_off[v + 1];
-    for (int s = a.vslot_off[v]; s < e; ++s) {
-        const float4 q = corr[s];
-        ax += q.x;
-        ay += q.y;
-        az += q.z;
-        aw += q.w;
-    }
-    float4 x = xin[v];
-    if (a.att0) attach_vertex(a, v, x, DAMP ? pprev[v] : make_float4(0, 0, 0, 0), ax, ay, az, aw);
+__device__ __forceinline__ void finish_vertex(const IterArgs &a, int v, float4 x, float4 pp, float ax, float ay,
+                                              float az, float aw, float4 *__restrict__ xout) {
+    if (a.att0) attach_vertex(a, v, x, pp, ax, ay, az, aw);
     if (MODE == MODE_ACCUM) {
         float4 o = a.dpw[v];
         o.x += ax;
@@ -597,10 +586,139 @@ __global__ __launch_bounds__(256) void k_flat_verts(IterArgs a, const float4 *__
     xout[v] = x;
 }
 
+template <int DAMP, int MODE>
+__global__ __launch_bounds__(256) void k_flat_verts(IterArgs a, const float4 *__restrict__ xin,
+                                                    float4 *__restrict__ xout,
+                                                    const float4 *__restrict__ pprev,
+                                                    const float4 *__restrict__ corr, int nv) {
+    const int v = blockIdx.x * 256 + threadIdx.x;
+    if (v >= nv) return;
+    float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
+    const int e = a.vslot_off[v + 1];
+    for (int s = a.vslot_off[v]; s < e; ++s) {
+        const float4 q = corr[s];
+        ax += q.x;
+        ay += q.y;
+        az += q.z;
+        aw += q.w;
+    }
+    finish_vertex<DAMP, MODE>(a, v, xin[v], DAMP ? pprev[v] : make_float4(0, 0, 0, 0), ax, ay, az, aw, xout);
+}
+
+// ---------------------------------------------------------------------------
+// "block" strategy: one workgroup per block of consecutive elements.  The
+// block's vertex table is staged in LDS, every element is projected once,
+// the four corner corrections go to LDS, and each local vertex sums its
+// corners in ascending slot order.  Vertices owned by this block alone are
+// finished here; the others get this block's partial sum, finished by
+// k_block_shared, which adds a vertex's partials in ascending block order.
+// The sum order is fixed by the layout, so results are run-to-run identical.
+// ---------------------------------------------------------------------------
+struct BlockArgs {
+    const BlockDesc *blk;
+    const int32_t *bvt, *bps;
+    const uint32_t *brun;
+    const ushort4 *blidx;
+    const uint16_t *bslot;
+    const int32_t *svid, *soff;
+    float4 *part;
+    int max_nloc;
+};
+
+// LDS: vertex table (and previous positions when damping), per-entry running
+// sums and slot cursors, and the corner corrections of one chunk of blockDim
+// elements.  Chunks are projected in order; after each chunk every local entry
+// consumes its slots that fall in the chunk, so slots are added in ascending
+// order whatever the chunk size.
+template <int FEAT, int DAMP, int LAM, int MODE>
+__global__ __launch_bounds__(512) void k_block_tets(IterArgs a, BlockArgs b, const float4 *__restrict__ xin,
+                                                    float4 *__restrict__ xout,
+                                                    const float4 *__restrict__ pprev) {
+    extern __shared__ float4 smem[];
+    const BlockDesc D = b.blk[blockIdx.x];
+    const int M = b.max_nloc;
+    float4 *sx = smem;
+    float4 *spp = smem + M;
+    float4 *sacc = smem + (DAMP ? 2 : 1) * M;
+    int2 *scur = (int2 *)(sacc + M);
+    float4 *sc = (float4 *)(scur + ((M + 1) & ~1));
+    const int tid = threadIdx.x, nth = blockDim.x;
+    for (int i = tid; i < D.nloc; i += nth) {
+        const int gv = b.bvt[D.vbeg + i];
+        sx[i] = xin[gv];
+        if (DAMP) spp[i] = pprev[gv];
+        sacc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        const uint32_t run = b.brun[D.vbeg + i];
+        scur[i] = make_int2((int)(run & 0xffffu), (int)(run & 0xffffu) + (int)(run >> 16));
+    }
+    __syncthreads();
+    const uint16_t *slots = b.bslot + 4 * (int64_t)D.tbeg;
+    for (int base = 0; base < D.ntet; base += nth) {
+        const int t = base + tid;
+        if (t < D.ntet) {
+            const int64_t g = (int64_t)D.tbeg + t;
+            const ushort4 li = b.blidx[g];
+            const int lv[4] = {li.x, li.y, li.z, li.w};
+            Corr c;
+            project_elem<FEAT, DAMP, LAM>(a, g, sx, spp, lv, c);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sc[4 * tid + k] = make_float4(c.d[k].x, c.d[k].y, c.d[k].z, c.cnt);
+        }
+        __syncthreads();
+        const int lo = 4 * base, lim = 4 * (base + nth);
+        for (int i = tid; i < D.nloc; i += nth) {
+            int2 cr = scur[i];
+            float4 acc = sacc[i];
+            while (cr.x < cr.y) {
+                const int s = slots[cr.x];
+                if (s >= lim) break;
+                const float4 q = sc[s - lo];
+                acc.x += q.x;
+                acc.y += q.y;
+                acc.z += q.z;
+                acc.w += q.w;
+                ++cr.x;
+            }
+            scur[i] = cr;
+            sacc[i] = acc;
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < D.nloc; i += nth) {
+        const float4 acc = sacc[i];
+        const int ps = b.bps[D.vbeg + i];
+        if (ps >= 0) {
+            b.part[ps] = acc;
+        } else {
+            finish_vertex<DAMP, MODE>(a, b.bvt[D.vbeg + i], sx[i], DAMP ? spp[i] : make_float4(0, 0, 0, 0), acc.x,
+                                      acc.y, acc.z, acc.w, xout);
+        }
+    }
+}
+
+template <int DAMP, int MODE>
+__global__ __launch_bounds__(256) void k_block_shared(IterArgs a, BlockArgs b, const float4 *__restrict__ xin,
+                                                      float4 *__restrict__ xout,
+                                                      const float4 *__restrict__ pprev, int ns) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= ns) return;
+    const int v = b.svid[i];
+    float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
+    const int e = b.soff[i + 1];
+    for (int s = b.soff[i]; s < e; ++s) {
+        const float4 q = b.part[s];
+        ax += q.x;
+        ay += q.y;
+        az += q.z;
+        aw += q.w;
+    }
+    finish_vertex<DAMP, MODE>(a, v, xin[v], DAMP ? pprev[v] : make_float4(0, 0, 0, 0), ax, ay, az, aw, xout);
+}
+
 
 // per-element input features of the warm-start network (22 used of 24):
 // F - I (9), fibre direction (3), fibre target length (1), previous-substep
-// multipliers divided by the output scales (3), log10 rest volume, det F - 1,
+// multipliers (3), log10 rest volume, det F - 1,
 // |F|^2 - 3, log10 mean inverse mass, log10 ARAP and volume compliances.
 __global__ void k_features(IterArgs a, const float4 *__restrict__ X, float *__restrict__ out,
                            const float *__restrict__ fmean, const float *__restrict__ finv,
@@ -623,7 +741,7 @@ __global__ void k_features(IterArgs a, const float4 *__restrict__ X, float *__re
     const float4 fb = (mask & M_ANISO) ? a.fib[g] : make_float4(0.f, 0.f, 0.f, 1.f);
     f[9] = fb.x; f[10] = fb.y; f[11] = fb.z; f[12] = fb.w;
     const float4 lp = a.lam ? a.lam[g] : make_float4(0.f, 0.f, 0.f, 0.f);
-    f[13] = lp.x / oscale[0]; f[14] = lp.y / oscale[1]; f[15] = lp.z / oscale[2];
+    f[13] = lp.x; f[14] = lp.y; f[15] = lp.z;
     const float detB = det3(B);
     f[16] = log10f(1.0f / (6.0f * fabsf(detB) + 1e-30f) + 1e-30f);
     float n2 = 0.f;
@@ -639,11 +757,14 @@ __global__ void k_features(IterArgs a, const float4 *__restrict__ X, float *__re
     for (int i = 0; i < 24; ++i) o[i] = normalize ? (f[i] - fmean[i]) * finv[i] : f[i];
 }
 
-// lambda0 = network output * per-type output scale
+// lambda0 = previous-substep multipliers + network output * per-type output
+// scale (the network predicts the change of the multipliers over one substep)
 __global__ void k_lam0(int64_t n, const float *__restrict__ Y, const float *__restrict__ oscale, float4 *lam) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
-    lam[g] = make_float4(Y[4 * g] * oscale[0], Y[4 * g + 1] * oscale[1], Y[4 * g + 2] * oscale[2], 0.f);
+    const float4 p = lam[g];
+    lam[g] = make_float4(p.x + Y[4 * g] * oscale[0], p.y + Y[4 * g + 1] * oscale[1], p.z + Y[4 * g + 2] * oscale[2],
+                         0.f);
 }
 
 // ---------------------------------------------------------------------------
@@ -1373,6 +1494,8 @@ struct xpbd_ctx {
     bool ready = false;
     int P = 256;
     bool flat = false;
+    int block = 0;          // block strategy: tets per block (0 = off)
+    int block_bs = 256;     // block strategy: threads per block
     int feature = 0;
     // device buffers
     std::vector<DBuf> bufs;
@@ -1387,6 +1510,13 @@ struct xpbd_ctx {
     int4 *d_gidx = nullptr;
     int32_t *d_vslot32 = nullptr;
     float4 *CORR = nullptr;
+    // block strategy tables
+    BlockDesc *d_blk = nullptr;
+    int32_t *d_bvt = nullptr, *d_bps = nullptr, *d_svid = nullptr, *d_soff = nullptr;
+    uint32_t *d_brun = nullptr;
+    ushort4 *d_blidx = nullptr;
+    uint16_t *d_bslot = nullptr;
+    float4 *BPART = nullptr;
     float *d_Bp = nullptr, *d_rvol = nullptr, *d_rsnh = nullptr, *d_act = nullptr;
     DevClass *d_cls = nullptr;
     int4 *d_tets = nullptr;
@@ -1416,10 +1546,17 @@ struct xpbd_ctx {
     DcdDev dcd{};
     uint32_t *d_scan_bsum = nullptr;
     int n_scan_bsum = 0;
-    // graph
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    int g_nsub = -1, g_iters = -1;
+    // captured frames, keyed by (substeps, iterations, lambda mode); a small
+    // cache so that plain and warm-started frames can alternate
+    struct GraphSlot {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t gexec = nullptr;
+        int nsub = -1, iters = -1, lam_mode = -1;
+        uint64_t used = 0;
+    };
+    static constexpr int NGRAPH = 4;
+    GraphSlot gslot[NGRAPH];
+    uint64_t gclock = 0;
     // timing
     bool timing = false;
     std::vector<EvPair> evs;
@@ -1460,19 +1597,22 @@ int dupload(xpbd_ctx *c, T **p, const T *src, size_t count) {
     return XPBD_OK;
 }
 
+void drop_graph_slot(xpbd_ctx::GraphSlot &g) {
+    if (g.gexec) { (void)hipGraphExecDestroy(g.gexec); g.gexec = nullptr; }
+    if (g.graph) { (void)hipGraphDestroy(g.graph); g.graph = nullptr; }
+    g.nsub = g.iters = g.lam_mode = -1;
+    g.used = 0;
+}
+
 void free_device(xpbd_ctx *c) {
-    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
-    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
-    c->g_nsub = c->g_iters = -1;
+    for (auto &g : c->gslot) drop_graph_slot(g);
     for (auto &b : c->bufs) (void)hipFree(b.p);
     c->bufs.clear();
     c->ready = false;
 }
 
 void invalidate_graph(xpbd_ctx *c) {
-    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
-    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
-    c->g_nsub = c->g_iters = -1;
+    for (auto &g : c->gslot) drop_graph_slot(g);
 }
 
 inline float inv_mass_of(const xpbd_ctx *c, int old) {
@@ -1634,6 +1774,60 @@ FlatVertKernel pick_flat_verts(int feat, int mode) {
     return damp ? k_flat_verts<1, MODE_FUSED> : k_flat_verts<0, MODE_FUSED>;
 }
 
+typedef void (*BlockTetKernel)(IterArgs, BlockArgs, const float4 *, float4 *, const float4 *);
+typedef void (*BlockVertKernel)(IterArgs, BlockArgs, const float4 *, float4 *, const float4 *, int);
+
+template <int FEAT, int DAMP>
+BlockTetKernel pick_block_lam(int lam) {
+    return lam == 2 ? k_block_tets<FEAT, DAMP, 2, MODE_FUSED>
+                    : (lam == 1 ? k_block_tets<FEAT, DAMP, 1, MODE_FUSED> : k_block_tets<FEAT, DAMP, 0, MODE_FUSED>);
+}
+
+template <int FEAT>
+BlockTetKernel pick_block_damp(int damp, int lam) {
+    return damp ? pick_block_lam<FEAT, 1>(lam) : pick_block_lam<FEAT, 0>(lam);
+}
+
+BlockTetKernel pick_block_tets(int feat, int lam, int mode) {
+    constexpr int AV = F_ARAP | F_VOL, AVA = F_ARAP | F_VOL | F_ANISO, SA = F_SNH | F_ANISO,
+                  ALL = F_ARAP | F_VOL | F_SNH | F_ANISO;
+    const int f = feat & ALL;
+    const int damp = (feat & F_DAMP) ? 1 : 0;
+    if (mode == MODE_ACCUM) return damp ? k_block_tets<ALL, 1, 0, MODE_ACCUM> : k_block_tets<ALL, 0, 0, MODE_ACCUM>;
+    if ((f & ~AV) == 0) return pick_block_damp<AV>(damp, lam);
+    if ((f & ~SA) == 0) return pick_block_damp<SA>(damp, lam);
+    if ((f & ~AVA) == 0) return pick_block_damp<AVA>(damp, lam);
+    return pick_block_damp<ALL>(damp, lam);
+}
+
+BlockVertKernel pick_block_shared(int feat, int mode) {
+    const int damp = (feat & F_DAMP) ? 1 : 0;
+    if (mode == MODE_ACCUM) return damp ? k_block_shared<1, MODE_ACCUM> : k_block_shared<0, MODE_ACCUM>;
+    return damp ? k_block_shared<1, MODE_FUSED> : k_block_shared<0, MODE_FUSED>;
+}
+
+size_t block_lds(const xpbd_ctx *c) {
+    const size_t M = (size_t)c->L.max_nloc;
+    const size_t damp = (c->feature & F_DAMP) ? 2 : 1;
+    return (damp + 1) * M * sizeof(float4) + ((M + 1) & ~(size_t)1) * sizeof(int2) +
+           4 * (size_t)c->block_bs * sizeof(float4);
+}
+
+BlockArgs make_block_args(xpbd_ctx *c) {
+    BlockArgs b;
+    b.blk = c->d_blk;
+    b.bvt = c->d_bvt;
+    b.bps = c->d_bps;
+    b.brun = c->d_brun;
+    b.blidx = c->d_blidx;
+    b.bslot = c->d_bslot;
+    b.svid = c->d_svid;
+    b.soff = c->d_soff;
+    b.part = c->BPART;
+    b.max_nloc = c->L.max_nloc;
+    return b;
+}
+
 size_t iter_lds(const xpbd_ctx *c) {
     const int damp = (c->feature & F_DAMP) ? 2 : 1;
     return ((size_t)damp * c->L.max_table + (size_t)c->P * 4) * sizeof(float4);
@@ -1649,7 +1843,20 @@ int launch_iter(xpbd_ctx *c, const float4 *xin, float4 *xout, int mode, int lam 
         HIPCHK(c, hipEventRecord(ev.a, c->stream));
     }
     const float4 *pp = c->PPv;
-    if (c->flat) {
+    if (c->block) {
+        BlockTetKernel kt = pick_block_tets(c->feature, lam, mode);
+        BlockVertKernel kv = pick_block_shared(c->feature, mode);
+        BlockArgs b = make_block_args(c);
+        const unsigned nb = (unsigned)c->L.blocks.size();
+        void *at[] = {(void *)&a, (void *)&b, (void *)&xin, (void *)&xout, (void *)&pp};
+        if (nb > 0)
+            HIPCHK(c, hipLaunchKernel((const void *)kt, dim3(nb), dim3(c->block_bs), at, block_lds(c), c->stream));
+        int ns = (int)c->L.svid.size();
+        void *av[] = {(void *)&a, (void *)&b, (void *)&xin, (void *)&xout, (void *)&pp, (void *)&ns};
+        if (ns > 0)
+            HIPCHK(c, hipLaunchKernel((const void *)kv, dim3((unsigned)((ns + 255) / 256)), dim3(256), av, 0,
+                                      c->stream));
+    } else if (c->flat) {
         FlatTetKernel kt = pick_flat_tets(c->feature, lam);
         FlatVertKernel kv = pick_flat_verts(c->feature, mode);
         const int64_t ne = (int64_t)c->L.meta.size();
@@ -2171,8 +2378,17 @@ int32_t xpbd_finalize(xpbd_ctx *c, const xpbd_tuning *t) {
     if (P != 64 && P != 128 && P != 256) return fail(c, XPBD_EINVAL, "patch_verts must be 64, 128 or 256");
     c->P = P;
     const int strategy = t ? t->strategy : 0;
-    if (strategy != 0 && strategy != 1) return fail(c, XPBD_EINVAL, "strategy must be 0 (patch) or 1 (flat)");
-    c->flat = strategy == 1;
+    if (strategy < 0 || strategy > 2)
+        return fail(c, XPBD_EINVAL, "strategy must be 0 (patch), 1 (flat) or 2 (block)");
+    c->flat = strategy >= 1;
+    c->block = 0;
+    if (strategy == 2) {
+        c->block = (t->block_tets > 0) ? t->block_tets : 1024;
+        c->block_bs = (t->block_threads > 0) ? t->block_threads : 256;
+        if (c->block < 64 || c->block > 8192) return fail(c, XPBD_EINVAL, "block_tets must be in [64, 8192]");
+        if (c->block_bs < 64 || c->block_bs > 512 || (c->block_bs & 63))
+            return fail(c, XPBD_EINVAL, "block_threads must be a multiple of 64 in [64, 512]");
+    }
     xpbd::HostInput in;
     in.nv = c->nv;
     in.nt = c->nt;
@@ -2193,18 +2409,27 @@ int32_t xpbd_finalize(xpbd_ctx *c, const xpbd_tuning *t) {
     in.restdir = c->c_rdir.data();
     in.patch_verts = P;
     in.flat = c->flat;
+    in.block_tets = c->block;
     c->L = xpbd::Layout();
     int code = 0;
     std::string m = xpbd::build_layout(in, c->L, &code);
     if (code) return fail(c, code, m);
     c->feature = c->L.feature;
-    const size_t lds = c->flat ? 0 : iter_lds(c);
+    const size_t lds = c->block ? block_lds(c) : (c->flat ? 0 : iter_lds(c));
     int maxlds = 0;
     HIPCHK(c, hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
     if (lds > (size_t)maxlds)
-        return fail(c, XPBD_ELAYOUT, "patch vertex table needs " + std::to_string(lds) + " B of LDS; limit " +
-                                         std::to_string(maxlds));
-    if (lds > 65536) {
+        return fail(c, XPBD_ELAYOUT, std::string(c->block ? "block" : "patch") + " tables need " +
+                                         std::to_string(lds) + " B of LDS; limit " + std::to_string(maxlds));
+    if (c->block) {
+        const int modes[2] = {MODE_FUSED, MODE_ACCUM};
+        for (int mode : modes)
+            for (int lam = 0; lam < 3; ++lam) {
+                BlockTetKernel k = pick_block_tets(c->feature, lam, mode);
+                HIPCHK(c, hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            }
+    }
+    if (!c->flat && lds > 65536) {
         IterKernel k = pick_iter(c->P, c->feature, MODE_FUSED);
         HIPCHK(c, hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         IterKernel k2 = pick_iter(c->P, c->feature, MODE_ACCUM);
@@ -2229,7 +2454,19 @@ int32_t xpbd_finalize(xpbd_ctx *c, const xpbd_tuning *t) {
     if (c->flat) {
         r = dupload(c, &c->d_gidx, (const int4 *)L.gidx.data(), nptet); if (r) return r;
         UP(c->d_vslot32, L.vslot32);
-        if ((r = dalloc(c, &c->CORR, 4 * std::max<size_t>(nptet, 1)))) return r;
+        if (c->block) {
+            r = dupload(c, &c->d_blk, L.blocks.data(), L.blocks.size()); if (r) return r;
+            UP(c->d_bvt, L.bvt);
+            UP(c->d_bps, L.bps);
+            UP(c->d_brun, L.brun);
+            r = dupload(c, &c->d_blidx, (const ushort4 *)L.blidx.data(), nptet); if (r) return r;
+            UP(c->d_bslot, L.bslot);
+            UP(c->d_svid, L.svid);
+            UP(c->d_soff, L.soff);
+            if ((r = dalloc(c, &c->BPART, std::max<size_t>((size_t)L.n_partial, 1)))) return r;
+        } else {
+            if ((r = dalloc(c, &c->CORR, 4 * std::max<size_t>(nptet, 1)))) return r;
+        }
         if ((r = dalloc(c, &c->LAMB, std::max<size_t>(nptet, 1)))) return r;
         HIPCHK(c, hipMemset(c->LAMB, 0, std::max<size_t>(nptet, 1) * sizeof(float4)));
         if ((r = dalloc(c, &c->FEAT, 24 * std::max<size_t>(nptet, 1)))) return r;
@@ -2333,9 +2570,9 @@ int32_t xpbd_get_stats(xpbd_ctx *c, xpbd_stats *o) {
     o->nv = c->nv;
     o->nt = c->nt;
     o->n_elems = c->L.n_elems;
-    o->n_patches = (int)c->L.patches.size();
+    o->n_patches = (int)(c->block ? c->L.blocks.size() : c->L.patches.size());
     o->n_patch_tets = (int)c->L.meta.size();
-    o->max_table = c->L.max_table;
+    o->max_table = c->block ? c->L.max_nloc : c->L.max_table;
     o->max_patch_tets = c->L.max_ptets;
     o->feature_mask = c->feature;
     o->bytes_per_iter_alg = c->L.bytes_alg;
@@ -2460,20 +2697,28 @@ int32_t xpbd_step_frame(xpbd_ctx *c, int32_t nsub, int32_t iters, int32_t use_gr
             if ((r = enqueue_substep(c, iters, s == 0))) return r;
         return XPBD_OK;
     }
-    if (!c->gexec || c->g_nsub != nsub || c->g_iters != iters) {
-        invalidate_graph(c);
+    xpbd_ctx::GraphSlot *hit = nullptr, *victim = &c->gslot[0];
+    for (auto &g : c->gslot) {
+        if (g.gexec && g.nsub == nsub && g.iters == iters && g.lam_mode == c->lam_mode) hit = &g;
+        if (g.used < victim->used) victim = &g;
+    }
+    if (!hit) {
+        drop_graph_slot(*victim);
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         for (int s = 0; s < nsub && r == XPBD_OK; ++s) r = enqueue_substep(c, iters, s == 0);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (r) { if (g) (void)hipGraphDestroy(g); return r; }
         if (e != hipSuccess) return fail(c, XPBD_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-        c->graph = g;
-        HIPCHK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
-        c->g_nsub = nsub;
-        c->g_iters = iters;
+        victim->graph = g;
+        HIPCHK(c, hipGraphInstantiate(&victim->gexec, g, nullptr, nullptr, 0));
+        victim->nsub = nsub;
+        victim->iters = iters;
+        victim->lam_mode = c->lam_mode;
+        hit = victim;
     }
-    HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    hit->used = ++c->gclock;
+    HIPCHK(c, hipGraphLaunch(hit->gexec, c->stream));
     return XPBD_OK;
 }
 
@@ -2633,7 +2878,6 @@ int32_t xpbd_set_lambda_mode(xpbd_ctx *c, int32_t mode) {
     if (mode && (c->feature & F_ARAP) && (c->feature & F_SNH))
         return fail(c, XPBD_EUNSUPPORTED, "lambda modes need ARAP-SVD and SNH on separate meshes");
     if (mode == 2 && !c->mlp_ready) return fail(c, XPBD_EINVAL, "lambda mode 2 needs xpbd_set_mlp()");
-    if (mode != c->lam_mode) invalidate_graph(c);
     c->lam_mode = mode;
     c->impulse_pending = false;
     return XPBD_OK;

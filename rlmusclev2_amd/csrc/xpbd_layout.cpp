@@ -178,6 +178,95 @@ std::string build_flat(const HostInput &in, Layout &L, const std::vector<Elem> &
     return std::string();
 }
 
+std::string build_blocks(const HostInput &in, Layout &L, int feature, int *code) {
+    const int nv = in.nv;
+    const int TB = in.block_tets;
+    if (TB < 64 || TB > 8192) { *code = 1; return "block_tets must be in [64, 8192]"; }
+    const int64_t ne = (int64_t)L.meta.size();
+    const int64_t nb = (ne + TB - 1) / TB;
+    L.block_tets = TB;
+    L.blocks.assign((size_t)nb, BlockDesc{});
+    L.bvt.clear();
+    L.bps.clear();
+    L.brun.clear();
+    L.blidx.assign(4 * (size_t)ne, 0);
+    L.bslot.assign(4 * (size_t)ne, 0);
+    std::vector<int32_t> nblk(nv, 0);
+    std::vector<int32_t> loc(nv, -1);
+    std::vector<int32_t> verts;
+    int max_nloc = 0;
+    // pass 1: local tables, local corner ids, per-vertex block counts
+    for (int64_t b = 0; b < nb; ++b) {
+        const int64_t t0 = b * TB, t1 = std::min<int64_t>(ne, t0 + TB);
+        verts.clear();
+        for (int64_t g = t0; g < t1; ++g)
+            for (int k = 0; k < 4; ++k) verts.push_back(L.gidx[4 * g + k]);
+        std::sort(verts.begin(), verts.end());
+        verts.erase(std::unique(verts.begin(), verts.end()), verts.end());
+        if (verts.size() > 65535) { *code = 1; return "block vertex table exceeds 65535 entries"; }
+        BlockDesc &d = L.blocks[(size_t)b];
+        d.tbeg = (int32_t)t0;
+        d.ntet = (int32_t)(t1 - t0);
+        d.vbeg = (int32_t)L.bvt.size();
+        d.nloc = (int32_t)verts.size();
+        max_nloc = std::max(max_nloc, d.nloc);
+        for (size_t i = 0; i < verts.size(); ++i) {
+            loc[verts[i]] = (int32_t)i;
+            nblk[verts[i]]++;
+            L.bvt.push_back(verts[i]);
+        }
+        // corner slots of each local entry in ascending slot order (counting sort)
+        std::vector<uint32_t> cnt(verts.size() + 1, 0);
+        for (int64_t g = t0; g < t1; ++g)
+            for (int k = 0; k < 4; ++k) {
+                const int l = loc[L.gidx[4 * g + k]];
+                L.blidx[4 * g + k] = (uint16_t)l;
+                cnt[l + 1]++;
+            }
+        for (size_t i = 0; i < verts.size(); ++i) cnt[i + 1] += cnt[i];
+        for (size_t i = 0; i < verts.size(); ++i) L.brun.push_back(cnt[i] | ((cnt[i + 1] - cnt[i]) << 16));
+        std::vector<uint32_t> fill(cnt.begin(), cnt.end() - 1);
+        for (int64_t g = t0; g < t1; ++g)
+            for (int k = 0; k < 4; ++k) {
+                const int l = L.blidx[4 * g + k];
+                L.bslot[4 * t0 + fill[l]++] = (uint16_t)((g - t0) * 4 + k);
+            }
+        for (int v : verts) loc[v] = -1;
+    }
+    L.max_nloc = max_nloc;
+    // pass 2: vertices finished by the second pass (shared by several blocks or
+    // touched by none), their partial ranges in vertex-major, block-ascending order
+    L.svid.clear();
+    L.soff.assign(1, 0);
+    std::vector<int32_t> pbase(nv, -1);
+    for (int v = 0; v < nv; ++v)
+        if (nblk[v] != 1) {
+            pbase[v] = L.soff.back();
+            L.svid.push_back(v);
+            L.soff.push_back(L.soff.back() + nblk[v]);
+        }
+    L.n_partial = L.soff.back();
+    L.bps.assign(L.bvt.size(), -1);
+    std::vector<int32_t> used(nv, 0);
+    for (int64_t b = 0; b < nb; ++b) {
+        const BlockDesc &d = L.blocks[(size_t)b];
+        for (int i = 0; i < d.nloc; ++i) {
+            const int v = L.bvt[d.vbeg + i];
+            if (nblk[v] > 1) L.bps[d.vbeg + i] = pbase[v] + used[v]++;
+        }
+    }
+    const double damp = (feature & F_DAMP) ? 2.0 : 1.0;
+    // elements: static data + local corner ids + corner slots; local entries:
+    // vertex id, partial slot, run, position gathers; partials written and read
+    // back; per-vertex output
+    L.bytes_impl = (elem_static_bytes(feature) + 8.0 + 8.0) * (double)ne +
+                   (12.0 + 16.0 * damp) * (double)L.bvt.size() + 32.0 * L.n_partial +
+                   (8.0 + 16.0 * damp) * (double)L.svid.size() + 16.0 * nv;
+    if (feature & F_PINS) L.bytes_impl += 16.0 * nv;
+    if (feature & F_ATTACH) L.bytes_impl += 32.0 * nv;
+    return std::string();
+}
+
 std::string build_layout(const HostInput &in, Layout &L, int *code) {
     *code = 0;
     const int nv = in.nv;
@@ -380,7 +469,9 @@ std::string build_layout(const HostInput &in, Layout &L, int *code) {
 
     if (in.flat) {
         renumber_vertex_records(L, attach_old, nv);
-        return build_flat(in, L, elems, ecls, eord, feature, code);
+        std::string err = build_flat(in, L, elems, ecls, eord, feature, code);
+        if (err.empty() && in.block_tets > 0) err = build_blocks(in, L, feature, code);
+        return err;
     }
 
     // ---- 5. (patch, element) entries ----------------------------------------

@@ -46,6 +46,18 @@ struct HostInput {
                 *restlength = nullptr, *restvector = nullptr, *restdir = nullptr;
     int patch_verts = 256;
     bool flat = false;   // one copy per element + per-vertex corner CSR (no patches)
+    int block_tets = 0;  // flat elements in blocks of this many, reduced in LDS (0 = off)
+};
+
+// "block" strategy: consecutive runs of elements (Morton order) form blocks.
+// A block stages the vertices its elements touch in LDS, projects its elements
+// once, and sums each local vertex's corner corrections in ascending slot
+// order.  A vertex touched by one block only is finished by that block; a
+// vertex touched by several blocks gets one partial sum per block, written to
+// a vertex-major partial array and finished by a second, per-vertex pass that
+// adds its partials in ascending block order.
+struct BlockDesc {
+    int32_t tbeg, ntet, vbeg, nloc;
 };
 
 struct Layout {
@@ -65,6 +77,16 @@ struct Layout {
     std::vector<uint16_t> vslot;             // slot = local tet*4 + corner
     std::vector<int32_t> gidx;               // flat: 4 new vertex ids per element
     std::vector<int32_t> vslot32;            // flat: vertex-major position of corner (element, k)
+    // block strategy
+    std::vector<BlockDesc> blocks;
+    std::vector<int32_t> bvt;                // per local entry: vertex id (new order)
+    std::vector<int32_t> bps;                // per local entry: partial slot, -1 = finished in the block
+    std::vector<uint32_t> brun;              // per local entry: first slot | slot count << 16
+    std::vector<uint16_t> blidx;             // 4 per element: local entry of each corner
+    std::vector<uint16_t> bslot;             // 4 per element, per block: corner slots grouped by local entry
+    std::vector<int32_t> svid;               // vertices finished by the second pass
+    std::vector<int32_t> soff;               // svid.size()+1 offsets into the partial array
+    int32_t max_nloc = 0, block_tets = 0, n_partial = 0;
     std::vector<ClassParams> classes;
     // attach records in new vertex order
     std::vector<float> att0, att1;           // 4 per vertex

@@ -26,7 +26,8 @@ from .scene import Scene
 class XpbdEngine:
     """Device state + phase primitives for one mesh (or one packed batch)."""
 
-    def __init__(self, scene: Scene, device: int = 0, patch_verts: int = 256, strategy: int = 0):
+    def __init__(self, scene: Scene, device: int = 0, patch_verts: int = 256, strategy: int = 0,
+                 block_tets: int = 1024, block_threads: int = 256):
         self.scene = scene
         self.ctx = capi.Context(device)
         self.n_verts = scene.nv
@@ -39,6 +40,8 @@ class XpbdEngine:
         self.constraint_solvers = []       # MuscleSim extras
         self._keep = {}
         self.strategy = int(strategy)
+        self.block_tets = int(block_tets)
+        self.block_threads = int(block_threads)
         self._upload(scene, patch_verts)
 
     # -- setup -----------------------------------------------------------
@@ -86,6 +89,8 @@ class XpbdEngine:
         tun = capi.XpbdTuning()
         tun.patch_verts = int(patch_verts)
         tun.strategy = self.strategy
+        tun.block_tets = self.block_tets
+        tun.block_threads = self.block_threads
         c.call("xpbd_finalize", C.byref(tun))
         if s.pin_vids is not None:
             tgt = f32(s.pos0, (-1, 3))[k["pin_vids"]]
@@ -277,15 +282,28 @@ class XpbdEngine:
     def warm_start(self):
         self.ctx.call("xpbd_warm_start")
 
+    # Device tensors exchanged with torch: torch's stream is drained before the
+    # library reads one, and the library's stream before torch reads one.
+    def _torch_sync(self):
+        import torch
+        torch.cuda.current_stream().synchronize()
+
     def features(self, out, normalized: bool = False):
         """per-element features into a (n_elems, 24) float32 CUDA tensor"""
+        self._torch_sync()
         self.ctx.call("xpbd_features", C.c_void_p(out.data_ptr()), int(bool(normalized)))
+        self.synchronize()
 
     def get_lambda(self, out):
+        """per-element multipliers into a (n_elems, 4) float32 CUDA tensor"""
+        self._torch_sync()
         self.ctx.call("xpbd_get_lambda", C.c_void_p(out.data_ptr()))
+        self.synchronize()
 
     def set_lambda(self, t):
+        self._torch_sync()
         self.ctx.call("xpbd_set_lambda", C.c_void_p(t.data_ptr()))
+        self.synchronize()
 
     def snapshot(self, save: bool):
         self.ctx.call("xpbd_snapshot", int(bool(save)))

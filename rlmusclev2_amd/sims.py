@@ -121,7 +121,8 @@ class MultiGridSim:
     """Pin-to-target muscle sim on the HIP substep (mg.py:85-1434, single level)."""
 
     def __init__(self, cfg: dict, workload: workloads.Workload | None = None, device: int = 0,
-                 patch_verts: int = 256, engine_cls=XpbdEngine, strategy: int = 0):
+                 patch_verts: int = 256, engine_cls=XpbdEngine, strategy: int = 0, block_tets: int = 1024,
+                 block_threads: int = 256):
         if cfg.get("use_multigrid", False):
             raise NotImplementedError("use_multigrid: the two-level V-cycle is not on the HIP path yet")
         if not cfg.get("use_jacobi", True):
@@ -133,7 +134,8 @@ class MultiGridSim:
         self.fine_iters = int(cfg["fine_iterations"])
         self.dt = self.scene.params.dt
         self.start_frame = int(cfg.get("start_frame", 1))
-        self.engine = engine_cls(self.scene, device=device, patch_verts=patch_verts, strategy=strategy) \
+        self.engine = engine_cls(self.scene, device=device, patch_verts=patch_verts, strategy=strategy,
+                                 block_tets=block_tets, block_threads=block_threads) \
             if engine_cls is XpbdEngine else engine_cls(self.scene)
         self.NV = self.scene.nv
         self.NT = self.scene.nt

@@ -74,8 +74,13 @@ class Dataset:
 
 def collect(sim, n_frames: int, seed: int = 0, samples_per_substep: int = 8192, conv_iters: int = 40,
             substep_stride: int = 2) -> Dataset:
-    """Plain-solver trajectory with randomized activations and bone motion; at
-    every ``substep_stride``-th substep record (features, converged multipliers)."""
+    """Plain-solver trajectory with randomized activations and bone motion.
+
+    At every substep, after integrate, a lambda-carrying solve of
+    ``conv_iters`` iterations from lambda = 0 gives the converged multipliers
+    (the state is restored afterwards, so the trajectory stays plain).  Every
+    ``substep_stride``-th substep records (features with the previous
+    substep's converged multipliers, change of the converged multipliers)."""
     import torch
     e = sim.engine
     w = sim.workload
@@ -83,9 +88,12 @@ def collect(sim, n_frames: int, seed: int = 0, samples_per_substep: int = 8192, 
     randomize_motion(w, rng)
     walk = ActivationWalk(tet_groups(w), rng)
     ne = e.stats()["n_elems"]
+    p0 = sim.scene.pos0
+    e.set_state(p0, p0, np.zeros_like(p0))
     feat = torch.empty((ne, NIN), dtype=torch.float32, device="cuda")
     lam = torch.zeros((ne, 4), dtype=torch.float32, device="cuda")
     lam_prev = torch.zeros((ne, 4), dtype=torch.float32, device="cuda")
+    gen = torch.Generator(device="cuda").manual_seed(seed)
     Xs, Ys = [], []
     has_fib = sim.scene.v_fiber_dir is not None
     t0 = time.perf_counter()
@@ -98,20 +106,21 @@ def collect(sim, n_frames: int, seed: int = 0, samples_per_substep: int = 8192, 
         for s in range(sim.nsubsteps):
             e.set_lambda_mode(MODE_PLAIN)
             e.integrate()
+            e.set_lambda(lam_prev)
             if sub % substep_stride == 0:
-                e.set_lambda(lam_prev)
                 e.features(feat, normalized=False)
-                e.snapshot(True)
-                e.set_lambda_mode(MODE_CARRY)
-                e.warm_start()
-                e.iterate(conv_iters)
-                e.get_lambda(lam)
-                e.snapshot(False)
-                e.set_lambda_mode(MODE_PLAIN)
-                idx = torch.randint(0, ne, (min(samples_per_substep, ne),), device="cuda")
+            e.snapshot(True)
+            e.set_lambda_mode(MODE_CARRY)
+            e.warm_start()
+            e.iterate(conv_iters)
+            e.get_lambda(lam)
+            e.snapshot(False)
+            e.set_lambda_mode(MODE_PLAIN)
+            if sub % substep_stride == 0:
+                idx = torch.randint(0, ne, (min(samples_per_substep, ne),), device="cuda", generator=gen)
                 Xs.append(feat[idx].clone())
-                Ys.append(lam[idx].clone())
-                lam_prev.copy_(lam)
+                Ys.append((lam[idx] - lam_prev[idx]).clone())
+            lam_prev.copy_(lam)
             e.iterate(sim.fine_iters)
             e.collide()
             e.update_velocities()
@@ -217,7 +226,7 @@ def iterations_to_reach(sim, mode: int, target: dict, kmax: int = 10):
     for k in range(1, kmax + 1):
         e.iterate(1)
         r = _res(e)
-        if all(r[t] <= target[t] for t in target):
+        if all(r[t] <= target[t] for t in target):   # False for NaN
             hit = k
             break
     e.snapshot(False)
@@ -225,9 +234,14 @@ def iterations_to_reach(sim, mode: int, target: dict, kmax: int = 10):
     return hit
 
 
-def compare(sim, n_frames: int, seed: int = 1, plain_iters: int = 10, modes=(MODE_CARRY, MODE_PREV, MODE_NET)):
-    """Per substep of a fresh randomized trajectory: the plain 10-iteration
-    residual is the target; record the iterations each mode needs to reach it."""
+def compare(sim, n_frames: int, seed: int = 1, plain_iters: int = 10, modes=(MODE_CARRY, MODE_PREV, MODE_NET),
+            conv_iters: int = 40, exact: bool = True):
+    """Per substep of a fresh randomized plain trajectory: the plain
+    ``plain_iters``-iteration residual is the target; record the iterations
+    each mode needs to reach it (None = not within ``plain_iters``).  Every
+    mode starts from the previous substep's converged multipliers.  With
+    ``exact``, key "exact" is the bound a perfect predictor would give
+    (lambda0 = this substep's own converged multipliers)."""
     import torch
     e = sim.engine
     w = sim.workload
@@ -239,6 +253,10 @@ def compare(sim, n_frames: int, seed: int = 1, plain_iters: int = 10, modes=(MOD
     lam_prev = torch.zeros((ne, 4), dtype=torch.float32, device="cuda")
     lam = torch.zeros_like(lam_prev)
     out = {m: [] for m in modes}
+    if exact:
+        out["exact"] = []
+    p0 = sim.scene.pos0
+    e.set_state(p0, p0, np.zeros_like(p0))
     for f in range(1, n_frames + 1):
         if has_fib:
             e.set_fiber_activation(walk.next())
@@ -251,19 +269,76 @@ def compare(sim, n_frames: int, seed: int = 1, plain_iters: int = 10, modes=(MOD
             e.iterate(plain_iters)
             target = _res(e)
             e.snapshot(False)
+            e.snapshot(True)
+            e.set_lambda_mode(MODE_CARRY)
+            e.warm_start()
+            e.iterate(conv_iters)
+            e.get_lambda(lam)
+            e.snapshot(False)
             for m in modes:
                 e.set_lambda(lam_prev)
                 out[m].append(iterations_to_reach(sim, m, target, kmax=plain_iters))
-            # advance the trajectory with the converged multipliers for the next substep's features
-            e.set_lambda_mode(MODE_CARRY)
-            e.warm_start()
-            e.snapshot(True)
-            e.iterate(40)
-            e.get_lambda(lam)
-            e.snapshot(False)
+            if exact:
+                e.set_lambda(lam)
+                out["exact"].append(iterations_to_reach(sim, MODE_PREV, target, kmax=plain_iters))
             lam_prev.copy_(lam)
             e.set_lambda_mode(MODE_PLAIN)
             e.iterate(plain_iters)
             e.collide()
             e.update_velocities()
     return out
+
+
+class WarmStartStepper:
+    """Frames with the learned warm start and the per-frame fallback to plain
+    zero-lambda XPBD.
+
+    A frame runs ``k_iters`` iterations per substep in mode 2.  Its residual
+    (per constraint type, at the end of the frame) is compared with the plain
+    ``plain_iters`` residual of the same frame; if any type is worse by more
+    than ``tol`` the frame is re-run plain from the saved start state.  The
+    plain residual is computed for real every ``audit_every`` frames (1 =
+    every frame, the exact rule); in between, the last audited value stands
+    in for it."""
+
+    def __init__(self, sim, k_iters: int, plain_iters: int | None = None, audit_every: int = 1,
+                 tol: float = 0.0, use_graph: bool = True):
+        self.sim = sim
+        self.k = int(k_iters)
+        self.plain_iters = int(plain_iters or sim.fine_iters)
+        self.audit_every = max(1, int(audit_every))
+        self.tol = float(tol)
+        self.use_graph = use_graph
+        self.ref = None
+        self.frames = 0
+        self.fallbacks = 0
+        self.audits = 0
+
+    def _plain_frame(self):
+        e = self.sim.engine
+        e.set_lambda_mode(MODE_PLAIN)
+        e.native_frame(self.sim.nsubsteps, self.plain_iters, self.use_graph)
+
+    def step_frame(self, frame: int) -> dict:
+        sim, e = self.sim, self.sim.engine
+        sim.load_frame_targets(frame)
+        e.apply_pintotarget()
+        e.snapshot(True)
+        audited = self.ref is None or self.frames % self.audit_every == 0
+        if audited:
+            self._plain_frame()
+            self.ref = e.residuals()
+            self.audits += 1
+            e.snapshot(False)
+        e.set_lambda_mode(MODE_NET)
+        e.native_frame(sim.nsubsteps, self.k, self.use_graph)
+        r = e.residuals()
+        bad = not all(r[t] <= self.ref[t] * (1.0 + self.tol) for t in self.ref)   # NaN counts as worse
+        if bad:
+            e.snapshot(False)
+            self._plain_frame()
+            self.ref = e.residuals()
+            self.fallbacks += 1
+        e.set_lambda_mode(MODE_PLAIN)
+        self.frames += 1
+        return {"fallback": bad, "audited": audited, "residual": r, "plain": dict(self.ref)}

@@ -1,0 +1,28 @@
+"""Block-strategy layout invariants, checked on the host (g++ build of the layout
+builder plus tests/cpp/check_block_layout.cpp; no GPU)."""
+import shutil
+import subprocess
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+@pytest.fixture(scope="module")
+def checker(tmp_path_factory):
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    exe = tmp_path_factory.mktemp("blk") / "check_block_layout"
+    subprocess.run(["g++", "-O2", "-std=c++17", f"-I{ROOT / 'include'}", f"-I{ROOT / 'rlmusclev2_amd' / 'csrc'}",
+                    str(ROOT / "tests" / "cpp" / "check_block_layout.cpp"),
+                    str(ROOT / "rlmusclev2_amd" / "csrc" / "xpbd_layout.cpp"), "-o", str(exe)], check=True)
+    return exe
+
+
+@pytest.mark.parametrize("n,tb", [(7, 64), (12, 256), (20, 1024), (30, 4096)])
+def test_block_layout_sums_match_flat(checker, n, tb):
+    r = subprocess.run([str(checker), str(n), str(tb)], capture_output=True, text=True)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "bad 0" in r.stdout

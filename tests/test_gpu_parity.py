@@ -22,8 +22,8 @@ def _bbox_diag(p):
     return float(np.linalg.norm(p.max(0) - p.min(0)))
 
 
-def _run_pair(cfg, w, frames, patch_verts=256, strategy=0):
-    sim = sims.MultiGridSim(cfg, workload=w, patch_verts=patch_verts, strategy=strategy)
+def _run_pair(cfg, w, frames, patch_verts=256, strategy=0, block_tets=512):
+    sim = sims.MultiGridSim(cfg, workload=w, patch_verts=patch_verts, strategy=strategy, block_tets=block_tets)
     ref = sims.MultiGridSim(cfg, workload=w, engine_cls=oracle.OracleSim)
     for f in range(1, frames + 1):
         print(f"frame {f}", flush=True)
@@ -47,7 +47,7 @@ def test_cantilever_10_frames_matches_oracle():
         assert abs(rs[k] - rr[k]) <= 0.01 * rr[k] + 1e-12, (k, rs[k], rr[k])
 
 
-@pytest.mark.parametrize("strategy,patch_verts", [(0, 64), (0, 256), (1, 256)])
+@pytest.mark.parametrize("strategy,patch_verts", [(0, 64), (0, 256), (1, 256), (2, 256)])
 def test_small_humanv2_pins_sdf_matches_oracle(strategy, patch_verts):
     cfg = _cfg("humanv2_scale.json")
     w = workloads.humanv2_scale(1, n_bones=2, per_bone=4, target_tets=12000, fascia_tets=3000, sdf_res=48)
@@ -64,13 +64,13 @@ def test_small_humanv2_pins_sdf_matches_oracle(strategy, patch_verts):
     assert sim.count_inverted() == ref.count_inverted()
 
 
-@pytest.mark.parametrize("strategy", [0, 1])
+@pytest.mark.parametrize("strategy", [0, 1, 2])
 def test_snh_aniso_activation_matches_oracle(strategy):
     """SNH + aniso_arap with an activation ramp (c3 constraint set, small mesh)."""
     cfg = _cfg("humanv2_snh_aniso.json")
     w = workloads.humanv2_scale(3, n_bones=1, per_bone=4, target_tets=8000, fascia_tets=1500, sdf_res=32)
     cfg["synthetic"]["sdf_res"] = 32
-    sim = sims.MultiGridSim(cfg, workload=w, strategy=strategy)
+    sim = sims.MultiGridSim(cfg, workload=w, strategy=strategy, block_tets=256)
     ref = sims.MultiGridSim(cfg, workload=w, engine_cls=oracle.OracleSim)
     for f in range(1, 6):
         a = np.full(w.nt, min(1.0, f / 5.0), np.float32)
@@ -89,12 +89,13 @@ def test_snh_aniso_activation_matches_oracle(strategy):
         assert abs(rs[k] - rr[k]) <= 0.01 * rr[k] + 1e-12, (k, rs[k], rr[k])
 
 
-def test_batch_instances_match_oracle():
+@pytest.mark.parametrize("strategy", [0, 2])
+def test_batch_instances_match_oracle(strategy):
     """uneven instances in one launch, each with its own bone SDF group."""
     cfg = _cfg("batch512.json")
     w = workloads.batch(4, n_inst=6, lo=500, hi=3000, sdf_res=24)
     cfg["synthetic"]["sdf_res"] = 24
-    sim = sims.MultiGridSim(cfg, workload=w, patch_verts=128)
+    sim = sims.MultiGridSim(cfg, workload=w, patch_verts=128, strategy=strategy, block_tets=512)
     ref = sims.MultiGridSim(cfg, workload=w, engine_cls=oracle.OracleSim)
     rng = np.random.default_rng(0)
     inst_act = rng.uniform(0, 1, w.info["n_inst"]).astype(np.float32)
@@ -117,8 +118,9 @@ def test_deterministic_and_graph_equals_stream():
     w = workloads.humanv2_scale(1, n_bones=1, per_bone=4, target_tets=6000, fascia_tets=1000, sdf_res=32)
     cfg["synthetic"]["sdf_res"] = 32
     outs = []
-    for use_graph, strat in ((True, 0), (True, 0), (False, 0), (True, 1), (True, 1)):
-        sim = sims.MultiGridSim(cfg, workload=w, strategy=strat)
+    for use_graph, strat in ((True, 0), (True, 0), (False, 0), (True, 1), (True, 1), (True, 2), (True, 2),
+                             (False, 2)):
+        sim = sims.MultiGridSim(cfg, workload=w, strategy=strat, block_tets=256)
         sim.use_graph = use_graph
         for f in range(1, 4):
             sim.step_frame(f)
@@ -127,15 +129,19 @@ def test_deterministic_and_graph_equals_stream():
     np.testing.assert_array_equal(outs[0], outs[2])   # graph replay == stream launches
     np.testing.assert_array_equal(outs[3], outs[4])   # flat strategy: run-to-run identical
     np.testing.assert_allclose(outs[0], outs[3], atol=1e-5)
+    np.testing.assert_array_equal(outs[5], outs[6])   # block strategy: run-to-run identical
+    np.testing.assert_array_equal(outs[5], outs[7])   # block strategy: graph replay == stream launches
+    np.testing.assert_allclose(outs[0], outs[5], atol=1e-5)
 
 
-def test_phase_api_equals_fused_iteration():
+@pytest.mark.parametrize("strategy", [1, 2])
+def test_phase_api_equals_fused_iteration(strategy):
     """clear / solve_constraints / apply_dP / apply_pintotarget == one fused iteration."""
     cfg = _cfg("humanv2_scale.json")
     w = workloads.humanv2_scale(1, n_bones=1, per_bone=4, target_tets=6000, fascia_tets=1000, sdf_res=32)
     cfg["synthetic"]["sdf_res"] = 32
     a = sims.MultiGridSim(cfg, workload=w)
-    b = sims.MultiGridSim(cfg, workload=w, strategy=1)
+    b = sims.MultiGridSim(cfg, workload=w, strategy=strategy, block_tets=256)
     for s in (a, b):
         s.load_frame_targets(3)
         s.engine.apply_pintotarget()
