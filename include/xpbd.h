@@ -134,6 +134,38 @@ int32_t xpbd_set_bone_transforms(xpbd_ctx *ctx, int32_t nbones, const float *xfo
 int32_t xpbd_finalize(xpbd_ctx *ctx, const xpbd_tuning *tuning);
 int32_t xpbd_get_stats(xpbd_ctx *ctx, xpbd_stats *out);
 
+/* Two-level hierarchical correction once per substep, after integrate and before the fine
+ * iterations (MultiGridSim._hpbd_vcycle, src/muscle/mg/mg.py:1064-1079; coarse sweeps
+ * _coarse_arap_sweep mg.py:939-952; restriction restrict_galerkin_scatter_kernel +
+ * block_solve_kernel, src/muscle/mg/kernel.py:138-180; prolong_delta_kernel kernel.py:9-40).
+ * The coarse cage is another finalized context (flat strategy): ARAP-SVD constraints, masses,
+ * stopped = coarse pins.  Coarse nodes are grouped in blocks (one per muscle, contiguous in the
+ * coarse context's caller order).  Per substep, per block b:
+ *   [A t]_b = (sum_{i in fit_b} x_i [r_i - c_b; 1]^T) S_b^-1      (r = fine rest positions)
+ *   q_j     = sum_l G_b[j, l] (sum_i P_il x_i + eps_b (A_b (q0_l - c_b) + t_b))
+ * then iters coarse Jacobi sweeps, then x_i += sum_j P_ij (q_j - q_prev_j) for every vertex
+ * that is not an active pin (also added to the previous positions when kinematic != 0, so the
+ * shift does not enter the velocity), then the fine pins are re-applied. */
+typedef struct {
+    const int32_t *p_idx;       /* nv*8 coarse node per trilinear corner (coarse caller order) */
+    const float *p_w;           /* nv*8 weights in [0, 1]                                       */
+    int32_t nblk;
+    const int32_t *blk_off;     /* nblk+1 coarse node ranges                                    */
+    const float *minv;          /* sum n_b^2: G_b = (P^T P + eps_b I)^-1, row-major per block   */
+    const float *eps;           /* nblk                                                         */
+    const int32_t *fit_off;     /* nblk+1 ranges into fit_fine                                  */
+    const int32_t *fit_fine;    /* fine vertices of each block's affine fit                     */
+    const float *fit_center;    /* nblk*3 c_b                                                   */
+    const float *fit_sinv;      /* nblk*16 S_b^-1, S_b = sum [r_i - c_b; 1][r_i - c_b; 1]^T     */
+    int32_t iters;              /* coarse sweeps per substep; <= 0 turns the level off          */
+    int32_t kinematic;
+} xpbd_coarse_desc;
+/* coarse = NULL or desc = NULL turns it off.  The coarse context must outlive its use;
+ * xpbd_finalize drops the link. */
+int32_t xpbd_set_coarse_level(xpbd_ctx *ctx, xpbd_ctx *coarse, const xpbd_coarse_desc *desc);
+/* the V-cycle alone, for callers that drive the phases one by one */
+int32_t xpbd_vcycle(xpbd_ctx *ctx);
+
 /* ---- state ----------------------------------------------------------- */
 int32_t xpbd_set_state(xpbd_ctx *ctx, const float *pos, const float *pprev, const float *vel);
 int32_t xpbd_get_state(xpbd_ctx *ctx, float *pos, float *pprev, float *vel);
@@ -205,10 +237,10 @@ int32_t xpbd_mlp_backward(const xpbd_mlp_dev *net, const float *X, const float *
 /* multiplier modes (flat strategy): 0 = plain zero-lambda Jacobi (the reference GPU path,
  * solve_kernels.py:271-281 with clear() every iteration, mg.py:1348); 1 = lambda carried across
  * the iterations of a substep from 0; 2 = learned warm start: after integrate the network
- * predicts, per tet, the change of the multipliers since the previous substep, lambda0 =
- * previous final multipliers + prediction; the first iteration applies lambda0 as an impulse
- * along the constraint gradients, the rest carry lambda; 3 = the same with lambda0 = the
- * previous substep's final multipliers (no network).  No reference counterpart. */
+ * predicts, per tet and constraint type, a gain on the previous substep's final multipliers,
+ * lambda0 = lambda_prev * clamp(1 + out_scale * y, 0, 2); the first iteration applies lambda0
+ * as an impulse along the constraint gradients, the rest carry lambda; 3 = the same with
+ * lambda0 = lambda_prev (no network).  No reference counterpart. */
 int32_t xpbd_set_lambda_mode(xpbd_ctx *ctx, int32_t mode);
 /* network weights (host fp32; W1, W2 stored as bf16), feature standardisation (IN each) and
  * per-output scales (OUT) */

@@ -241,8 +241,34 @@ class OracleSim:
     def update_velocities(self):
         self.lib.orc_update_velocities(C.byref(self._s), C.byref(self._prm))
 
+    # two-level correction, restated in NumPy from mg.py:1064-1079, mg/kernel.py:9-40, 138-180
+    # (least-squares restriction per muscle block, as built by rlmusclev2_amd/multigrid.py)
+    def set_coarse_level(self, coarse, level, kinematic=True):
+        self.coarse = coarse
+        self.mg = level
+        self._mg_shift_pprev = bool(kinematic)
+
+    def vcycle(self):
+        cc, lv = getattr(self, "coarse", None), getattr(self, "mg", None)
+        if cc is None or lv is None or lv.iters <= 0:
+            return
+        from rlmusclev2_amd.multigrid import restrict_numpy
+        cc.pos[:] = restrict_numpy(lv, self.pos, self.pos0).astype(np.float32)
+        qprev = cc.pos.astype(np.float64).copy()
+        cc.iterate(lv.iters)
+        d = cc.pos.astype(np.float64) - qprev
+        pidx = np.asarray(lv.p_idx, np.int64)
+        dx = np.einsum("ik,ikc->ic", lv.p_w.astype(np.float64), d[pidx])
+        if self.pin_vids is not None:      # pinned vertices are kinematic: no correction
+            dx[self.pin_vids[self.pin_active != 0]] = 0.0
+        self.pos[:] = (self.pos.astype(np.float64) + dx).astype(np.float32)
+        if getattr(self, "_mg_shift_pprev", False):
+            self.pprev[:] = (self.pprev.astype(np.float64) + dx).astype(np.float32)
+        self.apply_pintotarget()
+
     def native_substep(self, iters):
         self.integrate()
+        self.vcycle()
         self.iterate(iters)
         self.collide()
         self.update_velocities()

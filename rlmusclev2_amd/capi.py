@@ -42,6 +42,13 @@ class XpbdTuning(C.Structure):
                 ("block_threads", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
+class XpbdCoarseDesc(C.Structure):
+    _fields_ = [("p_idx", C.c_void_p), ("p_w", C.c_void_p), ("nblk", C.c_int32), ("blk_off", C.c_void_p),
+                ("minv", C.c_void_p), ("eps", C.c_void_p), ("fit_off", C.c_void_p), ("fit_fine", C.c_void_p),
+                ("fit_center", C.c_void_p), ("fit_sinv", C.c_void_p), ("iters", C.c_int32),
+                ("kinematic", C.c_int32)]
+
+
 class XpbdDcdParams(C.Structure):
     _fields_ = [("thickness", C.c_float), ("multiplier", C.c_float), ("friction_static", C.c_float),
                 ("friction_dynamic", C.c_float), ("offset", C.c_float), ("normal_cull", C.c_float),
@@ -110,6 +117,8 @@ SIGNATURES = {
     "xpbd_mlp_forward": [C.POINTER(XpbdMlpDev), _P, C.c_int64, _P, _P, _P, _P],
     "xpbd_mlp_grad_size": [],
     "xpbd_mlp_backward": [C.POINTER(XpbdMlpDev), _P, _P, _P, _P, C.c_int64, _P, _P, C.c_int64, _P],
+    "xpbd_set_coarse_level": [_P, _P, C.POINTER(XpbdCoarseDesc)],
+    "xpbd_vcycle": [_P],
     "xpbd_set_lambda_mode": [_P, _I],
     "xpbd_set_mlp": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "xpbd_warm_start": [_P],

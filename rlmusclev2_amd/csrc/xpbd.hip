@@ -757,14 +757,16 @@ __global__ void k_features(IterArgs a, const float4 *__restrict__ X, float *__re
     for (int i = 0; i < 24; ++i) o[i] = normalize ? (f[i] - fmean[i]) * finv[i] : f[i];
 }
 
-// lambda0 = previous-substep multipliers + network output * per-type output
-// scale (the network predicts the change of the multipliers over one substep)
+// lambda0 = previous-substep multipliers, each scaled by the network's
+// per-tet, per-type gain 1 + out_scale * y, clamped to [0, 2]
 __global__ void k_lam0(int64_t n, const float *__restrict__ Y, const float *__restrict__ oscale, float4 *lam) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const float4 p = lam[g];
-    lam[g] = make_float4(p.x + Y[4 * g] * oscale[0], p.y + Y[4 * g + 1] * oscale[1], p.z + Y[4 * g + 2] * oscale[2],
-                         0.f);
+    const float s0 = fminf(fmaxf(1.0f + Y[4 * g] * oscale[0], 0.0f), 2.0f);
+    const float s1 = fminf(fmaxf(1.0f + Y[4 * g + 1] * oscale[1], 0.0f), 2.0f);
+    const float s2 = fminf(fmaxf(1.0f + Y[4 * g + 2] * oscale[2], 0.0f), 2.0f);
+    lam[g] = make_float4(p.x * s0, p.y * s1, p.z * s2, 0.f);
 }
 
 // ---------------------------------------------------------------------------
@@ -926,6 +928,142 @@ __global__ void k_pins(int nv, float4 *X, const float4 *pin) {
     if (i >= nv) return;
     const float4 p = pin[i];
     if (p.w != 0.0f) { float4 x = X[i]; x.x = p.x; x.y = p.y; x.z = p.z; X[i] = x; }
+}
+
+// two-level correction (MultiGridSim._hpbd_vcycle, mg.py:1064-1079).
+// Restriction, per muscle block b with coarse nodes j (block-contiguous order):
+//   [A t]_b = (sum_i x_i [r_i - c_b; 1]^T) S_b^-1        affine fit of the block
+//   rhs_j   = sum_i P_ij x_i + eps_b (A_b (q0_j - c_b) + t_b)
+//   q_j     = sum_l G_b[j, l] rhs_l,  G_b = (P^T P + eps_b I)^-1
+// (restrict_galerkin_scatter_kernel + block_solve_kernel, mg/kernel.py:138-180,
+// with the ridge anchored at the block's affine motion).  Then the coarse
+// sweeps, and the delta prolongation (prolong_delta_kernel, mg/kernel.py:9-40).
+struct MgArgs {
+    const int32_t *fit_off, *fit_fine;   // per block: fine vertices (device ids)
+    const float4 *fit_rest;              // per fit entry: rest position - block centre
+    const float *fit_sinv;               // per block: 4x4
+    float *fitT;                         // per block: 3x4 result
+    const int32_t *ptoff, *ptfv;         // per coarse node (block order): P^T entries
+    const float *ptw;
+    const int32_t *nblk_of;              // per coarse node (block order): block id
+    const float4 *q0c;                   // per coarse node (block order): rest - block centre
+    const float *eps;                    // per block
+    float4 *rhs;                         // per coarse node (block order)
+    const int32_t *boff;                 // nblk + 1
+    const int64_t *goff;                 // per block: offset of G_b
+    const float *G;
+    const int32_t *dev_of;               // block order -> coarse device id
+};
+
+__global__ __launch_bounds__(256) void k_mg_fit(MgArgs m, const float4 *__restrict__ xf) {
+    __shared__ float red[12][256];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    float acc[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[k] = 0.f;
+    for (int e = m.fit_off[b] + tid; e < m.fit_off[b + 1]; e += 256) {
+        const float4 x = xf[m.fit_fine[e]];
+        const float4 r = m.fit_rest[e];
+        const float h[4] = {r.x, r.y, r.z, 1.0f};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            acc[k] += x.x * h[k];
+            acc[4 + k] += x.y * h[k];
+            acc[8 + k] += x.z * h[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 12; ++k) red[k][tid] = acc[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s)
+#pragma unroll
+            for (int k = 0; k < 12; ++k) red[k][tid] += red[k][tid + s];
+        __syncthreads();
+    }
+    if (tid < 12) {
+        const int row = tid / 4, col = tid % 4;
+        const float *S = m.fit_sinv + 16 * b;
+        float v = 0.f;
+        for (int k = 0; k < 4; ++k) v += red[4 * row + k][0] * S[4 * k + col];
+        m.fitT[12 * b + tid] = v;
+    }
+}
+
+__global__ void k_mg_rhs(MgArgs m, int nc, const float4 *__restrict__ xf) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nc) return;
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    for (int e = m.ptoff[j]; e < m.ptoff[j + 1]; ++e) {
+        const float w = m.ptw[e];
+        const float4 x = xf[m.ptfv[e]];
+        sx += w * x.x;
+        sy += w * x.y;
+        sz += w * x.z;
+    }
+    const int b = m.nblk_of[j];
+    const float *T = m.fitT + 12 * b;
+    const float4 q = m.q0c[j];
+    const float ep = m.eps[b];
+    sx += ep * (T[0] * q.x + T[1] * q.y + T[2] * q.z + T[3]);
+    sy += ep * (T[4] * q.x + T[5] * q.y + T[6] * q.z + T[7]);
+    sz += ep * (T[8] * q.x + T[9] * q.y + T[10] * q.z + T[11]);
+    m.rhs[j] = make_float4(sx, sy, sz, 0.f);
+}
+
+__global__ void k_mg_solve(MgArgs m, int nc, float4 *__restrict__ xc, float4 *__restrict__ qprev) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nc) return;
+    const int b = m.nblk_of[j];
+    const int s = m.boff[b], n = m.boff[b + 1] - s;
+    const float *row = m.G + m.goff[b] + (int64_t)(j - s) * n;
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    for (int l = 0; l < n; ++l) {
+        const float g = row[l];
+        const float4 r = m.rhs[s + l];
+        sx += g * r.x;
+        sy += g * r.y;
+        sz += g * r.z;
+    }
+    const int d = m.dev_of[j];
+    float4 q = xc[d];
+    q.x = sx;
+    q.y = sy;
+    q.z = sz;
+    xc[d] = q;
+    qprev[d] = q;
+}
+
+__global__ void k_mg_prolong(int nv, const int4 *__restrict__ pidx, const float4 *__restrict__ pw,
+                             const float4 *__restrict__ xc, const float4 *__restrict__ qprev, float4 *__restrict__ xf,
+                             float4 *__restrict__ pp, const float4 *__restrict__ pin) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    if (pin && pin[i].w != 0.0f) return;   // pinned vertices are kinematic (snapped right after)
+    const int4 ia = pidx[2 * i], ib = pidx[2 * i + 1];
+    const float4 wa = pw[2 * i], wb = pw[2 * i + 1];
+    const int id[8] = {ia.x, ia.y, ia.z, ia.w, ib.x, ib.y, ib.z, ib.w};
+    const float w[8] = {wa.x, wa.y, wa.z, wa.w, wb.x, wb.y, wb.z, wb.w};
+    float dx = 0.f, dy = 0.f, dz = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const float4 a = xc[id[k]], b = qprev[id[k]];
+        dx += w[k] * (a.x - b.x);
+        dy += w[k] * (a.y - b.y);
+        dz += w[k] * (a.z - b.z);
+    }
+    float4 x = xf[i];
+    x.x += dx;
+    x.y += dy;
+    x.z += dz;
+    xf[i] = x;
+    if (pp) {  // kinematic correction: the shift does not enter the velocity
+        float4 q = pp[i];
+        q.x += dx;
+        q.y += dy;
+        q.z += dz;
+        pp[i] = q;
+    }
 }
 
 __global__ void k_zero_i32(int n, int32_t *p) {
@@ -1517,6 +1655,15 @@ struct xpbd_ctx {
     ushort4 *d_blidx = nullptr;
     uint16_t *d_bslot = nullptr;
     float4 *BPART = nullptr;
+    // two-level correction: a finalized coarse context (not owned) and transfer tables
+    xpbd_ctx *coarse = nullptr;
+    int mg_iters = 0;
+    bool mg_pidx_set = false, mg_kinematic = false;
+    int mg_nblk = 0;
+    int32_t *d_mg_pidx = nullptr;
+    float *d_mg_pw = nullptr;
+    float4 *MG_QPREV = nullptr;
+    MgArgs mg{};
     float *d_Bp = nullptr, *d_rvol = nullptr, *d_rsnh = nullptr, *d_act = nullptr;
     DevClass *d_cls = nullptr;
     int4 *d_tets = nullptr;
@@ -2064,10 +2211,40 @@ int lam_for_iteration(xpbd_ctx *c) {
     return 1;
 }
 
+// restrict -> coarse Jacobi ARAP sweeps (coarse pins held as stopped vertices)
+// -> delta prolongation -> pins, on the fine context's stream
+int launch_vcycle(xpbd_ctx *c) {
+    xpbd_ctx *cc = c->coarse;
+    const int nc = cc->nv;
+    const dim3 gc((unsigned)((nc + 255) / 256));
+    hipLaunchKernelGGL(k_mg_fit, dim3((unsigned)c->mg_nblk), dim3(256), 0, c->stream, c->mg, (const float4 *)c->X[0]);
+    hipLaunchKernelGGL(k_mg_rhs, gc, dim3(256), 0, c->stream, c->mg, nc, (const float4 *)c->X[0]);
+    hipLaunchKernelGGL(k_mg_solve, gc, dim3(256), 0, c->stream, c->mg, nc, cc->X[0], c->MG_QPREV);
+    HIPCHK(c, hipGetLastError());
+    const hipStream_t saved = cc->stream;
+    cc->stream = c->stream;
+    int r = XPBD_OK;
+    for (int it = 0; it < c->mg_iters && r == XPBD_OK; ++it)
+        r = launch_iter(cc, cc->X[it & 1], cc->X[(it + 1) & 1], MODE_FUSED, 0);
+    cc->stream = saved;
+    if (r) return fail(c, r, std::string("coarse level: ") + cc->err);
+    hipLaunchKernelGGL(k_mg_prolong, vgrid(c->nv), dim3(256), 0, c->stream, c->nv, (const int4 *)c->d_mg_pidx,
+                       (const float4 *)c->d_mg_pw, (const float4 *)cc->X[c->mg_iters & 1],
+                       (const float4 *)c->MG_QPREV, c->X[0], c->mg_kinematic ? c->PPv : nullptr,
+                       c->pins_on ? (const float4 *)c->PIN : nullptr);
+    HIPCHK(c, hipGetLastError());
+    if (c->pins_on) {
+        hipLaunchKernelGGL(k_pins, vgrid(c->nv), dim3(256), 0, c->stream, c->nv, c->X[0], (const float4 *)c->PIN);
+        HIPCHK(c, hipGetLastError());
+    }
+    return XPBD_OK;
+}
+
 // one substep, positions start and end in X[0]
 int enqueue_substep(xpbd_ctx *c, int iters, bool dcd_rebuild) {
     int r = launch_integrate(c);
     if (r) return r;
+    if (c->coarse && c->mg_iters > 0 && (r = launch_vcycle(c))) return r;
     if (c->lam_mode && (r = launch_warm_start(c))) return r;
     for (int it = 0; it < iters; ++it) {
         r = launch_iter(c, c->X[it & 1], c->X[(it + 1) & 1], MODE_FUSED, lam_for_iteration(c));
@@ -2374,6 +2551,9 @@ int32_t xpbd_finalize(xpbd_ctx *c, const xpbd_tuning *t) {
     if (!c->params_set) return fail(c, XPBD_EINVAL, "call xpbd_set_params() first");
     HIPCHK(c, hipSetDevice(c->device));
     free_device(c);
+    c->coarse = nullptr;
+    c->mg_iters = 0;
+    c->mg_pidx_set = false;
     int P = (t && t->patch_verts > 0) ? t->patch_verts : 256;
     if (P != 64 && P != 128 && P != 256) return fail(c, XPBD_EINVAL, "patch_verts must be 64, 128 or 256");
     c->P = P;
@@ -2868,6 +3048,128 @@ int32_t xpbd_set_mlp(xpbd_ctx *c, const float *W1, const float *b1, const float 
     HIPCHK(c, hipMemcpy(c->oscale, out_scale, 4 * 4, hipMemcpyHostToDevice));
     c->mlp_ready = true;
     return XPBD_OK;
+}
+
+int32_t xpbd_set_coarse_level(xpbd_ctx *c, xpbd_ctx *cc, const xpbd_coarse_desc *d) {
+    int r = check_ready(c);
+    if (r) return r;
+    invalidate_graph(c);
+    if (!cc || !d || d->iters <= 0) {
+        c->coarse = nullptr;
+        c->mg_iters = 0;
+        return XPBD_OK;
+    }
+    if (cc == c) return fail(c, XPBD_EINVAL, "the coarse level must be another context");
+    if (!cc->ready) return fail(c, XPBD_ENOTREADY, "coarse context is not finalized");
+    if (cc->device != c->device) return fail(c, XPBD_EINVAL, "coarse context is on another device");
+    if (!cc->flat || cc->block) return fail(c, XPBD_EINVAL, "the coarse context must use the flat strategy");
+    if (c->mg_pidx_set) return fail(c, XPBD_EINVAL, "coarse level already set; finalize again to change it");
+    const int nv = c->nv, nc = cc->nv, nb = d->nblk;
+    if (!d->p_idx || !d->p_w || nb <= 0 || !d->blk_off || !d->minv || !d->eps || !d->fit_off || !d->fit_center ||
+        !d->fit_sinv || (d->fit_off[nb] > 0 && !d->fit_fine))
+        return fail(c, XPBD_EINVAL, "incomplete coarse level description");
+    if (d->blk_off[0] != 0 || d->blk_off[nb] != nc) return fail(c, XPBD_EINVAL, "blocks must cover the coarse nodes");
+    for (int b = 0; b < nb; ++b)
+        if (d->blk_off[b + 1] <= d->blk_off[b] || d->fit_off[b + 1] < d->fit_off[b])
+            return fail(c, XPBD_EINVAL, "block ranges must increase");
+    // prolongation, fine device order, coarse device ids
+    std::vector<int32_t> pidx(8 * (size_t)nv);
+    std::vector<float> pw(8 * (size_t)nv);
+    std::vector<std::vector<std::pair<int32_t, float>>> pt(nc);   // P^T per coarse node (block order)
+    for (int i = 0; i < nv; ++i) {
+        const int fi = c->L.iperm[i];
+        for (int k = 0; k < 8; ++k) {
+            const int j = d->p_idx[8 * (size_t)i + k];
+            const float w = d->p_w[8 * (size_t)i + k];
+            if (j < 0 || j >= nc) return fail(c, XPBD_EINVAL, "prolongation index out of range");
+            if (!(w >= 0.0f) || !(w <= 1.0f)) return fail(c, XPBD_EINVAL, "prolongation weight outside [0, 1]");
+            pidx[8 * (size_t)fi + k] = cc->L.iperm[j];
+            pw[8 * (size_t)fi + k] = w;
+            if (w > 0.0f) pt[j].push_back({fi, w});
+        }
+    }
+    std::vector<int32_t> ptoff(nc + 1, 0), ptfv, nblk_of(nc), dev_of(nc);
+    std::vector<float> ptw;
+    std::vector<float4> q0c(nc);
+    for (int b = 0; b < nb; ++b)
+        for (int j = d->blk_off[b]; j < d->blk_off[b + 1]; ++j) nblk_of[j] = b;
+    for (int j = 0; j < nc; ++j) {
+        std::sort(pt[j].begin(), pt[j].end());
+        for (const auto &e : pt[j]) {
+            ptfv.push_back(e.first);
+            ptw.push_back(e.second);
+        }
+        ptoff[j + 1] = (int32_t)ptfv.size();
+        dev_of[j] = cc->L.iperm[j];
+        const float *ctr = d->fit_center + 3 * (size_t)nblk_of[j];
+        q0c[j] = make_float4(cc->pos0[3 * (size_t)j] - ctr[0], cc->pos0[3 * (size_t)j + 1] - ctr[1],
+                             cc->pos0[3 * (size_t)j + 2] - ctr[2], 0.f);
+    }
+    if (ptfv.empty()) { ptfv.push_back(0); ptw.push_back(0.0f); }
+    // affine-fit entries: fine device ids and centred rest positions
+    const int nfit = d->fit_off[nb];
+    std::vector<int32_t> fitf(std::max(nfit, 1), 0);
+    std::vector<float4> fitr(std::max(nfit, 1), make_float4(0, 0, 0, 0));
+    for (int b = 0; b < nb; ++b)
+        for (int e = d->fit_off[b]; e < d->fit_off[b + 1]; ++e) {
+            const int i = d->fit_fine[e];
+            if (i < 0 || i >= nv) return fail(c, XPBD_EINVAL, "fit vertex out of range");
+            fitf[e] = c->L.iperm[i];
+            const float *ctr = d->fit_center + 3 * (size_t)b;
+            fitr[e] = make_float4(c->pos0[3 * (size_t)i] - ctr[0], c->pos0[3 * (size_t)i + 1] - ctr[1],
+                                  c->pos0[3 * (size_t)i + 2] - ctr[2], 1.f);
+        }
+    std::vector<int64_t> goff(nb + 1, 0);
+    for (int b = 0; b < nb; ++b) {
+        const int64_t n = d->blk_off[b + 1] - d->blk_off[b];
+        goff[b + 1] = goff[b] + n * n;
+    }
+    std::vector<int32_t> boff(d->blk_off, d->blk_off + nb + 1), foff(d->fit_off, d->fit_off + nb + 1);
+    MgArgs m{};
+    int32_t *p_fit_off, *p_fit_fine, *p_ptoff, *p_ptfv, *p_nblk, *p_boff, *p_dev;
+    float4 *p_fit_rest, *p_q0c, *p_rhs;
+    float *p_sinv, *p_T, *p_ptw, *p_eps, *p_G;
+    int64_t *p_goff;
+    if ((r = dupload(c, &c->d_mg_pidx, pidx.data(), pidx.size())) || (r = dupload(c, &c->d_mg_pw, pw.data(), pw.size())) ||
+        (r = dupload(c, &p_fit_off, foff.data(), foff.size())) || (r = dupload(c, &p_fit_fine, fitf.data(), fitf.size())) ||
+        (r = dupload(c, &p_fit_rest, fitr.data(), fitr.size())) || (r = dupload(c, &p_sinv, d->fit_sinv, 16 * (size_t)nb)) ||
+        (r = dalloc(c, &p_T, 12 * (size_t)nb)) || (r = dupload(c, &p_ptoff, ptoff.data(), ptoff.size())) ||
+        (r = dupload(c, &p_ptfv, ptfv.data(), ptfv.size())) || (r = dupload(c, &p_ptw, ptw.data(), ptw.size())) ||
+        (r = dupload(c, &p_nblk, nblk_of.data(), nblk_of.size())) || (r = dupload(c, &p_q0c, q0c.data(), q0c.size())) ||
+        (r = dupload(c, &p_eps, d->eps, (size_t)nb)) || (r = dalloc(c, &p_rhs, (size_t)nc)) ||
+        (r = dupload(c, &p_boff, boff.data(), boff.size())) || (r = dupload(c, &p_goff, goff.data(), goff.size())) ||
+        (r = dupload(c, &p_G, d->minv, (size_t)goff[nb])) || (r = dupload(c, &p_dev, dev_of.data(), dev_of.size())) ||
+        (r = dalloc(c, &c->MG_QPREV, (size_t)std::max(nc, 1))))
+        return r;
+    m.fit_off = p_fit_off;
+    m.fit_fine = p_fit_fine;
+    m.fit_rest = p_fit_rest;
+    m.fit_sinv = p_sinv;
+    m.fitT = p_T;
+    m.ptoff = p_ptoff;
+    m.ptfv = p_ptfv;
+    m.ptw = p_ptw;
+    m.nblk_of = p_nblk;
+    m.q0c = p_q0c;
+    m.eps = p_eps;
+    m.rhs = p_rhs;
+    m.boff = p_boff;
+    m.goff = p_goff;
+    m.G = p_G;
+    m.dev_of = p_dev;
+    c->mg = m;
+    c->mg_nblk = nb;
+    c->coarse = cc;
+    c->mg_iters = d->iters;
+    c->mg_kinematic = d->kinematic != 0;
+    c->mg_pidx_set = true;
+    return XPBD_OK;
+}
+
+int32_t xpbd_vcycle(xpbd_ctx *c) {
+    int r = check_ready(c);
+    if (r) return r;
+    return (c->coarse && c->mg_iters > 0) ? launch_vcycle(c) : XPBD_OK;
 }
 
 int32_t xpbd_set_lambda_mode(xpbd_ctx *c, int32_t mode) {

@@ -275,6 +275,24 @@ class XpbdEngine:
         self.ctx.call("xpbd_get_sdf_cells", ptr(out))
         return out
 
+    # -- two-level correction --------------------------------------------------
+    def set_coarse_level(self, coarse: "XpbdEngine", level, kinematic: bool = True):
+        """Link a coarse-cage engine (flat strategy) with the transfer tables of a
+        multigrid.CoarseLevel."""
+        k = self._keep
+        arrs = dict(p_idx=i32(level.p_idx, (-1, 8)), p_w=f32(level.p_w, (-1, 8)), blk_off=i32(level.blk_off),
+                    minv=f32(level.minv), eps=f32(level.eps), fit_off=i32(level.fit_off),
+                    fit_fine=i32(level.fit_fine), fit_center=f32(level.fit_center, (-1, 3)),
+                    fit_sinv=f32(level.fit_sinv, (-1, 16)))
+        k["mg"] = arrs
+        k["mg_coarse"] = coarse       # the coarse context must outlive this one's use of it
+        d = capi.XpbdCoarseDesc(nblk=len(level.blk_off) - 1, iters=int(level.iters), kinematic=int(bool(kinematic)),
+                                **{n: a.ctypes.data for n, a in arrs.items()})
+        self.ctx.call("xpbd_set_coarse_level", coarse.ctx.h, C.byref(d))
+
+    def vcycle(self):
+        self.ctx.call("xpbd_vcycle")
+
     # -- learned warm start ----------------------------------------------------
     def set_lambda_mode(self, mode: int):
         self.ctx.call("xpbd_set_lambda_mode", int(mode))

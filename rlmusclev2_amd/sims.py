@@ -14,7 +14,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import builder, workloads
+from . import builder, multigrid, workloads
 from .builder import ConsTable
 from .config import find_constraint
 from .engine import XpbdEngine
@@ -123,8 +123,6 @@ class MultiGridSim:
     def __init__(self, cfg: dict, workload: workloads.Workload | None = None, device: int = 0,
                  patch_verts: int = 256, engine_cls=XpbdEngine, strategy: int = 0, block_tets: int = 1024,
                  block_threads: int = 256):
-        if cfg.get("use_multigrid", False):
-            raise NotImplementedError("use_multigrid: the two-level V-cycle is not on the HIP path yet")
         if not cfg.get("use_jacobi", True):
             raise NotImplementedError("use_jacobi=false: Gauss-Seidel is not on the GPU path")
         self.cfg = cfg
@@ -139,10 +137,45 @@ class MultiGridSim:
             if engine_cls is XpbdEngine else engine_cls(self.scene)
         self.NV = self.scene.nv
         self.NT = self.scene.nt
+        self.coarse = None
+        if cfg.get("use_multigrid", False):
+            self._build_coarse(engine_cls, device)
         self.use_graph = True
         self.collision_solvers = []
         self.collision_solvers_iter = []
         self._frame = None
+
+    def _build_coarse(self, engine_cls, device):
+        """Two-level hierarchical correction on a factor-x coarser lattice cage
+        (use_multigrid with mg_mode "hpbd", mg.py:192-245, 1064-1079)."""
+        cfg = self.cfg
+        mode = str(cfg.get("mg_mode", "hpbd")).lower()
+        if mode != "hpbd":
+            raise NotImplementedError(f"mg_mode={mode!r}: only the hpbd V-cycle is on the HIP path")
+        w = self.workload
+        arap = find_constraint(cfg, "arap")
+        pin_mask = None
+        if self.scene.pin_vids is not None:
+            pin_mask = np.zeros(self.scene.nv, np.float32)
+            pin_mask[self.scene.pin_vids] = 1.0
+        mass = self.engine.get_rest()[2] if hasattr(self.engine, "get_rest") else None
+        comp = None
+        if w is not None and w.info.get("muscle_id") is not None:
+            comp = w.info["muscle_id"]
+        self.coarse = multigrid.build_coarse_level(
+            self.scene.pos0, self.scene.tets, mass, pin_mask=pin_mask,
+            factor=float(cfg.get("mg_coarse_factor", 4.0)), iters=int(cfg.get("mg_coarse_iters", 15)),
+            stiffness=float(arap["stiffness"]), pin_frac=float(cfg.get("mg_coarse_pin_frac", 0.5)), comp=comp,
+            skip=w.is_fascia if w is not None else None)
+        cs = multigrid.coarse_scene(self.coarse, self.scene.params)
+        self.coarse_engine = (engine_cls(cs, device=device, strategy=1) if engine_cls is XpbdEngine
+                              else engine_cls(cs))
+        # the coarse correction is applied kinematically (also shifted into the
+        # previous positions) unless mg_kinematic_correction is false: with the
+        # reference's dynamic form the prolonged shift enters the velocity and the
+        # pin-driven workloads here gain energy every substep
+        self.engine.set_coarse_level(self.coarse_engine, self.coarse,
+                                     kinematic=bool(cfg.get("mg_kinematic_correction", True)))
 
     # kinematic drive ---------------------------------------------------------
     def load_frame_targets(self, frame: int) -> None:
@@ -169,6 +202,8 @@ class MultiGridSim:
         if self.collision_solvers or self.collision_solvers_iter:
             e = self.engine
             e.integrate()
+            if self.coarse is not None:
+                e.vcycle()
             for it in range(self.fine_iters):
                 e.iterate(1)
                 for fn in self.collision_solvers_iter:

@@ -1,0 +1,99 @@
+"""Learned warm start on the GPU: the in-library feature -> MLP -> lambda0 path
+equals the same computation through the standalone MLP kernels, and the
+per-frame fallback returns exactly the plain zero-lambda frame."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from rlmusclev2_amd import sims, warmstart as ws, workloads
+from rlmusclev2_amd.mlp import NIN, Mlp
+
+pytestmark = pytest.mark.gpu
+CFG = Path(__file__).resolve().parents[1] / "rlmusclev2_amd" / "configs"
+
+
+def _sim(seed=1):
+    cfg = json.loads((CFG / "humanv2_scale.json").read_text())
+    cfg["synthetic"]["sdf_res"] = 32
+    w = workloads.humanv2_scale(seed, n_bones=1, per_bone=4, target_tets=6000, fascia_tets=1000, sdf_res=32)
+    return cfg, w, sims.MultiGridSim(cfg, workload=w, strategy=1)
+
+
+def _norm(rng):
+    return ws.Normalizer(rng.normal(0, 0.1, NIN).astype(np.float32),
+                         rng.uniform(0.5, 2.0, NIN).astype(np.float32),
+                         np.array([0.5, 0.7, 1.0, 1.0], np.float32))
+
+
+def test_network_lambda0_matches_standalone_mlp():
+    import torch
+    cfg, w, sim = _sim()
+    e = sim.engine
+    rng = np.random.default_rng(0)
+    net, norm = Mlp(seed=3), _norm(rng)
+    ws.install(e, net, norm)
+    sim.step_frame(1)
+    sim.load_frame_targets(2)
+    e.apply_pintotarget()
+    e.integrate()
+    ne = e.stats()["n_elems"]
+    lam_prev = torch.tensor(rng.normal(0, 1e-6, (ne, 4)).astype(np.float32), device="cuda")
+    lam_prev[:, 3] = 0
+    e.set_lambda(lam_prev)
+    feat = torch.empty((ne, NIN), dtype=torch.float32, device="cuda")
+    e.features(feat, normalized=True)
+    raw = torch.empty_like(feat)
+    e.features(raw, normalized=False)
+    mean, inv = torch.tensor(norm.mean, device="cuda"), torch.tensor(norm.invstd, device="cuda")
+    torch.testing.assert_close(feat, (raw - mean) * inv, rtol=1e-5, atol=1e-5)
+    y = net.forward(feat.contiguous())
+    gain = (1.0 + y[:, :3] * torch.tensor(norm.oscale[:3], device="cuda")).clamp(0.0, 2.0)
+    want = lam_prev.clone()
+    want[:, :3] *= gain
+    want[:, 3] = 0
+    e.set_lambda_mode(ws.MODE_NET)
+    e.warm_start()
+    got = torch.empty_like(lam_prev)
+    e.get_lambda(got)
+    e.set_lambda_mode(ws.MODE_PLAIN)
+    torch.testing.assert_close(got, want, rtol=1e-5, atol=1e-12)
+
+
+def test_fallback_reruns_the_plain_frame_exactly():
+    cfg, w, sim = _sim()
+    _, _, plain = _sim()
+    rng = np.random.default_rng(1)
+    net, norm = Mlp(seed=5), _norm(rng)
+    norm.oscale[:3] *= 1e4          # a useless network: gains saturate at 0 or 2
+    ws.install(sim.engine, net, norm)
+    st = ws.WarmStartStepper(sim, k_iters=3, audit_every=1)
+    for f in range(1, 4):
+        info = st.step_frame(f)
+        plain.step_frame(f)
+        print(f, info["fallback"], info["residual"], info["plain"])
+        assert info["fallback"]
+        np.testing.assert_array_equal(sim.positions_np(), plain.positions_np())
+    assert st.fallbacks == 3
+
+
+def test_trained_network_reaches_plain_residual_in_fewer_iterations():
+    """Train on plain trajectories (GPU kernels), then on a new randomized
+    trajectory count the iterations each start needs to reach the plain
+    10-iteration residual of the same substep."""
+    cfg, w, sim = _sim(seed=2)
+    ds = ws.collect(sim, n_frames=30, seed=0, samples_per_substep=2048)
+    norm = ws.fit_normalizer(ds)
+    net, hist = ws.train(ds, norm, epochs=15, batch=4096, lr=2e-3, seed=0)
+    print("dataset", ds.meta, "last epoch", hist[-1])
+    assert hist[-1]["val_loss"] < hist[-1]["prev_loss"] < hist[-1]["zero_loss"]
+    _, _, ev = _sim(seed=2)
+    ws.install(ev.engine, net, norm)
+    out = ws.compare(ev, n_frames=4, seed=7, modes=(ws.MODE_PREV, ws.MODE_NET))
+    mean = {}
+    for m, v in out.items():
+        vv = [11 if x is None else x for x in v]
+        mean[m] = float(np.mean(vv))
+        print("mode", m, "mean iterations", mean[m], "misses", sum(x is None for x in v), np.bincount(vv).tolist())
+    assert mean[ws.MODE_NET] <= 10.0 / 1.5       # >= 1.5x fewer iterations than plain
