@@ -28,6 +28,7 @@ WORKLOADS = {
     "snh_aniso": ("humanv2_snh_aniso.json", {}),
     "cantilever": ("cantilever.json", {}),
     "batch": ("batch512.json", {}),
+    "c5": ("humanv2_c5.json", {}),
 }
 
 
@@ -83,6 +84,82 @@ def cpu_baseline(cfg, w, threads, budget_s=20.0):
     return {"value": round(ms_frame, 3), "unit": "ms/frame", "cores": threads, "kind": "port",
             "sample": f"{n} substep(s) x {ref.fine_iters} Jacobi iterations of frame 1 of the same scene, "
                       f"{dt:.1f} s wall on {threads} OpenMP threads, scaled x{ref.nsubsteps / n:g} to one frame"}
+
+
+def warm_start_leg(cfg, w, tuning, out):
+    """Learned warm start on the same mesh: collect plain-solver frames under
+    randomized bone motion, train the per-tet MLP with the HIP kernels, count
+    the iterations each start needs to reach the plain 10-iteration residual,
+    then time plain frames against warm-started frames with the per-frame
+    fallback (audited every few frames)."""
+    import numpy as np
+    import torch
+    from rlmusclev2_amd import sims, warmstart as ws
+    wc = cfg.get("warm_start", {})
+    tuning = dict(tuning, strategy=tuning["strategy"] if tuning["strategy"] in (1, 2) else 1)
+    sim = sims.MultiGridSim(cfg, workload=w, **tuning)
+    t0 = time.perf_counter()
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    ds = ws.collect(sim, n_frames=int(wc.get("train_frames", 2000)), seed=0,
+                    samples_per_substep=int(wc.get("samples_per_substep", 4096)),
+                    conv_iters=int(wc.get("conv_iters", 10)), log=log)
+    t1 = time.perf_counter()
+    norm = ws.fit_normalizer(ds)
+    net, hist = ws.train(ds, norm, epochs=int(wc.get("epochs", 3)), batch=16384, lr=2e-3, seed=0,
+                         log=lambda h: log(f"train: {h}"))
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    n_samples = int(ds.X.shape[0])
+    del ds
+    ev = sims.MultiGridSim(cfg, workload=w, **tuning)
+    ws.install(ev.engine, net, norm)
+    log("evaluating iterations to the plain residual")
+    cmp = ws.compare(ev, n_frames=int(wc.get("eval_frames", 5)), seed=11,
+                     modes=(ws.MODE_PREV, ws.MODE_NET), conv_iters=int(wc.get("conv_iters", 10)))
+    iters = {}
+    for m, v in cmp.items():
+        vv = np.array([11 if x is None else x for x in v])
+        name = {ws.MODE_PREV: "previous_lambda", ws.MODE_NET: "network"}.get(m, str(m))
+        iters[name] = {"mean": round(float(vv.mean()), 3), "p90": float(np.percentile(vv, 90)),
+                       "misses": int(sum(x is None for x in v)), "substeps": len(v)}
+    vnet = np.array([11 if x is None else x for x in cmp[ws.MODE_NET]])
+    K = int(min(10, max(1, np.ceil(np.percentile(vnet, 90)))))
+    # timing: plain frames vs warm frames (graph replay), same trajectory start
+    nf = int(wc.get("timed_frames", 20))
+    ws.randomize_motion(w, np.random.default_rng(13))
+    tim = {}
+    for label in ("plain", "warm"):
+        s2 = sims.MultiGridSim(cfg, workload=w, **tuning)
+        ws.install(s2.engine, net, norm)
+        st = ws.WarmStartStepper(s2, K, audit_every=int(wc.get("audit_every", 8))) if label == "warm" else None
+        for f in range(1, 3):
+            st.step_frame(f) if st else s2.step_frame(f)
+        s2.engine.synchronize()
+        if st:
+            st.frames = st.fallbacks = st.audits = 0
+        ta = time.perf_counter()
+        for f in range(3, 3 + nf):
+            st.step_frame(f) if st else s2.step_frame(f)
+        s2.engine.synchronize()
+        tim[label] = (time.perf_counter() - ta) * 1e3 / nf
+        if st:
+            tim["fallbacks"] = st.fallbacks
+            tim["audits"] = st.audits
+            tim["residual"] = s2.residuals()
+            tim["plain_residual_last_audit"] = st.ref
+        else:
+            tim["plain_residual"] = s2.residuals()
+        s2.engine.close()
+    out["warm_start"] = {
+        "train_frames": int(wc.get("train_frames", 2000)), "samples": n_samples,
+        "collect_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2), "last_epoch": hist[-1],
+        "iterations_to_plain10_residual": iters, "k_iters": K,
+        "ms_per_frame_plain": round(tim["plain"], 4), "ms_per_frame_warm_with_fallback": round(tim["warm"], 4),
+        "speedup": round(tim["plain"] / tim["warm"], 3), "timed_frames": nf, "fallbacks": tim["fallbacks"],
+        "audits": tim["audits"], "residual_warm_end": tim["residual"], "residual_plain_end": tim["plain_residual"],
+    }
 
 
 def main():
@@ -187,6 +264,11 @@ def main():
         "residual_rms": res,
         "finite": finite,
     }
+    if sim.scene.dcd is not None:
+        out["dcd"] = dict(sim.engine.dcd_stats(), surface_tris=int(sim.scene.dcd.tris.shape[0]))
+    if "warm_start" in cfg:
+        sim.engine.close()
+        warm_start_leg(cfg, w, tuning, out)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(cfg, w, threads)

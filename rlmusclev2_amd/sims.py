@@ -14,7 +14,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import builder, multigrid, workloads
+from . import builder, collision, multigrid, workloads
 from .builder import ConsTable
 from .config import find_constraint
 from .engine import XpbdEngine
@@ -112,9 +112,29 @@ def scene_from_config(cfg: dict, w: workloads.Workload) -> Scene:
                            contact_offset=float(coll.get("contact_offset", 1e-5)),
                            max_push=float(coll.get("max_push", -1.0)), coll_w=m,
                            vertex_group=vg, group_bone_off=goff)
+    dcd = None
+    sc = cfg.get("demo", {}).get("self_collision", {})
+    if sc.get("enabled", False):
+        method = str(sc.get("method", "dcd")).lower()
+        if method != "dcd":
+            raise NotImplementedError(f"self_collision.method={method!r}: only the DCD pass is on the HIP path")
+        # surface triangles of the selected tets (humanV2.py:820-866 passes sim.surf_flat)
+        which = str(sc.get("surfaces", "all")).lower()
+        sel = np.ones(len(tets), bool)
+        if which in ("fascia", "muscle"):
+            if w.is_fascia is None:
+                raise ValueError(f"self_collision.surfaces={which!r} needs a fascia mask on the workload")
+            fas = w.is_fascia[tets].all(1)
+            sel = fas if which == "fascia" else ~fas
+        p = collision.DcdParams(thickness=float(sc.get("thickness", 1e-3)), multiplier=float(sc.get("multiplier", 0.3)),
+                                friction_static=float(sc.get("mu_s", 0.5)), friction_dynamic=float(sc.get("mu_d", 0.4)),
+                                normal_cull=float(sc.get("normal_cull", 1.0)), cell_size=float(sc.get("cell_size", 0.0)),
+                                iters=int(sc.get("iters", 2)), rebuild_substeps=bool(sc.get("rebuild_substeps", False)),
+                                max_pairs=int(sc.get("max_pairs", 1 << 22)))
+        dcd = collision.DcdSpec(tris=collision.surface_tris(tets[sel]), group=None, params=p)
     return Scene(pos0=pos0, tets=tets, density=density, cons=cons, params=params, v_fiber_dir=vf,
                  tendonmask=tm, stopped=w.stopped, pin_vids=pin_vids, pin_active=pin_active, sdf=sdf,
-                 meta=dict(workload=w.name, constraint_configs=ccfg))
+                 dcd=dcd, meta=dict(workload=w.name, constraint_configs=ccfg))
 
 
 class MultiGridSim:

@@ -81,7 +81,7 @@ class Dataset:
 
 
 def collect(sim, n_frames: int, seed: int = 0, samples_per_substep: int = 8192, conv_iters: int = 10,
-            substep_stride: int = 2) -> Dataset:
+            substep_stride: int = 2, log=None) -> Dataset:
     """Plain-solver trajectory with randomized activations and bone motion.
 
     At every substep, after integrate, a lambda-carrying solve of
@@ -134,6 +134,8 @@ def collect(sim, n_frames: int, seed: int = 0, samples_per_substep: int = 8192, 
             e.collide()
             e.update_velocities()
             sub += 1
+        if log and f % 100 == 0:
+            log(f"collect: frame {f}/{n_frames}, {time.perf_counter() - t0:.1f} s")
     torch.cuda.synchronize()
     return Dataset(torch.cat(Xs), torch.cat(Ps), torch.cat(Ts),
                    dict(frames=n_frames, samples=int(sum(x.shape[0] for x in Xs)),
