@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--strategy", type=int, default=-1, help="-1 = autotune, 0 = patch, 1 = flat, 2 = block")
     ap.add_argument("--block-tets", type=int, default=1024, help="block strategy: tets per workgroup")
     ap.add_argument("--block-threads", type=int, default=256, help="block strategy: threads per workgroup")
+    ap.add_argument("--ws-train-frames", type=int, default=0, help="warm-start leg: training frames (0 = config)")
+    ap.add_argument("--ws-eval-frames", type=int, default=0, help="warm-start leg: evaluation frames (0 = config)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
@@ -117,19 +119,23 @@ def warm_start_leg(cfg, w, tuning, out):
     ws.install(ev.engine, net, norm)
     log("evaluating iterations to the plain residual")
     cmp = ws.compare(ev, n_frames=int(wc.get("eval_frames", 5)), seed=11,
-                     modes=(ws.MODE_PREV, ws.MODE_NET), conv_iters=int(wc.get("conv_iters", 10)))
+                     modes=(ws.MODE_PREV, ws.MODE_NET), conv_iters=int(wc.get("conv_iters", 10)), log=log)
     iters = {}
     for m, v in cmp.items():
         vv = np.array([11 if x is None else x for x in v])
         name = {ws.MODE_PREV: "previous_lambda", ws.MODE_NET: "network"}.get(m, str(m))
         iters[name] = {"mean": round(float(vv.mean()), 3), "p90": float(np.percentile(vv, 90)),
                        "misses": int(sum(x is None for x in v)), "substeps": len(v)}
-    vnet = np.array([11 if x is None else x for x in cmp[ws.MODE_NET]])
-    K = int(min(10, max(1, np.ceil(np.percentile(vnet, 90)))))
+    # K: reached by 80 % of the evaluated substeps after the first frame (whose
+    # first substeps start from lambda_prev = 0)
+    nsub = ev.nsubsteps
+    vnet = np.array([11 if x is None else x for x in cmp[ws.MODE_NET][nsub:]] or [10])
+    K = int(min(10, max(1, np.ceil(np.percentile(vnet, 80)))))
     # timing: plain frames vs warm frames (graph replay), same trajectory start
     nf = int(wc.get("timed_frames", 20))
     ws.randomize_motion(w, np.random.default_rng(13))
     tim = {}
+    log(f"timing plain and warm frames (K={K})")
     for label in ("plain", "warm"):
         s2 = sims.MultiGridSim(cfg, workload=w, **tuning)
         ws.install(s2.engine, net, norm)
@@ -267,6 +273,10 @@ def main():
     if sim.scene.dcd is not None:
         out["dcd"] = dict(sim.engine.dcd_stats(), surface_tris=int(sim.scene.dcd.tris.shape[0]))
     if "warm_start" in cfg:
+        if args.ws_train_frames:
+            cfg["warm_start"]["train_frames"] = args.ws_train_frames
+        if args.ws_eval_frames:
+            cfg["warm_start"]["eval_frames"] = args.ws_eval_frames
         sim.engine.close()
         warm_start_leg(cfg, w, tuning, out)
     if not args.no_cpu_baseline:

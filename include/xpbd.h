@@ -209,6 +209,10 @@ int32_t xpbd_dcd_solve(xpbd_ctx *ctx, int32_t rebuild);
 int32_t xpbd_dcd_pairs(xpbd_ctx *ctx, int32_t *out, int64_t cap, int64_t *n_total);
 /* UdcfDcdCollider.stats (solver.py:442-457): raw pair count, contacts of the last pass, cell */
 int32_t xpbd_dcd_stats(xpbd_ctx *ctx, int64_t *n_pairs, int64_t *n_contacts, float *cell);
+/* broadphase limits hit at the last rebuild (0 and 0 on a valid mesh): triangles left out
+ * (bounding sphere too large for the capped cell, or non-finite centroid) and triangles
+ * whose candidate list was truncated.  No reference counterpart. */
+int32_t xpbd_dcd_overflow(xpbd_ctx *ctx, int64_t *left_out, int64_t *truncated);
 
 /* ---- learned warm start: per-tet MLP (no reference counterpart) ------- */
 /* 24 features (22 used) -> 64 -> 64 -> 4 outputs (3 used), ReLU hidden layers; layers 1-2
@@ -234,13 +238,19 @@ int32_t xpbd_mlp_backward(const xpbd_mlp_dev *net, const float *X, const float *
                           const float *dY, int64_t n, float *grad, float *scratch, int64_t scratch_floats,
                           void *stream);
 
-/* multiplier modes (flat strategy): 0 = plain zero-lambda Jacobi (the reference GPU path,
- * solve_kernels.py:271-281 with clear() every iteration, mg.py:1348); 1 = lambda carried across
- * the iterations of a substep from 0; 2 = learned warm start: after integrate the network
- * predicts, per tet and constraint type, a gain on the previous substep's final multipliers,
- * lambda0 = lambda_prev * clamp(1 + out_scale * y, 0, 2); the first iteration applies lambda0
- * as an impulse along the constraint gradients, the rest carry lambda; 3 = the same with
- * lambda0 = lambda_prev (no network).  No reference counterpart. */
+/* multiplier modes (flat and block strategies; no reference counterpart):
+ * 0 = plain zero-lambda Jacobi (the reference GPU path, solve_kernels.py:271-281 with clear()
+ *     every iteration, mg.py:1348);
+ * 1 = lambda carried across the iterations of a substep from 0 (a reference solve: at the end
+ *     of the substep its multipliers become the base of the next warm start);
+ * 2 = learned warm start: the network predicts, per tet and constraint type, a gain on the base,
+ *     lambda0 = base * clamp(1 + out_scale * y, 0, 2), which also becomes the next base; the
+ *     first iteration applies lambda0 as an impulse along the constraint gradients, the rest
+ *     carry lambda;
+ * 3 = the same with lambda0 = base (no network);
+ * 4 = frames with a schedule: the first substep of each xpbd_step_frame is mode 1 with the
+ *     reference iteration count (xpbd_set_reference_iters, default 10), the others are mode 2
+ *     with the frame's iteration count (xpbd_substep and the phase calls use mode 2). */
 int32_t xpbd_set_lambda_mode(xpbd_ctx *ctx, int32_t mode);
 /* network weights (host fp32; W1, W2 stored as bf16), feature standardisation (IN each) and
  * per-output scales (OUT) */
@@ -253,7 +263,8 @@ int32_t xpbd_warm_start(xpbd_ctx *ctx);
 int32_t xpbd_features(xpbd_ctx *ctx, float *dev_out, int32_t normalized);
 /* per-element multipliers (n_elems x 4 floats, device pointers) */
 int32_t xpbd_get_lambda(xpbd_ctx *ctx, float *dev_out);
-int32_t xpbd_set_lambda(xpbd_ctx *ctx, const float *dev_in);
+int32_t xpbd_set_lambda(xpbd_ctx *ctx, const float *dev_in);   /* sets the carried multipliers and the base */
+int32_t xpbd_set_reference_iters(xpbd_ctx *ctx, int32_t n);
 /* save (1) / restore (0) positions, previous positions and velocities on the device */
 int32_t xpbd_snapshot(xpbd_ctx *ctx, int32_t save);
 

@@ -114,6 +114,7 @@ SIGNATURES = {
     "xpbd_dcd_solve": [_P, _I],
     "xpbd_dcd_pairs": [_P, _P, C.c_int64, _P],
     "xpbd_dcd_stats": [_P, _P, _P, _P],
+    "xpbd_dcd_overflow": [_P, _P, _P],
     "xpbd_mlp_forward": [C.POINTER(XpbdMlpDev), _P, C.c_int64, _P, _P, _P, _P],
     "xpbd_mlp_grad_size": [],
     "xpbd_mlp_backward": [C.POINTER(XpbdMlpDev), _P, _P, _P, _P, C.c_int64, _P, _P, C.c_int64, _P],
@@ -125,6 +126,7 @@ SIGNATURES = {
     "xpbd_features": [_P, _P, _I],
     "xpbd_get_lambda": [_P, _P],
     "xpbd_set_lambda": [_P, _P],
+    "xpbd_set_reference_iters": [_P, _I],
     "xpbd_snapshot": [_P, _I],
     "xpbd_kernel_timing": [_P, _I],
     "xpbd_get_kernel_timing": [_P, _P, _P],

@@ -11,6 +11,7 @@
 // (src/muscle/constraint/solve_kernels.py, src/muscle/engine.py:689-698,
 // src/muscle/mg/kernel.py:212-226).
 #include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
 
 #include <algorithm>
 #include <cmath>
@@ -720,11 +721,8 @@ __global__ __launch_bounds__(256) void k_block_shared(IterArgs a, BlockArgs b, c
 // F - I (9), fibre direction (3), fibre target length (1), previous-substep
 // multipliers (3), log10 rest volume, det F - 1,
 // |F|^2 - 3, log10 mean inverse mass, log10 ARAP and volume compliances.
-__global__ void k_features(IterArgs a, const float4 *__restrict__ X, float *__restrict__ out,
-                           const float *__restrict__ fmean, const float *__restrict__ finv,
-                           const float *__restrict__ oscale, int normalize) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= a.nptet) return;
+__device__ __forceinline__ void elem_features(const IterArgs &a, int64_t g, const float4 *__restrict__ X,
+                                              const float4 *__restrict__ base, float f[24]) {
     const int4 gi = a.gidx[g];
     const uint32_t meta = a.meta[g];
     const uint32_t mask = (meta >> 16) & 15u;
@@ -736,11 +734,10 @@ __global__ void k_features(IterArgs a, const float4 *__restrict__ X, float *__re
     const float Ds[9] = {c0.x, c1.x, c2.x, c0.y, c1.y, c2.y, c0.z, c1.z, c2.z};
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) F[3 * i + j] = Ds[3 * i] * B[j] + Ds[3 * i + 1] * B[3 + j] + Ds[3 * i + 2] * B[6 + j];
-    float f[24];
     for (int i = 0; i < 9; ++i) f[i] = F[i] - ((i % 4 == 0) ? 1.0f : 0.0f);
     const float4 fb = (mask & M_ANISO) ? a.fib[g] : make_float4(0.f, 0.f, 0.f, 1.f);
     f[9] = fb.x; f[10] = fb.y; f[11] = fb.z; f[12] = fb.w;
-    const float4 lp = a.lam ? a.lam[g] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 lp = base[g];
     f[13] = lp.x; f[14] = lp.y; f[15] = lp.z;
     const float detB = det3(B);
     f[16] = log10f(1.0f / (6.0f * fabsf(detB) + 1e-30f) + 1e-30f);
@@ -753,21 +750,153 @@ __global__ void k_features(IterArgs a, const float4 *__restrict__ X, float *__re
     f[21] = log10f(dc.a_vol + 1e-30f);
     f[22] = 0.f;
     f[23] = 0.f;
+}
+
+__global__ void k_features(IterArgs a, const float4 *__restrict__ X, const float4 *__restrict__ base,
+                           float *__restrict__ out, const float *__restrict__ fmean, const float *__restrict__ finv,
+                           int normalize) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.nptet) return;
+    float f[24];
+    elem_features(a, g, X, base, f);
     float *o = out + g * 24;
     for (int i = 0; i < 24; ++i) o[i] = normalize ? (f[i] - fmean[i]) * finv[i] : f[i];
 }
 
-// lambda0 = previous-substep multipliers, each scaled by the network's
-// per-tet, per-type gain 1 + out_scale * y, clamped to [0, 2]
-__global__ void k_lam0(int64_t n, const float *__restrict__ Y, const float *__restrict__ oscale, float4 *lam) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
-    const float4 p = lam[g];
-    const float s0 = fminf(fmaxf(1.0f + Y[4 * g] * oscale[0], 0.0f), 2.0f);
-    const float s1 = fminf(fmaxf(1.0f + Y[4 * g + 1] * oscale[1], 0.0f), 2.0f);
-    const float s2 = fminf(fmaxf(1.0f + Y[4 * g + 2] * oscale[2], 0.0f), 2.0f);
-    lam[g] = make_float4(p.x * s0, p.y * s1, p.z * s2, 0.f);
+// ---------------------------------------------------------------------------
+// fused warm start: features -> 24x64 -> 64x64 (BF16 MFMA 32x32x16, FP32
+// accumulate, ReLU) -> 64x4 (VALU) -> lambda0 = lambda_prev * clamp(1 + s y).
+// One wavefront per 32 elements; the operand layout is the one of k_mlp_fwd
+// (row l%32, k-half l/32; D column l%32, rows (i/4)*8 + 4h + i%4).  Weights,
+// biases and the standardisation are staged in LDS once per workgroup, which
+// then walks its tiles grid-stride.
+// ---------------------------------------------------------------------------
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 ws_bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float ws_f32x16;
+
+struct WarmNet {
+    const uint16_t *W1, *W2;      // bf16 bits, row-major (out, in)
+    const float *b1, *b2, *W3, *b3, *fmean, *finv, *oscale;
+};
+
+// base: lambda_prev in, lambda0 out (the base of the next substep); lam: lambda0 out
+__global__ __launch_bounds__(256) void k_warm_start(IterArgs a, WarmNet net, const float4 *__restrict__ X,
+                                                    float4 *base, float4 *__restrict__ lam) {
+    constexpr int NI = XPBD_MLP_IN, NH = XPBD_MLP_HIDDEN, NO = XPBD_MLP_OUT;
+    __shared__ uint16_t sW1[NH * NI], sW2[NH * NH];
+    __shared__ float sb1[NH], sb2[NH], sW3[NO * NH], sb3[NO], smu[NI], sinv[NI], sos[NO];
+    __shared__ float sH[4][32 * NH];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < NH * NI; i += 256) sW1[i] = net.W1[i];
+    for (int i = tid; i < NH * NH; i += 256) sW2[i] = net.W2[i];
+    for (int i = tid; i < NO * NH; i += 256) sW3[i] = net.W3[i];
+    if (tid < NH) { sb1[tid] = net.b1[tid]; sb2[tid] = net.b2[tid]; }
+    if (tid < NI) { smu[tid] = net.fmean[tid]; sinv[tid] = net.finv[tid]; }
+    if (tid < NO) { sb3[tid] = net.b3[tid]; sos[tid] = net.oscale[tid]; }
+    __syncthreads();
+    const int wave = tid >> 6, l = tid & 63, col = l & 31, h = l >> 5;
+    float *H = sH[wave];
+    const int64_t ntile = (a.nptet + 127) / 128;
+    for (int64_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        const int64_t row = tile * 128 + wave * 32 + col;
+        const bool rv = row < a.nptet;
+        float f[24];
+        if (rv) {
+            elem_features(a, row, X, base, f);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 24; ++i) f[i] = 0.f;
+        }
+        // layer 1: K = 24 padded to 32, two k-steps of 16; this lane holds k = 16s + 8h + i
+        ws_f32x16 acc[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc[t][i] = 0.f;
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            ws_bf16x8 av;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int k = 16 * s + 8 * h + i;
+                // f[k] with k = 16s + 8h + i; only s = 1, h = 1 falls in the padding
+                const float fk = s == 0 ? (h ? f[8 + i] : f[i]) : (h ? 0.f : f[16 + i]);
+                av[i] = (__bf16)((rv && k < NI) ? (fk - smu[k]) * sinv[k] : 0.f);
+            }
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                ws_bf16x8 bv;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int k = 16 * s + 8 * h + i;
+                    bv[i] = k < NI ? __builtin_bit_cast(__bf16, sW1[(32 * t + col) * NI + k]) : (__bf16)0.f;
+                }
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[t], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int c = 32 * t + col;
+                H[((i >> 2) * 8 + 4 * h + (i & 3)) * NH + c] = fmaxf(acc[t][i] + sb1[c], 0.f);
+            }
+        __syncthreads();
+        // layer 2: K = 64, four k-steps
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc[t][i] = 0.f;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            ws_bf16x8 av;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) av[i] = (__bf16)H[col * NH + 16 * s + 8 * h + i];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                ws_bf16x8 bv;
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    bv[i] = __builtin_bit_cast(__bf16, sW2[(32 * t + col) * NH + 16 * s + 8 * h + i]);
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[t], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int c = 32 * t + col;
+                H[((i >> 2) * 8 + 4 * h + (i & 3)) * NH + c] = fmaxf(acc[t][i] + sb2[c], 0.f);
+            }
+        __syncthreads();
+        // layer 3 on the vector ALUs: this lane computes outputs 2h and 2h+1 of its row
+        if (rv) {
+            float y[2];
+#pragma unroll
+            for (int oo = 0; oo < 2; ++oo) {
+                const int o = 2 * h + oo;
+                float v = sb3[o];
+                for (int k = 0; k < NH; ++k) v += H[col * NH + k] * sW3[o * NH + k];
+                y[oo] = v;
+            }
+            const float4 lp = base[row];
+            float2 v;
+            if (h == 0) {
+                const float s0 = fminf(fmaxf(1.0f + y[0] * sos[0], 0.0f), 2.0f);
+                const float s1 = fminf(fmaxf(1.0f + y[1] * sos[1], 0.0f), 2.0f);
+                v = make_float2(lp.x * s0, lp.y * s1);
+            } else {
+                const float s2 = fminf(fmaxf(1.0f + y[0] * sos[2], 0.0f), 2.0f);
+                v = make_float2(lp.z * s2, 0.0f);
+            }
+            reinterpret_cast<float2 *>(lam + row)[h] = v;
+            reinterpret_cast<float2 *>(base + row)[h] = v;
+        }
+        __syncthreads();
+    }
 }
+
 
 // ---------------------------------------------------------------------------
 // per-vertex kernels
@@ -1071,6 +1200,11 @@ __global__ void k_zero_i32(int n, int32_t *p) {
     if (i < n) p[i] = 0;
 }
 
+__global__ void k_copy4(int64_t n, float4 *__restrict__ dst, const float4 *__restrict__ src) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 __global__ void k_zero4(int64_t n, float4 *p) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1194,7 +1328,8 @@ __global__ void k_dcd_tridata(DcdDev d, const float4 *X) {
     const V3 ce = V3{s.x / 3.0f, s.y / 3.0f, s.z / 3.0f};
     const V3 e0 = a - ce, e1 = b - ce, e2 = c - ce;
     const float r0 = sqrtf(dot(e0, e0)), r1 = sqrtf(dot(e1, e1)), r2 = sqrtf(dot(e2, e2));
-    const float r = fmaxf(fmaxf(r0, r1), r2);
+    float r = fmaxf(fmaxf(r0, r1), r2);
+    if (!(isfinite(r) && isfinite(ce.x) && isfinite(ce.y) && isfinite(ce.z))) r = INFINITY;
     d.cr[i] = make_float4(ce.x, ce.y, ce.z, r);
     const V3 n = cross(b - a, c - a);
     const float ln = sqrtf(dot(n, n));
@@ -1206,7 +1341,8 @@ __global__ void k_dcd_cellsize(DcdDev d) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const float rmax = __uint_as_float(*d.rmax_bits);
     // 1% margin keeps float rounding of c/cell from separating a valid pair by two cells
-    const float cell = 1.01f * fmaxf(fmaxf(d.mean_edge2, 2.0f * rmax + d.thickness), 1.0e-4f);
+    const float reach = fminf(2.0f * rmax + d.thickness, DCD_CELL_CAP * d.mean_edge2);
+    const float cell = 1.01f * fmaxf(fmaxf(d.mean_edge2, reach), 1.0e-4f);
     d.scal[0] = cell;
     d.scal[1] = 1.0f / cell;
     d.scal[2] = rmax;
@@ -1217,6 +1353,12 @@ __global__ void k_dcd_keys(DcdDev d) {
     if (i >= d.ntris) return;
     const float inv = d.scal[1];
     const float4 c = d.cr[i];
+    // a triangle whose sphere test could reach past the adjacent cells is left out
+    if (!(2.0f * c.w + d.thickness <= d.scal[0] * (1.0f / 1.01f))) {
+        d.cell[i] = make_int4(0, 0, 0, -1);
+        atomicAdd(&d.overflow[0], 1u);
+        return;
+    }
     const int x = (int)floorf(c.x * inv), y = (int)floorf(c.y * inv), z = (int)floorf(c.z * inv);
     const uint32_t h = dcd_hash(x, y, z, d.tbits);
     d.cell[i] = make_int4(x, y, z, (int)h);
@@ -1226,6 +1368,7 @@ __global__ void k_dcd_keys(DcdDev d) {
 __global__ void k_dcd_scatter(DcdDev d) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.ntris) return;
+    if (d.cell[i].w < 0) return;
     const uint32_t h = (uint32_t)d.cell[i].w;
     const uint32_t k = atomicAdd(&d.bucket_fill[h], 1u);
     d.sorted[d.bucket_off[h] + k] = i;
@@ -1257,16 +1400,21 @@ __global__ void k_dcd_pairs(DcdDev d) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.ntris) return;
     const int4 ci4 = d.cell[i];
+    uint32_t cnt = 0;
+    if (ci4.w < 0) {  // left out
+        if (!FILL) d.pair_cnt[i] = 0;
+        return;
+    }
     const int4 ti = d.tris[i];
     const float4 ci = d.cr[i];
     const float4 ni = d.nrm[i];
     const int gi = d.group ? d.group[i] : 0;
-    uint32_t cnt = 0;
     uint64_t base = 0;
     if (FILL) base = d.pair_off[i];
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
+    bool full = false;
+    for (int dz = -1; dz <= 1 && !full; ++dz)
+        for (int dy = -1; dy <= 1 && !full; ++dy)
+            for (int dx = -1; dx <= 1 && !full; ++dx) {
                 const int x = ci4.x + dx, y = ci4.y + dy, z = ci4.z + dz;
                 const uint32_t h = dcd_hash(x, y, z, d.tbits);
                 const uint32_t e = d.bucket_off[h + 1];
@@ -1276,6 +1424,10 @@ __global__ void k_dcd_pairs(DcdDev d) {
                     const int4 cj = d.cell[j];
                     if (cj.x != x || cj.y != y || cj.z != z) continue;  // visit j only from its own cell
                     if (!dcd_emit(d, i, j, ti, ci, ni, gi)) continue;
+                    if (cnt == DCD_MAX_PER_TRI) {  // same stop in both passes (same bucket order)
+                        full = true;
+                        break;
+                    }
                     if (FILL) {
                         const uint64_t o = base + cnt;
                         if ((int64_t)o < d.max_pairs) d.pairs[o] = make_int2(i, j);
@@ -1285,6 +1437,7 @@ __global__ void k_dcd_pairs(DcdDev d) {
             }
     if (!FILL) {
         d.pair_cnt[i] = cnt;
+        if (full) atomicAdd(&d.overflow[1], 1u);
         return;
     }
     // sort this triangle's segment by j (insertion sort; segments are short)
@@ -1677,8 +1830,10 @@ struct xpbd_ctx {
     // learned warm start
     int lam_mode = 0;             // 0 plain, 1 carry, 2 network warm start, 3 previous-substep warm start
     bool impulse_pending = false;
-    float4 *LAMB = nullptr;
-    float *FEAT = nullptr, *MLPY = nullptr;
+    int eff_mode = 0;             // mode of the current substep (mode 4 schedules 1 and 2)
+    int ref_iters = 10;           // mode 4: iterations of the reference substep
+    float4 *LAMB = nullptr;       // carried multipliers of the current substep
+    float4 *LAMB0 = nullptr;      // base of the next warm start (lambda0 of the last substep, or a reference solve)
     uint16_t *mW1 = nullptr, *mW2 = nullptr;
     float *mb1 = nullptr, *mb2 = nullptr, *mW3 = nullptr, *mb3 = nullptr;
     float *fmean = nullptr, *finv = nullptr, *oscale = nullptr;
@@ -1698,7 +1853,7 @@ struct xpbd_ctx {
     struct GraphSlot {
         hipGraph_t graph = nullptr;
         hipGraphExec_t gexec = nullptr;
-        int nsub = -1, iters = -1, lam_mode = -1;
+        int nsub = -1, iters = -1, lam_mode = -1, ref_iters = -1;
         uint64_t used = 0;
     };
     static constexpr int NGRAPH = 4;
@@ -1747,7 +1902,7 @@ int dupload(xpbd_ctx *c, T **p, const T *src, size_t count) {
 void drop_graph_slot(xpbd_ctx::GraphSlot &g) {
     if (g.gexec) { (void)hipGraphExecDestroy(g.gexec); g.gexec = nullptr; }
     if (g.graph) { (void)hipGraphDestroy(g.graph); g.graph = nullptr; }
-    g.nsub = g.iters = g.lam_mode = -1;
+    g.nsub = g.iters = g.lam_mode = g.ref_iters = -1;
     g.used = 0;
 }
 
@@ -2086,6 +2241,7 @@ int launch_dcd(xpbd_ctx *c, float4 *X, bool rebuild) {
     auto g1 = [](int64_t n) { return dim3((unsigned)((n + 255) / 256)); };
     if (rebuild) {
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, d.rmax_bits, (int64_t)1);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, d.overflow, (int64_t)2);
         hipLaunchKernelGGL(k_zero_u32, g1(T), dim3(256), 0, c->stream, d.bucket_cnt, T);
         hipLaunchKernelGGL(k_zero_u32, g1(T), dim3(256), 0, c->stream, d.bucket_fill, T);
         hipLaunchKernelGGL(k_dcd_snapshot, g1(c->nv), dim3(256), 0, c->stream, d.xprev, (const float4 *)X, c->nv);
@@ -2160,8 +2316,9 @@ int alloc_dcd(xpbd_ctx *c) {
         (r = dalloc(c, &d.pairs, (size_t)d.max_pairs)) || (r = dalloc(c, &d.contacts, (size_t)d.max_pairs)) ||
         (r = dalloc(c, &d.acc, 3 * (size_t)c->nv)) || (r = dalloc(c, &d.acc_n, (size_t)c->nv)) ||
         (r = dalloc(c, &d.xprev, (size_t)c->nv)) || (r = dalloc(c, &d.scal, 4)) ||
-        (r = dalloc(c, &d.rmax_bits, 1)) || (r = dalloc(c, &d.ncontact, 1)))
+        (r = dalloc(c, &d.rmax_bits, 1)) || (r = dalloc(c, &d.ncontact, 1)) || (r = dalloc(c, &d.overflow, 2)))
         return r;
+    HIPCHK(c, hipMemset(d.overflow, 0, 2 * sizeof(uint32_t)));
     HIPCHK(c, hipMemset(d.acc, 0, 3 * (size_t)c->nv * sizeof(unsigned long long)));
     HIPCHK(c, hipMemset(d.acc_n, 0, (size_t)c->nv * sizeof(uint32_t)));
     HIPCHK(c, hipMemset(d.pair_off, 0, ((size_t)m + 1) * sizeof(uint32_t)));
@@ -2176,34 +2333,42 @@ int alloc_dcd(xpbd_ctx *c) {
     return XPBD_OK;
 }
 
-// start-of-substep multiplier set-up for the lambda-carrying modes
+// start-of-substep multiplier set-up for the effective mode of this substep
 int launch_warm_start(xpbd_ctx *c) {
     const int64_t ne = (int64_t)c->L.meta.size();
     auto g1 = [](int64_t n) { return dim3((unsigned)((n + 255) / 256)); };
     c->impulse_pending = false;
-    if (c->lam_mode == 1) {
+    if (c->eff_mode == 1) {
         hipLaunchKernelGGL(k_zero4, g1(ne), dim3(256), 0, c->stream, ne, c->LAMB);
-    } else if (c->lam_mode == 2) {
+    } else if (c->eff_mode == 2) {
         if (!c->mlp_ready) return fail(c, XPBD_EINVAL, "lambda mode 2 needs xpbd_set_mlp()");
         IterArgs a = make_args(c, c->DPW);
-        hipLaunchKernelGGL(k_features, g1(ne), dim3(256), 0, c->stream, a, (const float4 *)c->X[0], c->FEAT,
-                           (const float *)c->fmean, (const float *)c->finv, (const float *)c->oscale, 1);
-        HIPCHK(c, hipGetLastError());
-        xpbd_mlp_dev net{c->mW1, c->mb1, c->mW2, c->mb2, c->mW3, c->mb3};
-        int r = xpbd_mlp_forward(&net, c->FEAT, ne, c->MLPY, nullptr, nullptr, c->stream);
-        if (r) return fail(c, r, "xpbd_mlp_forward failed");
-        hipLaunchKernelGGL(k_lam0, g1(ne), dim3(256), 0, c->stream, ne, (const float *)c->MLPY,
-                           (const float *)c->oscale, c->LAMB);
+        WarmNet net{c->mW1, c->mW2, c->mb1, c->mb2, c->mW3, c->mb3, c->fmean, c->finv, c->oscale};
+        const int64_t ntile = (ne + 127) / 128;
+        const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ntile, 1024));
+        hipLaunchKernelGGL(k_warm_start, dim3(grid), dim3(256), 0, c->stream, a, net, (const float4 *)c->X[0],
+                           c->LAMB0, c->LAMB);
         c->impulse_pending = true;
-    } else if (c->lam_mode == 3) {
-        c->impulse_pending = true;   // lambda0 = the multipliers the previous substep ended with
+    } else if (c->eff_mode == 3) {
+        hipLaunchKernelGGL(k_copy4, g1(ne), dim3(256), 0, c->stream, ne, c->LAMB, (const float4 *)c->LAMB0);
+        c->impulse_pending = true;
     }
     HIPCHK(c, hipGetLastError());
     return XPBD_OK;
 }
 
+// end of a substep: a reference solve (carried from zero) becomes the base
+int finish_lambda(xpbd_ctx *c) {
+    if (c->eff_mode != 1) return XPBD_OK;
+    const int64_t ne = (int64_t)c->L.meta.size();
+    hipLaunchKernelGGL(k_copy4, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, c->stream, ne, c->LAMB0,
+                       (const float4 *)c->LAMB);
+    HIPCHK(c, hipGetLastError());
+    return XPBD_OK;
+}
+
 int lam_for_iteration(xpbd_ctx *c) {
-    if (c->lam_mode == 0) return 0;
+    if (c->eff_mode == 0) return 0;
     if (c->impulse_pending) {
         c->impulse_pending = false;
         return 2;
@@ -2240,16 +2405,22 @@ int launch_vcycle(xpbd_ctx *c) {
     return XPBD_OK;
 }
 
-// one substep, positions start and end in X[0]
-int enqueue_substep(xpbd_ctx *c, int iters, bool dcd_rebuild) {
+// one substep, positions start and end in X[0]; sub = index within the frame
+int enqueue_substep(xpbd_ctx *c, int iters, bool dcd_rebuild, int sub = 0) {
     int r = launch_integrate(c);
     if (r) return r;
     if (c->coarse && c->mg_iters > 0 && (r = launch_vcycle(c))) return r;
-    if (c->lam_mode && (r = launch_warm_start(c))) return r;
+    // mode 4: the first substep of a frame is a reference solve (carried from
+    // zero, ref_iters iterations) that refreshes the base; the others are
+    // network warm starts with iters iterations
+    c->eff_mode = c->lam_mode == 4 ? (sub == 0 ? 1 : 2) : c->lam_mode;
+    if (c->lam_mode == 4 && sub == 0) iters = c->ref_iters;
+    if (c->eff_mode && (r = launch_warm_start(c))) return r;
     for (int it = 0; it < iters; ++it) {
         r = launch_iter(c, c->X[it & 1], c->X[(it + 1) & 1], MODE_FUSED, lam_for_iteration(c));
         if (r) return r;
     }
+    if ((r = finish_lambda(c))) return r;
     // post phase: self DCD passes, then bone push-out and velocities (mg.py:1362-1376)
     if (c->dcd_on)
         for (int k = 0; k < c->dcd_iters; ++k)
@@ -2649,8 +2820,8 @@ int32_t xpbd_finalize(xpbd_ctx *c, const xpbd_tuning *t) {
         }
         if ((r = dalloc(c, &c->LAMB, std::max<size_t>(nptet, 1)))) return r;
         HIPCHK(c, hipMemset(c->LAMB, 0, std::max<size_t>(nptet, 1) * sizeof(float4)));
-        if ((r = dalloc(c, &c->FEAT, 24 * std::max<size_t>(nptet, 1)))) return r;
-        if ((r = dalloc(c, &c->MLPY, 4 * std::max<size_t>(nptet, 1)))) return r;
+        if ((r = dalloc(c, &c->LAMB0, std::max<size_t>(nptet, 1)))) return r;
+        HIPCHK(c, hipMemset(c->LAMB0, 0, std::max<size_t>(nptet, 1) * sizeof(float4)));
         if ((r = dalloc(c, &c->mW1, (size_t)XPBD_MLP_HIDDEN * XPBD_MLP_IN)) ||
             (r = dalloc(c, &c->mW2, (size_t)XPBD_MLP_HIDDEN * XPBD_MLP_HIDDEN)) ||
             (r = dalloc(c, &c->mb1, XPBD_MLP_HIDDEN)) || (r = dalloc(c, &c->mb2, XPBD_MLP_HIDDEN)) ||
@@ -2839,6 +3010,7 @@ int32_t xpbd_iterate(xpbd_ctx *c, int32_t n) {
     int r = check_ready(c);
     if (r) return r;
     if (n < 0) return fail(c, XPBD_EINVAL, "negative iteration count");
+    c->eff_mode = c->lam_mode == 4 ? 2 : c->lam_mode;
     for (int it = 0; it < n; ++it)
         if ((r = launch_iter(c, c->X[it & 1], c->X[(it + 1) & 1], MODE_FUSED, lam_for_iteration(c)))) return r;
     if (n & 1) {
@@ -2861,7 +3033,7 @@ int32_t xpbd_substep(xpbd_ctx *c, int32_t iters) {
     int r = check_ready(c);
     if (r) return r;
     if (iters < 0) return fail(c, XPBD_EINVAL, "negative iteration count");
-    r = enqueue_substep(c, iters, c->dcd_fresh);
+    r = enqueue_substep(c, iters, c->dcd_fresh, c->lam_mode == 4 ? 1 : 0);
     c->dcd_fresh = false;
     return r;
 }
@@ -2874,18 +3046,19 @@ int32_t xpbd_step_frame(xpbd_ctx *c, int32_t nsub, int32_t iters, int32_t use_gr
     c->dcd_fresh = false;
     if (!use_graph || c->timing) {
         for (int s = 0; s < nsub; ++s)
-            if ((r = enqueue_substep(c, iters, s == 0))) return r;
+            if ((r = enqueue_substep(c, iters, s == 0, s))) return r;
         return XPBD_OK;
     }
     xpbd_ctx::GraphSlot *hit = nullptr, *victim = &c->gslot[0];
     for (auto &g : c->gslot) {
-        if (g.gexec && g.nsub == nsub && g.iters == iters && g.lam_mode == c->lam_mode) hit = &g;
+        if (g.gexec && g.nsub == nsub && g.iters == iters && g.lam_mode == c->lam_mode && g.ref_iters == c->ref_iters)
+            hit = &g;
         if (g.used < victim->used) victim = &g;
     }
     if (!hit) {
         drop_graph_slot(*victim);
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        for (int s = 0; s < nsub && r == XPBD_OK; ++s) r = enqueue_substep(c, iters, s == 0);
+        for (int s = 0; s < nsub && r == XPBD_OK; ++s) r = enqueue_substep(c, iters, s == 0, s);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (r) { if (g) (void)hipGraphDestroy(g); return r; }
@@ -2895,6 +3068,7 @@ int32_t xpbd_step_frame(xpbd_ctx *c, int32_t nsub, int32_t iters, int32_t use_gr
         victim->nsub = nsub;
         victim->iters = iters;
         victim->lam_mode = c->lam_mode;
+        victim->ref_iters = c->ref_iters;
         hit = victim;
     }
     hit->used = ++c->gclock;
@@ -2997,6 +3171,18 @@ int32_t xpbd_dcd_pairs(xpbd_ctx *c, int32_t *out, int64_t cap, int64_t *n_total)
         HIPCHK(c, hipMemcpyAsync(out, c->dcd.pairs, (size_t)n * sizeof(int2), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
+    return XPBD_OK;
+}
+
+int32_t xpbd_dcd_overflow(xpbd_ctx *c, int64_t *left_out, int64_t *truncated) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!c->dcd_on) return fail(c, XPBD_EINVAL, "no DCD surface set");
+    uint32_t o[2] = {0, 0};
+    HIPCHK(c, hipMemcpyAsync(o, c->dcd.overflow, sizeof(o), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (left_out) *left_out = o[0];
+    if (truncated) *truncated = o[1];
     return XPBD_OK;
 }
 
@@ -3175,11 +3361,11 @@ int32_t xpbd_vcycle(xpbd_ctx *c) {
 int32_t xpbd_set_lambda_mode(xpbd_ctx *c, int32_t mode) {
     int r = check_ready(c);
     if (r) return r;
-    if (mode < 0 || mode > 3) return fail(c, XPBD_EINVAL, "lambda mode must be 0..3");
+    if (mode < 0 || mode > 4) return fail(c, XPBD_EINVAL, "lambda mode must be 0..4");
     if (mode && !c->flat) return fail(c, XPBD_EUNSUPPORTED, "lambda modes run on the flat strategy");
     if (mode && (c->feature & F_ARAP) && (c->feature & F_SNH))
         return fail(c, XPBD_EUNSUPPORTED, "lambda modes need ARAP-SVD and SNH on separate meshes");
-    if (mode == 2 && !c->mlp_ready) return fail(c, XPBD_EINVAL, "lambda mode 2 needs xpbd_set_mlp()");
+    if ((mode == 2 || mode == 4) && !c->mlp_ready) return fail(c, XPBD_EINVAL, "lambda modes 2 and 4 need xpbd_set_mlp()");
     c->lam_mode = mode;
     c->impulse_pending = false;
     return XPBD_OK;
@@ -3188,7 +3374,8 @@ int32_t xpbd_set_lambda_mode(xpbd_ctx *c, int32_t mode) {
 int32_t xpbd_warm_start(xpbd_ctx *c) {
     int r = check_ready(c);
     if (r) return r;
-    return c->lam_mode ? launch_warm_start(c) : XPBD_OK;
+    c->eff_mode = c->lam_mode == 4 ? 2 : c->lam_mode;
+    return c->eff_mode ? launch_warm_start(c) : XPBD_OK;
 }
 
 int32_t xpbd_features(xpbd_ctx *c, float *dev_out, int32_t normalized) {
@@ -3199,8 +3386,8 @@ int32_t xpbd_features(xpbd_ctx *c, float *dev_out, int32_t normalized) {
     const int64_t ne = (int64_t)c->L.meta.size();
     IterArgs a = make_args(c, c->DPW);
     hipLaunchKernelGGL(k_features, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, c->stream, a,
-                       (const float4 *)c->X[0], dev_out, (const float *)c->fmean, (const float *)c->finv,
-                       (const float *)c->oscale, normalized ? 1 : 0);
+                       (const float4 *)c->X[0], (const float4 *)c->LAMB0, dev_out, (const float *)c->fmean,
+                       (const float *)c->finv, normalized ? 1 : 0);
     HIPCHK(c, hipGetLastError());
     return XPBD_OK;
 }
@@ -3218,6 +3405,15 @@ int32_t xpbd_set_lambda(xpbd_ctx *c, const float *dev_in) {
     if (r) return r;
     if (!c->flat || !dev_in) return fail(c, XPBD_EINVAL, "flat strategy and an input buffer required");
     HIPCHK(c, hipMemcpyAsync(c->LAMB, dev_in, c->L.meta.size() * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->LAMB0, dev_in, c->L.meta.size() * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+    return XPBD_OK;
+}
+
+int32_t xpbd_set_reference_iters(xpbd_ctx *c, int32_t n) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (n < 1) return fail(c, XPBD_EINVAL, "reference iterations must be >= 1");
+    c->ref_iters = n;
     return XPBD_OK;
 }
 

@@ -10,6 +10,11 @@
 //    max(2 x mean rest edge, 2 x max radius + thickness) so every pair that
 //    passes the sphere test lies in adjacent cells; each triangle is visited
 //    only from its own cell, so the candidate set equals the exhaustive one;
+//  * bounded work on broken geometry: the cell is capped at DCD_CELL_CAP x
+//    (2 x mean rest edge); a triangle too large for the capped cell (or with a
+//    non-finite centroid) takes no part and is counted, and a triangle keeps at
+//    most DCD_MAX_PER_TRI candidates (the rest are counted).  Neither limit is
+//    reached on a valid mesh, where the pair list equals the exhaustive one;
 //  * pairs are written per triangle at scanned offsets and each triangle's
 //    segment is sorted, giving the (i, j)-sorted pair list independent of
 //    hashing and scheduling;
@@ -29,6 +34,9 @@ struct DcdContact {
     float n[3], b1[3], b2[3];
     int32_t valid;
 };
+
+constexpr float DCD_CELL_CAP = 4.0f;
+constexpr uint32_t DCD_MAX_PER_TRI = 1024;
 
 struct DcdDev {
     int ntris;
@@ -53,6 +61,7 @@ struct DcdDev {
     float *scal;                // [0] cell, [1] inv cell, [2] rmax bits (as float), [3] mean edge
     uint32_t *rmax_bits;
     uint32_t *ncontact;         // valid contacts of the last narrow pass
+    uint32_t *overflow;         // [0] triangles left out (too large / non-finite), [1] truncated candidate lists
     float thickness, multiplier, mu_s, mu_d, offset_eps, normal_cull, mean_edge2;
     int allow_self;
 };

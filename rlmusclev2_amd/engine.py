@@ -123,7 +123,11 @@ class XpbdEngine:
         b = np.zeros(1, np.int64)
         cell = np.zeros(1, np.float32)
         self.ctx.call("xpbd_dcd_stats", ptr(a), ptr(b), ptr(cell))
-        return {"n_pairs": int(a[0]), "n_contacts": int(b[0]), "cell": float(cell[0])}
+        lo = np.zeros(1, np.int64)
+        tr = np.zeros(1, np.int64)
+        self.ctx.call("xpbd_dcd_overflow", ptr(lo), ptr(tr))
+        return {"n_pairs": int(a[0]), "n_contacts": int(b[0]), "cell": float(cell[0]),
+                "left_out_tris": int(lo[0]), "truncated_tris": int(tr[0])}
 
     def set_params(self, p):
         q = capi.XpbdParams()
@@ -322,6 +326,9 @@ class XpbdEngine:
         self._torch_sync()
         self.ctx.call("xpbd_set_lambda", C.c_void_p(t.data_ptr()))
         self.synchronize()
+
+    def set_reference_iters(self, n: int):
+        self.ctx.call("xpbd_set_reference_iters", int(n))
 
     def snapshot(self, save: bool):
         self.ctx.call("xpbd_snapshot", int(bool(save)))

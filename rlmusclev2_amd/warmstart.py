@@ -32,7 +32,7 @@ import numpy as np
 from . import workloads
 from .mlp import NIN, NOUT, Mlp
 
-MODE_PLAIN, MODE_CARRY, MODE_NET, MODE_PREV = 0, 1, 2, 3
+MODE_PLAIN, MODE_CARRY, MODE_NET, MODE_PREV, MODE_SCHED = 0, 1, 2, 3, 4
 
 
 def _dptr(t):
@@ -268,7 +268,7 @@ def iterations_to_reach(sim, mode: int, target: dict, kmax: int = 10):
 
 
 def compare(sim, n_frames: int, seed: int = 1, plain_iters: int = 10, modes=(MODE_CARRY, MODE_PREV, MODE_NET),
-            conv_iters: int = 10, exact: bool = True):
+            conv_iters: int = 10, exact: bool = True, log=None):
     """Per substep of a fresh randomized plain trajectory: the plain
     ``plain_iters``-iteration residual is the target; record the iterations
     each mode needs to reach it (None = not within ``plain_iters``).  Every
@@ -319,6 +319,8 @@ def compare(sim, n_frames: int, seed: int = 1, plain_iters: int = 10, modes=(MOD
             e.iterate(plain_iters)
             e.collide()
             e.update_velocities()
+            if log:
+                log(f"compare: frame {f} substep {s}: " + ", ".join(f"{m}={out[m][-1]}" for m in out))
     return out
 
 
@@ -326,52 +328,70 @@ class WarmStartStepper:
     """Frames with the learned warm start and the per-frame fallback to plain
     zero-lambda XPBD.
 
-    A frame runs ``k_iters`` iterations per substep in mode 2.  Its residual
-    (per constraint type, at the end of the frame) is compared with the plain
-    ``plain_iters`` residual of the same frame; if any type is worse by more
-    than ``tol`` the frame is re-run plain from the saved start state.  The
-    plain residual is computed for real every ``audit_every`` frames (1 =
-    every frame, the exact rule); in between, the last audited value stands
-    in for it."""
+    A warm frame runs in mode 4: its first substep is a reference solve
+    (lambda carried from zero, ``plain_iters`` iterations) whose multipliers
+    seed the network, and each later substep applies the network's lambda0
+    and runs ``k_iters`` iterations.  An audit
+    frame (every ``audit_every``-th frame; 1 = every frame, the exact rule)
+    also runs the plain ``plain_iters`` frame from the same start state and
+    compares the per-type residuals at the end of the frame: if any type of
+    the warm frame is worse by more than ``tol`` (or not finite), the frame is
+    re-run plain and the warm start stays off until the next audit that it
+    passes.  Between audits a warm frame still falls back at once when its
+    residual is not finite or exceeds the last audited plain residual by more
+    than ``guard``."""
 
     def __init__(self, sim, k_iters: int, plain_iters: int | None = None, audit_every: int = 1,
-                 tol: float = 0.0, use_graph: bool = True):
+                 tol: float = 0.0, guard: float = 1.0, use_graph: bool = True):
         self.sim = sim
         self.k = int(k_iters)
         self.plain_iters = int(plain_iters or sim.fine_iters)
+        sim.engine.set_reference_iters(self.plain_iters)
         self.audit_every = max(1, int(audit_every))
         self.tol = float(tol)
+        self.guard = float(guard)
         self.use_graph = use_graph
         self.ref = None
+        self.warm_ok = True
         self.frames = 0
         self.fallbacks = 0
         self.audits = 0
 
-    def _plain_frame(self):
+    def _frame(self, mode: int, iters: int):
         e = self.sim.engine
+        e.set_lambda_mode(mode)
+        e.native_frame(self.sim.nsubsteps, iters, self.use_graph)
         e.set_lambda_mode(MODE_PLAIN)
-        e.native_frame(self.sim.nsubsteps, self.plain_iters, self.use_graph)
+        return e.residuals()
+
+    @staticmethod
+    def _worse(r, ref, tol):
+        return not all(r[t] <= ref[t] * (1.0 + tol) for t in ref)   # NaN counts as worse
 
     def step_frame(self, frame: int) -> dict:
         sim, e = self.sim, self.sim.engine
         sim.load_frame_targets(frame)
         e.apply_pintotarget()
-        e.snapshot(True)
-        audited = self.ref is None or self.frames % self.audit_every == 0
+        audited = self.frames % self.audit_every == 0
+        r = None
         if audited:
-            self._plain_frame()
-            self.ref = e.residuals()
+            e.snapshot(True)
+            self.ref = self._frame(MODE_PLAIN, self.plain_iters)
             self.audits += 1
             e.snapshot(False)
-        e.set_lambda_mode(MODE_NET)
-        e.native_frame(sim.nsubsteps, self.k, self.use_graph)
-        r = e.residuals()
-        bad = not all(r[t] <= self.ref[t] * (1.0 + self.tol) for t in self.ref)   # NaN counts as worse
-        if bad:
-            e.snapshot(False)
-            self._plain_frame()
-            self.ref = e.residuals()
+            r = self._frame(MODE_SCHED, self.k)
+            self.warm_ok = not self._worse(r, self.ref, self.tol)
+            fallback = not self.warm_ok
+        elif self.warm_ok:
+            e.snapshot(True)
+            r = self._frame(MODE_SCHED, self.k)
+            fallback = self._worse(r, self.ref, self.guard)
+        else:
+            fallback = True
+        if fallback:
+            if r is not None:
+                e.snapshot(False)          # back to the start of the frame
+            self._frame(MODE_PLAIN, self.plain_iters)
             self.fallbacks += 1
-        e.set_lambda_mode(MODE_PLAIN)
         self.frames += 1
-        return {"fallback": bad, "audited": audited, "residual": r, "plain": dict(self.ref)}
+        return {"fallback": fallback, "audited": audited, "residual": r, "plain": dict(self.ref or {})}

@@ -22,6 +22,7 @@ def _setup(strategy, kinematic=True, iters=4):
     cfg["use_multigrid"] = True
     cfg["mg_coarse_iters"] = iters
     cfg["mg_kinematic_correction"] = kinematic
+    cfg["quasi_static"] = False
     w = workloads.humanv2_scale(3, n_bones=1, per_bone=4, target_tets=8000, fascia_tets=1500, sdf_res=32)
     sim = sims.MultiGridSim(cfg, workload=w, strategy=strategy, block_tets=256)
     ref = sims.MultiGridSim(cfg, workload=w, engine_cls=oracle.OracleSim)
