@@ -25,7 +25,7 @@ from .capi import XpbdError
 CACHE = Path(__file__).resolve().parent / "_tune" / "cache.json"
 # (strategy, patch_verts, block_tets, block_threads)
 CANDIDATES = ((1, 256, 0, 0), (0, 64, 0, 0), (0, 128, 0, 0), (0, 256, 0, 0),
-              (2, 256, 512, 256), (2, 256, 1024, 256), (2, 256, 1024, 512), (2, 256, 2048, 512))
+              (2, 256, 256, 256), (2, 256, 512, 256), (2, 256, 512, 512), (2, 256, 1024, 512))
 
 
 def name(c) -> str:

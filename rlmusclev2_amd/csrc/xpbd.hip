@@ -102,7 +102,10 @@ __device__ __forceinline__ bool polar_newton(const float *F, float *X) {
             X[i] = xn;
         }
         if (delta < 1.0e-4f) scale = false;
-        if (delta < 1.0e-10f) return true;
+        // near convergence the next error is about delta / 2 (quadratic
+        // convergence, delta = |X_k+1 - X_k|^2 ~ error_k^2): stop once that is
+        // below FP32 resolution of a rotation
+        if (delta < 1.0e-7f) return true;
     }
     return false;
 }
@@ -562,19 +565,12 @@ __global__ __launch_bounds__(256) void k_flat_tets(IterArgs a, const float4 *__r
 // finish one vertex: attach, then either accumulate (MODE_ACCUM) or apply
 // dP/dPw (apply_dP_kernel, engine.py:689-698) and the pin overwrite
 // (pintotarget_overwrite_masked_kernel, mg/kernel.py:212-226)
-template <int DAMP, int MODE>
-__device__ __forceinline__ void finish_vertex(const IterArgs &a, int v, float4 x, float4 pp, float ax, float ay,
-                                              float az, float aw, float4 *__restrict__ xout) {
+// finished position of a vertex after an iteration: attach, apply dP/dPw
+// (apply_dP_kernel, engine.py:689-698) and the pin overwrite
+// (pintotarget_overwrite_masked_kernel, mg/kernel.py:212-226)
+__device__ __forceinline__ float4 finish_value(const IterArgs &a, int v, float4 x, float4 pp, float ax, float ay,
+                                               float az, float aw) {
     if (a.att0) attach_vertex(a, v, x, pp, ax, ay, az, aw);
-    if (MODE == MODE_ACCUM) {
-        float4 o = a.dpw[v];
-        o.x += ax;
-        o.y += ay;
-        o.z += az;
-        o.w += aw;
-        a.dpw[v] = o;
-        return;
-    }
     if (aw > 1e-9f) {
         x.x = x.x + ax / aw;
         x.y = x.y + ay / aw;
@@ -584,9 +580,29 @@ __device__ __forceinline__ void finish_vertex(const IterArgs &a, int v, float4 x
         const float4 p = a.pin[v];
         if (p.w != 0.0f) { x.x = p.x; x.y = p.y; x.z = p.z; }
     }
-    xout[v] = x;
+    return x;
 }
 
+// finish one vertex: either accumulate (MODE_ACCUM, attach included) or write
+// the finished position
+template <int DAMP, int MODE>
+__device__ __forceinline__ void finish_vertex(const IterArgs &a, int v, float4 x, float4 pp, float ax, float ay,
+                                              float az, float aw, float4 *__restrict__ xout) {
+    if (MODE == MODE_ACCUM) {
+        if (a.att0) attach_vertex(a, v, x, pp, ax, ay, az, aw);
+        float4 o = a.dpw[v];
+        o.x += ax;
+        o.y += ay;
+        o.z += az;
+        o.w += aw;
+        a.dpw[v] = o;
+        return;
+    }
+    xout[v] = finish_value(a, v, x, pp, ax, ay, az, aw);
+}
+
+// Vertex pass of the flat strategy: each vertex adds its contiguous run of
+// corner corrections in ascending element order, then is finished.
 template <int DAMP, int MODE>
 __global__ __launch_bounds__(256) void k_flat_verts(IterArgs a, const float4 *__restrict__ xin,
                                                     float4 *__restrict__ xout,
@@ -596,12 +612,12 @@ __global__ __launch_bounds__(256) void k_flat_verts(IterArgs a, const float4 *__
     if (v >= nv) return;
     float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
     const int e = a.vslot_off[v + 1];
-    for (int s = a.vslot_off[v]; s < e; ++s) {
-        const float4 q = corr[s];
-        ax += q.x;
-        ay += q.y;
-        az += q.z;
-        aw += q.w;
+    for (int q = a.vslot_off[v]; q < e; ++q) {
+        const float4 c = corr[q];
+        ax += c.x;
+        ay += c.y;
+        az += c.z;
+        aw += c.w;
     }
     finish_vertex<DAMP, MODE>(a, v, xin[v], DAMP ? pprev[v] : make_float4(0, 0, 0, 0), ax, ay, az, aw, xout);
 }
@@ -623,76 +639,59 @@ struct BlockArgs {
     const uint16_t *bslot;
     const int32_t *svid, *soff;
     float4 *part;
-    int max_nloc;
+    int max_nloc, block_tets;
 };
 
-// LDS: vertex table (and previous positions when damping), per-entry running
-// sums and slot cursors, and the corner corrections of one chunk of blockDim
-// elements.  Chunks are projected in order; after each chunk every local entry
-// consumes its slots that fall in the chunk, so slots are added in ascending
-// order whatever the chunk size.
+// LDS: vertex table (and previous positions when damping), then the corner
+// corrections of all block elements, corner-major (sc[k * TB + t]) so that a
+// wavefront's stores are contiguous.  After one barrier every local entry adds
+// its corners in ascending (element, corner) order.
 template <int FEAT, int DAMP, int LAM, int MODE>
 __global__ __launch_bounds__(512) void k_block_tets(IterArgs a, BlockArgs b, const float4 *__restrict__ xin,
                                                     float4 *__restrict__ xout,
                                                     const float4 *__restrict__ pprev) {
     extern __shared__ float4 smem[];
     const BlockDesc D = b.blk[blockIdx.x];
-    const int M = b.max_nloc;
+    const int M = b.max_nloc, TB = b.block_tets;
     float4 *sx = smem;
     float4 *spp = smem + M;
-    float4 *sacc = smem + (DAMP ? 2 : 1) * M;
-    int2 *scur = (int2 *)(sacc + M);
-    float4 *sc = (float4 *)(scur + ((M + 1) & ~1));
+    float4 *sc = smem + (DAMP ? 2 : 1) * M;
     const int tid = threadIdx.x, nth = blockDim.x;
     for (int i = tid; i < D.nloc; i += nth) {
         const int gv = b.bvt[D.vbeg + i];
         sx[i] = xin[gv];
         if (DAMP) spp[i] = pprev[gv];
-        sacc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        const uint32_t run = b.brun[D.vbeg + i];
-        scur[i] = make_int2((int)(run & 0xffffu), (int)(run & 0xffffu) + (int)(run >> 16));
+    }
+    __syncthreads();
+    for (int t = tid; t < D.ntet; t += nth) {
+        const int64_t g = (int64_t)D.tbeg + t;
+        const ushort4 li = b.blidx[g];
+        const int lv[4] = {li.x, li.y, li.z, li.w};
+        Corr c;
+        project_elem<FEAT, DAMP, LAM>(a, g, sx, spp, lv, c);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) sc[k * TB + t] = make_float4(c.d[k].x, c.d[k].y, c.d[k].z, c.cnt);
     }
     __syncthreads();
     const uint16_t *slots = b.bslot + 4 * (int64_t)D.tbeg;
-    for (int base = 0; base < D.ntet; base += nth) {
-        const int t = base + tid;
-        if (t < D.ntet) {
-            const int64_t g = (int64_t)D.tbeg + t;
-            const ushort4 li = b.blidx[g];
-            const int lv[4] = {li.x, li.y, li.z, li.w};
-            Corr c;
-            project_elem<FEAT, DAMP, LAM>(a, g, sx, spp, lv, c);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) sc[4 * tid + k] = make_float4(c.d[k].x, c.d[k].y, c.d[k].z, c.cnt);
-        }
-        __syncthreads();
-        const int lo = 4 * base, lim = 4 * (base + nth);
-        for (int i = tid; i < D.nloc; i += nth) {
-            int2 cr = scur[i];
-            float4 acc = sacc[i];
-            while (cr.x < cr.y) {
-                const int s = slots[cr.x];
-                if (s >= lim) break;
-                const float4 q = sc[s - lo];
-                acc.x += q.x;
-                acc.y += q.y;
-                acc.z += q.z;
-                acc.w += q.w;
-                ++cr.x;
-            }
-            scur[i] = cr;
-            sacc[i] = acc;
-        }
-        __syncthreads();
-    }
     for (int i = tid; i < D.nloc; i += nth) {
-        const float4 acc = sacc[i];
+        const uint32_t run = b.brun[D.vbeg + i];
+        const int s0 = (int)(run & 0xffffu), n = (int)(run >> 16);
+        float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
+        for (int j = 0; j < n; ++j) {
+            const int sl = slots[s0 + j];
+            const float4 q = sc[(sl & 3) * TB + (sl >> 2)];
+            ax += q.x;
+            ay += q.y;
+            az += q.z;
+            aw += q.w;
+        }
         const int ps = b.bps[D.vbeg + i];
         if (ps >= 0) {
-            b.part[ps] = acc;
+            b.part[ps] = make_float4(ax, ay, az, aw);
         } else {
-            finish_vertex<DAMP, MODE>(a, b.bvt[D.vbeg + i], sx[i], DAMP ? spp[i] : make_float4(0, 0, 0, 0), acc.x,
-                                      acc.y, acc.z, acc.w, xout);
+            finish_vertex<DAMP, MODE>(a, b.bvt[D.vbeg + i], sx[i], DAMP ? spp[i] : make_float4(0, 0, 0, 0), ax, ay,
+                                      az, aw, xout);
         }
     }
 }
@@ -2111,8 +2110,7 @@ BlockVertKernel pick_block_shared(int feat, int mode) {
 size_t block_lds(const xpbd_ctx *c) {
     const size_t M = (size_t)c->L.max_nloc;
     const size_t damp = (c->feature & F_DAMP) ? 2 : 1;
-    return (damp + 1) * M * sizeof(float4) + ((M + 1) & ~(size_t)1) * sizeof(int2) +
-           4 * (size_t)c->block_bs * sizeof(float4);
+    return (damp * M + 4 * (size_t)c->block) * sizeof(float4);
 }
 
 BlockArgs make_block_args(xpbd_ctx *c) {
@@ -2127,6 +2125,7 @@ BlockArgs make_block_args(xpbd_ctx *c) {
     b.soff = c->d_soff;
     b.part = c->BPART;
     b.max_nloc = c->L.max_nloc;
+    b.block_tets = c->block;
     return b;
 }
 
