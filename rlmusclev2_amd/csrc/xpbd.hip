@@ -228,6 +228,47 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
     for (int i = 0; i < 4; ++i) out.d[i] = mk3(0.f, 0.f, 0.f);
     out.cnt = 0.f;
 
+    // the volume constraint goes first: it needs the positions and previous
+    // positions, which are then dead during the polar iteration (lower register
+    // pressure); the corrections of one element are summed per type, so the
+    // order does not change the result
+    // ---- volume: tet_volume_update_xpbd_fn, solve_kernels.py:187-281 ----
+    if ((FEAT & F_VOL) && (mask & M_VOL)) {
+        constexpr float SIXTH = 1.0f / 6.0f;
+        V3 gr[4];
+        gr[0] = SIXTH * cross(c1, c2);
+        gr[1] = SIXTH * cross(c2, c0);
+        gr[2] = SIXTH * cross(c0, c1);
+        gr[3] = -(gr[0] + gr[1] + gr[2]);
+        const float wsum = w[0] * dot(gr[0], gr[0]) + w[1] * dot(gr[1], gr[1]) +
+                           w[2] * dot(gr[2], gr[2]) + w[3] * dot(gr[3], gr[3]);
+        if (wsum > 1e-9f) {
+            const float vol = SIXTH * dot(cross(c0, c1), c2);
+            const float rest = a.rvol[g];
+            const bool comp = a.sp.has_compress && (vol < rest);
+            const float k = comp ? cp.kcomp_vol : cp.k_vol;
+            if (k != 0.0f) {
+                const float alpha = comp ? dc.a_comp : dc.a_vol;
+                const float C = vol - rest;
+                float dsum = 0.f, gamma = 1.f;
+                if (DAMP && cp.damp_vol > 0.0f) {
+                    gamma = comp ? dc.g_comp : dc.g_vol;
+                    for (int i = 0; i < 4; ++i) {
+                        const float4 pp = spp[lv[i]];
+                        dsum += dot(gr[i], P[i] - xyz(pp));
+                    }
+                    dsum = dsum * gamma;
+                    gamma = gamma + 1.0f;
+                }
+                const float dl = LAM == 2 ? lam.y : (-C - alpha * lam.y - dsum) * __builtin_amdgcn_rcpf(gamma * wsum + alpha);
+                if (LAM == 1) nl.y = lam.y + dl;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) out.d[i] = out.d[i] + (dl * w[i]) * gr[i];
+                out.cnt += 1.0f;
+            }
+        }
+    }
+
     float B[9], F[9];
     constexpr bool NEED_F = (FEAT & (F_ARAP | F_SNH | F_ANISO)) != 0;
     if (NEED_F && (mask & (M_ARAP | M_SNH | M_ANISO))) {
@@ -267,43 +308,6 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
                 if (LAM == 1) nl.x = lam.x + dL;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) out.d[i] = out.d[i] + (dL * w[i]) * gr[i];
-                out.cnt += 1.0f;
-            }
-        }
-    }
-
-    // ---- volume: tet_volume_update_xpbd_fn, solve_kernels.py:187-281 ----
-    if ((FEAT & F_VOL) && (mask & M_VOL)) {
-        constexpr float SIXTH = 1.0f / 6.0f;
-        V3 gr[4];
-        gr[0] = SIXTH * cross(c1, c2);
-        gr[1] = SIXTH * cross(c2, c0);
-        gr[2] = SIXTH * cross(c0, c1);
-        gr[3] = -(gr[0] + gr[1] + gr[2]);
-        const float wsum = w[0] * dot(gr[0], gr[0]) + w[1] * dot(gr[1], gr[1]) +
-                           w[2] * dot(gr[2], gr[2]) + w[3] * dot(gr[3], gr[3]);
-        if (wsum > 1e-9f) {
-            const float vol = SIXTH * dot(cross(c0, c1), c2);
-            const float rest = a.rvol[g];
-            const bool comp = a.sp.has_compress && (vol < rest);
-            const float k = comp ? cp.kcomp_vol : cp.k_vol;
-            if (k != 0.0f) {
-                const float alpha = comp ? dc.a_comp : dc.a_vol;
-                const float C = vol - rest;
-                float dsum = 0.f, gamma = 1.f;
-                if (DAMP && cp.damp_vol > 0.0f) {
-                    gamma = comp ? dc.g_comp : dc.g_vol;
-                    for (int i = 0; i < 4; ++i) {
-                        const float4 pp = spp[lv[i]];
-                        dsum += dot(gr[i], P[i] - xyz(pp));
-                    }
-                    dsum = dsum * gamma;
-                    gamma = gamma + 1.0f;
-                }
-                const float dl = LAM == 2 ? lam.y : (-C - alpha * lam.y - dsum) * __builtin_amdgcn_rcpf(gamma * wsum + alpha);
-                if (LAM == 1) nl.y = lam.y + dl;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) out.d[i] = out.d[i] + (dl * w[i]) * gr[i];
                 out.cnt += 1.0f;
             }
         }
@@ -601,24 +605,38 @@ __device__ __forceinline__ void finish_vertex(const IterArgs &a, int v, float4 x
     xout[v] = finish_value(a, v, x, pp, ax, ay, az, aw);
 }
 
-// Vertex pass of the flat strategy: each vertex adds its contiguous run of
-// corner corrections in ascending element order, then is finished.
+// Vertex pass of the flat strategy: four lanes per vertex read its contiguous
+// run of corner corrections (lane j takes corners j, j+4, ...; a wavefront's
+// loads then touch 16 runs rather than 64), then two cross-lane adds combine
+// them in a fixed order and the first lane finishes the vertex.
 template <int DAMP, int MODE>
 __global__ __launch_bounds__(256) void k_flat_verts(IterArgs a, const float4 *__restrict__ xin,
                                                     float4 *__restrict__ xout,
                                                     const float4 *__restrict__ pprev,
                                                     const float4 *__restrict__ corr, int nv) {
-    const int v = blockIdx.x * 256 + threadIdx.x;
-    if (v >= nv) return;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int v = t >> 2, j = t & 3;
     float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
-    const int e = a.vslot_off[v + 1];
-    for (int q = a.vslot_off[v]; q < e; ++q) {
-        const float4 c = corr[q];
-        ax += c.x;
-        ay += c.y;
-        az += c.z;
-        aw += c.w;
+    if (v < nv) {
+        const int e = a.vslot_off[v + 1];
+        for (int q = a.vslot_off[v] + j; q < e; q += 4) {
+            const float4 c = corr[q];
+            ax += c.x;
+            ay += c.y;
+            az += c.z;
+            aw += c.w;
+        }
     }
+    // lanes of a vertex are 4 consecutive lanes of one wavefront
+    ax += __shfl_xor(ax, 1);
+    ay += __shfl_xor(ay, 1);
+    az += __shfl_xor(az, 1);
+    aw += __shfl_xor(aw, 1);
+    ax += __shfl_xor(ax, 2);
+    ay += __shfl_xor(ay, 2);
+    az += __shfl_xor(az, 2);
+    aw += __shfl_xor(aw, 2);
+    if (v >= nv || j != 0) return;
     finish_vertex<DAMP, MODE>(a, v, xin[v], DAMP ? pprev[v] : make_float4(0, 0, 0, 0), ax, ay, az, aw, xout);
 }
 
@@ -2169,8 +2187,8 @@ int launch_iter(xpbd_ctx *c, const float4 *xin, float4 *xout, int mode, int lam 
         const float4 *cc = corr;
         int nv = c->nv;
         void *av[] = {(void *)&a, (void *)&xin, (void *)&xout, (void *)&pp, (void *)&cc, (void *)&nv};
-        HIPCHK(c, hipLaunchKernel((const void *)kv, dim3((unsigned)((nv + 255) / 256)), dim3(256), av, 0,
-                                  c->stream));
+        HIPCHK(c, hipLaunchKernel((const void *)kv, dim3((unsigned)((4 * (int64_t)nv + 255) / 256)), dim3(256), av,
+                                  0, c->stream));
     } else {
         IterKernel k = pick_iter(c->P, c->feature, mode);
         const size_t lds = iter_lds(c);
