@@ -281,6 +281,13 @@ int32_t xpbd_get_sdf_cells(xpbd_ctx *ctx, int32_t *cells);
 /* event timing of projection launches: enable=1 starts accumulating */
 int32_t xpbd_kernel_timing(xpbd_ctx *ctx, int32_t enable);
 int32_t xpbd_get_kernel_timing(xpbd_ctx *ctx, double *total_ms, int32_t *launches);
+/* average duration of one projection iteration (the kernels of xpbd_iterate,
+ * lambda mode 0): n iterations captured in a HIP graph, replayed reps times
+ * between two events on the context stream; X is restored afterwards.  Measures
+ * the kernels without per-launch host dispatch gaps.  No reference counterpart
+ * (the reference brackets sections with wp.synchronize() and perf_counter,
+ * tools/profile_humanV2.py:56-131). */
+int32_t xpbd_time_iterations(xpbd_ctx *ctx, int32_t n, int32_t reps, double *us_per_iter);
 
 #ifdef __cplusplus
 }

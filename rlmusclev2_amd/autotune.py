@@ -8,8 +8,8 @@ iteration, no replication, every corner correction goes through memory); and
 the "block" strategy (consecutive tets per workgroup, corner sums in LDS,
 only vertices shared between blocks go through memory) at several block
 sizes and thread counts.  Each candidate layout is built for the actual mesh,
-a few iterations are timed with HIP events around the projection launches,
-and the fastest is kept.  Results are cached per mesh signature in
+a few iterations are captured in a HIP graph and replayed between two HIP
+events (as they run inside a frame), and the fastest is kept.  Results are cached per mesh signature in
 ``rlmusclev2_amd/_tune/cache.json``.
 """
 from __future__ import annotations
@@ -61,12 +61,7 @@ def time_candidates(cfg: dict, w, candidates=CANDIDATES, iters: int = 20) -> dic
         except XpbdError:      # e.g. the layout needs more LDS than a workgroup can have
             continue
         sim.engine.iterate(2)
-        sim.engine.synchronize()
-        sim.engine.kernel_timing(True)
-        sim.engine.iterate(iters)
-        ms, n = sim.engine.get_kernel_timing()
-        sim.engine.kernel_timing(False)
-        out[c] = ms * 1e3 / max(n, 1)
+        out[c] = sim.engine.time_iterations(iters, 2)
         sim.engine.close()
     return out
 

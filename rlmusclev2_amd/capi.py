@@ -130,6 +130,7 @@ SIGNATURES = {
     "xpbd_snapshot": [_P, _I],
     "xpbd_kernel_timing": [_P, _I],
     "xpbd_get_kernel_timing": [_P, _P, _P],
+    "xpbd_time_iterations": [_P, _I, _I, _P],
 }
 _RESTYPE = {"xpbd_destroy": None, "xpbd_last_error": C.c_char_p}
 

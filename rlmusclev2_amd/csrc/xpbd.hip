@@ -199,6 +199,32 @@ __device__ __forceinline__ void closest_rotation(const float *F, float *R) {
     }
 }
 
+// ARAP-SVD constraint of one tet (solve_tetarap_svd_kernel, solve_kernels.py:1194-1307):
+// C = |sigma - 1| = |F - R|_F with R the closest rotation, dC/dF = (F - R) / C,
+// corner gradients g_k = dC/dF B[k,:]^T, g_3 = -sum.  Returns C, or 0 when the
+// constraint is satisfied (sum (sigma - 1)^2 < 1e-12).
+__device__ __forceinline__ float arap_grad(const float *F, const float *B, const float w[4], V3 gr[4], float &wsum) {
+    float R[9];
+    closest_rotation(F, R);
+    float G[9], c2s = 0.f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        G[i] = F[i] - R[i];
+        c2s += G[i] * G[i];
+    }
+    wsum = 0.f;
+    if (c2s < 1.0e-12f) return 0.f;
+    const float C = __builtin_amdgcn_sqrtf(c2s), invC = __builtin_amdgcn_rcpf(C);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) G[i] *= invC;
+    gr[0] = grad_row(G, B, 0);
+    gr[1] = grad_row(G, B, 1);
+    gr[2] = grad_row(G, B, 2);
+    gr[3] = -gr[0] - gr[1] - gr[2];
+    wsum = w[0] * dot(gr[0], gr[0]) + w[1] * dot(gr[1], gr[1]) + w[2] * dot(gr[2], gr[2]) + w[3] * dot(gr[3], gr[3]);
+    return C;
+}
+
 // ---------------------------------------------------------------------------
 // per-element projection (all constraint types of one tet, Jacobi, lambda = 0)
 // ---------------------------------------------------------------------------
@@ -284,25 +310,10 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
 
     // ---- ARAP-SVD: solve_tetarap_svd_kernel, solve_kernels.py:1194-1307 ----
     if ((FEAT & F_ARAP) && (mask & M_ARAP)) {
-        float R[9];
-        closest_rotation(F, R);
-        float G[9], c2s = 0.f;
-#pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            G[i] = F[i] - R[i];
-            c2s += G[i] * G[i];
-        }
-        if (!(c2s < 1.0e-12f)) {
-            const float C = __builtin_amdgcn_sqrtf(c2s), invC = __builtin_amdgcn_rcpf(C);
-#pragma unroll
-            for (int i = 0; i < 9; ++i) G[i] *= invC;
-            V3 gr[4];
-            gr[0] = grad_row(G, B, 0);
-            gr[1] = grad_row(G, B, 1);
-            gr[2] = grad_row(G, B, 2);
-            gr[3] = -gr[0] - gr[1] - gr[2];
-            const float wsum = w[0] * dot(gr[0], gr[0]) + w[1] * dot(gr[1], gr[1]) +
-                               w[2] * dot(gr[2], gr[2]) + w[3] * dot(gr[3], gr[3]);
+        V3 gr[4];
+        float wsum;
+        const float C = arap_grad(F, B, w, gr, wsum);
+        if (C > 0.0f) {
             if (wsum != 0.0f) {
                 const float dL = LAM == 2 ? lam.x : -(C + dc.at_arap * lam.x) * __builtin_amdgcn_rcpf(wsum + dc.at_arap);
                 if (LAM == 1) nl.x = lam.x + dL;
@@ -802,7 +813,8 @@ __global__ __launch_bounds__(256) void k_warm_start(IterArgs a, WarmNet net, con
     constexpr int NI = XPBD_MLP_IN, NH = XPBD_MLP_HIDDEN, NO = XPBD_MLP_OUT;
     __shared__ uint16_t sW1[NH * NI], sW2[NH * NH];
     __shared__ float sb1[NH], sb2[NH], sW3[NO * NH], sb3[NO], smu[NI], sinv[NI], sos[NO];
-    __shared__ float sH[4][32 * NH];
+    constexpr int NHP = NH + 1;   // padded rows: a lane reads its own row without bank conflicts
+    __shared__ float sH[4][32 * NHP];
     const int tid = threadIdx.x;
     for (int i = tid; i < NH * NI; i += 256) sW1[i] = net.W1[i];
     for (int i = tid; i < NH * NH; i += 256) sW2[i] = net.W2[i];
@@ -856,7 +868,7 @@ __global__ __launch_bounds__(256) void k_warm_start(IterArgs a, WarmNet net, con
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
                 const int c = 32 * t + col;
-                H[((i >> 2) * 8 + 4 * h + (i & 3)) * NH + c] = fmaxf(acc[t][i] + sb1[c], 0.f);
+                H[((i >> 2) * 8 + 4 * h + (i & 3)) * NHP + c] = fmaxf(acc[t][i] + sb1[c], 0.f);
             }
         __syncthreads();
         // layer 2: K = 64, four k-steps
@@ -868,7 +880,7 @@ __global__ __launch_bounds__(256) void k_warm_start(IterArgs a, WarmNet net, con
         for (int s = 0; s < 4; ++s) {
             ws_bf16x8 av;
 #pragma unroll
-            for (int i = 0; i < 8; ++i) av[i] = (__bf16)H[col * NH + 16 * s + 8 * h + i];
+            for (int i = 0; i < 8; ++i) av[i] = (__bf16)H[col * NHP + 16 * s + 8 * h + i];
 #pragma unroll
             for (int t = 0; t < 2; ++t) {
                 ws_bf16x8 bv;
@@ -884,7 +896,7 @@ __global__ __launch_bounds__(256) void k_warm_start(IterArgs a, WarmNet net, con
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
                 const int c = 32 * t + col;
-                H[((i >> 2) * 8 + 4 * h + (i & 3)) * NH + c] = fmaxf(acc[t][i] + sb2[c], 0.f);
+                H[((i >> 2) * 8 + 4 * h + (i & 3)) * NHP + c] = fmaxf(acc[t][i] + sb2[c], 0.f);
             }
         __syncthreads();
         // layer 3 on the vector ALUs: this lane computes outputs 2h and 2h+1 of its row
@@ -894,7 +906,7 @@ __global__ __launch_bounds__(256) void k_warm_start(IterArgs a, WarmNet net, con
             for (int oo = 0; oo < 2; ++oo) {
                 const int o = 2 * h + oo;
                 float v = sb3[o];
-                for (int k = 0; k < NH; ++k) v += H[col * NH + k] * sW3[o * NH + k];
+                for (int k = 0; k < NH; ++k) v += H[col * NHP + k] * sW3[o * NH + k];
                 y[oo] = v;
             }
             const float4 lp = base[row];
@@ -1099,6 +1111,21 @@ struct MgArgs {
     const int64_t *goff;                 // per block: offset of G_b
     const float *G;
     const int32_t *dev_of;               // block order -> coarse device id
+    float4 *q;                           // coarse positions, block (caller) order
+};
+
+// the whole coarse solve of one cage block in one workgroup: nodes and the
+// corner corrections of its tets live in LDS for all sweeps (_coarse_arap_sweep,
+// mg.py:939-952: clear, ARAP-SVD Jacobi with lambda = 0, apply dP / dPw)
+struct MgCoarse {
+    const int32_t *tet_off;      // nblk + 1
+    const ushort4 *tlidx;        // per tet: block-local node ids
+    const float *tB;             // per tet: rest inverse B, 9 floats row-major
+    const float *tat;            // per tet: 1 / (k dt^2)
+    const float *node_w;         // per node: inverse mass (0 = held)
+    const int32_t *slot_off;     // per node + 1: its corner slots (block-local, ascending)
+    const uint16_t *slots;
+    int max_nodes, max_tets;
 };
 
 __global__ __launch_bounds__(256) void k_mg_fit(MgArgs m, const float4 *__restrict__ xf) {
@@ -1136,17 +1163,30 @@ __global__ __launch_bounds__(256) void k_mg_fit(MgArgs m, const float4 *__restri
     }
 }
 
-__global__ void k_mg_rhs(MgArgs m, int nc, const float4 *__restrict__ xf) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+// sum over a wavefront in a fixed tree order
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// one wavefront per coarse node: the lanes split its P^T entries
+__global__ __launch_bounds__(256) void k_mg_rhs(MgArgs m, int nc, const float4 *__restrict__ xf) {
+    const int j = (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6);
+    const int l = threadIdx.x & 63;
     if (j >= nc) return;
     float sx = 0.f, sy = 0.f, sz = 0.f;
-    for (int e = m.ptoff[j]; e < m.ptoff[j + 1]; ++e) {
+    for (int e = m.ptoff[j] + l; e < m.ptoff[j + 1]; e += 64) {
         const float w = m.ptw[e];
         const float4 x = xf[m.ptfv[e]];
         sx += w * x.x;
         sy += w * x.y;
         sz += w * x.z;
     }
+    sx = wave_sum(sx);
+    sy = wave_sum(sy);
+    sz = wave_sum(sz);
+    if (l != 0) return;
     const int b = m.nblk_of[j];
     const float *T = m.fitT + 12 * b;
     const float4 q = m.q0c[j];
@@ -1157,27 +1197,106 @@ __global__ void k_mg_rhs(MgArgs m, int nc, const float4 *__restrict__ xf) {
     m.rhs[j] = make_float4(sx, sy, sz, 0.f);
 }
 
-__global__ void k_mg_solve(MgArgs m, int nc, float4 *__restrict__ xc, float4 *__restrict__ qprev) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+// one wavefront per coarse node: coalesced reads of its row of G_b
+__global__ __launch_bounds__(256) void k_mg_solve(MgArgs m, int nc, float4 *__restrict__ qprev) {
+    const int j = (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6);
+    const int l = threadIdx.x & 63;
     if (j >= nc) return;
     const int b = m.nblk_of[j];
     const int s = m.boff[b], n = m.boff[b + 1] - s;
     const float *row = m.G + m.goff[b] + (int64_t)(j - s) * n;
     float sx = 0.f, sy = 0.f, sz = 0.f;
-    for (int l = 0; l < n; ++l) {
-        const float g = row[l];
-        const float4 r = m.rhs[s + l];
+    for (int k = l; k < n; k += 64) {
+        const float g = row[k];
+        const float4 r = m.rhs[s + k];
         sx += g * r.x;
         sy += g * r.y;
         sz += g * r.z;
     }
-    const int d = m.dev_of[j];
-    float4 q = xc[d];
-    q.x = sx;
-    q.y = sy;
-    q.z = sz;
-    xc[d] = q;
-    qprev[d] = q;
+    sx = wave_sum(sx);
+    sy = wave_sum(sy);
+    sz = wave_sum(sz);
+    if (l != 0) return;
+    const float4 q = make_float4(sx, sy, sz, 0.f);
+    m.q[j] = q;
+    qprev[j] = q;
+}
+
+constexpr int MGC_T = 512;   // threads per coarse block workgroup
+
+__global__ __launch_bounds__(MGC_T) void k_mg_coarse(MgArgs m, MgCoarse mc, int iters, float4 *__restrict__ xc) {
+    extern __shared__ float4 smem[];
+    float4 *sq = smem;                         // xyz, inverse mass
+    float4 *sc = smem + mc.max_nodes;          // corner-major: sc[k * max_tets + t]
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const int n0 = m.boff[b], nn = m.boff[b + 1] - n0;
+    const int t0 = mc.tet_off[b], nt = mc.tet_off[b + 1] - t0;
+    const int MT = mc.max_tets;
+    for (int i = tid; i < nn; i += MGC_T) {
+        const float4 q = m.q[n0 + i];
+        sq[i] = make_float4(q.x, q.y, q.z, mc.node_w[n0 + i]);
+    }
+    for (int it = 0; it < iters; ++it) {
+        __syncthreads();
+        for (int t = tid; t < nt; t += MGC_T) {
+            const ushort4 li = mc.tlidx[t0 + t];
+            const float4 X0 = sq[li.x], X1 = sq[li.y], X2 = sq[li.z], X3 = sq[li.w];
+            const V3 c0 = xyz(X0) - xyz(X3), c1 = xyz(X1) - xyz(X3), c2 = xyz(X2) - xyz(X3);
+            const float w[4] = {X0.w, X1.w, X2.w, X3.w};
+            float B[9], F[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) B[q] = mc.tB[9 * (int64_t)(t0 + t) + q];
+            const float Ds[9] = {c0.x, c1.x, c2.x, c0.y, c1.y, c2.y, c0.z, c1.z, c2.z};
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    F[3 * i + j] = Ds[3 * i] * B[j] + Ds[3 * i + 1] * B[3 + j] + Ds[3 * i + 2] * B[6 + j];
+            V3 gr[4];
+            float wsum;
+            const float C = arap_grad(F, B, w, gr, wsum);
+            float dL = 0.f, cnt = 0.f;
+            if (C > 0.0f && wsum != 0.0f) {
+                const float at = mc.tat[t0 + t];
+                dL = -C * __builtin_amdgcn_rcpf(wsum + at);
+                cnt = 1.0f;
+            } else {
+                gr[0] = gr[1] = gr[2] = gr[3] = mk3(0.f, 0.f, 0.f);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const V3 d = (dL * w[k]) * gr[k];
+                sc[k * MT + t] = make_float4(d.x, d.y, d.z, cnt);
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i < nn; i += MGC_T) {
+            float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
+            const int e = mc.slot_off[n0 + i + 1];
+            for (int q = mc.slot_off[n0 + i]; q < e; ++q) {
+                const int sl = mc.slots[q];
+                const float4 c = sc[(sl & 3) * MT + (sl >> 2)];
+                ax += c.x;
+                ay += c.y;
+                az += c.z;
+                aw += c.w;
+            }
+            if (aw > 1e-9f) {   // apply_dP_kernel, engine.py:689-698
+                float4 x = sq[i];
+                x.x = x.x + ax / aw;
+                x.y = x.y + ay / aw;
+                x.z = x.z + az / aw;
+                sq[i] = x;
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < nn; i += MGC_T) {
+        const float4 x = sq[i];
+        m.q[n0 + i] = make_float4(x.x, x.y, x.z, 0.f);
+        float4 *o = xc + m.dev_of[n0 + i];   // the coarse context's own state, for readback
+        *o = make_float4(x.x, x.y, x.z, o->w);
+    }
 }
 
 __global__ void k_mg_prolong(int nv, const int4 *__restrict__ pidx, const float4 *__restrict__ pw,
@@ -1774,6 +1893,17 @@ struct EvPair {
 
 }  // namespace
 
+// Double-buffered pinned host staging for per-frame uploads: the host fills one
+// slot while the stream may still be copying from the other, so per-frame
+// setters never wait for the GPU (only for the copy issued two calls ago).
+struct HostStage {
+    void *buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool pending[2] = {false, false};
+    size_t cap = 0;
+    int next = 0;
+};
+
 struct xpbd_ctx {
     int device = 0;
     hipStream_t own_stream = nullptr, stream = nullptr;
@@ -1792,6 +1922,13 @@ struct xpbd_ctx {
     // pins
     std::vector<int32_t> pin_vids, pin_active;
     std::vector<float> pin_targets;
+    // per-frame target path: PIN already holds the full table for the current
+    // pin set, only the targeted active pins' rows are rewritten on the device
+    bool pins_full = false;
+    int n_pin_dev = 0, pin_cap = 0;
+    int32_t *d_pin_dev = nullptr;    // device vertex ids of targeted active pins (pin order)
+    float4 *d_pin_tgt = nullptr;
+    HostStage st_pin, st_act, st_bone;
     // sdf
     int nbones = 0, res = 0;
     std::vector<float> sdf_grids, sdf_origin, sdf_invh, coll_w, bone_xf;
@@ -1834,6 +1971,8 @@ struct xpbd_ctx {
     float *d_mg_pw = nullptr;
     float4 *MG_QPREV = nullptr;
     MgArgs mg{};
+    MgCoarse mgc{};
+    size_t mgc_lds = 0;
     float *d_Bp = nullptr, *d_rvol = nullptr, *d_rsnh = nullptr, *d_act = nullptr;
     DevClass *d_cls = nullptr;
     int4 *d_tets = nullptr;
@@ -1856,6 +1995,7 @@ struct xpbd_ctx {
     float *fmean = nullptr, *finv = nullptr, *oscale = nullptr;
     bool mlp_ready = false;
     float4 *S_X = nullptr, *S_PP = nullptr, *S_VEL = nullptr;
+    float4 *T_X = nullptr;   // scratch copy of X[0] while iterations are timed
     // tri-tri DCD
     bool dcd_on = false;
     int dcd_iters = 2, dcd_rebuild_substeps = 0;
@@ -1928,6 +2068,60 @@ void free_device(xpbd_ctx *c) {
     for (auto &b : c->bufs) (void)hipFree(b.p);
     c->bufs.clear();
     c->ready = false;
+    c->pins_full = false;
+    c->T_X = nullptr;
+    c->d_pin_dev = nullptr;
+    c->d_pin_tgt = nullptr;
+    c->pin_cap = 0;
+}
+
+void free_stage(HostStage &s) {
+    for (int i = 0; i < 2; ++i) {
+        if (s.ev[i]) (void)hipEventDestroy(s.ev[i]);
+        if (s.buf[i]) (void)hipHostFree(s.buf[i]);
+        s.ev[i] = nullptr;
+        s.buf[i] = nullptr;
+        s.pending[i] = false;
+    }
+    s.cap = 0;
+}
+
+// a pinned slot of at least `bytes` that no enqueued copy still reads
+int stage_acquire(xpbd_ctx *c, HostStage &s, size_t bytes, int &slot, void *&host) {
+    slot = s.next;
+    s.next ^= 1;
+    if (s.pending[slot]) {
+        HIPCHK(c, hipEventSynchronize(s.ev[slot]));
+        s.pending[slot] = false;
+    }
+    if (bytes > s.cap) {
+        for (int i = 0; i < 2; ++i)
+            if (s.pending[i]) {
+                HIPCHK(c, hipEventSynchronize(s.ev[i]));
+                s.pending[i] = false;
+            }
+        for (int i = 0; i < 2; ++i) {
+            if (s.buf[i]) (void)hipHostFree(s.buf[i]);
+            s.buf[i] = nullptr;
+        }
+        s.cap = 0;
+        for (int i = 0; i < 2; ++i) {
+            hipError_t e = hipHostMalloc(&s.buf[i], bytes, hipHostMallocDefault);
+            if (e != hipSuccess) return fail(c, XPBD_EOOM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+            if (!s.ev[i]) HIPCHK(c, hipEventCreateWithFlags(&s.ev[i], hipEventDisableTiming));
+        }
+        s.cap = bytes;
+    }
+    host = s.buf[slot];
+    return XPBD_OK;
+}
+
+// enqueue the host-to-device copy of a filled slot and mark it busy
+int stage_submit(xpbd_ctx *c, HostStage &s, int slot, void *dst, size_t bytes) {
+    HIPCHK(c, hipMemcpyAsync(dst, s.buf[slot], bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipEventRecord(s.ev[slot], c->stream));
+    s.pending[slot] = true;
+    return XPBD_OK;
 }
 
 void invalidate_graph(xpbd_ctx *c) {
@@ -1973,18 +2167,44 @@ int upload_pins(xpbd_ctx *c) {
         if (q[3] != 0.0f) { pin[n] = make_float4(q[0], q[1], q[2], 1.0f); any = true; }
     }
     const int np = (int)c->pin_vids.size();
+    std::vector<int32_t> dev;
     for (int i = 0; i < np; ++i) {
         if (!c->pin_active[i]) continue;
         const int n = c->L.iperm[c->pin_vids[i]];
         const float *t = c->pin_targets.empty() ? nullptr : &c->pin_targets[3 * (size_t)i];
         if (!t) continue;
         pin[n] = make_float4(t[0], t[1], t[2], 1.0f);
+        dev.push_back(n);
         any = true;
     }
     c->pins_on = any;
+    c->pins_full = false;
     HIPCHK(c, hipMemcpyAsync(c->PIN, pin.data(), nv * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+    if (!dev.empty()) {
+        if ((int)dev.size() > c->pin_cap) {
+            int r = dalloc(c, &c->d_pin_dev, (size_t)np);
+            if (!r) r = dalloc(c, &c->d_pin_tgt, (size_t)np);
+            if (r) return r;
+            c->pin_cap = np;
+        }
+        HIPCHK(c, hipMemcpyAsync(c->d_pin_dev, dev.data(), dev.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                                 c->stream));
+        c->n_pin_dev = (int)dev.size();
+        // a vertex pinned twice keeps the later target on the host; the device
+        // scatter would race, so such a pin set always takes the full upload
+        std::vector<int32_t> u(dev);
+        std::sort(u.begin(), u.end());
+        c->pins_full = std::adjacent_find(u.begin(), u.end()) == u.end();
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return XPBD_OK;
+}
+
+// per-frame pin targets: rewrite the targeted rows of the pin table in place
+__global__ __launch_bounds__(256) void k_pin_scatter(int n, const int32_t *__restrict__ dev,
+                                                     const float4 *__restrict__ tgt, float4 *__restrict__ pin) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) pin[dev[i]] = tgt[i];
 }
 
 std::vector<DevClass> dev_classes(const std::vector<ClassParams> &cls, const SimParams &sp) {
@@ -2396,24 +2616,16 @@ int lam_for_iteration(xpbd_ctx *c) {
 // restrict -> coarse Jacobi ARAP sweeps (coarse pins held as stopped vertices)
 // -> delta prolongation -> pins, on the fine context's stream
 int launch_vcycle(xpbd_ctx *c) {
-    xpbd_ctx *cc = c->coarse;
-    const int nc = cc->nv;
-    const dim3 gc((unsigned)((nc + 255) / 256));
+    const int nc = c->coarse->nv;
     hipLaunchKernelGGL(k_mg_fit, dim3((unsigned)c->mg_nblk), dim3(256), 0, c->stream, c->mg, (const float4 *)c->X[0]);
-    hipLaunchKernelGGL(k_mg_rhs, gc, dim3(256), 0, c->stream, c->mg, nc, (const float4 *)c->X[0]);
-    hipLaunchKernelGGL(k_mg_solve, gc, dim3(256), 0, c->stream, c->mg, nc, cc->X[0], c->MG_QPREV);
-    HIPCHK(c, hipGetLastError());
-    const hipStream_t saved = cc->stream;
-    cc->stream = c->stream;
-    int r = XPBD_OK;
-    for (int it = 0; it < c->mg_iters && r == XPBD_OK; ++it)
-        r = launch_iter(cc, cc->X[it & 1], cc->X[(it + 1) & 1], MODE_FUSED, 0);
-    cc->stream = saved;
-    if (r) return fail(c, r, std::string("coarse level: ") + cc->err);
+    const dim3 gw((unsigned)((64 * (int64_t)nc + 255) / 256));   // one wavefront per coarse node
+    hipLaunchKernelGGL(k_mg_rhs, gw, dim3(256), 0, c->stream, c->mg, nc, (const float4 *)c->X[0]);
+    hipLaunchKernelGGL(k_mg_solve, gw, dim3(256), 0, c->stream, c->mg, nc, c->MG_QPREV);
+    hipLaunchKernelGGL(k_mg_coarse, dim3((unsigned)c->mg_nblk), dim3(MGC_T), c->mgc_lds, c->stream, c->mg, c->mgc,
+                       c->mg_iters, c->coarse->X[0]);
     hipLaunchKernelGGL(k_mg_prolong, vgrid(c->nv), dim3(256), 0, c->stream, c->nv, (const int4 *)c->d_mg_pidx,
-                       (const float4 *)c->d_mg_pw, (const float4 *)cc->X[c->mg_iters & 1],
-                       (const float4 *)c->MG_QPREV, c->X[0], c->mg_kinematic ? c->PPv : nullptr,
-                       c->pins_on ? (const float4 *)c->PIN : nullptr);
+                       (const float4 *)c->d_mg_pw, (const float4 *)c->mg.q, (const float4 *)c->MG_QPREV, c->X[0],
+                       c->mg_kinematic ? c->PPv : nullptr, c->pins_on ? (const float4 *)c->PIN : nullptr);
     HIPCHK(c, hipGetLastError());
     if (c->pins_on) {
         hipLaunchKernelGGL(k_pins, vgrid(c->nv), dim3(256), 0, c->stream, c->nv, c->X[0], (const float4 *)c->PIN);
@@ -2485,6 +2697,9 @@ void xpbd_destroy(xpbd_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto &e : c->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
     free_device(c);
+    free_stage(c->st_pin);
+    free_stage(c->st_act);
+    free_stage(c->st_bone);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
 }
@@ -2573,14 +2788,18 @@ int32_t xpbd_set_activation(xpbd_ctx *c, const float *act) {
     if (!c || !act || c->nt == 0) return fail(c, XPBD_EINVAL, "bad activation");
     c->activation.assign(act, act + c->nt);
     if (c->ready && (c->feature & F_ANISO)) {
-        HIPCHK(c, hipMemcpyAsync(c->d_act, c->activation.data(), c->nt * sizeof(float),
-                                 hipMemcpyHostToDevice, c->stream));
+        int slot;
+        void *h;
+        const size_t bytes = (size_t)c->nt * sizeof(float);
+        int r = stage_acquire(c, c->st_act, bytes, slot, h);
+        if (r) return r;
+        std::memcpy(h, act, bytes);
+        if ((r = stage_submit(c, c->st_act, slot, c->d_act, bytes))) return r;
         const int64_t n = (int64_t)c->L.meta.size();
         hipLaunchKernelGGL(k_refresh_fiber, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n,
                            (const float4 *)c->FIBS, (const int32_t *)c->d_ptet_tetid, (const float *)c->d_act,
                            c->sp.contraction_ratio, c->FIB);
         HIPCHK(c, hipGetLastError());
-        HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     return XPBD_OK;
 }
@@ -2667,8 +2886,28 @@ int32_t xpbd_set_pins(xpbd_ctx *c, int32_t n, const int32_t *vids, const int32_t
 int32_t xpbd_set_pin_targets(xpbd_ctx *c, int32_t n, const float *xyz) {
     if (!c || n != (int)c->pin_vids.size() || (n > 0 && !xyz)) return fail(c, XPBD_EINVAL, "pin target count");
     const bool was_on = c->pins_on;
+    const bool had_targets = !c->pin_targets.empty();
     c->pin_targets.assign(xyz, xyz + 3 * (size_t)n);
     if (!c->ready) return XPBD_OK;
+    if (had_targets && c->pins_full && c->n_pin_dev > 0) {
+        // same pin set as the table on the device: stage the targets of the
+        // active pins (pin order, as d_pin_dev) and scatter them, no host wait
+        const int m = c->n_pin_dev;
+        int slot;
+        void *h;
+        int r = stage_acquire(c, c->st_pin, (size_t)m * sizeof(float4), slot, h);
+        if (r) return r;
+        float4 *t = (float4 *)h;
+        int k = 0;
+        for (int i = 0; i < n && k < m; ++i)
+            if (c->pin_active[i]) t[k++] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 1.0f);
+        if (k != m) return fail(c, XPBD_EINVAL, "pin set changed without xpbd_set_pins");
+        if ((r = stage_submit(c, c->st_pin, slot, c->d_pin_tgt, (size_t)m * sizeof(float4)))) return r;
+        hipLaunchKernelGGL(k_pin_scatter, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c->stream, m,
+                           (const int32_t *)c->d_pin_dev, (const float4 *)c->d_pin_tgt, c->PIN);
+        HIPCHK(c, hipGetLastError());
+        return XPBD_OK;
+    }
     int r = upload_pins(c);
     if (r) return r;
     if (was_on != c->pins_on) invalidate_graph(c);
@@ -2722,14 +2961,18 @@ int32_t xpbd_set_bone_transforms(xpbd_ctx *c, int32_t nbones, const float *xf) {
     if (!c || nbones != c->nbones || (nbones > 0 && !xf)) return fail(c, XPBD_EINVAL, "bone count mismatch");
     c->bone_xf.assign(xf, xf + 12 * (size_t)nbones);
     if (c->ready && nbones > 0) {
-        std::vector<float> bn(16 * (size_t)nbones, 0.f);
+        int slot;
+        void *h;
+        const size_t bytes = 16 * (size_t)nbones * sizeof(float);
+        int r = stage_acquire(c, c->st_bone, bytes, slot, h);
+        if (r) return r;
+        float *bn = (float *)h;
         for (int b = 0; b < nbones; ++b) {
             std::memcpy(&bn[16 * b], &c->bone_xf[12 * b], 12 * sizeof(float));
             std::memcpy(&bn[16 * b + 12], &c->sdf_origin[3 * b], 3 * sizeof(float));
             bn[16 * b + 15] = c->sdf_invh[b];
         }
-        HIPCHK(c, hipMemcpyAsync(c->d_bone, bn.data(), bn.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if ((r = stage_submit(c, c->st_bone, slot, c->d_bone, bytes))) return r;
     }
     return XPBD_OK;
 }
@@ -3286,7 +3529,7 @@ int32_t xpbd_set_coarse_level(xpbd_ctx *c, xpbd_ctx *cc, const xpbd_coarse_desc 
             const float w = d->p_w[8 * (size_t)i + k];
             if (j < 0 || j >= nc) return fail(c, XPBD_EINVAL, "prolongation index out of range");
             if (!(w >= 0.0f) || !(w <= 1.0f)) return fail(c, XPBD_EINVAL, "prolongation weight outside [0, 1]");
-            pidx[8 * (size_t)fi + k] = cc->L.iperm[j];
+            pidx[8 * (size_t)fi + k] = j;
             pw[8 * (size_t)fi + k] = w;
             if (w > 0.0f) pt[j].push_back({fi, w});
         }
@@ -3360,6 +3603,91 @@ int32_t xpbd_set_coarse_level(xpbd_ctx *c, xpbd_ctx *cc, const xpbd_coarse_desc 
     m.goff = p_goff;
     m.G = p_G;
     m.dev_of = p_dev;
+    // coarse cage per block: tets, rest matrices, compliances, inverse masses,
+    // per-node corner slots; all in the coarse context's caller order
+    {
+        const int nct = cc->nt;
+        std::vector<int32_t> tblk(nct, -1), toff(nb + 1, 0);
+        for (int t = 0; t < nct; ++t) {
+            const int v0 = cc->tets[4 * (size_t)t];
+            const int bb = (v0 >= 0 && v0 < nc) ? nblk_of[v0] : -1;
+            for (int k = 1; k < 4; ++k) {
+                const int vk = cc->tets[4 * (size_t)t + k];
+                if (vk < 0 || vk >= nc || nblk_of[vk] != bb) return fail(c, XPBD_EINVAL, "a coarse tet spans two blocks");
+            }
+            tblk[t] = bb;
+            toff[bb + 1]++;
+        }
+        for (int bb = 0; bb < nb; ++bb) toff[bb + 1] += toff[bb];
+        std::vector<int32_t> fill(toff.begin(), toff.end() - 1), order(nct);
+        for (int t = 0; t < nct; ++t) order[fill[tblk[t]]++] = t;
+        std::vector<ushort4> tl(std::max(nct, 1));
+        std::vector<float> tB(9 * (size_t)std::max(nct, 1)), tat(std::max(nct, 1));
+        // compliance from the coarse ARAP table (one constraint per tet)
+        std::vector<float> kt(nct, 0.f);
+        for (int q = 0; q < cc->ncons; ++q)
+            if (cc->c_type[q] == XPBD_CT_TETARAPSVD && cc->c_tetid[q] >= 0 && cc->c_tetid[q] < nct)
+                kt[cc->c_tetid[q]] = cc->c_stiff[q];
+        const float dt = cc->sp.dt;
+        std::vector<std::vector<uint16_t>> nslots(nc);
+        int max_nodes = 0, max_tets = 0;
+        for (int bb = 0; bb < nb; ++bb) {
+            max_nodes = std::max(max_nodes, d->blk_off[bb + 1] - d->blk_off[bb]);
+            max_tets = std::max(max_tets, toff[bb + 1] - toff[bb]);
+        }
+        if (4 * max_tets > 65535) return fail(c, XPBD_ELAYOUT, "coarse cage block too large for 16-bit slots");
+        for (int e = 0; e < nct; ++e) {
+            const int t = order[e];
+            const int bb = tblk[t], n0 = d->blk_off[bb], lt = e - toff[bb];
+            int lv[4];
+            for (int k = 0; k < 4; ++k) {
+                const int v = cc->tets[4 * (size_t)t + k];
+                lv[k] = v - n0;
+                nslots[v].push_back((uint16_t)(4 * lt + k));
+            }
+            tl[e] = make_ushort4((unsigned short)lv[0], (unsigned short)lv[1], (unsigned short)lv[2], (unsigned short)lv[3]);
+            for (int q = 0; q < 9; ++q) tB[9 * (size_t)e + q] = cc->rest_matrix[9 * (size_t)t + q];
+            tat[e] = kt[t] > 0.f ? (1.0f / kt[t]) / (dt * dt) : 0.f;
+        }
+        std::vector<int32_t> soff(nc + 1, 0);
+        std::vector<uint16_t> sl;
+        std::vector<float> nw(std::max(nc, 1), 0.f);
+        for (int v = 0; v < nc; ++v) {
+            std::sort(nslots[v].begin(), nslots[v].end());
+            sl.insert(sl.end(), nslots[v].begin(), nslots[v].end());
+            soff[v + 1] = (int32_t)sl.size();
+            nw[v] = inv_mass_of(cc, v);
+        }
+        if (sl.empty()) sl.push_back(0);
+        MgCoarse g{};
+        int32_t *p_toff, *p_soff;
+        ushort4 *p_tl;
+        float *p_tB, *p_tat, *p_nw;
+        uint16_t *p_sl;
+        if ((r = dupload(c, &p_toff, toff.data(), toff.size())) || (r = dupload(c, &p_tl, tl.data(), tl.size())) ||
+            (r = dupload(c, &p_tB, tB.data(), tB.size())) || (r = dupload(c, &p_tat, tat.data(), tat.size())) ||
+            (r = dupload(c, &p_nw, nw.data(), nw.size())) || (r = dupload(c, &p_soff, soff.data(), soff.size())) ||
+            (r = dupload(c, &p_sl, sl.data(), sl.size())) || (r = dalloc(c, &m.q, (size_t)std::max(nc, 1))))
+            return r;
+        g.tet_off = p_toff;
+        g.tlidx = p_tl;
+        g.tB = p_tB;
+        g.tat = p_tat;
+        g.node_w = p_nw;
+        g.slot_off = p_soff;
+        g.slots = p_sl;
+        g.max_nodes = max_nodes;
+        g.max_tets = std::max(max_tets, 1);
+        const size_t lds = ((size_t)max_nodes + 4 * (size_t)g.max_tets) * sizeof(float4);
+        int maxlds = 0;
+        HIPCHK(c, hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+        if (lds > (size_t)maxlds)
+            return fail(c, XPBD_ELAYOUT, "coarse cage block needs " + std::to_string(lds) + " B of LDS; limit " +
+                                             std::to_string(maxlds));
+        HIPCHK(c, hipFuncSetAttribute((const void *)k_mg_coarse, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        c->mgc = g;
+        c->mgc_lds = lds;
+    }
     c->mg = m;
     c->mg_nblk = nb;
     c->coarse = cc;
@@ -3459,6 +3787,48 @@ int32_t xpbd_kernel_timing(xpbd_ctx *c, int32_t enable) {
         c->timed_n = 0;
     }
     c->timing = enable != 0;
+    return XPBD_OK;
+}
+
+int32_t xpbd_time_iterations(xpbd_ctx *c, int32_t n, int32_t reps, double *us_per_iter) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (n < 2 || (n & 1) || reps < 1 || !us_per_iter)
+        return fail(c, XPBD_EINVAL, "time_iterations needs an even n >= 2, reps >= 1 and an output");
+    if (c->lam_mode != 0) return fail(c, XPBD_EUNSUPPORTED, "iteration timing runs the plain (lambda mode 0) kernels");
+    const size_t b = (size_t)c->nv * sizeof(float4);
+    if (!c->T_X && (r = dalloc(c, &c->T_X, (size_t)c->nv))) return r;
+    HIPCHK(c, hipMemcpyAsync(c->T_X, c->X[0], b, hipMemcpyDeviceToDevice, c->stream));
+    const bool tm = c->timing;
+    c->timing = false;
+    c->eff_mode = 0;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int it = 0; it < n && r == XPBD_OK; ++it) r = launch_iter(c, c->X[it & 1], c->X[(it + 1) & 1], MODE_FUSED, 0);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->timing = tm;
+    if (r) { if (g) (void)hipGraphDestroy(g); return r; }
+    if (e != hipSuccess) return fail(c, XPBD_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    hipGraphExec_t ge = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0.f;
+    hipError_t he = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (he == hipSuccess) he = hipEventCreate(&e0);
+    if (he == hipSuccess) he = hipEventCreate(&e1);
+    if (he == hipSuccess) he = hipGraphLaunch(ge, c->stream);   // warm
+    if (he == hipSuccess) he = hipEventRecord(e0, c->stream);
+    for (int k = 0; k < reps && he == hipSuccess; ++k) he = hipGraphLaunch(ge, c->stream);
+    if (he == hipSuccess) he = hipEventRecord(e1, c->stream);
+    if (he == hipSuccess) he = hipEventSynchronize(e1);
+    if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (ge) (void)hipGraphExecDestroy(ge);
+    (void)hipGraphDestroy(g);
+    if (he != hipSuccess) return fail(c, XPBD_EHIP, std::string("iteration timing: ") + hipGetErrorString(he));
+    HIPCHK(c, hipMemcpyAsync(c->X[0], c->T_X, b, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *us_per_iter = 1e3 * (double)ms / ((double)n * reps);
     return XPBD_OK;
 }
 

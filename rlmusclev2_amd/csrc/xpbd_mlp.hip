@@ -23,6 +23,7 @@ typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 constexpr int NIN = XPBD_MLP_IN;    // 24
 constexpr int NH = XPBD_MLP_HIDDEN; // 64
 constexpr int NOUT = XPBD_MLP_OUT;  // 4
+constexpr int NHP = NH + 1;         // padded LDS row of one activation vector
 
 __device__ __forceinline__ __bf16 tobf(float x) { return (__bf16)x; }
 __device__ __forceinline__ __bf16 bfraw(uint16_t v) { return __builtin_bit_cast(__bf16, v); }
@@ -39,7 +40,9 @@ __device__ __forceinline__ int drow(int i, int h) { return (i >> 2) * 8 + 4 * h 
 __global__ __launch_bounds__(256) void k_mlp_fwd(xpbd_mlp_dev net, const float *__restrict__ X, int64_t n,
                                                  float *__restrict__ Y, float *__restrict__ H1g,
                                                  float *__restrict__ H2g) {
-    __shared__ float sh[4][32 * NH];
+    // activation rows are padded to NHP floats: a lane reads its own row, and an
+    // unpadded stride of 64 floats would put all 32 rows in the same LDS bank
+    __shared__ float sh[4][32 * NHP];
     __shared__ float w3[NOUT * NH], b3[NOUT];
     const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, col = l & 31, h = l >> 5;
     for (int i = threadIdx.x; i < NOUT * NH; i += blockDim.x) w3[i] = net.W3[i];
@@ -71,20 +74,20 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(xpbd_mlp_dev net, const float *
     for (int t = 0; t < 2; ++t)
         for (int i = 0; i < 16; ++i) {
             const int c = 32 * t + col;
-            H[drow(i, h) * NH + c] = fmaxf(acc[t][i] + net.b1[c], 0.f);
+            H[drow(i, h) * NHP + c] = fmaxf(acc[t][i] + net.b1[c], 0.f);
         }
     __syncthreads();
     if (H1g)
         for (int i = l; i < 32 * NH; i += 64) {
             const int64_t rr = r0 + i / NH;
-            if (rr < n) H1g[rr * NH + (i % NH)] = H[i];
+            if (rr < n) H1g[rr * NH + (i % NH)] = H[(i / NH) * NHP + i % NH];
         }
     // ---- layer 2: K = 64 (4 k-steps) ----
     for (int t = 0; t < 2; ++t)
         for (int i = 0; i < 16; ++i) acc[t][i] = 0.f;
     for (int s = 0; s < 4; ++s) {
         bf16x8 a;
-        for (int i = 0; i < 8; ++i) a[i] = tobf(H[col * NH + 16 * s + 8 * h + i]);
+        for (int i = 0; i < 8; ++i) a[i] = tobf(H[col * NHP + 16 * s + 8 * h + i]);
         for (int t = 0; t < 2; ++t) {
             bf16x8 b;
             for (int i = 0; i < 8; ++i) b[i] = bfraw(net.W2[(32 * t + col) * NH + 16 * s + 8 * h + i]);
@@ -95,20 +98,20 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(xpbd_mlp_dev net, const float *
     for (int t = 0; t < 2; ++t)
         for (int i = 0; i < 16; ++i) {
             const int c = 32 * t + col;
-            H[drow(i, h) * NH + c] = fmaxf(acc[t][i] + net.b2[c], 0.f);
+            H[drow(i, h) * NHP + c] = fmaxf(acc[t][i] + net.b2[c], 0.f);
         }
     __syncthreads();
     if (H2g)
         for (int i = l; i < 32 * NH; i += 64) {
             const int64_t rr = r0 + i / NH;
-            if (rr < n) H2g[rr * NH + (i % NH)] = H[i];
+            if (rr < n) H2g[rr * NH + (i % NH)] = H[(i / NH) * NHP + i % NH];
         }
     // ---- layer 3 (64 -> 4) on the vector ALUs: lane (row col, outputs 2h, 2h+1) ----
     if (rv) {
         for (int oo = 0; oo < 2; ++oo) {
             const int o = 2 * h + oo;
             float y = b3[o];
-            for (int k = 0; k < NH; ++k) y += H[col * NH + k] * w3[o * NH + k];
+            for (int k = 0; k < NH; ++k) y += H[col * NHP + k] * w3[o * NH + k];
             Y[row * NOUT + o] = y;
         }
     }

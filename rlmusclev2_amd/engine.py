@@ -348,5 +348,11 @@ class XpbdEngine:
         self.ctx.call("xpbd_get_kernel_timing", ptr(ms), ptr(n))
         return float(ms[0]), int(n[0])
 
+    def time_iterations(self, n: int = 10, reps: int = 20) -> float:
+        """Graph-replayed projection iterations timed with HIP events: us per iteration."""
+        us = np.zeros(1, np.float64)
+        self.ctx.call("xpbd_time_iterations", int(n), int(reps), ptr(us))
+        return float(us[0])
+
     def close(self):
         self.ctx.close()

@@ -320,14 +320,27 @@ def bone_xforms(w: Workload, frame: int) -> np.ndarray:
     return out
 
 
+def _pin_groups(w: Workload):
+    """(rest positions of the pins, per-bone pin index lists), built once per workload."""
+    cache = w.info.get("_pin_groups")
+    if cache is None or cache[0] is not w.pin_vids:
+        x0 = w.pos0[w.pin_vids].astype(np.float64)
+        order = np.argsort(w.pin_group, kind="stable")
+        cuts = np.searchsorted(w.pin_group[order], np.arange(len(w.bones) + 1))
+        cache = (w.pin_vids, x0, [order[cuts[b]:cuts[b + 1]] for b in range(len(w.bones))])
+        w.info["_pin_groups"] = cache
+    return cache[1], cache[2]
+
+
 def pin_targets(w: Workload, frame: int) -> np.ndarray:
     """Rigid bone motion applied to the rest position of each pinned vertex."""
-    x0 = w.pos0[w.pin_vids].astype(np.float64)
+    x0, groups = _pin_groups(w)
     out = np.empty_like(x0)
     for b, bn in enumerate(w.bones):
-        sel = w.pin_group == b
-        R, t = bone_transform(bn, frame)
-        out[sel] = x0[sel] @ R.T + t
+        sel = groups[b]
+        if sel.size:
+            R, t = bone_transform(bn, frame)
+            out[sel] = x0[sel] @ R.T + t
     return out.astype(np.float32)
 
 

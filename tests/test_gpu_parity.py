@@ -172,3 +172,43 @@ def test_sdf_cells_and_push_bit_exact():
     assert (cr[:, 0] >= 0).sum() > 100
     np.testing.assert_array_equal(cg, cr)
     np.testing.assert_array_equal(sim.positions_np(), ref.positions_np())
+
+
+def test_pin_targets_fast_path_and_iteration_timing():
+    """per-frame targets go through the on-device scatter after the first full
+    upload: pinned vertices land exactly on them; timing leaves the state alone"""
+    cfg = _cfg("humanv2_scale.json")
+    w = workloads.humanv2_scale(1, n_bones=2, per_bone=4, target_tets=6000, fascia_tets=1000, sdf_res=32)
+    cfg["synthetic"]["sdf_res"] = 32
+    sim = sims.MultiGridSim(cfg, workload=w, strategy=2, block_tets=256)
+    for f in (1, 5, 9):                 # the first call uploads the table, the rest scatter
+        sim.step_frame(f)
+        t = workloads.pin_targets(w, f + 1)
+        sim.engine.set_pin_targets(t)
+        sim.engine.apply_pintotarget()
+        p = sim.positions_np()
+        np.testing.assert_array_equal(p[w.pin_vids], t)
+    x0 = sim.positions_np()
+    us = sim.engine.time_iterations(10, 3)
+    print("us per iteration", us)
+    assert us > 0.0
+    np.testing.assert_array_equal(sim.positions_np(), x0)
+
+
+def test_block_iteration_sequence_equals_single_iterations():
+    """xpbd_iterate(n) == n calls of xpbd_iterate(1), bitwise (block strategy,
+    odd and even n: the odd count ends with the copy back into X[0])"""
+    cfg = _cfg("humanv2_scale.json")
+    w = workloads.humanv2_scale(1, n_bones=2, per_bone=4, target_tets=6000, fascia_tets=1000, sdf_res=32)
+    cfg["synthetic"]["sdf_res"] = 32
+    a = sims.MultiGridSim(cfg, workload=w, strategy=2, block_tets=256)
+    b = sims.MultiGridSim(cfg, workload=w, strategy=2, block_tets=256)
+    for s in (a, b):
+        s.load_frame_targets(4)
+        s.engine.apply_pintotarget()
+        s.engine.integrate()
+    for n in (5, 4):
+        a.engine.iterate(n)
+        for _ in range(n):
+            b.engine.iterate(1)
+        np.testing.assert_array_equal(a.positions_np(), b.positions_np())
