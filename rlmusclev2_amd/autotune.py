@@ -25,7 +25,8 @@ from .capi import XpbdError
 CACHE = Path(__file__).resolve().parent / "_tune" / "cache.json"
 # (strategy, patch_verts, block_tets, block_threads)
 CANDIDATES = ((1, 256, 0, 0), (0, 64, 0, 0), (0, 128, 0, 0), (0, 256, 0, 0),
-              (2, 256, 256, 256), (2, 256, 512, 256), (2, 256, 512, 512), (2, 256, 1024, 512))
+              (2, 256, 128, 128), (2, 256, 192, 192), (2, 256, 256, 256), (2, 256, 384, 384),
+              (2, 256, 512, 256), (2, 256, 512, 512), (2, 256, 768, 384), (2, 256, 1024, 512))
 
 
 def name(c) -> str:

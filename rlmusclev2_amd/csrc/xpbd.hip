@@ -708,8 +708,7 @@ __global__ __launch_bounds__(512) void k_block_tets(IterArgs a, BlockArgs b, con
         const int s0 = (int)(run & 0xffffu), n = (int)(run >> 16);
         float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
         for (int j = 0; j < n; ++j) {
-            const int sl = slots[s0 + j];
-            const float4 q = sc[(sl & 3) * TB + (sl >> 2)];
+            const float4 q = sc[slots[s0 + j]];
             ax += q.x;
             ay += q.y;
             az += q.z;

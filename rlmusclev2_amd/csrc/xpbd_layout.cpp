@@ -229,7 +229,9 @@ std::string build_blocks(const HostInput &in, Layout &L, int feature, int *code)
         for (int64_t g = t0; g < t1; ++g)
             for (int k = 0; k < 4; ++k) {
                 const int l = L.blidx[4 * g + k];
-                L.bslot[4 * t0 + fill[l]++] = (uint16_t)((g - t0) * 4 + k);
+                // listed in ascending (element, corner) order; the value is the
+                // corner-major LDS index k * TB + element the kernel reads
+                L.bslot[4 * t0 + fill[l]++] = (uint16_t)(k * TB + (g - t0));
             }
         for (int v : verts) loc[v] = -1;
     }

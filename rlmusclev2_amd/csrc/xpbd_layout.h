@@ -83,7 +83,8 @@ struct Layout {
     std::vector<int32_t> bps;                // per local entry: partial slot, -1 = finished in the block
     std::vector<uint32_t> brun;              // per local entry: first slot | slot count << 16
     std::vector<uint16_t> blidx;             // 4 per element: local entry of each corner
-    std::vector<uint16_t> bslot;             // 4 per element, per block: corner slots grouped by local entry
+    std::vector<uint16_t> bslot;             // 4 per element, per block: corner slots grouped by local entry,
+                                             // ascending (element, corner); value = corner * block_tets + element
     std::vector<int32_t> svid;               // vertices finished by the second pass
     std::vector<int32_t> soff;               // svid.size()+1 offsets into the partial array
     int32_t max_nloc = 0, block_tets = 0, n_partial = 0;

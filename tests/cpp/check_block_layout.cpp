@@ -64,11 +64,13 @@ int main(int argc, char **argv) {
             int prev = -1;
             for (int j = 0; j < cnt; ++j) {
                 if (s0 + j >= 4 * D.ntet) { printf("slot range\n"); ++bad; break; }
-                int sl = L.bslot[4 * (size_t)D.tbeg + s0 + j];
-                if (sl >= 4 * D.ntet || sl <= prev) { printf("slot order/range\n"); ++bad; }
+                const int code = L.bslot[4 * (size_t)D.tbeg + s0 + j];   // corner * block_tets + element
+                const int el = code % L.block_tets, k = code / L.block_tets;
+                const int sl = 4 * el + k;
+                if (k > 3 || el >= D.ntet || sl <= prev) { printf("slot order/range\n"); ++bad; }
                 prev = sl;
                 seen[sl]++;
-                int g = D.tbeg + sl / 4, k = sl % 4;
+                int g = D.tbeg + el;
                 if (L.gidx[4 * g + k] != v || L.blidx[4 * g + k] != i) { printf("slot vertex mismatch\n"); ++bad; }
                 sum += val[4 * g + k];
             }
