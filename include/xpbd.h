@@ -193,7 +193,8 @@ typedef struct {
     float friction_static, friction_dynamic;
     float offset;             /* narrow-phase accept threshold (1e-6)                   */
     float normal_cull;        /* >= 1 disables the co-oriented self-pair cull           */
-    float cell_size;          /* 0 = 2 x mean rest edge (never below 2 r_max + thickness) */
+    float cell_size;          /* base cell of the grid hierarchy; 0 = 1.01 (2 r90 + thickness), r90 the
+                                 90th percentile triangle radius at each rebuild */
     int32_t allow_self;       /* movable-vs-movable pairs                               */
     int32_t iters;            /* DCD passes per substep (humanV2 "iters", default 2)    */
     int32_t rebuild_substeps; /* 1 = broadphase every substep, 0 = once per frame       */
@@ -210,8 +211,9 @@ int32_t xpbd_dcd_pairs(xpbd_ctx *ctx, int32_t *out, int64_t cap, int64_t *n_tota
 /* UdcfDcdCollider.stats (solver.py:442-457): raw pair count, contacts of the last pass, cell */
 int32_t xpbd_dcd_stats(xpbd_ctx *ctx, int64_t *n_pairs, int64_t *n_contacts, float *cell);
 /* broadphase limits hit at the last rebuild (0 and 0 on a valid mesh): triangles left out
- * (bounding sphere too large for the capped cell, or non-finite centroid) and triangles
- * whose candidate list was truncated.  No reference counterpart. */
+ * (bounding sphere above the top grid level, or a non-finite or out-of-range centroid) and
+ * truncated work (searches stopped at the visit or pair limit, plus pair segments too long
+ * to sort).  No reference counterpart. */
 int32_t xpbd_dcd_overflow(xpbd_ctx *ctx, int64_t *left_out, int64_t *truncated);
 
 /* ---- learned warm start: per-tet MLP (no reference counterpart) ------- */

@@ -1447,16 +1447,24 @@ __global__ void k_count_inverted(int nt, const int4 *tets, const float *B, const
 #define DCD_THRESH 1.0e-5f
 constexpr float DCD_FIX = 4294967296.0f;  // 2^32 fixed-point scale (metres)
 
-__device__ __forceinline__ uint32_t dcd_hash(int x, int y, int z, int bits) {
-    const uint32_t h = ((uint32_t)x * 73856093u) ^ ((uint32_t)y * 19349663u) ^ ((uint32_t)z * 83492791u);
+__device__ __forceinline__ uint32_t dcd_hash(int x, int y, int z, int level, int bits) {
+    const uint32_t h = ((uint32_t)x * 73856093u) ^ ((uint32_t)y * 19349663u) ^ ((uint32_t)z * 83492791u) ^
+                       ((uint32_t)level * 2654435761u);
     return h & ((1u << bits) - 1u);
 }
 
 // centroid_radius_kernel (:47-63) + face_normal_kernel (:66-84) + max radius
-__global__ void k_dcd_tridata(DcdDev d, const float4 *X) {
+// (max_radius_reduce_kernel :87-100) + a log2 histogram of the radii
+// (the maximum and the histogram are reduced in the workgroup first: one
+// device-scope atomic per wavefront / per non-empty bin rather than per triangle)
+__global__ __launch_bounds__(256) void k_dcd_tridata(DcdDev d, const float4 *X) {
 #pragma clang fp contract(off)
+    __shared__ uint32_t sh[DCD_HIST];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= d.ntris) return;
+    if (threadIdx.x < DCD_HIST) sh[threadIdx.x] = 0u;
+    __syncthreads();
+    uint32_t rbits = 0u;
+    if (i < d.ntris) {
     const int4 t = d.tris[i];
     const V3 a = xyz(X[t.x]), b = xyz(X[t.y]), c = xyz(X[t.z]);
     const V3 s = (a + b) + c;
@@ -1469,44 +1477,93 @@ __global__ void k_dcd_tridata(DcdDev d, const float4 *X) {
     const V3 n = cross(b - a, c - a);
     const float ln = sqrtf(dot(n, n));
     d.nrm[i] = ln > 0.0f ? make_float4(n.x / ln, n.y / ln, n.z / ln, 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
-    atomicMax(d.rmax_bits, __float_as_uint(r));  // r >= 0: uint order == float order
+    if (isfinite(r)) {
+        rbits = __float_as_uint(r);  // r >= 0: uint order == float order
+        // bin k holds r in [2^((k-80)/4), 2^((k-79)/4)); the ends are open
+        const int k = r > 0.0f ? (int)floorf(4.0f * log2f(r)) + 80 : 0;
+        atomicAdd(&sh[min(max(k, 0), DCD_HIST - 1)], 1u);
+    }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) rbits = max(rbits, (uint32_t)__shfl_xor((int)rbits, o));
+    if ((threadIdx.x & 63) == 0 && rbits) atomicMax(d.rmax_bits, rbits);
+    __syncthreads();
+    if (threadIdx.x < DCD_HIST && sh[threadIdx.x]) atomicAdd(&d.hist[threadIdx.x], sh[threadIdx.x]);
 }
 
+// base cell c0 = 1.01 (2 r90 + thickness), r90 the upper edge of the histogram
+// bin holding the 90th percentile radius (or the caller's fixed cell)
 __global__ void k_dcd_cellsize(DcdDev d) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const float rmax = __uint_as_float(*d.rmax_bits);
-    // 1% margin keeps float rounding of c/cell from separating a valid pair by two cells
-    const float reach = fminf(2.0f * rmax + d.thickness, DCD_CELL_CAP * d.mean_edge2);
-    const float cell = 1.01f * fmaxf(fmaxf(d.mean_edge2, reach), 1.0e-4f);
-    d.scal[0] = cell;
-    d.scal[1] = 1.0f / cell;
-    d.scal[2] = rmax;
+    float c0 = d.cell_fixed;
+    if (!(c0 > 0.0f)) {
+        uint32_t tot = 0;
+        for (int k = 0; k < DCD_HIST; ++k) tot += d.hist[k];
+        const uint32_t want = (uint32_t)(((uint64_t)tot * 9u + 9u) / 10u);
+        uint32_t acc = 0;
+        int k = 0;
+        for (; k < DCD_HIST - 1; ++k) {
+            acc += d.hist[k];
+            if (acc >= want) break;
+        }
+        const float r90 = exp2f((float)(k - 79) * 0.25f);
+        c0 = 1.01f * fmaxf(2.0f * r90 + d.thickness, 1.0e-4f);
+    }
+    d.scal[0] = c0;
+    d.scal[1] = 1.0f / c0;
+    d.scal[2] = __uint_as_float(*d.rmax_bits);
 }
 
-__global__ void k_dcd_keys(DcdDev d) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= d.ntris) return;
-    const float inv = d.scal[1];
+// level b cell and its inverse: exact power-of-two scalings of the base
+__device__ __forceinline__ float dcd_cell(const DcdDev &d, int b) { return ldexpf(d.scal[0], b); }
+__device__ __forceinline__ float dcd_inv(const DcdDev &d, int b) { return ldexpf(d.scal[1], -b); }
+
+// level, cell and hash key of one triangle; counts per level into the workgroup
+__device__ __forceinline__ void dcd_key_one(DcdDev &d, int i, uint32_t *sl) {
     const float4 c = d.cr[i];
-    // a triangle whose sphere test could reach past the adjacent cells is left out
-    if (!(2.0f * c.w + d.thickness <= d.scal[0] * (1.0f / 1.01f))) {
+    const float need = 2.0f * c.w + d.thickness;
+    int lv = 0;
+    while (lv < DCD_LEVELS && !(need <= dcd_cell(d, lv) * (1.0f / 1.01f))) ++lv;
+    if (lv == DCD_LEVELS) {   // non-finite or above the top level
+        d.tlvl[i] = -1;
+        d.cell[i] = make_int4(0, 0, 0, -1);
+        atomicAdd(&d.overflow[0], 1u);
+        return;
+    }
+    const float inv = dcd_inv(d, lv);
+    if (!(fabsf(c.x) * inv < 1.0e9f && fabsf(c.y) * inv < 1.0e9f && fabsf(c.z) * inv < 1.0e9f)) {
+        d.tlvl[i] = -1;   // cell coordinates would leave the int range
         d.cell[i] = make_int4(0, 0, 0, -1);
         atomicAdd(&d.overflow[0], 1u);
         return;
     }
     const int x = (int)floorf(c.x * inv), y = (int)floorf(c.y * inv), z = (int)floorf(c.z * inv);
-    const uint32_t h = dcd_hash(x, y, z, d.tbits);
+    const uint32_t h = dcd_hash(x, y, z, lv, d.tbits);
+    d.tlvl[i] = lv;
     d.cell[i] = make_int4(x, y, z, (int)h);
     atomicAdd(&d.bucket_cnt[h], 1u);
+    atomicAdd(&sl[lv], 1u);
 }
+
+__global__ __launch_bounds__(256) void k_dcd_keys(DcdDev d) {
+    __shared__ uint32_t sl[DCD_LEVELS];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (threadIdx.x < DCD_LEVELS) sl[threadIdx.x] = 0u;
+    __syncthreads();
+    if (i < d.ntris) dcd_key_one(d, i, sl);
+    __syncthreads();
+    if (threadIdx.x < DCD_LEVELS && sl[threadIdx.x]) atomicAdd(&d.lvl_cnt[threadIdx.x], sl[threadIdx.x]);
+}
+
 
 __global__ void k_dcd_scatter(DcdDev d) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.ntris) return;
-    if (d.cell[i].w < 0) return;
+    if (d.tlvl[i] < 0) return;
     const uint32_t h = (uint32_t)d.cell[i].w;
-    const uint32_t k = atomicAdd(&d.bucket_fill[h], 1u);
-    d.sorted[d.bucket_off[h] + k] = i;
+    const uint32_t q = d.bucket_off[h] + atomicAdd(&d.bucket_fill[h], 1u);
+    d.sorted[q] = i;
+    d.rec[q] = d.cr[i];
 }
 
 // emit rules of broadphase_find_pairs_kernel (:851-915) for candidate j of i
@@ -1530,54 +1587,120 @@ __device__ __forceinline__ bool dcd_emit(const DcdDev &d, int i, int j, const in
     return !(sqrtf(dot(dd, dd)) > ci.w + cj.w + d.thickness);
 }
 
+// FILL = 0 counts the pairs of every segment, FILL = 1 writes them; both passes
+// visit the same candidates.  DCD_LANES consecutive lanes share one triangle
+// (taken in bucket order, so a wavefront searches a few neighbouring cells)
+// and split each cell's candidates; the visit limit is checked on whole cells,
+// identically in every lane.
+constexpr int DCD_LANES = 16;
+
 template <int FILL>
-__global__ void k_dcd_pairs(DcdDev d) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= d.ntris) return;
-    const int4 ci4 = d.cell[i];
-    uint32_t cnt = 0;
-    if (ci4.w < 0) {  // left out
-        if (!FILL) d.pair_cnt[i] = 0;
-        return;
+__global__ __launch_bounds__(256) void k_dcd_pairs(DcdDev d) {
+#pragma clang fp contract(off)
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = (int)(g & (DCD_LANES - 1));
+    const int64_t q0 = g / DCD_LANES;
+    const bool live = q0 < (int64_t)d.bucket_off[(size_t)1 << d.tbits];
+    uint32_t local = 0;
+    bool stop = false;
+    int i = -1;
+    if (live) {
+        i = d.sorted[q0];
+        const int la = d.tlvl[i];
+        const int4 ti = d.tris[i];
+        const float4 ci = d.cr[i];
+        const float4 ni = d.nrm[i];
+        const int gi = d.group ? d.group[i] : 0;
+        uint32_t visits = 0;
+        for (int b = la; b < DCD_LEVELS && !stop; ++b) {
+            if (d.lvl_cnt[b] == 0) continue;
+            const float cb = dcd_cell(d, b), ib = dcd_inv(d, b);
+            // reach of a partner at level b: r_j <= (cell_b / 1.01 - thickness) / 2
+            const float R = 1.01f * (ci.w + 0.5f * (cb * (1.0f / 1.01f) - d.thickness) + d.thickness);
+            const int x0 = (int)floorf((ci.x - R) * ib), x1 = (int)floorf((ci.x + R) * ib);
+            const int y0 = (int)floorf((ci.y - R) * ib), y1 = (int)floorf((ci.y + R) * ib);
+            const int z0 = (int)floorf((ci.z - R) * ib), z1 = (int)floorf((ci.z + R) * ib);
+            for (int z = z0; z <= z1 && !stop; ++z)
+                for (int y = y0; y <= y1 && !stop; ++y)
+                    for (int x = x0; x <= x1 && !stop; ++x) {
+                        const uint32_t h = dcd_hash(x, y, z, b, d.tbits);
+                        const uint32_t s0 = d.bucket_off[h], e = d.bucket_off[h + 1];
+                        visits += e - s0;
+                        if (visits > DCD_MAX_VISIT) {
+                            stop = true;
+                            break;
+                        }
+                        for (uint32_t q = s0 + lane; q < e; q += DCD_LANES) {
+                            const float4 cj = d.rec[q];
+                            const V3 dd = V3{ci.x - cj.x, ci.y - cj.y, ci.z - cj.z};
+                            if (sqrtf(dot(dd, dd)) > ci.w + cj.w + d.thickness) continue;
+                            const int j = d.sorted[q];
+                            if (j == i || (b == la && j < i)) continue;
+                            const int4 cc = d.cell[j];   // visit j only from its own cell
+                            if (cc.x != x || cc.y != y || cc.z != z || d.tlvl[j] != b) continue;
+                            if (!dcd_emit(d, i, j, ti, ci, ni, gi)) continue;
+                            const int sg = min(i, j), ot = max(i, j);
+                            if (!FILL) {
+                                if (sg == i) ++local;
+                                else atomicAdd(&d.pair_cnt[sg], 1u);
+                            } else {
+                                const uint64_t o = (uint64_t)d.pair_off[sg] + atomicAdd(&d.pair_fill[sg], 1u);
+                                if ((int64_t)o < d.max_pairs) d.pairs[o] = make_int2(sg, ot);
+                            }
+                        }
+                    }
+        }
     }
-    const int4 ti = d.tris[i];
-    const float4 ci = d.cr[i];
-    const float4 ni = d.nrm[i];
-    const int gi = d.group ? d.group[i] : 0;
-    uint64_t base = 0;
-    if (FILL) base = d.pair_off[i];
-    bool full = false;
-    for (int dz = -1; dz <= 1 && !full; ++dz)
-        for (int dy = -1; dy <= 1 && !full; ++dy)
-            for (int dx = -1; dx <= 1 && !full; ++dx) {
-                const int x = ci4.x + dx, y = ci4.y + dy, z = ci4.z + dz;
-                const uint32_t h = dcd_hash(x, y, z, d.tbits);
-                const uint32_t e = d.bucket_off[h + 1];
-                for (uint32_t q = d.bucket_off[h]; q < e; ++q) {
-                    const int j = d.sorted[q];
-                    if (j <= i) continue;
-                    const int4 cj = d.cell[j];
-                    if (cj.x != x || cj.y != y || cj.z != z) continue;  // visit j only from its own cell
-                    if (!dcd_emit(d, i, j, ti, ci, ni, gi)) continue;
-                    if (cnt == DCD_MAX_PER_TRI) {  // same stop in both passes (same bucket order)
-                        full = true;
-                        break;
-                    }
-                    if (FILL) {
-                        const uint64_t o = base + cnt;
-                        if ((int64_t)o < d.max_pairs) d.pairs[o] = make_int2(i, j);
-                    }
-                    ++cnt;
-                }
-            }
     if (!FILL) {
-        d.pair_cnt[i] = cnt;
-        if (full) atomicAdd(&d.overflow[1], 1u);
+        // the lanes of one triangle are DCD_LANES consecutive lanes of a wavefront
+#pragma unroll
+        for (int o = 1; o < DCD_LANES; o <<= 1) local += (uint32_t)__shfl_xor((int)local, o);
+        if (live && lane == 0) {
+            if (local) atomicAdd(&d.pair_cnt[i], local);
+            if (stop) atomicAdd(&d.overflow[1], 1u);
+        }
+    }
+}
+
+// sort each triangle's segment by partner index.  DCD_LANES lanes share a
+// segment; up to 4 DCD_LANES entries each lane ranks its entries against the
+// whole segment (partners are distinct) and stores them at their ranks once
+// every lane of the group has read (the group lies in one wavefront, whose
+// loads complete before the dependent stores); longer segments are
+// insertion-sorted by the group's first lane.
+__global__ __launch_bounds__(256) void k_dcd_sortseg(DcdDev d) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = (int)(g & (DCD_LANES - 1));
+    const int64_t i = g / DCD_LANES;
+    int64_t lo = 0, hi = 0;
+    if (i < d.ntris) {
+        lo = min((int64_t)d.pair_off[i], d.max_pairs);
+        hi = min((int64_t)d.pair_off[i + 1], d.max_pairs);
+    }
+    const int64_t n = hi - lo;
+    if (n <= 4 * DCD_LANES) {
+        int y[4], rk[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int64_t e = lane + (int64_t)k * DCD_LANES;
+            y[k] = e < n ? d.pairs[lo + e].y : 0;
+            rk[k] = 0;
+        }
+        for (int64_t m = 0; m < n; ++m) {
+            const int ym = d.pairs[lo + m].y;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) rk[k] += ym < y[k] ? 1 : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (lane + (int64_t)k * DCD_LANES < n) d.pairs[lo + rk[k]] = make_int2((int)i, y[k]);
         return;
     }
-    // sort this triangle's segment by j (insertion sort; segments are short)
-    const int64_t lo = (int64_t)base;
-    const int64_t hi = min((int64_t)base + (int64_t)cnt, d.max_pairs);
+    if (lane != 0) return;
+    if (n > DCD_SORT_MAX) {
+        atomicAdd(&d.overflow[2], 1u);
+        return;
+    }
     for (int64_t a = lo + 1; a < hi; ++a) {
         const int2 v = d.pairs[a];
         int64_t b = a - 1;
@@ -2477,9 +2600,13 @@ int launch_dcd(xpbd_ctx *c, float4 *X, bool rebuild) {
     auto g1 = [](int64_t n) { return dim3((unsigned)((n + 255) / 256)); };
     if (rebuild) {
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, d.rmax_bits, (int64_t)1);
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, d.overflow, (int64_t)2);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, d.overflow, (int64_t)3);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, d.hist, (int64_t)DCD_HIST);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, d.lvl_cnt, (int64_t)DCD_LEVELS);
         hipLaunchKernelGGL(k_zero_u32, g1(T), dim3(256), 0, c->stream, d.bucket_cnt, T);
         hipLaunchKernelGGL(k_zero_u32, g1(T), dim3(256), 0, c->stream, d.bucket_fill, T);
+        hipLaunchKernelGGL(k_zero_u32, g1(m + 1), dim3(256), 0, c->stream, d.pair_cnt, (int64_t)m + 1);
+        hipLaunchKernelGGL(k_zero_u32, g1(m), dim3(256), 0, c->stream, d.pair_fill, (int64_t)m);
         hipLaunchKernelGGL(k_dcd_snapshot, g1(c->nv), dim3(256), 0, c->stream, d.xprev, (const float4 *)X, c->nv);
         hipLaunchKernelGGL(k_dcd_tridata, g1(m), dim3(256), 0, c->stream, d, (const float4 *)X);
         hipLaunchKernelGGL(k_dcd_cellsize, dim3(1), dim3(64), 0, c->stream, d);
@@ -2487,10 +2614,11 @@ int launch_dcd(xpbd_ctx *c, float4 *X, bool rebuild) {
         HIPCHK(c, hipGetLastError());
         if ((r = launch_scan(c, d.bucket_cnt, d.bucket_off, T, d.bucket_off + T))) return r;
         hipLaunchKernelGGL(k_dcd_scatter, g1(m), dim3(256), 0, c->stream, d);
-        hipLaunchKernelGGL(k_dcd_pairs<0>, g1(m), dim3(256), 0, c->stream, d);
+        hipLaunchKernelGGL(k_dcd_pairs<0>, g1((int64_t)DCD_LANES * m), dim3(256), 0, c->stream, d);
         HIPCHK(c, hipGetLastError());
         if ((r = launch_scan(c, d.pair_cnt, d.pair_off, m, d.pair_off + m))) return r;
-        hipLaunchKernelGGL(k_dcd_pairs<1>, g1(m), dim3(256), 0, c->stream, d);
+        hipLaunchKernelGGL(k_dcd_pairs<1>, g1((int64_t)DCD_LANES * m), dim3(256), 0, c->stream, d);
+        hipLaunchKernelGGL(k_dcd_sortseg, g1((int64_t)DCD_LANES * m), dim3(256), 0, c->stream, d);
         HIPCHK(c, hipGetLastError());
     }
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, d.ncontact, (int64_t)1);
@@ -2519,19 +2647,7 @@ int alloc_dcd(xpbd_ctx *c) {
     d.offset_eps = c->dcd_p.offset;
     d.normal_cull = c->dcd_p.normal_cull;
     d.allow_self = c->dcd_p.allow_self;
-    // 2 x mean rest edge length of the surface triangles
-    double es = 0.0;
-    for (int i = 0; i < m; ++i)
-        for (int k = 0; k < 3; ++k) {
-            const int a = c->dcd_tris[3 * i + k], b = c->dcd_tris[3 * i + (k + 1) % 3];
-            double q = 0;
-            for (int e = 0; e < 3; ++e) {
-                const double t = (double)c->pos0[3 * a + e] - c->pos0[3 * b + e];
-                q += t * t;
-            }
-            es += std::sqrt(q);
-        }
-    d.mean_edge2 = c->dcd_p.cell_size > 0.0f ? c->dcd_p.cell_size : (float)(m > 0 ? 2.0 * es / (3.0 * m) : 0.0);
+    d.cell_fixed = c->dcd_p.cell_size > 0.0f ? c->dcd_p.cell_size : 0.0f;
     std::vector<int4> tt(m);
     for (int i = 0; i < m; ++i)
         tt[i] = make_int4(c->L.iperm[c->dcd_tris[3 * i]], c->L.iperm[c->dcd_tris[3 * i + 1]],
@@ -2552,7 +2668,10 @@ int alloc_dcd(xpbd_ctx *c) {
         (r = dalloc(c, &d.pairs, (size_t)d.max_pairs)) || (r = dalloc(c, &d.contacts, (size_t)d.max_pairs)) ||
         (r = dalloc(c, &d.acc, 3 * (size_t)c->nv)) || (r = dalloc(c, &d.acc_n, (size_t)c->nv)) ||
         (r = dalloc(c, &d.xprev, (size_t)c->nv)) || (r = dalloc(c, &d.scal, 4)) ||
-        (r = dalloc(c, &d.rmax_bits, 1)) || (r = dalloc(c, &d.ncontact, 1)) || (r = dalloc(c, &d.overflow, 2)))
+        (r = dalloc(c, &d.rmax_bits, 1)) || (r = dalloc(c, &d.ncontact, 1)) || (r = dalloc(c, &d.overflow, 3)) ||
+        (r = dalloc(c, &d.tlvl, (size_t)std::max(m, 1))) || (r = dalloc(c, &d.rec, (size_t)std::max(m, 1))) ||
+        (r = dalloc(c, &d.hist, (size_t)DCD_HIST)) || (r = dalloc(c, &d.lvl_cnt, (size_t)DCD_LEVELS)) ||
+        (r = dalloc(c, &d.pair_fill, (size_t)m + 1)))
         return r;
     HIPCHK(c, hipMemset(d.overflow, 0, 2 * sizeof(uint32_t)));
     HIPCHK(c, hipMemset(d.acc, 0, 3 * (size_t)c->nv * sizeof(unsigned long long)));
@@ -3437,11 +3556,11 @@ int32_t xpbd_dcd_overflow(xpbd_ctx *c, int64_t *left_out, int64_t *truncated) {
     int r = check_ready(c);
     if (r) return r;
     if (!c->dcd_on) return fail(c, XPBD_EINVAL, "no DCD surface set");
-    uint32_t o[2] = {0, 0};
+    uint32_t o[3] = {0, 0, 0};
     HIPCHK(c, hipMemcpyAsync(o, c->dcd.overflow, sizeof(o), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (left_out) *left_out = o[0];
-    if (truncated) *truncated = o[1];
+    if (truncated) *truncated = (int64_t)o[1] + o[2];   // stopped searches + segments left unsorted
     return XPBD_OK;
 }
 

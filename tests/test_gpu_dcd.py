@@ -41,6 +41,27 @@ def test_pairs_bit_exact_vs_bruteforce(cell):
     np.testing.assert_array_equal(got, ref)
 
 
+def test_pairs_bit_exact_with_stretched_triangles():
+    """a few vertices pulled far out stretch their triangles over several grid
+    levels: the pair list still equals brute force and nothing is left out"""
+    sc, tris = _scene()
+    eng = XpbdEngine(sc)
+    rng = np.random.default_rng(1)
+    x = sc.pos0 + rng.normal(scale=1e-3, size=sc.pos0.shape).astype(np.float32)
+    sv = np.unique(tris)
+    pick = rng.choice(sv, size=max(8, sv.size // 100), replace=False)
+    x[pick] += rng.normal(scale=0.03, size=(pick.size, 3)).astype(np.float32)
+    eng.set_state(x, x, np.zeros_like(x))
+    eng.dcd_solve(True)
+    got = eng.dcd_pairs()
+    ref = oracle.dcd_pairs(tris, x, allow_self=True, thickness=1e-3)
+    st = eng.dcd_stats()
+    print("pairs", len(got), len(ref), st)
+    assert st["left_out_tris"] == 0 and st["truncated_tris"] == 0
+    assert len(ref) > 1000
+    np.testing.assert_array_equal(got, ref)
+
+
 def test_dcd_pass_matches_oracle():
     sc, tris = _scene()
     eng = XpbdEngine(sc)
