@@ -22,6 +22,13 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per projection iteration measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per
+# dispatch, scaled x2.0 and x1.0 by a plain-copy calibration), keyed by (tets, constraint types,
+# strategy, block tets, block threads): profiles/projection_block_humanv2_535k.md
+TRAFFIC = {
+    (534882, ("tetarapsvd", "volume"), 2, 512, 512): 71858074.0,
+}
+
 WORKLOADS = {
     "humanv2": ("humanv2_scale.json", {}),
     "humanv2_20": ("humanv2_scale.json", {"fine_iterations": 20}),
@@ -44,6 +51,7 @@ def parse():
     ap.add_argument("--block-threads", type=int, default=256, help="block strategy: threads per workgroup")
     ap.add_argument("--ws-train-frames", type=int, default=0, help="warm-start leg: training frames (0 = config)")
     ap.add_argument("--ws-eval-frames", type=int, default=0, help="warm-start leg: evaluation frames (0 = config)")
+    ap.add_argument("--no-warm-start", action="store_true", help="skip the warm-start leg of configs that have one")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
@@ -218,14 +226,17 @@ def main():
     sim.engine.synchronize()
     t1 = time.perf_counter()
     ms_frame = (t1 - t0) * 1e3 / args.steps
-    # the projection kernels alone: one substep's iterations captured in a HIP
-    # graph and replayed between two HIP events on the library stream (the
-    # stream the kernels run on), from the state the timed frames left
+    # the projection kernels alone: t_iters * t_reps iterations captured in one
+    # HIP graph and launched once between two HIP events on the library stream
+    # (the stream the kernels run on), from the state the timed frames left
     t_iters = iters if iters >= 2 and iters % 2 == 0 else 10
     t_reps = 20
     avg_us = sim.engine.time_iterations(t_iters, t_reps)
     k_n = t_iters * t_reps
     alg = st["bytes_per_iter_alg"]
+    ctypes_ = tuple(c["type"] for c in sim.scene.meta["constraint_configs"])
+    traffic = TRAFFIC.get((w.nt, ctypes_, strat, tuning.get("block_tets", 0) if strat == 2 else 0,
+                           tuning.get("block_threads", 0) if strat == 2 else 0))
     achieved = alg / (avg_us * 1e-6) / 1e9
     nt_elems = st["n_elems"]
     tet_it = w.nt * iters * nsub / (ms_frame * 1e-3)
@@ -258,10 +269,10 @@ def main():
         "env_substeps_per_s": round(n_inst * nsub / (ms_frame * 1e-3), 1),
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": ("k_iterate (fused Jacobi projection)", "k_flat_tets + k_flat_verts (one Jacobi iteration)",
                        "k_block_tets + k_block_shared (one Jacobi iteration)")[strat],
-            "avg_iteration_us": round(avg_us, 3), "timing": "graph replay of the iteration kernels, HIP events",
+            "avg_iteration_us": round(avg_us, 3), "timing": "one HIP graph of the iteration kernels between two HIP events",
             "iterations_timed": k_n, "alg_bytes_per_iteration": alg,
             "impl_bytes_per_iteration": st["bytes_per_iter_impl"],
         },
@@ -270,7 +281,7 @@ def main():
     }
     if sim.scene.dcd is not None:
         out["dcd"] = dict(sim.engine.dcd_stats(), surface_tris=int(sim.scene.dcd.tris.shape[0]))
-    if "warm_start" in cfg:
+    if "warm_start" in cfg and not args.no_warm_start:
         if args.ws_train_frames:
             cfg["warm_start"]["train_frames"] = args.ws_train_frames
         if args.ws_eval_frames:

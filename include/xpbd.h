@@ -284,9 +284,9 @@ int32_t xpbd_get_sdf_cells(xpbd_ctx *ctx, int32_t *cells);
 int32_t xpbd_kernel_timing(xpbd_ctx *ctx, int32_t enable);
 int32_t xpbd_get_kernel_timing(xpbd_ctx *ctx, double *total_ms, int32_t *launches);
 /* average duration of one projection iteration (the kernels of xpbd_iterate,
- * lambda mode 0): n iterations captured in a HIP graph, replayed reps times
- * between two events on the context stream; X is restored afterwards.  Measures
- * the kernels without per-launch host dispatch gaps.  No reference counterpart
+ * lambda mode 0): n * reps (<= 4096, n even) iterations captured in one HIP graph,
+ * launched once to warm up and once between two events on the context stream;
+ * X is restored afterwards.  Measures the kernels without host dispatch gaps.  No reference counterpart
  * (the reference brackets sections with wp.synchronize() and perf_counter,
  * tools/profile_humanV2.py:56-131). */
 int32_t xpbd_time_iterations(xpbd_ctx *ctx, int32_t n, int32_t reps, double *us_per_iter);

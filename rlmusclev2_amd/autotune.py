@@ -8,7 +8,7 @@ iteration, no replication, every corner correction goes through memory); and
 the "block" strategy (consecutive tets per workgroup, corner sums in LDS,
 only vertices shared between blocks go through memory) at several block
 sizes and thread counts.  Each candidate layout is built for the actual mesh,
-a few iterations are captured in a HIP graph and replayed between two HIP
+100 iterations are captured in one HIP graph and launched between two HIP
 events (as they run inside a frame), and the fastest is kept.  Results are cached per mesh signature in
 ``rlmusclev2_amd/_tune/cache.json``.
 """
@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import hashlib
 import json
+import sys
 from pathlib import Path
 
 import numpy as np
@@ -26,7 +27,8 @@ CACHE = Path(__file__).resolve().parent / "_tune" / "cache.json"
 # (strategy, patch_verts, block_tets, block_threads)
 CANDIDATES = ((1, 256, 0, 0), (0, 64, 0, 0), (0, 128, 0, 0), (0, 256, 0, 0),
               (2, 256, 128, 128), (2, 256, 192, 192), (2, 256, 256, 256), (2, 256, 384, 384),
-              (2, 256, 512, 256), (2, 256, 512, 512), (2, 256, 768, 384), (2, 256, 1024, 512))
+              (2, 256, 512, 256), (2, 256, 512, 512), (2, 256, 768, 384), (2, 256, 1024, 512),
+              (2, 256, 2048, 512))
 
 
 def name(c) -> str:
@@ -62,8 +64,9 @@ def time_candidates(cfg: dict, w, candidates=CANDIDATES, iters: int = 20) -> dic
         except XpbdError:      # e.g. the layout needs more LDS than a workgroup can have
             continue
         sim.engine.iterate(2)
-        out[c] = sim.engine.time_iterations(iters, 2)
+        out[c] = sim.engine.time_iterations(iters, 5)
         sim.engine.close()
+        print(f"autotune {name(c)}: {out[c]:.2f} us/iteration", file=sys.stderr, flush=True)
     return out
 
 

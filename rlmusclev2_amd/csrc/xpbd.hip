@@ -3911,8 +3911,8 @@ int32_t xpbd_kernel_timing(xpbd_ctx *c, int32_t enable) {
 int32_t xpbd_time_iterations(xpbd_ctx *c, int32_t n, int32_t reps, double *us_per_iter) {
     int r = check_ready(c);
     if (r) return r;
-    if (n < 2 || (n & 1) || reps < 1 || !us_per_iter)
-        return fail(c, XPBD_EINVAL, "time_iterations needs an even n >= 2, reps >= 1 and an output");
+    if (n < 2 || (n & 1) || reps < 1 || (int64_t)n * reps > 4096 || !us_per_iter)
+        return fail(c, XPBD_EINVAL, "time_iterations needs an even n >= 2, reps >= 1, n * reps <= 4096 and an output");
     if (c->lam_mode != 0) return fail(c, XPBD_EUNSUPPORTED, "iteration timing runs the plain (lambda mode 0) kernels");
     const size_t b = (size_t)c->nv * sizeof(float4);
     if (!c->T_X && (r = dalloc(c, &c->T_X, (size_t)c->nv))) return r;
@@ -3920,8 +3920,12 @@ int32_t xpbd_time_iterations(xpbd_ctx *c, int32_t n, int32_t reps, double *us_pe
     const bool tm = c->timing;
     c->timing = false;
     c->eff_mode = 0;
+    // all n * reps iterations go into one graph, so the timed region holds a
+    // single graph launch and no host gaps between replays
+    const int total = n * reps;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    for (int it = 0; it < n && r == XPBD_OK; ++it) r = launch_iter(c, c->X[it & 1], c->X[(it + 1) & 1], MODE_FUSED, 0);
+    for (int it = 0; it < total && r == XPBD_OK; ++it)
+        r = launch_iter(c, c->X[it & 1], c->X[(it + 1) & 1], MODE_FUSED, 0);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     c->timing = tm;
@@ -3935,7 +3939,7 @@ int32_t xpbd_time_iterations(xpbd_ctx *c, int32_t n, int32_t reps, double *us_pe
     if (he == hipSuccess) he = hipEventCreate(&e1);
     if (he == hipSuccess) he = hipGraphLaunch(ge, c->stream);   // warm
     if (he == hipSuccess) he = hipEventRecord(e0, c->stream);
-    for (int k = 0; k < reps && he == hipSuccess; ++k) he = hipGraphLaunch(ge, c->stream);
+    if (he == hipSuccess) he = hipGraphLaunch(ge, c->stream);
     if (he == hipSuccess) he = hipEventRecord(e1, c->stream);
     if (he == hipSuccess) he = hipEventSynchronize(e1);
     if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
@@ -3946,7 +3950,7 @@ int32_t xpbd_time_iterations(xpbd_ctx *c, int32_t n, int32_t reps, double *us_pe
     if (he != hipSuccess) return fail(c, XPBD_EHIP, std::string("iteration timing: ") + hipGetErrorString(he));
     HIPCHK(c, hipMemcpyAsync(c->X[0], c->T_X, b, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    *us_per_iter = 1e3 * (double)ms / ((double)n * reps);
+    *us_per_iter = 1e3 * (double)ms / (double)total;
     return XPBD_OK;
 }
 
