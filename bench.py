@@ -139,15 +139,20 @@ def warm_start_leg(cfg, w, tuning, out):
     nsub = ev.nsubsteps
     vnet = np.array([11 if x is None else x for x in cmp[ws.MODE_NET][nsub:]] or [10])
     K = int(min(10, max(1, np.ceil(np.percentile(vnet, 80)))))
-    # timing: plain frames vs warm frames (graph replay), same trajectory start
+    R = int(wc.get("ref_every", 3))
+    # timing: plain frames vs warm frames (graph replay, audited fallback), same trajectory start
     nf = int(wc.get("timed_frames", 20))
     ws.randomize_motion(w, np.random.default_rng(13))
+    K2 = int(wc.get("k_iters", 4))
+    runs = [("plain", None, None, None), ("network", K, R, True), ("previous_lambda", K, R, False),
+            (f"network_k{K2}", K2, R, True), (f"previous_lambda_k{K2}", K2, R, False)]
     tim = {}
-    log(f"timing plain and warm frames (K={K})")
-    for label in ("plain", "warm"):
+    for label, k, r_every, use_net in runs:
+        log(f"timing {label} frames (K={k}, reference every {r_every})")
         s2 = sims.MultiGridSim(cfg, workload=w, **tuning)
         ws.install(s2.engine, net, norm)
-        st = ws.WarmStartStepper(s2, K, audit_every=int(wc.get("audit_every", 8))) if label == "warm" else None
+        st = (ws.WarmStartStepper(s2, k, audit_every=int(wc.get("audit_every", 8)), tol=float(wc.get("tol", 0.0)),
+                                  guard=None, ref_every=r_every, use_network=use_net) if k else None)
         for f in range(1, 3):
             st.step_frame(f) if st else s2.step_frame(f)
         s2.engine.synchronize()
@@ -157,22 +162,25 @@ def warm_start_leg(cfg, w, tuning, out):
         for f in range(3, 3 + nf):
             st.step_frame(f) if st else s2.step_frame(f)
         s2.engine.synchronize()
-        tim[label] = (time.perf_counter() - ta) * 1e3 / nf
+        ms = (time.perf_counter() - ta) * 1e3 / nf
+        rec = {"ms_per_frame": round(ms, 4), "residual_end": s2.residuals()}
         if st:
-            tim["fallbacks"] = st.fallbacks
-            tim["audits"] = st.audits
-            tim["residual"] = s2.residuals()
-            tim["plain_residual_last_audit"] = st.ref
+            refs = (nsub + r_every - 1) // r_every if r_every else 1
+            rec.update(k_iters=k, ref_every=r_every, fallbacks=st.fallbacks, audits=st.audits,
+                       iterations_per_frame=refs * st.plain_iters + (nsub - refs) * k)
         else:
-            tim["plain_residual"] = s2.residuals()
+            rec["iterations_per_frame"] = nsub * s2.fine_iters
+        tim[label] = rec
         s2.engine.close()
+    plain_ms = tim["plain"]["ms_per_frame"]
+    for label in tim:
+        if label != "plain":
+            tim[label]["speedup"] = round(plain_ms / tim[label]["ms_per_frame"], 3)
     out["warm_start"] = {
         "train_frames": int(wc.get("train_frames", 2000)), "samples": n_samples,
         "collect_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2), "last_epoch": hist[-1],
-        "iterations_to_plain10_residual": iters, "k_iters": K,
-        "ms_per_frame_plain": round(tim["plain"], 4), "ms_per_frame_warm_with_fallback": round(tim["warm"], 4),
-        "speedup": round(tim["plain"] / tim["warm"], 3), "timed_frames": nf, "fallbacks": tim["fallbacks"],
-        "audits": tim["audits"], "residual_warm_end": tim["residual"], "residual_plain_end": tim["plain_residual"],
+        "iterations_to_plain10_residual": iters, "timed_frames": nf,
+        "fallback_tolerance": float(wc.get("tol", 0.0)), "frames": tim,
     }
 
 

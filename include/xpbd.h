@@ -248,11 +248,11 @@ int32_t xpbd_mlp_backward(const xpbd_mlp_dev *net, const float *X, const float *
  * 2 = learned warm start: the network predicts, per tet and constraint type, a gain on the base,
  *     lambda0 = base * clamp(1 + out_scale * y, 0, 2), which also becomes the next base; the
  *     first iteration applies lambda0 as an impulse along the constraint gradients, the rest
- *     carry lambda;
+ *     are plain zero-lambda iterations;
  * 3 = the same with lambda0 = base (no network);
- * 4 = frames with a schedule: the first substep of each xpbd_step_frame is mode 1 with the
- *     reference iteration count (xpbd_set_reference_iters, default 10), the others are mode 2
- *     with the frame's iteration count (xpbd_substep and the phase calls use mode 2). */
+ * 4 = frames with a schedule (xpbd_set_warm_schedule): reference substeps run mode 1 with the
+ *     reference iteration count, the others mode 2 (network) or mode 3 (base only) with the
+ *     frame's iteration count; xpbd_substep and the phase calls use mode 2 or 3. */
 int32_t xpbd_set_lambda_mode(xpbd_ctx *ctx, int32_t mode);
 /* network weights (host fp32; W1, W2 stored as bf16), feature standardisation (IN each) and
  * per-output scales (OUT) */
@@ -267,6 +267,11 @@ int32_t xpbd_features(xpbd_ctx *ctx, float *dev_out, int32_t normalized);
 int32_t xpbd_get_lambda(xpbd_ctx *ctx, float *dev_out);
 int32_t xpbd_set_lambda(xpbd_ctx *ctx, const float *dev_in);   /* sets the carried multipliers and the base */
 int32_t xpbd_set_reference_iters(xpbd_ctx *ctx, int32_t n);
+/* mode 4 schedule: reference substeps (lambda carried from zero, ref_iters iterations) are
+ * those of index % ref_every == 0 within xpbd_step_frame (ref_every 0: the first substep of
+ * each frame only, the default); between them the base of the last reference is applied as
+ * an impulse, scaled by the network when use_network is 1.  No reference counterpart. */
+int32_t xpbd_set_warm_schedule(xpbd_ctx *ctx, int32_t ref_iters, int32_t ref_every, int32_t use_network);
 /* save (1) / restore (0) positions, previous positions and velocities on the device */
 int32_t xpbd_snapshot(xpbd_ctx *ctx, int32_t save);
 

@@ -127,6 +127,7 @@ SIGNATURES = {
     "xpbd_get_lambda": [_P, _P],
     "xpbd_set_lambda": [_P, _P],
     "xpbd_set_reference_iters": [_P, _I],
+    "xpbd_set_warm_schedule": [_P, _I, _I, _I],
     "xpbd_snapshot": [_P, _I],
     "xpbd_kernel_timing": [_P, _I],
     "xpbd_get_kernel_timing": [_P, _P, _P],

@@ -2109,7 +2109,10 @@ struct xpbd_ctx {
     int lam_mode = 0;             // 0 plain, 1 carry, 2 network warm start, 3 previous-substep warm start
     bool impulse_pending = false;
     int eff_mode = 0;             // mode of the current substep (mode 4 schedules 1 and 2)
-    int ref_iters = 10;           // mode 4: iterations of the reference substep
+    int ref_iters = 10;           // mode 4: iterations of a reference substep
+    int ref_every = 0;            // mode 4: reference substeps are those with index % ref_every == 0
+                                  // (0: the first substep of a frame only)
+    int warm_net = 1;             // mode 4: between references, network (mode 2) or base only (mode 3)
     float4 *LAMB = nullptr;       // carried multipliers of the current substep
     float4 *LAMB0 = nullptr;      // base of the next warm start (lambda0 of the last substep, or a reference solve)
     uint16_t *mW1 = nullptr, *mW2 = nullptr;
@@ -2132,7 +2135,7 @@ struct xpbd_ctx {
     struct GraphSlot {
         hipGraph_t graph = nullptr;
         hipGraphExec_t gexec = nullptr;
-        int nsub = -1, iters = -1, lam_mode = -1, ref_iters = -1;
+        int nsub = -1, iters = -1, lam_mode = -1, ref_iters = -1, ref_every = -1, warm_net = -1;
         uint64_t used = 0;
     };
     static constexpr int NGRAPH = 4;
@@ -2181,7 +2184,7 @@ int dupload(xpbd_ctx *c, T **p, const T *src, size_t count) {
 void drop_graph_slot(xpbd_ctx::GraphSlot &g) {
     if (g.gexec) { (void)hipGraphExecDestroy(g.gexec); g.gexec = nullptr; }
     if (g.graph) { (void)hipGraphDestroy(g.graph); g.graph = nullptr; }
-    g.nsub = g.iters = g.lam_mode = g.ref_iters = -1;
+    g.nsub = g.iters = g.lam_mode = g.ref_iters = g.ref_every = g.warm_net = -1;
     g.used = 0;
 }
 
@@ -2722,13 +2725,18 @@ int finish_lambda(xpbd_ctx *c) {
     return XPBD_OK;
 }
 
+// lambda template of the next iteration: mode 1 carries lambda from zero; modes
+// 2 and 3 apply lambda0 as an impulse in their first iteration and then run
+// plain zero-lambda iterations, which keep lowering the plain solver's own
+// residual (a carried lambda converges towards the compliant solution, whose
+// residual is alpha * lambda, and overshoots the plain residual over a frame)
 int lam_for_iteration(xpbd_ctx *c) {
     if (c->eff_mode == 0) return 0;
     if (c->impulse_pending) {
         c->impulse_pending = false;
         return 2;
     }
-    return 1;
+    return c->eff_mode == 1 ? 1 : 0;
 }
 
 // restrict -> coarse Jacobi ARAP sweeps (coarse pins held as stopped vertices)
@@ -2757,11 +2765,17 @@ int enqueue_substep(xpbd_ctx *c, int iters, bool dcd_rebuild, int sub = 0) {
     int r = launch_integrate(c);
     if (r) return r;
     if (c->coarse && c->mg_iters > 0 && (r = launch_vcycle(c))) return r;
-    // mode 4: the first substep of a frame is a reference solve (carried from
-    // zero, ref_iters iterations) that refreshes the base; the others are
-    // network warm starts with iters iterations
-    c->eff_mode = c->lam_mode == 4 ? (sub == 0 ? 1 : 2) : c->lam_mode;
-    if (c->lam_mode == 4 && sub == 0) iters = c->ref_iters;
+    // mode 4: reference substeps (the first of a frame, or every ref_every-th)
+    // are solves carried from zero with ref_iters iterations that refresh the
+    // base; the others are warm starts (network or base) with iters iterations
+    bool ref = false;
+    if (c->lam_mode == 4) {
+        ref = c->ref_every > 0 ? (sub % c->ref_every == 0) : (sub == 0);
+        c->eff_mode = ref ? 1 : (c->warm_net ? 2 : 3);
+        if (ref) iters = c->ref_iters;
+    } else {
+        c->eff_mode = c->lam_mode;
+    }
     if (c->eff_mode && (r = launch_warm_start(c))) return r;
     for (int it = 0; it < iters; ++it) {
         r = launch_iter(c, c->X[it & 1], c->X[(it + 1) & 1], MODE_FUSED, lam_for_iteration(c));
@@ -3388,7 +3402,7 @@ int32_t xpbd_iterate(xpbd_ctx *c, int32_t n) {
     int r = check_ready(c);
     if (r) return r;
     if (n < 0) return fail(c, XPBD_EINVAL, "negative iteration count");
-    c->eff_mode = c->lam_mode == 4 ? 2 : c->lam_mode;
+    c->eff_mode = c->lam_mode == 4 ? (c->warm_net ? 2 : 3) : c->lam_mode;
     for (int it = 0; it < n; ++it)
         if ((r = launch_iter(c, c->X[it & 1], c->X[(it + 1) & 1], MODE_FUSED, lam_for_iteration(c)))) return r;
     if (n & 1) {
@@ -3429,7 +3443,8 @@ int32_t xpbd_step_frame(xpbd_ctx *c, int32_t nsub, int32_t iters, int32_t use_gr
     }
     xpbd_ctx::GraphSlot *hit = nullptr, *victim = &c->gslot[0];
     for (auto &g : c->gslot) {
-        if (g.gexec && g.nsub == nsub && g.iters == iters && g.lam_mode == c->lam_mode && g.ref_iters == c->ref_iters)
+        if (g.gexec && g.nsub == nsub && g.iters == iters && g.lam_mode == c->lam_mode && g.ref_iters == c->ref_iters &&
+            g.ref_every == c->ref_every && g.warm_net == c->warm_net)
             hit = &g;
         if (g.used < victim->used) victim = &g;
     }
@@ -3447,6 +3462,8 @@ int32_t xpbd_step_frame(xpbd_ctx *c, int32_t nsub, int32_t iters, int32_t use_gr
         victim->iters = iters;
         victim->lam_mode = c->lam_mode;
         victim->ref_iters = c->ref_iters;
+        victim->ref_every = c->ref_every;
+        victim->warm_net = c->warm_net;
         hit = victim;
     }
     hit->used = ++c->gclock;
@@ -3828,7 +3845,8 @@ int32_t xpbd_set_lambda_mode(xpbd_ctx *c, int32_t mode) {
     if (mode && !c->flat) return fail(c, XPBD_EUNSUPPORTED, "lambda modes run on the flat strategy");
     if (mode && (c->feature & F_ARAP) && (c->feature & F_SNH))
         return fail(c, XPBD_EUNSUPPORTED, "lambda modes need ARAP-SVD and SNH on separate meshes");
-    if ((mode == 2 || mode == 4) && !c->mlp_ready) return fail(c, XPBD_EINVAL, "lambda modes 2 and 4 need xpbd_set_mlp()");
+    if ((mode == 2 || (mode == 4 && c->warm_net)) && !c->mlp_ready)
+        return fail(c, XPBD_EINVAL, "lambda mode 2 and network schedules need xpbd_set_mlp()");
     c->lam_mode = mode;
     c->impulse_pending = false;
     return XPBD_OK;
@@ -3837,7 +3855,7 @@ int32_t xpbd_set_lambda_mode(xpbd_ctx *c, int32_t mode) {
 int32_t xpbd_warm_start(xpbd_ctx *c) {
     int r = check_ready(c);
     if (r) return r;
-    c->eff_mode = c->lam_mode == 4 ? 2 : c->lam_mode;
+    c->eff_mode = c->lam_mode == 4 ? (c->warm_net ? 2 : 3) : c->lam_mode;
     return c->eff_mode ? launch_warm_start(c) : XPBD_OK;
 }
 
@@ -3877,6 +3895,17 @@ int32_t xpbd_set_reference_iters(xpbd_ctx *c, int32_t n) {
     if (r) return r;
     if (n < 1) return fail(c, XPBD_EINVAL, "reference iterations must be >= 1");
     c->ref_iters = n;
+    return XPBD_OK;
+}
+
+int32_t xpbd_set_warm_schedule(xpbd_ctx *c, int32_t ref_iters, int32_t ref_every, int32_t use_network) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (ref_iters < 1 || ref_every < 0) return fail(c, XPBD_EINVAL, "reference iterations >= 1, ref_every >= 0");
+    if (use_network && !c->mlp_ready) return fail(c, XPBD_EINVAL, "a network schedule needs xpbd_set_mlp()");
+    c->ref_iters = ref_iters;
+    c->ref_every = ref_every;
+    c->warm_net = use_network ? 1 : 0;
     return XPBD_OK;
 }
 

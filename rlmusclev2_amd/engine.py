@@ -330,6 +330,10 @@ class XpbdEngine:
     def set_reference_iters(self, n: int):
         self.ctx.call("xpbd_set_reference_iters", int(n))
 
+    def set_warm_schedule(self, ref_iters: int, ref_every: int = 0, use_network: bool = True):
+        """mode 4: a reference solve every ``ref_every``-th substep (0: first of a frame)."""
+        self.ctx.call("xpbd_set_warm_schedule", int(ref_iters), int(ref_every), int(bool(use_network)))
+
     def snapshot(self, save: bool):
         self.ctx.call("xpbd_snapshot", int(bool(save)))
 

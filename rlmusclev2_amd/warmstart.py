@@ -4,7 +4,9 @@ The reference GPU path is "plain zero-lambda XPBD": every Jacobi iteration
 starts from lambda = 0 (solve_kernels.py:271-281, clear() each iteration at
 mg.py:1348).  Here the first iteration of a substep applies a multiplier
 estimate lambda0 as an impulse along the constraint gradients and the
-remaining iterations carry lambda.  A small per-tet MLP predicts lambda0 as a
+remaining iterations are plain zero-lambda iterations from that start (a
+carried lambda would converge towards the compliant solution, whose residual
+alpha * lambda is above the plain solver's).  A small per-tet MLP predicts lambda0 as a
 per-constraint-type gain on the previous substep's final multipliers,
 ``lambda0 = lambda_prev * clamp(1 + y, 0, 2)``, from local state at the start
 of the substep (deformation gradient, fibre data, activation, the previous
@@ -328,10 +330,11 @@ class WarmStartStepper:
     """Frames with the learned warm start and the per-frame fallback to plain
     zero-lambda XPBD.
 
-    A warm frame runs in mode 4: its first substep is a reference solve
-    (lambda carried from zero, ``plain_iters`` iterations) whose multipliers
-    seed the network, and each later substep applies the network's lambda0
-    and runs ``k_iters`` iterations.  An audit
+    A warm frame runs in mode 4: its reference substeps (the first one, or
+    every ``ref_every``-th) are solves with lambda carried from zero and
+    ``plain_iters`` iterations whose multipliers become the base; every other
+    substep applies the base as an impulse (scaled by the network when
+    ``use_network``) and runs ``k_iters`` iterations in all.  An audit
     frame (every ``audit_every``-th frame; 1 = every frame, the exact rule)
     also runs the plain ``plain_iters`` frame from the same start state and
     compares the per-type residuals at the end of the frame: if any type of
@@ -339,17 +342,20 @@ class WarmStartStepper:
     re-run plain and the warm start stays off until the next audit that it
     passes.  Between audits a warm frame still falls back at once when its
     residual is not finite or exceeds the last audited plain residual by more
-    than ``guard``."""
+    than ``guard``; with ``guard=None`` frames between audits are not checked
+    (no residual readback, so the host does not wait for the GPU)."""
 
     def __init__(self, sim, k_iters: int, plain_iters: int | None = None, audit_every: int = 1,
-                 tol: float = 0.0, guard: float = 1.0, use_graph: bool = True):
+                 tol: float = 0.0, guard: float | None = 1.0, use_graph: bool = True, ref_every: int = 0,
+                 use_network: bool = True):
         self.sim = sim
         self.k = int(k_iters)
         self.plain_iters = int(plain_iters or sim.fine_iters)
-        sim.engine.set_reference_iters(self.plain_iters)
+        self.ref_every = int(ref_every)
+        sim.engine.set_warm_schedule(self.plain_iters, self.ref_every, use_network)
         self.audit_every = max(1, int(audit_every))
         self.tol = float(tol)
-        self.guard = float(guard)
+        self.guard = None if guard is None else float(guard)
         self.use_graph = use_graph
         self.ref = None
         self.warm_ok = True
@@ -357,12 +363,12 @@ class WarmStartStepper:
         self.fallbacks = 0
         self.audits = 0
 
-    def _frame(self, mode: int, iters: int):
+    def _frame(self, mode: int, iters: int, residual: bool = True):
         e = self.sim.engine
         e.set_lambda_mode(mode)
         e.native_frame(self.sim.nsubsteps, iters, self.use_graph)
         e.set_lambda_mode(MODE_PLAIN)
-        return e.residuals()
+        return e.residuals() if residual else None
 
     @staticmethod
     def _worse(r, ref, tol):
@@ -382,6 +388,9 @@ class WarmStartStepper:
             r = self._frame(MODE_SCHED, self.k)
             self.warm_ok = not self._worse(r, self.ref, self.tol)
             fallback = not self.warm_ok
+        elif self.warm_ok and self.guard is None:
+            self._frame(MODE_SCHED, self.k, residual=False)
+            fallback = False
         elif self.warm_ok:
             e.snapshot(True)
             r = self._frame(MODE_SCHED, self.k)

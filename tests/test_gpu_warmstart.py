@@ -97,3 +97,22 @@ def test_trained_network_reaches_plain_residual_in_fewer_iterations():
         mean[m] = float(np.mean(vv))
         print("mode", m, "mean iterations", mean[m], "misses", sum(x is None for x in v), np.bincount(vv).tolist())
     assert mean[ws.MODE_NET] <= 10.0 / 1.5       # >= 1.5x fewer iterations than plain
+
+
+def test_scheduled_warm_frames_reach_plain_residual_with_fewer_iterations():
+    """A reference solve (lambda carried from zero, 10 iterations) every third
+    substep and, in between, the last reference's multipliers as a first-
+    iteration impulse with 4 iterations in all: every audited frame ends within
+    3 % of the plain 10-iteration residual of the same frame for every
+    constraint type, with 64 instead of 100 iterations per frame."""
+    cfg, w, sim = _sim(seed=3)
+    for f in range(1, 3):
+        sim.step_frame(f)
+    st = ws.WarmStartStepper(sim, k_iters=4, audit_every=1, tol=0.03, ref_every=3, use_network=False)
+    for f in range(3, 7):
+        info = st.step_frame(f)
+        print(f, info["fallback"], info["residual"], info["plain"])
+        assert not info["fallback"]
+    refs = (sim.nsubsteps + 2) // 3
+    iters = refs * 10 + (sim.nsubsteps - refs) * 4
+    assert iters * 1.5 <= sim.nsubsteps * 10
