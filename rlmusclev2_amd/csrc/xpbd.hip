@@ -233,16 +233,39 @@ struct Corr {
     float cnt;
 };
 
+// an element's own data: everything the projection reads that does not depend
+// on the positions, loaded apart so that a caller can issue the loads before
+// it stages the vertices (every per-element array is sized for all elements)
+template <int FEAT, int LAM>
+struct ElemIn {
+    float4 lam, fib;
+    uint32_t meta;
+    float rvol, rsnh;
+    float B[9];
+};
+
+template <int FEAT, int LAM>
+__device__ __forceinline__ void load_elem(const IterArgs &a, int64_t g, ElemIn<FEAT, LAM> &e) {
+    e.lam = LAM ? a.lam[g] : make_float4(0.f, 0.f, 0.f, 0.f);
+    e.meta = a.meta[g];
+    e.rvol = (FEAT & F_VOL) ? a.rvol[g] : 0.f;
+    e.rsnh = (FEAT & F_SNH) ? a.rsnh[g] : 0.f;
+    e.fib = (FEAT & F_ANISO) ? a.fib[g] : make_float4(0.f, 0.f, 0.f, 0.f);
+    constexpr bool NEED_F = (FEAT & (F_ARAP | F_SNH | F_ANISO)) != 0;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) e.B[q] = NEED_F ? a.Bp[(int64_t)q * a.nptet + g] : 0.f;
+}
+
 // LAM = 0: plain zero-lambda Jacobi (the reference GPU path); 1: lambda carried
 // across the iterations of a substep (a.lam read and updated in place, one
 // thread per element); 2: warm-start impulse, the step applies dlambda = lambda0
 // from a.lam along each constraint gradient and keeps lambda = lambda0.
 template <int FEAT, int DAMP, int LAM = 0>
-__device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const float4 *sx,
-                                             const float4 *spp, const int lv[4], Corr &out) {
-    const float4 lam = LAM ? a.lam[g] : make_float4(0.f, 0.f, 0.f, 0.f);
+__device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const ElemIn<FEAT, LAM> &e,
+                                             const float4 *sx, const float4 *spp, const int lv[4], Corr &out) {
+    const float4 lam = e.lam;
     float4 nl = lam;
-    const uint32_t meta = a.meta[g];
+    const uint32_t meta = e.meta;
     const uint32_t mask = (meta >> 16) & 15u;
     const DevClass dc = a.cls[meta & 0xffffu];
     const ClassParams &cp = dc.p;
@@ -270,7 +293,7 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
                            w[2] * dot(gr[2], gr[2]) + w[3] * dot(gr[3], gr[3]);
         if (wsum > 1e-9f) {
             const float vol = SIXTH * dot(cross(c0, c1), c2);
-            const float rest = a.rvol[g];
+            const float rest = e.rvol;
             const bool comp = a.sp.has_compress && (vol < rest);
             const float k = comp ? cp.kcomp_vol : cp.k_vol;
             if (k != 0.0f) {
@@ -299,7 +322,7 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
     constexpr bool NEED_F = (FEAT & (F_ARAP | F_SNH | F_ANISO)) != 0;
     if (NEED_F && (mask & (M_ARAP | M_SNH | M_ANISO))) {
 #pragma unroll
-        for (int q = 0; q < 9; ++q) B[q] = a.Bp[(int64_t)q * a.nptet + g];
+        for (int q = 0; q < 9; ++q) B[q] = e.B[q];
         const float Ds[9] = {c0.x, c1.x, c2.x, c0.y, c1.y, c2.y, c0.z, c1.z, c2.z};
 #pragma unroll
         for (int i = 0; i < 3; ++i)
@@ -346,7 +369,7 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
             if (dsw != 0.0f) {
                 const float mu = cp.mu, lamS = cp.lam;
                 const float hC = det3(F) - (1.0f + mu / lamS);
-                const float restv = a.rsnh[g];
+                const float restv = e.rsnh;
                 const float inv_detDm = 1.0f / fmaxf(fabsf(restv) * 6.0f, 1.0e-20f);
                 V3 hg[4];
                 hg[0] = inv_detDm * cross(c1, c2);
@@ -400,7 +423,7 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
 
     // ---- aniso_arap: solve_tet_aniso_arap_kernel, solve_kernels.py:1310-1426 ----
     if ((FEAT & F_ANISO) && (mask & M_ANISO)) {
-        const float4 fb = a.fib[g];  // f0 (normalised at build), fiberscale
+        const float4 fb = e.fib;  // f0 (normalised at build), fiberscale
         const V3 f0 = xyz(fb);
         const V3 d = V3{F[0] * f0.x + F[1] * f0.y + F[2] * f0.z, F[3] * f0.x + F[4] * f0.y + F[5] * f0.z,
                         F[6] * f0.x + F[7] * f0.y + F[8] * f0.z};
@@ -506,7 +529,9 @@ __global__ __launch_bounds__(BS) void k_iterate(IterArgs a, const float4 *__rest
             Corr c;
             const ushort4 li = a.lidx[(int64_t)P.tbeg + t];
             const int lv[4] = {li.x, li.y, li.z, li.w};
-            project_elem<FEAT, DAMP>(a, (int64_t)P.tbeg + t, sx, spp, lv, c);
+            ElemIn<FEAT, 0> e;
+            load_elem(a, (int64_t)P.tbeg + t, e);
+            project_elem<FEAT, DAMP>(a, (int64_t)P.tbeg + t, e, sx, spp, lv, c);
 #pragma unroll
             for (int k = 0; k < 4; ++k) sc[tid * 4 + k] = make_float4(c.d[k].x, c.d[k].y, c.d[k].z, c.cnt);
         }
@@ -569,7 +594,9 @@ __global__ __launch_bounds__(256) void k_flat_tets(IterArgs a, const float4 *__r
     const int4 gi = a.gidx[g];
     const int lv[4] = {gi.x, gi.y, gi.z, gi.w};
     Corr c;
-    project_elem<FEAT, DAMP, LAM>(a, g, xin, pprev, lv, c);
+    ElemIn<FEAT, LAM> e;
+    load_elem(a, g, e);
+    project_elem<FEAT, DAMP, LAM>(a, g, e, xin, pprev, lv, c);
     const int4 so = a.slot_of[g];
     corr[so.x] = make_float4(c.d[0].x, c.d[0].y, c.d[0].z, c.cnt);
     corr[so.y] = make_float4(c.d[1].x, c.d[1].y, c.d[1].z, c.cnt);
@@ -686,6 +713,55 @@ __global__ __launch_bounds__(512) void k_block_tets(IterArgs a, BlockArgs b, con
     float4 *spp = smem + M;
     float4 *sc = smem + (DAMP ? 2 : 1) * M;
     const int tid = threadIdx.x, nth = blockDim.x;
+    const uint16_t *slots = b.bslot + 4 * (int64_t)D.tbeg;
+    if (D.ntet <= nth && D.nloc <= nth) {
+        // one element and at most one local vertex per thread: the element's
+        // data and the vertex's tables are loaded before the staging barrier,
+        // so their latency overlaps the vertex gather
+        const bool has_t = tid < D.ntet, has_v = tid < D.nloc;
+        const int64_t g = (int64_t)D.tbeg + tid;
+        ElemIn<FEAT, LAM> e;
+        ushort4 li = make_ushort4(0, 0, 0, 0);
+        if (has_t) {
+            load_elem(a, g, e);
+            li = b.blidx[g];
+        }
+        int gv = 0, ps = -1;
+        uint32_t run = 0;
+        if (has_v) {
+            gv = b.bvt[D.vbeg + tid];
+            run = b.brun[D.vbeg + tid];
+            ps = b.bps[D.vbeg + tid];
+            sx[tid] = xin[gv];
+            if (DAMP) spp[tid] = pprev[gv];
+        }
+        __syncthreads();
+        if (has_t) {
+            const int lv[4] = {li.x, li.y, li.z, li.w};
+            Corr c;
+            project_elem<FEAT, DAMP, LAM>(a, g, e, sx, spp, lv, c);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sc[k * TB + tid] = make_float4(c.d[k].x, c.d[k].y, c.d[k].z, c.cnt);
+        }
+        __syncthreads();
+        if (!has_v) return;
+        const int s0 = (int)(run & 0xffffu), n = (int)(run >> 16);
+        float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
+        for (int j = 0; j < n; ++j) {
+            const float4 q = sc[slots[s0 + j]];
+            ax += q.x;
+            ay += q.y;
+            az += q.z;
+            aw += q.w;
+        }
+        if (ps >= 0) {
+            b.part[ps] = make_float4(ax, ay, az, aw);
+        } else {
+            finish_vertex<DAMP, MODE>(a, gv, sx[tid], DAMP ? spp[tid] : make_float4(0, 0, 0, 0), ax, ay, az, aw,
+                                      xout);
+        }
+        return;
+    }
     for (int i = tid; i < D.nloc; i += nth) {
         const int gv = b.bvt[D.vbeg + i];
         sx[i] = xin[gv];
@@ -697,12 +773,13 @@ __global__ __launch_bounds__(512) void k_block_tets(IterArgs a, BlockArgs b, con
         const ushort4 li = b.blidx[g];
         const int lv[4] = {li.x, li.y, li.z, li.w};
         Corr c;
-        project_elem<FEAT, DAMP, LAM>(a, g, sx, spp, lv, c);
+        ElemIn<FEAT, LAM> e;
+        load_elem(a, g, e);
+        project_elem<FEAT, DAMP, LAM>(a, g, e, sx, spp, lv, c);
 #pragma unroll
         for (int k = 0; k < 4; ++k) sc[k * TB + t] = make_float4(c.d[k].x, c.d[k].y, c.d[k].z, c.cnt);
     }
     __syncthreads();
-    const uint16_t *slots = b.bslot + 4 * (int64_t)D.tbeg;
     for (int i = tid; i < D.nloc; i += nth) {
         const uint32_t run = b.brun[D.vbeg + i];
         const int s0 = (int)(run & 0xffffu), n = (int)(run >> 16);
