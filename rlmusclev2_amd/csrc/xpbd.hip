@@ -1298,7 +1298,7 @@ __global__ __launch_bounds__(256) void k_mg_solve(MgArgs m, int nc, float4 *__re
     qprev[j] = q;
 }
 
-constexpr int MGC_T = 512;   // threads per coarse block workgroup
+constexpr int MGC_T = 1024;  // threads per coarse block workgroup
 
 __global__ __launch_bounds__(MGC_T) void k_mg_coarse(MgArgs m, MgCoarse mc, int iters, float4 *__restrict__ xc) {
     extern __shared__ float4 smem[];
