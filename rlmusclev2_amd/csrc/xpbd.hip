@@ -1497,8 +1497,11 @@ __global__ __launch_bounds__(BS) void k_residual(IterArgs a, const float4 *__res
     }
 }
 
+// without contraction, as the oracle: the integer count is bit-exact also for
+// tets whose determinant is zero up to rounding
 __global__ void k_count_inverted(int nt, const int4 *tets, const float *B, const float4 *X,
                                  int *count) {
+#pragma clang fp contract(off)
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     int inv = 0;
     if (c < nt) {
@@ -1511,7 +1514,9 @@ __global__ void k_count_inverted(int nt, const int4 *tets, const float *B, const
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j)
                 F[3 * i + j] = Ds[3 * i] * b[j] + Ds[3 * i + 1] * b[3 + j] + Ds[3 * i + 2] * b[6 + j];
-        inv = det3(F) <= 0.0f ? 1 : 0;
+        const float det = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+                          F[2] * (F[3] * F[7] - F[4] * F[6]);
+        inv = det <= 0.0f ? 1 : 0;
     }
     // integer reduction is order-free
     if (inv) atomicAdd(count, 1);

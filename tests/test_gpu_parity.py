@@ -212,3 +212,46 @@ def test_block_iteration_sequence_equals_single_iterations():
         for _ in range(n):
             b.engine.iterate(1)
         np.testing.assert_array_equal(a.positions_np(), b.positions_np())
+
+
+@pytest.mark.parametrize("strategy", [1, 2])
+@pytest.mark.parametrize("kind", ["inverted", "flat"])
+def test_inverted_and_flat_tets_match_oracle(strategy, kind):
+    """Tets pushed through themselves to half their height on the other side
+    (det F < 0 with a distinct smallest singular value: the SVD fallback and
+    the reference's sign convention) or squashed to a tenth of their height
+    (det F small and positive), projected once on the GPU and in the oracle
+    from the same state.  Not parity cases, in the reference as here: a tet
+    flattened exactly (det F = 0 up to rounding: the rotation follows the
+    sign of the rounding) or mirrored exactly (F a reflection, all singular
+    values 1: every axis is as close, and which one flips is the SVD's
+    choice)."""
+    cfg = _cfg("cantilever.json")
+    w = workloads.cantilever(0)
+    sim = sims.MultiGridSim(cfg, workload=w, strategy=strategy, block_tets=128)
+    ref = sims.MultiGridSim(cfg, workload=w, engine_cls=oracle.OracleSim)
+    x = w.pos0.copy()
+    t = w.tets
+    rng = np.random.default_rng(0)
+    pick = rng.choice(len(t), size=12, replace=False)
+    for e in pick:
+        a, b, c, d = t[e]
+        n = np.cross(x[b] - x[a], x[c] - x[a])
+        n /= np.linalg.norm(n)
+        h = np.dot(x[d] - x[a], n)
+        x[d] = x[d] - (1.5 if kind == "inverted" else 0.9) * h * n
+    for s in (sim, ref):
+        s.engine.set_state(x, x, np.zeros_like(x))
+    n_inv = ref.count_inverted()
+    print(kind, "inverted before", n_inv, sim.count_inverted())
+    assert sim.count_inverted() == n_inv
+    assert (n_inv > 0) == (kind == "inverted")
+    for s in (sim, ref):
+        s.engine.iterate(1)
+    p, q = sim.positions_np(), ref.positions_np()
+    assert np.all(np.isfinite(p))
+    tol = 1e-4 * _bbox_diag(w.pos0)
+    err = np.abs(p - q).max()
+    print("max |dx| after one iteration", err, "tol", tol, "inverted after", sim.count_inverted(), ref.count_inverted())
+    assert err <= tol
+    assert sim.count_inverted() == ref.count_inverted()
