@@ -692,16 +692,17 @@ struct BlockArgs {
     const int32_t *bvt, *bps;
     const uint32_t *brun;
     const ushort4 *blidx;
-    const uint16_t *bslot;
+    const ushort4 *bcpos;        // per element: LDS positions of its four corners (vertex-major)
     const int32_t *svid, *soff;
     float4 *part;
     int max_nloc, block_tets;
 };
 
 // LDS: vertex table (and previous positions when damping), then the corner
-// corrections of all block elements, corner-major (sc[k * TB + t]) so that a
-// wavefront's stores are contiguous.  After one barrier every local entry adds
-// its corners in ascending (element, corner) order.
+// corrections of all block elements in vertex-major order: each element stores
+// its four corners at their places in the block's local CSR (bcpos), so after
+// one barrier every local entry adds a contiguous run, in ascending (element,
+// corner) order.
 template <int FEAT, int DAMP, int LAM, int MODE>
 __global__ __launch_bounds__(512) void k_block_tets(IterArgs a, BlockArgs b, const float4 *__restrict__ xin,
                                                     float4 *__restrict__ xout,
@@ -713,7 +714,6 @@ __global__ __launch_bounds__(512) void k_block_tets(IterArgs a, BlockArgs b, con
     float4 *spp = smem + M;
     float4 *sc = smem + (DAMP ? 2 : 1) * M;
     const int tid = threadIdx.x, nth = blockDim.x;
-    const uint16_t *slots = b.bslot + 4 * (int64_t)D.tbeg;
     if (D.ntet <= nth && D.nloc <= nth) {
         // one element and at most one local vertex per thread: the element's
         // data and the vertex's tables are loaded before the staging barrier,
@@ -721,10 +721,11 @@ __global__ __launch_bounds__(512) void k_block_tets(IterArgs a, BlockArgs b, con
         const bool has_t = tid < D.ntet, has_v = tid < D.nloc;
         const int64_t g = (int64_t)D.tbeg + tid;
         ElemIn<FEAT, LAM> e;
-        ushort4 li = make_ushort4(0, 0, 0, 0);
+        ushort4 li = make_ushort4(0, 0, 0, 0), cp = li;
         if (has_t) {
             load_elem(a, g, e);
             li = b.blidx[g];
+            cp = b.bcpos[g];
         }
         int gv = 0, ps = -1;
         uint32_t run = 0;
@@ -740,15 +741,17 @@ __global__ __launch_bounds__(512) void k_block_tets(IterArgs a, BlockArgs b, con
             const int lv[4] = {li.x, li.y, li.z, li.w};
             Corr c;
             project_elem<FEAT, DAMP, LAM>(a, g, e, sx, spp, lv, c);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) sc[k * TB + tid] = make_float4(c.d[k].x, c.d[k].y, c.d[k].z, c.cnt);
+            sc[cp.x] = make_float4(c.d[0].x, c.d[0].y, c.d[0].z, c.cnt);
+            sc[cp.y] = make_float4(c.d[1].x, c.d[1].y, c.d[1].z, c.cnt);
+            sc[cp.z] = make_float4(c.d[2].x, c.d[2].y, c.d[2].z, c.cnt);
+            sc[cp.w] = make_float4(c.d[3].x, c.d[3].y, c.d[3].z, c.cnt);
         }
         __syncthreads();
         if (!has_v) return;
         const int s0 = (int)(run & 0xffffu), n = (int)(run >> 16);
         float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
         for (int j = 0; j < n; ++j) {
-            const float4 q = sc[slots[s0 + j]];
+            const float4 q = sc[s0 + j];
             ax += q.x;
             ay += q.y;
             az += q.z;
@@ -776,8 +779,11 @@ __global__ __launch_bounds__(512) void k_block_tets(IterArgs a, BlockArgs b, con
         ElemIn<FEAT, LAM> e;
         load_elem(a, g, e);
         project_elem<FEAT, DAMP, LAM>(a, g, e, sx, spp, lv, c);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) sc[k * TB + t] = make_float4(c.d[k].x, c.d[k].y, c.d[k].z, c.cnt);
+        const ushort4 cp = b.bcpos[g];
+        sc[cp.x] = make_float4(c.d[0].x, c.d[0].y, c.d[0].z, c.cnt);
+        sc[cp.y] = make_float4(c.d[1].x, c.d[1].y, c.d[1].z, c.cnt);
+        sc[cp.z] = make_float4(c.d[2].x, c.d[2].y, c.d[2].z, c.cnt);
+        sc[cp.w] = make_float4(c.d[3].x, c.d[3].y, c.d[3].z, c.cnt);
     }
     __syncthreads();
     for (int i = tid; i < D.nloc; i += nth) {
@@ -785,7 +791,7 @@ __global__ __launch_bounds__(512) void k_block_tets(IterArgs a, BlockArgs b, con
         const int s0 = (int)(run & 0xffffu), n = (int)(run >> 16);
         float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
         for (int j = 0; j < n; ++j) {
-            const float4 q = sc[slots[s0 + j]];
+            const float4 q = sc[s0 + j];
             ax += q.x;
             ay += q.y;
             az += q.z;
@@ -2164,7 +2170,7 @@ struct xpbd_ctx {
     int32_t *d_bvt = nullptr, *d_bps = nullptr, *d_svid = nullptr, *d_soff = nullptr;
     uint32_t *d_brun = nullptr;
     ushort4 *d_blidx = nullptr;
-    uint16_t *d_bslot = nullptr;
+    ushort4 *d_bcpos = nullptr;
     float4 *BPART = nullptr;
     // two-level correction: a finalized coarse context (not owned) and transfer tables
     xpbd_ctx *coarse = nullptr;
@@ -2565,7 +2571,7 @@ BlockArgs make_block_args(xpbd_ctx *c) {
     b.bps = c->d_bps;
     b.brun = c->d_brun;
     b.blidx = c->d_blidx;
-    b.bslot = c->d_bslot;
+    b.bcpos = c->d_bcpos;
     b.svid = c->d_svid;
     b.soff = c->d_soff;
     b.part = c->BPART;
@@ -3285,7 +3291,7 @@ int32_t xpbd_finalize(xpbd_ctx *c, const xpbd_tuning *t) {
             UP(c->d_bps, L.bps);
             UP(c->d_brun, L.brun);
             r = dupload(c, &c->d_blidx, (const ushort4 *)L.blidx.data(), nptet); if (r) return r;
-            UP(c->d_bslot, L.bslot);
+            r = dupload(c, &c->d_bcpos, (const ushort4 *)L.bcpos.data(), nptet); if (r) return r;
             UP(c->d_svid, L.svid);
             UP(c->d_soff, L.soff);
             if ((r = dalloc(c, &c->BPART, std::max<size_t>((size_t)L.n_partial, 1)))) return r;

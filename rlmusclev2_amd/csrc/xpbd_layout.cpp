@@ -191,6 +191,7 @@ std::string build_blocks(const HostInput &in, Layout &L, int feature, int *code)
     L.brun.clear();
     L.blidx.assign(4 * (size_t)ne, 0);
     L.bslot.assign(4 * (size_t)ne, 0);
+    L.bcpos.assign(4 * (size_t)ne, 0);
     std::vector<int32_t> nblk(nv, 0);
     std::vector<int32_t> loc(nv, -1);
     std::vector<int32_t> verts;
@@ -230,8 +231,11 @@ std::string build_blocks(const HostInput &in, Layout &L, int feature, int *code)
             for (int k = 0; k < 4; ++k) {
                 const int l = L.blidx[4 * g + k];
                 // listed in ascending (element, corner) order; the value is the
-                // corner-major LDS index k * TB + element the kernel reads
-                L.bslot[4 * t0 + fill[l]++] = (uint16_t)(k * TB + (g - t0));
+                // corner-major index k * TB + element; the corner's position in
+                // this vertex-major list is where the kernel stores it in LDS
+                const uint32_t pos = fill[l]++;
+                L.bslot[4 * t0 + pos] = (uint16_t)(k * TB + (g - t0));
+                L.bcpos[4 * g + k] = (uint16_t)pos;
             }
         for (int v : verts) loc[v] = -1;
     }

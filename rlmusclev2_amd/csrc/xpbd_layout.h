@@ -83,6 +83,8 @@ struct Layout {
     std::vector<int32_t> bps;                // per local entry: partial slot, -1 = finished in the block
     std::vector<uint32_t> brun;              // per local entry: first slot | slot count << 16
     std::vector<uint16_t> blidx;             // 4 per element: local entry of each corner
+    std::vector<uint16_t> bcpos;             // 4 per element: the corner's position in the block's
+                                             // vertex-major corner list (the inverse of bslot)
     std::vector<uint16_t> bslot;             // 4 per element, per block: corner slots grouped by local entry,
                                              // ascending (element, corner); value = corner * block_tets + element
     std::vector<int32_t> svid;               // vertices finished by the second pass

@@ -71,6 +71,8 @@ int main(int argc, char **argv) {
                 prev = sl;
                 seen[sl]++;
                 int g = D.tbeg + el;
+                // the corner's LDS position is its place in this vertex-major list
+                if (L.bcpos[4 * (size_t)g + k] != (uint16_t)(s0 + j)) { printf("bcpos mismatch\n"); ++bad; }
                 if (L.gidx[4 * g + k] != v || L.blidx[4 * g + k] != i) { printf("slot vertex mismatch\n"); ++bad; }
                 sum += val[4 * g + k];
             }
