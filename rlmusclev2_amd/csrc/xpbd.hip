@@ -1205,8 +1205,8 @@ struct MgCoarse {
     const float *tB;             // per tet: rest inverse B, 9 floats row-major
     const float *tat;            // per tet: 1 / (k dt^2)
     const float *node_w;         // per node: inverse mass (0 = held)
-    const int32_t *slot_off;     // per node + 1: its corner slots (block-local, ascending)
-    const uint16_t *slots;
+    const int32_t *slot_off;     // per node + 1: its run in the block's vertex-major corner list
+    const ushort4 *tcpos;        // per tet: its corners' places in that list (block-local)
     int max_nodes, max_tets;
 };
 
@@ -1306,77 +1306,106 @@ __global__ __launch_bounds__(256) void k_mg_solve(MgArgs m, int nc, float4 *__re
 
 constexpr int MGC_T = 1024;  // threads per coarse block workgroup
 
+// projection of one coarse tet from the block's LDS node table; its four
+// corner corrections go to their places in the block's vertex-major list
+__device__ __forceinline__ void mg_coarse_tet(const float4 *sq, float4 *sc, ushort4 li, ushort4 cp, const float *B,
+                                              float at) {
+    const float4 X0 = sq[li.x], X1 = sq[li.y], X2 = sq[li.z], X3 = sq[li.w];
+    const V3 c0 = xyz(X0) - xyz(X3), c1 = xyz(X1) - xyz(X3), c2 = xyz(X2) - xyz(X3);
+    const float w[4] = {X0.w, X1.w, X2.w, X3.w};
+    float F[9];
+    const float Ds[9] = {c0.x, c1.x, c2.x, c0.y, c1.y, c2.y, c0.z, c1.z, c2.z};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            F[3 * i + j] = Ds[3 * i] * B[j] + Ds[3 * i + 1] * B[3 + j] + Ds[3 * i + 2] * B[6 + j];
+    V3 gr[4];
+    float wsum;
+    const float C = arap_grad(F, B, w, gr, wsum);
+    float dL = 0.f, cnt = 0.f;
+    if (C > 0.0f && wsum != 0.0f) {
+        dL = -C * __builtin_amdgcn_rcpf(wsum + at);
+        cnt = 1.0f;
+    } else {
+        gr[0] = gr[1] = gr[2] = gr[3] = mk3(0.f, 0.f, 0.f);
+    }
+    const V3 d0 = (dL * w[0]) * gr[0], d1 = (dL * w[1]) * gr[1], d2 = (dL * w[2]) * gr[2], d3 = (dL * w[3]) * gr[3];
+    sc[cp.x] = make_float4(d0.x, d0.y, d0.z, cnt);
+    sc[cp.y] = make_float4(d1.x, d1.y, d1.z, cnt);
+    sc[cp.z] = make_float4(d2.x, d2.y, d2.z, cnt);
+    sc[cp.w] = make_float4(d3.x, d3.y, d3.z, cnt);
+}
+
+// all sweeps of one muscle's cage in one workgroup.  A thread keeps its (at
+// most two) tets' data in registers and its node's corner range across the
+// sweeps, so a sweep touches only LDS: project every tet, barrier, each node
+// adds its contiguous corner run (ascending (tet, corner) order) and applies
+// dP / dPw, barrier.
 __global__ __launch_bounds__(MGC_T) void k_mg_coarse(MgArgs m, MgCoarse mc, int iters, float4 *__restrict__ xc) {
     extern __shared__ float4 smem[];
     float4 *sq = smem;                         // xyz, inverse mass
-    float4 *sc = smem + mc.max_nodes;          // corner-major: sc[k * max_tets + t]
+    float4 *sc = smem + mc.max_nodes;          // corner corrections, vertex-major
     const int b = blockIdx.x, tid = threadIdx.x;
     const int n0 = m.boff[b], nn = m.boff[b + 1] - n0;
     const int t0 = mc.tet_off[b], nt = mc.tet_off[b + 1] - t0;
-    const int MT = mc.max_tets;
-    for (int i = tid; i < nn; i += MGC_T) {
-        const float4 q = m.q[n0 + i];
-        sq[i] = make_float4(q.x, q.y, q.z, mc.node_w[n0 + i]);
+    const bool has0 = tid < nt, has1 = tid + MGC_T < nt;
+    ushort4 li0 = make_ushort4(0, 0, 0, 0), li1 = li0, cp0 = li0, cp1 = li0;
+    float B0[9], B1[9], at0 = 0.f, at1 = 0.f;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) B0[q] = B1[q] = 0.f;
+    if (has0) {
+        const int t = t0 + tid;
+        li0 = mc.tlidx[t];
+        cp0 = mc.tcpos[t];
+        at0 = mc.tat[t];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) B0[q] = mc.tB[9 * (int64_t)t + q];
+    }
+    if (has1) {
+        const int t = t0 + tid + MGC_T;
+        li1 = mc.tlidx[t];
+        cp1 = mc.tcpos[t];
+        at1 = mc.tat[t];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) B1[q] = mc.tB[9 * (int64_t)t + q];
+    }
+    const bool hasn = tid < nn;
+    int s0 = 0, ns = 0;
+    if (hasn) {
+        const int base = mc.slot_off[n0];
+        s0 = mc.slot_off[n0 + tid] - base;
+        ns = mc.slot_off[n0 + tid + 1] - mc.slot_off[n0 + tid];
+        const float4 q = m.q[n0 + tid];
+        sq[tid] = make_float4(q.x, q.y, q.z, mc.node_w[n0 + tid]);
     }
     for (int it = 0; it < iters; ++it) {
         __syncthreads();
-        for (int t = tid; t < nt; t += MGC_T) {
-            const ushort4 li = mc.tlidx[t0 + t];
-            const float4 X0 = sq[li.x], X1 = sq[li.y], X2 = sq[li.z], X3 = sq[li.w];
-            const V3 c0 = xyz(X0) - xyz(X3), c1 = xyz(X1) - xyz(X3), c2 = xyz(X2) - xyz(X3);
-            const float w[4] = {X0.w, X1.w, X2.w, X3.w};
-            float B[9], F[9];
-#pragma unroll
-            for (int q = 0; q < 9; ++q) B[q] = mc.tB[9 * (int64_t)(t0 + t) + q];
-            const float Ds[9] = {c0.x, c1.x, c2.x, c0.y, c1.y, c2.y, c0.z, c1.z, c2.z};
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    F[3 * i + j] = Ds[3 * i] * B[j] + Ds[3 * i + 1] * B[3 + j] + Ds[3 * i + 2] * B[6 + j];
-            V3 gr[4];
-            float wsum;
-            const float C = arap_grad(F, B, w, gr, wsum);
-            float dL = 0.f, cnt = 0.f;
-            if (C > 0.0f && wsum != 0.0f) {
-                const float at = mc.tat[t0 + t];
-                dL = -C * __builtin_amdgcn_rcpf(wsum + at);
-                cnt = 1.0f;
-            } else {
-                gr[0] = gr[1] = gr[2] = gr[3] = mk3(0.f, 0.f, 0.f);
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const V3 d = (dL * w[k]) * gr[k];
-                sc[k * MT + t] = make_float4(d.x, d.y, d.z, cnt);
-            }
-        }
+        if (has0) mg_coarse_tet(sq, sc, li0, cp0, B0, at0);
+        if (has1) mg_coarse_tet(sq, sc, li1, cp1, B1, at1);
         __syncthreads();
-        for (int i = tid; i < nn; i += MGC_T) {
+        if (hasn) {
             float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
-            const int e = mc.slot_off[n0 + i + 1];
-            for (int q = mc.slot_off[n0 + i]; q < e; ++q) {
-                const int sl = mc.slots[q];
-                const float4 c = sc[(sl & 3) * MT + (sl >> 2)];
+            for (int j = 0; j < ns; ++j) {
+                const float4 c = sc[s0 + j];
                 ax += c.x;
                 ay += c.y;
                 az += c.z;
                 aw += c.w;
             }
             if (aw > 1e-9f) {   // apply_dP_kernel, engine.py:689-698
-                float4 x = sq[i];
+                float4 x = sq[tid];
                 x.x = x.x + ax / aw;
                 x.y = x.y + ay / aw;
                 x.z = x.z + az / aw;
-                sq[i] = x;
+                sq[tid] = x;
             }
         }
     }
-    __syncthreads();
-    for (int i = tid; i < nn; i += MGC_T) {
-        const float4 x = sq[i];
-        m.q[n0 + i] = make_float4(x.x, x.y, x.z, 0.f);
-        float4 *o = xc + m.dev_of[n0 + i];   // the coarse context's own state, for readback
+    if (hasn) {
+        const float4 x = sq[tid];
+        m.q[n0 + tid] = make_float4(x.x, x.y, x.z, 0.f);
+        float4 *o = xc + m.dev_of[n0 + tid];   // the coarse context's own state, for readback
         *o = make_float4(x.x, x.y, x.z, o->w);
     }
 }
@@ -3859,6 +3888,7 @@ int32_t xpbd_set_coarse_level(xpbd_ctx *c, xpbd_ctx *cc, const xpbd_coarse_desc 
             max_tets = std::max(max_tets, toff[bb + 1] - toff[bb]);
         }
         if (4 * max_tets > 65535) return fail(c, XPBD_ELAYOUT, "coarse cage block too large for 16-bit slots");
+        if (max_tets > 2 * MGC_T) return fail(c, XPBD_ELAYOUT, "coarse cage block has more than 2048 tets");
         for (int e = 0; e < nct; ++e) {
             const int t = order[e];
             const int bb = tblk[t], n0 = d->blk_off[bb], lt = e - toff[bb];
@@ -3872,25 +3902,32 @@ int32_t xpbd_set_coarse_level(xpbd_ctx *c, xpbd_ctx *cc, const xpbd_coarse_desc 
             for (int q = 0; q < 9; ++q) tB[9 * (size_t)e + q] = cc->rest_matrix[9 * (size_t)t + q];
             tat[e] = kt[t] > 0.f ? (1.0f / kt[t]) / (dt * dt) : 0.f;
         }
+        // vertex-major corner list per block: node by node, each node's corners in
+        // ascending (tet, corner) order; a corner's place is relative to its block
         std::vector<int32_t> soff(nc + 1, 0);
-        std::vector<uint16_t> sl;
+        std::vector<ushort4> tcp(std::max(nct, 1), make_ushort4(0, 0, 0, 0));
         std::vector<float> nw(std::max(nc, 1), 0.f);
         for (int v = 0; v < nc; ++v) {
             std::sort(nslots[v].begin(), nslots[v].end());
-            sl.insert(sl.end(), nslots[v].begin(), nslots[v].end());
-            soff[v + 1] = (int32_t)sl.size();
+            const int bb = nblk_of[v], base = soff[d->blk_off[bb]];
+            for (size_t e = 0; e < nslots[v].size(); ++e) {
+                const int code = nslots[v][e], lt = code >> 2, k = code & 3;
+                const unsigned short pos = (unsigned short)(soff[v] + (int)e - base);
+                ushort4 &c4 = tcp[toff[bb] + lt];
+                if (k == 0) c4.x = pos; else if (k == 1) c4.y = pos; else if (k == 2) c4.z = pos; else c4.w = pos;
+            }
+            soff[v + 1] = soff[v] + (int32_t)nslots[v].size();
             nw[v] = inv_mass_of(cc, v);
         }
-        if (sl.empty()) sl.push_back(0);
         MgCoarse g{};
         int32_t *p_toff, *p_soff;
         ushort4 *p_tl;
         float *p_tB, *p_tat, *p_nw;
-        uint16_t *p_sl;
+        ushort4 *p_tcp;
         if ((r = dupload(c, &p_toff, toff.data(), toff.size())) || (r = dupload(c, &p_tl, tl.data(), tl.size())) ||
             (r = dupload(c, &p_tB, tB.data(), tB.size())) || (r = dupload(c, &p_tat, tat.data(), tat.size())) ||
             (r = dupload(c, &p_nw, nw.data(), nw.size())) || (r = dupload(c, &p_soff, soff.data(), soff.size())) ||
-            (r = dupload(c, &p_sl, sl.data(), sl.size())) || (r = dalloc(c, &m.q, (size_t)std::max(nc, 1))))
+            (r = dupload(c, &p_tcp, tcp.data(), tcp.size())) || (r = dalloc(c, &m.q, (size_t)std::max(nc, 1))))
             return r;
         g.tet_off = p_toff;
         g.tlidx = p_tl;
@@ -3898,7 +3935,7 @@ int32_t xpbd_set_coarse_level(xpbd_ctx *c, xpbd_ctx *cc, const xpbd_coarse_desc 
         g.tat = p_tat;
         g.node_w = p_nw;
         g.slot_off = p_soff;
-        g.slots = p_sl;
+        g.tcpos = p_tcp;
         g.max_nodes = max_nodes;
         g.max_tets = std::max(max_tets, 1);
         const size_t lds = ((size_t)max_nodes + 4 * (size_t)g.max_tets) * sizeof(float4);
