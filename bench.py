@@ -24,9 +24,11 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per projection iteration measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per
 # dispatch, scaled x2.0 and x1.0 by a plain-copy calibration), keyed by (tets, constraint types,
-# strategy, block tets, block threads): profiles/projection_block_humanv2_535k.md
+# strategy, block tets, block threads); the runs are summarised under profiles/
 TRAFFIC = {
-    (534882, ("tetarapsvd", "volume"), 2, 512, 512): 71858074.0,
+    (534882, ("tetarapsvd", "volume"), 2, 512, 512): 71432706.0,                    # humanV2-scale, plain
+    (534882, ("snh", "aniso_arap"), 2, 256, 256): 78346826.0,                       # humanV2-scale, SNH + aniso
+    (7063998, ("tetarapsvd", "volume", "aniso_arap"), 2, 256, 256): 1000229263.0,  # 512 instances
 }
 
 WORKLOADS = {
