@@ -198,7 +198,7 @@ typedef struct {
     int32_t allow_self;       /* movable-vs-movable pairs                               */
     int32_t iters;            /* DCD passes per substep (humanV2 "iters", default 2)    */
     int32_t rebuild_substeps; /* 1 = broadphase every substep, 0 = once per frame       */
-    int64_t max_pairs;        /* pair / contact capacity (default 1 << 22)              */
+    int64_t max_pairs;        /* pair / contact capacity (default 1 << 22, below 2^31)  */
 } xpbd_dcd_params;
 /* surface triangles (ntris*3 vertex ids) and optional group (0 movable, 1 static);
  * enables the DCD post pass before the bone push (humanV2.py:820-866) */
@@ -206,14 +206,16 @@ int32_t xpbd_set_dcd(xpbd_ctx *ctx, int32_t ntris, const int32_t *tris, const in
                      const xpbd_dcd_params *p);
 /* one DCD pass on the current positions (UdcfDcdCollider.solve, solver.py:365-440) */
 int32_t xpbd_dcd_solve(xpbd_ctx *ctx, int32_t rebuild);
-/* the (i<j)-sorted candidate pair list of the last broadphase (int32 pairs, 2 per entry) */
+/* the (i<j)-sorted candidate pair list of the last broadphase (int32 pairs, 2 per entry).
+ * The broadphase keeps its pairs in append order; this call sorts a copy (segments by first
+ * index, each by the second).  n_total = pairs found; beyond max_pairs some were dropped. */
 int32_t xpbd_dcd_pairs(xpbd_ctx *ctx, int32_t *out, int64_t cap, int64_t *n_total);
 /* UdcfDcdCollider.stats (solver.py:442-457): raw pair count, contacts of the last pass, cell */
 int32_t xpbd_dcd_stats(xpbd_ctx *ctx, int64_t *n_pairs, int64_t *n_contacts, float *cell);
 /* broadphase limits hit at the last rebuild (0 and 0 on a valid mesh): triangles left out
  * (bounding sphere above the top grid level, or a non-finite or out-of-range centroid) and
- * truncated work (searches stopped at the visit or pair limit, plus pair segments too long
- * to sort).  No reference counterpart. */
+ * truncated work (searches stopped at the visit limit, pairs dropped beyond max_pairs, and
+ * pair segments too long to sort in the last pair-list query).  No reference counterpart. */
 int32_t xpbd_dcd_overflow(xpbd_ctx *ctx, int64_t *left_out, int64_t *truncated);
 
 /* ---- learned warm start: per-tet MLP (no reference counterpart) ------- */

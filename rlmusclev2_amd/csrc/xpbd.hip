@@ -1679,18 +1679,21 @@ __global__ void k_dcd_scatter(DcdDev d) {
     if (d.tlvl[i] < 0) return;
     const uint32_t h = (uint32_t)d.cell[i].w;
     const uint32_t q = d.bucket_off[h] + atomicAdd(&d.bucket_fill[h], 1u);
-    d.sorted[q] = i;
+    const int4 c = d.cell[i];
+    const int4 t = d.tris[i];
     d.rec[q] = d.cr[i];
+    d.key[q] = make_int4(i | (d.tlvl[i] << DCD_LVL_SHIFT), c.x, c.y, c.z);
+    d.tv[q] = make_int4(t.x, t.y, t.z, d.group ? d.group[i] : 0);
 }
 
-// emit rules of broadphase_find_pairs_kernel (:851-915) for candidate j of i
-__device__ __forceinline__ bool dcd_emit(const DcdDev &d, int i, int j, const int4 ti, float4 ci, float4 ni, int gi) {
+// emit rules of broadphase_find_pairs_kernel (:851-915) for a candidate j of i
+// that passed the sphere test; tj = j's vertex ids, w = its group
+__device__ __forceinline__ bool dcd_emit(const DcdDev &d, int j, const int4 ti, float4 ni, int gi, int4 tj) {
 #pragma clang fp contract(off)
-    const int gj = d.group ? d.group[j] : 0;
+    const int gj = tj.w;
     if (gi == 1 && gj == 1) return false;
     if (gi == 0 && gj == 0) {
         if (!d.allow_self) return false;
-        const int4 tj = d.tris[j];
         if (ti.x == tj.x || ti.x == tj.y || ti.x == tj.z || ti.y == tj.x || ti.y == tj.y || ti.y == tj.z ||
             ti.z == tj.x || ti.z == tj.y || ti.z == tj.z)
             return false;
@@ -1699,35 +1702,36 @@ __device__ __forceinline__ bool dcd_emit(const DcdDev &d, int i, int j, const in
             if (ni.x * nj.x + ni.y * nj.y + ni.z * nj.z > d.normal_cull) return false;
         }
     }
-    const float4 cj = d.cr[j];
-    const V3 dd = V3{ci.x - cj.x, ci.y - cj.y, ci.z - cj.z};
-    return !(sqrtf(dot(dd, dd)) > ci.w + cj.w + d.thickness);
+    return true;
 }
 
-// FILL = 0 counts the pairs of every segment, FILL = 1 writes them; both passes
-// visit the same candidates.  DCD_LANES consecutive lanes share one triangle
+// Broadphase search, one pass.  DCD_LANES consecutive lanes share one triangle
 // (taken in bucket order, so a wavefront searches a few neighbouring cells)
 // and split each cell's candidates; the visit limit is checked on whole cells,
-// identically in every lane.
+// identically in every lane.  Emitted pairs (min, max) are staged in LDS and
+// appended to the pair list with one cursor atomic per workgroup (a lane
+// whose pair finds the stage full appends it directly).
 constexpr int DCD_LANES = 16;
 
-template <int FILL>
-__global__ __launch_bounds__(256) void k_dcd_pairs(DcdDev d) {
+__global__ __launch_bounds__(256) void k_dcd_search(DcdDev d) {
 #pragma clang fp contract(off)
+    __shared__ int2 stage[DCD_STAGE];
+    __shared__ uint32_t s_n;
+    __shared__ unsigned long long s_base;
+    if (threadIdx.x == 0) s_n = 0u;
+    __syncthreads();
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = (int)(g & (DCD_LANES - 1));
     const int64_t q0 = g / DCD_LANES;
     const bool live = q0 < (int64_t)d.bucket_off[(size_t)1 << d.tbits];
-    uint32_t local = 0;
     bool stop = false;
-    int i = -1;
     if (live) {
-        i = d.sorted[q0];
-        const int la = d.tlvl[i];
-        const int4 ti = d.tris[i];
-        const float4 ci = d.cr[i];
-        const float4 ni = d.nrm[i];
-        const int gi = d.group ? d.group[i] : 0;
+        const int4 ki = d.key[q0];
+        const int i = ki.x & ((1 << DCD_LVL_SHIFT) - 1);
+        const int la = ki.x >> DCD_LVL_SHIFT;
+        const int4 ti = d.tv[q0];
+        const float4 ci = d.rec[q0];
+        const float4 ni = d.normal_cull < 1.0f ? d.nrm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
         uint32_t visits = 0;
         for (int b = la; b < DCD_LEVELS && !stop; ++b) {
             if (d.lvl_cnt[b] == 0) continue;
@@ -1749,34 +1753,49 @@ __global__ __launch_bounds__(256) void k_dcd_pairs(DcdDev d) {
                         }
                         for (uint32_t q = s0 + lane; q < e; q += DCD_LANES) {
                             const float4 cj = d.rec[q];
+                            const int4 kj = d.key[q];
                             const V3 dd = V3{ci.x - cj.x, ci.y - cj.y, ci.z - cj.z};
                             if (sqrtf(dot(dd, dd)) > ci.w + cj.w + d.thickness) continue;
-                            const int j = d.sorted[q];
+                            const int j = kj.x & ((1 << DCD_LVL_SHIFT) - 1);
+                            // visit j only from its own cell and level
+                            if ((kj.x >> DCD_LVL_SHIFT) != b || kj.y != x || kj.z != y || kj.w != z) continue;
                             if (j == i || (b == la && j < i)) continue;
-                            const int4 cc = d.cell[j];   // visit j only from its own cell
-                            if (cc.x != x || cc.y != y || cc.z != z || d.tlvl[j] != b) continue;
-                            if (!dcd_emit(d, i, j, ti, ci, ni, gi)) continue;
-                            const int sg = min(i, j), ot = max(i, j);
-                            if (!FILL) {
-                                if (sg == i) ++local;
-                                else atomicAdd(&d.pair_cnt[sg], 1u);
+                            if (!dcd_emit(d, j, ti, ni, ti.w, d.tv[q])) continue;
+                            const int2 pr = make_int2(min(i, j), max(i, j));
+                            const uint32_t p = atomicAdd(&s_n, 1u);
+                            if (p < (uint32_t)DCD_STAGE) {
+                                stage[p] = pr;
                             } else {
-                                const uint64_t o = (uint64_t)d.pair_off[sg] + atomicAdd(&d.pair_fill[sg], 1u);
-                                if ((int64_t)o < d.max_pairs) d.pairs[o] = make_int2(sg, ot);
+                                const unsigned long long o = atomicAdd(d.npairs, 1ull);
+                                if (o < (unsigned long long)d.max_pairs) d.pairs[o] = pr;
                             }
                         }
                     }
         }
+        if (stop && lane == 0) atomicAdd(&d.overflow[1], 1u);
     }
-    if (!FILL) {
-        // the lanes of one triangle are DCD_LANES consecutive lanes of a wavefront
-#pragma unroll
-        for (int o = 1; o < DCD_LANES; o <<= 1) local += (uint32_t)__shfl_xor((int)local, o);
-        if (live && lane == 0) {
-            if (local) atomicAdd(&d.pair_cnt[i], local);
-            if (stop) atomicAdd(&d.overflow[1], 1u);
-        }
+    __syncthreads();
+    const uint32_t n = min(s_n, (uint32_t)DCD_STAGE);
+    if (threadIdx.x == 0) s_base = n ? atomicAdd(d.npairs, (unsigned long long)n) : 0ull;
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) {
+        const unsigned long long o = s_base + t;
+        if (o < (unsigned long long)d.max_pairs) d.pairs[o] = stage[t];
     }
+}
+
+// the (i, j)-sorted pair list, built on request: count per first index,
+// scan, scatter into segments, then sort each segment by the second index
+__global__ void k_dcd_segcount(DcdDev d, int64_t n) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) atomicAdd(&d.pair_cnt[d.pairs[k].x], 1u);
+}
+
+__global__ void k_dcd_segscatter(DcdDev d, int64_t n) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int2 pr = d.pairs[k];
+    d.pairs_seg[d.pair_off[pr.x] + atomicAdd(&d.pair_fill[pr.x], 1u)] = pr;
 }
 
 // sort each triangle's segment by partner index.  DCD_LANES lanes share a
@@ -1800,17 +1819,17 @@ __global__ __launch_bounds__(256) void k_dcd_sortseg(DcdDev d) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int64_t e = lane + (int64_t)k * DCD_LANES;
-            y[k] = e < n ? d.pairs[lo + e].y : 0;
+            y[k] = e < n ? d.pairs_seg[lo + e].y : 0;
             rk[k] = 0;
         }
         for (int64_t m = 0; m < n; ++m) {
-            const int ym = d.pairs[lo + m].y;
+            const int ym = d.pairs_seg[lo + m].y;
 #pragma unroll
             for (int k = 0; k < 4; ++k) rk[k] += ym < y[k] ? 1 : 0;
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            if (lane + (int64_t)k * DCD_LANES < n) d.pairs[lo + rk[k]] = make_int2((int)i, y[k]);
+            if (lane + (int64_t)k * DCD_LANES < n) d.pairs_seg[lo + rk[k]] = make_int2((int)i, y[k]);
         return;
     }
     if (lane != 0) return;
@@ -1819,13 +1838,13 @@ __global__ __launch_bounds__(256) void k_dcd_sortseg(DcdDev d) {
         return;
     }
     for (int64_t a = lo + 1; a < hi; ++a) {
-        const int2 v = d.pairs[a];
+        const int2 v = d.pairs_seg[a];
         int64_t b = a - 1;
-        while (b >= lo && d.pairs[b].y > v.y) {
-            d.pairs[b + 1] = d.pairs[b];
+        while (b >= lo && d.pairs_seg[b].y > v.y) {
+            d.pairs_seg[b + 1] = d.pairs_seg[b];
             --b;
         }
-        d.pairs[b + 1] = v;
+        d.pairs_seg[b + 1] = v;
     }
 }
 
@@ -1884,102 +1903,37 @@ __device__ __forceinline__ int dcd_gen_t(V3 N, V3 v0, V3 v1, V3 v2, float d, V3 
     return 1;
 }
 
-// narrow_kernel (:317-474); the contact stays in its pair slot
-__global__ void k_dcd_narrow(DcdDev d, const float4 *X) {
+// narrow_kernel (:317-474) with respond_kernel (:714-799) fused in.  The
+// straddle test (each triangle's plane cuts the other triangle, overlapping
+// intervals on the intersection line) is all that most pairs reach; a pair
+// that passes gets its deepest points (C1, C2) and, if the contact is valid,
+// its response at once.  The response reads positions that no kernel of the
+// pass writes before the apply and sums in fixed point, so fusing it changes
+// nothing in the result.
+struct DcdGeo {
+    V3 q00, q01, q02, q10, q11, q12, N1, N2;
+    float d1, d2;
+};
+
+__device__ __forceinline__ bool dcd_straddle(const float4 *X, int4 A, int4 Bt, DcdGeo &g) {
 #pragma clang fp contract(off)
-    const int64_t total = min((int64_t)d.pair_off[d.ntris], d.max_pairs);
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
-         k += (int64_t)gridDim.x * blockDim.x) {
-        DcdContact o;
-        o.valid = 0;
-        const int2 pr = d.pairs[k];
-        const int4 A = d.tris[pr.x], Bt = d.tris[pr.y];
-        const V3 q00 = xyz(X[A.x]), q01 = xyz(X[A.y]), q02 = xyz(X[A.z]);
-        const V3 q10 = xyz(X[Bt.x]), q11 = xyz(X[Bt.y]), q12 = xyz(X[Bt.z]);
-        const V3 n1f = cross(q01 - q00, q02 - q00), n2f = cross(q11 - q10, q12 - q10);
-        const float l1 = sqrtf(dot(n1f, n1f)), l2 = sqrtf(dot(n2f, n2f));
-        bool ok = !(l1 == 0.0f || l2 == 0.0f);
-        V3 N1{}, N2{}, D{};
-        float d1 = 0.f, d2 = 0.f;
-        if (ok) {
-            N1 = n1f / l1;
-            N2 = n2f / l2;
-            d1 = -dot(N1, q00);
-            d2 = -dot(N2, q10);
-            const V3 cp = cross(N1, N2);
-            const float lcp = sqrtf(dot(cp, cp));
-            ok = !(lcp < 1.0e-8f);
-            if (ok) D = cp / lcp;
-        }
-        float t1, t2, t3, t4;
-        if (ok) ok = dcd_gen_t(N2, q00, q01, q02, d2, D, t1, t2) && dcd_gen_t(N1, q10, q11, q12, d1, D, t3, t4);
-        if (ok) ok = (t2 >= t3 && t4 >= t1);
-        if (ok) {
-            float C1 = 0.f, C2 = 0.f;
-            V3 b1{0, 0, 0}, b2{0, 0, 0}, b3{0, 0, 0}, b4{0, 0, 0};
-            for (int j = 0; j < 3; ++j) {
-                const V3 q0j = dcd_pick(q00, q01, q02, j), q1j = dcd_pick(q10, q11, q12, j);
-                const V3 P0j = q0j - dot(N2, q0j - q10) * N2;
-                if (dcd_inside(q10, q11, q12, P0j)) {
-                    const float cand = dot(N2, q0j) + d2;
-                    if (C1 < -cand) { C1 = -cand; b1 = dcd_bary(q00, q01, q02, q0j); b2 = dcd_bary(q10, q11, q12, q0j - cand * N2); }
-                }
-                const float dq = dot(N1, N2);
-                if (dq != 0.0f) {
-                    const V3 Q0j = q1j + (dot(N1, q00 - q1j) / dq) * N2;
-                    if (dcd_inside(q00, q01, q02, Q0j)) {
-                        const float cand = dot(N2, Q0j) + d2;
-                        if (C1 < -cand) { C1 = -cand; b1 = dcd_bary(q00, q01, q02, Q0j); b2 = dcd_bary(q10, q11, q12, Q0j - cand * N2); }
-                    }
-                }
-                const V3 P1j = q1j - dot(N1, q1j - q00) * N1;
-                if (dcd_inside(q00, q01, q02, P1j)) {
-                    const float cand = dot(N1, q1j) + d1;
-                    if (C2 < -cand) { C2 = -cand; b3 = dcd_bary(q10, q11, q12, q1j); b4 = dcd_bary(q00, q01, q02, q1j - cand * N1); }
-                }
-                const float dq2 = dot(N2, N1);
-                if (dq2 != 0.0f) {
-                    const V3 Q1j = q0j + (dot(N2, q10 - q0j) / dq2) * N1;
-                    if (dcd_inside(q10, q11, q12, Q1j)) {
-                        const float cand = dot(N1, Q1j) + d1;
-                        if (C2 < -cand) { C2 = -cand; b3 = dcd_bary(q10, q11, q12, Q1j); b4 = dcd_bary(q00, q01, q02, Q1j - cand * N1); }
-                    }
-                }
-                const int jn = (j + 1) % 3;
-                const V3 q0jn = dcd_pick(q00, q01, q02, jn), q1jn = dcd_pick(q10, q11, q12, jn);
-                const V3 P0jn = q0jn - dot(N2, q0jn - q10) * N2;
-                const V3 P1jn = q1jn - dot(N1, q1jn - q00) * N1;
-                for (int kk = 0; kk < 3; ++kk) {
-                    const int kn = (kk + 1) % 3;
-                    V3 hit;
-                    if (dcd_line(P0j, P0jn, dcd_pick(q10, q11, q12, kk), dcd_pick(q10, q11, q12, kn), q0j, q0jn, hit)) {
-                        const float cand = dot(N2, hit) + d2;
-                        if (C1 < -cand) { C1 = -cand; b1 = dcd_bary(q00, q01, q02, hit); b2 = dcd_bary(q10, q11, q12, hit - cand * N2); }
-                    }
-                    if (dcd_line(P1j, P1jn, dcd_pick(q00, q01, q02, kk), dcd_pick(q00, q01, q02, kn), q1j, q1jn, hit)) {
-                        const float cand = dot(N1, hit) + d1;
-                        if (C2 < -cand) { C2 = -cand; b3 = dcd_bary(q10, q11, q12, hit); b4 = dcd_bary(q00, q01, q02, hit - cand * N1); }
-                    }
-                }
-            }
-            if (!(C1 <= d.offset_eps || C2 <= d.offset_eps)) {
-                o.valid = 1;
-                o.C1 = C1;
-                o.C2 = C2;
-                const bool first = C1 < C2;
-                const int4 cu = first ? A : Bt, cl = first ? Bt : A;
-                o.cur[0] = cu.x; o.cur[1] = cu.y; o.cur[2] = cu.z;
-                o.closest[0] = cl.x; o.closest[1] = cl.y; o.closest[2] = cl.z;
-                o.C = first ? C1 : C2;
-                const V3 n = first ? N2 : N1, ba = first ? b1 : b3, bb = first ? b2 : b4;
-                o.n[0] = n.x; o.n[1] = n.y; o.n[2] = n.z;
-                o.b1[0] = ba.x; o.b1[1] = ba.y; o.b1[2] = ba.z;
-                o.b2[0] = bb.x; o.b2[1] = bb.y; o.b2[2] = bb.z;
-                atomicAdd(d.ncontact, 1u);
-            }
-        }
-        d.contacts[k] = o;
-    }
+    g.q00 = xyz(X[A.x]); g.q01 = xyz(X[A.y]); g.q02 = xyz(X[A.z]);
+    g.q10 = xyz(X[Bt.x]); g.q11 = xyz(X[Bt.y]); g.q12 = xyz(X[Bt.z]);
+    const V3 n1f = cross(g.q01 - g.q00, g.q02 - g.q00), n2f = cross(g.q11 - g.q10, g.q12 - g.q10);
+    const float l1 = sqrtf(dot(n1f, n1f)), l2 = sqrtf(dot(n2f, n2f));
+    if (l1 == 0.0f || l2 == 0.0f) return false;
+    g.N1 = n1f / l1;
+    g.N2 = n2f / l2;
+    g.d1 = -dot(g.N1, g.q00);
+    g.d2 = -dot(g.N2, g.q10);
+    const V3 cp = cross(g.N1, g.N2);
+    const float lcp = sqrtf(dot(cp, cp));
+    if (lcp < 1.0e-8f) return false;
+    const V3 D = cp / lcp;
+    float t1, t2, t3, t4;
+    if (!dcd_gen_t(g.N2, g.q00, g.q01, g.q02, g.d2, D, t1, t2)) return false;
+    if (!dcd_gen_t(g.N1, g.q10, g.q11, g.q12, g.d1, D, t3, t4)) return false;
+    return t2 >= t3 && t4 >= t1;
 }
 
 __device__ __forceinline__ void dcd_add(const DcdDev &d, int v, V3 c) {
@@ -1989,19 +1943,76 @@ __device__ __forceinline__ void dcd_add(const DcdDev &d, int v, V3 c) {
     atomicAdd(&d.acc_n[v], 1u);
 }
 
-// respond_kernel (:714-799)
-__global__ void k_dcd_respond(DcdDev d, const float4 *X) {
+__global__ __launch_bounds__(256) void k_dcd_narrow(DcdDev d, const float4 *X) {
 #pragma clang fp contract(off)
-    const int64_t total = min((int64_t)d.pair_off[d.ntris], d.max_pairs);
+    const int64_t total = (int64_t)min(*d.npairs, (unsigned long long)d.max_pairs);
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
          k += (int64_t)gridDim.x * blockDim.x) {
-        const DcdContact &c = d.contacts[k];
-        if (!c.valid) continue;
-        const float pen = c.C;
+        const int2 pr = d.pairs[k];
+        const int4 A = d.tris[pr.x], Bt = d.tris[pr.y];
+        DcdGeo g;
+        if (!dcd_straddle(X, A, Bt, g)) continue;
+        const V3 q00 = g.q00, q01 = g.q01, q02 = g.q02, q10 = g.q10, q11 = g.q11, q12 = g.q12;
+        const V3 N1 = g.N1, N2 = g.N2;
+        const float d1 = g.d1, d2 = g.d2;
+        float C1 = 0.f, C2 = 0.f;
+        V3 b1{0, 0, 0}, b2{0, 0, 0}, b3{0, 0, 0}, b4{0, 0, 0};
+        for (int j = 0; j < 3; ++j) {
+            const V3 q0j = dcd_pick(q00, q01, q02, j), q1j = dcd_pick(q10, q11, q12, j);
+            const V3 P0j = q0j - dot(N2, q0j - q10) * N2;
+            if (dcd_inside(q10, q11, q12, P0j)) {
+                const float cand = dot(N2, q0j) + d2;
+                if (C1 < -cand) { C1 = -cand; b1 = dcd_bary(q00, q01, q02, q0j); b2 = dcd_bary(q10, q11, q12, q0j - cand * N2); }
+            }
+            const float dq = dot(N1, N2);
+            if (dq != 0.0f) {
+                const V3 Q0j = q1j + (dot(N1, q00 - q1j) / dq) * N2;
+                if (dcd_inside(q00, q01, q02, Q0j)) {
+                    const float cand = dot(N2, Q0j) + d2;
+                    if (C1 < -cand) { C1 = -cand; b1 = dcd_bary(q00, q01, q02, Q0j); b2 = dcd_bary(q10, q11, q12, Q0j - cand * N2); }
+                }
+            }
+            const V3 P1j = q1j - dot(N1, q1j - q00) * N1;
+            if (dcd_inside(q00, q01, q02, P1j)) {
+                const float cand = dot(N1, q1j) + d1;
+                if (C2 < -cand) { C2 = -cand; b3 = dcd_bary(q10, q11, q12, q1j); b4 = dcd_bary(q00, q01, q02, q1j - cand * N1); }
+            }
+            const float dq2 = dot(N2, N1);
+            if (dq2 != 0.0f) {
+                const V3 Q1j = q0j + (dot(N2, q10 - q0j) / dq2) * N1;
+                if (dcd_inside(q10, q11, q12, Q1j)) {
+                    const float cand = dot(N1, Q1j) + d1;
+                    if (C2 < -cand) { C2 = -cand; b3 = dcd_bary(q10, q11, q12, Q1j); b4 = dcd_bary(q00, q01, q02, Q1j - cand * N1); }
+                }
+            }
+            const int jn = (j + 1) % 3;
+            const V3 q0jn = dcd_pick(q00, q01, q02, jn), q1jn = dcd_pick(q10, q11, q12, jn);
+            const V3 P0jn = q0jn - dot(N2, q0jn - q10) * N2;
+            const V3 P1jn = q1jn - dot(N1, q1jn - q00) * N1;
+            for (int kk = 0; kk < 3; ++kk) {
+                const int kn = (kk + 1) % 3;
+                V3 hit;
+                if (dcd_line(P0j, P0jn, dcd_pick(q10, q11, q12, kk), dcd_pick(q10, q11, q12, kn), q0j, q0jn, hit)) {
+                    const float cand = dot(N2, hit) + d2;
+                    if (C1 < -cand) { C1 = -cand; b1 = dcd_bary(q00, q01, q02, hit); b2 = dcd_bary(q10, q11, q12, hit - cand * N2); }
+                }
+                if (dcd_line(P1j, P1jn, dcd_pick(q00, q01, q02, kk), dcd_pick(q00, q01, q02, kn), q1j, q1jn, hit)) {
+                    const float cand = dot(N1, hit) + d1;
+                    if (C2 < -cand) { C2 = -cand; b3 = dcd_bary(q10, q11, q12, hit); b4 = dcd_bary(q00, q01, q02, hit - cand * N1); }
+                }
+            }
+        }
+        if (C1 <= d.offset_eps || C2 <= d.offset_eps) continue;
+        atomicAdd(&d.ncontact[0], 1u);
+        const bool first = C1 < C2;
+        const int4 cu4 = first ? A : Bt, cl4 = first ? Bt : A;
+        const int cu[3] = {cu4.x, cu4.y, cu4.z}, cl[3] = {cl4.x, cl4.y, cl4.z};
+        const V3 nrm = first ? N2 : N1;
+        const V3 cb1 = first ? b1 : b3, cb2 = first ? b2 : b4;
+        // respond_kernel (:714-799)
+        const float pen = first ? C1 : C2;
         float C = pen * d.multiplier;
         if (C == 0.0f) continue;
-        const V3 nrm = V3{c.n[0], c.n[1], c.n[2]};
-        const int cu[3] = {c.cur[0], c.cur[1], c.cur[2]}, cl[3] = {c.closest[0], c.closest[1], c.closest[2]};
         float wcu[3], wcl[3];
         for (int j = 0; j < 3; ++j) { wcu[j] = X[cu[j]].w; wcl[j] = X[cl[j]].w; }
         const bool solid_cur = wcu[0] == 0.0f || wcu[1] == 0.0f || wcu[2] == 0.0f;
@@ -2014,11 +2025,11 @@ __global__ void k_dcd_respond(DcdDev d, const float4 *X) {
             vcl[j] = (xyz(X[cl[j]]) + push_clo) - xyz(d.xprev[cl[j]]);
             vcu[j] = (xyz(X[cu[j]]) + push_cur) - xyz(d.xprev[cu[j]]);
         }
-        const V3 g = nrm / sqrtf(dot(nrm, nrm));
-        const V3 vclo = (c.b1[0] * vcl[0] + c.b1[1] * vcl[1]) + c.b1[2] * vcl[2];
-        const V3 perp_clo = vclo - dot(vclo, g) * g;
-        const V3 vcur = (c.b2[0] * vcu[0] + c.b2[1] * vcu[1]) + c.b2[2] * vcu[2];
-        const V3 perp_cur = vcur - dot(vcur, g) * g;
+        const V3 gn = nrm / sqrtf(dot(nrm, nrm));
+        const V3 vclo = (cb1.x * vcl[0] + cb1.y * vcl[1]) + cb1.z * vcl[2];
+        const V3 perp_clo = vclo - dot(vclo, gn) * gn;
+        const V3 vcur = (cb2.x * vcu[0] + cb2.y * vcu[1]) + cb2.z * vcu[2];
+        const V3 perp_cur = vcur - dot(vcur, gn) * gn;
         V3 diff = perp_cur - perp_clo;
         const float dn = sqrtf(dot(diff, diff));
         if (dn > pen * d.mu_s && dn > 0.0f) diff = ((pen * d.mu_d) / dn) * diff;
@@ -2032,9 +2043,14 @@ __global__ void k_dcd_respond(DcdDev d, const float4 *X) {
     }
 }
 
-// apply_kernel (:802-815), averaged; clears the accumulators for the next pass
+// apply_kernel (:802-815), averaged; clears the accumulators and the contact
+// counter for the next pass (the count is kept as the last pass's)
 __global__ void k_dcd_apply(DcdDev d, float4 *X, int nv) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        d.ncontact[1] = d.ncontact[0];
+        d.ncontact[0] = 0u;
+    }
     if (i >= nv) return;
     const uint32_t n = d.acc_n[i];
     if (n > 0) {
@@ -2048,6 +2064,22 @@ __global__ void k_dcd_apply(DcdDev d, float4 *X, int nv) {
         d.acc[3 * i + 0] = d.acc[3 * i + 1] = d.acc[3 * i + 2] = 0ull;
         d.acc_n[i] = 0u;
     }
+}
+
+// broadphase counters back to zero (one fill kernel: graph-safe, one launch)
+__global__ void k_dcd_reset(DcdDev d, int64_t T) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < T) {
+        d.bucket_cnt[i] = 0u;
+        d.bucket_fill[i] = 0u;
+    }
+    if (i == 0) {
+        *d.rmax_bits = 0u;
+        *d.npairs = 0ull;
+    }
+    if (i < 3) d.overflow[i] = 0u;
+    if (i < DCD_HIST) d.hist[i] = 0u;
+    if (i < DCD_LEVELS) d.lvl_cnt[i] = 0u;
 }
 
 __global__ void k_dcd_snapshot(float4 *dst, const float4 *src, int n) {
@@ -2719,14 +2751,7 @@ int launch_dcd(xpbd_ctx *c, float4 *X, bool rebuild) {
     int r;
     auto g1 = [](int64_t n) { return dim3((unsigned)((n + 255) / 256)); };
     if (rebuild) {
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, d.rmax_bits, (int64_t)1);
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, d.overflow, (int64_t)3);
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, d.hist, (int64_t)DCD_HIST);
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, d.lvl_cnt, (int64_t)DCD_LEVELS);
-        hipLaunchKernelGGL(k_zero_u32, g1(T), dim3(256), 0, c->stream, d.bucket_cnt, T);
-        hipLaunchKernelGGL(k_zero_u32, g1(T), dim3(256), 0, c->stream, d.bucket_fill, T);
-        hipLaunchKernelGGL(k_zero_u32, g1(m + 1), dim3(256), 0, c->stream, d.pair_cnt, (int64_t)m + 1);
-        hipLaunchKernelGGL(k_zero_u32, g1(m), dim3(256), 0, c->stream, d.pair_fill, (int64_t)m);
+        hipLaunchKernelGGL(k_dcd_reset, g1(T), dim3(256), 0, c->stream, d, T);
         hipLaunchKernelGGL(k_dcd_snapshot, g1(c->nv), dim3(256), 0, c->stream, d.xprev, (const float4 *)X, c->nv);
         hipLaunchKernelGGL(k_dcd_tridata, g1(m), dim3(256), 0, c->stream, d, (const float4 *)X);
         hipLaunchKernelGGL(k_dcd_cellsize, dim3(1), dim3(64), 0, c->stream, d);
@@ -2734,17 +2759,11 @@ int launch_dcd(xpbd_ctx *c, float4 *X, bool rebuild) {
         HIPCHK(c, hipGetLastError());
         if ((r = launch_scan(c, d.bucket_cnt, d.bucket_off, T, d.bucket_off + T))) return r;
         hipLaunchKernelGGL(k_dcd_scatter, g1(m), dim3(256), 0, c->stream, d);
-        hipLaunchKernelGGL(k_dcd_pairs<0>, g1((int64_t)DCD_LANES * m), dim3(256), 0, c->stream, d);
-        HIPCHK(c, hipGetLastError());
-        if ((r = launch_scan(c, d.pair_cnt, d.pair_off, m, d.pair_off + m))) return r;
-        hipLaunchKernelGGL(k_dcd_pairs<1>, g1((int64_t)DCD_LANES * m), dim3(256), 0, c->stream, d);
-        hipLaunchKernelGGL(k_dcd_sortseg, g1((int64_t)DCD_LANES * m), dim3(256), 0, c->stream, d);
+        hipLaunchKernelGGL(k_dcd_search, g1((int64_t)DCD_LANES * m), dim3(256), 0, c->stream, d);
         HIPCHK(c, hipGetLastError());
     }
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, d.ncontact, (int64_t)1);
     const unsigned gp = 2048;
     hipLaunchKernelGGL(k_dcd_narrow, dim3(gp), dim3(256), 0, c->stream, d, (const float4 *)X);
-    hipLaunchKernelGGL(k_dcd_respond, dim3(gp), dim3(256), 0, c->stream, d, (const float4 *)X);
     hipLaunchKernelGGL(k_dcd_apply, g1(c->nv), dim3(256), 0, c->stream, d, X, c->nv);
     HIPCHK(c, hipGetLastError());
     return XPBD_OK;
@@ -2783,20 +2802,22 @@ int alloc_dcd(xpbd_ctx *c) {
     }
     if ((r = dalloc(c, &d.cr, m)) || (r = dalloc(c, &d.nrm, m)) || (r = dalloc(c, &d.cell, m)) ||
         (r = dalloc(c, &d.bucket_cnt, T)) || (r = dalloc(c, &d.bucket_off, T + 1)) ||
-        (r = dalloc(c, &d.bucket_fill, T)) || (r = dalloc(c, &d.sorted, m)) ||
+        (r = dalloc(c, &d.bucket_fill, T)) || (r = dalloc(c, &d.key, m)) || (r = dalloc(c, &d.tv, m)) ||
+        (r = dalloc(c, &d.npairs, 1)) || (r = dalloc(c, &d.pairs_seg, (size_t)d.max_pairs)) ||
         (r = dalloc(c, &d.pair_cnt, (size_t)m + 1)) || (r = dalloc(c, &d.pair_off, (size_t)m + 1)) ||
-        (r = dalloc(c, &d.pairs, (size_t)d.max_pairs)) || (r = dalloc(c, &d.contacts, (size_t)d.max_pairs)) ||
+        (r = dalloc(c, &d.pairs, (size_t)d.max_pairs)) ||
         (r = dalloc(c, &d.acc, 3 * (size_t)c->nv)) || (r = dalloc(c, &d.acc_n, (size_t)c->nv)) ||
         (r = dalloc(c, &d.xprev, (size_t)c->nv)) || (r = dalloc(c, &d.scal, 4)) ||
-        (r = dalloc(c, &d.rmax_bits, 1)) || (r = dalloc(c, &d.ncontact, 1)) || (r = dalloc(c, &d.overflow, 3)) ||
+        (r = dalloc(c, &d.rmax_bits, 1)) || (r = dalloc(c, &d.ncontact, 2)) || (r = dalloc(c, &d.overflow, 3)) ||
         (r = dalloc(c, &d.tlvl, (size_t)std::max(m, 1))) || (r = dalloc(c, &d.rec, (size_t)std::max(m, 1))) ||
         (r = dalloc(c, &d.hist, (size_t)DCD_HIST)) || (r = dalloc(c, &d.lvl_cnt, (size_t)DCD_LEVELS)) ||
         (r = dalloc(c, &d.pair_fill, (size_t)m + 1)))
         return r;
-    HIPCHK(c, hipMemset(d.overflow, 0, 2 * sizeof(uint32_t)));
+    HIPCHK(c, hipMemset(d.overflow, 0, 3 * sizeof(uint32_t)));
     HIPCHK(c, hipMemset(d.acc, 0, 3 * (size_t)c->nv * sizeof(unsigned long long)));
     HIPCHK(c, hipMemset(d.acc_n, 0, (size_t)c->nv * sizeof(uint32_t)));
-    HIPCHK(c, hipMemset(d.pair_off, 0, ((size_t)m + 1) * sizeof(uint32_t)));
+    HIPCHK(c, hipMemset(d.npairs, 0, sizeof(unsigned long long)));
+    HIPCHK(c, hipMemset(d.ncontact, 0, 2 * sizeof(uint32_t)));
     const int64_t need = std::max<int64_t>(T, (int64_t)m + 1);
     const int nb = (int)((need + SCAN_BLOCK - 1) / SCAN_BLOCK) + 1;
     if (nb > c->n_scan_bsum) {
@@ -3646,6 +3667,8 @@ int32_t xpbd_get_sdf_cells(xpbd_ctx *c, int32_t *cells) {
 int32_t xpbd_set_dcd(xpbd_ctx *c, int32_t ntris, const int32_t *tris, const int32_t *group,
                      const xpbd_dcd_params *p) {
     if (!c || c->nv == 0 || ntris < 0 || (ntris > 0 && (!tris || !p))) return fail(c, XPBD_EINVAL, "bad dcd");
+    if (ntris >= (1 << DCD_LVL_SHIFT)) return fail(c, XPBD_EINVAL, "too many dcd triangles (limit 2^27)");
+    if (p && p->max_pairs >= ((int64_t)1 << 31)) return fail(c, XPBD_EINVAL, "dcd max_pairs must be below 2^31");
     for (int64_t i = 0; i < 3 * (int64_t)ntris; ++i)
         if (tris[i] < 0 || tris[i] >= c->nv) return fail(c, XPBD_EINVAL, "dcd triangle vertex out of range");
     if (group)
@@ -3674,15 +3697,28 @@ int32_t xpbd_dcd_pairs(xpbd_ctx *c, int32_t *out, int64_t cap, int64_t *n_total)
     int r = check_ready(c);
     if (r) return r;
     if (!c->dcd_on) return fail(c, XPBD_EINVAL, "no DCD surface set");
-    uint32_t tot = 0;
-    HIPCHK(c, hipMemcpyAsync(&tot, c->dcd.pair_off + c->dcd.ntris, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    DcdDev &d = c->dcd;
+    unsigned long long tot = 0;
+    HIPCHK(c, hipMemcpyAsync(&tot, d.npairs, sizeof(tot), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (n_total) *n_total = tot;
-    const int64_t n = std::min<int64_t>(std::min<int64_t>(tot, c->dcd.max_pairs), cap);
-    if (out && n > 0) {
-        HIPCHK(c, hipMemcpyAsync(out, c->dcd.pairs, (size_t)n * sizeof(int2), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-    }
+    if (n_total) *n_total = (int64_t)tot;
+    const int64_t have = std::min<int64_t>((int64_t)tot, d.max_pairs);
+    const int64_t n = std::min<int64_t>(have, cap);
+    if (!out || n <= 0) return XPBD_OK;
+    // segments by first index, each sorted by the second
+    const int m = d.ntris;
+    auto g1 = [](int64_t k) { return dim3((unsigned)((k + 255) / 256)); };
+    hipLaunchKernelGGL(k_zero_u32, g1(m + 1), dim3(256), 0, c->stream, d.pair_cnt, (int64_t)m + 1);
+    hipLaunchKernelGGL(k_zero_u32, g1(m), dim3(256), 0, c->stream, d.pair_fill, (int64_t)m);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, d.overflow + 2, (int64_t)1);
+    hipLaunchKernelGGL(k_dcd_segcount, g1(have), dim3(256), 0, c->stream, d, have);
+    HIPCHK(c, hipGetLastError());
+    if ((r = launch_scan(c, d.pair_cnt, d.pair_off, m, d.pair_off + m))) return r;
+    hipLaunchKernelGGL(k_dcd_segscatter, g1(have), dim3(256), 0, c->stream, d, have);
+    hipLaunchKernelGGL(k_dcd_sortseg, g1((int64_t)DCD_LANES * m), dim3(256), 0, c->stream, d);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(out, d.pairs_seg, (size_t)n * sizeof(int2), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return XPBD_OK;
 }
 
@@ -3691,10 +3727,14 @@ int32_t xpbd_dcd_overflow(xpbd_ctx *c, int64_t *left_out, int64_t *truncated) {
     if (r) return r;
     if (!c->dcd_on) return fail(c, XPBD_EINVAL, "no DCD surface set");
     uint32_t o[3] = {0, 0, 0};
+    unsigned long long tot = 0;
     HIPCHK(c, hipMemcpyAsync(o, c->dcd.overflow, sizeof(o), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&tot, c->dcd.npairs, sizeof(tot), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int64_t dropped = std::max<int64_t>((int64_t)tot - c->dcd.max_pairs, 0);
     if (left_out) *left_out = o[0];
-    if (truncated) *truncated = (int64_t)o[1] + o[2];   // stopped searches + segments left unsorted
+    // stopped searches + pairs beyond the capacity + segments left unsorted
+    if (truncated) *truncated = (int64_t)o[1] + dropped + o[2];
     return XPBD_OK;
 }
 
@@ -3702,13 +3742,14 @@ int32_t xpbd_dcd_stats(xpbd_ctx *c, int64_t *n_pairs, int64_t *n_contacts, float
     int r = check_ready(c);
     if (r) return r;
     if (!c->dcd_on) return fail(c, XPBD_EINVAL, "no DCD surface set");
-    uint32_t tot = 0, nc = 0;
+    unsigned long long tot = 0;
+    uint32_t nc = 0;
     float sc[4];
-    HIPCHK(c, hipMemcpyAsync(&tot, c->dcd.pair_off + c->dcd.ntris, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(&nc, c->dcd.ncontact, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&tot, c->dcd.npairs, sizeof(tot), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&nc, c->dcd.ncontact + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(sc, c->dcd.scal, sizeof(sc), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (n_pairs) *n_pairs = tot;
+    if (n_pairs) *n_pairs = (int64_t)tot;
     if (n_contacts) *n_contacts = nc;
     if (cell) *cell = sc[0];
     return XPBD_OK;

@@ -21,12 +21,17 @@
 //    a non-finite centroid) takes no part and is counted; a search stops
 //    before the cell that would take it past DCD_MAX_VISIT candidates
 //    (counted).  Neither limit is reached on a valid mesh;
-//  * a pair (i, j), i < j, goes to i's segment whichever triangle found it
-//    (slots by atomic counters), and each segment is then sorted by j, giving
-//    the (i, j)-sorted pair list independent of hashing and scheduling;
-//  * contacts stay in their pair slot (no atomic compaction) and corrections
-//    are summed in 2^-32 m fixed point with integer atomics, so the result is
-//    the same on every run.
+//  * the bucket-ordered copy of each triangle's centroid/radius, key (index,
+//    level, cell) and vertices is what the search reads, so a candidate costs
+//    contiguous loads and no gather through the triangle index;
+//  * a workgroup collects the pairs (min, max) its searches emit in LDS and
+//    appends them with one atomic on the pair cursor, in no particular order;
+//  * the response is fused into the narrow phase: a valid contact is applied
+//    where it is found.  No order matters: corrections are summed in 2^-32 m
+//    fixed point with integer atomics, so the result is the same on every run.  The
+//    (i, j)-sorted list that the reference builds is made on request
+//    (segments by first index, each sorted by the second) for the pair-list
+//    query.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -34,17 +39,12 @@
 
 namespace xpbd {
 
-struct DcdContact {
-    int32_t cur[3], closest[3];
-    float C, C1, C2;
-    float n[3], b1[3], b2[3];
-    int32_t valid;
-};
-
 constexpr int DCD_LEVELS = 12;                 // cells c0 .. 2048 c0
 constexpr int DCD_HIST = 64;                   // radius histogram, 4 bins per octave
 constexpr uint32_t DCD_MAX_VISIT = 1u << 15;   // candidates one search may visit
 constexpr int DCD_SORT_MAX = 8192;             // longer segments stay unsorted (counted)
+constexpr int DCD_STAGE = 2048;                // pairs one workgroup stages in LDS
+constexpr int DCD_LVL_SHIFT = 27;              // key.x = triangle | level << 27
 
 struct DcdDev {
     int ntris;
@@ -62,18 +62,20 @@ struct DcdDev {
     uint32_t *bucket_cnt;       // T
     uint32_t *bucket_off;       // T + 1
     uint32_t *bucket_fill;      // T
-    int32_t *sorted;            // ntris
-    uint32_t *pair_cnt;         // ntris (+1 for the scan)
-    uint32_t *pair_off;         // ntris + 1
-    uint32_t *pair_fill;        // ntris
-    int2 *pairs;                // max_pairs
-    DcdContact *contacts;       // max_pairs
+    int4 *key;                  // bucket order: triangle | level << 27, cell x, y, z
+    int4 *tv;                   // bucket order: vertex ids, w = group
+    unsigned long long *npairs; // pair cursor (pairs found, may exceed max_pairs)
+    int2 *pairs;                // max_pairs, in append order
+    uint32_t *pair_cnt;         // ntris (+1 for the scan)      } the sorted list,
+    uint32_t *pair_off;         // ntris + 1                     } built on request
+    uint32_t *pair_fill;        // ntris                         }
+    int2 *pairs_seg;            // max_pairs                     }
     unsigned long long *acc;    // 3 per vertex, fixed point
     uint32_t *acc_n;            // per vertex
     float4 *xprev;              // snapshot at broadphase
     float *scal;                // [0] base cell c0, [1] 1 / c0, [2] max radius, [3] unused
     uint32_t *rmax_bits;
-    uint32_t *ncontact;         // valid contacts of the last narrow pass
+    uint32_t *ncontact;         // [0] contacts of the running pass, [1] of the last finished pass
     uint32_t *overflow;         // [0] triangles left out (too large / non-finite), [1] truncated searches,
                                 // [2] segments too long to sort
     float thickness, multiplier, mu_s, mu_d, offset_eps, normal_cull, cell_fixed;

@@ -258,10 +258,10 @@ class MultiGridSim:
 class MuscleSim:
     """MuscleSim program on the HIP substep: 1 Jacobi iteration per substep."""
 
-    def __init__(self, scene: Scene, num_substeps: int, device: int = 0):
+    def __init__(self, scene: Scene, num_substeps: int, device: int = 0, engine_cls=None):
         self.scene = scene
         self.num_substeps = int(num_substeps)
-        self.engine = XpbdEngine(scene, device=device)
+        self.engine = XpbdEngine(scene, device=device) if engine_cls is None else engine_cls(scene)
         self.n_verts = scene.nv
         self.dt = scene.params.dt
         self.step_cnt = 0
@@ -271,7 +271,7 @@ class MuscleSim:
     def from_procedural(cls, vertices, tets, fiber_dirs_per_tet, bone_targets=None, *,
                         constraint_configs, dts, num_substeps=1, device=0, density=1060.0,
                         veldamping=0.02, gravity=0.0, contraction_ratio=0.0,
-                        fiber_stiffness_scale=1.0, stopped=None, **_unused):
+                        fiber_stiffness_scale=1.0, stopped=None, engine_cls=None, **_unused):
         """musclesim.py:181-287 (gravity is the y component, as the reference engine)."""
         v = np.asarray(vertices, np.float32)
         t = np.asarray(tets, np.int32)
@@ -284,7 +284,7 @@ class MuscleSim:
         scene = Scene(pos0=v, tets=t, density=float(density), cons=cons, params=params,
                       v_fiber_dir=vf, tendonmask=np.zeros(t.shape[0], np.float32),
                       stopped=None if stopped is None else np.asarray(stopped, np.int32))
-        return cls(scene, num_substeps, device=device)
+        return cls(scene, num_substeps, device=device, engine_cls=engine_cls)
 
     def step(self):
         """integrate_phase, solve_phase, finalize_phase per substep (musclesim.py:107-142)."""
