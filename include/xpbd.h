@@ -193,8 +193,8 @@ typedef struct {
     float friction_static, friction_dynamic;
     float offset;             /* narrow-phase accept threshold (1e-6)                   */
     float normal_cull;        /* >= 1 disables the co-oriented self-pair cull           */
-    float cell_size;          /* base cell of the grid hierarchy; 0 = 1.01 (2 r90 + thickness), r90 the
-                                 90th percentile triangle radius at each rebuild */
+    float cell_size;          /* base cell of the grid hierarchy; 0 = 1.01 (2 r50 + thickness), r50 the
+                                 median triangle radius at each rebuild */
     int32_t allow_self;       /* movable-vs-movable pairs                               */
     int32_t iters;            /* DCD passes per substep (humanV2 "iters", default 2)    */
     int32_t rebuild_substeps; /* 1 = broadphase every substep, 0 = once per frame       */

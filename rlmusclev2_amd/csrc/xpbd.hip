@@ -1608,23 +1608,25 @@ __global__ __launch_bounds__(256) void k_dcd_tridata(DcdDev d, const float4 *X) 
     if (threadIdx.x < DCD_HIST && sh[threadIdx.x]) atomicAdd(&d.hist[threadIdx.x], sh[threadIdx.x]);
 }
 
-// base cell c0 = 1.01 (2 r90 + thickness), r90 the upper edge of the histogram
-// bin holding the 90th percentile radius (or the caller's fixed cell)
+// base cell c0 = 1.01 (2 r50 + thickness), r50 the upper edge of the histogram
+// bin holding the median radius (or the caller's fixed cell): triangles up to
+// the median size live at level 0, larger ones one or more levels up, which
+// keeps the candidates per search low when the sizes are spread out
 __global__ void k_dcd_cellsize(DcdDev d) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     float c0 = d.cell_fixed;
     if (!(c0 > 0.0f)) {
         uint32_t tot = 0;
         for (int k = 0; k < DCD_HIST; ++k) tot += d.hist[k];
-        const uint32_t want = (uint32_t)(((uint64_t)tot * 9u + 9u) / 10u);
+        const uint32_t want = (uint32_t)(((uint64_t)tot + 1u) / 2u);
         uint32_t acc = 0;
         int k = 0;
         for (; k < DCD_HIST - 1; ++k) {
             acc += d.hist[k];
             if (acc >= want) break;
         }
-        const float r90 = exp2f((float)(k - 79) * 0.25f);
-        c0 = 1.01f * fmaxf(2.0f * r90 + d.thickness, 1.0e-4f);
+        const float r50 = exp2f((float)(k - 79) * 0.25f);
+        c0 = 1.01f * fmaxf(2.0f * r50 + d.thickness, 1.0e-4f);
     }
     d.scal[0] = c0;
     d.scal[1] = 1.0f / c0;
@@ -1705,24 +1707,64 @@ __device__ __forceinline__ bool dcd_emit(const DcdDev &d, int j, const int4 ti, 
     return true;
 }
 
-// Broadphase search, one pass.  DCD_LANES consecutive lanes share one triangle
-// (taken in bucket order, so a wavefront searches a few neighbouring cells)
-// and split each cell's candidates; the visit limit is checked on whole cells,
-// identically in every lane.  Emitted pairs (min, max) are staged in LDS and
-// appended to the pair list with one cursor atomic per workgroup (a lane
-// whose pair finds the stage full appends it directly).
+// Broadphase search, one pass.  DCD_LANES consecutive lanes of a wavefront
+// share one triangle (taken in bucket order, so a wavefront searches a few
+// neighbouring cells).  Per grid level the group first reads the bucket ranges
+// of all cells of its search box at once (lane L takes cells L and L + 16, in
+// z, y, x order) and scans their sizes; the cells up to the visit limit are
+// kept (whole cells, as a cell-by-cell walk would keep them) and their
+// candidates form one flattened range that the lanes split evenly, four
+// candidates per lane per round with all loads issued first.  Emitted pairs
+// (min, max) are staged in LDS and appended to the pair list with one cursor
+// atomic per workgroup (a lane whose pair finds the stage full appends it
+// directly).
 constexpr int DCD_LANES = 16;
+constexpr int DCD_BOX = 27;        // cells of one search box (at most 3 per axis)
+constexpr int DCD_UNROLL = 2;
+
+// candidate j (bucket slot q, bucket-ordered record cj / key kj) of triangle i
+// searching cell (cx, cy, cz) of level b: sphere test, own-cell rule, index
+// rule, emit rules; an emitted pair goes to the workgroup's LDS stage
+__device__ __forceinline__ void dcd_try(const DcdDev &d, int i, int la, int b, int4 ti, float4 ni, float4 ci,
+                                        float4 cj, int4 kj, int cx, int cy, int cz, uint32_t q, int2 *stage,
+                                        uint32_t *s_n) {
+#pragma clang fp contract(off)
+    const V3 dd = V3{ci.x - cj.x, ci.y - cj.y, ci.z - cj.z};
+    if (sqrtf(dot(dd, dd)) > ci.w + cj.w + d.thickness) return;
+    const int j = kj.x & ((1 << DCD_LVL_SHIFT) - 1);
+    // visit j only from its own cell and level
+    if ((kj.x >> DCD_LVL_SHIFT) != b || kj.y != cx || kj.z != cy || kj.w != cz) return;
+    if (j == i || (b == la && j < i)) return;
+    if (!dcd_emit(d, j, ti, ni, ti.w, d.tv[q])) return;
+    const int2 pr = make_int2(min(i, j), max(i, j));
+    const uint32_t p = atomicAdd(s_n, 1u);
+    if (p < (uint32_t)DCD_STAGE) {
+        stage[p] = pr;
+    } else {
+        const unsigned long long o = atomicAdd(d.npairs, 1ull);
+        if (o < (unsigned long long)d.max_pairs) d.pairs[o] = pr;
+    }
+}
+
+__device__ __forceinline__ void dcd_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 __global__ __launch_bounds__(256) void k_dcd_search(DcdDev d) {
 #pragma clang fp contract(off)
+    constexpr int NG = 256 / DCD_LANES;
     __shared__ int2 stage[DCD_STAGE];
+    __shared__ int4 s_cell[NG][DCD_BOX];        // cell x, y, z, first slot - flattened start
+    __shared__ uint32_t s_incl[NG][DCD_BOX];    // inclusive end of the cell in the flattened range
     __shared__ uint32_t s_n;
     __shared__ unsigned long long s_base;
     if (threadIdx.x == 0) s_n = 0u;
     __syncthreads();
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = (int)(g & (DCD_LANES - 1));
-    const int64_t q0 = g / DCD_LANES;
+    const int lane = (int)(threadIdx.x & (DCD_LANES - 1));
+    const int grp = (int)(threadIdx.x / DCD_LANES);
+    const int64_t q0 = (int64_t)blockIdx.x * NG + grp;
     const bool live = q0 < (int64_t)d.bucket_off[(size_t)1 << d.tbits];
     bool stop = false;
     if (live) {
@@ -1741,36 +1783,102 @@ __global__ __launch_bounds__(256) void k_dcd_search(DcdDev d) {
             const int x0 = (int)floorf((ci.x - R) * ib), x1 = (int)floorf((ci.x + R) * ib);
             const int y0 = (int)floorf((ci.y - R) * ib), y1 = (int)floorf((ci.y + R) * ib);
             const int z0 = (int)floorf((ci.z - R) * ib), z1 = (int)floorf((ci.z + R) * ib);
-            for (int z = z0; z <= z1 && !stop; ++z)
-                for (int y = y0; y <= y1 && !stop; ++y)
-                    for (int x = x0; x <= x1 && !stop; ++x) {
-                        const uint32_t h = dcd_hash(x, y, z, b, d.tbits);
-                        const uint32_t s0 = d.bucket_off[h], e = d.bucket_off[h + 1];
-                        visits += e - s0;
-                        if (visits > DCD_MAX_VISIT) {
-                            stop = true;
-                            break;
-                        }
-                        for (uint32_t q = s0 + lane; q < e; q += DCD_LANES) {
-                            const float4 cj = d.rec[q];
-                            const int4 kj = d.key[q];
-                            const V3 dd = V3{ci.x - cj.x, ci.y - cj.y, ci.z - cj.z};
-                            if (sqrtf(dot(dd, dd)) > ci.w + cj.w + d.thickness) continue;
-                            const int j = kj.x & ((1 << DCD_LVL_SHIFT) - 1);
-                            // visit j only from its own cell and level
-                            if ((kj.x >> DCD_LVL_SHIFT) != b || kj.y != x || kj.z != y || kj.w != z) continue;
-                            if (j == i || (b == la && j < i)) continue;
-                            if (!dcd_emit(d, j, ti, ni, ti.w, d.tv[q])) continue;
-                            const int2 pr = make_int2(min(i, j), max(i, j));
-                            const uint32_t p = atomicAdd(&s_n, 1u);
-                            if (p < (uint32_t)DCD_STAGE) {
-                                stage[p] = pr;
-                            } else {
-                                const unsigned long long o = atomicAdd(d.npairs, 1ull);
-                                if (o < (unsigned long long)d.max_pairs) d.pairs[o] = pr;
+            if (x1 - x0 > 2 || y1 - y0 > 2 || z1 - z0 > 2) {
+                // a box wider than 3 cells (rounding at a level boundary): walk it cell by cell
+                for (int z = z0; z <= z1 && !stop; ++z)
+                    for (int y = y0; y <= y1 && !stop; ++y)
+                        for (int x = x0; x <= x1 && !stop; ++x) {
+                            const uint32_t h = dcd_hash(x, y, z, b, d.tbits);
+                            const uint32_t e0 = d.bucket_off[h], e = d.bucket_off[h + 1];
+                            visits += e - e0;
+                            if (visits > DCD_MAX_VISIT) {
+                                stop = true;
+                                break;
                             }
+                            for (uint32_t q = e0 + lane; q < e; q += DCD_LANES)
+                                dcd_try(d, i, la, b, ti, ni, ci, d.rec[q], d.key[q], x, y, z, q, stage, &s_n);
                         }
+                continue;
+            }
+            const int nx = x1 - x0 + 1, ny = y1 - y0 + 1, nz = z1 - z0 + 1;
+            const int ncell = nx * ny * nz;
+            // bucket ranges of cells lane and lane + 16
+            uint32_t cnt[2] = {0u, 0u}, s0[2] = {0u, 0u};
+            int cx[2] = {0, 0}, cy[2] = {0, 0}, cz[2] = {0, 0};
+#pragma unroll
+            for (int hh = 0; hh < 2; ++hh) {
+                const int c = lane + hh * DCD_LANES;
+                if (c < ncell) {
+                    cx[hh] = x0 + c % nx;
+                    cy[hh] = y0 + (c / nx) % ny;
+                    cz[hh] = z0 + c / (nx * ny);
+                    const uint32_t h = dcd_hash(cx[hh], cy[hh], cz[hh], b, d.tbits);
+                    s0[hh] = d.bucket_off[h];
+                    cnt[hh] = d.bucket_off[h + 1] - s0[hh];
+                }
+            }
+            // inclusive scan of the sizes over the 32 cells
+            uint32_t inc[2] = {cnt[0], cnt[1]};
+#pragma unroll
+            for (int o = 1; o < DCD_LANES; o <<= 1) {
+#pragma unroll
+                for (int hh = 0; hh < 2; ++hh) {
+                    const uint32_t v = (uint32_t)__shfl_up((int)inc[hh], o, DCD_LANES);
+                    if (lane >= o) inc[hh] += v;
+                }
+            }
+            inc[1] += (uint32_t)__shfl((int)inc[0], DCD_LANES - 1, DCD_LANES);
+            // keep the cells whose running visit count stays within the limit
+            uint32_t T = 0u;
+            bool over = false;
+#pragma unroll
+            for (int hh = 0; hh < 2; ++hh) {
+                const int c = lane + hh * DCD_LANES;
+                if (c < ncell) {
+                    if (visits + inc[hh] <= DCD_MAX_VISIT) {
+                        T = max(T, inc[hh]);
+                        s_cell[grp][c] = make_int4(cx[hh], cy[hh], cz[hh], (int)(s0[hh] - (inc[hh] - cnt[hh])));
+                        s_incl[grp][c] = inc[hh];
+                    } else {
+                        over = true;
                     }
+                }
+            }
+#pragma unroll
+            for (int o = 1; o < DCD_LANES; o <<= 1) {
+                T = max(T, (uint32_t)__shfl_xor((int)T, o, DCD_LANES));
+                over = over || (__shfl_xor((int)over, o, DCD_LANES) != 0);
+            }
+            dcd_wave_sync();
+            // the flattened candidates: f = lane, lane + 16, ...
+            int c = 0;
+            for (uint32_t f0 = (uint32_t)lane; f0 < T; f0 += DCD_UNROLL * DCD_LANES) {
+                float4 cj[DCD_UNROLL];
+                int4 kj[DCD_UNROLL];
+                int cc[DCD_UNROLL];
+#pragma unroll
+                for (int u = 0; u < DCD_UNROLL; ++u) {
+                    const uint32_t f = f0 + (uint32_t)(u * DCD_LANES);
+                    cc[u] = -1;
+                    if (f < T) {
+                        while (s_incl[grp][c] <= f) ++c;
+                        cc[u] = c;
+                        const uint32_t q = (uint32_t)s_cell[grp][c].w + f;
+                        cj[u] = d.rec[q];
+                        kj[u] = d.key[q];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < DCD_UNROLL; ++u) {
+                    if (cc[u] < 0) continue;
+                    const int4 cl = s_cell[grp][cc[u]];
+                    dcd_try(d, i, la, b, ti, ni, ci, cj[u], kj[u], cl.x, cl.y, cl.z,
+                            (uint32_t)cl.w + f0 + (uint32_t)(u * DCD_LANES), stage, &s_n);
+                }
+            }
+            dcd_wave_sync();
+            visits += T;
+            stop = over;
         }
         if (stop && lane == 0) atomicAdd(&d.overflow[1], 1u);
     }
@@ -1915,10 +2023,14 @@ struct DcdGeo {
     float d1, d2;
 };
 
-__device__ __forceinline__ bool dcd_straddle(const float4 *X, int4 A, int4 Bt, DcdGeo &g) {
-#pragma clang fp contract(off)
+__device__ __forceinline__ void dcd_load(const float4 *X, int4 A, int4 Bt, DcdGeo &g) {
     g.q00 = xyz(X[A.x]); g.q01 = xyz(X[A.y]); g.q02 = xyz(X[A.z]);
     g.q10 = xyz(X[Bt.x]); g.q11 = xyz(X[Bt.y]); g.q12 = xyz(X[Bt.z]);
+}
+
+// the straddle test on loaded vertices; fills in the unit normals and offsets
+__device__ __forceinline__ bool dcd_straddle(DcdGeo &g) {
+#pragma clang fp contract(off)
     const V3 n1f = cross(g.q01 - g.q00, g.q02 - g.q00), n2f = cross(g.q11 - g.q10, g.q12 - g.q10);
     const float l1 = sqrtf(dot(n1f, n1f)), l2 = sqrtf(dot(n2f, n2f));
     if (l1 == 0.0f || l2 == 0.0f) return false;
@@ -1943,103 +2055,108 @@ __device__ __forceinline__ void dcd_add(const DcdDev &d, int v, V3 c) {
     atomicAdd(&d.acc_n[v], 1u);
 }
 
-__global__ __launch_bounds__(256) void k_dcd_narrow(DcdDev d, const float4 *X) {
+// contact and response of one pair that passed the straddle test
+__device__ __forceinline__ void dcd_pair_contact(const DcdDev &d, const float4 *X, int4 A, int4 Bt, const DcdGeo &g) {
 #pragma clang fp contract(off)
+    const V3 q00 = g.q00, q01 = g.q01, q02 = g.q02, q10 = g.q10, q11 = g.q11, q12 = g.q12;
+    const V3 N1 = g.N1, N2 = g.N2;
+    const float d1 = g.d1, d2 = g.d2;
+    float C1 = 0.f, C2 = 0.f;
+    V3 b1{0, 0, 0}, b2{0, 0, 0}, b3{0, 0, 0}, b4{0, 0, 0};
+    for (int j = 0; j < 3; ++j) {
+        const V3 q0j = dcd_pick(q00, q01, q02, j), q1j = dcd_pick(q10, q11, q12, j);
+        const V3 P0j = q0j - dot(N2, q0j - q10) * N2;
+        if (dcd_inside(q10, q11, q12, P0j)) {
+            const float cand = dot(N2, q0j) + d2;
+            if (C1 < -cand) { C1 = -cand; b1 = dcd_bary(q00, q01, q02, q0j); b2 = dcd_bary(q10, q11, q12, q0j - cand * N2); }
+        }
+        const float dq = dot(N1, N2);
+        if (dq != 0.0f) {
+            const V3 Q0j = q1j + (dot(N1, q00 - q1j) / dq) * N2;
+            if (dcd_inside(q00, q01, q02, Q0j)) {
+                const float cand = dot(N2, Q0j) + d2;
+                if (C1 < -cand) { C1 = -cand; b1 = dcd_bary(q00, q01, q02, Q0j); b2 = dcd_bary(q10, q11, q12, Q0j - cand * N2); }
+            }
+        }
+        const V3 P1j = q1j - dot(N1, q1j - q00) * N1;
+        if (dcd_inside(q00, q01, q02, P1j)) {
+            const float cand = dot(N1, q1j) + d1;
+            if (C2 < -cand) { C2 = -cand; b3 = dcd_bary(q10, q11, q12, q1j); b4 = dcd_bary(q00, q01, q02, q1j - cand * N1); }
+        }
+        const float dq2 = dot(N2, N1);
+        if (dq2 != 0.0f) {
+            const V3 Q1j = q0j + (dot(N2, q10 - q0j) / dq2) * N1;
+            if (dcd_inside(q10, q11, q12, Q1j)) {
+                const float cand = dot(N1, Q1j) + d1;
+                if (C2 < -cand) { C2 = -cand; b3 = dcd_bary(q10, q11, q12, Q1j); b4 = dcd_bary(q00, q01, q02, Q1j - cand * N1); }
+            }
+        }
+        const int jn = (j + 1) % 3;
+        const V3 q0jn = dcd_pick(q00, q01, q02, jn), q1jn = dcd_pick(q10, q11, q12, jn);
+        const V3 P0jn = q0jn - dot(N2, q0jn - q10) * N2;
+        const V3 P1jn = q1jn - dot(N1, q1jn - q00) * N1;
+        for (int kk = 0; kk < 3; ++kk) {
+            const int kn = (kk + 1) % 3;
+            V3 hit;
+            if (dcd_line(P0j, P0jn, dcd_pick(q10, q11, q12, kk), dcd_pick(q10, q11, q12, kn), q0j, q0jn, hit)) {
+                const float cand = dot(N2, hit) + d2;
+                if (C1 < -cand) { C1 = -cand; b1 = dcd_bary(q00, q01, q02, hit); b2 = dcd_bary(q10, q11, q12, hit - cand * N2); }
+            }
+            if (dcd_line(P1j, P1jn, dcd_pick(q00, q01, q02, kk), dcd_pick(q00, q01, q02, kn), q1j, q1jn, hit)) {
+                const float cand = dot(N1, hit) + d1;
+                if (C2 < -cand) { C2 = -cand; b3 = dcd_bary(q10, q11, q12, hit); b4 = dcd_bary(q00, q01, q02, hit - cand * N1); }
+            }
+        }
+    }
+    if (C1 <= d.offset_eps || C2 <= d.offset_eps) return;
+    atomicAdd(&d.ncontact[0], 1u);
+    const bool first = C1 < C2;
+    const int4 cu4 = first ? A : Bt, cl4 = first ? Bt : A;
+    const int cu[3] = {cu4.x, cu4.y, cu4.z}, cl[3] = {cl4.x, cl4.y, cl4.z};
+    const V3 nrm = first ? N2 : N1;
+    const V3 cb1 = first ? b1 : b3, cb2 = first ? b2 : b4;
+    // respond_kernel (:714-799)
+    const float pen = first ? C1 : C2;
+    float C = pen * d.multiplier;
+    if (C == 0.0f) return;
+    float wcu[3], wcl[3];
+    for (int j = 0; j < 3; ++j) { wcu[j] = X[cu[j]].w; wcl[j] = X[cl[j]].w; }
+    const bool solid_cur = wcu[0] == 0.0f || wcu[1] == 0.0f || wcu[2] == 0.0f;
+    const bool solid_clo = wcl[0] == 0.0f || wcl[1] == 0.0f || wcl[2] == 0.0f;
+    if (!solid_cur && !solid_clo) C = C * 0.5f;
+    const V3 push_clo = solid_clo ? V3{0, 0, 0} : (-C) * nrm;
+    const V3 push_cur = solid_cur ? V3{0, 0, 0} : C * nrm;
+    V3 vcl[3], vcu[3];
+    for (int j = 0; j < 3; ++j) {
+        vcl[j] = (xyz(X[cl[j]]) + push_clo) - xyz(d.xprev[cl[j]]);
+        vcu[j] = (xyz(X[cu[j]]) + push_cur) - xyz(d.xprev[cu[j]]);
+    }
+    const V3 gn = nrm / sqrtf(dot(nrm, nrm));
+    const V3 vclo = (cb1.x * vcl[0] + cb1.y * vcl[1]) + cb1.z * vcl[2];
+    const V3 perp_clo = vclo - dot(vclo, gn) * gn;
+    const V3 vcur = (cb2.x * vcu[0] + cb2.y * vcu[1]) + cb2.z * vcu[2];
+    const V3 perp_cur = vcur - dot(vcur, gn) * gn;
+    V3 diff = perp_cur - perp_clo;
+    const float dn = sqrtf(dot(diff, diff));
+    if (dn > pen * d.mu_s && dn > 0.0f) diff = ((pen * d.mu_d) / dn) * diff;
+    const V3 diffV = 0.5f * diff;
+    const V3 diffT = diff - diffV;
+    const V3 dcur = push_cur - diffV, dclo = push_clo + diffT;
+    for (int j = 0; j < 3; ++j) {
+        if (wcu[j] > 0.0f) dcd_add(d, cu[j], dcur);
+        if (wcl[j] > 0.0f) dcd_add(d, cl[j], dclo);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dcd_narrow(DcdDev d, const float4 *X) {
     const int64_t total = (int64_t)min(*d.npairs, (unsigned long long)d.max_pairs);
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
          k += (int64_t)gridDim.x * blockDim.x) {
         const int2 pr = d.pairs[k];
         const int4 A = d.tris[pr.x], Bt = d.tris[pr.y];
         DcdGeo g;
-        if (!dcd_straddle(X, A, Bt, g)) continue;
-        const V3 q00 = g.q00, q01 = g.q01, q02 = g.q02, q10 = g.q10, q11 = g.q11, q12 = g.q12;
-        const V3 N1 = g.N1, N2 = g.N2;
-        const float d1 = g.d1, d2 = g.d2;
-        float C1 = 0.f, C2 = 0.f;
-        V3 b1{0, 0, 0}, b2{0, 0, 0}, b3{0, 0, 0}, b4{0, 0, 0};
-        for (int j = 0; j < 3; ++j) {
-            const V3 q0j = dcd_pick(q00, q01, q02, j), q1j = dcd_pick(q10, q11, q12, j);
-            const V3 P0j = q0j - dot(N2, q0j - q10) * N2;
-            if (dcd_inside(q10, q11, q12, P0j)) {
-                const float cand = dot(N2, q0j) + d2;
-                if (C1 < -cand) { C1 = -cand; b1 = dcd_bary(q00, q01, q02, q0j); b2 = dcd_bary(q10, q11, q12, q0j - cand * N2); }
-            }
-            const float dq = dot(N1, N2);
-            if (dq != 0.0f) {
-                const V3 Q0j = q1j + (dot(N1, q00 - q1j) / dq) * N2;
-                if (dcd_inside(q00, q01, q02, Q0j)) {
-                    const float cand = dot(N2, Q0j) + d2;
-                    if (C1 < -cand) { C1 = -cand; b1 = dcd_bary(q00, q01, q02, Q0j); b2 = dcd_bary(q10, q11, q12, Q0j - cand * N2); }
-                }
-            }
-            const V3 P1j = q1j - dot(N1, q1j - q00) * N1;
-            if (dcd_inside(q00, q01, q02, P1j)) {
-                const float cand = dot(N1, q1j) + d1;
-                if (C2 < -cand) { C2 = -cand; b3 = dcd_bary(q10, q11, q12, q1j); b4 = dcd_bary(q00, q01, q02, q1j - cand * N1); }
-            }
-            const float dq2 = dot(N2, N1);
-            if (dq2 != 0.0f) {
-                const V3 Q1j = q0j + (dot(N2, q10 - q0j) / dq2) * N1;
-                if (dcd_inside(q10, q11, q12, Q1j)) {
-                    const float cand = dot(N1, Q1j) + d1;
-                    if (C2 < -cand) { C2 = -cand; b3 = dcd_bary(q10, q11, q12, Q1j); b4 = dcd_bary(q00, q01, q02, Q1j - cand * N1); }
-                }
-            }
-            const int jn = (j + 1) % 3;
-            const V3 q0jn = dcd_pick(q00, q01, q02, jn), q1jn = dcd_pick(q10, q11, q12, jn);
-            const V3 P0jn = q0jn - dot(N2, q0jn - q10) * N2;
-            const V3 P1jn = q1jn - dot(N1, q1jn - q00) * N1;
-            for (int kk = 0; kk < 3; ++kk) {
-                const int kn = (kk + 1) % 3;
-                V3 hit;
-                if (dcd_line(P0j, P0jn, dcd_pick(q10, q11, q12, kk), dcd_pick(q10, q11, q12, kn), q0j, q0jn, hit)) {
-                    const float cand = dot(N2, hit) + d2;
-                    if (C1 < -cand) { C1 = -cand; b1 = dcd_bary(q00, q01, q02, hit); b2 = dcd_bary(q10, q11, q12, hit - cand * N2); }
-                }
-                if (dcd_line(P1j, P1jn, dcd_pick(q00, q01, q02, kk), dcd_pick(q00, q01, q02, kn), q1j, q1jn, hit)) {
-                    const float cand = dot(N1, hit) + d1;
-                    if (C2 < -cand) { C2 = -cand; b3 = dcd_bary(q10, q11, q12, hit); b4 = dcd_bary(q00, q01, q02, hit - cand * N1); }
-                }
-            }
-        }
-        if (C1 <= d.offset_eps || C2 <= d.offset_eps) continue;
-        atomicAdd(&d.ncontact[0], 1u);
-        const bool first = C1 < C2;
-        const int4 cu4 = first ? A : Bt, cl4 = first ? Bt : A;
-        const int cu[3] = {cu4.x, cu4.y, cu4.z}, cl[3] = {cl4.x, cl4.y, cl4.z};
-        const V3 nrm = first ? N2 : N1;
-        const V3 cb1 = first ? b1 : b3, cb2 = first ? b2 : b4;
-        // respond_kernel (:714-799)
-        const float pen = first ? C1 : C2;
-        float C = pen * d.multiplier;
-        if (C == 0.0f) continue;
-        float wcu[3], wcl[3];
-        for (int j = 0; j < 3; ++j) { wcu[j] = X[cu[j]].w; wcl[j] = X[cl[j]].w; }
-        const bool solid_cur = wcu[0] == 0.0f || wcu[1] == 0.0f || wcu[2] == 0.0f;
-        const bool solid_clo = wcl[0] == 0.0f || wcl[1] == 0.0f || wcl[2] == 0.0f;
-        if (!solid_cur && !solid_clo) C = C * 0.5f;
-        const V3 push_clo = solid_clo ? V3{0, 0, 0} : (-C) * nrm;
-        const V3 push_cur = solid_cur ? V3{0, 0, 0} : C * nrm;
-        V3 vcl[3], vcu[3];
-        for (int j = 0; j < 3; ++j) {
-            vcl[j] = (xyz(X[cl[j]]) + push_clo) - xyz(d.xprev[cl[j]]);
-            vcu[j] = (xyz(X[cu[j]]) + push_cur) - xyz(d.xprev[cu[j]]);
-        }
-        const V3 gn = nrm / sqrtf(dot(nrm, nrm));
-        const V3 vclo = (cb1.x * vcl[0] + cb1.y * vcl[1]) + cb1.z * vcl[2];
-        const V3 perp_clo = vclo - dot(vclo, gn) * gn;
-        const V3 vcur = (cb2.x * vcu[0] + cb2.y * vcu[1]) + cb2.z * vcu[2];
-        const V3 perp_cur = vcur - dot(vcur, gn) * gn;
-        V3 diff = perp_cur - perp_clo;
-        const float dn = sqrtf(dot(diff, diff));
-        if (dn > pen * d.mu_s && dn > 0.0f) diff = ((pen * d.mu_d) / dn) * diff;
-        const V3 diffV = 0.5f * diff;
-        const V3 diffT = diff - diffV;
-        const V3 dcur = push_cur - diffV, dclo = push_clo + diffT;
-        for (int j = 0; j < 3; ++j) {
-            if (wcu[j] > 0.0f) dcd_add(d, cu[j], dcur);
-            if (wcl[j] > 0.0f) dcd_add(d, cl[j], dclo);
-        }
+        dcd_load(X, A, Bt, g);
+        if (dcd_straddle(g)) dcd_pair_contact(d, X, A, Bt, g);
     }
 }
 
@@ -2762,8 +2879,7 @@ int launch_dcd(xpbd_ctx *c, float4 *X, bool rebuild) {
         hipLaunchKernelGGL(k_dcd_search, g1((int64_t)DCD_LANES * m), dim3(256), 0, c->stream, d);
         HIPCHK(c, hipGetLastError());
     }
-    const unsigned gp = 2048;
-    hipLaunchKernelGGL(k_dcd_narrow, dim3(gp), dim3(256), 0, c->stream, d, (const float4 *)X);
+    hipLaunchKernelGGL(k_dcd_narrow, dim3(4096), dim3(256), 0, c->stream, d, (const float4 *)X);
     hipLaunchKernelGGL(k_dcd_apply, g1(c->nv), dim3(256), 0, c->stream, d, X, c->nv);
     HIPCHK(c, hipGetLastError());
     return XPBD_OK;

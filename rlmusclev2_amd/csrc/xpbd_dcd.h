@@ -10,8 +10,7 @@
 //    table (key = cell and level).  Level b has cell c0 * 2^b; a triangle
 //    lives at the lowest level whose cell is at least 1.01 (2 r + thickness),
 //    so one stretched triangle no longer inflates everybody's cell.  c0 comes
-//    from the 90th percentile of the current radii (or the caller's
-//    cell_size).  A triangle searches its own level for partners of higher
+//    from the median of the current radii (or the caller's cell_size).  A triangle searches its own level for partners of higher
 //    index and every higher level for any partner, over the cells that the
 //    box of half-width 1.01 (r_i + r_max(level) + thickness) overlaps (at most
 //    3 per axis) and takes each candidate only from its own cell, so every
