@@ -31,7 +31,7 @@ class DcdParams:
     friction_dynamic: float = 0.4
     offset: float = 1.0e-6
     normal_cull: float = 1.0
-    cell_size: float = 0.0        # 0 = 2 x mean rest edge
+    cell_size: float = 0.0        # 0 = from the median triangle radius at each rebuild
     allow_self: bool = True
     iters: int = 2
     rebuild_substeps: bool = False

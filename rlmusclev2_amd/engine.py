@@ -102,6 +102,7 @@ class XpbdEngine:
         q = capi.XpbdDcdParams()
         for f, _ in capi.XpbdDcdParams._fields_:
             setattr(q, f, type(getattr(q, f))(getattr(p, f)))  # float or int field
+        self._dcd_max_pairs = int(p.max_pairs) if p.max_pairs > 0 else (1 << 22)
         k = self._keep
         k["dcd_tris"] = i32(spec.tris, (-1, 3))
         k["dcd_group"] = None if spec.group is None else i32(spec.group)
@@ -112,10 +113,13 @@ class XpbdEngine:
         self.ctx.call("xpbd_dcd_solve", int(bool(rebuild)))
 
     def dcd_pairs(self) -> np.ndarray:
+        """The (i<j)-sorted pair list of the last broadphase (at most max_pairs
+        of them; the stats give the number found)."""
         n = np.zeros(1, np.int64)
         self.ctx.call("xpbd_dcd_pairs", None, 0, ptr(n))
-        out = np.zeros((int(n[0]), 2), np.int32)
-        self.ctx.call("xpbd_dcd_pairs", ptr(out), int(n[0]), ptr(n))
+        m = min(int(n[0]), self._dcd_max_pairs)
+        out = np.zeros((m, 2), np.int32)
+        self.ctx.call("xpbd_dcd_pairs", ptr(out), m, ptr(n))
         return out
 
     def dcd_stats(self) -> dict:
