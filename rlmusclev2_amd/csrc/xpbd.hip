@@ -1013,11 +1013,8 @@ __global__ __launch_bounds__(256) void k_warm_start(IterArgs a, WarmNet net, con
 // per-vertex kernels
 // ---------------------------------------------------------------------------
 // integrate_kernel (engine.py:431-464) + apply_pintotarget (mg.py:1337)
-__global__ void k_integrate(int nv, float4 *X, float4 *PP, float4 *VEL, const float4 *FRC,
-                            const uint32_t *flags, const float4 *pin, SimParams sp) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nv) return;
-    float4 x = X[i], pp = PP[i], v = VEL[i];
+__device__ __forceinline__ void integrate_vertex(int i, float4 &x, float4 &pp, float4 &v, const float4 *FRC,
+                                                 const uint32_t *flags, const float4 *pin, const SimParams &sp) {
     if (flags[i] & 1u) {
         pp.x = x.x; pp.y = x.y; pp.z = x.z;
         v.x = v.y = v.z = 0.f;
@@ -1042,6 +1039,14 @@ __global__ void k_integrate(int nv, float4 *X, float4 *PP, float4 *VEL, const fl
         const float4 p = pin[i];
         if (p.w != 0.0f) { x.x = p.x; x.y = p.y; x.z = p.z; }
     }
+}
+
+__global__ void k_integrate(int nv, float4 *X, float4 *PP, float4 *VEL, const float4 *FRC,
+                            const uint32_t *flags, const float4 *pin, SimParams sp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    float4 x = X[i], pp = PP[i], v = VEL[i];
+    integrate_vertex(i, x, pp, v, FRC, flags, pin, sp);
     X[i] = x;
     PP[i] = pp;
     VEL[i] = v;
@@ -1096,13 +1101,19 @@ __device__ SdfHit sdf_eval(const float *grid, int res, const float *bn, V3 x) {
 }
 
 // post phase: bone SDF push (collision_solvers, mg.py:1370-1371) then
-// update_velocities_kernel (engine.py:582-605); result lands in X (buffer A)
-__global__ void k_post(int nv, const float4 *Xc, float4 *X, const float4 *PP, float4 *VEL,
-                       SdfParams sd, int32_t *cells, SimParams sp, int do_sdf, int do_vel) {
+// update_velocities_kernel (engine.py:582-605); result lands in X (buffer A).
+// INTEG = 1 goes on with the next substep's integrate of the same vertex
+// (nothing runs between the two inside a frame), one launch instead of two.
+template <int INTEG>
+__global__ void k_post(int nv, const float4 *Xc, float4 *X, float4 *PP, float4 *VEL,
+                       SdfParams sd, int32_t *cells, SimParams sp, int do_sdf, int do_vel, const float4 *FRC,
+                       const uint32_t *flags, const float4 *pin) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nv) return;
     float4 x = Xc[i];
     const float4 pp = PP[i];
+    float4 v_new = make_float4(0.f, 0.f, 0.f, 0.f);
+    bool vel_set = false;
     if (do_sdf) {
 #pragma clang fp contract(off)
         int bb = -1, bc = -1;
@@ -1145,7 +1156,17 @@ __global__ void k_post(int nv, const float4 *Xc, float4 *X, const float4 *PP, fl
             }
         }
         v.x = vv.x; v.y = vv.y; v.z = vv.z;
+        v_new = v;
+        vel_set = true;
+    }
+    if (INTEG) {
+        float4 v = vel_set ? v_new : VEL[i];
+        float4 p2 = pp;
+        integrate_vertex(i, x, p2, v, FRC, flags, pin, sp);
+        PP[i] = p2;
         VEL[i] = v;
+    } else if (vel_set) {
+        VEL[i] = v_new;
     }
     X[i] = x;
 }
@@ -2824,7 +2845,8 @@ int launch_integrate(xpbd_ctx *c) {
     return XPBD_OK;
 }
 
-int launch_post(xpbd_ctx *c, const float4 *xc, int do_sdf, int do_vel) {
+// integ: also the next substep's integrate (k_post<1>)
+int launch_post(xpbd_ctx *c, const float4 *xc, int do_sdf, int do_vel, bool integ = false) {
     SdfParams sd{};
     sd.nbones = c->nbones;
     sd.res = c->res;
@@ -2835,9 +2857,16 @@ int launch_post(xpbd_ctx *c, const float4 *xc, int do_sdf, int do_vel) {
     sd.vgroup = c->d_vgroup;
     sd.goff = c->d_goff;
     const int sdf_on = (do_sdf && c->nbones > 0 && c->d_grids) ? 1 : 0;
-    hipLaunchKernelGGL(k_post, vgrid(c->nv), dim3(256), 0, c->stream, c->nv, xc, c->X[0],
-                       (const float4 *)c->PPv, c->VEL, sd, sdf_on ? c->d_cells : nullptr, c->sp, sdf_on,
-                       do_vel);
+    const float4 *frc = c->has_force ? (const float4 *)c->FRC : nullptr;
+    const float4 *pin = c->pins_on ? (const float4 *)c->PIN : nullptr;
+    if (integ)
+        hipLaunchKernelGGL(k_post<1>, vgrid(c->nv), dim3(256), 0, c->stream, c->nv, xc, c->X[0], c->PPv, c->VEL, sd,
+                           sdf_on ? c->d_cells : nullptr, c->sp, sdf_on, do_vel, frc, (const uint32_t *)c->FLAGS,
+                           pin);
+    else
+        hipLaunchKernelGGL(k_post<0>, vgrid(c->nv), dim3(256), 0, c->stream, c->nv, xc, c->X[0], c->PPv, c->VEL, sd,
+                           sdf_on ? c->d_cells : nullptr, c->sp, sdf_on, do_vel, frc, (const uint32_t *)c->FLAGS,
+                           pin);
     HIPCHK(c, hipGetLastError());
     return XPBD_OK;
 }
@@ -3015,8 +3044,11 @@ int launch_vcycle(xpbd_ctx *c) {
 }
 
 // one substep, positions start and end in X[0]; sub = index within the frame
-int enqueue_substep(xpbd_ctx *c, int iters, bool dcd_rebuild, int sub = 0) {
-    int r = launch_integrate(c);
+// integrated: the previous substep's post already ran this substep's
+// integrate; fuse_next: this substep's post runs the next one's integrate
+int enqueue_substep(xpbd_ctx *c, int iters, bool dcd_rebuild, int sub = 0, bool integrated = false,
+                    bool fuse_next = false) {
+    int r = integrated ? XPBD_OK : launch_integrate(c);
     if (r) return r;
     if (c->coarse && c->mg_iters > 0 && (r = launch_vcycle(c))) return r;
     // mode 4: reference substeps (the first of a frame, or every ref_every-th)
@@ -3040,7 +3072,7 @@ int enqueue_substep(xpbd_ctx *c, int iters, bool dcd_rebuild, int sub = 0) {
     if (c->dcd_on)
         for (int k = 0; k < c->dcd_iters; ++k)
             if ((r = launch_dcd(c, c->X[iters & 1], k == 0 && (dcd_rebuild || c->dcd_rebuild_substeps)))) return r;
-    return launch_post(c, c->X[iters & 1], 1, 1);
+    return launch_post(c, c->X[iters & 1], 1, 1, fuse_next);
 }
 
 int check_ready(xpbd_ctx *c) {
@@ -3692,7 +3724,7 @@ int32_t xpbd_step_frame(xpbd_ctx *c, int32_t nsub, int32_t iters, int32_t use_gr
     c->dcd_fresh = false;
     if (!use_graph || c->timing) {
         for (int s = 0; s < nsub; ++s)
-            if ((r = enqueue_substep(c, iters, s == 0, s))) return r;
+            if ((r = enqueue_substep(c, iters, s == 0, s, s > 0, s + 1 < nsub))) return r;
         return XPBD_OK;
     }
     xpbd_ctx::GraphSlot *hit = nullptr, *victim = &c->gslot[0];
@@ -3705,7 +3737,7 @@ int32_t xpbd_step_frame(xpbd_ctx *c, int32_t nsub, int32_t iters, int32_t use_gr
     if (!hit) {
         drop_graph_slot(*victim);
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        for (int s = 0; s < nsub && r == XPBD_OK; ++s) r = enqueue_substep(c, iters, s == 0, s);
+        for (int s = 0; s < nsub && r == XPBD_OK; ++s) r = enqueue_substep(c, iters, s == 0, s, s > 0, s + 1 < nsub);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (r) { if (g) (void)hipGraphDestroy(g); return r; }
