@@ -1758,11 +1758,26 @@ __device__ __forceinline__ void dcd_try(const DcdDev &d, int i, int la, int b, i
     if (j == i || (b == la && j < i)) return;
     if (!dcd_emit(d, j, ti, ni, ti.w, d.tv[q])) return;
     const int2 pr = make_int2(min(i, j), max(i, j));
-    const uint32_t p = atomicAdd(s_n, 1u);
+    // the lanes of the wavefront that emit here take consecutive stage places
+    // with one LDS atomic; lanes past the stage's end append with one cursor
+    // atomic for all of them
+    const int lane = (int)__lane_id();
+    const uint64_t m = __ballot(1);
+    const int lead = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == lead) base = atomicAdd(s_n, (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, lead);
+    const uint32_t p = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
     if (p < (uint32_t)DCD_STAGE) {
         stage[p] = pr;
     } else {
-        const unsigned long long o = atomicAdd(d.npairs, 1ull);
+        const uint64_t mo = __ballot(1);
+        const int lo = __ffsll((unsigned long long)mo) - 1;
+        unsigned long long g = 0ull;
+        if (lane == lo) g = atomicAdd(d.npairs, (unsigned long long)__popcll(mo));
+        const uint32_t glo = (uint32_t)__shfl((int)(uint32_t)g, lo), ghi = (uint32_t)__shfl((int)(uint32_t)(g >> 32), lo);
+        const unsigned long long o = (((unsigned long long)ghi << 32) | glo) +
+                                     __builtin_amdgcn_mbcnt_hi((uint32_t)(mo >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mo, 0u));
         if (o < (unsigned long long)d.max_pairs) d.pairs[o] = pr;
     }
 }

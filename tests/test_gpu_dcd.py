@@ -150,3 +150,21 @@ def test_pairs_beyond_capacity_are_dropped_and_counted():
     for _ in range(3):                                     # passes over the kept pairs
         eng.dcd_solve(False)
     assert np.isfinite(eng.positions_np()).all()
+
+
+def test_crowded_surface_stops_at_the_visit_limit():
+    """34,000 triangles packed in 1.5 mm: every search would visit them all
+    (more than the 32,768-candidate limit), so the searches stop, the pairs
+    beyond max_pairs are dropped, both are counted, and a pass over the kept
+    pairs stays finite"""
+    cap = 1 << 20
+    sc, tris, x = _soup(n_tris=34000, box=0.0015, edge=0.0005, seed=9, max_pairs=cap)
+    eng = XpbdEngine(sc)
+    eng.dcd_solve(True)
+    st = eng.dcd_stats()
+    print(st)
+    assert st["left_out_tris"] == 0
+    assert st["n_pairs"] > cap
+    assert st["truncated_tris"] > st["n_pairs"] - cap      # stopped searches on top of the dropped pairs
+    eng.dcd_solve(False)
+    assert np.isfinite(eng.positions_np()).all()
