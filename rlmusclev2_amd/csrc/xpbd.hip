@@ -814,16 +814,32 @@ __global__ __launch_bounds__(256) void k_block_shared(IterArgs a, BlockArgs b, c
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= ns) return;
     const int v = b.svid[i];
+    const int s0 = b.soff[i], e = b.soff[i + 1];
+    // a shared vertex has a few partials: the first four are loaded together,
+    // with the vertex's own position, and added in slot order
+    const int n = e - s0;
+    float4 q[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) q[k] = k < n ? b.part[s0 + k] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 x = xin[v];
+    const float4 pp = DAMP ? pprev[v] : make_float4(0, 0, 0, 0);
     float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
-    const int e = b.soff[i + 1];
-    for (int s = b.soff[i]; s < e; ++s) {
-        const float4 q = b.part[s];
-        ax += q.x;
-        ay += q.y;
-        az += q.z;
-        aw += q.w;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (k < n) {
+            ax += q[k].x;
+            ay += q[k].y;
+            az += q[k].z;
+            aw += q[k].w;
+        }
+    for (int s = s0 + 4; s < e; ++s) {
+        const float4 r = b.part[s];
+        ax += r.x;
+        ay += r.y;
+        az += r.z;
+        aw += r.w;
     }
-    finish_vertex<DAMP, MODE>(a, v, xin[v], DAMP ? pprev[v] : make_float4(0, 0, 0, 0), ax, ay, az, aw, xout);
+    finish_vertex<DAMP, MODE>(a, v, x, pp, ax, ay, az, aw, xout);
 }
 
 
