@@ -2200,15 +2200,43 @@ __device__ __forceinline__ void dcd_pair_contact(const DcdDev &d, const float4 *
     }
 }
 
+// A workgroup takes DCD_NR rounds of 256 pairs, queues the pairs that pass
+// the straddle test in LDS (a few per thousand) and then runs the contact
+// stage on the queue with consecutive lanes, so one wavefront, not every
+// wavefront that holds a passing pair, goes through the long contact code.
+constexpr int DCD_NR = 4;
+constexpr int DCD_QCAP = DCD_NR * 256;   // a round's pairs all fit: the queue never overflows
+
 __global__ __launch_bounds__(256) void k_dcd_narrow(DcdDev d, const float4 *X) {
+    // launched with 256 threads
+    __shared__ int64_t s_q[DCD_QCAP];
+    __shared__ uint32_t s_nq;
     const int64_t total = (int64_t)min(*d.npairs, (unsigned long long)d.max_pairs);
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
-         k += (int64_t)gridDim.x * blockDim.x) {
-        const int2 pr = d.pairs[k];
-        const int4 A = d.tris[pr.x], Bt = d.tris[pr.y];
-        DcdGeo g;
-        dcd_load(X, A, Bt, g);
-        if (dcd_straddle(g)) dcd_pair_contact(d, X, A, Bt, g);
+    const int64_t span = (int64_t)DCD_NR * 256;
+    for (int64_t base = (int64_t)blockIdx.x * span; base < total; base += (int64_t)gridDim.x * span) {
+        if (threadIdx.x == 0) s_nq = 0u;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < DCD_NR; ++r) {
+            const int64_t k = base + (int64_t)r * 256 + threadIdx.x;
+            if (k >= total) continue;
+            const int2 pr = d.pairs[k];
+            const int4 A = d.tris[pr.x], Bt = d.tris[pr.y];
+            DcdGeo g;
+            dcd_load(X, A, Bt, g);
+            if (!dcd_straddle(g)) continue;
+            s_q[atomicAdd(&s_nq, 1u)] = k;
+        }
+        __syncthreads();
+        const uint32_t nq = s_nq;
+        for (uint32_t t = threadIdx.x; t < nq; t += 256) {
+            const int2 pr = d.pairs[s_q[t]];
+            const int4 A = d.tris[pr.x], Bt = d.tris[pr.y];
+            DcdGeo g;
+            dcd_load(X, A, Bt, g);
+            if (dcd_straddle(g)) dcd_pair_contact(d, X, A, Bt, g);   // the same test: always passes
+        }
+        __syncthreads();
     }
 }
 
@@ -2939,7 +2967,7 @@ int launch_dcd(xpbd_ctx *c, float4 *X, bool rebuild) {
         hipLaunchKernelGGL(k_dcd_search, g1((int64_t)DCD_LANES * m), dim3(256), 0, c->stream, d);
         HIPCHK(c, hipGetLastError());
     }
-    hipLaunchKernelGGL(k_dcd_narrow, dim3(4096), dim3(256), 0, c->stream, d, (const float4 *)X);
+    hipLaunchKernelGGL(k_dcd_narrow, dim3(1024), dim3(256), 0, c->stream, d, (const float4 *)X);
     hipLaunchKernelGGL(k_dcd_apply, g1(c->nv), dim3(256), 0, c->stream, d, X, c->nv);
     HIPCHK(c, hipGetLastError());
     return XPBD_OK;
