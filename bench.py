@@ -190,7 +190,7 @@ def warm_start_leg(cfg, w, tuning, out):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only "
+        print(json.dumps({"error": "the solver runs on one GPU; multi-GPU sharding is a design note only "
                                    "(instances or mesh partitions per GPU), --gpus must be 1"}))
         return 2
     import numpy as np

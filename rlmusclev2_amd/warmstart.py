@@ -20,7 +20,7 @@ of ``conv_iters`` iterations (10, the plain iteration count) accumulates from
 each substep start: applied as the first-iteration impulse, those multipliers
 reach the plain 10-iteration residual in about 4 iterations (measured with the
 exact multipliers of the same substep); longer solves accumulate multipliers
-whose one-shot impulse overshoots.  Plain XPBD is both the judge (its
+whose one-shot impulse overshoots.  Plain XPBD is both the yardstick (its
 10-iteration residual is the target) and the per-frame fallback.
 """
 from __future__ import annotations
