@@ -1247,46 +1247,69 @@ struct MgCoarse {
     int max_nodes, max_tets;
 };
 
-__global__ __launch_bounds__(256) void k_mg_fit(MgArgs m, const float4 *__restrict__ xf) {
-    __shared__ float red[12][256];
-    const int b = blockIdx.x, tid = threadIdx.x;
-    float acc[12];
-#pragma unroll
-    for (int k = 0; k < 12; ++k) acc[k] = 0.f;
-    for (int e = m.fit_off[b] + tid; e < m.fit_off[b + 1]; e += 256) {
-        const float4 x = xf[m.fit_fine[e]];
-        const float4 r = m.fit_rest[e];
-        const float h[4] = {r.x, r.y, r.z, 1.0f};
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            acc[k] += x.x * h[k];
-            acc[4 + k] += x.y * h[k];
-            acc[8 + k] += x.z * h[k];
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < 12; ++k) red[k][tid] = acc[k];
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (tid < s)
-#pragma unroll
-            for (int k = 0; k < 12; ++k) red[k][tid] += red[k][tid + s];
-        __syncthreads();
-    }
-    if (tid < 12) {
-        const int row = tid / 4, col = tid % 4;
-        const float *S = m.fit_sinv + 16 * b;
-        float v = 0.f;
-        for (int k = 0; k < 4; ++k) v += red[4 * row + k][0] * S[4 * k + col];
-        m.fitT[12 * b + tid] = v;
-    }
-}
-
 // sum over a wavefront in a fixed tree order
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
+}
+
+// per-muscle affine fit: 1,024 threads per muscle, two vertices per thread and
+// round with both gathers issued first; wavefront sums, then the 16 wavefront
+// partials added in order (a fixed reduction order, so the fit is the same on
+// every run)
+constexpr int MGF_T = 1024;
+__global__ __launch_bounds__(MGF_T) void k_mg_fit(MgArgs m, const float4 *__restrict__ xf) {
+    __shared__ float red[12][MGF_T / 64];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    float acc[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[k] = 0.f;
+    const int e0 = m.fit_off[b], e1 = m.fit_off[b + 1];
+    for (int e = e0 + tid; e < e1; e += 2 * MGF_T) {
+        const bool two = e + MGF_T < e1;
+        const float4 xa = xf[m.fit_fine[e]];
+        const float4 ra = m.fit_rest[e];
+        const float4 xb = two ? xf[m.fit_fine[e + MGF_T]] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 rb = two ? m.fit_rest[e + MGF_T] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float ha[4] = {ra.x, ra.y, ra.z, 1.0f};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            acc[k] += xa.x * ha[k];
+            acc[4 + k] += xa.y * ha[k];
+            acc[8 + k] += xa.z * ha[k];
+        }
+        if (two) {
+            const float hb[4] = {rb.x, rb.y, rb.z, 1.0f};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                acc[k] += xb.x * hb[k];
+                acc[4 + k] += xb.y * hb[k];
+                acc[8 + k] += xb.z * hb[k];
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[k] = wave_sum(acc[k]);
+    if ((tid & 63) == 0)
+#pragma unroll
+        for (int k = 0; k < 12; ++k) red[k][tid >> 6] = acc[k];
+    __syncthreads();
+    if (tid < 12) {
+        // this entry's row of sums over the 16 wavefronts, in order
+        float sk[4];
+        const int row = tid / 4, col = tid % 4;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float v = 0.f;
+            for (int w = 0; w < MGF_T / 64; ++w) v += red[4 * row + k][w];
+            sk[k] = v;
+        }
+        const float *S = m.fit_sinv + 16 * b;
+        float v = 0.f;
+        for (int k = 0; k < 4; ++k) v += sk[k] * S[4 * k + col];
+        m.fitT[12 * b + tid] = v;
+    }
 }
 
 // one wavefront per coarse node: the lanes split its P^T entries
@@ -3085,7 +3108,7 @@ int lam_for_iteration(xpbd_ctx *c) {
 // -> delta prolongation -> pins, on the fine context's stream
 int launch_vcycle(xpbd_ctx *c) {
     const int nc = c->coarse->nv;
-    hipLaunchKernelGGL(k_mg_fit, dim3((unsigned)c->mg_nblk), dim3(256), 0, c->stream, c->mg, (const float4 *)c->X[0]);
+    hipLaunchKernelGGL(k_mg_fit, dim3((unsigned)c->mg_nblk), dim3(MGF_T), 0, c->stream, c->mg, (const float4 *)c->X[0]);
     const dim3 gw((unsigned)((64 * (int64_t)nc + 255) / 256));   // one wavefront per coarse node
     hipLaunchKernelGGL(k_mg_rhs, gw, dim3(256), 0, c->stream, c->mg, nc, (const float4 *)c->X[0]);
     hipLaunchKernelGGL(k_mg_solve, gw, dim3(256), 0, c->stream, c->mg, nc, c->MG_QPREV);
