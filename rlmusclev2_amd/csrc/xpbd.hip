@@ -1430,39 +1430,51 @@ __global__ __launch_bounds__(MGC_T) void k_mg_coarse(MgArgs m, MgCoarse mc, int 
 #pragma unroll
         for (int q = 0; q < 9; ++q) B1[q] = mc.tB[9 * (int64_t)t + q];
     }
-    const bool hasn = tid < nn;
+    if (tid < nn) {
+        const float4 q = m.q[n0 + tid];
+        sq[tid] = make_float4(q.x, q.y, q.z, mc.node_w[n0 + tid]);
+    }
+    // a node's corner run is summed by two adjacent lanes (even and odd
+    // places), which then combine, when the cage has at most MGC_T / 2 nodes
+    const int lpn = (2 * nn <= MGC_T) ? 2 : 1;
+    const int node = lpn == 2 ? (tid >> 1) : tid, part = lpn == 2 ? (tid & 1) : 0;
+    const bool hasn = node < nn;
     int s0 = 0, ns = 0;
     if (hasn) {
         const int base = mc.slot_off[n0];
-        s0 = mc.slot_off[n0 + tid] - base;
-        ns = mc.slot_off[n0 + tid + 1] - mc.slot_off[n0 + tid];
-        const float4 q = m.q[n0 + tid];
-        sq[tid] = make_float4(q.x, q.y, q.z, mc.node_w[n0 + tid]);
+        s0 = mc.slot_off[n0 + node] - base;
+        ns = mc.slot_off[n0 + node + 1] - mc.slot_off[n0 + node];
     }
     for (int it = 0; it < iters; ++it) {
         __syncthreads();
         if (has0) mg_coarse_tet(sq, sc, li0, cp0, B0, at0);
         if (has1) mg_coarse_tet(sq, sc, li1, cp1, B1, at1);
         __syncthreads();
-        if (hasn) {
-            float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
-            for (int j = 0; j < ns; ++j) {
+        float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
+        if (hasn)
+            for (int j = part; j < ns; j += lpn) {
                 const float4 c = sc[s0 + j];
                 ax += c.x;
                 ay += c.y;
                 az += c.z;
                 aw += c.w;
             }
-            if (aw > 1e-9f) {   // apply_dP_kernel, engine.py:689-698
-                float4 x = sq[tid];
-                x.x = x.x + ax / aw;
-                x.y = x.y + ay / aw;
-                x.z = x.z + az / aw;
-                sq[tid] = x;
-            }
+        if (lpn == 2) {   // the node's two lanes are adjacent lanes of one wavefront
+            ax += __shfl_xor(ax, 1);
+            ay += __shfl_xor(ay, 1);
+            az += __shfl_xor(az, 1);
+            aw += __shfl_xor(aw, 1);
+        }
+        if (hasn && part == 0 && aw > 1e-9f) {   // apply_dP_kernel, engine.py:689-698
+            float4 x = sq[node];
+            x.x = x.x + ax / aw;
+            x.y = x.y + ay / aw;
+            x.z = x.z + az / aw;
+            sq[node] = x;
         }
     }
-    if (hasn) {
+    __syncthreads();
+    if (tid < nn) {
         const float4 x = sq[tid];
         m.q[n0 + tid] = make_float4(x.x, x.y, x.z, 0.f);
         float4 *o = xc + m.dev_of[n0 + tid];   // the coarse context's own state, for readback
