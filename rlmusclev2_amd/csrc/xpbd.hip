@@ -3118,7 +3118,7 @@ int lam_for_iteration(xpbd_ctx *c) {
 
 // restrict -> coarse Jacobi ARAP sweeps (coarse pins held as stopped vertices)
 // -> delta prolongation -> pins, on the fine context's stream
-int launch_vcycle(xpbd_ctx *c) {
+int launch_vcycle(xpbd_ctx *c, bool pins = true) {
     const int nc = c->coarse->nv;
     hipLaunchKernelGGL(k_mg_fit, dim3((unsigned)c->mg_nblk), dim3(MGF_T), 0, c->stream, c->mg, (const float4 *)c->X[0]);
     const dim3 gw((unsigned)((64 * (int64_t)nc + 255) / 256));   // one wavefront per coarse node
@@ -3130,7 +3130,10 @@ int launch_vcycle(xpbd_ctx *c) {
                        (const float4 *)c->d_mg_pw, (const float4 *)c->mg.q, (const float4 *)c->MG_QPREV, c->X[0],
                        c->mg_kinematic ? c->PPv : nullptr, c->pins_on ? (const float4 *)c->PIN : nullptr);
     HIPCHK(c, hipGetLastError());
-    if (c->pins_on) {
+    // pinned vertices: inside a substep the integrate that precedes the
+    // correction put them on their targets and the prolongation leaves them
+    // alone, so only a stand-alone V-cycle ends with the pin pass
+    if (pins && c->pins_on) {
         hipLaunchKernelGGL(k_pins, vgrid(c->nv), dim3(256), 0, c->stream, c->nv, c->X[0], (const float4 *)c->PIN);
         HIPCHK(c, hipGetLastError());
     }
@@ -3144,7 +3147,7 @@ int enqueue_substep(xpbd_ctx *c, int iters, bool dcd_rebuild, int sub = 0, bool 
                     bool fuse_next = false) {
     int r = integrated ? XPBD_OK : launch_integrate(c);
     if (r) return r;
-    if (c->coarse && c->mg_iters > 0 && (r = launch_vcycle(c))) return r;
+    if (c->coarse && c->mg_iters > 0 && (r = launch_vcycle(c, false))) return r;
     // mode 4: reference substeps (the first of a frame, or every ref_every-th)
     // are solves carried from zero with ref_iters iterations that refresh the
     // base; the others are warm starts (network or base) with iters iterations
