@@ -1785,11 +1785,12 @@ __device__ __forceinline__ bool dcd_emit(const DcdDev &d, int j, const int4 ti, 
 // of all cells of its search box at once (lane L takes cells L and L + 16, in
 // z, y, x order) and scans their sizes; the cells up to the visit limit are
 // kept (whole cells, as a cell-by-cell walk would keep them) and their
-// candidates form one flattened range that the lanes split evenly, four
-// candidates per lane per round with all loads issued first.  Emitted pairs
-// (min, max) are staged in LDS and appended to the pair list with one cursor
-// atomic per workgroup (a lane whose pair finds the stage full appends it
-// directly).
+// candidates form one flattened range that the lanes split evenly,
+// DCD_UNROLL candidates per lane per round with all loads issued first (a box
+// wider than three cells, from rounding at a level boundary, is walked cell by
+// cell).  Emitted pairs (min, max) are staged in LDS and appended to the pair
+// list with one cursor atomic per workgroup (past a full stage, one cursor
+// atomic per wavefront and emit point).
 constexpr int DCD_LANES = 16;
 constexpr int DCD_BOX = 27;        // cells of one search box (at most 3 per axis)
 constexpr int DCD_UNROLL = 2;
