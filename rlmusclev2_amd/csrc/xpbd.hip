@@ -610,26 +610,28 @@ __global__ __launch_bounds__(256) void k_flat_tets(IterArgs a, const float4 *__r
 // finished position of a vertex after an iteration: attach, apply dP/dPw
 // (apply_dP_kernel, engine.py:689-698) and the pin overwrite
 // (pintotarget_overwrite_masked_kernel, mg/kernel.py:212-226)
+// pv: the vertex's pin record (pin_of), loaded by the caller ahead of time
 __device__ __forceinline__ float4 finish_value(const IterArgs &a, int v, float4 x, float4 pp, float ax, float ay,
-                                               float az, float aw) {
+                                               float az, float aw, float4 pv) {
     if (a.att0) attach_vertex(a, v, x, pp, ax, ay, az, aw);
     if (aw > 1e-9f) {
         x.x = x.x + ax / aw;
         x.y = x.y + ay / aw;
         x.z = x.z + az / aw;
     }
-    if (a.pin) {
-        const float4 p = a.pin[v];
-        if (p.w != 0.0f) { x.x = p.x; x.y = p.y; x.z = p.z; }
-    }
+    if (a.pin && pv.w != 0.0f) { x.x = pv.x; x.y = pv.y; x.z = pv.z; }
     return x;
+}
+
+__device__ __forceinline__ float4 pin_of(const IterArgs &a, int v) {
+    return a.pin ? a.pin[v] : make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // finish one vertex: either accumulate (MODE_ACCUM, attach included) or write
 // the finished position
 template <int DAMP, int MODE>
 __device__ __forceinline__ void finish_vertex(const IterArgs &a, int v, float4 x, float4 pp, float ax, float ay,
-                                              float az, float aw, float4 *__restrict__ xout) {
+                                              float az, float aw, float4 pv, float4 *__restrict__ xout) {
     if (MODE == MODE_ACCUM) {
         if (a.att0) attach_vertex(a, v, x, pp, ax, ay, az, aw);
         float4 o = a.dpw[v];
@@ -640,7 +642,7 @@ __device__ __forceinline__ void finish_vertex(const IterArgs &a, int v, float4 x
         a.dpw[v] = o;
         return;
     }
-    xout[v] = finish_value(a, v, x, pp, ax, ay, az, aw);
+    xout[v] = finish_value(a, v, x, pp, ax, ay, az, aw, pv);
 }
 
 // Vertex pass of the flat strategy: four lanes per vertex read its contiguous
@@ -675,7 +677,8 @@ __global__ __launch_bounds__(256) void k_flat_verts(IterArgs a, const float4 *__
     az += __shfl_xor(az, 2);
     aw += __shfl_xor(aw, 2);
     if (v >= nv || j != 0) return;
-    finish_vertex<DAMP, MODE>(a, v, xin[v], DAMP ? pprev[v] : make_float4(0, 0, 0, 0), ax, ay, az, aw, xout);
+    finish_vertex<DAMP, MODE>(a, v, xin[v], DAMP ? pprev[v] : make_float4(0, 0, 0, 0), ax, ay, az, aw,
+                              MODE == MODE_FUSED ? pin_of(a, v) : make_float4(0, 0, 0, 0), xout);
 }
 
 // ---------------------------------------------------------------------------
@@ -729,12 +732,15 @@ __global__ __launch_bounds__(512) void k_block_tets(IterArgs a, BlockArgs b, con
         }
         int gv = 0, ps = -1;
         uint32_t run = 0;
+        float4 pv = make_float4(0.f, 0.f, 0.f, 0.f);
         if (has_v) {
             gv = b.bvt[D.vbeg + tid];
             run = b.brun[D.vbeg + tid];
             ps = b.bps[D.vbeg + tid];
             sx[tid] = xin[gv];
             if (DAMP) spp[tid] = pprev[gv];
+            // the pin record of a vertex this block finishes, loaded with the gather
+            if (MODE == MODE_FUSED && ps < 0) pv = pin_of(a, gv);
         }
         __syncthreads();
         if (has_t) {
@@ -760,7 +766,7 @@ __global__ __launch_bounds__(512) void k_block_tets(IterArgs a, BlockArgs b, con
         if (ps >= 0) {
             b.part[ps] = make_float4(ax, ay, az, aw);
         } else {
-            finish_vertex<DAMP, MODE>(a, gv, sx[tid], DAMP ? spp[tid] : make_float4(0, 0, 0, 0), ax, ay, az, aw,
+            finish_vertex<DAMP, MODE>(a, gv, sx[tid], DAMP ? spp[tid] : make_float4(0, 0, 0, 0), ax, ay, az, aw, pv,
                                       xout);
         }
         return;
@@ -801,8 +807,9 @@ __global__ __launch_bounds__(512) void k_block_tets(IterArgs a, BlockArgs b, con
         if (ps >= 0) {
             b.part[ps] = make_float4(ax, ay, az, aw);
         } else {
-            finish_vertex<DAMP, MODE>(a, b.bvt[D.vbeg + i], sx[i], DAMP ? spp[i] : make_float4(0, 0, 0, 0), ax, ay,
-                                      az, aw, xout);
+            const int gv = b.bvt[D.vbeg + i];
+            finish_vertex<DAMP, MODE>(a, gv, sx[i], DAMP ? spp[i] : make_float4(0, 0, 0, 0), ax, ay, az, aw,
+                                      MODE == MODE_FUSED ? pin_of(a, gv) : make_float4(0, 0, 0, 0), xout);
         }
     }
 }
@@ -823,6 +830,7 @@ __global__ __launch_bounds__(256) void k_block_shared(IterArgs a, BlockArgs b, c
     for (int k = 0; k < 4; ++k) q[k] = k < n ? b.part[s0 + k] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 x = xin[v];
     const float4 pp = DAMP ? pprev[v] : make_float4(0, 0, 0, 0);
+    const float4 pv = MODE == MODE_FUSED ? pin_of(a, v) : make_float4(0, 0, 0, 0);
     float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
 #pragma unroll
     for (int k = 0; k < 4; ++k)
@@ -839,7 +847,7 @@ __global__ __launch_bounds__(256) void k_block_shared(IterArgs a, BlockArgs b, c
         az += r.z;
         aw += r.w;
     }
-    finish_vertex<DAMP, MODE>(a, v, x, pp, ax, ay, az, aw, xout);
+    finish_vertex<DAMP, MODE>(a, v, x, pp, ax, ay, az, aw, pv, xout);
 }
 
 
