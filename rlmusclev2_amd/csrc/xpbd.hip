@@ -711,25 +711,30 @@ __global__ __launch_bounds__(512) void k_block_tets(IterArgs a, BlockArgs b, con
                                                     float4 *__restrict__ xout,
                                                     const float4 *__restrict__ pprev) {
     extern __shared__ float4 smem[];
-    const BlockDesc D = b.blk[blockIdx.x];
     const int M = b.max_nloc, TB = b.block_tets;
     float4 *sx = smem;
     float4 *spp = smem + M;
     float4 *sc = smem + (DAMP ? 2 : 1) * M;
     const int tid = threadIdx.x, nth = blockDim.x;
+    // blocks are consecutive runs of TB elements, so the element loads of the
+    // one-element-per-thread path are issued before the block record arrives
+    const int64_t tb0 = (int64_t)blockIdx.x * TB;
+    const int nte = (int)min((int64_t)TB, (int64_t)a.nptet - tb0);
+    const bool has_t = nte <= nth && tid < nte;
+    const int64_t g = tb0 + tid;
+    ElemIn<FEAT, LAM> e;
+    ushort4 li = make_ushort4(0, 0, 0, 0), cp = li;
+    if (has_t) {
+        load_elem(a, g, e);
+        li = b.blidx[g];
+        cp = b.bcpos[g];
+    }
+    const BlockDesc D = b.blk[blockIdx.x];
     if (D.ntet <= nth && D.nloc <= nth) {
         // one element and at most one local vertex per thread: the element's
         // data and the vertex's tables are loaded before the staging barrier,
         // so their latency overlaps the vertex gather
-        const bool has_t = tid < D.ntet, has_v = tid < D.nloc;
-        const int64_t g = (int64_t)D.tbeg + tid;
-        ElemIn<FEAT, LAM> e;
-        ushort4 li = make_ushort4(0, 0, 0, 0), cp = li;
-        if (has_t) {
-            load_elem(a, g, e);
-            li = b.blidx[g];
-            cp = b.bcpos[g];
-        }
+        const bool has_v = tid < D.nloc;
         int gv = 0, ps = -1;
         uint32_t run = 0;
         float4 pv = make_float4(0.f, 0.f, 0.f, 0.f);
