@@ -168,3 +168,27 @@ def test_crowded_surface_stops_at_the_visit_limit():
     assert st["truncated_tris"] > st["n_pairs"] - cap      # stopped searches on top of the dropped pairs
     eng.dcd_solve(False)
     assert np.isfinite(eng.positions_np()).all()
+
+
+def test_triangles_with_a_non_finite_vertex_are_left_out():
+    """a vertex turned NaN: every triangle on it is left out of the broadphase
+    and counted, and the pairs of the others equal brute force over them"""
+    sc, tris = _scene()
+    eng = XpbdEngine(sc)
+    rng = np.random.default_rng(3)
+    x = sc.pos0 + rng.normal(scale=2e-3, size=sc.pos0.shape).astype(np.float32)
+    bad = int(tris[len(tris) // 2, 0])
+    x[bad] = np.nan
+    eng.set_state(x, x, np.zeros_like(x))
+    eng.dcd_solve(True)
+    got = eng.dcd_pairs()
+    st = eng.dcd_stats()
+    on_bad = np.nonzero((tris == bad).any(axis=1))[0]
+    keep = np.setdiff1d(np.arange(len(tris)), on_bad)
+    sub = oracle.dcd_pairs(tris[keep], x, allow_self=True, thickness=1e-3)
+    ref = np.sort(np.stack([keep[sub[:, 0]], keep[sub[:, 1]]], axis=1), axis=1)
+    ref = ref[np.lexsort((ref[:, 1], ref[:, 0]))]
+    print("left out", st["left_out_tris"], len(on_bad), "pairs", len(got), len(ref))
+    assert st["left_out_tris"] == len(on_bad) > 0
+    assert not np.isin(got, on_bad).any()
+    np.testing.assert_array_equal(got, ref)
