@@ -26,10 +26,10 @@ HBM_PEAK_GBS = 8000.0
 # dispatch, scaled x2.0 and x1.0 by a plain-copy calibration), keyed by (tets, constraint types,
 # strategy, block tets, block threads); the runs are summarised under profiles/
 TRAFFIC = {
-    (534882, ("tetarapsvd", "volume"), 2, 512, 512): 71432706.0,                    # humanV2-scale, plain
-    (534882, ("tetarapsvd", "volume"), 2, 256, 256): 78991175.0,                    # the same, smaller blocks
-    (534882, ("snh", "aniso_arap"), 2, 256, 256): 78346826.0,                       # humanV2-scale, SNH + aniso
-    (7063998, ("tetarapsvd", "volume", "aniso_arap"), 2, 256, 256): 1000229263.0,  # 512 instances
+    (534882, ("tetarapsvd", "volume"), 2, 512, 512): 73468826.0,                    # humanV2-scale, plain
+    (534882, ("tetarapsvd", "volume"), 2, 256, 256): 81038234.0,                    # the same, smaller blocks
+    (534882, ("snh", "aniso_arap"), 2, 256, 256): 78355866.0,                       # humanV2-scale, SNH + aniso
+    (7063998, ("tetarapsvd", "volume", "aniso_arap"), 2, 256, 256): 1005044531.0,  # 512 instances
 }
 
 WORKLOADS = {
