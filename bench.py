@@ -28,7 +28,9 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC = {
     (534882, ("tetarapsvd", "volume"), 2, 512, 512): 73468826.0,                    # humanV2-scale, plain
     (534882, ("tetarapsvd", "volume"), 2, 256, 256): 81038234.0,                    # the same, smaller blocks
+    (534882, ("tetarapsvd", "volume"), 2, 192, 192): 84720845.0,
     (534882, ("snh", "aniso_arap"), 2, 256, 256): 78355866.0,                       # humanV2-scale, SNH + aniso
+    (534882, ("snh", "aniso_arap"), 2, 192, 192): 81385779.0,
     (7063998, ("tetarapsvd", "volume", "aniso_arap"), 2, 256, 256): 1005044531.0,  # 512 instances
 }
 
