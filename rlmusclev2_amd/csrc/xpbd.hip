@@ -1655,17 +1655,17 @@ __device__ __forceinline__ uint32_t dcd_hash(int x, int y, int z, int level, int
     return h & ((1u << bits) - 1u);
 }
 
-// centroid_radius_kernel (:47-63) + face_normal_kernel (:66-84) + max radius
-// (max_radius_reduce_kernel :87-100) + a log2 histogram of the radii
-// (the maximum and the histogram are reduced in the workgroup first: one
-// device-scope atomic per wavefront / per non-empty bin rather than per triangle)
+// centroid_radius_kernel (:47-63) + face_normal_kernel (:66-84) + a log2
+// histogram of the radii, in place of max_radius_reduce_kernel (:87-100): the
+// grid hierarchy takes its base cell from the radius distribution, not from
+// the maximum.  The histogram is reduced in the workgroup first: one
+// device-scope atomic per non-empty bin rather than per triangle.
 __global__ __launch_bounds__(256) void k_dcd_tridata(DcdDev d, const float4 *X) {
 #pragma clang fp contract(off)
     __shared__ uint32_t sh[DCD_HIST];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (threadIdx.x < DCD_HIST) sh[threadIdx.x] = 0u;
     __syncthreads();
-    uint32_t rbits = 0u;
     if (i < d.ntris) {
     const int4 t = d.tris[i];
     const V3 a = xyz(X[t.x]), b = xyz(X[t.y]), c = xyz(X[t.z]);
@@ -1680,15 +1680,11 @@ __global__ __launch_bounds__(256) void k_dcd_tridata(DcdDev d, const float4 *X) 
     const float ln = sqrtf(dot(n, n));
     d.nrm[i] = ln > 0.0f ? make_float4(n.x / ln, n.y / ln, n.z / ln, 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
     if (isfinite(r)) {
-        rbits = __float_as_uint(r);  // r >= 0: uint order == float order
         // bin k holds r in [2^((k-80)/4), 2^((k-79)/4)); the ends are open
         const int k = r > 0.0f ? (int)floorf(4.0f * log2f(r)) + 80 : 0;
         atomicAdd(&sh[min(max(k, 0), DCD_HIST - 1)], 1u);
     }
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) rbits = max(rbits, (uint32_t)__shfl_xor((int)rbits, o));
-    if ((threadIdx.x & 63) == 0 && rbits) atomicMax(d.rmax_bits, rbits);
     __syncthreads();
     if (threadIdx.x < DCD_HIST && sh[threadIdx.x]) atomicAdd(&d.hist[threadIdx.x], sh[threadIdx.x]);
 }
@@ -1715,7 +1711,6 @@ __global__ void k_dcd_cellsize(DcdDev d) {
     }
     d.scal[0] = c0;
     d.scal[1] = 1.0f / c0;
-    d.scal[2] = __uint_as_float(*d.rmax_bits);
 }
 
 // level b cell and its inverse: exact power-of-two scalings of the base
@@ -2320,7 +2315,6 @@ __global__ void k_dcd_reset(DcdDev d, int64_t T) {
         d.bucket_fill[i] = 0u;
     }
     if (i == 0) {
-        *d.rmax_bits = 0u;
         *d.npairs = 0ull;
     }
     if (i < 3) d.overflow[i] = 0u;
@@ -3061,7 +3055,7 @@ int alloc_dcd(xpbd_ctx *c) {
         (r = dalloc(c, &d.pairs, (size_t)d.max_pairs)) ||
         (r = dalloc(c, &d.acc, 3 * (size_t)c->nv)) || (r = dalloc(c, &d.acc_n, (size_t)c->nv)) ||
         (r = dalloc(c, &d.xprev, (size_t)c->nv)) || (r = dalloc(c, &d.scal, 4)) ||
-        (r = dalloc(c, &d.rmax_bits, 1)) || (r = dalloc(c, &d.ncontact, 2)) || (r = dalloc(c, &d.overflow, 3)) ||
+        (r = dalloc(c, &d.ncontact, 2)) || (r = dalloc(c, &d.overflow, 3)) ||
         (r = dalloc(c, &d.tlvl, (size_t)std::max(m, 1))) || (r = dalloc(c, &d.rec, (size_t)std::max(m, 1))) ||
         (r = dalloc(c, &d.hist, (size_t)DCD_HIST)) || (r = dalloc(c, &d.lvl_cnt, (size_t)DCD_LEVELS)) ||
         (r = dalloc(c, &d.pair_fill, (size_t)m + 1)))

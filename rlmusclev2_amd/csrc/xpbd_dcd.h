@@ -72,8 +72,7 @@ struct DcdDev {
     unsigned long long *acc;    // 3 per vertex, fixed point
     uint32_t *acc_n;            // per vertex
     float4 *xprev;              // snapshot at broadphase
-    float *scal;                // [0] base cell c0, [1] 1 / c0, [2] max radius, [3] unused
-    uint32_t *rmax_bits;
+    float *scal;                // [0] base cell c0, [1] 1 / c0, [2] [3] unused
     uint32_t *ncontact;         // [0] contacts of the running pass, [1] of the last finished pass
     uint32_t *overflow;         // [0] triangles left out (too large / non-finite), [1] truncated searches,
                                 // [2] segments too long to sort
