@@ -1751,7 +1751,10 @@ __global__ __launch_bounds__(256) void k_dcd_keys(DcdDev d) {
     __syncthreads();
     if (i < d.ntris) dcd_key_one(d, i, sl);
     __syncthreads();
-    if (threadIdx.x < DCD_LEVELS && sl[threadIdx.x]) atomicAdd(&d.lvl_cnt[threadIdx.x], sl[threadIdx.x]);
+    // the search only asks whether a level holds triangles: a flag, set with a
+    // plain store (every writer stores 1) once this workgroup's copy reads 0,
+    // instead of an atomic count on one of a dozen addresses per workgroup
+    if (threadIdx.x < DCD_LEVELS && sl[threadIdx.x] && d.lvl_cnt[threadIdx.x] == 0u) d.lvl_cnt[threadIdx.x] = 1u;
 }
 
 

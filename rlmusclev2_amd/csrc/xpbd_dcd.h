@@ -57,7 +57,7 @@ struct DcdDev {
     int32_t *tlvl;              // grid level per triangle, -1 = left out
     float4 *rec;                // centroid, radius in bucket order
     uint32_t *hist;             // DCD_HIST radius bins
-    uint32_t *lvl_cnt;          // triangles per level
+    uint32_t *lvl_cnt;          // per level: 1 when a triangle lives there
     uint32_t *bucket_cnt;       // T
     uint32_t *bucket_off;       // T + 1
     uint32_t *bucket_fill;      // T
