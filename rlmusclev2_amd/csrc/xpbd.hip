@@ -2155,7 +2155,8 @@ __device__ __forceinline__ void dcd_add(const DcdDev &d, int v, V3 c) {
 }
 
 // contact and response of one pair that passed the straddle test
-__device__ __forceinline__ void dcd_pair_contact(const DcdDev &d, const float4 *X, int4 A, int4 Bt, const DcdGeo &g) {
+// returns whether the pair made a valid contact (counted by the caller)
+__device__ __forceinline__ bool dcd_pair_contact(const DcdDev &d, const float4 *X, int4 A, int4 Bt, const DcdGeo &g) {
 #pragma clang fp contract(off)
     const V3 q00 = g.q00, q01 = g.q01, q02 = g.q02, q10 = g.q10, q11 = g.q11, q12 = g.q12;
     const V3 N1 = g.N1, N2 = g.N2;
@@ -2207,8 +2208,7 @@ __device__ __forceinline__ void dcd_pair_contact(const DcdDev &d, const float4 *
             }
         }
     }
-    if (C1 <= d.offset_eps || C2 <= d.offset_eps) return;
-    atomicAdd(&d.ncontact[0], 1u);
+    if (C1 <= d.offset_eps || C2 <= d.offset_eps) return false;
     const bool first = C1 < C2;
     const int4 cu4 = first ? A : Bt, cl4 = first ? Bt : A;
     const int cu[3] = {cu4.x, cu4.y, cu4.z}, cl[3] = {cl4.x, cl4.y, cl4.z};
@@ -2217,7 +2217,7 @@ __device__ __forceinline__ void dcd_pair_contact(const DcdDev &d, const float4 *
     // respond_kernel (:714-799)
     const float pen = first ? C1 : C2;
     float C = pen * d.multiplier;
-    if (C == 0.0f) return;
+    if (C == 0.0f) return true;
     float wcu[3], wcl[3];
     for (int j = 0; j < 3; ++j) { wcu[j] = X[cu[j]].w; wcl[j] = X[cl[j]].w; }
     const bool solid_cur = wcu[0] == 0.0f || wcu[1] == 0.0f || wcu[2] == 0.0f;
@@ -2245,6 +2245,7 @@ __device__ __forceinline__ void dcd_pair_contact(const DcdDev &d, const float4 *
         if (wcu[j] > 0.0f) dcd_add(d, cu[j], dcur);
         if (wcl[j] > 0.0f) dcd_add(d, cl[j], dclo);
     }
+    return true;
 }
 
 // A workgroup takes DCD_NR rounds of 256 pairs, queues the pairs that pass
@@ -2257,7 +2258,8 @@ constexpr int DCD_QCAP = DCD_NR * 256;   // a round's pairs all fit: the queue n
 __global__ __launch_bounds__(256) void k_dcd_narrow(DcdDev d, const float4 *X) {
     // launched with 256 threads
     __shared__ int64_t s_q[DCD_QCAP];
-    __shared__ uint32_t s_nq;
+    __shared__ uint32_t s_nq, s_nc;
+    if (threadIdx.x == 0) s_nc = 0u;
     const int64_t total = (int64_t)min(*d.npairs, (unsigned long long)d.max_pairs);
     const int64_t span = (int64_t)DCD_NR * 256;
     for (int64_t base = (int64_t)blockIdx.x * span; base < total; base += (int64_t)gridDim.x * span) {
@@ -2281,10 +2283,13 @@ __global__ __launch_bounds__(256) void k_dcd_narrow(DcdDev d, const float4 *X) {
             const int4 A = d.tris[pr.x], Bt = d.tris[pr.y];
             DcdGeo g;
             dcd_load(X, A, Bt, g);
-            if (dcd_straddle(g)) dcd_pair_contact(d, X, A, Bt, g);   // the same test: always passes
+            // the same test as above: always passes
+            if (dcd_straddle(g) && dcd_pair_contact(d, X, A, Bt, g)) atomicAdd(&s_nc, 1u);
         }
         __syncthreads();
     }
+    // contacts counted per workgroup, one device-scope atomic each
+    if (threadIdx.x == 0 && s_nc) atomicAdd(&d.ncontact[0], s_nc);
 }
 
 // apply_kernel (:802-815), averaged; clears the accumulators and the contact
