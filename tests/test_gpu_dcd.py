@@ -192,3 +192,31 @@ def test_triangles_with_a_non_finite_vertex_are_left_out():
     assert st["left_out_tris"] == len(on_bad) > 0
     assert not np.isin(got, on_bad).any()
     np.testing.assert_array_equal(got, ref)
+
+
+def test_frames_with_dcd_equal_the_phase_composition():
+    """whole frames (one hipGraph: integrate, iterations, two DCD passes, SDF
+    push and velocities fused with the next integrate) equal the same substeps
+    composed from the phase calls, bit for bit"""
+    sc, tris = _scene()
+    a, b = XpbdEngine(sc), XpbdEngine(sc)
+    rng = np.random.default_rng(4)
+    x = sc.pos0 + rng.normal(scale=1.5e-3, size=sc.pos0.shape).astype(np.float32)
+    for e in (a, b):
+        e.set_state(x, x, np.zeros_like(x))
+    nsub, iters = 4, 5
+    for f in range(2):
+        print(f"frame {f}", flush=True)
+        a.native_frame(nsub, iters)
+        for s in range(nsub):
+            b.integrate()
+            b.iterate(iters)
+            for k in range(2):
+                b.dcd_solve(s == 0 and k == 0)
+            b.collide()
+            b.update_velocities()
+    pa, pb = a.positions_np(), b.positions_np()
+    assert np.isfinite(pa).all() and a.dcd_stats()["n_pairs"] > 0
+    assert float(np.abs(pa - x).max()) > 1e-4
+    np.testing.assert_array_equal(pa, pb)
+    np.testing.assert_array_equal(a.velocities_np(), b.velocities_np())
