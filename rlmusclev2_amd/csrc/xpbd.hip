@@ -1133,13 +1133,33 @@ __device__ SdfHit sdf_eval(const float *grid, int res, const float *bn, V3 x) {
 // update_velocities_kernel (engine.py:582-605); result lands in X (buffer A).
 // INTEG = 1 goes on with the next substep's integrate of the same vertex
 // (nothing runs between the two inside a frame), one launch instead of two.
+// dacc != null: the substep's last DCD pass is applied here first, with the
+// arithmetic of k_dcd_apply (averaged 2^-32 m fixed-point sums, accumulators
+// and the contact counter reset for the next pass)
 template <int INTEG>
 __global__ void k_post(int nv, const float4 *Xc, float4 *X, float4 *PP, float4 *VEL,
                        SdfParams sd, int32_t *cells, SimParams sp, int do_sdf, int do_vel, const float4 *FRC,
-                       const uint32_t *flags, const float4 *pin) {
+                       const uint32_t *flags, const float4 *pin, unsigned long long *dacc, uint32_t *dacc_n,
+                       uint32_t *dnc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nv) return;
     float4 x = Xc[i];
+    if (dacc) {
+        if (i == 0) {
+            dnc[1] = dnc[0];
+            dnc[0] = 0u;
+        }
+        const uint32_t n = dacc_n[i];
+        if (n > 0) {
+            const float w = (float)n;
+            const float inv = 1.0f / DCD_FIX;
+            x.x = x.x + ((float)(long long)dacc[3 * i + 0] * inv) / w;
+            x.y = x.y + ((float)(long long)dacc[3 * i + 1] * inv) / w;
+            x.z = x.z + ((float)(long long)dacc[3 * i + 2] * inv) / w;
+            dacc[3 * i + 0] = dacc[3 * i + 1] = dacc[3 * i + 2] = 0ull;
+            dacc_n[i] = 0u;
+        }
+    }
     const float4 pp = PP[i];
     float4 v_new = make_float4(0.f, 0.f, 0.f, 0.f);
     bool vel_set = false;
@@ -1647,7 +1667,6 @@ __global__ void k_count_inverted(int nt, const int4 *tets, const float *B, const
 // tri-tri DCD kernels (semantics: src/muscle/collision/udcf/kernel.py)
 // ---------------------------------------------------------------------------
 #define DCD_THRESH 1.0e-5f
-constexpr float DCD_FIX = 4294967296.0f;  // 2^32 fixed-point scale (metres)
 
 __device__ __forceinline__ uint32_t dcd_hash(int x, int y, int z, int level, int bits) {
     const uint32_t h = ((uint32_t)x * 73856093u) ^ ((uint32_t)y * 19349663u) ^ ((uint32_t)z * 83492791u) ^
@@ -2955,8 +2974,9 @@ int launch_integrate(xpbd_ctx *c) {
     return XPBD_OK;
 }
 
-// integ: also the next substep's integrate (k_post<1>)
-int launch_post(xpbd_ctx *c, const float4 *xc, int do_sdf, int do_vel, bool integ = false) {
+// integ: also the next substep's integrate (k_post<1>); dcd_apply: also the
+// apply of the substep's last DCD pass
+int launch_post(xpbd_ctx *c, const float4 *xc, int do_sdf, int do_vel, bool integ = false, bool dcd_apply = false) {
     SdfParams sd{};
     sd.nbones = c->nbones;
     sd.res = c->res;
@@ -2969,14 +2989,16 @@ int launch_post(xpbd_ctx *c, const float4 *xc, int do_sdf, int do_vel, bool inte
     const int sdf_on = (do_sdf && c->nbones > 0 && c->d_grids) ? 1 : 0;
     const float4 *frc = c->has_force ? (const float4 *)c->FRC : nullptr;
     const float4 *pin = c->pins_on ? (const float4 *)c->PIN : nullptr;
+    unsigned long long *dacc = dcd_apply ? c->dcd.acc : nullptr;
+    uint32_t *dacc_n = dcd_apply ? c->dcd.acc_n : nullptr, *dnc = dcd_apply ? c->dcd.ncontact : nullptr;
     if (integ)
         hipLaunchKernelGGL(k_post<1>, vgrid(c->nv), dim3(256), 0, c->stream, c->nv, xc, c->X[0], c->PPv, c->VEL, sd,
                            sdf_on ? c->d_cells : nullptr, c->sp, sdf_on, do_vel, frc, (const uint32_t *)c->FLAGS,
-                           pin);
+                           pin, dacc, dacc_n, dnc);
     else
         hipLaunchKernelGGL(k_post<0>, vgrid(c->nv), dim3(256), 0, c->stream, c->nv, xc, c->X[0], c->PPv, c->VEL, sd,
                            sdf_on ? c->d_cells : nullptr, c->sp, sdf_on, do_vel, frc, (const uint32_t *)c->FLAGS,
-                           pin);
+                           pin, dacc, dacc_n, dnc);
     HIPCHK(c, hipGetLastError());
     return XPBD_OK;
 }
@@ -2999,8 +3021,9 @@ int launch_scan(xpbd_ctx *c, const uint32_t *in, uint32_t *out, int64_t n, uint3
     return XPBD_OK;
 }
 
-// one DCD pass on positions X (in place); rebuild = broadphase first
-int launch_dcd(xpbd_ctx *c, float4 *X, bool rebuild) {
+// one DCD pass on positions X (in place); rebuild = broadphase first;
+// apply = false leaves the correction in the accumulators for k_post
+int launch_dcd(xpbd_ctx *c, float4 *X, bool rebuild, bool apply = true) {
     DcdDev &d = c->dcd;
     const int m = d.ntris;
     const int64_t T = (int64_t)1 << d.tbits;
@@ -3019,7 +3042,7 @@ int launch_dcd(xpbd_ctx *c, float4 *X, bool rebuild) {
         HIPCHK(c, hipGetLastError());
     }
     hipLaunchKernelGGL(k_dcd_narrow, dim3(1024), dim3(256), 0, c->stream, d, (const float4 *)X);
-    hipLaunchKernelGGL(k_dcd_apply, g1(c->nv), dim3(256), 0, c->stream, d, X, c->nv);
+    if (apply) hipLaunchKernelGGL(k_dcd_apply, g1(c->nv), dim3(256), 0, c->stream, d, X, c->nv);
     HIPCHK(c, hipGetLastError());
     return XPBD_OK;
 }
@@ -3182,10 +3205,14 @@ int enqueue_substep(xpbd_ctx *c, int iters, bool dcd_rebuild, int sub = 0, bool 
     }
     if ((r = finish_lambda(c))) return r;
     // post phase: self DCD passes, then bone push-out and velocities (mg.py:1362-1376)
-    if (c->dcd_on)
+    // the last pass's apply runs inside the post kernel
+    const bool dcd = c->dcd_on && c->dcd_iters > 0;
+    if (dcd)
         for (int k = 0; k < c->dcd_iters; ++k)
-            if ((r = launch_dcd(c, c->X[iters & 1], k == 0 && (dcd_rebuild || c->dcd_rebuild_substeps)))) return r;
-    return launch_post(c, c->X[iters & 1], 1, 1, fuse_next);
+            if ((r = launch_dcd(c, c->X[iters & 1], k == 0 && (dcd_rebuild || c->dcd_rebuild_substeps),
+                                k + 1 < c->dcd_iters)))
+                return r;
+    return launch_post(c, c->X[iters & 1], 1, 1, fuse_next, dcd);
 }
 
 int check_ready(xpbd_ctx *c) {

@@ -38,6 +38,7 @@
 
 namespace xpbd {
 
+constexpr float DCD_FIX = 4294967296.0f;       // 2^32 fixed-point scale of the corrections (metres)
 constexpr int DCD_LEVELS = 12;                 // cells c0 .. 2048 c0
 constexpr int DCD_HIST = 64;                   // radius histogram, 4 bins per octave
 constexpr uint32_t DCD_MAX_VISIT = 1u << 15;   // candidates one search may visit
