@@ -7,8 +7,10 @@
  * last failure from xpbd_last_error().  No exceptions cross the boundary.
  *
  * All host arrays are in the caller's vertex / tet / constraint order; the
- * library keeps its own device layout (vertices grouped into patches, tets
- * replicated per patch) and converts on upload / download.
+ * library keeps its own device layout (vertices in Morton order, one fused
+ * element per tet holding all of its constraint types, elements in Morton
+ * order and cut into blocks of consecutive elements for the block strategy,
+ * the autotuner's pick) and converts on upload / download.
  *
  * Each declaration names the reference interface it replaces.
  */
@@ -139,8 +141,11 @@ int32_t xpbd_get_stats(xpbd_ctx *ctx, xpbd_stats *out);
  * _coarse_arap_sweep mg.py:939-952; restriction restrict_galerkin_scatter_kernel +
  * block_solve_kernel, src/muscle/mg/kernel.py:138-180; prolong_delta_kernel kernel.py:9-40).
  * The coarse cage is another finalized context (flat strategy): ARAP-SVD constraints, masses,
- * stopped = coarse pins.  Coarse nodes are grouped in blocks (one per muscle, contiguous in the
- * coarse context's caller order).  Per substep, per block b:
+ * and its active pins are the coarse pins: they take part in every coarse projection with their
+ * inverse mass and keep their restricted positions (the fine level's pin rule, mg.py:1337,1355);
+ * stopped nodes are held with zero inverse mass.  Coarse nodes are grouped in blocks (one per
+ * muscle, contiguous in the coarse context's caller order; at most 1024 nodes and 2048 tets per
+ * block).  Per substep, per block b:
  *   [A t]_b = (sum_{i in fit_b} x_i [r_i - c_b; 1]^T) S_b^-1      (r = fine rest positions)
  *   q_j     = sum_l G_b[j, l] (sum_i P_il x_i + eps_b (A_b (q0_l - c_b) + t_b))
  * then iters coarse Jacobi sweeps, then x_i += sum_j P_ij (q_j - q_prev_j) for every vertex

@@ -1,15 +1,18 @@
 // MI355X (gfx950) XPBD muscle substep: kernels + C ABI (include/xpbd.h).
 //
-// Hot loop: one launch per Jacobi iteration, one workgroup per patch of
-// vertices.  The workgroup stages the patch's vertex table in LDS, evaluates
-// every tet element that touches an owned vertex (ARAP-SVD / volume / SNH /
-// aniso fused per tet), parks the corner corrections in LDS and sums each
-// owned vertex's corrections in a fixed slot order (deterministic; no
-// atomics), then applies dP/dPw, attach and pins and writes the new position
-// into the other half of a ping-pong pair.  Same semantics as the reference
-// per-type kernels + update_dP atomics + apply_dP + pin overwrite
-// (src/muscle/constraint/solve_kernels.py, src/muscle/engine.py:689-698,
-// src/muscle/mg/kernel.py:212-226).
+// Hot loop (block strategy, the autotuner's pick on every workload here): per
+// Jacobi iteration, k_block_tets gives each workgroup a block of consecutive
+// Morton-ordered tet elements (ARAP-SVD / volume / SNH / aniso fused per tet),
+// stages the vertices they touch in LDS, projects one element per thread and
+// parks the four corner corrections in LDS; every local vertex then sums its
+// corners in a fixed (element, corner) order.  A vertex touched by this block
+// only is finished in place (dP/dPw, attach, pins); a shared one gets one
+// partial per block, which k_block_shared adds in ascending block order.  No
+// float atomics, so a run repeats bit for bit.  The patch (k_iterate) and flat
+// (k_flat_tets + k_flat_verts) strategies are the alternatives the autotuner
+// weighs.  Same semantics as the reference per-type kernels + update_dP atomics
+// + apply_dP + pin overwrite (src/muscle/constraint/solve_kernels.py,
+// src/muscle/engine.py:689-698, src/muscle/mg/kernel.py:212-226).
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 
@@ -1275,6 +1278,7 @@ struct MgCoarse {
     const float *tB;             // per tet: rest inverse B, 9 floats row-major
     const float *tat;            // per tet: 1 / (k dt^2)
     const float *node_w;         // per node: inverse mass (0 = held)
+    const uint8_t *node_pin;     // per node: 1 = coarse pin (projected with its inverse mass, never moved)
     const int32_t *slot_off;     // per node + 1: its run in the block's vertex-major corner list
     const ushort4 *tcpos;        // per tet: its corners' places in that list (block-local)
     int max_nodes, max_tets;
@@ -1434,7 +1438,9 @@ __device__ __forceinline__ void mg_coarse_tet(const float4 *sq, float4 *sc, usho
 // most two) tets' data in registers and its node's corner range across the
 // sweeps, so a sweep touches only LDS: project every tet, barrier, each node
 // adds its contiguous corner run (ascending (tet, corner) order) and applies
-// dP / dPw, barrier.
+// dP / dPw, barrier.  A coarse pin takes part in the projections with its
+// inverse mass and keeps its restricted position (the fine level's pin rule:
+// apply, then snap back to the target, mg.py:1337,1355).
 __global__ __launch_bounds__(MGC_T) void k_mg_coarse(MgArgs m, MgCoarse mc, int iters, float4 *__restrict__ xc) {
     extern __shared__ float4 smem[];
     float4 *sq = smem;                         // xyz, inverse mass
@@ -1473,7 +1479,9 @@ __global__ __launch_bounds__(MGC_T) void k_mg_coarse(MgArgs m, MgCoarse mc, int 
     const int node = lpn == 2 ? (tid >> 1) : tid, part = lpn == 2 ? (tid & 1) : 0;
     const bool hasn = node < nn;
     int s0 = 0, ns = 0;
+    bool pinned = false;
     if (hasn) {
+        pinned = mc.node_pin[n0 + node] != 0;
         const int base = mc.slot_off[n0];
         s0 = mc.slot_off[n0 + node] - base;
         ns = mc.slot_off[n0 + node + 1] - mc.slot_off[n0 + node];
@@ -1498,7 +1506,7 @@ __global__ __launch_bounds__(MGC_T) void k_mg_coarse(MgArgs m, MgCoarse mc, int 
             az += __shfl_xor(az, 1);
             aw += __shfl_xor(aw, 1);
         }
-        if (hasn && part == 0 && aw > 1e-9f) {   // apply_dP_kernel, engine.py:689-698
+        if (hasn && part == 0 && !pinned && aw > 1e-9f) {   // apply_dP_kernel, engine.py:689-698
             float4 x = sq[node];
             x.x = x.x + ax / aw;
             x.y = x.y + ay / aw;
@@ -3155,8 +3163,9 @@ int lam_for_iteration(xpbd_ctx *c) {
     return c->eff_mode == 1 ? 1 : 0;
 }
 
-// restrict -> coarse Jacobi ARAP sweeps (coarse pins held as stopped vertices)
-// -> delta prolongation -> pins, on the fine context's stream
+// restrict -> coarse Jacobi ARAP sweeps (coarse pins projected with their
+// inverse mass and kept at their restricted positions) -> delta prolongation
+// -> pins, on the fine context's stream
 int launch_vcycle(xpbd_ctx *c, bool pins = true) {
     const int nc = c->coarse->nv;
     hipLaunchKernelGGL(k_mg_fit, dim3((unsigned)c->mg_nblk), dim3(MGF_T), 0, c->stream, c->mg, (const float4 *)c->X[0]);
@@ -3967,7 +3976,7 @@ int32_t xpbd_set_dcd(xpbd_ctx *c, int32_t ntris, const int32_t *tris, const int3
     else c->dcd_group.clear();
     if (p) {
         c->dcd_p = *p;
-        c->dcd_iters = p->iters > 0 ? p->iters : 2;
+        c->dcd_iters = std::max(1, (int)p->iters);   // K = max(1, iters) passes (humanV2.py _make_self_collider)
         c->dcd_rebuild_substeps = p->rebuild_substeps;
     }
     if (c->ready) free_device(c);
@@ -4218,6 +4227,8 @@ int32_t xpbd_set_coarse_level(xpbd_ctx *c, xpbd_ctx *cc, const xpbd_coarse_desc 
         }
         if (4 * max_tets > 65535) return fail(c, XPBD_ELAYOUT, "coarse cage block too large for 16-bit slots");
         if (max_tets > 2 * MGC_T) return fail(c, XPBD_ELAYOUT, "coarse cage block has more than 2048 tets");
+        // k_mg_coarse loads, sums and writes back one node per thread
+        if (max_nodes > MGC_T) return fail(c, XPBD_ELAYOUT, "coarse cage block has more than 1024 nodes");
         for (int e = 0; e < nct; ++e) {
             const int t = order[e];
             const int bb = tblk[t], n0 = d->blk_off[bb], lt = e - toff[bb];
@@ -4236,6 +4247,9 @@ int32_t xpbd_set_coarse_level(xpbd_ctx *c, xpbd_ctx *cc, const xpbd_coarse_desc 
         std::vector<int32_t> soff(nc + 1, 0);
         std::vector<ushort4> tcp(std::max(nct, 1), make_ushort4(0, 0, 0, 0));
         std::vector<float> nw(std::max(nc, 1), 0.f);
+        std::vector<uint8_t> npin(std::max(nc, 1), 0);
+        for (size_t i = 0; i < cc->pin_vids.size(); ++i)
+            if (cc->pin_active[i]) npin[cc->pin_vids[i]] = 1;
         for (int v = 0; v < nc; ++v) {
             std::sort(nslots[v].begin(), nslots[v].end());
             const int bb = nblk_of[v], base = soff[d->blk_off[bb]];
@@ -4252,10 +4266,12 @@ int32_t xpbd_set_coarse_level(xpbd_ctx *c, xpbd_ctx *cc, const xpbd_coarse_desc 
         int32_t *p_toff, *p_soff;
         ushort4 *p_tl;
         float *p_tB, *p_tat, *p_nw;
+        uint8_t *p_npin;
         ushort4 *p_tcp;
         if ((r = dupload(c, &p_toff, toff.data(), toff.size())) || (r = dupload(c, &p_tl, tl.data(), tl.size())) ||
             (r = dupload(c, &p_tB, tB.data(), tB.size())) || (r = dupload(c, &p_tat, tat.data(), tat.size())) ||
-            (r = dupload(c, &p_nw, nw.data(), nw.size())) || (r = dupload(c, &p_soff, soff.data(), soff.size())) ||
+            (r = dupload(c, &p_nw, nw.data(), nw.size())) || (r = dupload(c, &p_npin, npin.data(), npin.size())) ||
+            (r = dupload(c, &p_soff, soff.data(), soff.size())) ||
             (r = dupload(c, &p_tcp, tcp.data(), tcp.size())) || (r = dalloc(c, &m.q, (size_t)std::max(nc, 1))))
             return r;
         g.tet_off = p_toff;
@@ -4263,6 +4279,7 @@ int32_t xpbd_set_coarse_level(xpbd_ctx *c, xpbd_ctx *cc, const xpbd_coarse_desc 
         g.tB = p_tB;
         g.tat = p_tat;
         g.node_w = p_nw;
+        g.node_pin = p_npin;
         g.slot_off = p_soff;
         g.tcpos = p_tcp;
         g.max_nodes = max_nodes;

@@ -46,12 +46,17 @@ for _a, _b, _c in ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1,
     _KUHN.append((np.zeros(3, np.int64), e[_a], e[_a] + e[_b], e[_a] + e[_b] + e[_c]))
 
 
+# limits of the one-workgroup-per-block coarse solve (k_mg_coarse: 1,024 threads)
+MAX_BLOCK_NODES = 1024
+MAX_BLOCK_TETS = 2048
+
+
 @dataclass
 class CoarseLevel:
     pos0: np.ndarray        # (nc, 3) coarse rest positions (lattice nodes)
     tets: np.ndarray        # (nct, 4)
     mass: np.ndarray        # (nc,) sum_i P_ij m_i
-    stopped: np.ndarray     # (nc,) 1 = coarse pin
+    stopped: np.ndarray     # (nc,) 1 = coarse pin (kept at its restricted position by every sweep)
     p_idx: np.ndarray       # (nv, 8) coarse node of each trilinear corner
     p_w: np.ndarray         # (nv, 8) trilinear weights (sum to 1; 0 for vertices without a cage)
     blk_off: np.ndarray     # (nblk+1,) coarse node range of each muscle block
@@ -131,6 +136,12 @@ def build_coarse_level(pos0, tets, mass, pin_mask=None, factor: float = 4.0, ite
             return order[np.searchsorted(skeys, key(ijk))]
 
         n = len(nodes)
+        # the one-workgroup coarse solve holds a block's nodes one per thread and its
+        # tets two per thread
+        if n > MAX_BLOCK_NODES or 6 * len(used) > MAX_BLOCK_TETS:
+            raise ValueError(f"coarse cage of component {c} has {n} nodes / {6 * len(used)} tets; the "
+                             f"coarse solve takes at most {MAX_BLOCK_NODES} nodes and {MAX_BLOCK_TETS} "
+                             f"tets per block (raise mg_coarse_factor)")
         Pl = np.zeros((len(vids), n))
         for k in range(8):
             off = corner_off[k]
@@ -212,12 +223,25 @@ def restrict_numpy(level: CoarseLevel, x, rest) -> np.ndarray:
 
 
 def coarse_scene(level: CoarseLevel, params):
-    """Scene of the coarse cage: ARAP-SVD only, no gravity, masses from P."""
+    """Scene of the coarse cage: ARAP-SVD only, no gravity, masses from P.
+
+    Coarse pins follow the fine level's pin rule (MultiGridSim keeps its pinned
+    vertices in the Jacobi sums with their inverse mass and snaps them back
+    after every iteration, mg.py:1337,1355): a pinned node takes part in every
+    projection and keeps its restricted position.  Held with zero inverse mass
+    instead (``coarse_stopped``, mg.py:860-866), a tet whose other corners are
+    held moves one corner along the gradient of C = ||sigma - 1||; that step
+    overshoots the minimum of C along its line by 1/cos^2 of the angle between
+    F - R and the one-corner direction, and the sweeps amplify a perturbation
+    20-60x each."""
     from .scene import Scene, SimParams
     cons = builder.build_from_configs(level.pos0, level.tets,
                                       [dict(type="tetarapsvd", name="arap", stiffness=level.stiffness,
                                             dampingratio=0.0)])
     p = SimParams(dt=params.dt, gravity=(0.0, 0.0, 0.0), veldamping=0.0, quasi_static=False,
                   contraction_ratio=0.0, fiber_stiffness_scale=1.0)
+    pins = np.nonzero(level.stopped)[0].astype(np.int32)
     return Scene(pos0=level.pos0, tets=level.tets, density=1.0, cons=cons, params=p, mass=level.mass,
-                 stopped=level.stopped, meta=dict(workload="coarse cage"))
+                 stopped=None, pin_vids=pins if pins.size else None,
+                 pin_active=np.ones(pins.size, np.int32) if pins.size else None,
+                 meta=dict(workload="coarse cage"))

@@ -190,12 +190,11 @@ class MultiGridSim:
         cs = multigrid.coarse_scene(self.coarse, self.scene.params)
         self.coarse_engine = (engine_cls(cs, device=device, strategy=1) if engine_cls is XpbdEngine
                               else engine_cls(cs))
-        # the coarse correction is applied kinematically (also shifted into the
-        # previous positions) unless mg_kinematic_correction is false: with the
-        # reference's dynamic form the prolonged shift enters the velocity and the
-        # pin-driven workloads here gain energy every substep
+        # the reference's dynamic form by default: the prolonged shift enters the
+        # velocity (prolong_delta_kernel, mg/kernel.py:9-40); mg_kinematic_correction
+        # also shifts the previous positions, so the shift does not
         self.engine.set_coarse_level(self.coarse_engine, self.coarse,
-                                     kinematic=bool(cfg.get("mg_kinematic_correction", True)))
+                                     kinematic=bool(cfg.get("mg_kinematic_correction", False)))
 
     # kinematic drive ---------------------------------------------------------
     def load_frame_targets(self, frame: int) -> None:

@@ -215,6 +215,10 @@ def test_frames_with_dcd_equal_the_phase_composition():
                 b.dcd_solve(s == 0 and k == 0)
             b.collide()
             b.update_velocities()
+        # the fused last-pass apply in the post kernel only acts on contacts
+        na, nb = a.dcd_stats()["n_contacts"], b.dcd_stats()["n_contacts"]
+        print("contacts of the last pass", na, nb, flush=True)
+        assert na > 0 and na == nb
     pa, pb = a.positions_np(), b.positions_np()
     assert np.isfinite(pa).all() and a.dcd_stats()["n_pairs"] > 0
     assert float(np.abs(pa - x).max()) > 1e-4

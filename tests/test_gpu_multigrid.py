@@ -1,5 +1,6 @@
 """Two-level hierarchical correction (coarse lattice cage, trilinear prolongation)
-through the C ABI against the oracle's NumPy restatement."""
+through the C ABI against the oracle (its own float64 restriction, the C
+restatement's coarse sweeps)."""
 import json
 from pathlib import Path
 
@@ -13,36 +14,36 @@ pytestmark = pytest.mark.gpu
 CFG = Path(__file__).resolve().parents[1] / "rlmusclev2_amd" / "configs"
 
 
-def _setup(strategy, kinematic=True, iters=4):
-    # 4 coarse sweeps: with more, the coarse Jacobi ARAP on strongly strained
-    # cages amplifies FP32 rounding differences (GPU and oracle agree to ~1e-6
-    # per V-cycle up to 4 sweeps; at 15 sweeps they drift apart by ~10 %)
+def _setup(strategy, quasi_static=None, kinematic=None):
+    """the c3 configuration's own settings (15 coarse sweeps, its pin rule and
+    prolongation) on a small mesh"""
     cfg = json.loads((CFG / "humanv2_snh_aniso.json").read_text())
     cfg["synthetic"]["sdf_res"] = 32
-    cfg["use_multigrid"] = True
-    cfg["mg_coarse_iters"] = iters
-    cfg["mg_kinematic_correction"] = kinematic
-    cfg["quasi_static"] = False
+    assert cfg["use_multigrid"] and cfg["mg_coarse_iters"] == 15
+    if quasi_static is not None:
+        cfg["quasi_static"] = quasi_static
+    if kinematic is not None:
+        cfg["mg_kinematic_correction"] = kinematic
     w = workloads.humanv2_scale(3, n_bones=1, per_bone=4, target_tets=8000, fascia_tets=1500, sdf_res=32)
     sim = sims.MultiGridSim(cfg, workload=w, strategy=strategy, block_tets=256)
     ref = sims.MultiGridSim(cfg, workload=w, engine_cls=oracle.OracleSim)
     return cfg, w, sim, ref
 
 
-def _same_start(sim, ref, frame=1):
-    for s in (sim, ref):
-        s.load_frame_targets(frame)
-        s.engine.apply_pintotarget()
-        s.engine.integrate()
+def _diag(w):
+    return float(np.linalg.norm(w.pos0.max(0) - w.pos0.min(0)))
 
 
 @pytest.mark.parametrize("strategy", [0, 1, 2])
 def test_one_vcycle_matches_oracle(strategy):
-    """restrict -> coarse sweeps -> prolong -> pins, from the same integrated state"""
+    """restrict -> 15 coarse sweeps -> prolong -> pins, from the same integrated state"""
     cfg, w, sim, ref = _setup(strategy)
     print("coarse nodes", sim.coarse.pos0.shape[0], "coarse tets", sim.coarse.tets.shape[0],
-          "blocks", sim.coarse.n_components, flush=True)
-    _same_start(sim, ref)
+          "blocks", sim.coarse.n_components, "coarse pins", int(sim.coarse.stopped.sum()), flush=True)
+    for s in (sim, ref):
+        s.load_frame_targets(1)
+        s.engine.apply_pintotarget()
+        s.engine.integrate()
     x0 = ref.engine.positions_np()
     np.testing.assert_allclose(sim.positions_np(), x0, atol=1e-6)
     sim.engine.vcycle()
@@ -50,26 +51,52 @@ def test_one_vcycle_matches_oracle(strategy):
     p, q = sim.positions_np(), ref.positions_np()
     moved = np.abs(q - x0).max()
     err = np.abs(p - q).max()
-    print("vcycle moved", moved, "gpu-oracle", err)
+    cp, cq = sim.coarse_engine.positions_np(), ref.coarse_engine.positions_np()
+    cerr = np.abs(cp - cq).max()
+    print("vcycle moved", moved, "gpu-oracle fine", err, "coarse", cerr)
     assert moved > 1e-4
-    assert err <= 2e-3 * moved
-    cp, cq = sim.coarse_engine.positions_np(), ref.coarse_engine.positions_np()   # after an even sweep count
-    assert np.abs(cp - cq).max() <= 5e-3 * moved
+    assert err <= 1e-3 * moved
+    assert cerr <= 1e-3 * moved
+
+
+@pytest.mark.parametrize("quasi_static", [False, True])
+def test_c3_frames_with_vcycle_match_oracle(quasi_static):
+    """10 frames of the c3 settings (SNH + aniso with an activation ramp, bone
+    SDF push, 15 coarse sweeps per substep): positions within 1e-4 x bbox
+    diagonal, per-type residuals within 1 %"""
+    cfg, w, sim, ref = _setup(2, quasi_static=quasi_static)
+    for f in range(1, 11):
+        a = np.full(w.nt, min(1.0, f / 10.0), np.float32)
+        for s in (sim, ref):
+            s.set_fiber_activation(a)
+            s.step_frame(f)
+        print(f"frame {f} max |dx|", float(np.abs(sim.positions_np() - ref.positions_np()).max()), flush=True)
+    p, q = sim.positions_np(), ref.positions_np()
+    tol = 1e-4 * _diag(w)
+    err = np.abs(p - q).max()
+    print("10 frames max |dx|", err, "tol", tol)
+    assert np.all(np.isfinite(p)) and err <= tol
+    rs, rr = sim.residuals(), ref.residuals()
+    print("residuals", rs, rr)
+    assert set(rr) == {"snh", "aniso"}
+    for k in rr:
+        assert abs(rs[k] - rr[k]) <= 0.01 * rr[k] + 1e-12, (k, rs[k], rr[k])
 
 
 @pytest.mark.parametrize("kinematic", [True, False])
 def test_substeps_with_vcycle_match_oracle(kinematic):
-    cfg, w, sim, ref = _setup(1, kinematic=kinematic)
-    sim.load_frame_targets(1)
-    ref.load_frame_targets(1)
+    """the prolonged shift in the velocity (the reference's form) or also in the
+    previous positions, flat strategy, 3 substeps"""
+    cfg, w, sim, ref = _setup(1, quasi_static=False, kinematic=kinematic)
     for s in (sim, ref):
+        s.load_frame_targets(1)
         s.engine.apply_pintotarget()
-        for _ in range(2):
+        for _ in range(3):
             s.engine.native_substep(s.fine_iters)
     p, q = sim.positions_np(), ref.positions_np()
-    tol = 1e-4 * float(np.linalg.norm(w.pos0.max(0) - w.pos0.min(0)))
+    tol = 1e-4 * _diag(w)
     err = np.abs(p - q).max()
-    print("2 substeps max |dx|", err, "tol", tol)
+    print("3 substeps max |dx|", err, "tol", tol)
     assert np.all(np.isfinite(p)) and err <= tol
     rs, rr = sim.residuals(), ref.residuals()
     for k in rr:
@@ -82,8 +109,9 @@ def test_rigid_motion_restricts_exactly_and_gives_no_correction():
     th = 0.2
     R = np.array([[np.cos(th), -np.sin(th), 0.0], [np.sin(th), np.cos(th), 0.0], [0.0, 0.0, 1.0]])
     x = (w.pos0 @ R.T + np.array([0.01, 0.02, 0.0])).astype(np.float32)
-    q = multigrid.restrict_numpy(lv, x, w.pos0)
-    np.testing.assert_allclose(q, lv.pos0 @ R.T + np.array([0.01, 0.02, 0.0]), atol=1e-5)
+    expect = lv.pos0 @ R.T + np.array([0.01, 0.02, 0.0])
+    np.testing.assert_allclose(multigrid.restrict_numpy(lv, x, w.pos0), expect, atol=1e-5)
+    np.testing.assert_allclose(oracle.CageRestriction(lv)(x, w.pos0), expect, atol=1e-6)
     e = sim.engine
     e.set_state(x, x, np.zeros_like(x))
     e.set_pin_targets(x[sim.scene.pin_vids])
