@@ -242,10 +242,21 @@ int32_t xpbd_mlp_forward(const xpbd_mlp_dev *net, const float *X, int64_t n, flo
                          float *H2, void *stream);
 /* number of floats of the gradient vector: W1, b1, W2, b2, W3, b3 flattened in that order */
 int32_t xpbd_mlp_grad_size(void);
-/* d loss / d params from dY (n x OUT); scratch >= ceil(n/32) * grad_size floats */
+/* d loss / d params from dY (n x OUT); work: device workspace of >= ceil(n/32) * grad_size floats */
 int32_t xpbd_mlp_backward(const xpbd_mlp_dev *net, const float *X, const float *H1, const float *H2,
-                          const float *dY, int64_t n, float *grad, float *scratch, int64_t scratch_floats,
+                          const float *dY, int64_t n, float *grad, float *work, int64_t work_floats,
                           void *stream);
+/* xpbd_mlp_backward without the workspace argument; allocates nothing (no hipMalloc / hipFree,
+ * no allocating library call), for callers that cannot allocate device memory: code under HIP
+ * graph capture, training loops that must not allocate per step.  No reference counterpart.
+ * The workspace is a static device array sized for n <= XPBD_MLP_NOALLOC_MAX_ROWS rows (12.3 MB,
+ * held on each GPU the library runs on until the process ends); larger n returns XPBD_EINVAL
+ * before anything is launched.  Runs on the current device (stream must belong to it).  All calls
+ * on one GPU share that array: calls that overlap in time on one GPU race. */
+#define XPBD_MLP_NOALLOC_MAX_ROWS 16384
+int32_t xpbd_mlp_backward_noalloc(const xpbd_mlp_dev *net, const float *X, const float *H1,
+                                  const float *H2, const float *dY, int64_t n, float *grad,
+                                  void *stream);
 
 /* multiplier modes (flat and block strategies; no reference counterpart):
  * 0 = plain zero-lambda Jacobi (the reference GPU path, solve_kernels.py:271-281 with clear()

@@ -118,6 +118,7 @@ SIGNATURES = {
     "xpbd_mlp_forward": [C.POINTER(XpbdMlpDev), _P, C.c_int64, _P, _P, _P, _P],
     "xpbd_mlp_grad_size": [],
     "xpbd_mlp_backward": [C.POINTER(XpbdMlpDev), _P, _P, _P, _P, C.c_int64, _P, _P, C.c_int64, _P],
+    "xpbd_mlp_backward_noalloc": [C.POINTER(XpbdMlpDev), _P, _P, _P, _P, C.c_int64, _P, _P],
     "xpbd_set_coarse_level": [_P, _P, C.POINTER(XpbdCoarseDesc)],
     "xpbd_vcycle": [_P],
     "xpbd_set_lambda_mode": [_P, _I],

@@ -13,6 +13,8 @@
 #include <hip/hip_bf16.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "../../include/xpbd.h"
 
 namespace {
@@ -242,6 +244,47 @@ __global__ void k_mlp_reduce(const float *partial, int nblk, float *grad, int ng
     grad[i] = s;
 }
 
+constexpr int NG = NH * NIN + NH + NH * NH + NH + NOUT * NH + NOUT;   // gradient floats (6,020)
+
+// Workspace of xpbd_mlp_backward_noalloc: the per-block partial gradients of
+// up to XPBD_MLP_NOALLOC_MAX_ROWS rows (512 blocks x 6,020 floats, 12.3 MB on
+// each GPU the library runs on, held until the process ends).  k_mlp_bwd
+// writes every entry of the blocks it runs and k_mlp_reduce reads only those,
+// so nothing has to start at zero.  Host code must use the per-device address
+// from hipGetSymbolAddress, never this name.
+constexpr int64_t NOALLOC_BLOCKS = XPBD_MLP_NOALLOC_MAX_ROWS / 32;
+static __device__ __align__(256) float scratch[NOALLOC_BLOCKS * NG];
+
+constexpr int MAX_DEVICES = 64;
+struct ScratchSlot {
+    std::once_flag once;
+    float *ptr = nullptr;
+    hipError_t err = hipSuccess;
+};
+ScratchSlot g_scratch[MAX_DEVICES];
+
+// device address of the workspace on the current device, looked up once per
+// device (each GPU holds its own copy of the array)
+int32_t scratch_on_current_device(float **out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return XPBD_EHIP;
+    ScratchSlot &s = g_scratch[dev];
+    std::call_once(s.once, [&s] { s.err = hipGetSymbolAddress((void **)&s.ptr, HIP_SYMBOL(scratch)); });
+    if (s.err != hipSuccess || !s.ptr) return XPBD_EHIP;
+    *out = s.ptr;
+    return XPBD_OK;
+}
+
+int32_t launch_backward(const xpbd_mlp_dev *net, const float *X, const float *H1, const float *H2,
+                        const float *dY, int64_t n, float *grad, float *partial, void *stream) {
+    const int64_t nblk = (n + 31) / 32;
+    hipLaunchKernelGGL(k_mlp_bwd, dim3((unsigned)nblk), dim3(64), 0, (hipStream_t)stream, *net, X, H1, H2, dY, n,
+                       partial);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((unsigned)((NG + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       (const float *)partial, (int)nblk, grad, NG);
+    return hipGetLastError() == hipSuccess ? XPBD_OK : XPBD_EHIP;
+}
+
 }  // namespace
 
 extern "C" {
@@ -256,20 +299,25 @@ int32_t xpbd_mlp_forward(const xpbd_mlp_dev *net, const float *X, int64_t n, flo
     return hipGetLastError() == hipSuccess ? XPBD_OK : XPBD_EHIP;
 }
 
-int32_t xpbd_mlp_grad_size(void) { return NH * NIN + NH + NH * NH + NH + NOUT * NH + NOUT; }
+int32_t xpbd_mlp_grad_size(void) { return NG; }
 
 int32_t xpbd_mlp_backward(const xpbd_mlp_dev *net, const float *X, const float *H1, const float *H2,
-                          const float *dY, int64_t n, float *grad, float *scratch, int64_t scratch_floats,
+                          const float *dY, int64_t n, float *grad, float *work, int64_t work_floats,
                           void *stream) {
-    if (!net || !X || !H1 || !H2 || !dY || !grad || !scratch || n <= 0) return XPBD_EINVAL;
-    const int ng = xpbd_mlp_grad_size();
+    if (!net || !X || !H1 || !H2 || !dY || !grad || !work || n <= 0) return XPBD_EINVAL;
     const int64_t nblk = (n + 31) / 32;
-    if (nblk * ng > scratch_floats) return XPBD_EINVAL;
-    hipLaunchKernelGGL(k_mlp_bwd, dim3((unsigned)nblk), dim3(64), 0, (hipStream_t)stream, *net, X, H1, H2, dY, n,
-                       scratch);
-    hipLaunchKernelGGL(k_mlp_reduce, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
-                       (const float *)scratch, (int)nblk, grad, ng);
-    return hipGetLastError() == hipSuccess ? XPBD_OK : XPBD_EHIP;
+    if (nblk * NG > work_floats) return XPBD_EINVAL;
+    return launch_backward(net, X, H1, H2, dY, n, grad, work, stream);
+}
+
+int32_t xpbd_mlp_backward_noalloc(const xpbd_mlp_dev *net, const float *X, const float *H1, const float *H2,
+                                  const float *dY, int64_t n, float *grad, void *stream) {
+    if (!net || !X || !H1 || !H2 || !dY || !grad || n <= 0) return XPBD_EINVAL;
+    if (n > XPBD_MLP_NOALLOC_MAX_ROWS) return XPBD_EINVAL;   // beyond the static workspace
+    float *ws = nullptr;
+    const int32_t r = scratch_on_current_device(&ws);
+    if (r) return r;
+    return launch_backward(net, X, H1, H2, dY, n, grad, ws, stream);
 }
 
 }  // extern "C"
