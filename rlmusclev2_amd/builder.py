@@ -138,12 +138,17 @@ def create_tet_snh(tets, valid, volumes, params) -> ConsTable:
                       params.get("dampingratio", 0.0), restdir0=params.get("lam", 10000.0))
 
 
-def create_attach(vertices, targets, stiffness, dampingratio, restlength=0.0) -> ConsTable:
-    """ATTACH per vertex (MultiGridSim._build_fine_belly_attach, mg.py:700-723)."""
+def create_attach(vertices, targets, stiffness, dampingratio, restlength=0.0, target_ids=None,
+                  tetid=None) -> ConsTable:
+    """ATTACH per vertex: pts = [v, -1, target id or -1, -1], target in restvector
+    (MultiGridSim._build_fine_belly_attach, mg.py:700-723; create_attach_constraints,
+    constraint/builder.py:347-398)."""
     v = np.asarray(vertices, np.int32)
     n = v.shape[0]
     pts = np.full((n, 4), -1, np.int32)
     pts[:, 0] = v
+    if target_ids is not None:
+        pts[:, 2] = np.asarray(target_ids, np.int32)
     rv = np.zeros((n, 4), np.float32)
     rv[:, 3] = 1.0
     if targets is not None:
@@ -152,8 +157,10 @@ def create_attach(vertices, targets, stiffness, dampingratio, restlength=0.0) ->
         type=np.full(n, int(ConstraintType.ATTACH), np.int32), pts=pts,
         stiffness=np.full(n, stiffness, np.float32),
         compressionstiffness=np.full(n, -1.0, np.float32),
-        dampingratio=np.full(n, dampingratio, np.float32), tetid=np.full(n, -1, np.int32),
-        L=np.zeros((n, 3), np.float32), restlength=np.full(n, restlength, np.float32),
+        dampingratio=np.full(n, dampingratio, np.float32),
+        tetid=np.full(n, -1, np.int32) if tetid is None else np.asarray(tetid, np.int32),
+        L=np.zeros((n, 3), np.float32),
+        restlength=np.broadcast_to(np.asarray(restlength, np.float32), (n,)).copy(),
         restvector=rv, restdir=np.zeros((n, 3), np.float32))
 
 
@@ -189,10 +196,15 @@ def build_from_configs(pos0, tets, constraint_configs) -> ConsTable:
             parts.append(create_tet_arap(tets, valid, volumes, params, ConstraintType.TETARAPANISO))
         elif ctype == "snh":
             parts.append(create_tet_snh(tets, valid, volumes, params))
+        elif ctype in ("attach", "pintotarget"):
+            raise NotImplementedError(
+                f"constraint type {ctype!r} needs per-vertex targets: MultiGridSim builds ATTACH from its "
+                "'belly_attach' entry, MuscleSim.from_procedural from bone_targets and a vertex mask "
+                "(create_attach / create_pintotarget take explicit targets)")
         else:
             raise NotImplementedError(
                 f"constraint type {ctype!r} is not on the HIP substep path "
-                "(supported: tetarapsvd, volume, snh, aniso_arap, attach, pintotarget)")
+                "(tet types here: tetarapsvd, volume, snh, aniso_arap)")
     return ConsTable.concat(parts).sorted_by_type().contiguous()
 
 

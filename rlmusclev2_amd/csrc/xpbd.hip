@@ -2474,7 +2474,10 @@ struct xpbd_ctx {
     int n_pin_dev = 0, pin_cap = 0;
     int32_t *d_pin_dev = nullptr;    // device vertex ids of targeted active pins (pin order)
     float4 *d_pin_tgt = nullptr;
-    HostStage st_pin, st_act, st_bone;
+    // per-frame attach targets: staged compactly (attach order) and scattered into ATT0
+    int32_t *d_att_dev = nullptr;    // device vertex of attach constraint #i (table order)
+    float4 *d_att_tgt = nullptr;
+    HostStage st_pin, st_act, st_bone, st_att;
     // sdf
     int nbones = 0, res = 0;
     std::vector<float> sdf_grids, sdf_origin, sdf_invh, coll_w, bone_xf;
@@ -2621,6 +2624,8 @@ void free_device(xpbd_ctx *c) {
     c->T_X = nullptr;
     c->d_pin_dev = nullptr;
     c->d_pin_tgt = nullptr;
+    c->d_att_dev = nullptr;
+    c->d_att_tgt = nullptr;
     c->pin_cap = 0;
 }
 
@@ -2754,6 +2759,18 @@ __global__ __launch_bounds__(256) void k_pin_scatter(int n, const int32_t *__res
                                                      const float4 *__restrict__ tgt, float4 *__restrict__ pin) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) pin[dev[i]] = tgt[i];
+}
+
+// per-frame attach targets (_update_belly_attach_targets, mg.py:725-734): the
+// target xyz of each attach record, its stiffness (w) kept
+__global__ __launch_bounds__(256) void k_att_scatter(int n, const int32_t *__restrict__ dev,
+                                                     const float4 *__restrict__ tgt, float4 *__restrict__ att0) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        const float4 t = tgt[i];
+        float4 *a = att0 + dev[i];
+        *a = make_float4(t.x, t.y, t.z, a->w);
+    }
 }
 
 std::vector<DevClass> dev_classes(const std::vector<ClassParams> &cls, const SimParams &sp) {
@@ -3267,6 +3284,7 @@ void xpbd_destroy(xpbd_ctx *c) {
     free_stage(c->st_pin);
     free_stage(c->st_act);
     free_stage(c->st_bone);
+    free_stage(c->st_att);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
 }
@@ -3398,15 +3416,23 @@ int32_t xpbd_set_attach_targets(xpbd_ctx *c, int32_t n, const float *xyz) {
     if (r) return r;
     if (n != (int)c->L.attach_vertex.size() || (n > 0 && !xyz))
         return fail(c, XPBD_EINVAL, "attach target count mismatch");
-    for (int i = 0; i < n; ++i) {
+    if (n == 0) return XPBD_OK;
+    for (int i = 0; i < n; ++i) {   // host copy: a later finalize uploads the current targets
         float *a0 = &c->L.att0[4 * (size_t)c->L.attach_vertex[i]];
         a0[0] = xyz[3 * i];
         a0[1] = xyz[3 * i + 1];
         a0[2] = xyz[3 * i + 2];
     }
-    HIPCHK(c, hipMemcpyAsync(c->ATT0, c->L.att0.data(), c->L.att0.size() * sizeof(float), hipMemcpyHostToDevice,
-                             c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // staged in pinned memory and scattered on the device: no host wait
+    int slot;
+    void *h;
+    if ((r = stage_acquire(c, c->st_att, (size_t)n * sizeof(float4), slot, h))) return r;
+    float4 *t = (float4 *)h;
+    for (int i = 0; i < n; ++i) t[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0.0f);
+    if ((r = stage_submit(c, c->st_att, slot, c->d_att_tgt, (size_t)n * sizeof(float4)))) return r;
+    hipLaunchKernelGGL(k_att_scatter, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n,
+                       (const int32_t *)c->d_att_dev, (const float4 *)c->d_att_tgt, c->ATT0);
+    HIPCHK(c, hipGetLastError());
     return XPBD_OK;
 }
 
@@ -3627,6 +3653,10 @@ int32_t xpbd_finalize(xpbd_ctx *c, const xpbd_tuning *t) {
     if ((r = dalloc(c, &c->FLAGS, nv))) return r;
     r = dupload(c, &c->ATT0, (const float4 *)L.att0.data(), (size_t)nv); if (r) return r;
     r = dupload(c, &c->ATT1, (const float4 *)L.att1.data(), (size_t)nv); if (r) return r;
+    if (!L.attach_vertex.empty()) {
+        UP(c->d_att_dev, L.attach_vertex);
+        if ((r = dalloc(c, &c->d_att_tgt, L.attach_vertex.size()))) return r;
+    }
     UP(c->d_patch, L.patches);
     UP(c->d_halo, L.halo);
     if (c->flat) {

@@ -63,6 +63,7 @@ class Scene:
     pin_active: np.ndarray | None = None
     sdf: SdfColliders | None = None
     dcd: object | None = None                  # collision.DcdSpec
+    attach_vids: np.ndarray | None = None      # vertex of each ATTACH constraint, table order
     meta: dict = field(default_factory=dict)
 
     @property

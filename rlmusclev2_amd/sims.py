@@ -97,6 +97,25 @@ def scene_from_config(cfg: dict, w: workloads.Workload) -> Scene:
         pin_vids = w.pin_vids
         region = str(pin.get("region", "tips")).lower()
         pin_active = np.ones(len(pin_vids), np.int32) if region != "none" else np.zeros(len(pin_vids), np.int32)
+    # belly attach (mg.py:466-469, 700-734): every muscle vertex that is not an
+    # active hard pin gets an ATTACH toward its per-frame bone goal (fascia is a
+    # separate sub-solver in the reference and gets none)
+    attach_vids = None
+    belly = find_constraint(cfg, "belly_attach")
+    if belly is not None:
+        free = np.ones(w.nv, bool)
+        if w.is_fascia is not None:
+            free &= ~np.asarray(w.is_fascia, bool)
+        if pin_vids is not None:
+            free[pin_vids[pin_active != 0]] = False
+        if w.stopped is not None:
+            free &= np.asarray(w.stopped) == 0
+        if w.vertex_group is None and w.bones:
+            raise ValueError("belly_attach needs a bone per vertex (workload vertex_group)")
+        attach_vids = np.nonzero(free)[0].astype(np.int32)
+        att = builder.create_attach(attach_vids, pos0[attach_vids], float(belly["stiffness"]),
+                                    float(belly.get("dampingratio", 0.0)))
+        cons = builder.ConsTable.concat([cons, att]).sorted_by_type().contiguous()
     sdf = None
     coll = cfg.get("demo", {}).get("collision", {})
     if coll.get("enabled", False) and w.bones:
@@ -132,9 +151,12 @@ def scene_from_config(cfg: dict, w: workloads.Workload) -> Scene:
                                 iters=int(sc.get("iters", 2)), rebuild_substeps=bool(sc.get("rebuild_substeps", False)),
                                 max_pairs=int(sc.get("max_pairs", 1 << 22)))
         dcd = collision.DcdSpec(tris=collision.surface_tris(tets[sel]), group=None, params=p)
+    if attach_vids is not None:
+        ccfg = ccfg + [dict(type="attach", name="belly_attach", stiffness=float(belly["stiffness"]),
+                            dampingratio=float(belly.get("dampingratio", 0.0)))]
     return Scene(pos0=pos0, tets=tets, density=density, cons=cons, params=params, v_fiber_dir=vf,
                  tendonmask=tm, stopped=w.stopped, pin_vids=pin_vids, pin_active=pin_active, sdf=sdf,
-                 dcd=dcd, meta=dict(workload=w.name, constraint_configs=ccfg))
+                 dcd=dcd, attach_vids=attach_vids, meta=dict(workload=w.name, constraint_configs=ccfg))
 
 
 class MultiGridSim:
@@ -198,10 +220,13 @@ class MultiGridSim:
 
     # kinematic drive ---------------------------------------------------------
     def load_frame_targets(self, frame: int) -> None:
-        """Per-frame bone pose -> pin targets and SDF transforms (mg.py:1233-1249)."""
+        """Per-frame bone pose -> pin targets, belly-attach targets and SDF
+        transforms (mg.py:1233-1249)."""
         w = self.workload
         if self.scene.pin_vids is not None and w.bones:
             self.engine.set_pin_targets(workloads.pin_targets(w, frame))
+        if self.scene.attach_vids is not None and w.bones:
+            self.engine.set_attach_targets(workloads.follow_targets(w, frame, self.scene.attach_vids))
         if self.scene.sdf is not None:
             self.engine.set_bone_transforms(workloads.bone_xforms(w, frame))
         self._frame = frame
@@ -254,6 +279,36 @@ class MultiGridSim:
         return self.engine.residuals()
 
 
+def _procedural_attach(v, t, bone, attrs, params):
+    """create_attach_constraints (constraint/builder.py:347-398) for a procedural
+    mesh: mask -> sources, nearest bone target (scipy cKDTree, as the reference),
+    rest length = distance, tetid = the first tet holding the source."""
+    import warnings
+    from scipy.spatial import cKDTree
+    name = params.get("mask_name")
+    mask = attrs.get(name) if name is not None else None
+    if mask is None:
+        warnings.warn(f"mask '{name}' not found in geometry.")
+        return None
+    src = np.nonzero(np.asarray(mask, np.float32) > float(params.get("mask_threshold", 0.75)))[0].astype(np.int32)
+    if src.size == 0:
+        warnings.warn(f"No vertices with mask > {params.get('mask_threshold', 0.75)}")
+        return None
+    if params.get("target_group") is not None or params.get("source_nearest_group") is not None:
+        # bone vertex groups come from a loaded bone geometry; a procedural mesh has none
+        warnings.warn(f"bone target groups not found: {params.get('target_group') or params.get('source_nearest_group')}")
+        return None
+    if bone.shape[0] == 0:
+        return None
+    dist, tgt = cKDTree(bone).query(v[src], k=1)
+    first = np.full(v.shape[0], t.shape[0], np.int64)
+    np.minimum.at(first, t.ravel(), np.repeat(np.arange(t.shape[0]), 4))
+    tetid = np.where(first[src] < t.shape[0], first[src], -1)
+    return builder.create_attach(src, bone[tgt], float(params.get("stiffness", 1e10)),
+                                 float(params.get("dampingratio", 0.0)), restlength=dist.astype(np.float32),
+                                 target_ids=tgt, tetid=tetid)
+
+
 class MuscleSim:
     """MuscleSim program on the HIP substep: 1 Jacobi iteration per substep."""
 
@@ -269,21 +324,41 @@ class MuscleSim:
     @classmethod
     def from_procedural(cls, vertices, tets, fiber_dirs_per_tet, bone_targets=None, *,
                         constraint_configs, dts, num_substeps=1, device=0, density=1060.0,
-                        veldamping=0.02, gravity=0.0, contraction_ratio=0.0,
-                        fiber_stiffness_scale=1.0, stopped=None, engine_cls=None, **_unused):
-        """musclesim.py:181-287 (gravity is the y component, as the reference engine)."""
+                        veldamping=0.02, gravity=0.0, sigma0=0.0, lambda_opt=1.0, contraction_ratio=0.0,
+                        fiber_stiffness_scale=1.0, v_max_norm=0.0, d_damp=0.0, stopped=None, engine_cls=None,
+                        vertex_attrs=None):
+        """musclesim.py:181-287 (gravity is the y component, as the reference engine).
+
+        ``attach`` entries follow create_attach_constraints (constraint/builder.py:347-398):
+        the vertices whose ``mask_name`` attribute exceeds ``mask_threshold`` attach to their
+        nearest point of ``bone_targets`` (rest length = that distance).  The reference reads
+        the mask from the loaded geometry, which a procedural mesh does not have, so it warns
+        and adds none; ``vertex_attrs`` (name -> per-vertex array, no reference counterpart)
+        supplies such attributes here.  ``sigma0 > 0`` (the explicit active fibre force,
+        musclesim.py:400-412) is not on the HIP path and raises; ``lambda_opt``,
+        ``v_max_norm`` and ``d_damp`` only act through it."""
+        if float(sigma0) > 0.0:
+            raise NotImplementedError("sigma0 > 0: the explicit active fibre force (musclesim.py:400-412) "
+                                      "is not on the HIP substep path")
         v = np.asarray(vertices, np.float32)
         t = np.asarray(tets, np.int32)
         vf = builder.vertex_fiber_from_tets(v.shape[0], t, fiber_dirs_per_tet)
-        cons = builder.build_from_configs(v, t, [c for c in constraint_configs
-                                                 if str(c["type"]).lower() != "attach"])
+        kinds = [str(c["type"]).lower() for c in constraint_configs]
+        parts = [builder.build_from_configs(v, t, [c for c, k in zip(constraint_configs, kinds) if k != "attach"])]
+        bone = np.zeros((0, 3), np.float32) if bone_targets is None else np.asarray(bone_targets, np.float32)
+        for c, k in zip(constraint_configs, kinds):
+            if k == "attach" and c.get("enable", True):
+                parts.append(_procedural_attach(v, t, bone, vertex_attrs or {}, c))
+        cons = builder.ConsTable.concat(parts).sorted_by_type().contiguous()
         params = SimParams(dt=float(dts) / num_substeps, gravity=(0.0, float(gravity), 0.0),
                            veldamping=float(veldamping), contraction_ratio=float(contraction_ratio),
                            fiber_stiffness_scale=float(fiber_stiffness_scale))
         scene = Scene(pos0=v, tets=t, density=float(density), cons=cons, params=params,
                       v_fiber_dir=vf, tendonmask=np.zeros(t.shape[0], np.float32),
                       stopped=None if stopped is None else np.asarray(stopped, np.int32))
-        return cls(scene, num_substeps, device=device, engine_cls=engine_cls)
+        sim = cls(scene, num_substeps, device=device, engine_cls=engine_cls)
+        sim.bone_pos = bone
+        return sim
 
     def step(self):
         """integrate_phase, solve_phase, finalize_phase per substep (musclesim.py:107-142)."""

@@ -344,6 +344,30 @@ def pin_targets(w: Workload, frame: int) -> np.ndarray:
     return out.astype(np.float32)
 
 
+def follow_targets(w: Workload, frame: int, vids: np.ndarray) -> np.ndarray:
+    """Rest positions of ``vids`` moved rigidly by the bone of each vertex
+    (``vertex_group``): the belly-attach goal target_pos + rest_offset of
+    MultiGridSim._update_belly_attach_targets (mg.py:725-734) for a synthetic
+    scene whose targets ride the bones."""
+    key = ("_follow", id(vids))
+    cache = w.info.get(key)
+    if cache is None or cache[0] is not vids:
+        v = np.asarray(vids, np.int64)
+        grp = np.asarray(w.vertex_group, np.int64)[v]
+        order = np.argsort(grp, kind="stable")
+        cuts = np.searchsorted(grp[order], np.arange(len(w.bones) + 1))
+        cache = (vids, w.pos0[v].astype(np.float64), [order[cuts[b]:cuts[b + 1]] for b in range(len(w.bones))])
+        w.info[key] = cache
+    x0, groups = cache[1], cache[2]
+    out = x0.copy()
+    for b, bn in enumerate(w.bones):
+        sel = groups[b]
+        if sel.size:
+            R, t = bone_transform(bn, frame)
+            out[sel] = x0[sel] @ R.T + t
+    return out.astype(np.float32)
+
+
 def batch(seed: int = 0, n_inst: int = 512, lo: int = 2000, hi: int = 40000, h: float = 0.004,
           sdf_res: int = 64) -> Workload:
     """Independent spindle instances, tet count log-uniform in [lo, hi], each with

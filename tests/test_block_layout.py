@@ -26,3 +26,18 @@ def test_block_layout_sums_match_flat(checker, n, tb):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "bad 0" in r.stdout
+
+
+def test_attach_records_and_duplicate_rejection(tmp_path):
+    """ATTACH records follow their vertex through the renumbering; a second
+    ATTACH on one vertex is rejected with XPBD_EUNSUPPORTED"""
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    exe = tmp_path / "check_attach_layout"
+    subprocess.run(["g++", "-O2", "-std=c++17", f"-I{ROOT / 'include'}", f"-I{ROOT / 'rlmusclev2_amd' / 'csrc'}",
+                    str(ROOT / "tests" / "cpp" / "check_attach_layout.cpp"),
+                    str(ROOT / "rlmusclev2_amd" / "csrc" / "xpbd_layout.cpp"), "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "bad 0" in r.stdout
