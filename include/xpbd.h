@@ -100,6 +100,13 @@ int32_t xpbd_set_force(xpbd_ctx *ctx, const float *force);
 int32_t xpbd_set_fiber(xpbd_ctx *ctx, const float *v_fiber_dir, const float *tendonmask);
 /* per-tet activation (nt); MultiGridSim.set_fiber_activation (mg.py:1098-1111) */
 int32_t xpbd_set_activation(xpbd_ctx *ctx, const float *activation);
+/* activation by group, for batched instances driven by one activation each: tet_group (nt ids in
+ * [0, ngroups)) once, then per frame xpbd_set_group_activation with ngroups values, expanded to
+ * the tets on the device (ngroups floats cross PCIe instead of nt).  Same result as
+ * xpbd_set_activation with activation[t] = act[tet_group[t]].  No reference counterpart (the
+ * reference sets activation per tet, mg.py:1098-1111). */
+int32_t xpbd_set_activation_groups(xpbd_ctx *ctx, int32_t ngroups, const int32_t *tet_group);
+int32_t xpbd_set_group_activation(xpbd_ctx *ctx, int32_t ngroups, const float *act);
 /* the SoA constraint table of XpbdEngine._upload_cons (engine.py:1179-1211);
  * pts n*4, restvector n*4, restdir n*3, others n. */
 int32_t xpbd_upload_constraints(xpbd_ctx *ctx, int32_t n, const int32_t *type,

@@ -195,6 +195,12 @@ class OracleSim:
     def set_fiber_activation(self, act):
         self.activation[:] = np.asarray(act, np.float32)
 
+    def set_activation_groups(self, tet_group):
+        self._act_group = np.asarray(tet_group, np.int64)
+
+    def set_group_activation(self, act):
+        self.activation[:] = np.asarray(act, np.float32)[self._act_group]
+
     def set_attach_targets(self, xyz):
         att = np.nonzero(self.cons.type == ORC_ATTACH)[0]
         self.cons.restvector[att, :3] = np.asarray(xyz, np.float32).reshape(-1, 3)

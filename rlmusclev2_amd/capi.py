@@ -84,6 +84,8 @@ SIGNATURES = {
     "xpbd_set_force": [_P, _P],
     "xpbd_set_fiber": [_P, _P, _P],
     "xpbd_set_activation": [_P, _P],
+    "xpbd_set_activation_groups": [_P, _I, _P],
+    "xpbd_set_group_activation": [_P, _I, _P],
     "xpbd_upload_constraints": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "xpbd_set_attach_targets": [_P, _I, _P],
     "xpbd_set_params": [_P, C.POINTER(XpbdParams)],

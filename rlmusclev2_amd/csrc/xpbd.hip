@@ -921,14 +921,29 @@ struct WarmNet {
     const float *b1, *b2, *W3, *b3, *fmean, *finv, *oscale;
 };
 
-// base: lambda_prev in, lambda0 out (the base of the next substep); lam: lambda0 out
+// LDS written by one lane of a wavefront and read by another lane of the same
+// wavefront: complete the writes and keep the compiler from moving the reads
+// above them (no workgroup barrier: waves work on their own LDS rows)
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// base: lambda_prev in, lambda0 out (the base of the next substep); lam: lambda0 out.
+// Each wavefront runs its own 32-element tiles start to finish (its own LDS
+// rows, wavefront-level synchronisation only), so the waves of a workgroup
+// overlap one another's gathers with their matrix work.
 __global__ __launch_bounds__(256) void k_warm_start(IterArgs a, WarmNet net, const float4 *__restrict__ X,
                                                     float4 *base, float4 *__restrict__ lam) {
     constexpr int NI = XPBD_MLP_IN, NH = XPBD_MLP_HIDDEN, NO = XPBD_MLP_OUT;
     __shared__ uint16_t sW1[NH * NI], sW2[NH * NH];
-    __shared__ float sb1[NH], sb2[NH], sW3[NO * NH], sb3[NO], smu[NI], sinv[NI], sos[NO];
-    constexpr int NHP = NH + 1;   // padded rows: a lane reads its own row without bank conflicts
-    __shared__ float sH[4][32 * NHP];
+    __shared__ __align__(16) float sW3[NO * NH];
+    __shared__ float sb1[NH], sb2[NH], sb3[NO], smu[NI], sinv[NI], sos[NO];
+    // activation rows padded to 68 floats: 16-byte aligned rows for the layer-3
+    // float4 reads, rows of the two k-halves 16 banks apart for the writes
+    constexpr int NHP = NH + 4;
+    __shared__ __align__(16) float sH[4][32 * NHP];
     const int tid = threadIdx.x;
     for (int i = tid; i < NH * NI; i += 256) sW1[i] = net.W1[i];
     for (int i = tid; i < NH * NH; i += 256) sW2[i] = net.W2[i];
@@ -939,9 +954,10 @@ __global__ __launch_bounds__(256) void k_warm_start(IterArgs a, WarmNet net, con
     __syncthreads();
     const int wave = tid >> 6, l = tid & 63, col = l & 31, h = l >> 5;
     float *H = sH[wave];
-    const int64_t ntile = (a.nptet + 127) / 128;
-    for (int64_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
-        const int64_t row = tile * 128 + wave * 32 + col;
+    const int64_t nwt = (a.nptet + 31) / 32;                    // 32-element tiles
+    const int64_t wstride = (int64_t)gridDim.x * 4;
+    for (int64_t wt = (int64_t)blockIdx.x * 4 + wave; wt < nwt; wt += wstride) {
+        const int64_t row = wt * 32 + col;
         const bool rv = row < a.nptet;
         float f[24];
         if (rv) {
@@ -950,6 +966,7 @@ __global__ __launch_bounds__(256) void k_warm_start(IterArgs a, WarmNet net, con
 #pragma unroll
             for (int i = 0; i < 24; ++i) f[i] = 0.f;
         }
+        const float4 lp = rv ? base[row] : make_float4(0.f, 0.f, 0.f, 0.f);
         // layer 1: K = 24 padded to 32, two k-steps of 16; this lane holds k = 16s + 8h + i
         ws_f32x16 acc[2];
 #pragma unroll
@@ -977,6 +994,7 @@ __global__ __launch_bounds__(256) void k_warm_start(IterArgs a, WarmNet net, con
                 acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[t], 0, 0, 0);
             }
         }
+        wave_lds_sync();   // the previous tile's layer-3 reads of H are done
 #pragma unroll
         for (int t = 0; t < 2; ++t)
 #pragma unroll
@@ -984,7 +1002,7 @@ __global__ __launch_bounds__(256) void k_warm_start(IterArgs a, WarmNet net, con
                 const int c = 32 * t + col;
                 H[((i >> 2) * 8 + 4 * h + (i & 3)) * NHP + c] = fmaxf(acc[t][i] + sb1[c], 0.f);
             }
-        __syncthreads();
+        wave_lds_sync();
         // layer 2: K = 64, four k-steps
 #pragma unroll
         for (int t = 0; t < 2; ++t)
@@ -1004,7 +1022,7 @@ __global__ __launch_bounds__(256) void k_warm_start(IterArgs a, WarmNet net, con
                 acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[t], 0, 0, 0);
             }
         }
-        __syncthreads();
+        wave_lds_sync();
 #pragma unroll
         for (int t = 0; t < 2; ++t)
 #pragma unroll
@@ -1012,31 +1030,38 @@ __global__ __launch_bounds__(256) void k_warm_start(IterArgs a, WarmNet net, con
                 const int c = 32 * t + col;
                 H[((i >> 2) * 8 + 4 * h + (i & 3)) * NHP + c] = fmaxf(acc[t][i] + sb2[c], 0.f);
             }
-        __syncthreads();
-        // layer 3 on the vector ALUs: this lane computes outputs 2h and 2h+1 of its row
+        wave_lds_sync();
+        // layer 3 on the vector ALUs: this lane computes outputs 2h and 2h+1 of
+        // its row, reading the row and the two W3 rows four floats at a time
         if (rv) {
-            float y[2];
+            const float4 *hr = reinterpret_cast<const float4 *>(H + col * NHP);
+            const float4 *w0 = reinterpret_cast<const float4 *>(sW3 + (2 * h) * NH);
+            const float4 *w1 = reinterpret_cast<const float4 *>(sW3 + (2 * h + 1) * NH);
+            float y0 = sb3[2 * h], y1 = sb3[2 * h + 1];
 #pragma unroll
-            for (int oo = 0; oo < 2; ++oo) {
-                const int o = 2 * h + oo;
-                float v = sb3[o];
-                for (int k = 0; k < NH; ++k) v += H[col * NHP + k] * sW3[o * NH + k];
-                y[oo] = v;
+            for (int q = 0; q < NH / 4; ++q) {
+                const float4 hv = hr[q], a0 = w0[q], a1 = w1[q];
+                y0 += hv.x * a0.x;
+                y0 += hv.y * a0.y;
+                y0 += hv.z * a0.z;
+                y0 += hv.w * a0.w;
+                y1 += hv.x * a1.x;
+                y1 += hv.y * a1.y;
+                y1 += hv.z * a1.z;
+                y1 += hv.w * a1.w;
             }
-            const float4 lp = base[row];
             float2 v;
             if (h == 0) {
-                const float s0 = fminf(fmaxf(1.0f + y[0] * sos[0], 0.0f), 2.0f);
-                const float s1 = fminf(fmaxf(1.0f + y[1] * sos[1], 0.0f), 2.0f);
+                const float s0 = fminf(fmaxf(1.0f + y0 * sos[0], 0.0f), 2.0f);
+                const float s1 = fminf(fmaxf(1.0f + y1 * sos[1], 0.0f), 2.0f);
                 v = make_float2(lp.x * s0, lp.y * s1);
             } else {
-                const float s2 = fminf(fmaxf(1.0f + y[0] * sos[2], 0.0f), 2.0f);
+                const float s2 = fminf(fmaxf(1.0f + y0 * sos[2], 0.0f), 2.0f);
                 v = make_float2(lp.z * s2, 0.0f);
             }
             reinterpret_cast<float2 *>(lam + row)[h] = v;
             reinterpret_cast<float2 *>(base + row)[h] = v;
         }
-        __syncthreads();
     }
 }
 
@@ -1577,6 +1602,16 @@ __global__ void k_refresh_fiber(int64_t nptet, const float4 *fstat, const int32_
     if (i >= nptet) return;
     float4 f = fstat[i];
     f.w = 1.0f - f.w * act[tetid[i]] * cr;
+    fib[i] = f;
+}
+
+// the same from per-group activations: the element's tet's group selects the value
+__global__ void k_refresh_fiber_groups(int64_t nptet, const float4 *fstat, const int32_t *egroup,
+                                       const float *gact, float cr, float4 *fib) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nptet) return;
+    float4 f = fstat[i];
+    f.w = 1.0f - f.w * gact[egroup[i]] * cr;
     fib[i] = f;
 }
 
@@ -2457,6 +2492,14 @@ struct xpbd_ctx {
     // host definition
     int nv = 0, nt = 0;
     std::vector<float> pos0, rest_volume, rest_matrix, mass, force, fiber, tendonmask, activation;
+    // activation by group (xpbd_set_activation_groups): per-tet group, the groups' values, whether
+    // the host copy of the per-tet activation is stale (the groups are expanded on the device)
+    std::vector<int32_t> act_group;
+    std::vector<float> group_act;
+    int n_act_groups = 0;
+    bool act_host_stale = false;
+    int32_t *d_elem_group = nullptr;   // per element: its tet's group
+    float *d_gact = nullptr;
     std::vector<int32_t> tets, stopped;
     bool has_force = false;
     // constraint table copy
@@ -2477,7 +2520,7 @@ struct xpbd_ctx {
     // per-frame attach targets: staged compactly (attach order) and scattered into ATT0
     int32_t *d_att_dev = nullptr;    // device vertex of attach constraint #i (table order)
     float4 *d_att_tgt = nullptr;
-    HostStage st_pin, st_act, st_bone, st_att;
+    HostStage st_pin, st_act, st_bone, st_att, st_gact;
     // sdf
     int nbones = 0, res = 0;
     std::vector<float> sdf_grids, sdf_origin, sdf_invh, coll_w, bone_xf;
@@ -2626,6 +2669,8 @@ void free_device(xpbd_ctx *c) {
     c->d_pin_tgt = nullptr;
     c->d_att_dev = nullptr;
     c->d_att_tgt = nullptr;
+    c->d_elem_group = nullptr;
+    c->d_gact = nullptr;
     c->pin_cap = 0;
 }
 
@@ -3143,8 +3188,9 @@ int launch_warm_start(xpbd_ctx *c) {
         if (!c->mlp_ready) return fail(c, XPBD_EINVAL, "lambda mode 2 needs xpbd_set_mlp()");
         IterArgs a = make_args(c, c->DPW);
         WarmNet net{c->mW1, c->mW2, c->mb1, c->mb2, c->mW3, c->mb3, c->fmean, c->finv, c->oscale};
+        // four 32-element tiles per workgroup pass; 2,048 workgroups walk the rest grid-stride
         const int64_t ntile = (ne + 127) / 128;
-        const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ntile, 1024));
+        const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ntile, 2048));
         hipLaunchKernelGGL(k_warm_start, dim3(grid), dim3(256), 0, c->stream, a, net, (const float4 *)c->X[0],
                            c->LAMB0, c->LAMB);
         c->impulse_pending = true;
@@ -3285,6 +3331,7 @@ void xpbd_destroy(xpbd_ctx *c) {
     free_stage(c->st_act);
     free_stage(c->st_bone);
     free_stage(c->st_att);
+    free_stage(c->st_gact);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
 }
@@ -3369,9 +3416,67 @@ int32_t xpbd_set_fiber(xpbd_ctx *c, const float *vf, const float *tm) {
     return XPBD_OK;
 }
 
+// per-tet activation on the host from the last group values (only when a path needs it)
+void sync_host_activation(xpbd_ctx *c) {
+    if (!c->act_host_stale) return;
+    for (int t = 0; t < c->nt; ++t) c->activation[t] = c->group_act[c->act_group[t]];
+    c->act_host_stale = false;
+}
+
+// device tables of the activation groups (after every finalize)
+int upload_act_groups(xpbd_ctx *c) {
+    if (c->act_group.empty() || !c->ready) return XPBD_OK;
+    const size_t n = c->L.meta.size();
+    std::vector<int32_t> eg(std::max<size_t>(n, 1), 0);
+    for (size_t g = 0; g < n; ++g) {
+        const int t = c->L.ptet_tetid[g];
+        eg[g] = (t >= 0 && t < c->nt) ? c->act_group[t] : 0;
+    }
+    int r;
+    if ((r = dupload(c, &c->d_elem_group, eg.data(), eg.size()))) return r;
+    if ((r = dalloc(c, &c->d_gact, (size_t)std::max(c->n_act_groups, 1)))) return r;
+    return XPBD_OK;
+}
+
+int32_t xpbd_set_activation_groups(xpbd_ctx *c, int32_t ngroups, const int32_t *tet_group) {
+    if (!c || c->nt == 0) return fail(c, XPBD_EINVAL, "no mesh");
+    if (ngroups <= 0 || !tet_group) return fail(c, XPBD_EINVAL, "bad activation groups");
+    for (int t = 0; t < c->nt; ++t)
+        if (tet_group[t] < 0 || tet_group[t] >= ngroups) return fail(c, XPBD_EINVAL, "activation group out of range");
+    sync_host_activation(c);
+    c->act_group.assign(tet_group, tet_group + c->nt);
+    c->n_act_groups = ngroups;
+    c->group_act.assign(ngroups, 0.f);
+    return upload_act_groups(c);
+}
+
+int32_t xpbd_set_group_activation(xpbd_ctx *c, int32_t ngroups, const float *act) {
+    if (!c || !act) return fail(c, XPBD_EINVAL, "bad activation");
+    if (c->act_group.empty() || ngroups != c->n_act_groups)
+        return fail(c, XPBD_EINVAL, "group count differs from xpbd_set_activation_groups");
+    c->group_act.assign(act, act + ngroups);
+    c->act_host_stale = true;
+    if (c->ready && (c->feature & F_ANISO)) {
+        int slot;
+        void *h;
+        const size_t bytes = (size_t)ngroups * sizeof(float);
+        int r = stage_acquire(c, c->st_gact, bytes, slot, h);
+        if (r) return r;
+        std::memcpy(h, act, bytes);
+        if ((r = stage_submit(c, c->st_gact, slot, c->d_gact, bytes))) return r;
+        const int64_t n = (int64_t)c->L.meta.size();
+        hipLaunchKernelGGL(k_refresh_fiber_groups, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n,
+                           (const float4 *)c->FIBS, (const int32_t *)c->d_elem_group, (const float *)c->d_gact,
+                           c->sp.contraction_ratio, c->FIB);
+        HIPCHK(c, hipGetLastError());
+    }
+    return XPBD_OK;
+}
+
 int32_t xpbd_set_activation(xpbd_ctx *c, const float *act) {
     if (!c || !act || c->nt == 0) return fail(c, XPBD_EINVAL, "bad activation");
-    c->activation.assign(act, act + c->nt);
+    if (act != c->activation.data()) c->activation.assign(act, act + c->nt);
+    c->act_host_stale = false;
     if (c->ready && (c->feature & F_ANISO)) {
         int slot;
         void *h;
@@ -3461,7 +3566,10 @@ int32_t xpbd_set_params(xpbd_ctx *c, const xpbd_params *p) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
         HIPCHK(c, hipMemcpy(c->d_cls, dcl.data(), dcl.size() * sizeof(DevClass), hipMemcpyHostToDevice));
     }
-    if (c->ready && cr_changed && (c->feature & F_ANISO)) return xpbd_set_activation(c, c->activation.data());
+    if (c->ready && cr_changed && (c->feature & F_ANISO)) {
+        sync_host_activation(c);
+        return xpbd_set_activation(c, c->activation.data());
+    }
     return XPBD_OK;
 }
 
@@ -3708,6 +3816,7 @@ int32_t xpbd_finalize(xpbd_ctx *c, const xpbd_tuning *t) {
     }
     UP(c->d_vslot_off, L.vslot_off);
     UP(c->d_vslot, L.vslot);
+    sync_host_activation(c);
     UP(c->d_act, c->activation);
     // per-ptet global vertex ids (residual kernel)
     if (c->flat) {
@@ -3766,6 +3875,7 @@ int32_t xpbd_finalize(xpbd_ctx *c, const xpbd_tuning *t) {
     if ((r = upload_pins(c))) return r;
     if (c->has_force && (r = xpbd_set_force(c, c->force.data()))) return r;
     if ((r = xpbd_set_bone_transforms(c, c->nbones, c->bone_xf.data()))) return r;
+    if ((r = upload_act_groups(c))) return r;
     if (c->feature & F_ANISO)
         if ((r = xpbd_set_activation(c, c->activation.data()))) return r;
     return XPBD_OK;

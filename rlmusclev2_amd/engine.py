@@ -167,6 +167,21 @@ class XpbdEngine:
         self._keep["act"] = a
         self.ctx.call("xpbd_set_activation", ptr(a))
 
+    def set_activation_groups(self, tet_group):
+        """one activation per group of tets (e.g. per batched instance), set once"""
+        g = i32(tet_group)
+        if g.shape[0] != self.NT:
+            raise ValueError(f"tet_group has {g.shape[0]} entries, expected NT={self.NT}")
+        self._keep["act_group"] = g
+        self._n_act_groups = int(g.max()) + 1 if g.size else 0
+        self.ctx.call("xpbd_set_activation_groups", self._n_act_groups, ptr(g))
+
+    def set_group_activation(self, act):
+        """per-group activation, expanded to the tets on the device"""
+        a = f32(act)
+        self._keep["gact"] = a
+        self.ctx.call("xpbd_set_group_activation", a.shape[0], ptr(a))
+
     def set_bone_transforms(self, xforms):
         x = f32(xforms, (-1, 12))
         self._keep["xf"] = x

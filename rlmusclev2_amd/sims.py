@@ -236,6 +236,15 @@ class MultiGridSim:
             raise RuntimeError("set_fiber_activation requires a 'fiber' constraint")
         self.engine.set_fiber_activation(acti)
 
+    def set_activation_groups(self, tet_group) -> None:
+        """one activation per group of tets from now on (batched instances)"""
+        if find_constraint(self.cfg, "fiber") is None:
+            raise RuntimeError("activation groups require a 'fiber' constraint")
+        self.engine.set_activation_groups(tet_group)
+
+    def set_group_activation(self, act) -> None:
+        self.engine.set_group_activation(act)
+
     def apply_pintotarget(self) -> None:
         self.engine.apply_pintotarget()
 

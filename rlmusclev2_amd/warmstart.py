@@ -61,8 +61,12 @@ class ActivationWalk:
         self.a = rng.uniform(0, 1, self.n).astype(np.float32)
 
     def next(self):
+        return self.next_groups()[self.g]
+
+    def next_groups(self):
+        """the next step of the walk, one value per group"""
         self.a = np.clip(self.a + self.rng.normal(0, self.step, self.n), 0, 1).astype(np.float32)
-        return self.a[self.g]
+        return self.a
 
 
 def tet_groups(w):

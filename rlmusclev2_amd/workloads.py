@@ -165,12 +165,13 @@ def capsule_sdf_grid(a, b, radius, res, margin):
     h = float(np.max(hi - lo) / (res - 1))
     origin = 0.5 * (lo + hi) - 0.5 * h * (res - 1)
     t = np.arange(res) * h
-    Z, Y, X = np.meshgrid(origin[2] + t, origin[1] + t, origin[0] + t, indexing="ij")
-    P = np.stack([X, Y, Z], axis=-1)
+    # node coordinates relative to a, broadcast over (z, y, x)
+    dz = (origin[2] + t - a[2])[:, None, None]
+    dy = (origin[1] + t - a[1])[None, :, None]
+    dx = (origin[0] + t - a[0])[None, None, :]
     ab = b - a
-    u = np.clip(((P - a) @ ab) / float(ab @ ab), 0.0, 1.0)
-    q = a + u[..., None] * ab
-    phi = np.linalg.norm(P - q, axis=-1) - radius
+    u = np.clip((dx * ab[0] + dy * ab[1] + dz * ab[2]) / float(ab @ ab), 0.0, 1.0)
+    phi = np.sqrt((dx - u * ab[0]) ** 2 + (dy - u * ab[1]) ** 2 + (dz - u * ab[2]) ** 2) - radius
     return phi.astype(np.float32), origin.astype(np.float32), np.float32(h)
 
 
