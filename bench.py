@@ -251,7 +251,7 @@ def warm_start_leg(cfg, w, tuning, args):
     t0 = time.perf_counter()
     ds = ws.collect(sim, n_frames=int(wc.get("train_frames", 2000)), seed=0,
                     samples_per_substep=int(wc.get("samples_per_substep", 4096)),
-                    conv_iters=int(wc.get("conv_iters", 10)), log=log)
+                    conv_iters=int(wc.get("conv_iters", 10)), lags=(1, 2), log=log)
     t1 = time.perf_counter()
     norm = ws.fit_normalizer(ds)
     net, hist = ws.train(ds, norm, epochs=int(wc.get("epochs", 3)), batch=16384, lr=2e-3, seed=0,

@@ -63,11 +63,12 @@ __device__ __forceinline__ V3 grad_row(const float *G, const float *B, int k) {
 // R = U V^T, solve_kernels.py:85-132).  ARAP-SVD's C = ||sigma - 1|| equals
 // ||F - R||_F and dC/dF = (F - R) / C, so only R is needed.
 //
-// det F > 0: Higham's scaled Newton iteration X <- (g X + X^{-T}/g)/2 with
+// det F > 0: near rest the Bjorck-Bowie (Newton-Schulz) iteration below,
+// otherwise Higham's scaled Newton iteration X <- (g X + X^{-T}/g)/2 with
 // Frobenius-norm scaling (N. J. Higham, "Computing the polar decomposition —
-// with applications", SIAM J. Sci. Stat. Comput. 7, 1986), quadratically
-// convergent; 2 steps near rest.  Otherwise (inverted / degenerate, or no
-// convergence) a one-sided Jacobi SVD gives R = U diag(1,1,det U det V) V^T.
+// with applications", SIAM J. Sci. Stat. Comput. 7, 1986); both quadratically
+// convergent.  Otherwise (inverted / degenerate, or no convergence) a
+// one-sided Jacobi SVD gives R = U diag(1,1,det U det V) V^T.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bool polar_newton(const float *F, float *X) {
 #pragma unroll
@@ -109,6 +110,45 @@ __device__ __forceinline__ bool polar_newton(const float *F, float *X) {
         // convergence, delta = |X_k+1 - X_k|^2 ~ error_k^2): stop once that is
         // below FP32 resolution of a rotation
         if (delta < 1.0e-7f) return true;
+    }
+    return false;
+}
+
+// Near rest, Bjorck and Bowie's orthogonalisation (the Newton-Schulz iteration
+// for the polar factor; A. Bjorck and C. Bowie, "An iterative algorithm for
+// computing the best estimate of an orthogonal matrix", SIAM J. Numer. Anal.
+// 8, 1971; N. J. Higham, "Functions of Matrices", SIAM 2008, sec. 8.3):
+// X <- X (3 I - X^T X) / 2 = X (I + E / 2) with E = I - X^T X.  It converges
+// quadratically (E' = 3/4 E^2 + 1/4 E^3) to the polar factor when |E| < 1 and
+// needs no inverse, division or square root: about 54 FP32 operations a step
+// against about 100 for a scaled Newton step.  Taken when |E0|_F < 0.5 (all
+// singular values of F within (0.7, 1.23) for det F > 0); it stops once
+// 3/4 |E|^2, the error after the step, is below FP32 resolution.
+__device__ __forceinline__ bool polar_schulz(const float *F, float *X) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) X[i] = F[i];
+    for (int it = 0; it < 8; ++it) {
+        // S = X^T X (symmetric), E = I - S
+        const float s00 = X[0] * X[0] + X[3] * X[3] + X[6] * X[6];
+        const float s11 = X[1] * X[1] + X[4] * X[4] + X[7] * X[7];
+        const float s22 = X[2] * X[2] + X[5] * X[5] + X[8] * X[8];
+        const float s01 = X[0] * X[1] + X[3] * X[4] + X[6] * X[7];
+        const float s02 = X[0] * X[2] + X[3] * X[5] + X[6] * X[8];
+        const float s12 = X[1] * X[2] + X[4] * X[5] + X[7] * X[8];
+        const float e00 = 1.0f - s00, e11 = 1.0f - s11, e22 = 1.0f - s22;
+        const float en = e00 * e00 + e11 * e11 + e22 * e22 + 2.0f * (s01 * s01 + s02 * s02 + s12 * s12);
+        if (it == 0 && !(en < 0.25f)) return false;
+        // M = I + E / 2
+        const float m00 = 1.0f + 0.5f * e00, m11 = 1.0f + 0.5f * e11, m22 = 1.0f + 0.5f * e22;
+        const float m01 = -0.5f * s01, m02 = -0.5f * s02, m12 = -0.5f * s12;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const float x0 = X[3 * r], x1 = X[3 * r + 1], x2 = X[3 * r + 2];
+            X[3 * r] = x0 * m00 + x1 * m01 + x2 * m02;
+            X[3 * r + 1] = x0 * m01 + x1 * m11 + x2 * m12;
+            X[3 * r + 2] = x0 * m02 + x1 * m12 + x2 * m22;
+        }
+        if (en < 8.0e-8f) return true;
     }
     return false;
 }
@@ -192,7 +232,7 @@ __device__ __noinline__ M9 rotation_from_svd(M9 Fm) {
 }
 
 __device__ __forceinline__ void closest_rotation(const float *F, float *R) {
-    if (!(det3(F) > 1.0e-20f) || !polar_newton(F, R)) {
+    if (!(det3(F) > 1.0e-20f) || !(polar_schulz(F, R) || polar_newton(F, R))) {
         M9 f;
 #pragma unroll
         for (int i = 0; i < 9; ++i) f.m[i] = F[i];
@@ -863,12 +903,32 @@ __global__ __launch_bounds__(256) void k_block_shared(IterArgs a, BlockArgs b, c
 // F - I (9), fibre direction (3), fibre target length (1), previous-substep
 // multipliers (3), log10 rest volume, det F - 1,
 // |F|^2 - 3, log10 mean inverse mass, log10 ARAP and volume compliances.
+// the four features that do not change during a run (rest volume, inverse
+// masses, compliances), per element, as elem_features computes them
+__device__ __forceinline__ float4 elem_static_features(const IterArgs &a, int64_t g, const float4 *__restrict__ X) {
+    const int4 gi = a.gidx[g];
+    const uint32_t meta = a.meta[g];
+    const DevClass dc = a.cls[meta & 0xffffu];
+    float B[9];
+    for (int q = 0; q < 9; ++q) B[q] = a.Bp[(int64_t)q * a.nptet + g];
+    const float detB = det3(B);
+    const float wsum = X[gi.x].w + X[gi.y].w + X[gi.z].w + X[gi.w].w;
+    return make_float4(log10f(1.0f / (6.0f * fabsf(detB) + 1e-30f) + 1e-30f), log10f(0.25f * wsum + 1e-30f),
+                       log10f(dc.at_arap + 1e-30f), log10f(dc.a_vol + 1e-30f));
+}
+
+__global__ void k_static_features(IterArgs a, const float4 *__restrict__ X, float4 *__restrict__ out) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < a.nptet) out[g] = elem_static_features(a, g, X);
+}
+
+// sfeat: the element's static features (k_static_features)
 __device__ __forceinline__ void elem_features(const IterArgs &a, int64_t g, const float4 *__restrict__ X,
-                                              const float4 *__restrict__ base, float f[24]) {
+                                              const float4 *__restrict__ base, const float4 *__restrict__ sfeat,
+                                              float f[24]) {
     const int4 gi = a.gidx[g];
     const uint32_t meta = a.meta[g];
     const uint32_t mask = (meta >> 16) & 15u;
-    const DevClass dc = a.cls[meta & 0xffffu];
     const float4 X0 = X[gi.x], X1 = X[gi.y], X2 = X[gi.z], X3 = X[gi.w];
     const V3 c0 = xyz(X0) - xyz(X3), c1 = xyz(X1) - xyz(X3), c2 = xyz(X2) - xyz(X3);
     float B[9], F[9];
@@ -881,26 +941,26 @@ __device__ __forceinline__ void elem_features(const IterArgs &a, int64_t g, cons
     f[9] = fb.x; f[10] = fb.y; f[11] = fb.z; f[12] = fb.w;
     const float4 lp = base[g];
     f[13] = lp.x; f[14] = lp.y; f[15] = lp.z;
-    const float detB = det3(B);
-    f[16] = log10f(1.0f / (6.0f * fabsf(detB) + 1e-30f) + 1e-30f);
+    const float4 sf = sfeat[g];
+    f[16] = sf.x;
     float n2 = 0.f;
     for (int i = 0; i < 9; ++i) n2 += F[i] * F[i];
     f[17] = det3(F) - 1.0f;
     f[18] = n2 - 3.0f;
-    f[19] = log10f(0.25f * (X0.w + X1.w + X2.w + X3.w) + 1e-30f);
-    f[20] = log10f(dc.at_arap + 1e-30f);
-    f[21] = log10f(dc.a_vol + 1e-30f);
+    f[19] = sf.y;
+    f[20] = sf.z;
+    f[21] = sf.w;
     f[22] = 0.f;
     f[23] = 0.f;
 }
 
 __global__ void k_features(IterArgs a, const float4 *__restrict__ X, const float4 *__restrict__ base,
-                           float *__restrict__ out, const float *__restrict__ fmean, const float *__restrict__ finv,
-                           int normalize) {
+                           const float4 *__restrict__ sfeat, float *__restrict__ out,
+                           const float *__restrict__ fmean, const float *__restrict__ finv, int normalize) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= a.nptet) return;
     float f[24];
-    elem_features(a, g, X, base, f);
+    elem_features(a, g, X, base, sfeat, f);
     float *o = out + g * 24;
     for (int i = 0; i < 24; ++i) o[i] = normalize ? (f[i] - fmean[i]) * finv[i] : f[i];
 }
@@ -934,8 +994,11 @@ __device__ __forceinline__ void wave_lds_sync() {
 // Each wavefront runs its own 32-element tiles start to finish (its own LDS
 // rows, wavefront-level synchronisation only), so the waves of a workgroup
 // overlap one another's gathers with their matrix work.
+// write_base = 0 keeps the base (the schedule's reference multipliers) for the
+// next warm substeps instead of replacing it with lambda0
 __global__ __launch_bounds__(256) void k_warm_start(IterArgs a, WarmNet net, const float4 *__restrict__ X,
-                                                    float4 *base, float4 *__restrict__ lam) {
+                                                    float4 *base, const float4 *__restrict__ sfeat,
+                                                    float4 *__restrict__ lam, int write_base) {
     constexpr int NI = XPBD_MLP_IN, NH = XPBD_MLP_HIDDEN, NO = XPBD_MLP_OUT;
     __shared__ uint16_t sW1[NH * NI], sW2[NH * NH];
     __shared__ __align__(16) float sW3[NO * NH];
@@ -961,7 +1024,7 @@ __global__ __launch_bounds__(256) void k_warm_start(IterArgs a, WarmNet net, con
         const bool rv = row < a.nptet;
         float f[24];
         if (rv) {
-            elem_features(a, row, X, base, f);
+            elem_features(a, row, X, base, sfeat, f);
         } else {
 #pragma unroll
             for (int i = 0; i < 24; ++i) f[i] = 0.f;
@@ -1060,7 +1123,7 @@ __global__ __launch_bounds__(256) void k_warm_start(IterArgs a, WarmNet net, con
                 v = make_float2(lp.z * s2, 0.0f);
             }
             reinterpret_cast<float2 *>(lam + row)[h] = v;
-            reinterpret_cast<float2 *>(base + row)[h] = v;
+            if (write_base) reinterpret_cast<float2 *>(base + row)[h] = v;
         }
     }
 }
@@ -2585,6 +2648,8 @@ struct xpbd_ctx {
     int warm_net = 1;             // mode 4: between references, network (mode 2) or base only (mode 3)
     float4 *LAMB = nullptr;       // carried multipliers of the current substep
     float4 *LAMB0 = nullptr;      // base of the next warm start (lambda0 of the last substep, or a reference solve)
+    float4 *SFEAT = nullptr;      // static network features per element (k_static_features)
+    bool sfeat_ok = false;
     uint16_t *mW1 = nullptr, *mW2 = nullptr;
     float *mb1 = nullptr, *mb2 = nullptr, *mW3 = nullptr, *mb3 = nullptr;
     float *fmean = nullptr, *finv = nullptr, *oscale = nullptr;
@@ -2671,6 +2736,8 @@ void free_device(xpbd_ctx *c) {
     c->d_att_tgt = nullptr;
     c->d_elem_group = nullptr;
     c->d_gact = nullptr;
+    c->SFEAT = nullptr;
+    c->sfeat_ok = false;
     c->pin_cap = 0;
 }
 
@@ -2735,6 +2802,7 @@ inline float inv_mass_of(const xpbd_ctx *c, int old) {
 
 // upload the per-vertex W (inverse mass) / mass / flags after mass or stopped change
 int upload_vertex_static(xpbd_ctx *c) {
+    c->sfeat_ok = false;   // the inverse masses enter the network's static features
     const int nv = c->nv;
     std::vector<float4> x(nv), v(nv), pp(nv);
     HIPCHK(c, hipMemcpyAsync(x.data(), c->X[0], nv * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
@@ -3177,6 +3245,19 @@ int alloc_dcd(xpbd_ctx *c) {
     return XPBD_OK;
 }
 
+// static network features, computed once per layout, masses and parameters
+// (called outside graph capture, before the first use)
+int ensure_static_features(xpbd_ctx *c) {
+    if (c->sfeat_ok || !c->flat || !c->SFEAT) return XPBD_OK;
+    const int64_t ne = (int64_t)c->L.meta.size();
+    IterArgs a = make_args(c, c->DPW);
+    hipLaunchKernelGGL(k_static_features, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, c->stream, a,
+                       (const float4 *)c->X[0], c->SFEAT);
+    HIPCHK(c, hipGetLastError());
+    c->sfeat_ok = true;
+    return XPBD_OK;
+}
+
 // start-of-substep multiplier set-up for the effective mode of this substep
 int launch_warm_start(xpbd_ctx *c) {
     const int64_t ne = (int64_t)c->L.meta.size();
@@ -3191,8 +3272,11 @@ int launch_warm_start(xpbd_ctx *c) {
         // four 32-element tiles per workgroup pass; 2,048 workgroups walk the rest grid-stride
         const int64_t ntile = (ne + 127) / 128;
         const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ntile, 2048));
+        int r = ensure_static_features(c);
+        if (r) return r;
+        // inside a schedule (mode 4) the base stays the last reference solve's multipliers
         hipLaunchKernelGGL(k_warm_start, dim3(grid), dim3(256), 0, c->stream, a, net, (const float4 *)c->X[0],
-                           c->LAMB0, c->LAMB);
+                           c->LAMB0, (const float4 *)c->SFEAT, c->LAMB, c->lam_mode == 4 ? 0 : 1);
         c->impulse_pending = true;
     } else if (c->eff_mode == 3) {
         hipLaunchKernelGGL(k_copy4, g1(ne), dim3(256), 0, c->stream, ne, c->LAMB, (const float4 *)c->LAMB0);
@@ -3559,6 +3643,7 @@ int32_t xpbd_set_params(xpbd_ctx *c, const xpbd_params *p) {
     s.const_damp = p->const_damp;
     const bool cr_changed = s.contraction_ratio != c->sp.contraction_ratio;
     c->sp = s;
+    c->sfeat_ok = false;   // the compliances enter the network's static features
     c->params_set = true;
     invalidate_graph(c);
     if (c->ready && c->d_cls) {
@@ -3787,6 +3872,8 @@ int32_t xpbd_finalize(xpbd_ctx *c, const xpbd_tuning *t) {
         HIPCHK(c, hipMemset(c->LAMB, 0, std::max<size_t>(nptet, 1) * sizeof(float4)));
         if ((r = dalloc(c, &c->LAMB0, std::max<size_t>(nptet, 1)))) return r;
         HIPCHK(c, hipMemset(c->LAMB0, 0, std::max<size_t>(nptet, 1) * sizeof(float4)));
+        if ((r = dalloc(c, &c->SFEAT, std::max<size_t>(nptet, 1)))) return r;
+        c->sfeat_ok = false;
         if ((r = dalloc(c, &c->mW1, (size_t)XPBD_MLP_HIDDEN * XPBD_MLP_IN)) ||
             (r = dalloc(c, &c->mW2, (size_t)XPBD_MLP_HIDDEN * XPBD_MLP_HIDDEN)) ||
             (r = dalloc(c, &c->mb1, XPBD_MLP_HIDDEN)) || (r = dalloc(c, &c->mb2, XPBD_MLP_HIDDEN)) ||
@@ -4009,6 +4096,7 @@ int32_t xpbd_step_frame(xpbd_ctx *c, int32_t nsub, int32_t iters, int32_t use_gr
     int r = check_ready(c);
     if (r) return r;
     if (nsub < 0 || iters < 0) return fail(c, XPBD_EINVAL, "negative count");
+    if (c->lam_mode && (r = ensure_static_features(c))) return r;   // before any capture
     // the DCD broadphase is rebuilt once per frame (humanV2.py:853-865)
     c->dcd_fresh = false;
     if (!use_graph || c->timing) {
@@ -4476,10 +4564,11 @@ int32_t xpbd_features(xpbd_ctx *c, float *dev_out, int32_t normalized) {
     if (!c->flat) return fail(c, XPBD_EUNSUPPORTED, "features run on the flat strategy");
     if (!dev_out) return fail(c, XPBD_EINVAL, "null output");
     const int64_t ne = (int64_t)c->L.meta.size();
+    if ((r = ensure_static_features(c))) return r;
     IterArgs a = make_args(c, c->DPW);
     hipLaunchKernelGGL(k_features, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, c->stream, a,
-                       (const float4 *)c->X[0], (const float4 *)c->LAMB0, dev_out, (const float *)c->fmean,
-                       (const float *)c->finv, normalized ? 1 : 0);
+                       (const float4 *)c->X[0], (const float4 *)c->LAMB0, (const float4 *)c->SFEAT, dev_out,
+                       (const float *)c->fmean, (const float *)c->finv, normalized ? 1 : 0);
     HIPCHK(c, hipGetLastError());
     return XPBD_OK;
 }

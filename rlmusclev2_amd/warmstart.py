@@ -87,14 +87,16 @@ class Dataset:
 
 
 def collect(sim, n_frames: int, seed: int = 0, samples_per_substep: int = 8192, conv_iters: int = 10,
-            substep_stride: int = 2, log=None) -> Dataset:
+            substep_stride: int = 2, lags=(1,), log=None) -> Dataset:
     """Plain-solver trajectory with randomized activations and bone motion.
 
     At every substep, after integrate, a lambda-carrying solve of
     ``conv_iters`` iterations from lambda = 0 gives the converged multipliers
     (the state is restored afterwards, so the trajectory stays plain).  Every
-    ``substep_stride``-th substep records (features with the previous
-    substep's converged multipliers, change of the converged multipliers)."""
+    ``substep_stride``-th substep records, for each lag in ``lags``, (features
+    with the converged multipliers of that many substeps back as the base,
+    this substep's converged multipliers): a schedule with a reference solve
+    every third substep applies a base one or two substeps old."""
     import torch
     e = sim.engine
     w = sim.workload
@@ -107,6 +109,8 @@ def collect(sim, n_frames: int, seed: int = 0, samples_per_substep: int = 8192, 
     feat = torch.empty((ne, NIN), dtype=torch.float32, device="cuda")
     lam = torch.zeros((ne, 4), dtype=torch.float32, device="cuda")
     lam_prev = torch.zeros((ne, 4), dtype=torch.float32, device="cuda")
+    lam_prev2 = torch.zeros_like(lam_prev)
+    feat2 = torch.empty_like(feat) if 2 in lags else None
     gen = torch.Generator(device="cuda").manual_seed(seed)
     Xs, Ps, Ts = [], [], []
     has_fib = sim.scene.v_fiber_dir is not None
@@ -120,8 +124,12 @@ def collect(sim, n_frames: int, seed: int = 0, samples_per_substep: int = 8192, 
         for s in range(sim.nsubsteps):
             e.set_lambda_mode(MODE_PLAIN)
             e.integrate()
+            rec = sub % substep_stride == 0
+            if rec and feat2 is not None:
+                e.set_lambda(lam_prev2)
+                e.features(feat2, normalized=False)
             e.set_lambda(lam_prev)
-            if sub % substep_stride == 0:
+            if rec:
                 e.features(feat, normalized=False)
             e.snapshot(True)
             e.set_lambda_mode(MODE_CARRY)
@@ -130,11 +138,17 @@ def collect(sim, n_frames: int, seed: int = 0, samples_per_substep: int = 8192, 
             e.get_lambda(lam)
             e.snapshot(False)
             e.set_lambda_mode(MODE_PLAIN)
-            if sub % substep_stride == 0:
+            if rec:
                 idx = torch.randint(0, ne, (min(samples_per_substep, ne),), device="cuda", generator=gen)
-                Xs.append(feat[idx].clone())
-                Ps.append(lam_prev[idx].clone())
-                Ts.append(lam[idx].clone())
+                if 1 in lags:
+                    Xs.append(feat[idx].clone())
+                    Ps.append(lam_prev[idx].clone())
+                    Ts.append(lam[idx].clone())
+                if feat2 is not None:
+                    Xs.append(feat2[idx].clone())
+                    Ps.append(lam_prev2[idx].clone())
+                    Ts.append(lam[idx].clone())
+            lam_prev2.copy_(lam_prev)
             lam_prev.copy_(lam)
             e.iterate(sim.fine_iters)
             e.collide()
