@@ -53,6 +53,7 @@ WORKLOADS = {
     "humanv2": ("humanv2_scale.json", {}),
     "humanv2_20": ("humanv2_scale.json", {"fine_iterations": 20}),
     "snh_aniso": ("humanv2_snh_aniso.json", {}),
+    "snh_aniso_voxel": ("humanv2_snh_aniso.json", {"mg_cage": "voxel"}),
     "cantilever": ("cantilever.json", {}),
     "batch": ("batch512.json", {}),
     "c5": ("humanv2_c5.json", {}),

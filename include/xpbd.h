@@ -171,10 +171,18 @@ typedef struct {
     const float *fit_sinv;      /* nblk*16 S_b^-1, S_b = sum [r_i - c_b; 1][r_i - c_b; 1]^T     */
     int32_t iters;              /* coarse sweeps per substep; <= 0 turns the level off          */
     int32_t kinematic;
+    int32_t pin_rule;           /* 0: coarse pins take part in the sweeps and keep their restricted
+                                   positions; 1: the reference's rule, pins (stopped in the coarse
+                                   context) start at their goal (xpbd_set_coarse_pin_targets) and are
+                                   held (_coarse_solve, mg.py:1047-1062)                            */
 } xpbd_coarse_desc;
 /* coarse = NULL or desc = NULL turns it off.  The coarse context must outlive its use;
  * xpbd_finalize drops the link. */
 int32_t xpbd_set_coarse_level(xpbd_ctx *ctx, xpbd_ctx *coarse, const xpbd_coarse_desc *desc);
+/* pin rule 1: the restricted bone goal of every coarse node (n = coarse nodes, xyz n*3, coarse
+ * caller order; only the pins' entries are read), per frame (_refresh_coarse_target,
+ * mg.py:1013-1030) */
+int32_t xpbd_set_coarse_pin_targets(xpbd_ctx *ctx, int32_t n, const float *xyz);
 /* the V-cycle alone, for callers that drive the phases one by one */
 int32_t xpbd_vcycle(xpbd_ctx *ctx);
 

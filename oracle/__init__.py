@@ -262,8 +262,16 @@ class OracleSim:
             self._mg_restrict = CageRestriction(lv)
         cc.pos[:] = self._mg_restrict(self.pos, self.pos0).astype(np.float32)
         qprev = cc.pos.astype(np.float64).copy()
-        if cc.pin_vids is not None:        # coarse pins keep their restricted positions
-            cc.set_pin_targets(cc.pos[cc.pin_vids])
+        if cc.pin_vids is not None:
+            if getattr(lv, "pin_rule", 0) == 1:
+                # the reference's rule (_coarse_kinematic_pin then stopped nodes held,
+                # mg.py:999-1062): pins start at the restricted bone goal
+                goal = getattr(self, "_mg_pin_goal", None)
+                goal = lv.pos0 if goal is None else goal
+                cc.set_pin_targets(goal[cc.pin_vids])
+                cc.pos[cc.pin_vids] = goal[cc.pin_vids]
+            else:                          # coarse pins keep their restricted positions
+                cc.set_pin_targets(cc.pos[cc.pin_vids])
         cc.iterate(lv.iters)
         d = cc.pos.astype(np.float64) - qprev
         pidx = np.asarray(lv.p_idx, np.int64)
@@ -274,6 +282,9 @@ class OracleSim:
         if getattr(self, "_mg_shift_pprev", False):
             self.pprev[:] = (self.pprev.astype(np.float64) + dx).astype(np.float32)
         self.apply_pintotarget()
+
+    def set_coarse_pin_targets(self, xyz):
+        self._mg_pin_goal = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
 
     def native_substep(self, iters):
         self.integrate()
@@ -334,7 +345,9 @@ class CageRestriction:
     vertices onto their current positions.  Assembled from p_idx / p_w and solved
     here in float64 (Cholesky per block), independent of the precomputed
     inverses the GPU path uploads.  ``eps_scale`` multiplies eps_b (0 gives the
-    plain least-squares restriction, where P_b has full column rank)."""
+    plain least-squares restriction, where P_b has full column rank).  A block
+    with eps_b = 0 (the reference's voxel cage) is the Galerkin projection
+    pinv(P^T P) P^T x with rcond 1e-8 (tools/gen_coarse_cage.py)."""
 
     def __init__(self, level, eps_scale: float = 1.0):
         from scipy.linalg import cho_factor
@@ -356,7 +369,10 @@ class CageRestriction:
                 for m in range(8):
                     np.add.at(PtP, (p_idx[vids, k] - s, p_idx[vids, m] - s), wv[:, k] * wv[:, m])
             eps = float(level.eps[b]) * float(eps_scale)
-            fac = cho_factor(PtP + eps * np.eye(n))
+            if eps > 0.0:
+                fac = ("cho", cho_factor(PtP + eps * np.eye(n)))
+            else:   # Galerkin projection pinv(P^T P) P^T (tools/gen_coarse_cage.py, rcond 1e-8)
+                fac = ("pinv", np.linalg.pinv(PtP, rcond=1e-8))
             self.blocks.append((s, e, vids, rows, cols, wv.ravel(), eps, fac))
 
     def __call__(self, x, rest) -> np.ndarray:
@@ -373,7 +389,7 @@ class CageRestriction:
                 rh = np.concatenate([rest[vids], np.ones((len(vids), 1))], 1)
                 T = np.linalg.lstsq(rh, x[vids], rcond=None)[0]          # (4, 3): x ~ [r 1] T
                 rhs += eps * (np.concatenate([q0[s:e], np.ones((e - s, 1))], 1) @ T)
-            q[s:e] = cho_solve(fac, rhs)
+            q[s:e] = cho_solve(fac[1], rhs) if fac[0] == "cho" else fac[1] @ rhs
         return q
 
 

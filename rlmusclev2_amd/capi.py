@@ -46,7 +46,7 @@ class XpbdCoarseDesc(C.Structure):
     _fields_ = [("p_idx", C.c_void_p), ("p_w", C.c_void_p), ("nblk", C.c_int32), ("blk_off", C.c_void_p),
                 ("minv", C.c_void_p), ("eps", C.c_void_p), ("fit_off", C.c_void_p), ("fit_fine", C.c_void_p),
                 ("fit_center", C.c_void_p), ("fit_sinv", C.c_void_p), ("iters", C.c_int32),
-                ("kinematic", C.c_int32)]
+                ("kinematic", C.c_int32), ("pin_rule", C.c_int32)]
 
 
 class XpbdDcdParams(C.Structure):
@@ -122,6 +122,7 @@ SIGNATURES = {
     "xpbd_mlp_backward": [C.POINTER(XpbdMlpDev), _P, _P, _P, _P, C.c_int64, _P, _P, C.c_int64, _P],
     "xpbd_mlp_backward_noalloc": [C.POINTER(XpbdMlpDev), _P, _P, _P, _P, C.c_int64, _P, _P],
     "xpbd_set_coarse_level": [_P, _P, C.POINTER(XpbdCoarseDesc)],
+    "xpbd_set_coarse_pin_targets": [_P, _I, _P],
     "xpbd_vcycle": [_P],
     "xpbd_set_lambda_mode": [_P, _I],
     "xpbd_set_mlp": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P],

@@ -1366,7 +1366,9 @@ struct MgCoarse {
     const float *tB;             // per tet: rest inverse B, 9 floats row-major
     const float *tat;            // per tet: 1 / (k dt^2)
     const float *node_w;         // per node: inverse mass (0 = held)
-    const uint8_t *node_pin;     // per node: 1 = coarse pin (projected with its inverse mass, never moved)
+    const uint8_t *node_pin;     // per node: 1 = coarse pin (never moved by a sweep)
+    const float4 *pin_tgt;       // pin rule 1: per node (block order) the restricted bone goal of a pin
+    int pin_rule;                // 0: pins keep their restricted positions; 1: they start at pin_tgt
     const int32_t *slot_off;     // per node + 1: its run in the block's vertex-major corner list
     const ushort4 *tcpos;        // per tet: its corners' places in that list (block-local)
     int max_nodes, max_tets;
@@ -1558,7 +1560,10 @@ __global__ __launch_bounds__(MGC_T) void k_mg_coarse(MgArgs m, MgCoarse mc, int 
         for (int q = 0; q < 9; ++q) B1[q] = mc.tB[9 * (int64_t)t + q];
     }
     if (tid < nn) {
-        const float4 q = m.q[n0 + tid];
+        float4 q = m.q[n0 + tid];
+        // the reference's rule: a pin moves to its restricted bone goal, then is held
+        // (_coarse_kinematic_pin, mg.py:999-1011; zero inverse mass via coarse_stopped)
+        if (mc.pin_rule && mc.node_pin[n0 + tid]) q = mc.pin_tgt[n0 + tid];
         sq[tid] = make_float4(q.x, q.y, q.z, mc.node_w[n0 + tid]);
     }
     // a node's corner run is summed by two adjacent lanes (even and odd
@@ -2622,6 +2627,8 @@ struct xpbd_ctx {
     int mg_iters = 0;
     bool mg_pidx_set = false, mg_kinematic = false;
     int mg_nblk = 0;
+    float4 *d_mg_pin_tgt = nullptr;   // pin rule 1: restricted bone goal per coarse node (block order)
+    HostStage st_mgpin;
     int32_t *d_mg_pidx = nullptr;
     float *d_mg_pw = nullptr;
     float4 *MG_QPREV = nullptr;
@@ -2736,6 +2743,7 @@ void free_device(xpbd_ctx *c) {
     c->d_att_tgt = nullptr;
     c->d_elem_group = nullptr;
     c->d_gact = nullptr;
+    c->d_mg_pin_tgt = nullptr;
     c->SFEAT = nullptr;
     c->sfeat_ok = false;
     c->pin_cap = 0;
@@ -3416,6 +3424,7 @@ void xpbd_destroy(xpbd_ctx *c) {
     free_stage(c->st_bone);
     free_stage(c->st_att);
     free_stage(c->st_gact);
+    free_stage(c->st_mgpin);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
 }
@@ -4508,6 +4517,15 @@ int32_t xpbd_set_coarse_level(xpbd_ctx *c, xpbd_ctx *cc, const xpbd_coarse_desc 
         g.tat = p_tat;
         g.node_w = p_nw;
         g.node_pin = p_npin;
+        g.pin_rule = d->pin_rule ? 1 : 0;
+        {
+            // pin goals start at the coarse rest (until xpbd_set_coarse_pin_targets)
+            std::vector<float4> pt(std::max(nc, 1), make_float4(0.f, 0.f, 0.f, 0.f));
+            for (int v = 0; v < nc; ++v)
+                pt[v] = make_float4(cc->pos0[3 * (size_t)v], cc->pos0[3 * (size_t)v + 1], cc->pos0[3 * (size_t)v + 2], 0.f);
+            if ((r = dupload(c, &c->d_mg_pin_tgt, pt.data(), pt.size()))) return r;
+        }
+        g.pin_tgt = c->d_mg_pin_tgt;
         g.slot_off = p_soff;
         g.tcpos = p_tcp;
         g.max_nodes = max_nodes;
@@ -4529,6 +4547,20 @@ int32_t xpbd_set_coarse_level(xpbd_ctx *c, xpbd_ctx *cc, const xpbd_coarse_desc 
     c->mg_kinematic = d->kinematic != 0;
     c->mg_pidx_set = true;
     return XPBD_OK;
+}
+
+int32_t xpbd_set_coarse_pin_targets(xpbd_ctx *c, int32_t n, const float *xyz) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!c->coarse || !c->mg_pidx_set) return fail(c, XPBD_EINVAL, "no coarse level set");
+    if (n != c->coarse->nv || !xyz) return fail(c, XPBD_EINVAL, "one coarse pin goal per coarse node expected");
+    int slot;
+    void *h;
+    const size_t bytes = (size_t)n * sizeof(float4);
+    if ((r = stage_acquire(c, c->st_mgpin, bytes, slot, h))) return r;
+    float4 *t = (float4 *)h;
+    for (int i = 0; i < n; ++i) t[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0.f);
+    return stage_submit(c, c->st_mgpin, slot, c->d_mg_pin_tgt, bytes);
 }
 
 int32_t xpbd_vcycle(xpbd_ctx *c) {

@@ -310,8 +310,15 @@ class XpbdEngine:
         k["mg"] = arrs
         k["mg_coarse"] = coarse       # the coarse context must outlive this one's use of it
         d = capi.XpbdCoarseDesc(nblk=len(level.blk_off) - 1, iters=int(level.iters), kinematic=int(bool(kinematic)),
+                                pin_rule=int(getattr(level, "pin_rule", 0)),
                                 **{n: a.ctypes.data for n, a in arrs.items()})
         self.ctx.call("xpbd_set_coarse_level", coarse.ctx.h, C.byref(d))
+
+    def set_coarse_pin_targets(self, xyz):
+        """pin rule 1: the restricted bone goal of every coarse node (coarse order)"""
+        a = f32(xyz, (-1, 3))
+        self._keep["mg_pin_tgt"] = a
+        self.ctx.call("xpbd_set_coarse_pin_targets", a.shape[0], ptr(a))
 
     def vcycle(self):
         self.ctx.call("xpbd_vcycle")

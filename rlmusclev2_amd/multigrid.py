@@ -70,6 +70,11 @@ class CoarseLevel:
     stiffness: float
     spacing: np.ndarray     # per block
     n_components: int
+    # pin rule: 0 = coarse pins take part in the sweeps and keep their restricted
+    # positions; 1 = the reference's (mg.py:1013-1062): pins are moved to the
+    # restricted bone goal and held with zero inverse mass
+    pin_rule: int = 0
+    blk_vgroup: np.ndarray | None = None   # (nblk,) the fine vertex group (bone) of each block
 
 
 def components(nv: int, tets: np.ndarray) -> np.ndarray:
@@ -197,6 +202,151 @@ def build_coarse_level(pos0, tets, mass, pin_mask=None, factor: float = 4.0, ite
                        n_components=len(spacing))
 
 
+# 6-tet split of a unit cube (corner b at offset (b & 1, b >> 1 & 1, b >> 2 & 1)), the
+# split of the reference's voxel cage (tools/gen_coarse_cage.py CUBE)
+_VOXEL_TETS = np.array([(0, 1, 3, 7), (0, 3, 2, 7), (0, 2, 6, 7), (0, 6, 4, 7), (0, 4, 5, 7), (0, 5, 1, 7)],
+                       np.int64)
+_VOXEL_CORNER = np.array([(b & 1, (b >> 1) & 1, (b >> 2) & 1) for b in range(8)], np.int64)
+
+
+def _voxel_barycentric(u):
+    """(tet index within the voxel, 4 barycentric weights) of local points u in
+    [0, 1]^3: the voxel tet with the largest smallest weight (the containing one)."""
+    best_t = np.zeros(len(u), np.int64)
+    best_w = np.zeros((len(u), 4))
+    best_s = np.full(len(u), -np.inf)
+    for t, tet in enumerate(_VOXEL_TETS):
+        p = _VOXEL_CORNER[tet].astype(np.float64)
+        T = np.stack([p[0] - p[3], p[1] - p[3], p[2] - p[3]], axis=1)
+        lam = np.linalg.solve(T, (u - p[3]).T).T
+        w = np.concatenate([lam, 1.0 - lam.sum(1, keepdims=True)], axis=1)
+        m = w.min(1)
+        upd = m > best_s
+        best_s[upd], best_t[upd], best_w[upd] = m[upd], t, w[upd]
+    return best_t, best_w
+
+
+def build_voxel_level(pos0, tets, total_mass: float, pin_mask=None, voxel: float | None = None,
+                      factor: float = 4.0, iters: int = 15, stiffness: float = 1.0e6, pin_frac: float = 0.5,
+                      comp=None, skip=None, vgroup=None) -> CoarseLevel:
+    """The reference's two-level cage (tools/gen_coarse_cage.py --method voxel,
+    MultiGridSim._init_coarse, mg.py:744-866): per muscle, the voxels (side
+    ``voxel``, default ``factor`` x the fine lattice spacing) that hold a fine
+    vertex, each split into 6 tets; a fine vertex interpolates from the voxel
+    tet that contains it (barycentric weights in [0, 1]); the restriction is the
+    Galerkin least-squares projection pinv(P^T P) P^T per muscle (rcond 1e-8, so
+    a corner that no fine vertex references restricts to the origin, as in the
+    reference); the coarse rest is the restriction of the fine rest (FAS); the
+    coarse masses spread ``total_mass`` over the cage by tet volume
+    (_node_mass_redistribute, mg.py:56-68); a node is a pin when the pinned
+    share of its |weight| is at least ``pin_frac`` (mg.py:835-853).  The fit
+    tables are those of the lattice level; with eps = 0 they do not enter the
+    restriction."""
+    pos0 = np.asarray(pos0, np.float64).reshape(-1, 3)
+    tets = np.asarray(tets, np.int64).reshape(-1, 4)
+    nv = pos0.shape[0]
+    comp = components(nv, tets) if comp is None else np.asarray(comp, np.int64)
+    skip = np.zeros(nv, bool) if skip is None else np.asarray(skip, bool)
+    vols = np.abs(np.einsum("ij,ij->i", np.cross(pos0[tets[:, 1]] - pos0[tets[:, 0]],
+                                                 pos0[tets[:, 2]] - pos0[tets[:, 0]]),
+                            pos0[tets[:, 3]] - pos0[tets[:, 0]])) / 6.0
+    tcomp = comp[tets[:, 0]]
+    p_idx = np.zeros((nv, 8), np.int64)
+    p_w = np.zeros((nv, 8), np.float64)
+    cage_pos, all_tets, q0_all, spacing, blk_off, minv = [], [], [], [], [0], []
+    fit_off, fit_fine, fit_c, fit_s, bgroup = [0], [], [], [], []
+    base = 0
+    for c in (np.unique(comp[~skip]) if nv else []):
+        vids = np.nonzero((comp == c) & ~skip)[0]
+        tv = vols[tcomp == c]
+        h = voxel if voxel else factor * ((6.0 * tv.mean()) ** (1.0 / 3.0) if tv.size else 1.0)
+        x = pos0[vids]
+        mn = x.min(0)
+        rel = (x - mn) / h
+        ijk = np.floor(rel).astype(np.int64)
+        occ, vox_of = np.unique(ijk, axis=0, return_inverse=True)
+        vox_of = vox_of.reshape(-1)
+        corners = (occ[:, None, :] + _VOXEL_CORNER[None]).reshape(-1, 3)
+        nodes, node_of = np.unique(corners, axis=0, return_inverse=True)
+        node_of = node_of.reshape(len(occ), 8)
+        n = len(nodes)
+        if n > MAX_BLOCK_NODES or 6 * len(occ) > MAX_BLOCK_TETS:
+            raise ValueError(f"voxel cage of component {c} has {n} nodes / {6 * len(occ)} tets; the coarse "
+                             f"solve takes at most {MAX_BLOCK_NODES} nodes and {MAX_BLOCK_TETS} tets per block "
+                             f"(use a larger voxel)")
+        ctet = node_of[:, _VOXEL_TETS].reshape(-1, 4)                # voxel-major, 6 per voxel
+        tloc, w4 = _voxel_barycentric(rel - ijk)
+        w4 = np.clip(w4, 0.0, 1.0)                                   # rounding at a tet face
+        cn = ctet[6 * vox_of + tloc]                                  # (nv_b, 4) block-local nodes
+        p_idx[vids, :4] = base + cn
+        p_idx[vids, 4:] = (base + cn[:, :1])        # unused slots: weight 0, same block
+        p_w[vids, :4] = w4
+        P = np.zeros((len(vids), n))
+        np.add.at(P, (np.repeat(np.arange(len(vids)), 4), cn.ravel()), w4.ravel())
+        M = np.linalg.pinv(P.T @ P, rcond=1e-8)
+        minv.append(M.ravel())
+        cage_pos.append(mn + nodes * h)
+        q0_all.append(M @ (P.T @ x))                                  # FAS rest of the block
+        all_tets.append(base + ctet)
+        ctr = x.mean(0)
+        xh = np.concatenate([x - ctr, np.ones((len(vids), 1))], 1)
+        fit_s.append(np.linalg.inv(xh.T @ xh))
+        fit_c.append(ctr)
+        fit_fine.append(vids)
+        fit_off.append(fit_off[-1] + len(vids))
+        bgroup.append(int(vgroup[vids[0]]) if vgroup is not None else -1)
+        spacing.append(h)
+        base += n
+        blk_off.append(base)
+    nc = base
+    cpos = np.concatenate(cage_pos) if cage_pos else np.zeros((0, 3))
+    q0 = np.concatenate(q0_all) if q0_all else np.zeros((0, 3))
+    ctets = np.concatenate(all_tets).astype(np.int32) if all_tets else np.zeros((0, 4), np.int32)
+    # masses: total_mass spread by cage tet volume (the baked cage rest, as the reference)
+    r = cpos[ctets]
+    vol = np.abs(np.linalg.det(np.stack([r[:, 1] - r[:, 0], r[:, 2] - r[:, 0], r[:, 3] - r[:, 0]], axis=-1))) / 6.0
+    cmass = np.zeros(nc)
+    np.add.at(cmass, ctets.reshape(-1), np.repeat(vol / 4.0, 4))
+    if cmass.sum() > 0:
+        cmass *= float(total_mass) / cmass.sum()
+    stopped = np.zeros(nc, np.int32)
+    if pin_mask is not None:
+        wacc = np.zeros(nc)
+        np.add.at(wacc, p_idx.ravel(), np.abs(p_w).ravel())
+        pacc = np.zeros(nc)
+        np.add.at(pacc, p_idx.ravel(), (np.abs(p_w) * np.asarray(pin_mask, np.float64)[:, None]).ravel())
+        with np.errstate(divide="ignore", invalid="ignore"):
+            frac = np.where(wacc > 1e-9, pacc / wacc, 0.0)
+        stopped = ((frac >= pin_frac) & (wacc > 1e-9)).astype(np.int32)
+    return CoarseLevel(pos0=q0.astype(np.float32), tets=ctets, mass=cmass.astype(np.float32), stopped=stopped,
+                       p_idx=p_idx.astype(np.int32), p_w=p_w.astype(np.float32),
+                       blk_off=np.asarray(blk_off, np.int32),
+                       minv=(np.concatenate(minv) if minv else np.zeros(0)).astype(np.float32),
+                       eps=np.zeros(len(spacing), np.float32), fit_off=np.asarray(fit_off, np.int32),
+                       fit_fine=(np.concatenate(fit_fine) if fit_fine else np.zeros(0)).astype(np.int32),
+                       fit_center=np.asarray(fit_c, np.float32).reshape(-1, 3),
+                       fit_sinv=np.asarray(fit_s, np.float32).reshape(-1, 4, 4),
+                       iters=int(iters), stiffness=float(stiffness), spacing=np.array(spacing),
+                       n_components=len(spacing), pin_rule=1, blk_vgroup=np.asarray(bgroup, np.int32))
+
+
+def coarse_pin_targets(level: CoarseLevel, w, frame: int) -> np.ndarray:
+    """Restricted bone goal of every coarse node (_refresh_coarse_target,
+    mg.py:1013-1030): each muscle's fine goals follow its bone rigidly, the
+    restriction is linear and reproduces constants, so the restricted goal is
+    the coarse rest moved by that bone."""
+    from . import workloads
+    q0 = level.pos0.astype(np.float64)
+    out = q0.copy()
+    for b in range(len(level.blk_off) - 1):
+        g = int(level.blk_vgroup[b]) if level.blk_vgroup is not None else -1
+        if 0 <= g < len(w.bones):
+            s, e = int(level.blk_off[b]), int(level.blk_off[b + 1])
+            R, t = workloads.bone_transform(w.bones[g], frame)
+            out[s:e] = q0[s:e] @ R.T + t
+    return out.astype(np.float32)
+
+
 def restrict_numpy(level: CoarseLevel, x, rest) -> np.ndarray:
     """q = (P^T P + eps I)^-1 (P^T x + eps q_ref) per block, float64 (the oracle's
     restriction); ``rest`` are the fine rest positions."""
@@ -241,7 +391,8 @@ def coarse_scene(level: CoarseLevel, params):
     p = SimParams(dt=params.dt, gravity=(0.0, 0.0, 0.0), veldamping=0.0, quasi_static=False,
                   contraction_ratio=0.0, fiber_stiffness_scale=1.0)
     pins = np.nonzero(level.stopped)[0].astype(np.int32)
+    held = getattr(level, "pin_rule", 0) == 1      # the reference's rule: pins stopped at their goal
     return Scene(pos0=level.pos0, tets=level.tets, density=1.0, cons=cons, params=p, mass=level.mass,
-                 stopped=None, pin_vids=pins if pins.size else None,
+                 stopped=level.stopped if held else None, pin_vids=pins if pins.size else None,
                  pin_active=np.ones(pins.size, np.int32) if pins.size else None,
                  meta=dict(workload="coarse cage"))

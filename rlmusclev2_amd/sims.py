@@ -50,6 +50,11 @@ def mg_constraint_configs(cfg: dict) -> list:
     return out
 
 
+def multigrid_volume(scene) -> float:
+    volumes, valid = builder.tet_rest_data(scene.pos0, scene.tets)
+    return float(np.abs(volumes[valid]).sum())
+
+
 def build_workload(cfg: dict) -> workloads.Workload:
     syn = cfg.get("synthetic")
     if not syn:
@@ -204,11 +209,23 @@ class MultiGridSim:
         comp = None
         if w is not None and w.info.get("muscle_id") is not None:
             comp = w.info["muscle_id"]
-        self.coarse = multigrid.build_coarse_level(
-            self.scene.pos0, self.scene.tets, mass, pin_mask=pin_mask,
-            factor=float(cfg.get("mg_coarse_factor", 4.0)), iters=int(cfg.get("mg_coarse_iters", 15)),
-            stiffness=float(arap["stiffness"]), pin_frac=float(cfg.get("mg_coarse_pin_frac", 0.5)), comp=comp,
-            skip=w.is_fascia if w is not None else None)
+        cage = str(cfg.get("mg_cage", "lattice")).lower()
+        common = dict(pin_mask=pin_mask, iters=int(cfg.get("mg_coarse_iters", 15)),
+                      stiffness=float(arap["stiffness"]), pin_frac=float(cfg.get("mg_coarse_pin_frac", 0.5)),
+                      comp=comp, skip=w.is_fascia if w is not None else None)
+        if cage == "lattice":
+            self.coarse = multigrid.build_coarse_level(self.scene.pos0, self.scene.tets, mass,
+                                                       factor=float(cfg.get("mg_coarse_factor", 4.0)), **common)
+        elif cage == "voxel":
+            # the reference's scheme (tools/gen_coarse_cage.py --method voxel, mg.py:744-1079)
+            total = float(cfg["total_mass"]) if float(cfg.get("total_mass", -1.0)) > 0 else float(
+                self.scene.density * multigrid_volume(self.scene))
+            vox = float(cfg.get("mg_coarse_voxel", 0.0)) or None
+            self.coarse = multigrid.build_voxel_level(
+                self.scene.pos0, self.scene.tets, total, voxel=vox, factor=float(cfg.get("mg_coarse_factor", 4.0)),
+                vgroup=w.vertex_group if w is not None else None, **common)
+        else:
+            raise NotImplementedError(f"mg_cage={cage!r}: 'lattice' or 'voxel'")
         cs = multigrid.coarse_scene(self.coarse, self.scene.params)
         self.coarse_engine = (engine_cls(cs, device=device, strategy=1) if engine_cls is XpbdEngine
                               else engine_cls(cs))
@@ -229,6 +246,8 @@ class MultiGridSim:
             self.engine.set_attach_targets(workloads.follow_targets(w, frame, self.scene.attach_vids))
         if self.scene.sdf is not None:
             self.engine.set_bone_transforms(workloads.bone_xforms(w, frame))
+        if self.coarse is not None and self.coarse.pin_rule == 1 and w.bones:
+            self.engine.set_coarse_pin_targets(multigrid.coarse_pin_targets(self.coarse, w, frame))
         self._frame = frame
 
     def set_fiber_activation(self, acti) -> None:

@@ -117,3 +117,59 @@ def test_rigid_motion_restricts_exactly_and_gives_no_correction():
     e.set_pin_targets(x[sim.scene.pin_vids])
     e.vcycle()
     np.testing.assert_allclose(e.positions_np(), x, atol=2e-6)
+
+
+def _setup_voxel(strategy, quasi_static=None):
+    """the reference's scheme: voxel cage, barycentric prolongation, Galerkin
+    projection, pins held at the restricted bone goal (mg_cage "voxel")"""
+    cfg = json.loads((CFG / "humanv2_snh_aniso.json").read_text())
+    cfg["synthetic"]["sdf_res"] = 32
+    cfg["mg_cage"] = "voxel"
+    if quasi_static is not None:
+        cfg["quasi_static"] = quasi_static
+    w = workloads.humanv2_scale(3, n_bones=1, per_bone=4, target_tets=8000, fascia_tets=1500, sdf_res=32)
+    sim = sims.MultiGridSim(cfg, workload=w, strategy=strategy, block_tets=256)
+    ref = sims.MultiGridSim(cfg, workload=w, engine_cls=oracle.OracleSim)
+    assert sim.coarse.pin_rule == 1
+    return cfg, w, sim, ref
+
+
+@pytest.mark.parametrize("strategy", [1, 2])
+def test_voxel_vcycle_matches_oracle(strategy):
+    cfg, w, sim, ref = _setup_voxel(strategy)
+    print("voxel cage nodes", sim.coarse.pos0.shape[0], "tets", sim.coarse.tets.shape[0], "pins",
+          int(sim.coarse.stopped.sum()), flush=True)
+    for s in (sim, ref):
+        s.load_frame_targets(4)
+        s.engine.apply_pintotarget()
+        s.engine.integrate()
+    x0 = ref.engine.positions_np()
+    sim.engine.vcycle()
+    ref.engine.vcycle()
+    p, q = sim.positions_np(), ref.positions_np()
+    moved = np.abs(q - x0).max()
+    err = np.abs(p - q).max()
+    cerr = np.abs(sim.coarse_engine.positions_np() - ref.coarse_engine.positions_np()).max()
+    print("voxel vcycle moved", moved, "gpu-oracle fine", err, "coarse", cerr)
+    assert moved > 1e-4
+    assert err <= 1e-3 * moved and cerr <= 1e-3 * moved
+
+
+@pytest.mark.parametrize("quasi_static", [False, True])
+def test_voxel_frames_match_oracle(quasi_static):
+    """10 frames of the c3 constraint set with the reference's V-cycle"""
+    cfg, w, sim, ref = _setup_voxel(2, quasi_static=quasi_static)
+    for f in range(1, 11):
+        a = np.full(w.nt, min(1.0, f / 10.0), np.float32)
+        for s in (sim, ref):
+            s.set_fiber_activation(a)
+            s.step_frame(f)
+        print(f"frame {f} max |dx|", float(np.abs(sim.positions_np() - ref.positions_np()).max()), flush=True)
+    p, q = sim.positions_np(), ref.positions_np()
+    tol = 1e-4 * _diag(w)
+    err = np.abs(p - q).max()
+    print("voxel 10 frames max |dx|", err, "tol", tol)
+    assert np.all(np.isfinite(p)) and err <= tol
+    rs, rr = sim.residuals(), ref.residuals()
+    for k in rr:
+        assert abs(rs[k] - rr[k]) <= 0.01 * rr[k] + 1e-12, (k, rs[k], rr[k])

@@ -109,3 +109,93 @@ def test_vcycle_substeps_do_not_amplify_a_perturbation(quasi_static):
     d = float(np.abs(a.positions_np() - b.positions_np()).max())
     print("divergence / diag", d / diag)
     assert d <= 1e-5 * diag
+
+
+def _voxel():
+    w = workloads.humanv2_scale(3, n_bones=1, per_bone=4, target_tets=8000, fascia_tets=1500, sdf_res=32)
+    pin = np.zeros(w.nv, np.float32)
+    pin[w.pin_vids] = 1.0
+    lv = multigrid.build_voxel_level(w.pos0, w.tets, 70.0, pin_mask=pin, comp=w.info["muscle_id"],
+                                     skip=w.is_fascia, vgroup=w.vertex_group)
+    return w, pin, lv
+
+
+def test_voxel_cage_embeds_every_muscle_vertex():
+    """tools/gen_coarse_cage.py's voxel cage: every fine muscle vertex lies in a
+    tet of its own muscle's cage with barycentric weights in [0, 1] that sum
+    to 1 and reproduce its rest position from the cage lattice; fascia gets no
+    weights"""
+    w, pin, lv = _voxel()
+    m = ~w.is_fascia
+    pw = lv.p_w.astype(np.float64)
+    assert pw.min() >= 0.0 and pw.max() <= 1.0
+    np.testing.assert_allclose(pw[m].sum(1), 1.0, atol=1e-6)
+    assert np.all(pw[~m] == 0.0)
+    # each vertex's four corners are one cage tet
+    tet_keys = {tuple(sorted(t)) for t in lv.tets.tolist()}
+    assert all(tuple(sorted(r)) in tet_keys for r in lv.p_idx[m][:, :4].tolist())
+    # the FAS rest (restriction of the fine rest) interpolates the fine rest exactly
+    x = np.einsum("ik,ikc->ic", pw, lv.pos0[lv.p_idx].astype(np.float64))
+    np.testing.assert_allclose(x[m], w.pos0[m], atol=1e-6)
+    # mass: the total spread by cage tet volume
+    np.testing.assert_allclose(lv.mass.sum(), 70.0, rtol=1e-5)
+    # a pin: at least pin_frac of the node's |weight| comes from pinned fine vertices
+    wacc = np.zeros(len(lv.pos0))
+    pacc = np.zeros(len(lv.pos0))
+    np.add.at(wacc, lv.p_idx.ravel(), pw.ravel())
+    np.add.at(pacc, lv.p_idx.ravel(), (pw * pin[:, None]).ravel())
+    ref = wacc > 1e-9
+    np.testing.assert_array_equal(lv.stopped.astype(bool), ref & (pacc >= 0.5 * wacc))
+    assert 0 < lv.stopped.sum() < len(lv.pos0) and lv.pin_rule == 1
+
+
+def test_voxel_restriction_is_the_galerkin_projection():
+    """eps = 0: q = pinv(P^T P) P^T x per muscle, assembled densely here; an
+    affine map of the rest pose restricts to the same map of the FAS rest on
+    every node that a fine vertex references"""
+    w, pin, lv = _voxel()
+    rng = np.random.default_rng(4)
+    x = w.pos0 + rng.normal(scale=2e-3, size=w.pos0.shape)
+    q = oracle.CageRestriction(lv)(x, w.pos0)
+    q2 = multigrid.restrict_numpy(lv, x, w.pos0)
+    for b in range(lv.n_components):
+        s, e = int(lv.blk_off[b]), int(lv.blk_off[b + 1])
+        vids = lv.fit_fine[lv.fit_off[b]:lv.fit_off[b + 1]]
+        P = np.zeros((len(vids), e - s))
+        for k in range(8):
+            np.add.at(P, (np.arange(len(vids)), lv.p_idx[vids, k] - s), lv.p_w[vids, k].astype(np.float64))
+        want = np.linalg.pinv(P.T @ P, rcond=1e-8) @ (P.T @ x[vids])
+        np.testing.assert_allclose(q[s:e], want, atol=1e-9)
+        np.testing.assert_allclose(q2[s:e], want, atol=1e-4 * max(1.0, np.abs(want).max()))
+    A = np.array([[1.1, 0.05, 0.0], [0.0, 0.95, 0.02], [0.01, 0.0, 1.05]])
+    t = np.array([0.02, 0.0, -0.01])
+    qa = oracle.CageRestriction(lv)(w.pos0.astype(np.float64) @ A.T + t, w.pos0)
+    refd = np.zeros(len(lv.pos0), bool)
+    refd[lv.p_idx[lv.p_w > 1e-6]] = True
+    np.testing.assert_allclose(qa[refd], lv.pos0[refd].astype(np.float64) @ A.T + t, atol=1e-5)
+
+
+@pytest.mark.parametrize("quasi_static", [True, False])
+def test_voxel_vcycle_substeps_do_not_amplify_a_perturbation(quasi_static):
+    """the reference's scheme (voxel cage, pins held at the restricted bone
+    goal, 15 sweeps): two oracle runs 1e-7 m apart stay within 1e-5 x bbox
+    diagonal over two frames"""
+    cfg = json.loads((CFG / "humanv2_snh_aniso.json").read_text())
+    cfg["synthetic"]["sdf_res"] = 32
+    cfg.update(quasi_static=quasi_static, mg_cage="voxel")
+    w = workloads.humanv2_scale(3, n_bones=1, per_bone=4, target_tets=8000, fascia_tets=1500, sdf_res=32)
+    a = sims.MultiGridSim(cfg, workload=w, engine_cls=oracle.OracleSim)
+    b = sims.MultiGridSim(cfg, workload=w, engine_cls=oracle.OracleSim)
+    assert a.coarse.pin_rule == 1 and a.coarse.stopped.sum() > 0
+    rng = np.random.default_rng(0)
+    x = b.engine.positions_np()
+    b.engine.set_state(pos=x + rng.normal(scale=1e-7, size=x.shape).astype(np.float32))
+    diag = float(np.linalg.norm(w.pos0.max(0) - w.pos0.min(0)))
+    for f in (1, 2):
+        act = np.full(w.nt, f / 5.0, np.float32)
+        for s in (a, b):
+            s.set_fiber_activation(act)
+            s.step_frame(f)
+    d = float(np.abs(a.positions_np() - b.positions_np()).max())
+    print("divergence / diag", d / diag)
+    assert d <= 1e-5 * diag
