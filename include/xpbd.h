@@ -197,7 +197,7 @@ int32_t xpbd_solve_constraints(xpbd_ctx *ctx);        /* solve_constraints(run_e
 int32_t xpbd_apply_dP(xpbd_ctx *ctx);                 /* apply_dP()                          */
 int32_t xpbd_apply_pins(xpbd_ctx *ctx);               /* apply_pintotarget()                 */
 int32_t xpbd_iterate(xpbd_ctx *ctx, int32_t n);       /* n x (clear, dispatch, apply_dP, pin), fused */
-int32_t xpbd_collide(xpbd_ctx *ctx);                  /* collision_solvers: bone SDF push    */
+int32_t xpbd_collide(xpbd_ctx *ctx);                  /* collision_solvers: bone push (grid SDF or bone mesh) */
 int32_t xpbd_update_velocities(xpbd_ctx *ctx);        /* update_velocities()                 */
 /* MultiGridSim.substep (mg.py:1322-1384): integrate, pin, iters x iterate, collide, velocity */
 int32_t xpbd_substep(xpbd_ctx *ctx, int32_t iters);
@@ -205,6 +205,48 @@ int32_t xpbd_substep(xpbd_ctx *ctx, int32_t iters);
  * captured hipGraph of the whole frame (re-captured when iters/nsub change) */
 int32_t xpbd_step_frame(xpbd_ctx *ctx, int32_t nsub, int32_t iters, int32_t use_graph);
 int32_t xpbd_synchronize(xpbd_ctx *ctx);
+
+/* ---- exact-mesh bone collider ------------------------------------------
+ * AnimatedBoneMeshCollider (src/example/humanV2.py:319-418): one triangle mesh over the
+ * skeleton (wp.Mesh, built once; points refreshed and refit per frame), the push of
+ * solve_mesh_contact_pos_kernel (src/muscle/collision/sdf/kernel.py:123-170) and the
+ * penetration meter _mark_inside_kernel (humanV2.py:67-87) with CollisionMeter.measure
+ * (humanV2.py:163-167).  Query: the closest point among the faces nearer than max_dist, signed
+ * by the angle-weighted pseudonormal of the closest feature (face, edge or vertex), as Warp's
+ * mesh_query_point_sign_normal; push: x -= n err with err = d - contact_offset < 0,
+ * n = (x - closest) / |x - closest| * sign, skipped for |x - closest| < 1e-9 and, when
+ * max_push > 0, for -err > max_push.  Vertex groups restrict which sub-meshes ("bones") a
+ * vertex tests (the closest point over the union of those sub-meshes). */
+typedef struct {
+    int32_t nverts;             /* bone mesh vertices                                          */
+    const float *verts;         /* nverts*3 positions of the current frame                     */
+    int32_t ntris;
+    const int32_t *tris;        /* ntris*3 vertex ids, (b - a) x (c - a) pointing out of the bone
+                                   (the reference flips the SNU bones' winding to get this)     */
+    int32_t nbones;             /* sub-meshes: triangles tri_off[b] .. tri_off[b + 1] - 1        */
+    const int32_t *tri_off;     /* nbones+1; NULL with nbones <= 1: one sub-mesh                 */
+    int32_t ngroups;            /* 0: every vertex tests every sub-mesh                          */
+    const int32_t *goff;        /* ngroups+1: group g tests sub-meshes goff[g] .. goff[g + 1] - 1 */
+    const int32_t *vgroup;      /* nv group ids, caller's vertex order                           */
+    const float *coll_w;        /* nv, 0 = not pushed (coll_inv_mass, humanV2.py:361-366); NULL: all */
+    const float *meter_w;       /* nv, 0 = not metered (meter_mask, humanV2.py:367-375); NULL: all  */
+    float max_dist, contact_offset, max_push;
+    int32_t resolve;            /* 1: the substep's collision stage pushes with this mesh (the grid
+                                   SDF is then not used); 0: meter only                           */
+} xpbd_bone_mesh_desc;
+/* desc = NULL removes the collider.  Allowed before or after xpbd_finalize. */
+int32_t xpbd_set_bone_mesh(xpbd_ctx *ctx, const xpbd_bone_mesh_desc *desc);
+/* per frame: new bone vertex positions (nverts*3, same topology), then the refit
+ * (AnimatedBoneMeshCollider.set_frame, humanV2.py:386-398); staged like the other setters */
+int32_t xpbd_set_bone_mesh_points(xpbd_ctx *ctx, const float *verts);
+/* turn the push in the collision stage on or off (the sim's collision_solvers list) */
+int32_t xpbd_set_bone_mesh_resolve(xpbd_ctx *ctx, int32_t resolve);
+/* one push of the current positions (AnimatedBoneMeshCollider.__call__, humanV2.py:400-406) */
+int32_t xpbd_bone_mesh_push(xpbd_ctx *ctx);
+/* the meter on the current positions: depth (nv floats, caller order, > 0 inside; may be NULL),
+ * the number of metered vertices deeper than collide_eps and the largest depth; synchronises */
+int32_t xpbd_bone_penetration(xpbd_ctx *ctx, float collide_eps, float *depth, int32_t *count,
+                              float *max_depth);
 
 /* ---- tri-tri DCD (UdcfDcdCollider, src/muscle/collision/udcf/solver.py:146-457) --- */
 typedef struct {

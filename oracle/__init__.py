@@ -78,6 +78,11 @@ def load():
         lib.orc_sdf_grid_push.argtypes = [C.c_int, _P, _P, C.c_int, C.c_int, _P, _P, _P, _P, _P, _P,
                                           C.c_float, C.c_float, _P]
         lib.orc_sdf_grid_push.restype = C.c_int
+        lib.orc_mesh_normals.argtypes = [C.c_int, _P, C.c_int, _P, _P, _P]
+        lib.orc_mesh_query.argtypes = [C.c_int, _P, C.c_int, _P, C.c_int, _P, C.c_int, _P, _P, _P, _P, _P, _P,
+                                       C.c_float, _P, _P, _P, _P, C.c_int]
+        lib.orc_mesh_push.argtypes = [C.c_int, _P, _P, _P, _P, _P, C.c_float, C.c_float]
+        lib.orc_mesh_push.restype = C.c_int
         lib.orc_residuals.argtypes = [C.c_int, _P, _P, _P, _P, _P, _P, _P]
         lib.orc_count_inverted.argtypes = [C.c_int, _P, _P, _P]
         lib.orc_count_inverted.restype = C.c_int
@@ -111,6 +116,81 @@ def precompute_rest(pos0, tets, density):
     tot = np.zeros(1, np.float32)
     lib.orc_precompute_rest(nt, _p(pos0), _p(tets), C.c_float(density), _p(rv), _p(rm), _p(m), _p(tot))
     return rv, rm, m
+
+
+def mesh_adjacency(tris) -> np.ndarray:
+    """(ntris, 3) neighbour across edge ab, bc, ca: the first other triangle (in
+    triangle order) on the same undirected edge, -1 on a boundary edge."""
+    t = np.asarray(tris, np.int64).reshape(-1, 3)
+    nt = len(t)
+    u = np.concatenate([t[:, 0], t[:, 1], t[:, 2]])
+    v = np.concatenate([t[:, 1], t[:, 2], t[:, 0]])
+    f = np.tile(np.arange(nt), 3)
+    e = np.repeat(np.arange(3), nt)
+    key = np.minimum(u, v) * (int(t.max()) + 1) + np.maximum(u, v)
+    order = np.lexsort((e, f, key))
+    k, fs = key[order], f[order]
+    n = len(k)
+    start = np.r_[0, np.nonzero(np.diff(k))[0] + 1]
+    grp = np.repeat(np.arange(len(start)), np.diff(np.r_[start, n]))
+    first = fs[start][grp]
+    cand = np.where(fs != first, np.arange(n), n)
+    sec_pos = np.minimum.reduceat(cand, start)
+    sec = np.where(sec_pos < n, fs[np.minimum(sec_pos, n - 1)], -1)[grp]
+    nb = np.where(fs != first, first, sec)
+    adj = np.full((nt, 3), -1, np.int32)
+    adj[f[order], e[order]] = nb
+    return adj
+
+
+class MeshQuery:
+    """Brute-force closest point + pseudonormal sign over a bone triangle mesh
+    (orc_mesh_query): the checker of the library's BVH query."""
+
+    def __init__(self, verts, tris, tri_off=None):
+        self.lib = load()
+        self.tris = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
+        nt = self.tris.shape[0]
+        self.tri_off = np.ascontiguousarray([0, nt] if tri_off is None else tri_off, np.int32)
+        self.adj = np.ascontiguousarray(mesh_adjacency(self.tris), np.int32)
+        self.set_points(verts)
+
+    def set_points(self, verts):
+        self.verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+        nt = self.tris.shape[0]
+        self.fn = np.zeros((nt, 3), np.float64)
+        self.vn = np.zeros((self.verts.shape[0], 3), np.float64)
+        self.lib.orc_mesh_normals(self.verts.shape[0], _p(self.verts), nt, _p(self.tris), _p(self.fn), _p(self.vn))
+
+    def query(self, q, max_dist, vgroup=None, goff=None, nthreads=8):
+        q = np.ascontiguousarray(q, np.float32).reshape(-1, 3)
+        n = q.shape[0]
+        sd = np.zeros(n, np.float64)
+        cp = np.zeros((n, 3), np.float64)
+        face = np.zeros(n, np.int32)
+        region = np.zeros(n, np.int32)
+        vg = None if vgroup is None else np.ascontiguousarray(vgroup, np.int32)
+        go = None if goff is None else np.ascontiguousarray(goff, np.int32)
+        self.lib.orc_mesh_query(n, _p(q), self.verts.shape[0], _p(self.verts), self.tris.shape[0], _p(self.tris),
+                                len(self.tri_off) - 1, _p(self.tri_off), _p(self.fn), _p(self.vn), _p(self.adj),
+                                _p(vg), _p(go), C.c_float(max_dist), _p(sd), _p(cp), _p(face), _p(region),
+                                int(nthreads))
+        return sd, cp, face, region
+
+    def push(self, pos, max_dist, contact_offset, max_push, coll_w=None, vgroup=None, goff=None, nthreads=8):
+        """solve_mesh_contact_pos_kernel on pos (float32, in place); returns the number pushed"""
+        sd, cp, face, _ = self.query(pos, max_dist, vgroup, goff, nthreads)
+        cw = None if coll_w is None else np.ascontiguousarray(coll_w, np.float32)
+        return self.lib.orc_mesh_push(pos.shape[0], _p(pos), _p(cw), _p(sd), _p(cp), _p(face),
+                                      C.c_float(contact_offset), C.c_float(max_push))
+
+    def depth(self, pos, max_dist, meter_w=None, vgroup=None, goff=None, nthreads=8):
+        """_mark_inside_kernel: > 0 inside, 0 outside or unmetered"""
+        sd, _, face, _ = self.query(pos, max_dist, vgroup, goff, nthreads)
+        d = np.where((face >= 0) & (sd < 0.0), -sd, 0.0)
+        if meter_w is not None:
+            d[np.asarray(meter_w) == 0] = 0.0
+        return d
 
 
 class OracleSim:
@@ -168,6 +248,35 @@ class OracleSim:
             self.vgroup = None if sd.vertex_group is None else np.ascontiguousarray(sd.vertex_group, np.int32)
             self.goff = None if sd.group_bone_off is None else np.ascontiguousarray(sd.group_bone_off, np.int32)
             self.cells = np.full((self.NV, 2), -1, np.int32)
+
+        self.bm = None
+        if s.bone_mesh is not None:
+            self.set_bone_mesh(s.bone_mesh)
+
+    # exact-mesh bone collider -------------------------------------------------
+    def set_bone_mesh(self, bm):
+        self.bm = bm
+        if bm is None:
+            self.bmq = None
+            return
+        self.bmq = MeshQuery(bm.verts, bm.tris, bm.tri_off)
+        self.bm_resolve = bool(bm.resolve)
+
+    def set_bone_mesh_points(self, verts):
+        self.bmq.set_points(verts)
+
+    def set_bone_mesh_resolve(self, on):
+        self.bm_resolve = bool(on)
+
+    def bone_mesh_push(self):
+        bm = self.bm
+        return self.bmq.push(self.pos, bm.max_dist, bm.contact_offset, bm.max_push, bm.coll_w, bm.vertex_group,
+                             bm.group_bone_off, self.nthreads)
+
+    def bone_penetration(self, collide_eps=1e-6, want_depth=True):
+        bm = self.bm
+        d = self.bmq.depth(self.pos, bm.max_dist, bm.meter_w, bm.vertex_group, bm.group_bone_off, self.nthreads)
+        return (d.astype(np.float32) if want_depth else None), int((d > collide_eps).sum()), float(d.max(initial=0.0))
 
     def set_params(self, p):
         q = OrcParams()
@@ -236,6 +345,8 @@ class OracleSim:
             self.apply_pintotarget()
 
     def collide(self):
+        if self.bm is not None and self.bm_resolve:
+            return self.bone_mesh_push()
         if self.sdf is None:
             return 0
         sd = self.sdf

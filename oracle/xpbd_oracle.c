@@ -1100,3 +1100,193 @@ void orc_dcd_apply(int n, float *pos, const float *dP, const float *dPw, int use
         }
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* exact-mesh bone collider                                            */
+/*                                                                     */
+/* solve_mesh_contact_pos_kernel (src/muscle/collision/sdf/kernel.py:  */
+/* 123-170) and the penetration meter _mark_inside_kernel              */
+/* (src/example/humanV2.py:67-87) query the bone mesh through Warp     */
+/* 1.13.0's mesh_query_point_sign_normal (warp-lang, uv.lock:831-833), */
+/* which is not vendored: its documented behaviour is restated here.   */
+/* The closest point on the mesh among the faces nearer than max_dist  */
+/* (none: no result), found by brute force over every triangle, in     */
+/* double precision; the inside/outside sign is that of the            */
+/* angle-weighted pseudonormal of the closest feature (Baerentzen and  */
+/* Aanaes, "Signed distance computation using the angle weighted       */
+/* pseudonormal", IEEE TVCG 11(3), 2005): the face normal inside a     */
+/* face, the sum of the two face normals on an edge, the sum of the    */
+/* incident face normals weighted by their corner angles at a vertex.  */
+/* Closest point on a triangle: Ericson, Real-Time Collision Detection */
+/* (2005), section 5.1.5 (Voronoi regions of the vertices, edges and   */
+/* face).                                                              */
+/* ------------------------------------------------------------------ */
+typedef struct { double x, y, z; } d3;
+static inline d3 dmk(double x, double y, double z) { d3 r = {x, y, z}; return r; }
+static inline d3 dld(const float *a, int i) { return dmk(a[3 * i], a[3 * i + 1], a[3 * i + 2]); }
+static inline d3 dsub(d3 a, d3 b) { return dmk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline d3 dadd(d3 a, d3 b) { return dmk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline d3 dscl(double s, d3 a) { return dmk(s * a.x, s * a.y, s * a.z); }
+static inline double ddot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline d3 dcross(d3 a, d3 b) {
+    return dmk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+/* closest point of segment a-b to p; *t = parameter */
+static d3 seg_closest(d3 p, d3 a, d3 b, double *t) {
+    d3 ab = dsub(b, a);
+    double l2 = ddot(ab, ab), s = 0.0;
+    if (l2 > 0.0) {
+        s = ddot(dsub(p, a), ab) / l2;
+        s = s < 0.0 ? 0.0 : (s > 1.0 ? 1.0 : s);
+    }
+    *t = s;
+    return dadd(a, dscl(s, ab));
+}
+
+/* region: 0 face, 1 2 3 vertex a b c, 4 edge ab, 5 edge bc, 6 edge ca */
+static d3 tri_closest(d3 p, d3 a, d3 b, d3 c, int *region) {
+    d3 ab = dsub(b, a), ac = dsub(c, a), ap = dsub(p, a);
+    double d1 = ddot(ab, ap), d2 = ddot(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0) { *region = 1; return a; }
+    d3 bp = dsub(p, b);
+    double d3_ = ddot(ab, bp), d4 = ddot(ac, bp);
+    if (d3_ >= 0.0 && d4 <= d3_) { *region = 2; return b; }
+    double vc = d1 * d4 - d3_ * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3_ <= 0.0) { *region = 4; return dadd(a, dscl(d1 / (d1 - d3_), ab)); }
+    d3 cp = dsub(p, c);
+    double d5 = ddot(ab, cp), d6 = ddot(ac, cp);
+    if (d6 >= 0.0 && d5 <= d6) { *region = 3; return c; }
+    double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) { *region = 6; return dadd(a, dscl(d2 / (d2 - d6), ac)); }
+    double va = d3_ * d6 - d5 * d4;
+    if (va <= 0.0 && (d4 - d3_) >= 0.0 && (d5 - d6) >= 0.0) {
+        *region = 5;
+        return dadd(b, dscl((d4 - d3_) / ((d4 - d3_) + (d5 - d6)), dsub(c, b)));
+    }
+    double den = va + vb + vc;
+    if (!(den > 0.0)) {          /* zero-area triangle: nearest of its three edges */
+        double t;
+        d3 q[3];
+        q[0] = seg_closest(p, a, b, &t);
+        q[1] = seg_closest(p, b, c, &t);
+        q[2] = seg_closest(p, c, a, &t);
+        int k = 0;
+        double best = ddot(dsub(p, q[0]), dsub(p, q[0]));
+        for (int j = 1; j < 3; ++j) {
+            double e = ddot(dsub(p, q[j]), dsub(p, q[j]));
+            if (e < best) { best = e; k = j; }
+        }
+        *region = 4 + k;
+        return q[k];
+    }
+    *region = 0;
+    return dadd(a, dadd(dscl(vb / den, ab), dscl(vc / den, ac)));
+}
+
+/* unit face normals (ntris x 3) and angle-weighted vertex pseudonormals (nbv x 3) */
+void orc_mesh_normals(int nbv, const float *bv, int ntris, const int *tris, double *fn, double *vn) {
+    memset(vn, 0, sizeof(double) * 3 * (size_t)nbv);
+    for (int f = 0; f < ntris; ++f) {
+        const int *t = tris + 3 * f;
+        d3 P[3] = {dld(bv, t[0]), dld(bv, t[1]), dld(bv, t[2])};
+        d3 n = dcross(dsub(P[1], P[0]), dsub(P[2], P[0]));
+        double l = sqrt(ddot(n, n));
+        n = l > 0.0 ? dscl(1.0 / l, n) : dmk(0, 0, 0);
+        fn[3 * f] = n.x; fn[3 * f + 1] = n.y; fn[3 * f + 2] = n.z;
+        for (int k = 0; k < 3; ++k) {
+            d3 e1 = dsub(P[(k + 1) % 3], P[k]), e2 = dsub(P[(k + 2) % 3], P[k]);
+            double l1 = sqrt(ddot(e1, e1)), l2 = sqrt(ddot(e2, e2));
+            double cang = (l1 > 0.0 && l2 > 0.0) ? ddot(e1, e2) / (l1 * l2) : 1.0;
+            cang = cang > 1.0 ? 1.0 : (cang < -1.0 ? -1.0 : cang);
+            double ang = acos(cang);
+            double *v = vn + 3 * t[k];
+            v[0] += ang * n.x; v[1] += ang * n.y; v[2] += ang * n.z;
+        }
+    }
+}
+
+/* closest point over triangles [t0, t1) nearer than sqrt(*best_d2); returns the face or -1 */
+static int mesh_scan(d3 p, const float *bv, const int *tris, int t0, int t1, double *best_d2, d3 *best_cp,
+                     int *best_region) {
+    int face = -1;
+    for (int f = t0; f < t1; ++f) {
+        const int *t = tris + 3 * f;
+        int region;
+        d3 q = tri_closest(p, dld(bv, t[0]), dld(bv, t[1]), dld(bv, t[2]), &region);
+        d3 d = dsub(p, q);
+        double e = ddot(d, d);
+        if (e < *best_d2) { *best_d2 = e; *best_cp = q; *best_region = region; face = f; }
+    }
+    return face;
+}
+
+static double feature_sign(d3 p, d3 cp, int face, int region, const int *tris, const double *fn,
+                           const double *vn, const int *adj) {
+    const double *nf = fn + 3 * face;
+    d3 N = dmk(nf[0], nf[1], nf[2]);
+    if (region >= 1 && region <= 3) {
+        const double *v = vn + 3 * tris[3 * face + region - 1];
+        N = dmk(v[0], v[1], v[2]);
+    } else if (region >= 4) {
+        /* edge ab = 0, bc = 1, ca = 2 */
+        int g = adj ? adj[3 * face + (region - 4)] : -1;
+        if (g >= 0) N = dadd(N, dmk(fn[3 * g], fn[3 * g + 1], fn[3 * g + 2]));
+    }
+    return ddot(dsub(p, cp), N) < 0.0 ? -1.0 : 1.0;
+}
+
+/* query nq points; bones b0..b1 of the point's group (vgroup/goff NULL: every bone);
+ * out: sd (signed distance, < 0 inside; 0 without a result), cp (3 per point), face
+ * (-1: no face nearer than max_dist), region */
+void orc_mesh_query(int nq, const float *q, int nbv, const float *bv, int ntris, const int *tris,
+                    int nbones, const int *tri_off, const double *fn, const double *vn, const int *adj,
+                    const int *vgroup, const int *goff, float max_dist, double *sd, double *cp, int *face,
+                    int *region, int nthreads) {
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int i = 0; i < nq; ++i) {
+        d3 p = dld(q, i);
+        double best = (double)max_dist * (double)max_dist;
+        d3 bcp = dmk(0, 0, 0);
+        int breg = -1, bf = -1;
+        int b0 = 0, b1 = nbones;
+        if (vgroup && goff) { b0 = goff[vgroup[i]]; b1 = goff[vgroup[i] + 1]; }
+        for (int b = b0; b < b1; ++b) {
+            int f = mesh_scan(p, bv, tris, tri_off[b], tri_off[b + 1], &best, &bcp, &breg);
+            if (f >= 0) bf = f;
+        }
+        face[i] = bf;
+        region[i] = breg;
+        if (bf < 0) {
+            sd[i] = 0.0;
+            cp[3 * i] = cp[3 * i + 1] = cp[3 * i + 2] = 0.0;
+            continue;
+        }
+        cp[3 * i] = bcp.x; cp[3 * i + 1] = bcp.y; cp[3 * i + 2] = bcp.z;
+        sd[i] = sqrt(best) * feature_sign(p, bcp, bf, breg, tris, fn, vn, adj);
+    }
+}
+
+/* the push of solve_mesh_contact_pos_kernel (kernel.py:155-170) from the query above:
+ * vertices with coll_w == 0 are skipped; returns the number pushed */
+int orc_mesh_push(int nv, float *pos, const float *coll_w, const double *sd, const double *cp, const int *face,
+                  float contact_offset, float max_push) {
+    int pushed = 0;
+    for (int i = 0; i < nv; ++i) {
+        if (coll_w && coll_w[i] == 0.0f) continue;
+        if (face[i] < 0) continue;
+        d3 x = dld(pos, i);
+        d3 delta = dsub(x, dmk(cp[3 * i], cp[3 * i + 1], cp[3 * i + 2]));
+        double dlen = sqrt(ddot(delta, delta));
+        if (dlen < 1.0e-9) continue;
+        double sgn = sd[i] < 0.0 ? -1.0 : 1.0;
+        double err = dlen * sgn - (double)contact_offset;
+        if (err < 0.0 && (max_push <= 0.0f || -err <= (double)max_push)) {
+            d3 n = dscl(sgn / dlen, delta);
+            d3 y = dsub(x, dscl(err, n));
+            pos[3 * i] = (float)y.x; pos[3 * i + 1] = (float)y.y; pos[3 * i + 2] = (float)y.z;
+            ++pushed;
+        }
+    }
+    return pushed;
+}

@@ -126,3 +126,18 @@ void orc_dcd_apply(int n, float *pos, const float *dP, const float *dPw, int use
 #ifdef __cplusplus
 }
 #endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* ---- exact-mesh bone collider (src/muscle/collision/sdf/kernel.py:123-170) -- */
+void orc_mesh_normals(int nbv, const float *bv, int ntris, const int *tris, double *fn, double *vn);
+void orc_mesh_query(int nq, const float *q, int nbv, const float *bv, int ntris, const int *tris,
+                    int nbones, const int *tri_off, const double *fn, const double *vn, const int *adj,
+                    const int *vgroup, const int *goff, float max_dist, double *sd, double *cp, int *face,
+                    int *region, int nthreads);
+int  orc_mesh_push(int nv, float *pos, const float *coll_w, const double *sd, const double *cp,
+                   const int *face, float contact_offset, float max_push);
+#ifdef __cplusplus
+}
+#endif

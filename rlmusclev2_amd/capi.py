@@ -56,6 +56,14 @@ class XpbdDcdParams(C.Structure):
                 ("rebuild_substeps", C.c_int32), ("max_pairs", C.c_int64)]
 
 
+class XpbdBoneMeshDesc(C.Structure):
+    _fields_ = [("nverts", C.c_int32), ("verts", C.c_void_p), ("ntris", C.c_int32), ("tris", C.c_void_p),
+                ("nbones", C.c_int32), ("tri_off", C.c_void_p), ("ngroups", C.c_int32), ("goff", C.c_void_p),
+                ("vgroup", C.c_void_p), ("coll_w", C.c_void_p), ("meter_w", C.c_void_p),
+                ("max_dist", C.c_float), ("contact_offset", C.c_float), ("max_push", C.c_float),
+                ("resolve", C.c_int32)]
+
+
 class XpbdMlpDev(C.Structure):
     _fields_ = [("W1", C.c_void_p), ("b1", C.c_void_p), ("W2", C.c_void_p), ("b2", C.c_void_p),
                 ("W3", C.c_void_p), ("b3", C.c_void_p)]
@@ -113,6 +121,11 @@ SIGNATURES = {
     "xpbd_count_inverted": [_P, _P],
     "xpbd_get_sdf_cells": [_P, _P],
     "xpbd_set_dcd": [_P, _I, _P, _P, C.POINTER(XpbdDcdParams)],
+    "xpbd_set_bone_mesh": [_P, C.POINTER(XpbdBoneMeshDesc)],
+    "xpbd_set_bone_mesh_points": [_P, _P],
+    "xpbd_set_bone_mesh_resolve": [_P, _I],
+    "xpbd_bone_mesh_push": [_P],
+    "xpbd_bone_penetration": [_P, _F, _P, _P, _P],
     "xpbd_dcd_solve": [_P, _I],
     "xpbd_dcd_pairs": [_P, _P, C.c_int64, _P],
     "xpbd_dcd_stats": [_P, _P, _P, _P],

@@ -1311,6 +1311,8 @@ __global__ void k_post(int nv, const float4 *Xc, float4 *X, float4 *PP, float4 *
     X[i] = x;
 }
 
+#include "xpbd_bonemesh.h"
+
 __global__ void k_apply_dpw(int nv, float4 *X, const float4 *dpw, const float4 *pin, int do_pin) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nv) return;
@@ -2553,6 +2555,23 @@ struct HostStage {
     int next = 0;
 };
 
+// exact-mesh bone collider: the caller's mesh, the BVH built from it and its device copy
+struct BoneMesh {
+    bool set = false, resolve = false, built = false;
+    int nbv = 0, ntris = 0, nbones = 0;
+    std::vector<float> pts;                       // nbv*3, current frame
+    std::vector<int32_t> tris, tri_off, goff, vgroup;
+    std::vector<float> cw, mw;
+    float max_dist = 0.2f, contact_offset = 0.f, max_push = -1.f;
+    BmTables tab;                                 // the tree's tables (BVH order)
+    std::vector<void *> bufs;
+    BoneMeshDev dev{};
+    bm_f4 *d_pts = nullptr;
+    float *d_depth = nullptr;
+    uint32_t *d_stat = nullptr;
+    HostStage st;
+};
+
 struct xpbd_ctx {
     int device = 0;
     hipStream_t own_stream = nullptr, stream = nullptr;
@@ -2670,6 +2689,7 @@ struct xpbd_ctx {
     std::vector<int32_t> dcd_tris, dcd_group;
     xpbd_dcd_params dcd_p{};
     DcdDev dcd{};
+    BoneMesh bm;
     uint32_t *d_scan_bsum = nullptr;
     int n_scan_bsum = 0;
     // captured frames, keyed by (substeps, iterations, lambda mode); a small
@@ -2730,7 +2750,18 @@ void drop_graph_slot(xpbd_ctx::GraphSlot &g) {
     g.used = 0;
 }
 
+void bm_free(BoneMesh &b) {
+    for (void *p : b.bufs) (void)hipFree(p);
+    b.bufs.clear();
+    b.built = false;
+    b.dev = BoneMeshDev{};
+    b.d_pts = nullptr;
+    b.d_depth = nullptr;
+    b.d_stat = nullptr;
+}
+
 void free_device(xpbd_ctx *c) {
+    bm_free(c->bm);
     for (auto &g : c->gslot) drop_graph_slot(g);
     for (auto &b : c->bufs) (void)hipFree(b.p);
     c->bufs.clear();
@@ -2800,6 +2831,108 @@ int stage_submit(xpbd_ctx *c, HostStage &s, int slot, void *dst, size_t bytes) {
 
 void invalidate_graph(xpbd_ctx *c) {
     for (auto &g : c->gslot) drop_graph_slot(g);
+}
+
+// ---- exact-mesh bone collider: BVH build (host, once per mesh) and refit / query launches
+template <typename T>
+int bm_alloc(xpbd_ctx *c, T **p, size_t count) {
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(count * sizeof(T), 16));
+    if (e != hipSuccess) return fail(c, XPBD_EOOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->bm.bufs.push_back(q);
+    *p = (T *)q;
+    return XPBD_OK;
+}
+
+template <typename T>
+int bm_upload(xpbd_ctx *c, T **p, const std::vector<T> &v) {
+    int r = bm_alloc(c, p, v.size());
+    if (r) return r;
+    if (!v.empty()) HIPCHK(c, hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return XPBD_OK;
+}
+
+int bm_refit(xpbd_ctx *c) {
+    BoneMesh &b = c->bm;
+    auto g = [](int n) { return dim3((unsigned)((n + 255) / 256)); };
+    hipLaunchKernelGGL(k_bm_soup, g(b.ntris), dim3(256), 0, c->stream, b.dev);
+    hipLaunchKernelGGL(k_bm_vnorm, g(b.nbv), dim3(256), 0, c->stream, b.dev);
+    hipLaunchKernelGGL(k_bm_refit_low, dim3((unsigned)b.tab.sub.size()), dim3(1 << BM_SUB_LOG), 0, c->stream, b.dev);
+    hipLaunchKernelGGL(k_bm_refit_top, dim3((unsigned)b.nbones), dim3(1024), 0, c->stream, b.dev);
+    HIPCHK(c, hipGetLastError());
+    return XPBD_OK;
+}
+
+int bm_push(xpbd_ctx *c, float4 *X) {
+    hipLaunchKernelGGL(k_bm_query<0>, dim3((unsigned)((c->nv + 255) / 256)), dim3(256), 0, c->stream, c->nv, X,
+                       c->bm.dev, nullptr, 0.0f, nullptr);
+    HIPCHK(c, hipGetLastError());
+    return XPBD_OK;
+}
+
+int bm_upload_points(xpbd_ctx *c) {
+    BoneMesh &b = c->bm;
+    int slot;
+    void *h;
+    const size_t bytes = (size_t)b.nbv * sizeof(float4);
+    int r = stage_acquire(c, b.st, bytes, slot, h);
+    if (r) return r;
+    bm_f4 *q = (bm_f4 *)h;
+    for (int i = 0; i < b.nbv; ++i) q[i] = bm_f4{b.pts[3 * i], b.pts[3 * i + 1], b.pts[3 * i + 2], 0.f};
+    if ((r = stage_submit(c, b.st, slot, b.d_pts, bytes))) return r;
+    return bm_refit(c);
+}
+
+// the tree's tables from the first pose (bm_build_tables), vertex-side tables
+// in the library's vertex order, device copies, then the first refit
+int bm_build(xpbd_ctx *c) {
+    BoneMesh &b = c->bm;
+    bm_free(b);
+    if (!b.set) return XPBD_OK;
+    const int nb = b.nbones, nt = b.ntris, nbv = b.nbv, nv = c->nv;
+    BmTables &T = b.tab;
+    std::string e = bm_build_tables(nbv, b.pts.data(), nt, b.tris.data(), nb, b.tri_off.data(), T);
+    if (!e.empty()) return fail(c, XPBD_EINVAL, e);
+    std::vector<int32_t> vg(nv), goff = b.goff;
+    std::vector<float> cw(nv), mw;
+    if (goff.empty()) goff = {0, nb};
+    if (!b.mw.empty()) mw.resize(nv);
+    for (int n = 0; n < nv; ++n) {
+        const int o = c->L.perm[n];
+        vg[n] = b.vgroup.empty() ? 0 : b.vgroup[o];
+        cw[n] = b.cw.empty() ? 1.0f : b.cw[o];
+        if (!mw.empty()) mw[n] = b.mw[o];
+    }
+    BoneMeshDev &d = b.dev;
+    d.nbones = nb;
+    d.ntris = nt;
+    d.nbv = nbv;
+    d.max_dist = b.max_dist;
+    d.contact_offset = b.contact_offset;
+    d.max_push = b.max_push;
+    int r;
+    bm_i4 *bone = nullptr, *tv = nullptr, *adj = nullptr;
+    bm_i2 *sub = nullptr;
+    int32_t *vt_off = nullptr, *vt = nullptr, *dvg = nullptr, *dgoff = nullptr;
+    float *dcw = nullptr, *dmw = nullptr;
+    if ((r = bm_upload(c, &bone, T.bone)) || (r = bm_upload(c, &tv, T.tv)) || (r = bm_upload(c, &adj, T.adj)) ||
+        (r = bm_upload(c, &sub, T.sub)) || (r = bm_upload(c, &vt_off, T.vt_off)) || (r = bm_upload(c, &vt, T.vt)) ||
+        (r = bm_upload(c, &dvg, vg)) || (r = bm_upload(c, &dgoff, goff)) || (r = bm_upload(c, &dcw, cw)))
+        return r;
+    if (!mw.empty() && (r = bm_upload(c, &dmw, mw))) return r;
+    d.bone = bone; d.tv = tv; d.adj = adj; d.sub = sub; d.vt_off = vt_off; d.vt = vt;
+    d.vgroup = dvg; d.goff = dgoff; d.cw = dcw; d.mw = dmw;
+    bm_f4 *pts = nullptr;
+    if ((r = bm_alloc(c, &d.soup, 3 * (size_t)nt)) || (r = bm_alloc(c, &d.fn, (size_t)nt)) ||
+        (r = bm_alloc(c, &d.vn, (size_t)nbv)) || (r = bm_alloc(c, &d.node, 2 * (size_t)T.nnodes)) ||
+        (r = bm_alloc(c, &pts, (size_t)nbv)) || (r = bm_alloc(c, &b.d_depth, (size_t)nv)) ||
+        (r = bm_alloc(c, &b.d_stat, 2)))
+        return r;
+    b.d_pts = pts;
+    d.pts = pts;
+    b.built = true;
+    invalidate_graph(c);
+    return bm_upload_points(c);
 }
 
 inline float inv_mass_of(const xpbd_ctx *c, int old) {
@@ -3132,7 +3265,19 @@ int launch_post(xpbd_ctx *c, const float4 *xc, int do_sdf, int do_vel, bool inte
     sd.bone = c->d_bone;
     sd.vgroup = c->d_vgroup;
     sd.goff = c->d_goff;
-    const int sdf_on = (do_sdf && c->nbones > 0 && c->d_grids) ? 1 : 0;
+    // the bone mesh, when it resolves contacts, takes the grid SDF's place: the
+    // DCD pass's apply first, then the mesh push in place, then velocities
+    const bool mesh = do_sdf && c->bm.built && c->bm.resolve;
+    if (mesh) {
+        if (dcd_apply) {
+            hipLaunchKernelGGL(k_dcd_apply, vgrid(c->nv), dim3(256), 0, c->stream, c->dcd, (float4 *)xc, c->nv);
+            HIPCHK(c, hipGetLastError());
+            dcd_apply = false;
+        }
+        int r = bm_push(c, (float4 *)xc);
+        if (r) return r;
+    }
+    const int sdf_on = (!mesh && do_sdf && c->nbones > 0 && c->d_grids) ? 1 : 0;
     const float4 *frc = c->has_force ? (const float4 *)c->FRC : nullptr;
     const float4 *pin = c->pins_on ? (const float4 *)c->PIN : nullptr;
     unsigned long long *dacc = dcd_apply ? c->dcd.acc : nullptr;
@@ -3425,6 +3570,7 @@ void xpbd_destroy(xpbd_ctx *c) {
     free_stage(c->st_att);
     free_stage(c->st_gact);
     free_stage(c->st_mgpin);
+    free_stage(c->bm.st);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
 }
@@ -3772,6 +3918,124 @@ int32_t xpbd_set_bone_transforms(xpbd_ctx *c, int32_t nbones, const float *xf) {
     return XPBD_OK;
 }
 
+int32_t xpbd_set_bone_mesh(xpbd_ctx *c, const xpbd_bone_mesh_desc *d) {
+    if (!c || c->nv == 0) return fail(c, XPBD_EINVAL, "no mesh");
+    HIPCHK(c, hipSetDevice(c->device));
+    BoneMesh &b = c->bm;
+    if (!d) {
+        if (c->stream) HIPCHK(c, hipStreamSynchronize(c->stream));
+        bm_free(b);
+        free_stage(b.st);
+        b = BoneMesh{};
+        invalidate_graph(c);
+        return XPBD_OK;
+    }
+    if (d->nverts < 1 || !d->verts || d->ntris < 1 || !d->tris) return fail(c, XPBD_EINVAL, "empty bone mesh");
+    if (d->ntris >= (1 << 29)) return fail(c, XPBD_EINVAL, "bone mesh has too many triangles");
+    for (int64_t i = 0; i < 3 * (int64_t)d->ntris; ++i)
+        if (d->tris[i] < 0 || d->tris[i] >= d->nverts) return fail(c, XPBD_EINVAL, "bone triangle vertex out of range");
+    for (int64_t i = 0; i < 3 * (int64_t)d->nverts; ++i)
+        if (!std::isfinite(d->verts[i])) return fail(c, XPBD_EINVAL, "non-finite bone vertex");
+    std::vector<int32_t> toff;
+    int nb = d->nbones;
+    if (!d->tri_off) {
+        if (nb > 1) return fail(c, XPBD_EINVAL, "tri_off is needed with more than one sub-mesh");
+        nb = 1;
+        toff = {0, d->ntris};
+    } else {
+        if (nb < 1) return fail(c, XPBD_EINVAL, "nbones must be >= 1");
+        toff.assign(d->tri_off, d->tri_off + nb + 1);
+        if (toff[0] != 0 || toff[nb] != d->ntris) return fail(c, XPBD_EINVAL, "tri_off must span [0, ntris]");
+        for (int k = 0; k < nb; ++k)
+            if (toff[k + 1] < toff[k]) return fail(c, XPBD_EINVAL, "tri_off must be non-decreasing");
+    }
+    std::vector<int32_t> goff, vg;
+    if (d->ngroups > 0) {
+        if (!d->goff || !d->vgroup) return fail(c, XPBD_EINVAL, "groups need goff and vgroup");
+        goff.assign(d->goff, d->goff + d->ngroups + 1);
+        if (goff[0] != 0 || goff[d->ngroups] != nb) return fail(c, XPBD_EINVAL, "group offsets must span [0, nbones]");
+        for (int g = 0; g < d->ngroups; ++g)
+            if (goff[g + 1] < goff[g]) return fail(c, XPBD_EINVAL, "group offsets must be non-decreasing");
+        vg.assign(d->vgroup, d->vgroup + c->nv);
+        for (int i = 0; i < c->nv; ++i)
+            if (vg[i] < 0 || vg[i] >= d->ngroups) return fail(c, XPBD_EINVAL, "vertex group out of range");
+    }
+    if (!(d->max_dist > 0.0f) || !std::isfinite(d->max_dist)) return fail(c, XPBD_EINVAL, "max_dist must be > 0");
+    if (c->stream) HIPCHK(c, hipStreamSynchronize(c->stream));
+    bm_free(b);
+    b.set = true;
+    b.resolve = d->resolve != 0;
+    b.nbv = d->nverts;
+    b.ntris = d->ntris;
+    b.nbones = nb;
+    b.pts.assign(d->verts, d->verts + 3 * (size_t)d->nverts);
+    b.tris.assign(d->tris, d->tris + 3 * (size_t)d->ntris);
+    b.tri_off = toff;
+    b.goff = goff;
+    b.vgroup = vg;
+    if (d->coll_w) b.cw.assign(d->coll_w, d->coll_w + c->nv); else b.cw.clear();
+    if (d->meter_w) b.mw.assign(d->meter_w, d->meter_w + c->nv); else b.mw.clear();
+    b.max_dist = d->max_dist;
+    b.contact_offset = d->contact_offset;
+    b.max_push = d->max_push;
+    invalidate_graph(c);
+    return c->ready ? bm_build(c) : XPBD_OK;
+}
+
+int32_t xpbd_set_bone_mesh_points(xpbd_ctx *c, const float *verts) {
+    if (!c || !c->bm.set) return fail(c, XPBD_EINVAL, "no bone mesh");
+    if (!verts) return fail(c, XPBD_EINVAL, "null bone points");
+    BoneMesh &b = c->bm;
+    for (int64_t i = 0; i < 3 * (int64_t)b.nbv; ++i)
+        if (!std::isfinite(verts[i])) return fail(c, XPBD_EINVAL, "non-finite bone vertex");
+    b.pts.assign(verts, verts + 3 * (size_t)b.nbv);
+    return b.built ? bm_upload_points(c) : XPBD_OK;
+}
+
+int32_t xpbd_set_bone_mesh_resolve(xpbd_ctx *c, int32_t resolve) {
+    if (!c || !c->bm.set) return fail(c, XPBD_EINVAL, "no bone mesh");
+    if ((resolve != 0) != c->bm.resolve) {
+        c->bm.resolve = resolve != 0;
+        invalidate_graph(c);
+    }
+    return XPBD_OK;
+}
+
+int32_t xpbd_bone_mesh_push(xpbd_ctx *c) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!c->bm.built) return fail(c, XPBD_EINVAL, "no bone mesh");
+    return bm_push(c, c->X[0]);
+}
+
+int32_t xpbd_bone_penetration(xpbd_ctx *c, float collide_eps, float *depth, int32_t *count, float *max_depth) {
+    int r = check_ready(c);
+    if (r) return r;
+    BoneMesh &b = c->bm;
+    if (!b.built) return fail(c, XPBD_EINVAL, "no bone mesh");
+    HIPCHK(c, hipMemsetAsync(b.d_stat, 0, 2 * sizeof(uint32_t), c->stream));
+    hipLaunchKernelGGL(k_bm_query<1>, vgrid(c->nv), dim3(256), 0, c->stream, c->nv, c->X[0], b.dev, b.d_depth,
+                       collide_eps, b.d_stat);
+    HIPCHK(c, hipGetLastError());
+    uint32_t st[2];
+    HIPCHK(c, hipMemcpyAsync(st, b.d_stat, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+    std::vector<float> dd;
+    if (depth) {
+        dd.resize(c->nv);
+        HIPCHK(c, hipMemcpyAsync(dd.data(), b.d_depth, c->nv * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (depth)
+        for (int n = 0; n < c->nv; ++n) depth[c->L.perm[n]] = dd[n];
+    if (count) *count = (int32_t)st[0];
+    if (max_depth) {
+        float m;
+        std::memcpy(&m, &st[1], sizeof(m));
+        *max_depth = m;
+    }
+    return XPBD_OK;
+}
+
 int32_t xpbd_finalize(xpbd_ctx *c, const xpbd_tuning *t) {
     if (!c || c->nv == 0) return fail(c, XPBD_EINVAL, "no mesh");
     if (!c->params_set) return fail(c, XPBD_EINVAL, "call xpbd_set_params() first");
@@ -3967,6 +4231,7 @@ int32_t xpbd_finalize(xpbd_ctx *c, const xpbd_tuning *t) {
     }
     c->ready = true;
     if (!c->dcd_tris.empty() && (r = alloc_dcd(c))) return r;
+    if (c->bm.set && (r = bm_build(c))) return r;
     if ((r = upload_vertex_static(c))) return r;
     if ((r = upload_pins(c))) return r;
     if (c->has_force && (r = xpbd_set_force(c, c->force.data()))) return r;

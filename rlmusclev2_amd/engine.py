@@ -86,6 +86,8 @@ class XpbdEngine:
                 c.call("xpbd_set_sdf_groups", len(k["goff"]) - 1, ptr(k["goff"]), ptr(k["vgroup"]))
         if s.dcd is not None:
             self._set_dcd(s.dcd)
+        if s.bone_mesh is not None:
+            self.set_bone_mesh(s.bone_mesh)
         tun = capi.XpbdTuning()
         tun.patch_verts = int(patch_verts)
         tun.strategy = self.strategy
@@ -95,6 +97,59 @@ class XpbdEngine:
         if s.pin_vids is not None:
             tgt = f32(s.pos0, (-1, 3))[k["pin_vids"]]
             self.set_pin_targets(tgt)
+
+    # exact-mesh bone collider (AnimatedBoneMeshCollider, humanV2.py:319-418) -----
+    def set_bone_mesh(self, bm) -> None:
+        """Install (or with None remove) the bone-mesh collider; allowed after finalize."""
+        if bm is None:
+            self.ctx.call("xpbd_set_bone_mesh", None)
+            return
+        k = self._keep
+        k["bm_verts"] = f32(bm.verts, (-1, 3))
+        k["bm_tris"] = i32(bm.tris, (-1, 3))
+        k["bm_toff"] = None if bm.tri_off is None else i32(bm.tri_off)
+        k["bm_goff"] = None if bm.group_bone_off is None else i32(bm.group_bone_off)
+        k["bm_vgroup"] = None if bm.vertex_group is None else i32(bm.vertex_group)
+        k["bm_cw"] = None if bm.coll_w is None else f32(bm.coll_w)
+        k["bm_mw"] = None if bm.meter_w is None else f32(bm.meter_w)
+        d = capi.XpbdBoneMeshDesc()
+        d.nverts = k["bm_verts"].shape[0]
+        d.verts = ptr(k["bm_verts"])
+        d.ntris = k["bm_tris"].shape[0]
+        d.tris = ptr(k["bm_tris"])
+        d.nbones = bm.nbones
+        d.tri_off = ptr(k["bm_toff"])
+        d.ngroups = 0 if k["bm_goff"] is None else len(k["bm_goff"]) - 1
+        d.goff = ptr(k["bm_goff"])
+        d.vgroup = ptr(k["bm_vgroup"])
+        d.coll_w = ptr(k["bm_cw"])
+        d.meter_w = ptr(k["bm_mw"])
+        d.max_dist = float(bm.max_dist)
+        d.contact_offset = float(bm.contact_offset)
+        d.max_push = float(bm.max_push)
+        d.resolve = int(bool(bm.resolve))
+        self.ctx.call("xpbd_set_bone_mesh", C.byref(d))
+        self._bm_nbv = d.nverts
+
+    def set_bone_mesh_points(self, verts) -> None:
+        v = f32(verts, (-1, 3))
+        if v.shape[0] != getattr(self, "_bm_nbv", -1):
+            raise ValueError(f"bone mesh has {getattr(self, '_bm_nbv', 0)} vertices, got {v.shape[0]}")
+        self.ctx.call("xpbd_set_bone_mesh_points", ptr(v))
+
+    def set_bone_mesh_resolve(self, on: bool) -> None:
+        self.ctx.call("xpbd_set_bone_mesh_resolve", int(bool(on)))
+
+    def bone_mesh_push(self) -> None:
+        self.ctx.call("xpbd_bone_mesh_push")
+
+    def bone_penetration(self, collide_eps: float = 1e-6, want_depth: bool = True):
+        """(depth per vertex or None, count deeper than collide_eps, largest depth)"""
+        depth = np.zeros(self.NV, np.float32) if want_depth else None
+        cnt = C.c_int32(0)
+        mx = C.c_float(0.0)
+        self.ctx.call("xpbd_bone_penetration", C.c_float(collide_eps), ptr(depth), C.byref(cnt), C.byref(mx))
+        return depth, int(cnt.value), float(mx.value)
 
     def _set_dcd(self, spec):
         from .collision import DcdParams

@@ -49,6 +49,27 @@ class SdfColliders:
 
 
 @dataclass
+class BoneMeshCollider:
+    """Triangle mesh over the bones (AnimatedBoneMeshCollider, src/example/humanV2.py:319-384):
+    one or more closed sub-meshes, outward winding; per-frame vertex positions, fixed topology."""
+    verts: np.ndarray                         # (nbv, 3) positions of the first frame
+    tris: np.ndarray                          # (ntris, 3) vertex ids, (b - a) x (c - a) outward
+    tri_off: np.ndarray | None = None         # (nbones+1,) sub-mesh triangle ranges
+    max_dist: float = 0.2
+    contact_offset: float = 1e-5
+    max_push: float = -1.0
+    coll_w: np.ndarray | None = None          # (nv,) 0 = never pushed
+    meter_w: np.ndarray | None = None         # (nv,) 0 = not metered
+    vertex_group: np.ndarray | None = None    # (nv,) group per vertex
+    group_bone_off: np.ndarray | None = None  # (ngroups+1,) sub-mesh ranges per group
+    resolve: bool = True                      # False: penetration meter only
+
+    @property
+    def nbones(self) -> int:
+        return 1 if self.tri_off is None else int(len(self.tri_off) - 1)
+
+
+@dataclass
 class Scene:
     pos0: np.ndarray
     tets: np.ndarray
@@ -62,6 +83,7 @@ class Scene:
     pin_vids: np.ndarray | None = None
     pin_active: np.ndarray | None = None
     sdf: SdfColliders | None = None
+    bone_mesh: BoneMeshCollider | None = None
     dcd: object | None = None                  # collision.DcdSpec
     attach_vids: np.ndarray | None = None      # vertex of each ATTACH constraint, table order
     meta: dict = field(default_factory=dict)

@@ -18,7 +18,7 @@ from . import builder, collision, multigrid, workloads
 from .builder import ConsTable
 from .config import find_constraint
 from .engine import XpbdEngine
-from .scene import Scene, SdfColliders, SimParams
+from .scene import BoneMeshCollider, Scene, SdfColliders, SimParams
 
 
 def mg_constraint_configs(cfg: dict) -> list:
@@ -73,6 +73,12 @@ def build_workload(cfg: dict) -> workloads.Workload:
     raise ValueError(f"unknown synthetic workload {name!r}")
 
 
+def _mesh_res(coll: dict) -> dict:
+    """tessellation of the synthetic capsule bones (around, per cap, along)"""
+    r = coll.get("mesh_res", [96, 16, 96])
+    return dict(n_around=int(r[0]), n_cap=int(r[1]), n_len=int(r[2]))
+
+
 def scene_from_config(cfg: dict, w: workloads.Workload) -> Scene:
     """Scene for the MultiGridSim program on workload ``w``."""
     pos0 = w.pos0.astype(np.float32)
@@ -121,10 +127,12 @@ def scene_from_config(cfg: dict, w: workloads.Workload) -> Scene:
         att = builder.create_attach(attach_vids, pos0[attach_vids], float(belly["stiffness"]),
                                     float(belly.get("dampingratio", 0.0)))
         cons = builder.ConsTable.concat([cons, att]).sorted_by_type().contiguous()
-    sdf = None
+    sdf = bone_mesh = None
     coll = cfg.get("demo", {}).get("collision", {})
-    if coll.get("enabled", False) and w.bones:
-        grids, origins, spacing = workloads.bake_bone_sdfs(w, int(cfg.get("synthetic", {}).get("sdf_res", 64)))
+    kind = str(coll.get("collider", "grid")).lower()
+    if kind not in ("grid", "mesh"):
+        raise ValueError(f"demo.collision.collider={kind!r}: 'grid' (bone SDF grids) or 'mesh' (bone triangle mesh)")
+    if w.bones and (coll.get("enabled", False) or kind == "mesh"):
         m = np.ones(w.nv, np.float32)
         if w.is_fascia is not None:
             m[w.is_fascia] = 0.0   # fascia excluded from bone collision (humanV2.py:361-366)
@@ -132,10 +140,23 @@ def scene_from_config(cfg: dict, w: workloads.Workload) -> Scene:
         if w.vertex_group is not None and w.name == "batch":
             vg = w.vertex_group.astype(np.int32)
             goff = np.arange(len(w.bones) + 1, dtype=np.int32)
-        sdf = SdfColliders(grids=grids, origins=origins, spacing=spacing,
-                           contact_offset=float(coll.get("contact_offset", 1e-5)),
-                           max_push=float(coll.get("max_push", -1.0)), coll_w=m,
-                           vertex_group=vg, group_bone_off=goff)
+        if kind == "grid":
+            grids, origins, spacing = workloads.bake_bone_sdfs(w, int(cfg.get("synthetic", {}).get("sdf_res", 64)))
+            sdf = SdfColliders(grids=grids, origins=origins, spacing=spacing,
+                               contact_offset=float(coll.get("contact_offset", 1e-5)),
+                               max_push=float(coll.get("max_push", -1.0)), coll_w=m,
+                               vertex_group=vg, group_bone_off=goff)
+        else:
+            # AnimatedBoneMeshCollider (humanV2.py:319-384): the skeleton as one triangle mesh,
+            # pushing when collision is enabled, the penetration meter always
+            verts, tris, toff, _ = workloads.bone_meshes(w, **_mesh_res(coll))
+            bone_mesh = BoneMeshCollider(verts=workloads.bone_mesh_points(w, int(cfg.get("start_frame", 1)),
+                                                                          **_mesh_res(coll)),
+                                         tris=tris, tri_off=toff, max_dist=float(coll.get("max_dist", 0.2)),
+                                         contact_offset=float(coll.get("contact_offset", 1e-5)),
+                                         max_push=float(coll.get("max_push", -1.0)), coll_w=m, meter_w=m.copy(),
+                                         vertex_group=vg, group_bone_off=goff,
+                                         resolve=bool(coll.get("enabled", False)))
     dcd = None
     sc = cfg.get("demo", {}).get("self_collision", {})
     if sc.get("enabled", False):
@@ -161,6 +182,7 @@ def scene_from_config(cfg: dict, w: workloads.Workload) -> Scene:
                             dampingratio=float(belly.get("dampingratio", 0.0)))]
     return Scene(pos0=pos0, tets=tets, density=density, cons=cons, params=params, v_fiber_dir=vf,
                  tendonmask=tm, stopped=w.stopped, pin_vids=pin_vids, pin_active=pin_active, sdf=sdf,
+                 bone_mesh=bone_mesh,
                  dcd=dcd, attach_vids=attach_vids, meta=dict(workload=w.name, constraint_configs=ccfg))
 
 
@@ -188,6 +210,7 @@ class MultiGridSim:
         if cfg.get("use_multigrid", False):
             self._build_coarse(engine_cls, device)
         self.use_graph = True
+        self._native_collider = None
         self.collision_solvers = []
         self.collision_solvers_iter = []
         self._frame = None
@@ -246,6 +269,9 @@ class MultiGridSim:
             self.engine.set_attach_targets(workloads.follow_targets(w, frame, self.scene.attach_vids))
         if self.scene.sdf is not None:
             self.engine.set_bone_transforms(workloads.bone_xforms(w, frame))
+        if self.scene.bone_mesh is not None and w.bones:
+            coll = self.cfg.get("demo", {}).get("collision", {})
+            self.engine.set_bone_mesh_points(workloads.bone_mesh_points(w, frame, **_mesh_res(coll)))
         if self.coarse is not None and self.coarse.pin_rule == 1 and w.bones:
             self.engine.set_coarse_pin_targets(multigrid.coarse_pin_targets(self.coarse, w, frame))
         self._frame = frame
@@ -267,11 +293,33 @@ class MultiGridSim:
     def apply_pintotarget(self) -> None:
         self.engine.apply_pintotarget()
 
+    # collision_solvers: Python post-phase hooks, plus at most one native bone
+    # collider (collider.AnimatedBoneMeshCollider), which runs inside the native
+    # substep (and its graph) as the last word of the collision stage, as the
+    # reference's bone collider does (humanV2.py:616-622)
+    @property
+    def collision_solvers(self):
+        return self._collision_solvers
+
+    @collision_solvers.setter
+    def collision_solvers(self, fns):
+        fns = list(fns)
+        nat = [f for f in fns if getattr(f, "native_stage", False)]
+        if len(nat) > 1:
+            raise ValueError("at most one native bone collider per sim")
+        if nat and nat[0].sim is not self:
+            raise ValueError("the bone collider belongs to another sim")
+        if nat or self._native_collider is not None:
+            self.engine.set_bone_mesh_resolve(bool(nat))
+        self._native_collider = nat[0] if nat else None
+        self._collision_solvers = fns
+        self._py_solvers = [f for f in fns if not getattr(f, "native_stage", False)]
+
     # program -------------------------------------------------------------------
     def substep(self) -> None:
         """integrate -> pin -> fine_iters x (clear, dispatch, apply_dP, pin) ->
         collision_solvers -> update_velocities (mg.py:1322-1384)."""
-        if self.collision_solvers or self.collision_solvers_iter:
+        if self._py_solvers or self.collision_solvers_iter:
             e = self.engine
             e.integrate()
             if self.coarse is not None:
@@ -280,9 +328,9 @@ class MultiGridSim:
                 e.iterate(1)
                 for fn in self.collision_solvers_iter:
                     fn(it)
-            e.collide()
-            for fn in self.collision_solvers:
+            for fn in self._py_solvers:
                 fn()
+            e.collide()            # bone push last (humanV2.py:616-622)
             e.update_velocities()
         else:
             self.engine.native_substep(self.fine_iters)
@@ -291,7 +339,7 @@ class MultiGridSim:
         """Kinematic step once, then nsubsteps substeps (mg.py:1409-1418)."""
         self.load_frame_targets(frame)
         self.engine.apply_pintotarget()
-        if self.collision_solvers or self.collision_solvers_iter:
+        if self._py_solvers or self.collision_solvers_iter:
             for _ in range(self.nsubsteps):
                 self.substep()
         else:

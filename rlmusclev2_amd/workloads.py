@@ -184,6 +184,76 @@ def capsule_phi(p, a, b, radius):
     return np.linalg.norm(p - q, axis=-1) - radius
 
 
+def capsule_mesh(a, b, radius, n_around=96, n_cap=16, n_len=96):
+    """Closed triangle mesh of the capsule (segment a-b, radius), outward winding:
+    (p1 - p0) x (p2 - p0) points out.  Rings of n_around vertices: n_cap latitudes
+    on each cap (the equators included), n_len - 1 rings along the cylinder, and
+    the two poles.  Returns (verts (n, 3) float64, tris (m, 3) int32)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    u = b - a
+    u /= np.linalg.norm(u)
+    e1 = np.cross(u, [1.0, 0.0, 0.0] if abs(u[0]) < 0.9 else [0.0, 1.0, 0.0])
+    e1 /= np.linalg.norm(e1)
+    e2 = np.cross(u, e1)                      # e1 x e2 = u
+    th = 2.0 * np.pi * np.arange(n_around) / n_around
+    ring_dir = np.cos(th)[:, None] * e1 + np.sin(th)[:, None] * e2
+    rings = []
+    for i in range(1, n_cap + 1):             # bottom cap, from the pole side up to a's equator
+        ph = -0.5 * np.pi + 0.5 * np.pi * i / n_cap
+        rings.append(a + radius * (np.cos(ph) * ring_dir + np.sin(ph) * u))
+    for j in range(1, n_len):
+        rings.append(a + (j / n_len) * (b - a) + radius * ring_dir)
+    for i in range(n_cap):                    # top cap, from b's equator towards the pole
+        ph = 0.5 * np.pi * i / n_cap
+        rings.append(b + radius * (np.cos(ph) * ring_dir + np.sin(ph) * u))
+    nr = len(rings)
+    verts = np.concatenate([np.stack(rings).reshape(-1, 3), [a - radius * u], [b + radius * u]])
+    bot, top = nr * n_around, nr * n_around + 1
+    j = np.arange(n_around)
+    jn = (j + 1) % n_around
+    tris = []
+    for k in range(nr - 1):
+        p00, p01 = k * n_around + j, k * n_around + jn
+        p10, p11 = (k + 1) * n_around + j, (k + 1) * n_around + jn
+        tris.append(np.stack([p00, p01, p11], 1))
+        tris.append(np.stack([p00, p11, p10], 1))
+    tris.append(np.stack([np.full(n_around, bot), jn, j], 1))
+    last = (nr - 1) * n_around
+    tris.append(np.stack([last + j, last + jn, np.full(n_around, top)], 1))
+    return verts, np.concatenate(tris).astype(np.int32)
+
+
+def bone_meshes(w, n_around=96, n_cap=16, n_len=96):
+    """Triangle meshes of the workload's capsule bones in the rest pose, one
+    sub-mesh per bone: (verts, tris, tri_off, vertex bone)."""
+    key = ("_bone_mesh", n_around, n_cap, n_len)
+    if key not in w.info:
+        vs, ts, toff, vb = [], [], [0], []
+        off = 0
+        for i, bn in enumerate(w.bones):
+            v, t = capsule_mesh(bn["a"], bn["b"], bn["radius"], n_around, n_cap, n_len)
+            vs.append(v)
+            ts.append(t + off)
+            toff.append(toff[-1] + t.shape[0])
+            vb.append(np.full(v.shape[0], i, np.int32))
+            off += v.shape[0]
+        w.info[key] = (np.concatenate(vs).astype(np.float32), np.concatenate(ts),
+                       np.asarray(toff, np.int32), np.concatenate(vb))
+    return w.info[key]
+
+
+def bone_mesh_points(w, frame: int, **res) -> np.ndarray:
+    """Bone mesh vertices at ``frame``: each bone's rigid motion on its rest vertices."""
+    verts, _, _, vb = bone_meshes(w, **res)
+    out = np.empty_like(verts)
+    for i, bn in enumerate(w.bones):
+        R, t = bone_transform(bn, frame)
+        sel = vb == i
+        out[sel] = (verts[sel].astype(np.float64) @ R.T + t).astype(np.float32)
+    return out
+
+
 def rotation_about(axis, angle):
     a = np.asarray(axis, np.float64)
     a = a / np.linalg.norm(a)
