@@ -18,6 +18,10 @@ plain @same residual".  Prints ONE JSON line.
   batch: network trained by the library's MFMA kernels on plain-solver frames,
   iterations to reach the plain 10-iteration residual, ms/frame of plain and
   warm frames with the audited per-frame fallback.
+* "bone_mesh": the humanV2-scale frame with the exact-mesh bone collider (a
+  capsule skeleton of 97,536 triangles, BVH refit per frame, closest-point
+  push in every substep) in place of the grid SDFs, with the push, refit and
+  penetration meter timed alone.
 
 ``--workload W`` runs one workload alone (the form the profiling commands use).
 """
@@ -58,6 +62,7 @@ WORKLOADS = {
     "batch": ("batch512.json", {}),
     "c5": ("humanv2_c5.json", {}),
     "c5_batch": ("batch64_c5.json", {}),
+    "mesh_collider": ("humanv2_mesh_collider.json", {}),
 }
 # the block shape the autotuner picks on the 512-instance batch (tuned once per mesh; re-tuning it in
 # every bench run would rebuild the 7 M-tet layout 13 times)
@@ -327,6 +332,63 @@ def warm_start_leg(cfg, w, tuning, args):
     }
 
 
+def bone_mesh_leg(args, tuning, grid_ms):
+    """The humanV2-scale frame with the exact-mesh bone collider in place of the
+    grid SDFs (the reference's AnimatedBoneMeshCollider), with its pieces timed
+    on the host around synchronised calls (launch included): one push of every
+    vertex, the per-frame upload + refit, the penetration meter; with coherent
+    queries (a vertex proven outside by its last query and the motion since is
+    not queried again; same results) and with a query for every vertex."""
+    import dataclasses
+    from rlmusclev2_amd import sims, workloads
+    cfg = load_cfg("mesh_collider")
+    w = sims.build_workload(cfg)
+    sim, rec = run_frames(cfg, w, tuning, args.steps, args.warmup, "mesh_collider")
+    e = sim.engine
+    bm = sim.scene.bone_mesh
+    eps = float(cfg["demo"]["collision"].get("collide_eps", 1e-6))
+    last = args.warmup + args.steps
+    nrep = 20
+
+    def per_call(fn, n=nrep):
+        fn()
+        e.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        e.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / n
+
+    _, n_in, max_d = e.bone_penetration(eps, want_depth=False)
+    state = e.get_state()
+    out = {}
+    for mode, coh in (("coherent", bm.coherent), ("every_vertex", 0)):
+        e.set_bone_mesh(dataclasses.replace(bm, verts=workloads.bone_mesh_points(w, last), coherent=coh))
+        e.set_state(*state)
+        meter_ms = per_call(lambda: e.bone_penetration(eps, want_depth=False), 10)
+        push_ms = per_call(e.bone_mesh_push)
+        e.set_state(*state)
+        q0 = e.bone_mesh_queries()
+        t0 = time.perf_counter()
+        for f in range(last + 1, last + 4):
+            sim.step_frame(f)
+        e.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / 3
+        out[mode] = {"ms_per_frame": round(ms, 4), "push_ms": round(push_ms, 4), "meter_ms": round(meter_ms, 4),
+                     "queried_fraction_in_frames": round((e.bone_mesh_queries() - q0) / (3 * sim.nsubsteps * w.nv), 4)}
+    pts = workloads.bone_mesh_points(w, last + 3)
+    refit_ms = per_call(lambda: e.set_bone_mesh_points(pts))
+    e.close()
+    return {
+        "workload": f"{w.name}: {w.nt} tets / {w.nv} verts against a {len(bm.tris)}-triangle skeleton mesh "
+                    f"({bm.nbones} capsule bones), max_dist {bm.max_dist} m, contact_offset {bm.contact_offset} m",
+        "ms_per_frame": rec["ms_per_frame"], "grid_sdf_ms_per_frame": grid_ms, **out,
+        "refit_ms": round(refit_ms, 4), "inside_after_frames": n_in, "max_depth_after_frames": max_d,
+        "timing": "frames: graph replays, host clock; push / meter / refit: host clock around synchronised calls",
+        "finite": rec["finite"], "residual_rms": rec["residual_rms"],
+    }
+
+
 def one_workload(args, name):
     """A single workload (its own line; the profiling commands use this form)."""
     from rlmusclev2_amd import autotune, sims
@@ -340,6 +402,10 @@ def one_workload(args, name):
         tuning = dict(BATCH_TUNING)
     else:
         tuning, tune_table = autotune.pick(cfg, w)
+    if name == "mesh_collider":
+        del w
+        return {"metric": "ms/frame (humanV2-scale, exact-mesh bone collider)", "unit": "ms/frame",
+                "bone_mesh": bone_mesh_leg(args, tuning, None)}
     sim, rec = run_frames(cfg, w, tuning, args.steps, args.warmup, name)
     sim.engine.close()
     rec["autotune_us_per_iteration"] = tune_table
@@ -420,6 +486,8 @@ def main():
         **rec4,
     }
     del w4
+    # ---- exact-mesh bone collider on the headline mesh -------------------
+    out["bone_mesh"] = bone_mesh_leg(args, tuning, rec["ms_per_frame"])
     # ---- c5: warm start vs plain at the same residual, with DCD ----------
     if not args.no_warm_start:
         out["c5"] = {}

@@ -233,6 +233,13 @@ typedef struct {
     float max_dist, contact_offset, max_push;
     int32_t resolve;            /* 1: the substep's collision stage pushes with this mesh (the grid
                                    SDF is then not used); 0: meter only                           */
+    int32_t coherent;           /* 1: skip the query of a vertex that its last query and the motion
+                                   since (its own, and the largest bone-vertex displacement of each
+                                   upload) prove to be outside by more than the contact offset; the
+                                   results equal querying every vertex when each sub-mesh bounds a
+                                   solid and the sub-meshes a vertex tests do not overlap; needs
+                                   closed sub-meshes (EINVAL otherwise).  0: query every vertex.
+                                   2: 1 if the mesh is closed, else 0                              */
 } xpbd_bone_mesh_desc;
 /* desc = NULL removes the collider.  Allowed before or after xpbd_finalize. */
 int32_t xpbd_set_bone_mesh(xpbd_ctx *ctx, const xpbd_bone_mesh_desc *desc);
@@ -243,6 +250,9 @@ int32_t xpbd_set_bone_mesh_points(xpbd_ctx *ctx, const float *verts);
 int32_t xpbd_set_bone_mesh_resolve(xpbd_ctx *ctx, int32_t resolve);
 /* one push of the current positions (AnimatedBoneMeshCollider.__call__, humanV2.py:400-406) */
 int32_t xpbd_bone_mesh_push(xpbd_ctx *ctx);
+/* queries made since the collider was set (push and meter; with coherent queries the vertices
+ * that were not certified outside), synchronises.  No reference counterpart. */
+int32_t xpbd_bone_mesh_queries(xpbd_ctx *ctx, int64_t *queries);
 /* the meter on the current positions: depth (nv floats, caller order, > 0 inside; may be NULL),
  * the number of metered vertices deeper than collide_eps and the largest depth; synchronises */
 int32_t xpbd_bone_penetration(xpbd_ctx *ctx, float collide_eps, float *depth, int32_t *count,

@@ -61,7 +61,7 @@ class XpbdBoneMeshDesc(C.Structure):
                 ("nbones", C.c_int32), ("tri_off", C.c_void_p), ("ngroups", C.c_int32), ("goff", C.c_void_p),
                 ("vgroup", C.c_void_p), ("coll_w", C.c_void_p), ("meter_w", C.c_void_p),
                 ("max_dist", C.c_float), ("contact_offset", C.c_float), ("max_push", C.c_float),
-                ("resolve", C.c_int32)]
+                ("resolve", C.c_int32), ("coherent", C.c_int32)]
 
 
 class XpbdMlpDev(C.Structure):
@@ -126,6 +126,7 @@ SIGNATURES = {
     "xpbd_set_bone_mesh_resolve": [_P, _I],
     "xpbd_bone_mesh_push": [_P],
     "xpbd_bone_penetration": [_P, _F, _P, _P, _P],
+    "xpbd_bone_mesh_queries": [_P, _P],
     "xpbd_dcd_solve": [_P, _I],
     "xpbd_dcd_pairs": [_P, _P, C.c_int64, _P],
     "xpbd_dcd_stats": [_P, _P, _P, _P],

@@ -2563,10 +2563,12 @@ struct BoneMesh {
     std::vector<int32_t> tris, tri_off, goff, vgroup;
     std::vector<float> cw, mw;
     float max_dist = 0.2f, contact_offset = 0.f, max_push = -1.f;
+    int coherent_req = 0;                          // as asked: 0 off, 1 on, 2 when closed
+    double motion = 0.0;                           // running sum of the largest bone-vertex displacement per upload
     BmTables tab;                                 // the tree's tables (BVH order)
     std::vector<void *> bufs;
     BoneMeshDev dev{};
-    bm_f4 *d_pts = nullptr;
+    bm_f4 *d_pts = nullptr;                        // nbv positions, then (M, 0, 0, 0)
     float *d_depth = nullptr;
     uint32_t *d_stat = nullptr;
     HostStage st;
@@ -2852,6 +2854,8 @@ int bm_upload(xpbd_ctx *c, T **p, const std::vector<T> &v) {
     return XPBD_OK;
 }
 
+inline dim3 vgrid_bm(int n) { return dim3((unsigned)((n + 255) / 256)); }
+
 int bm_refit(xpbd_ctx *c) {
     BoneMesh &b = c->bm;
     auto g = [](int n) { return dim3((unsigned)((n + 255) / 256)); };
@@ -2874,11 +2878,13 @@ int bm_upload_points(xpbd_ctx *c) {
     BoneMesh &b = c->bm;
     int slot;
     void *h;
-    const size_t bytes = (size_t)b.nbv * sizeof(float4);
+    const size_t bytes = ((size_t)b.nbv + 1) * sizeof(float4);
     int r = stage_acquire(c, b.st, bytes, slot, h);
     if (r) return r;
     bm_f4 *q = (bm_f4 *)h;
     for (int i = 0; i < b.nbv; ++i) q[i] = bm_f4{b.pts[3 * i], b.pts[3 * i + 1], b.pts[3 * i + 2], 0.f};
+    // M rounded up, so that a certificate never under-counts the bones' motion
+    q[b.nbv] = bm_f4{std::nextafter((float)b.motion, INFINITY), 0.f, 0.f, 0.f};
     if ((r = stage_submit(c, b.st, slot, b.d_pts, bytes))) return r;
     return bm_refit(c);
 }
@@ -2893,6 +2899,9 @@ int bm_build(xpbd_ctx *c) {
     BmTables &T = b.tab;
     std::string e = bm_build_tables(nbv, b.pts.data(), nt, b.tris.data(), nb, b.tri_off.data(), T);
     if (!e.empty()) return fail(c, XPBD_EINVAL, e);
+    if (b.coherent_req == 1 && !T.closed)
+        return fail(c, XPBD_EINVAL, "coherent queries need closed sub-meshes (every edge in two triangles, "
+                                    "opposite directions)");
     std::vector<int32_t> vg(nv), goff = b.goff;
     std::vector<float> cw(nv), mw;
     if (goff.empty()) goff = {0, nb};
@@ -2910,6 +2919,7 @@ int bm_build(xpbd_ctx *c) {
     d.max_dist = b.max_dist;
     d.contact_offset = b.contact_offset;
     d.max_push = b.max_push;
+    d.coherent = (b.coherent_req == 1 || (b.coherent_req == 2 && T.closed)) ? 1 : 0;
     int r;
     bm_i4 *bone = nullptr, *tv = nullptr, *adj = nullptr;
     bm_i2 *sub = nullptr;
@@ -2925,11 +2935,15 @@ int bm_build(xpbd_ctx *c) {
     bm_f4 *pts = nullptr;
     if ((r = bm_alloc(c, &d.soup, 3 * (size_t)nt)) || (r = bm_alloc(c, &d.fn, (size_t)nt)) ||
         (r = bm_alloc(c, &d.vn, (size_t)nbv)) || (r = bm_alloc(c, &d.node, 2 * (size_t)T.nnodes)) ||
-        (r = bm_alloc(c, &pts, (size_t)nbv)) || (r = bm_alloc(c, &b.d_depth, (size_t)nv)) ||
-        (r = bm_alloc(c, &b.d_stat, 2)))
+        (r = bm_alloc(c, &pts, (size_t)nbv + 1)) || (r = bm_alloc(c, &b.d_depth, (size_t)nv)) ||
+        (r = bm_alloc(c, &b.d_stat, 2)) || (r = bm_alloc(c, &d.cache, (size_t)nv)) ||
+        (r = bm_alloc(c, &d.nquery, 1)))
         return r;
+    HIPCHK(c, hipMemsetAsync(d.nquery, 0, sizeof(unsigned long long), c->stream));
     b.d_pts = pts;
     d.pts = pts;
+    hipLaunchKernelGGL(k_bm_cache_reset, vgrid_bm(nv), dim3(256), 0, c->stream, d, nv);
+    HIPCHK(c, hipGetLastError());
     b.built = true;
     invalidate_graph(c);
     return bm_upload_points(c);
@@ -3961,6 +3975,7 @@ int32_t xpbd_set_bone_mesh(xpbd_ctx *c, const xpbd_bone_mesh_desc *d) {
             if (vg[i] < 0 || vg[i] >= d->ngroups) return fail(c, XPBD_EINVAL, "vertex group out of range");
     }
     if (!(d->max_dist > 0.0f) || !std::isfinite(d->max_dist)) return fail(c, XPBD_EINVAL, "max_dist must be > 0");
+    if (d->coherent < 0 || d->coherent > 2) return fail(c, XPBD_EINVAL, "coherent must be 0, 1 or 2");
     if (c->stream) HIPCHK(c, hipStreamSynchronize(c->stream));
     bm_free(b);
     b.set = true;
@@ -3978,6 +3993,8 @@ int32_t xpbd_set_bone_mesh(xpbd_ctx *c, const xpbd_bone_mesh_desc *d) {
     b.max_dist = d->max_dist;
     b.contact_offset = d->contact_offset;
     b.max_push = d->max_push;
+    b.coherent_req = d->coherent;
+    b.motion = 0.0;
     invalidate_graph(c);
     return c->ready ? bm_build(c) : XPBD_OK;
 }
@@ -3988,6 +4005,13 @@ int32_t xpbd_set_bone_mesh_points(xpbd_ctx *c, const float *verts) {
     BoneMesh &b = c->bm;
     for (int64_t i = 0; i < 3 * (int64_t)b.nbv; ++i)
         if (!std::isfinite(verts[i])) return fail(c, XPBD_EINVAL, "non-finite bone vertex");
+    double mx = 0.0;
+    for (int i = 0; i < b.nbv; ++i) {
+        const double dx = (double)verts[3 * i] - b.pts[3 * i], dy = (double)verts[3 * i + 1] - b.pts[3 * i + 1],
+                     dz = (double)verts[3 * i + 2] - b.pts[3 * i + 2];
+        mx = std::max(mx, std::sqrt(dx * dx + dy * dy + dz * dz));
+    }
+    b.motion += mx;
     b.pts.assign(verts, verts + 3 * (size_t)b.nbv);
     return b.built ? bm_upload_points(c) : XPBD_OK;
 }
@@ -4006,6 +4030,17 @@ int32_t xpbd_bone_mesh_push(xpbd_ctx *c) {
     if (r) return r;
     if (!c->bm.built) return fail(c, XPBD_EINVAL, "no bone mesh");
     return bm_push(c, c->X[0]);
+}
+
+int32_t xpbd_bone_mesh_queries(xpbd_ctx *c, int64_t *queries) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!c->bm.built || !queries) return fail(c, XPBD_EINVAL, "no bone mesh");
+    unsigned long long n = 0;
+    HIPCHK(c, hipMemcpyAsync(&n, c->bm.dev.nquery, sizeof(n), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *queries = (int64_t)n;
+    return XPBD_OK;
 }
 
 int32_t xpbd_bone_penetration(xpbd_ctx *c, float collide_eps, float *depth, int32_t *count, float *max_depth) {

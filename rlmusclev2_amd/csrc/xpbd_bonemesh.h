@@ -14,8 +14,8 @@
 //
 // Mechanism (not the reference's): per bone, a binary BVH in implicit heap
 // order (children of node k: 2k + 1, 2k + 2) over leaves of BM_LEAF
-// consecutive triangles of a Morton-sorted triangle list, padded to a power of
-// two; the tree's shape is fixed by the first pose, only its boxes are refitted
+// consecutive triangles, padded to a power of two, the triangles ordered by
+// top-down median splits along the longest axis; the tree's shape is fixed by the first pose, only its boxes are refitted
 // per frame (new bone vertex positions, same topology, like wp.Mesh.refit).
 // Per frame: triangle soup and face normals (k_bm_soup), vertex pseudonormals
 // from a fixed incidence list (k_bm_vnorm, no atomics), leaf boxes plus the
@@ -24,6 +24,18 @@
 // without a stack (the heap index of the current node and one bit per level
 // saying whether it is the second child taken), every box pruned against the
 // best squared distance so far.
+//
+// Coherent mode (closed meshes): a vertex whose last query found it outside at
+// unsigned distance s, and which has since moved by |dx| while no bone vertex
+// moved by more than dM in total, is still outside at distance >= s - |dx| - dM
+// (distance to the surface is 1-Lipschitz in both).  When that bound exceeds
+// the contact offset (the meter: zero) the query cannot change the outcome and
+// is skipped; the results are those of querying every vertex.  This needs the
+// pseudonormal sign to be the true inside test: every sub-mesh closed (each
+// edge shared by two triangles in opposite directions, checked when the tree is
+// built) and the sub-meshes one vertex tests not overlapping (the caller's
+// assurance).  Per vertex cache: the last query's position and s + M, where M
+// is the running sum of the largest bone-vertex displacement of each upload.
 #pragma once
 
 #include <math.h>
@@ -80,7 +92,10 @@ struct BoneMeshDev {
     const int32_t *goff;       // ngroups + 1: bones goff[g] .. goff[g + 1] - 1
     const float *cw;           // per vertex (library order): 0 = not pushed
     const float *mw;           // per vertex (library order): 0 = not metered; null = all
+    bm_f4 *cache;              // coherent mode, per vertex (library order): last query position, s + M (-inf: none)
+    unsigned long long *nquery; // running count of the queries made (the vertices not certified)
     float max_dist, contact_offset, max_push;
+    int coherent;              // 1: skip vertices certified outside (pts[nbv].x holds M)
 };
 
 // ------------------------------------------------------------ per-frame data
@@ -264,11 +279,9 @@ BM_HD bool bm_query(const BoneMeshDev &m, int g, bm_v3 p, BmHit &h, float &sign)
     return true;
 }
 
-// the push of solve_mesh_contact_pos_kernel (kernel.py:155-170); false = unchanged
-BM_HD bool bm_push_point(const BoneMeshDev &m, int g, bm_v3 &p) {
-    BmHit h;
-    float sg;
-    if (!bm_query(m, g, p, h, sg)) return false;
+// the push of solve_mesh_contact_pos_kernel (kernel.py:155-170) from a query
+// result; false = unchanged
+BM_HD bool bm_push_from(const BoneMeshDev &m, const BmHit &h, float sg, bm_v3 &p) {
     const bm_v3 delta = p - h.cp;
     const float dlen = sqrtf(bm_dot(delta, delta));
     if (dlen < 1.0e-9f) return false;
@@ -280,13 +293,39 @@ BM_HD bool bm_push_point(const BoneMeshDev &m, int g, bm_v3 &p) {
 }
 
 // _mark_inside_kernel (humanV2.py:76-87): > 0 inside, 0 outside
-BM_HD float bm_depth_point(const BoneMeshDev &m, int g, bm_v3 p) {
-    BmHit h;
-    float sg;
-    if (!bm_query(m, g, p, h, sg)) return 0.f;
+BM_HD float bm_depth_from(const BmHit &h, float sg, bm_v3 p) {
     const bm_v3 delta = p - h.cp;
     const float d = sqrtf(bm_dot(delta, delta)) * sg;
     return d < 0.f ? -d : 0.f;
+}
+
+BM_HD bool bm_push_point(const BoneMeshDev &m, int g, bm_v3 &p) {
+    BmHit h;
+    float sg;
+    return bm_query(m, g, p, h, sg) && bm_push_from(m, h, sg, p);
+}
+
+BM_HD float bm_depth_point(const BoneMeshDev &m, int g, bm_v3 p) {
+    BmHit h;
+    float sg;
+    return bm_query(m, g, p, h, sg) ? bm_depth_from(h, sg, p) : 0.f;
+}
+
+// coherent mode: true when the cached certificate shows vertex i outside by more
+// than `clear` (the contact offset for the push, 0 for the meter)
+BM_HD bool bm_certified_outside(const BoneMeshDev &m, int i, bm_v3 p, float clear) {
+    const bm_f4 c = m.cache[i];
+    const float M = m.pts[m.nbv].x;
+    const bm_v3 d = p - bm_xyz(c);
+    const float dx = sqrtf(bm_dot(d, d));
+    // 10 um plus a relative term cover the rounding of s, M and |dx|
+    return c.w - M - dx > fmaxf(clear, 0.f) + 1.0e-5f + 1.0e-6f * (fabsf(c.w) + M);
+}
+
+BM_HD void bm_certify(const BoneMeshDev &m, int i, bm_v3 p, bool found, const BmHit &h, float sg) {
+    float s = -INFINITY;
+    if (found && sg > 0.f) s = sqrtf(h.d2) + m.pts[m.nbv].x;
+    m.cache[i] = bm_f4{p.x, p.y, p.z, s};
 }
 
 // ------------------------------------------------------------ host tables
@@ -295,21 +334,12 @@ struct BmTables {
     std::vector<bm_i2> sub;
     std::vector<int32_t> vt_off, vt;
     int nnodes = 0;
+    bool closed = false;    // every edge shared by exactly two triangles, in opposite directions
 };
 
-// 10 bits of u in [0, 1] spread to every third bit
-inline uint32_t bm_spread10(float u) {
-    uint32_t x = (uint32_t)std::min(1023.0f, std::max(0.0f, u * 1024.0f));
-    x = (x | (x << 16)) & 0x030000FFu;
-    x = (x | (x << 8)) & 0x0300F00Fu;
-    x = (x | (x << 4)) & 0x030C30C3u;
-    x = (x | (x << 2)) & 0x09249249u;
-    return x;
-}
-
-// per sub-mesh: triangles sorted by the Morton code of their centroid in the
-// given pose (ties by index), leaves of BM_LEAF consecutive triangles padded to
-// a power of two; edge neighbours: the first other triangle (caller's order) on
+// per sub-mesh: triangles ordered by median splits of their centroids in the
+// given pose, leaves of BM_LEAF consecutive triangles padded to a power of two;
+// edge neighbours: the first other triangle (caller's order) on
 // the same undirected edge; vertex incidences by triangle then corner.
 // Returns an error message, empty on success.  Inputs are already validated.
 inline std::string bm_build_tables(int nbv, const float *pts, int nt, const int32_t *tris, int nb,
@@ -323,27 +353,46 @@ inline std::string bm_build_tables(int nbv, const float *pts, int nt, const int3
     for (int k = 0; k < nb; ++k) {
         const int t0 = tri_off[k], n = tri_off[k + 1] - t0;
         std::vector<float> cen(3 * (size_t)n);
-        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
         for (int j = 0; j < n; ++j)
             for (int a = 0; a < 3; ++a) {
                 const int32_t *t = &tris[3 * (size_t)(t0 + j)];
-                const float v = (pts[3 * t[0] + a] + pts[3 * t[1] + a] + pts[3 * t[2] + a]) / 3.0f;
-                cen[3 * j + a] = v;
-                lo[a] = std::min(lo[a], v);
-                hi[a] = std::max(hi[a], v);
+                cen[3 * j + a] = (pts[3 * t[0] + a] + pts[3 * t[1] + a] + pts[3 * t[2] + a]) / 3.0f;
             }
-        std::vector<std::pair<uint32_t, int>> key(n);
-        for (int j = 0; j < n; ++j) {
-            uint32_t code = 0;
-            for (int a = 0; a < 3; ++a) {
-                const float ext = hi[a] - lo[a];
-                code |= bm_spread10(ext > 0.f ? (cen[3 * j + a] - lo[a]) / ext : 0.f) << a;
+        // top-down median splits along the longest axis of each node's centroid box, at the
+        // leaf boundary that halves the node's leaves (equal-count splits keep the implicit
+        // heap layout); ties by triangle index
+        const int nleaf = std::max(1, (n + BM_LEAF - 1) / BM_LEAF);
+        int Pk = 1;
+        while (Pk < nleaf) Pk <<= 1;
+        std::vector<int> perm(n);
+        for (int j = 0; j < n; ++j) perm[j] = j;
+        std::vector<std::pair<int, int>> todo{{0, Pk}};
+        while (!todo.empty()) {
+            const int l0 = todo.back().first, l1 = todo.back().second;
+            todo.pop_back();
+            if (l1 - l0 < 2) continue;
+            const int a0 = std::min(l0 * BM_LEAF, n), a1 = std::min(l1 * BM_LEAF, n);
+            const int lm = (l0 + l1) / 2, am = std::min(lm * BM_LEAF, n);
+            if (am > a0 && am < a1) {
+                float blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+                for (int j = a0; j < a1; ++j)
+                    for (int a = 0; a < 3; ++a) {
+                        blo[a] = std::min(blo[a], cen[3 * perm[j] + a]);
+                        bhi[a] = std::max(bhi[a], cen[3 * perm[j] + a]);
+                    }
+                int ax = 0;
+                for (int a = 1; a < 3; ++a)
+                    if (bhi[a] - blo[a] > bhi[ax] - blo[ax]) ax = a;
+                std::nth_element(perm.begin() + a0, perm.begin() + am, perm.begin() + a1, [&](int x, int y) {
+                    const float cx = cen[3 * x + ax], cy = cen[3 * y + ax];
+                    return cx != cy ? cx < cy : x < y;
+                });
             }
-            key[j] = {code, t0 + j};
+            todo.push_back({l0, lm});
+            todo.push_back({lm, l1});
         }
-        std::sort(key.begin(), key.end());
         for (int j = 0; j < n; ++j) {
-            const int f = key[j].second, s = t0 + j;
+            const int f = t0 + perm[j], s = t0 + j;
             sorted_of[f] = s;
             o.tv[s] = bm_i4{tris[3 * (size_t)f], tris[3 * (size_t)f + 1], tris[3 * (size_t)f + 2], f};
         }
@@ -369,9 +418,12 @@ inline std::string bm_build_tables(int nbv, const float *pts, int nt, const int3
     std::sort(es.begin(), es.end(), [](const E &x, const E &y) {
         return x.key != y.key ? x.key < y.key : (x.f != y.f ? x.f < y.f : x.e < y.e);
     });
+    o.closed = true;
     for (size_t g0 = 0; g0 < es.size();) {
         size_t g1 = g0 + 1;
         while (g1 < es.size() && es[g1].key == es[g0].key) ++g1;
+        if (g1 - g0 != 2 || tris[3 * (size_t)es[g0].f + es[g0].e] == tris[3 * (size_t)es[g0 + 1].f + es[g0 + 1].e])
+            o.closed = false;
         for (size_t i = g0; i < g1; ++i) {
             int nbr = -1;
             for (size_t q = g0; q < g1; ++q)
@@ -503,6 +555,11 @@ __global__ __launch_bounds__(1024) void k_bm_refit_top(BoneMeshDev m) {
     }
 }
 
+__global__ void k_bm_cache_reset(BoneMeshDev m, int nv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nv) m.cache[i] = bm_f4{0.f, 0.f, 0.f, -INFINITY};
+}
+
 // MODE 0: push X in place (collision stage); MODE 1: penetration depth per vertex
 // into depth, and the count of depths > eps and the largest depth into stat
 template <int MODE>
@@ -510,19 +567,33 @@ __global__ __launch_bounds__(256) void k_bm_query(int nv, float4 *X, BoneMeshDev
                                                   uint32_t *stat) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     float dep = 0.f;
+    __shared__ uint32_t sq;
+    if (threadIdx.x == 0) sq = 0u;
+    __syncthreads();
     if (i < nv) {
         const bool on = MODE == 0 ? m.cw[i] != 0.f : (m.mw == nullptr || m.mw[i] != 0.f);
         if (on) {
             const float4 x = X[i];
             bm_v3 p = bm_mk(x.x, x.y, x.z);
-            if (MODE == 0) {
-                if (bm_push_point(m, m.vgroup[i], p)) X[i] = make_float4(p.x, p.y, p.z, x.w);
-            } else {
-                dep = bm_depth_point(m, m.vgroup[i], p);
+            if (!(m.coherent && bm_certified_outside(m, i, p, MODE == 0 ? m.contact_offset : 0.f))) {
+                atomicAdd(&sq, 1u);
+                BmHit h;
+                float sg = 1.f;
+                const bool found = bm_query(m, m.vgroup[i], p, h, sg);
+                if (m.coherent) bm_certify(m, i, p, found, h, sg);
+                if (found) {
+                    if (MODE == 0) {
+                        if (bm_push_from(m, h, sg, p)) X[i] = make_float4(p.x, p.y, p.z, x.w);
+                    } else {
+                        dep = bm_depth_from(h, sg, p);
+                    }
+                }
             }
         }
         if (MODE == 1) depth[i] = dep;
     }
+    __syncthreads();
+    if (threadIdx.x == 0 && sq) atomicAdd(m.nquery, (unsigned long long)sq);
     if (MODE == 1) {
         // count and maximum over the workgroup, then one atomic each (order-free results)
         __shared__ uint32_t scnt, smax;

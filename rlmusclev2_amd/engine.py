@@ -128,6 +128,7 @@ class XpbdEngine:
         d.contact_offset = float(bm.contact_offset)
         d.max_push = float(bm.max_push)
         d.resolve = int(bool(bm.resolve))
+        d.coherent = int(bm.coherent)
         self.ctx.call("xpbd_set_bone_mesh", C.byref(d))
         self._bm_nbv = d.nverts
 
@@ -142,6 +143,12 @@ class XpbdEngine:
 
     def bone_mesh_push(self) -> None:
         self.ctx.call("xpbd_bone_mesh_push")
+
+    def bone_mesh_queries(self) -> int:
+        """closest-point queries made so far (vertices not certified outside)"""
+        n = np.zeros(1, np.int64)
+        self.ctx.call("xpbd_bone_mesh_queries", ptr(n))
+        return int(n[0])
 
     def bone_penetration(self, collide_eps: float = 1e-6, want_depth: bool = True):
         """(depth per vertex or None, count deeper than collide_eps, largest depth)"""

@@ -63,6 +63,7 @@ class BoneMeshCollider:
     vertex_group: np.ndarray | None = None    # (nv,) group per vertex
     group_bone_off: np.ndarray | None = None  # (ngroups+1,) sub-mesh ranges per group
     resolve: bool = True                      # False: penetration meter only
+    coherent: int = 2                         # skip vertices certified outside: 0 off, 1 on, 2 when closed
 
     @property
     def nbones(self) -> int:

@@ -156,7 +156,8 @@ def scene_from_config(cfg: dict, w: workloads.Workload) -> Scene:
                                          contact_offset=float(coll.get("contact_offset", 1e-5)),
                                          max_push=float(coll.get("max_push", -1.0)), coll_w=m, meter_w=m.copy(),
                                          vertex_group=vg, group_bone_off=goff,
-                                         resolve=bool(coll.get("enabled", False)))
+                                         resolve=bool(coll.get("enabled", False)),
+                                         coherent=int(coll.get("coherent", 2)))
     dcd = None
     sc = cfg.get("demo", {}).get("self_collision", {})
     if sc.get("enabled", False):

@@ -110,6 +110,6 @@ int main(int argc, char **argv) {
     fwrite(pushed.data(), 4, pushed.size(), o);
     fwrite(depth.data(), 4, depth.size(), o);
     fclose(o);
-    printf("ok %d nodes\n", T.nnodes);
+    printf("ok %d nodes closed %d\n", T.nnodes, T.closed ? 1 : 0);
     return 0;
 }

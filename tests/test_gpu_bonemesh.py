@@ -207,3 +207,45 @@ def test_full_size_push_sample_and_meter():
     _compare_push(p[idx], po, q[idx], osd, bm.contact_offset, scale, mq, bm.max_dist)
     d2, n2, _ = sim.engine.bone_penetration(1e-6)
     assert n2 == 0
+
+
+def test_coherent_queries_equal_querying_every_vertex():
+    """the certificate skips only queries whose outcome it proves: frames, the meter
+    and a state set from outside (vertices teleported into the bones) give the
+    same bits with and without it"""
+    w = _small(6)
+    runs = {}
+    for coh in (0, 2):
+        cfg = _cfg(coherent=coh, mesh_res=[64, 12, 48])
+        sim = sims.MultiGridSim(cfg, workload=w, strategy=2, block_tets=256, block_threads=256)
+        meters = []
+        for f in range(1, 9):
+            print(f"coherent {coh} frame {f}", flush=True)
+            if f == 5:
+                sim.engine.set_state(pos=_perturbed(w, np.random.default_rng(9)))
+            sim.step_frame(f)
+            meters.append(sim.engine.bone_penetration(1e-6))
+        runs[coh] = (sim.positions_np(), meters)
+    p0, m0 = runs[0]
+    p2, m2 = runs[2]
+    assert np.array_equal(p0, p2)
+    for (d0, n0, x0), (d2, n2, x2) in zip(m0, m2):
+        assert np.array_equal(d0, d2) and n0 == n2 and x0 == x2
+
+
+def test_coherent_mode_needs_a_closed_mesh():
+    from rlmusclev2_amd.capi import XpbdError
+    cfg = _cfg(mesh_res=[24, 4, 16])
+    w = _small(7)
+    sim = sims.MultiGridSim(cfg, workload=w, strategy=2, block_tets=256, block_threads=256)
+    bm = sim.scene.bone_mesh
+    open_mesh = dataclasses.replace(bm, tris=bm.tris[1:], tri_off=np.r_[0, bm.tri_off[1:] - 1].astype(np.int32),
+                                    coherent=1)
+    with pytest.raises(XpbdError, match="closed"):
+        sim.engine.set_bone_mesh(open_mesh)
+    sim.engine.set_bone_mesh(dataclasses.replace(open_mesh, coherent=2))      # automatic: off on an open mesh
+    q = _perturbed(w, np.random.default_rng(7))
+    sim.engine.set_state(pos=q)
+    d, n, _ = sim.engine.bone_penetration(1e-6)
+    mq = oracle.MeshQuery(bm.verts, open_mesh.tris, open_mesh.tri_off)
+    np.testing.assert_allclose(d, mq.depth(q, bm.max_dist, bm.meter_w), atol=2e-6 * float(np.abs(q).max()))

@@ -220,6 +220,7 @@ def _run_tree(exe, tmp_path, v, t, toff, q, max_dist, off, max_push):
         np.array([max_dist, off, max_push], np.float32).tofile(f)
     r = subprocess.run([str(exe), str(inp), str(out)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+    _run_tree.closed = r.stdout.split()[-1] == "1"
     raw = np.fromfile(out, np.uint8)
     o = 0
 
@@ -298,3 +299,18 @@ def test_library_tree_matches_oracle(tmp_path, case):
         land, _, _, _ = mq.query(pushed[tie], max_dist)       # outside, at most off from the surface
         assert (land >= -2e-6 * scale).all() and (land <= off + 2e-6 * scale).all()
     np.testing.assert_allclose(depth, mq.depth(q, max_dist), atol=2e-6 * scale)
+
+
+def test_closed_mesh_detection(tmp_path):
+    """the coherent mode's precondition, as the library's table builder sees it"""
+    exe = _tree_checker(tmp_path)
+    v, t = workloads.capsule_mesh([0, 0, 0], [0, 0, 0.5], 0.025, 24, 4, 12)
+    q = np.zeros((1, 3), np.float32)
+    cases = {"closed": (t, True), "hole": (t[1:], False),
+             "one flipped": (np.r_[t[:1, [0, 2, 1]], t[1:]], False)}
+    for name, (tt, want) in cases.items():
+        _run_tree(exe, tmp_path, v, tt, np.array([0, len(tt)]), q, 0.2, 0.0, -1.0)
+        assert _run_tree.closed == want, name
+    (wv, wt), _ = wedge(10.0)
+    _run_tree(exe, tmp_path, wv, wt, np.array([0, len(wt)]), q, 0.2, 0.0, -1.0)
+    assert _run_tree.closed
