@@ -183,6 +183,32 @@ int32_t xpbd_set_coarse_level(xpbd_ctx *ctx, xpbd_ctx *coarse, const xpbd_coarse
  * caller order; only the pins' entries are read), per frame (_refresh_coarse_target,
  * mg.py:1013-1030) */
 int32_t xpbd_set_coarse_pin_targets(xpbd_ctx *ctx, int32_t n, const float *xyz);
+/* ---- fascia shell, ride mode (Fascia, src/muscle/fascia/fascia.py:45-572, fascia_mode "ride") ----
+ * The fascia is another finalized context (its own tet mesh, SNH constraints, quasi-static
+ * parameters).  xpbd_set_fascia binds it at rest (_build_muscle_bind, fascia.py:443-490): each
+ * fascia vertex gets the closest point on this context's surface triangles among the faces nearer
+ * than bind_max_dist (barycentric weights and the normal gap (x - closest)·n, n the triangle's unit
+ * normal); at near-ties the lowest triangle index wins.  Bound vertices become the fascia's active
+ * pins.  From then on every substep of this context (after its iterations, before its DCD and bone
+ * push; MultiGridSim post_phase, mg.py:1362-1366) moves each bound fascia vertex to its barycentric
+ * point on the current surface plus gap along the current unit normal, then runs one fascia substep
+ * of iters iterations (ride, integrate, iters x (projection, apply, ride);
+ * Fascia.build_program_ride, fascia.py:518-538).  The fascia never moves this context.  The fascia
+ * runs on this context's stream; finalizing either context drops the link.  NULL fascia or desc
+ * removes it. */
+typedef struct {
+    int32_t nsurf;
+    const int32_t *surf_tris;   /* nsurf*3 vertex ids of this context's surface (caller order) */
+    float bind_max_dist;        /* fascia_bind_max_dist                                        */
+    int32_t iters;              /* fascia relax iterations per substep (fascia_glue.iterations) */
+} xpbd_fascia_desc;
+int32_t xpbd_set_fascia(xpbd_ctx *ctx, xpbd_ctx *fascia, const xpbd_fascia_desc *desc);
+/* the fascia's substep alone on the current positions (Fascia.substep, fascia.py:567-572), for
+ * callers that drive the phases one by one */
+int32_t xpbd_fascia_substep(xpbd_ctx *ctx);
+/* the bind, per fascia vertex (caller order; any pointer may be NULL): surface triangle (-1:
+ * free), 3 barycentric weights, gap; and the number bound */
+int32_t xpbd_get_fascia_bind(xpbd_ctx *ctx, int32_t *face, float *w, float *gap, int32_t *nbound);
 /* the V-cycle alone, for callers that drive the phases one by one */
 int32_t xpbd_vcycle(xpbd_ctx *ctx);
 

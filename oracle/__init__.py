@@ -83,6 +83,7 @@ def load():
                                        C.c_float, _P, _P, _P, _P, C.c_int]
         lib.orc_mesh_push.argtypes = [C.c_int, _P, _P, _P, _P, _P, C.c_float, C.c_float]
         lib.orc_mesh_push.restype = C.c_int
+        lib.orc_mesh_bind.argtypes = [C.c_int, _P, _P, C.c_int, _P, C.c_float, _P, _P, C.c_int]
         lib.orc_residuals.argtypes = [C.c_int, _P, _P, _P, _P, _P, _P, _P]
         lib.orc_count_inverted.argtypes = [C.c_int, _P, _P, _P]
         lib.orc_count_inverted.restype = C.c_int
@@ -191,6 +192,55 @@ class MeshQuery:
         if meter_w is not None:
             d[np.asarray(meter_w) == 0] = 0.0
         return d
+
+
+def fascia_bind(surf_pos, surf_tris, points, max_dist, nthreads=8):
+    """_build_muscle_bind (fascia.py:443-490) by brute force: per point the nearest
+    surface triangle among those nearer than max_dist (lowest index at near-ties),
+    barycentric weights of the closest point and the normal gap (x - closest)·n.
+    Returns face (-1 free), weights (n, 3) and gap (n,), in float64."""
+    lib = load()
+    sp = np.ascontiguousarray(surf_pos, np.float32).reshape(-1, 3)
+    tr = np.ascontiguousarray(surf_tris, np.int32).reshape(-1, 3)
+    q = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    n = len(q)
+    face = np.zeros(n, np.int32)
+    cp = np.zeros((n, 3), np.float64)
+    lib.orc_mesh_bind(n, _p(q), _p(sp), len(tr), _p(tr), C.c_float(max_dist), _p(face), _p(cp), int(nthreads))
+    w = np.zeros((n, 3))
+    gap = np.zeros(n)
+    ok = face >= 0
+    A, B, Cc = (sp[tr[face[ok], k]].astype(np.float64) for k in range(3))
+    v0, v1, v2 = B - A, Cc - A, cp[ok] - A
+    d00, d01, d11 = (v0 * v0).sum(1), (v0 * v1).sum(1), (v1 * v1).sum(1)
+    d20, d21 = (v2 * v0).sum(1), (v2 * v1).sum(1)
+    den = d00 * d11 - d01 * d01
+    safe = den > 0
+    wb = np.where(safe, (d11 * d20 - d01 * d21) / np.where(safe, den, 1.0), 0.0).clip(0.0)
+    wc = np.where(safe, (d00 * d21 - d01 * d20) / np.where(safe, den, 1.0), 0.0).clip(0.0)
+    over = wb + wc > 1.0
+    ssum = np.where(over, wb + wc, 1.0)
+    wb, wc = wb / ssum, wc / ssum
+    w[ok] = np.c_[1.0 - wb - wc, wb, wc]
+    nrm = np.cross(v0, v1)
+    nl = np.linalg.norm(nrm, axis=1, keepdims=True)
+    nrm = np.where(nl > 0, nrm / np.where(nl > 0, nl, 1.0), 0.0)
+    gap[ok] = ((q[ok].astype(np.float64) - cp[ok]) * nrm).sum(1)
+    return face, w, gap
+
+
+def fascia_ride_targets(x, surf_tris, face, w, gap):
+    """fascia_ride_kernel (fascia.py:159-176) in FP32: barycentric point of the current
+    triangle plus gap along its unit normal (rows with face >= 0)"""
+    tr = np.asarray(surf_tris, np.int64).reshape(-1, 3)[face]
+    x = np.asarray(x, np.float32)
+    a, b, c = x[tr[:, 0]], x[tr[:, 1]], x[tr[:, 2]]
+    w = np.asarray(w, np.float32)
+    cp = (w[:, 0:1] * a + w[:, 1:2] * b) + w[:, 2:3] * c
+    n = np.cross(b - a, c - a).astype(np.float32)
+    ln = np.sqrt((n * n).sum(1, keepdims=True))
+    u = np.where(ln > 0, n / np.where(ln > 0, ln, 1), 0).astype(np.float32)
+    return (cp + np.asarray(gap, np.float32)[:, None] * u).astype(np.float32)
 
 
 class OracleSim:
@@ -397,10 +447,39 @@ class OracleSim:
     def set_coarse_pin_targets(self, xyz):
         self._mg_pin_goal = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
 
+    # fascia shell, ride mode (Fascia, fascia.py:245-547): another OracleSim, bound at rest
+    def set_fascia(self, fascia, surf_tris, bind_max_dist, iters):
+        if fascia is None:
+            self._fascia = None
+            return
+        face, w, gap = fascia_bind(self.pos0, surf_tris, fascia.pos0, bind_max_dist, max(self.nthreads, 8))
+        bound = np.nonzero(face >= 0)[0].astype(np.int32)
+        tris = np.asarray(surf_tris, np.int32).reshape(-1, 3)
+        fascia.pin_vids = bound
+        fascia.pin_active = np.ones(len(bound), np.int32)
+        fascia.pin_target = fascia_ride_targets(self.pos0, tris, face[bound], w[bound], gap[bound])
+        fascia.pin_zero = np.zeros_like(fascia.pin_target)
+        self._fascia = (fascia, tris, bound, face[bound], w[bound], gap[bound], int(iters))
+        self.fascia_bind = (face, w, gap)
+
+    def fascia_substep(self):
+        if getattr(self, "_fascia", None) is not None:
+            self._fascia_substep()
+
+    def _fascia_substep(self):
+        f, tris, bound, face, w, gap, iters = self._fascia
+        if len(bound):
+            g = fascia_ride_targets(self.pos, tris, face, w, gap)
+            f.pos[bound] = g            # the ride snaps before the fascia's integrate
+            f.pin_target = g
+        f.native_substep(iters)
+
     def native_substep(self, iters):
         self.integrate()
         self.vcycle()
         self.iterate(iters)
+        if getattr(self, "_fascia", None) is not None:
+            self._fascia_substep()
         self.collide()
         self.update_velocities()
 

@@ -1290,3 +1290,38 @@ int orc_mesh_push(int nv, float *pos, const float *coll_w, const double *sd, con
     }
     return pushed;
 }
+
+/* the fascia bind's closest point (fascia_bind_kernel, src/muscle/fascia/fascia.py:135-157):
+ * for each point the face of the triangle mesh nearest to it among the faces nearer than
+ * max_dist (brute force, double); at near-ties (within 1e-6 relative of the nearest distance)
+ * the lowest triangle index, the rule the library's bind uses as well; face -1 = none */
+void orc_mesh_bind(int nq, const float *q, const float *bv, int ntris, const int *tris, float max_dist,
+                   int *face, double *cp, int nthreads) {
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int i = 0; i < nq; ++i) {
+        d3 p = dld(q, i);
+        double md2 = (double)max_dist * (double)max_dist, best = INFINITY;
+        for (int f = 0; f < ntris; ++f) {
+            int reg;
+            const int *t = tris + 3 * f;
+            d3 c = tri_closest(p, dld(bv, t[0]), dld(bv, t[1]), dld(bv, t[2]), &reg);
+            d3 d = dsub(p, c);
+            double e = ddot(d, d);
+            if (e < md2 && e < best) best = e;
+        }
+        face[i] = -1;
+        if (!(best < INFINITY)) continue;
+        for (int f = 0; f < ntris; ++f) {
+            int reg;
+            const int *t = tris + 3 * f;
+            d3 c = tri_closest(p, dld(bv, t[0]), dld(bv, t[1]), dld(bv, t[2]), &reg);
+            d3 d = dsub(p, c);
+            double e = ddot(d, d);
+            if (e < md2 && e <= best * (1.0 + 2e-6) + 1e-14) {
+                face[i] = f;
+                cp[3 * i] = c.x; cp[3 * i + 1] = c.y; cp[3 * i + 2] = c.z;
+                break;
+            }
+        }
+    }
+}

@@ -138,6 +138,8 @@ void orc_mesh_query(int nq, const float *q, int nbv, const float *bv, int ntris,
                     int *region, int nthreads);
 int  orc_mesh_push(int nv, float *pos, const float *coll_w, const double *sd, const double *cp,
                    const int *face, float contact_offset, float max_push);
+void orc_mesh_bind(int nq, const float *q, const float *bv, int ntris, const int *tris, float max_dist,
+                   int *face, double *cp, int nthreads);
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,11 @@ class XpbdBoneMeshDesc(C.Structure):
                 ("resolve", C.c_int32), ("coherent", C.c_int32)]
 
 
+class XpbdFasciaDesc(C.Structure):
+    _fields_ = [("nsurf", C.c_int32), ("surf_tris", C.c_void_p), ("bind_max_dist", C.c_float),
+                ("iters", C.c_int32)]
+
+
 class XpbdMlpDev(C.Structure):
     _fields_ = [("W1", C.c_void_p), ("b1", C.c_void_p), ("W2", C.c_void_p), ("b2", C.c_void_p),
                 ("W3", C.c_void_p), ("b3", C.c_void_p)]
@@ -127,6 +132,9 @@ SIGNATURES = {
     "xpbd_bone_mesh_push": [_P],
     "xpbd_bone_penetration": [_P, _F, _P, _P, _P],
     "xpbd_bone_mesh_queries": [_P, _P],
+    "xpbd_set_fascia": [_P, _P, C.POINTER(XpbdFasciaDesc)],
+    "xpbd_get_fascia_bind": [_P, _P, _P, _P, _P],
+    "xpbd_fascia_substep": [_P],
     "xpbd_dcd_solve": [_P, _I],
     "xpbd_dcd_pairs": [_P, _P, C.c_int64, _P],
     "xpbd_dcd_stats": [_P, _P, _P, _P],

@@ -1326,6 +1326,28 @@ __global__ void k_apply_dpw(int nv, float4 *X, const float4 *dpw, const float4 *
     X[i] = x;
 }
 
+// fascia ride (fascia_ride_kernel, src/muscle/fascia/fascia.py:159-176): each bound fascia
+// vertex goes to its barycentric point on the current surface triangle plus the rest normal gap
+// along the triangle's unit normal; written to the fascia positions (before its integrate, as the
+// reference snaps before it) and to its pin targets (re-applied after every fascia iteration)
+__global__ void k_fascia_ride(int n, const float4 *Xm, const int32_t *vid, const int4 *tri, const float4 *w,
+                              float4 *Xf, float4 *PINf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int4 t = tri[i];
+    const float4 ww = w[i];
+    const V3 a = xyz(Xm[t.x]), b = xyz(Xm[t.y]), c = xyz(Xm[t.z]);
+    const V3 cp = ww.x * a + ww.y * b + ww.z * c;
+    const V3 nn = cross(b - a, c - a);
+    const float l = sqrtf(dot(nn, nn));
+    const V3 u = l > 0.f ? nn / l : mk3(0.f, 0.f, 0.f);
+    const V3 g = cp + ww.w * u;
+    const int k = vid[i];
+    const float4 x = Xf[k];
+    Xf[k] = make_float4(g.x, g.y, g.z, x.w);
+    PINf[k] = make_float4(g.x, g.y, g.z, 1.0f);
+}
+
 __global__ void k_pins(int nv, float4 *X, const float4 *pin) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nv) return;
@@ -2645,6 +2667,17 @@ struct xpbd_ctx {
     float4 *BPART = nullptr;
     // two-level correction: a finalized coarse context (not owned) and transfer tables
     xpbd_ctx *coarse = nullptr;
+    // fascia shell in ride mode: a finalized context (not owned) whose bound vertices follow this
+    // context's surface; gen counts this context's finalizes, so that a stale link is detected
+    uint64_t gen = 0;
+    xpbd_ctx *fascia = nullptr;
+    uint64_t fascia_gen = 0;
+    int fascia_iters = 0, fascia_nbound = 0;
+    int32_t *d_fb_vid = nullptr;      // fascia library index of each bound vertex
+    int4 *d_fb_tri = nullptr;         // this context's library indices of its triangle
+    float4 *d_fb_w = nullptr;         // barycentric weights of a, b, c and the normal gap
+    std::vector<int32_t> fb_face, fb_ok;   // per fascia vertex (caller order): surface triangle, bound
+    std::vector<float> fb_w, fb_gap;       // 3 weights, gap
     int mg_iters = 0;
     bool mg_pidx_set = false, mg_kinematic = false;
     int mg_nblk = 0;
@@ -2737,6 +2770,19 @@ int dalloc(xpbd_ctx *c, T **p, size_t count) {
     return XPBD_OK;
 }
 
+// free one buffer of the context's list (the stream must not use it any more)
+template <typename T>
+void dfree(xpbd_ctx *c, T *&p) {
+    if (!p) return;
+    for (size_t i = 0; i < c->bufs.size(); ++i)
+        if (c->bufs[i].p == (void *)p) {
+            (void)hipFree(c->bufs[i].p);
+            c->bufs.erase(c->bufs.begin() + i);
+            break;
+        }
+    p = nullptr;
+}
+
 template <typename T>
 int dupload(xpbd_ctx *c, T **p, const T *src, size_t count) {
     int r = dalloc(c, p, count);
@@ -2777,6 +2823,9 @@ void free_device(xpbd_ctx *c) {
     c->d_elem_group = nullptr;
     c->d_gact = nullptr;
     c->d_mg_pin_tgt = nullptr;
+    c->d_fb_vid = nullptr;
+    c->d_fb_tri = nullptr;
+    c->d_fb_w = nullptr;
     c->SFEAT = nullptr;
     c->sfeat_ok = false;
     c->pin_cap = 0;
@@ -3502,11 +3551,30 @@ int launch_vcycle(xpbd_ctx *c, bool pins = true) {
     return XPBD_OK;
 }
 
+int enqueue_substep(xpbd_ctx *c, int iters, bool dcd_rebuild, int sub = 0, bool integrated = false,
+                    bool fuse_next = false);
+
+// the fascia shell's substep (Fascia.build_program_ride, fascia.py:518-538): ride, integrate,
+// iters x (projection, apply, ride) -- the ride after each iteration is the pin overwrite
+int launch_fascia(xpbd_ctx *c, const float4 *Xm) {
+    xpbd_ctx *f = c->fascia;
+    if (f->gen != c->fascia_gen || !f->ready)
+        return fail(c, XPBD_ENOTREADY, "the fascia context was finalized again; call xpbd_set_fascia again");
+    if (c->fascia_nbound > 0) {
+        hipLaunchKernelGGL(k_fascia_ride, dim3((unsigned)((c->fascia_nbound + 255) / 256)), dim3(256), 0, c->stream,
+                           c->fascia_nbound, Xm, (const int32_t *)c->d_fb_vid, (const int4 *)c->d_fb_tri,
+                           (const float4 *)c->d_fb_w, f->X[0], f->PIN);
+        HIPCHK(c, hipGetLastError());
+    }
+    const int r = enqueue_substep(f, c->fascia_iters, false, 0, false, false);
+    if (r) return fail(c, r, std::string("fascia substep: ") + f->err);
+    return XPBD_OK;
+}
+
 // one substep, positions start and end in X[0]; sub = index within the frame
 // integrated: the previous substep's post already ran this substep's
 // integrate; fuse_next: this substep's post runs the next one's integrate
-int enqueue_substep(xpbd_ctx *c, int iters, bool dcd_rebuild, int sub = 0, bool integrated = false,
-                    bool fuse_next = false) {
+int enqueue_substep(xpbd_ctx *c, int iters, bool dcd_rebuild, int sub, bool integrated, bool fuse_next) {
     int r = integrated ? XPBD_OK : launch_integrate(c);
     if (r) return r;
     if (c->coarse && c->mg_iters > 0 && (r = launch_vcycle(c, false))) return r;
@@ -3527,7 +3595,9 @@ int enqueue_substep(xpbd_ctx *c, int iters, bool dcd_rebuild, int sub = 0, bool 
         if (r) return r;
     }
     if ((r = finish_lambda(c))) return r;
-    // post phase: self DCD passes, then bone push-out and velocities (mg.py:1362-1376)
+    // post phase (mg.py:1362-1376): the fascia shell's substep, the self DCD passes, then the
+    // bone push-out and velocities
+    if (c->fascia && (r = launch_fascia(c, c->X[iters & 1]))) return r;
     // the last pass's apply runs inside the post kernel
     const bool dcd = c->dcd_on && c->dcd_iters > 0;
     if (dcd)
@@ -3595,6 +3665,10 @@ int32_t xpbd_set_stream(xpbd_ctx *c, void *stream) {
     if (!c) return XPBD_EINVAL;
     c->stream = stream ? (hipStream_t)stream : c->own_stream;
     invalidate_graph(c);
+    if (c->fascia) {                 // the fascia's substeps run inside this context's
+        c->fascia->stream = c->stream;
+        invalidate_graph(c->fascia);
+    }
     return XPBD_OK;
 }
 
@@ -4077,6 +4151,9 @@ int32_t xpbd_finalize(xpbd_ctx *c, const xpbd_tuning *t) {
     HIPCHK(c, hipSetDevice(c->device));
     free_device(c);
     c->coarse = nullptr;
+    c->fascia = nullptr;
+    c->fascia_nbound = 0;
+    ++c->gen;
     c->mg_iters = 0;
     c->mg_pidx_set = false;
     int P = (t && t->patch_verts > 0) ? t->patch_verts : 256;
@@ -4620,6 +4697,208 @@ int32_t xpbd_set_mlp(xpbd_ctx *c, const float *W1, const float *b1, const float 
     HIPCHK(c, hipMemcpy(c->finv, feat_invstd, 24 * 4, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->oscale, out_scale, 4 * 4, hipMemcpyHostToDevice));
     c->mlp_ready = true;
+    return XPBD_OK;
+}
+
+namespace {
+
+// closest point of triangle a b c to p in double precision (Ericson, Real-Time
+// Collision Detection, 5.1.5), for the one-time fascia bind
+struct D3 { double x, y, z; };
+D3 d3sub(D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+double d3dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+D3 d3axpy(D3 a, double s, D3 d) { return D3{a.x + s * d.x, a.y + s * d.y, a.z + s * d.z}; }
+D3 d3cross(D3 a, D3 b) { return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+
+D3 tri_closest_d(D3 p, D3 a, D3 b, D3 c) {
+    const D3 ab = d3sub(b, a), ac = d3sub(c, a), ap = d3sub(p, a);
+    const double d1 = d3dot(ab, ap), d2 = d3dot(ac, ap);
+    if (d1 <= 0 && d2 <= 0) return a;
+    const D3 bp = d3sub(p, b);
+    const double d3 = d3dot(ab, bp), d4 = d3dot(ac, bp);
+    if (d3 >= 0 && d4 <= d3) return b;
+    const double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0 && d1 >= 0 && d3 <= 0) return d3axpy(a, d1 / (d1 - d3), ab);
+    const D3 cq = d3sub(p, c);
+    const double d5 = d3dot(ab, cq), d6 = d3dot(ac, cq);
+    if (d6 >= 0 && d5 <= d6) return c;
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0 && d2 >= 0 && d6 <= 0) return d3axpy(a, d2 / (d2 - d6), ac);
+    const double va = d3 * d6 - d5 * d4;
+    if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) return d3axpy(b, (d4 - d3) / ((d4 - d3) + (d5 - d6)), d3sub(c, b));
+    const double den = va + vb + vc;
+    if (!(den > 0)) {            // zero-area triangle: the nearest point of its three edges
+        auto seg = [&](D3 s0, D3 s1) {
+            const D3 e = d3sub(s1, s0);
+            const double l2 = d3dot(e, e);
+            double t = l2 > 0 ? d3dot(d3sub(p, s0), e) / l2 : 0.0;
+            t = t < 0 ? 0 : (t > 1 ? 1 : t);
+            return d3axpy(s0, t, e);
+        };
+        const D3 q0 = seg(a, b), q1 = seg(b, c), q2 = seg(c, a);
+        const double e0 = d3dot(d3sub(p, q0), d3sub(p, q0)), e1 = d3dot(d3sub(p, q1), d3sub(p, q1)),
+                     e2 = d3dot(d3sub(p, q2), d3sub(p, q2));
+        return e0 <= e1 && e0 <= e2 ? q0 : (e1 <= e2 ? q1 : q2);
+    }
+    return d3axpy(d3axpy(a, vb / den, ab), vc / den, ac);
+}
+
+}  // namespace
+
+int32_t xpbd_set_fascia(xpbd_ctx *c, xpbd_ctx *f, const xpbd_fascia_desc *d) {
+    if (!c) return XPBD_EINVAL;
+    if (!f || !d) {
+        c->fascia = nullptr;
+        c->fascia_nbound = 0;
+        invalidate_graph(c);
+        return XPBD_OK;
+    }
+    c->fascia = nullptr;             // until the new link is complete
+    int r = check_ready(c);
+    if (r) return r;
+    if (f == c) return fail(c, XPBD_EINVAL, "a context cannot be its own fascia");
+    if (!f->ready) return fail(c, XPBD_ENOTREADY, "finalize the fascia context first");
+    if (f->device != c->device) return fail(c, XPBD_EINVAL, "the fascia context is on another device");
+    if (d->nsurf < 1 || !d->surf_tris) return fail(c, XPBD_EINVAL, "no surface triangles");
+    if (!(d->bind_max_dist > 0.0f) || d->iters < 0) return fail(c, XPBD_EINVAL, "bad bind_max_dist or iters");
+    for (int64_t i = 0; i < 3 * (int64_t)d->nsurf; ++i)
+        if (d->surf_tris[i] < 0 || d->surf_tris[i] >= c->nv)
+            return fail(c, XPBD_EINVAL, "surface triangle vertex out of range");
+    // the bind (fascia_bind_kernel, fascia.py:135-157): the closest point of the rest surface to
+    // each fascia rest vertex among the faces nearer than bind_max_dist, found with the bone-mesh
+    // tree on the host; at near-ties (two faces within 1e-6 relative of the nearest distance) the
+    // lowest triangle index wins, so that the choice does not depend on the tree
+    const int ns = d->nsurf, nfv = f->nv;
+    BmTables T;
+    const int32_t off[2] = {0, ns};
+    std::string e = bm_build_tables(c->nv, c->pos0.data(), ns, d->surf_tris, 1, off, T);
+    if (!e.empty()) return fail(c, XPBD_EINVAL, e);
+    std::vector<bm_f4> pts(c->nv + 1), soup(3 * (size_t)ns), fn(ns), vn(c->nv), node(2 * (size_t)T.nnodes);
+    for (int i = 0; i < c->nv; ++i) pts[i] = bm_f4{c->pos0[3 * i], c->pos0[3 * i + 1], c->pos0[3 * i + 2], 0.f};
+    const int32_t goff[2] = {0, 1}, vg0 = 0;
+    BoneMeshDev m{};
+    m.nbones = 1; m.ntris = ns; m.nbv = c->nv;
+    m.bone = T.bone.data(); m.tv = T.tv.data(); m.adj = T.adj.data(); m.sub = T.sub.data();
+    m.vt_off = T.vt_off.data(); m.vt = T.vt.data(); m.pts = pts.data(); m.soup = soup.data(); m.fn = fn.data();
+    m.vn = vn.data(); m.node = node.data(); m.vgroup = &vg0; m.goff = goff;
+    m.max_dist = d->bind_max_dist;
+    bm_refit_host(m);
+    auto P = [&](int v) { return D3{c->pos0[3 * v], c->pos0[3 * v + 1], c->pos0[3 * v + 2]}; };
+    c->fb_face.assign(nfv, -1);
+    c->fb_ok.assign(nfv, 0);
+    c->fb_w.assign(3 * (size_t)nfv, 0.f);
+    c->fb_gap.assign(nfv, 0.f);
+    std::vector<int32_t> bvid;
+    std::vector<int4> btri;
+    std::vector<float4> bw;
+    std::vector<float> tgt;
+    const double md2 = (double)d->bind_max_dist * d->bind_max_dist;
+    std::vector<int> stack;
+    for (int i = 0; i < nfv; ++i) {
+        const D3 p{f->pos0[3 * i], f->pos0[3 * i + 1], f->pos0[3 * i + 2]};
+        BmHit h;
+        float sg;
+        if (!bm_query(m, 0, bm_mk((float)p.x, (float)p.y, (float)p.z), h, sg)) continue;
+        // every face whose box is within the float result plus slack, then exact distances
+        const double r2 = std::min(md2, (double)h.d2 * (1.0 + 1e-3) + 1e-12);
+        const bm_i4 bn = T.bone[0];
+        const int D = bm_depth(bn.y);
+        double best = INFINITY;
+        std::vector<std::pair<double, int>> cand;
+        stack.assign(1, 0);
+        std::vector<int> depth(1, 0);
+        while (!stack.empty()) {
+            const int k = stack.back(), dk = depth.back();
+            stack.pop_back();
+            depth.pop_back();
+            if ((double)bm_box_d2(node.data() + 2 * (size_t)bn.x, k, bm_mk((float)p.x, (float)p.y, (float)p.z)) >
+                r2 * (1.0 + 1e-3) + 1e-12)
+                continue;
+            if (dk == D) {
+                const int leaf = k - (bn.y - 1);
+                for (int j = leaf * BM_LEAF; j < std::min(leaf * BM_LEAF + BM_LEAF, bn.w); ++j) {
+                    const int ft = T.tv[bn.z + j].w;
+                    const int32_t *tt = d->surf_tris + 3 * (size_t)ft;
+                    const D3 q = tri_closest_d(p, P(tt[0]), P(tt[1]), P(tt[2]));
+                    const D3 dd = d3sub(p, q);
+                    const double e2 = d3dot(dd, dd);
+                    if (e2 < md2) {
+                        cand.push_back({e2, ft});
+                        best = std::min(best, e2);
+                    }
+                }
+            } else {
+                stack.push_back(2 * k + 1); depth.push_back(dk + 1);
+                stack.push_back(2 * k + 2); depth.push_back(dk + 1);
+            }
+        }
+        if (cand.empty()) continue;
+        int ft = -1;
+        for (auto &q : cand)
+            if (q.first <= best * (1.0 + 2e-6) + 1e-14 && (ft < 0 || q.second < ft)) ft = q.second;
+        const int32_t *tt = d->surf_tris + 3 * (size_t)ft;
+        const D3 A = P(tt[0]), B = P(tt[1]), C = P(tt[2]);
+        const D3 q = tri_closest_d(p, A, B, C);
+        // barycentric weights of q (Cramer's rule), clamped onto the triangle
+        const D3 v0 = d3sub(B, A), v1 = d3sub(C, A), v2 = d3sub(q, A);
+        const double d00 = d3dot(v0, v0), d01 = d3dot(v0, v1), d11 = d3dot(v1, v1), d20 = d3dot(v2, v0),
+                     d21 = d3dot(v2, v1), den = d00 * d11 - d01 * d01;
+        double wb = 0, wc = 0;
+        if (den > 0) { wb = (d11 * d20 - d01 * d21) / den; wc = (d00 * d21 - d01 * d20) / den; }
+        wb = std::max(0.0, wb);
+        wc = std::max(0.0, wc);
+        if (wb + wc > 1.0) { const double s = wb + wc; wb /= s; wc /= s; }
+        const double wa = 1.0 - wb - wc;
+        D3 nrm = d3cross(v0, v1);
+        const double nl = std::sqrt(d3dot(nrm, nrm));
+        nrm = nl > 0 ? D3{nrm.x / nl, nrm.y / nl, nrm.z / nl} : D3{0, 0, 0};
+        const double gap = d3dot(d3sub(p, q), nrm);
+        c->fb_face[i] = ft;
+        c->fb_ok[i] = 1;
+        c->fb_w[3 * i] = (float)wa; c->fb_w[3 * i + 1] = (float)wb; c->fb_w[3 * i + 2] = (float)wc;
+        c->fb_gap[i] = (float)gap;
+        bvid.push_back(i);
+        btri.push_back(make_int4(c->L.iperm[tt[0]], c->L.iperm[tt[1]], c->L.iperm[tt[2]], 0));
+        bw.push_back(make_float4((float)wa, (float)wb, (float)wc, (float)gap));
+        const D3 g = d3axpy(q, gap, nrm);
+        tgt.push_back((float)g.x); tgt.push_back((float)g.y); tgt.push_back((float)g.z);
+    }
+    // the bound vertices become the fascia's active pins (their targets at rest: the bind)
+    const int nb = (int)bvid.size();
+    std::vector<int32_t> act(nb, 1), fvid(nb);
+    if ((r = xpbd_set_pins(f, nb, bvid.data(), act.data())) || (r = xpbd_set_pin_targets(f, nb, tgt.data())))
+        return fail(c, r, std::string("fascia pins: ") + f->err);
+    for (int k = 0; k < nb; ++k) fvid[k] = f->L.iperm[bvid[k]];
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(c, c->d_fb_vid);
+    dfree(c, c->d_fb_tri);
+    dfree(c, c->d_fb_w);
+    if ((r = dupload(c, &c->d_fb_vid, fvid.data(), (size_t)nb)) ||
+        (r = dupload(c, &c->d_fb_tri, btri.data(), (size_t)nb)) || (r = dupload(c, &c->d_fb_w, bw.data(), (size_t)nb)))
+        return r;
+    c->fascia = f;
+    c->fascia_gen = f->gen;
+    c->fascia_iters = d->iters;
+    c->fascia_nbound = nb;
+    f->stream = c->stream;
+    invalidate_graph(f);
+    invalidate_graph(c);
+    return XPBD_OK;
+}
+
+int32_t xpbd_fascia_substep(xpbd_ctx *c) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!c->fascia) return fail(c, XPBD_EINVAL, "no fascia");
+    return launch_fascia(c, c->X[0]);
+}
+
+int32_t xpbd_get_fascia_bind(xpbd_ctx *c, int32_t *face, float *w, float *gap, int32_t *nbound) {
+    if (!c || !c->fascia) return fail(c, XPBD_EINVAL, "no fascia");
+    if (face) std::copy(c->fb_face.begin(), c->fb_face.end(), face);
+    if (w) std::copy(c->fb_w.begin(), c->fb_w.end(), w);
+    if (gap) std::copy(c->fb_gap.begin(), c->fb_gap.end(), gap);
+    if (nbound) *nbound = c->fascia_nbound;
     return XPBD_OK;
 }
 

@@ -158,6 +158,36 @@ class XpbdEngine:
         self.ctx.call("xpbd_bone_penetration", C.c_float(collide_eps), ptr(depth), C.byref(cnt), C.byref(mx))
         return depth, int(cnt.value), float(mx.value)
 
+    # fascia shell, ride mode (Fascia, src/muscle/fascia/fascia.py:45-572) -----------
+    def set_fascia(self, fascia, surf_tris, bind_max_dist: float, iters: int) -> None:
+        """Bind the finalized fascia engine to this mesh's surface; None unlinks."""
+        if fascia is None:
+            self.ctx.call("xpbd_set_fascia", None, None)
+            self._fascia = None
+            return
+        k = self._keep
+        k["fascia_tris"] = i32(surf_tris, (-1, 3))
+        d = capi.XpbdFasciaDesc()
+        d.nsurf = k["fascia_tris"].shape[0]
+        d.surf_tris = ptr(k["fascia_tris"])
+        d.bind_max_dist = float(bind_max_dist)
+        d.iters = int(iters)
+        self.ctx.call("xpbd_set_fascia", fascia.ctx.h, C.byref(d))
+        self._fascia = fascia
+
+    def fascia_substep(self) -> None:
+        self.ctx.call("xpbd_fascia_substep")
+
+    def fascia_bind(self):
+        """(face, weights (n, 3), gap, n bound) of the bind, per fascia vertex"""
+        n = self._fascia.NV
+        face = np.zeros(n, np.int32)
+        w = np.zeros((n, 3), np.float32)
+        gap = np.zeros(n, np.float32)
+        nb = np.zeros(1, np.int32)
+        self.ctx.call("xpbd_get_fascia_bind", ptr(face), ptr(w), ptr(gap), ptr(nb))
+        return face, w, gap, int(nb[0])
+
     def _set_dcd(self, spec):
         from .collision import DcdParams
         p = spec.params or DcdParams()

@@ -50,6 +50,52 @@ def mg_constraint_configs(cfg: dict) -> list:
     return out
 
 
+def fascia_scene(cfg: dict, pos, tets, dt: float) -> Scene:
+    """The fascia shell in ride mode (Fascia.__init__, fascia.py:245-330): Stable
+    Neo-Hookean from the 'fascia_snh' entry, density from fascia_total_mass (else the
+    muscle's total_mass) over the shell's rest volume, quasi-static, no gravity."""
+    snh = find_constraint(cfg, "fascia_snh") or {}
+    ccfg = [dict(type="snh", name="fascia_snh", mu=float(snh.get("mu", 1.0e3)), lam=float(snh.get("lam", 1.0e8)),
+                 dampingratio=float(snh.get("dampingratio", 0.0)))]
+    pos = np.asarray(pos, np.float32)
+    tets = np.asarray(tets, np.int32)
+    cons = builder.build_from_configs(pos, tets, ccfg)
+    volumes, valid = builder.tet_rest_data(pos, tets)
+    total = float(cfg.get("fascia_total_mass", cfg.get("total_mass", 1.0)))
+    density = total / float(max(volumes[valid].sum(), 1e-30))
+    params = SimParams(dt=dt, gravity=(0.0, 0.0, 0.0), quasi_static=True)
+    return Scene(pos0=pos, tets=tets, density=density, cons=cons, params=params,
+                 meta=dict(workload="fascia", constraint_configs=ccfg))
+
+
+class FasciaShell:
+    """Fascia sub-solver in ride mode (Fascia, fascia.py:217-572): its own engine on
+    its own mesh, bound to the muscle surface; its substep runs inside the muscle's."""
+
+    def __init__(self, parent, engine_cls, device, tuning):
+        cfg = parent.cfg
+        self.mode = str(cfg.get("fascia_mode", "squeeze"))
+        if self.mode != "ride":
+            raise NotImplementedError(f"fascia_mode={self.mode!r}: only 'ride' (the humanV2 default) is on the "
+                                      "HIP path")
+        w = parent.workload
+        if w is None or "fascia_mesh" not in w.info:
+            raise ValueError("use_fascia needs a fascia mesh on the workload")
+        fpos, ftets = w.info["fascia_mesh"]
+        self.scene = fascia_scene(cfg, fpos, ftets, parent.dt)
+        self.NV, self.NT = self.scene.nv, self.scene.nt
+        self.engine = (engine_cls(self.scene, device=device, **tuning) if engine_cls is XpbdEngine
+                       else engine_cls(self.scene))
+        glue = find_constraint(cfg, "fascia_glue") or {}
+        self.iters = int(glue.get("iterations", parent.fine_iters))
+        self.bind_max_dist = float(cfg.get("fascia_bind_max_dist", 0.05))
+        self.surf_tris = collision.surface_tris(parent.scene.tets)
+        parent.engine.set_fascia(self.engine, self.surf_tris, self.bind_max_dist, self.iters)
+
+    def positions_np(self):
+        return self.engine.positions_np()
+
+
 def multigrid_volume(scene) -> float:
     volumes, valid = builder.tet_rest_data(scene.pos0, scene.tets)
     return float(np.abs(volumes[valid]).sum())
@@ -64,9 +110,14 @@ def build_workload(cfg: dict) -> workloads.Workload:
     if name == "cantilever":
         return workloads.cantilever(seed)
     if name == "humanv2_scale":
-        return workloads.humanv2_scale(seed, target_tets=int(syn.get("muscle_tets", 468_000)),
-                                       fascia_tets=int(syn.get("fascia_tets", 67_000)),
-                                       sdf_res=int(syn.get("sdf_res", 128)))
+        w = workloads.humanv2_scale(seed, target_tets=int(syn.get("muscle_tets", 468_000)),
+                                    fascia_tets=int(syn.get("fascia_tets", 67_000)),
+                                    sdf_res=int(syn.get("sdf_res", 128)))
+        if cfg.get("use_fascia", False):
+            # the fascia tubes become the fascia sub-solver's own mesh
+            w, fpos, ftets = workloads.split_fascia(w)
+            w.info["fascia_mesh"] = (fpos, ftets)
+        return w
     if name == "batch":
         return workloads.batch(seed, n_inst=int(syn.get("instances", 512)), lo=int(syn.get("min_tets", 2000)),
                                hi=int(syn.get("max_tets", 40000)), sdf_res=int(syn.get("sdf_res", 64)))
@@ -210,6 +261,11 @@ class MultiGridSim:
         self.coarse = None
         if cfg.get("use_multigrid", False):
             self._build_coarse(engine_cls, device)
+        self.fascia = None
+        if cfg.get("use_fascia", False):
+            self.fascia = FasciaShell(self, engine_cls, device,
+                                      dict(strategy=strategy, patch_verts=patch_verts, block_tets=block_tets,
+                                           block_threads=block_threads))
         self.use_graph = True
         self._native_collider = None
         self.collision_solvers = []
@@ -329,6 +385,8 @@ class MultiGridSim:
                 e.iterate(1)
                 for fn in self.collision_solvers_iter:
                     fn(it)
+            if self.fascia is not None:
+                e.fascia_substep()     # Fascia.substep first in post_phase (mg.py:1365-1366)
             for fn in self._py_solvers:
                 fn()
             e.collide()            # bone push last (humanV2.py:616-622)

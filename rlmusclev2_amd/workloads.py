@@ -371,6 +371,39 @@ def humanv2_scale(seed: int = 0, n_bones=4, per_bone=10, target_tets=468_000,
                   sdf_res=sdf_res))
 
 
+def split_fascia(w: Workload):
+    """The humanV2-scale workload with its fascia tubes taken out as their own mesh
+    (the reference's fascia is a separate sub-solver with its own tet mesh,
+    fascia.py:45-60). Returns (muscle workload, fascia positions, fascia tets)."""
+    fas = np.asarray(w.is_fascia, bool)
+    keep = ~fas
+    tf = fas[w.tets].all(1)
+    if (fas[w.tets].any(1) & ~tf).any():
+        raise ValueError("fascia and muscle tets share vertices")
+    new = np.full(w.nv, -1, np.int64)
+    new[keep] = np.arange(int(keep.sum()))
+    fnew = np.full(w.nv, -1, np.int64)
+    fnew[fas] = np.arange(int(fas.sum()))
+    pin_keep = keep[w.pin_vids] if w.pin_vids is not None else None
+    info = dict(w.info)
+    info.pop("_pin_groups", None)
+    if info.get("muscle_id") is not None:
+        info["muscle_id"] = np.asarray(info["muscle_id"])[keep]
+    if info.get("s") is not None:
+        info["s"] = np.asarray(info["s"])[keep]
+    m = Workload(
+        w.name, w.pos0[keep], new[w.tets[~tf]].astype(np.int32), density=w.density,
+        v_fiber_dir=None if w.v_fiber_dir is None else w.v_fiber_dir[keep],
+        v_tendonmask=None if w.v_tendonmask is None else w.v_tendonmask[keep],
+        vertex_group=None if w.vertex_group is None else w.vertex_group[keep],
+        tet_instance=None if w.tet_instance is None else w.tet_instance[~tf],
+        is_fascia=np.zeros(int(keep.sum()), bool),
+        pin_vids=None if w.pin_vids is None else new[w.pin_vids[pin_keep]].astype(np.int32),
+        pin_group=None if w.pin_group is None else w.pin_group[pin_keep],
+        stopped=None if w.stopped is None else w.stopped[keep], bones=w.bones, info=info)
+    return m, w.pos0[fas].astype(np.float32), fnew[w.tets[tf]].astype(np.int32)
+
+
 def bake_bone_sdfs(w: Workload, res: int | None = None, margin: float = 0.03):
     """Capsule SDF per bone on a res^3 grid (own spacing per bone)."""
     res = int(res or w.info.get("sdf_res", 64))

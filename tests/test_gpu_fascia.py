@@ -1,0 +1,113 @@
+"""Fascia ride on the GPU against the oracle, through the C ABI (xpbd_set_fascia).
+
+The fascia shell is its own context (SNH, quasi-static), bound to the muscle
+surface at rest and moved every muscle substep (Fascia, fascia.py:45-572,
+fascia_mode "ride", the humanV2 default).
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from rlmusclev2_amd import sims, workloads
+
+pytestmark = pytest.mark.gpu
+CFG = Path(__file__).resolve().parents[1] / "rlmusclev2_amd" / "configs"
+
+
+def _cfg(iters=10, max_dist=0.02):
+    cfg = json.loads((CFG / "humanv2_fascia_ride.json").read_text())
+    cfg["fascia_bind_max_dist"] = max_dist
+    for c in cfg["constraints"]:
+        if c["name"] == "fascia_glue":
+            c["iterations"] = iters
+    cfg["synthetic"]["sdf_res"] = 32
+    return cfg
+
+
+def _small(seed=1, per_bone=4, tets=12000, ftets=3000):
+    w = workloads.humanv2_scale(seed, n_bones=2, per_bone=per_bone, target_tets=tets, fascia_tets=ftets, sdf_res=32)
+    w, fp, ft = workloads.split_fascia(w)
+    w.info["fascia_mesh"] = (fp, ft)
+    return w
+
+
+def _diag(p):
+    return float(np.linalg.norm(p.max(0) - p.min(0)))
+
+
+def test_bind_matches_oracle():
+    w = _small()
+    cfg = _cfg(max_dist=0.05)
+    sim = sims.MultiGridSim(cfg, workload=w, strategy=2, block_tets=256, block_threads=256)
+    face, wts, gap, nb = sim.engine.fascia_bind()
+    of, ow, og = oracle.fascia_bind(w.pos0, sim.fascia.surf_tris, w.info["fascia_mesh"][0], 0.05)
+    print("bound", nb, "of", len(face), flush=True)
+    np.testing.assert_array_equal(face, of)
+    assert nb == int((of >= 0).sum()) > 0
+    np.testing.assert_allclose(wts, ow, atol=2e-6)
+    np.testing.assert_allclose(gap, og, atol=2e-7)
+
+
+@pytest.mark.parametrize("strategy", [1, 2])
+def test_frames_with_fascia_match_oracle(strategy):
+    w = _small(2)
+    cfg = _cfg()
+    sim = sims.MultiGridSim(cfg, workload=w, strategy=strategy, block_tets=256, block_threads=256)
+    ref = sims.MultiGridSim(cfg, workload=w, engine_cls=oracle.OracleSim)
+    for f in range(1, 7):
+        print(f"frame {f}", flush=True)
+        sim.step_frame(f)
+        ref.step_frame(f)
+    fp = w.info["fascia_mesh"][0]
+    p, q = sim.fascia.positions_np(), ref.fascia.positions_np()
+    tol = 1e-4 * _diag(fp)
+    err = float(np.abs(p - q).max())
+    moved = float(np.abs(q - fp).max())
+    print("fascia max |dx|", err, "tol", tol, "moved", moved)
+    assert moved > 10 * tol
+    assert err <= tol
+    pm, qm = sim.positions_np(), ref.positions_np()
+    assert float(np.abs(pm - qm).max()) <= 1e-4 * _diag(w.pos0)
+
+
+def test_fascia_is_one_way_and_graph_equals_stream():
+    w = _small(3)
+    cfg = _cfg()
+    a = sims.MultiGridSim(cfg, workload=w, strategy=2, block_tets=256, block_threads=256)
+    b = sims.MultiGridSim(cfg, workload=w, strategy=2, block_tets=256, block_threads=256)
+    b.use_graph = False
+    c = sims.MultiGridSim(dict(cfg, use_fascia=False), workload=w, strategy=2, block_tets=256, block_threads=256)
+    for f in range(1, 5):
+        print(f"frame {f}", flush=True)
+        a.step_frame(f)
+        b.step_frame(f)
+        c.step_frame(f)
+    assert np.array_equal(a.positions_np(), c.positions_np())
+    assert np.array_equal(a.positions_np(), b.positions_np())
+    assert np.array_equal(a.fascia.positions_np(), b.fascia.positions_np())
+
+
+def test_full_size_bind_sample_and_frame():
+    """the humanV2-scale split: 468 k muscle tets, the 67 k-tet fascia tubes as the shell"""
+    cfg = json.loads((CFG / "humanv2_fascia_ride.json").read_text())
+    w = sims.build_workload(cfg)
+    sim = sims.MultiGridSim(cfg, workload=w, strategy=2, block_tets=256, block_threads=256)
+    fp = w.info["fascia_mesh"][0]
+    face, wts, gap, nb = sim.engine.fascia_bind()
+    print(f"full size: {w.nt} muscle tets, fascia {len(fp)} verts, {nb} bound", flush=True)
+    rng = np.random.default_rng(0)
+    idx = rng.choice(len(fp), 1500, replace=False)
+    of, ow, og = oracle.fascia_bind(w.pos0, sim.fascia.surf_tris, fp[idx], 0.05, nthreads=16)
+    np.testing.assert_array_equal(face[idx], of)
+    np.testing.assert_allclose(wts[idx], ow, atol=2e-6)
+    np.testing.assert_allclose(gap[idx], og, atol=2e-7)
+    sim.step_frame(1)
+    p = sim.fascia.positions_np()
+    assert np.isfinite(p).all()
+    # bound vertices sit on their ride targets after the frame (the last fascia iteration re-pins them)
+    tgt = oracle.fascia_ride_targets(sim.positions_np(), sim.fascia.surf_tris, face[face >= 0], wts[face >= 0],
+                                     gap[face >= 0])
+    np.testing.assert_allclose(p[face >= 0], tgt, atol=2e-6)
