@@ -18,6 +18,9 @@ plain @same residual".  Prints ONE JSON line.
   batch: network trained by the library's MFMA kernels on plain-solver frames,
   iterations to reach the plain 10-iteration residual, ms/frame of plain and
   warm frames with the audited per-frame fallback.
+* "fascia": the humanV2-scale scene with its fascia tubes as the fascia
+  sub-solver in ride mode (bound to the muscle surface, 20 Stable Neo-Hookean
+  shell iterations per substep), against the same muscles without it.
 * "bone_mesh": the humanV2-scale frame with the exact-mesh bone collider (a
   capsule skeleton of 97,536 triangles, BVH refit per frame, closest-point
   push in every substep) in place of the grid SDFs, with the push, refit and
@@ -63,6 +66,7 @@ WORKLOADS = {
     "c5": ("humanv2_c5.json", {}),
     "c5_batch": ("batch64_c5.json", {}),
     "mesh_collider": ("humanv2_mesh_collider.json", {}),
+    "fascia_ride": ("humanv2_fascia_ride.json", {}),
 }
 # the block shape the autotuner picks on the 512-instance batch (tuned once per mesh; re-tuning it in
 # every bench run would rebuild the 7 M-tet layout 13 times)
@@ -389,6 +393,33 @@ def bone_mesh_leg(args, tuning, grid_ms):
     }
 
 
+def fascia_leg(args, tuning):
+    """The humanV2-scale scene with its fascia tubes as the fascia sub-solver in ride mode
+    (the reference's humanV2 default, fascia.py:45-572): muscles 468 k tets, the shell
+    67 k tets of Stable Neo-Hookean bound to the muscle surface, 20 shell iterations per
+    substep; against the same muscles without the shell."""
+    from rlmusclev2_amd import sims
+    cfg = load_cfg("fascia_ride")
+    w = sims.build_workload(cfg)
+    sim, rec = run_frames(cfg, w, tuning, args.steps, args.warmup, "fascia_ride")
+    face, _, _, nb = sim.engine.fascia_bind()
+    fs = sim.fascia
+    sim.engine.close()
+    fs.engine.close()
+    sim0, rec0 = run_frames(dict(cfg, use_fascia=False), w, tuning, args.steps, args.warmup, "fascia_ride_off")
+    sim0.engine.close()
+    return {
+        "workload": f"{w.name} muscles {w.nt} tets / {w.nv} verts; fascia shell {fs.NT} tets / {fs.NV} verts, "
+                    f"{nb} bound to {len(fs.surf_tris)} muscle surface triangles (bind_max_dist "
+                    f"{fs.bind_max_dist} m), {fs.iters} shell iterations per substep",
+        "ms_per_frame": rec["ms_per_frame"], "ms_per_frame_without_fascia": rec0["ms_per_frame"],
+        "fascia_ms_per_frame": round(rec["ms_per_frame"] - rec0["ms_per_frame"], 4),
+        "fascia_tet_iterations_per_s": round(fs.NT * fs.iters * sim.nsubsteps /
+                                             max(1e-9, (rec["ms_per_frame"] - rec0["ms_per_frame"]) * 1e-3), 1),
+        "bound": nb, "finite": rec["finite"],
+    }
+
+
 def one_workload(args, name):
     """A single workload (its own line; the profiling commands use this form)."""
     from rlmusclev2_amd import autotune, sims
@@ -406,6 +437,10 @@ def one_workload(args, name):
         del w
         return {"metric": "ms/frame (humanV2-scale, exact-mesh bone collider)", "unit": "ms/frame",
                 "bone_mesh": bone_mesh_leg(args, tuning, None)}
+    if name == "fascia_ride":
+        del w
+        return {"metric": "ms/frame (humanV2-scale, fascia shell in ride mode)", "unit": "ms/frame",
+                "fascia": fascia_leg(args, tuning)}
     sim, rec = run_frames(cfg, w, tuning, args.steps, args.warmup, name)
     sim.engine.close()
     rec["autotune_us_per_iteration"] = tune_table
@@ -488,6 +523,8 @@ def main():
     del w4
     # ---- exact-mesh bone collider on the headline mesh -------------------
     out["bone_mesh"] = bone_mesh_leg(args, tuning, rec["ms_per_frame"])
+    # ---- the fascia sub-solver (ride) on the same scene --------------------
+    out["fascia"] = fascia_leg(args, tuning)
     # ---- c5: warm start vs plain at the same residual, with DCD ----------
     if not args.no_warm_start:
         out["c5"] = {}
