@@ -183,26 +183,56 @@ int32_t xpbd_set_coarse_level(xpbd_ctx *ctx, xpbd_ctx *coarse, const xpbd_coarse
  * caller order; only the pins' entries are read), per frame (_refresh_coarse_target,
  * mg.py:1013-1030) */
 int32_t xpbd_set_coarse_pin_targets(xpbd_ctx *ctx, int32_t n, const float *xyz);
-/* ---- fascia shell, ride mode (Fascia, src/muscle/fascia/fascia.py:45-572, fascia_mode "ride") ----
- * The fascia is another finalized context (its own tet mesh, SNH constraints, quasi-static
- * parameters).  xpbd_set_fascia binds it at rest (_build_muscle_bind, fascia.py:443-490): each
- * fascia vertex gets the closest point on this context's surface triangles among the faces nearer
- * than bind_max_dist (barycentric weights and the normal gap (x - closest)·n, n the triangle's unit
- * normal); at near-ties the lowest triangle index wins.  Bound vertices become the fascia's active
- * pins.  From then on every substep of this context (after its iterations, before its DCD and bone
- * push; MultiGridSim post_phase, mg.py:1362-1366) moves each bound fascia vertex to its barycentric
- * point on the current surface plus gap along the current unit normal, then runs one fascia substep
- * of iters iterations (ride, integrate, iters x (projection, apply, ride);
- * Fascia.build_program_ride, fascia.py:518-538).  The fascia never moves this context.  The fascia
- * runs on this context's stream; finalizing either context drops the link.  NULL fascia or desc
- * removes it. */
+/* ---- fascia shell (Fascia, src/muscle/fascia/fascia.py:45-572) ----------
+ * The fascia is another finalized context (its own tet mesh and constraints, quasi-static
+ * parameters) that runs one substep inside every substep of this context, after this context's
+ * iterations and before its DCD and bone push (MultiGridSim post_phase, mg.py:1362-1366), on this
+ * context's stream.  Finalizing either context drops the link; NULL fascia or desc removes it.
+ *
+ * mode 0, ride (fascia_mode "ride", the humanV2 default; _build_muscle_bind fascia.py:443-490,
+ * build_program_ride :518-538): xpbd_set_fascia binds each fascia rest vertex to the closest point
+ * of this context's surface triangles among the faces nearer than bind_max_dist (barycentric
+ * weights and the normal gap (x - closest)·n, n the triangle's unit normal; at near-ties the lowest
+ * triangle index) and makes the bound vertices the fascia's active pins.  Each substep moves them
+ * to their barycentric point on the current surface plus gap along the current normal, then runs
+ * the fascia's integrate and iters iterations, each re-applying the pins.  The fascia never moves
+ * this context.
+ *
+ * mode 1, squeeze (_build_coupling :354-424, build_program :547-565, advect_bone :540-545): the
+ * fascia follows the skeleton and a one-way glue pulls this context's surface vertices towards it.
+ * follow0 gives, per fascia vertex, the bone-driven world position of its follow point at rest
+ * (the reference's target_pos + rest_offset of the nearest muscle vertex); xpbd_set_fascia_follow
+ * gives it per frame and adds its change to every fascia vertex (advect_bone).  Each substep snaps
+ * the anchors to their follow point plus anchor_offset, integrates, then runs iters x (fascia
+ * iteration, glue): per pair, d = |x_f - x_m|, C = d - restlen, dl = -C / (w_m + compliance/dt^2)
+ * (the glue multiplier restarts at zero each iteration), x_m += omega * (-dl * w_m) (x_f - x_m)/d,
+ * skipped for d < 1e-9 or a denominator below 1e-12; w_m is this context's inverse mass. */
 typedef struct {
     int32_t nsurf;
-    const int32_t *surf_tris;   /* nsurf*3 vertex ids of this context's surface (caller order) */
-    float bind_max_dist;        /* fascia_bind_max_dist                                        */
-    int32_t iters;              /* fascia relax iterations per substep (fascia_glue.iterations) */
+    const int32_t *surf_tris;   /* ride: nsurf*3 vertex ids of this context's surface (caller order) */
+    float bind_max_dist;        /* ride: fascia_bind_max_dist                                       */
+    int32_t iters;              /* fascia iterations per substep (fascia_glue.iterations)           */
+    int32_t mode;               /* 0 ride, 1 squeeze                                                */
+    const float *follow0;       /* squeeze: fascia nv*3 follow positions at rest (fascia caller order) */
+    int32_t nglue;
+    const int32_t *glue_fascia; /* squeeze: nglue fascia vertices (fascia caller order)              */
+    const int32_t *glue_muscle; /* squeeze: nglue vertices of this context (caller order), distinct  */
+    const float *glue_restlen;  /* squeeze: nglue rest distances                                    */
+    const float *glue_compliance; /* squeeze: nglue 1/stiffness                                     */
+    float omega;                /* squeeze: the glue's relaxation factor (the parent's omega)       */
+    int32_t nanchor;
+    const int32_t *anchor;      /* squeeze: fascia vertices snapped each substep (caller order)    */
+    const float *anchor_offset; /* squeeze: nanchor*3, fascia rest minus its follow point at rest  */
 } xpbd_fascia_desc;
 int32_t xpbd_set_fascia(xpbd_ctx *ctx, xpbd_ctx *fascia, const xpbd_fascia_desc *desc);
+/* the rest matrices B = Dm^-1 times 1/scale (Fascia._apply_rest_shrink, fascia.py:341-352): the
+ * shape constraints see the rest shape scaled by `scale`; rest volumes and masses are kept.  After
+ * xpbd_set_mesh, before xpbd_finalize. */
+int32_t xpbd_scale_rest_shape(xpbd_ctx *ctx, float scale);
+/* squeeze: this frame's follow positions (fascia nv*3, fascia caller order); enqueues the
+ * advection of every fascia vertex by their change (advect_bone, fascia.py:540-545; per frame,
+ * before the substeps) */
+int32_t xpbd_set_fascia_follow(xpbd_ctx *ctx, const float *xyz);
 /* the fascia's substep alone on the current positions (Fascia.substep, fascia.py:567-572), for
  * callers that drive the phases one by one */
 int32_t xpbd_fascia_substep(xpbd_ctx *ctx);

@@ -243,6 +243,24 @@ def fascia_ride_targets(x, surf_tris, face, w, gap):
     return (cp + np.asarray(gap, np.float32)[:, None] * u).astype(np.float32)
 
 
+def glue_step(xf, xm, wj, restlen, alpha, omega):
+    """solve_glue_kernel (fascia.py:81-125) in FP32 with the multiplier at zero: the
+    kinematic fascia point xf pulls xm along the pair by omega * C * wj / (wj + alpha),
+    C = |xf - xm| - restlen; pairs with |xf - xm| < 1e-9 or wj + alpha < 1e-12 stay.
+    Returns the new xm."""
+    xf = np.asarray(xf, np.float32)
+    xm = np.asarray(xm, np.float32)
+    n = xf - xm
+    d = np.sqrt((n * n).sum(1)).astype(np.float32)
+    denom = (wj + alpha).astype(np.float32)
+    ok = (d >= np.float32(1e-9)) & (denom >= np.float32(1e-12))
+    dirn = n / np.where(ok, d, np.float32(1.0))[:, None]
+    C = d - np.asarray(restlen, np.float32)
+    dl = -C / np.where(ok, denom, np.float32(1.0))
+    m = ((-(np.float32(omega) * dl) * wj)[:, None] * dirn).astype(np.float32)
+    return np.where(ok[:, None], xm + m, xm).astype(np.float32)
+
+
 class OracleSim:
     """Same interface as XpbdEngine, computed by the C oracle on the CPU."""
 
@@ -255,6 +273,9 @@ class OracleSim:
         self.pos0 = np.ascontiguousarray(s.pos0, np.float32).reshape(-1, 3)
         self.tets = np.ascontiguousarray(s.tets, np.int32).reshape(-1, 4)
         self.rest_volume, self.rest_matrix, mass = precompute_rest(self.pos0, self.tets, s.density)
+        rs = float(np.float32(getattr(s, "rest_scale", 1.0)))
+        if abs(rs - 1.0) >= 1e-9:         # Fascia._apply_rest_shrink (fascia.py:341-352)
+            self.rest_matrix = (self.rest_matrix.astype(np.float64) * (1.0 / rs)).astype(np.float32)
         self.mass = np.ascontiguousarray(s.mass if s.mass is not None else mass, np.float32)
         self.pos = self.pos0.copy()
         self.pprev = self.pos0.copy()
@@ -447,32 +468,68 @@ class OracleSim:
     def set_coarse_pin_targets(self, xyz):
         self._mg_pin_goal = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
 
-    # fascia shell, ride mode (Fascia, fascia.py:245-547): another OracleSim, bound at rest
-    def set_fascia(self, fascia, surf_tris, bind_max_dist, iters):
+    # fascia shell (Fascia, fascia.py:245-572): another OracleSim, in ride or squeeze mode
+    def set_fascia(self, fascia, link):
         if fascia is None:
             self._fascia = None
             return
-        face, w, gap = fascia_bind(self.pos0, surf_tris, fascia.pos0, bind_max_dist, max(self.nthreads, 8))
+        iters = int(link["iters"])
+        if int(link.get("mode", 0)) == 1:
+            self._fascia = dict(mode=1, f=fascia, iters=iters,
+                                prev=np.asarray(link["follow0"], np.float32).reshape(-1, 3).copy(),
+                                gf=np.asarray(link["glue_fascia"], np.int64), gm=np.asarray(link["glue_muscle"], np.int64),
+                                rl=np.asarray(link["glue_restlen"], np.float32),
+                                cm=np.asarray(link["glue_compliance"], np.float32), omega=np.float32(link["omega"]),
+                                anc=np.asarray(link["anchor"], np.int64),
+                                off=np.asarray(link["anchor_offset"], np.float32).reshape(-1, 3))
+            return
+        surf_tris = link["surf_tris"]
+        face, w, gap = fascia_bind(self.pos0, surf_tris, fascia.pos0, link["bind_max_dist"], max(self.nthreads, 8))
         bound = np.nonzero(face >= 0)[0].astype(np.int32)
         tris = np.asarray(surf_tris, np.int32).reshape(-1, 3)
         fascia.pin_vids = bound
         fascia.pin_active = np.ones(len(bound), np.int32)
         fascia.pin_target = fascia_ride_targets(self.pos0, tris, face[bound], w[bound], gap[bound])
         fascia.pin_zero = np.zeros_like(fascia.pin_target)
-        self._fascia = (fascia, tris, bound, face[bound], w[bound], gap[bound], int(iters))
+        self._fascia = dict(mode=0, f=fascia, tris=tris, bound=bound, face=face[bound], w=w[bound], gap=gap[bound],
+                            iters=iters)
         self.fascia_bind = (face, w, gap)
+
+    def set_fascia_follow(self, xyz):
+        """squeeze: advect_bone (fascia.py:540-545)"""
+        fz = self._fascia
+        cur = np.asarray(xyz, np.float32).reshape(-1, 3)
+        fz["f"].pos[:] = fz["f"].pos + (cur - fz["prev"])
+        fz["prev"] = cur.copy()
+
+    def _inv_mass(self):
+        w = np.where(self.mass > 0, (np.float32(1.0) / np.where(self.mass > 0, self.mass, 1)).astype(np.float32),
+                     np.float32(0.0))
+        return np.where(self.stopped != 0, np.float32(0.0), w).astype(np.float32)
 
     def fascia_substep(self):
         if getattr(self, "_fascia", None) is not None:
             self._fascia_substep()
 
     def _fascia_substep(self):
-        f, tris, bound, face, w, gap, iters = self._fascia
-        if len(bound):
-            g = fascia_ride_targets(self.pos, tris, face, w, gap)
-            f.pos[bound] = g            # the ride snaps before the fascia's integrate
-            f.pin_target = g
-        f.native_substep(iters)
+        fz = self._fascia
+        f = fz["f"]
+        if fz["mode"] == 0:
+            if len(fz["bound"]):
+                g = fascia_ride_targets(self.pos, fz["tris"], fz["face"], fz["w"], fz["gap"])
+                f.pos[fz["bound"]] = g          # the ride snaps before the fascia's integrate
+                f.pin_target = g
+            f.native_substep(fz["iters"])
+            return
+        # squeeze (Fascia.build_program, fascia.py:547-565)
+        f.pos[fz["anc"]] = fz["prev"][fz["anc"]] + fz["off"]
+        f.integrate()
+        wj = self._inv_mass()[fz["gm"]]
+        inv_dt2 = np.float32(1.0) / np.float32(np.float32(f.params.dt) * np.float32(f.params.dt))
+        alpha = fz["cm"] * inv_dt2
+        for _ in range(fz["iters"]):
+            f.iterate(1)
+            self.pos[fz["gm"]] = glue_step(f.pos[fz["gf"]], self.pos[fz["gm"]], wj, fz["rl"], alpha, fz["omega"])
 
     def native_substep(self, iters):
         self.integrate()

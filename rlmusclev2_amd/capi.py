@@ -66,7 +66,10 @@ class XpbdBoneMeshDesc(C.Structure):
 
 class XpbdFasciaDesc(C.Structure):
     _fields_ = [("nsurf", C.c_int32), ("surf_tris", C.c_void_p), ("bind_max_dist", C.c_float),
-                ("iters", C.c_int32)]
+                ("iters", C.c_int32), ("mode", C.c_int32), ("follow0", C.c_void_p), ("nglue", C.c_int32),
+                ("glue_fascia", C.c_void_p), ("glue_muscle", C.c_void_p), ("glue_restlen", C.c_void_p),
+                ("glue_compliance", C.c_void_p), ("omega", C.c_float), ("nanchor", C.c_int32),
+                ("anchor", C.c_void_p), ("anchor_offset", C.c_void_p)]
 
 
 class XpbdMlpDev(C.Structure):
@@ -135,6 +138,8 @@ SIGNATURES = {
     "xpbd_set_fascia": [_P, _P, C.POINTER(XpbdFasciaDesc)],
     "xpbd_get_fascia_bind": [_P, _P, _P, _P, _P],
     "xpbd_fascia_substep": [_P],
+    "xpbd_set_fascia_follow": [_P, _P],
+    "xpbd_scale_rest_shape": [_P, _F],
     "xpbd_dcd_solve": [_P, _I],
     "xpbd_dcd_pairs": [_P, _P, C.c_int64, _P],
     "xpbd_dcd_stats": [_P, _P, _P, _P],

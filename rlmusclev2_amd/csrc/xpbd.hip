@@ -1348,6 +1348,58 @@ __global__ void k_fascia_ride(int n, const float4 *Xm, const int32_t *vid, const
     PINf[k] = make_float4(g.x, g.y, g.z, 1.0f);
 }
 
+// fascia squeeze: advect_bone (fascia.py:45-61), once per frame: every fascia vertex moves by
+// the change of its follow point since the last frame
+__global__ void k_fascia_advect(int n, const float4 *ff, float4 *prev, float4 *Xf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 c = ff[i], p = prev[i];
+    float4 x = Xf[i];
+    x.x = x.x + (c.x - p.x);
+    x.y = x.y + (c.y - p.y);
+    x.z = x.z + (c.z - p.z);
+    Xf[i] = x;
+    prev[i] = c;
+}
+
+// fascia squeeze: fascia_anchor_bone_kernel (fascia.py:64-78), the anchors snapped to their
+// follow point plus the rest offset at the start of the fascia substep
+__global__ void k_fascia_anchor(int n, const int32_t *anc, const float4 *off, const float4 *ff, float4 *Xf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int k = anc[i];
+    const float4 c = ff[k], o = off[i];
+    const float4 x = Xf[k];
+    Xf[k] = make_float4(c.x + o.x, c.y + o.y, c.z + o.z, x.w);
+}
+
+// fascia squeeze: solve_glue_kernel (fascia.py:81-125) with the multiplier at zero (the reference
+// zeroes it before every launch); the fascia side is kinematic, the muscle vertex moves by
+// omega * dl * w_m along the pair; every muscle vertex is in one pair at most
+__global__ void k_fascia_glue(int n, const int4 *pairs, const float4 *Xf, float4 *Xm, float inv_dt2, float omega) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int4 q = pairs[c];
+    const float4 xf = Xf[q.x];
+    float4 xm = Xm[q.y];
+    const V3 nn = V3{xf.x - xm.x, xf.y - xm.y, xf.z - xm.z};
+    const float d = sqrtf(dot(nn, nn));
+    if (d < 1.0e-9f) return;
+    const V3 dirn = nn / d;
+    const float C = d - __int_as_float(q.z);
+    const float wj = xm.w;
+    const float alpha = __int_as_float(q.w) * inv_dt2;
+    const float denom = wj + alpha;
+    if (denom < 1.0e-12f) return;
+    const float dl = -C / denom;
+    const float f = omega * dl;
+    const V3 m = (-f * wj) * dirn;
+    xm.x = xm.x + m.x;
+    xm.y = xm.y + m.y;
+    xm.z = xm.z + m.z;
+    Xm[q.y] = xm;
+}
+
 __global__ void k_pins(int nv, float4 *X, const float4 *pin) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nv) return;
@@ -2678,6 +2730,14 @@ struct xpbd_ctx {
     float4 *d_fb_w = nullptr;         // barycentric weights of a, b, c and the normal gap
     std::vector<int32_t> fb_face, fb_ok;   // per fascia vertex (caller order): surface triangle, bound
     std::vector<float> fb_w, fb_gap;       // 3 weights, gap
+    // fascia squeeze: follow positions (fascia library order), the previous frame's, glue and anchors
+    int fascia_mode = 0, fascia_nglue = 0, fascia_nanchor = 0;
+    float fascia_omega = 1.f;
+    float4 *d_ff = nullptr, *d_ff_prev = nullptr;
+    int4 *d_fglue = nullptr;               // fascia vertex, muscle vertex (library orders), restlen, compliance bits
+    int32_t *d_fanc = nullptr;             // anchor: fascia library index
+    float4 *d_fanc_off = nullptr;
+    HostStage st_ff;
     int mg_iters = 0;
     bool mg_pidx_set = false, mg_kinematic = false;
     int mg_nblk = 0;
@@ -2826,6 +2886,9 @@ void free_device(xpbd_ctx *c) {
     c->d_fb_vid = nullptr;
     c->d_fb_tri = nullptr;
     c->d_fb_w = nullptr;
+    c->d_ff = c->d_ff_prev = c->d_fanc_off = nullptr;
+    c->d_fglue = nullptr;
+    c->d_fanc = nullptr;
     c->SFEAT = nullptr;
     c->sfeat_ok = false;
     c->pin_cap = 0;
@@ -3556,10 +3619,35 @@ int enqueue_substep(xpbd_ctx *c, int iters, bool dcd_rebuild, int sub = 0, bool 
 
 // the fascia shell's substep (Fascia.build_program_ride, fascia.py:518-538): ride, integrate,
 // iters x (projection, apply, ride) -- the ride after each iteration is the pin overwrite
-int launch_fascia(xpbd_ctx *c, const float4 *Xm) {
+int launch_fascia(xpbd_ctx *c, float4 *Xm) {
     xpbd_ctx *f = c->fascia;
     if (f->gen != c->fascia_gen || !f->ready)
         return fail(c, XPBD_ENOTREADY, "the fascia context was finalized again; call xpbd_set_fascia again");
+    if (c->fascia_mode == 1) {
+        // squeeze (Fascia.build_program, fascia.py:547-565): anchors, integrate, iters x (fascia
+        // iteration, glue on this context's positions)
+        auto g = [](int n) { return dim3((unsigned)((n + 255) / 256)); };
+        if (c->fascia_nanchor > 0)
+            hipLaunchKernelGGL(k_fascia_anchor, g(c->fascia_nanchor), dim3(256), 0, c->stream, c->fascia_nanchor,
+                               (const int32_t *)c->d_fanc, (const float4 *)c->d_fanc_off, (const float4 *)c->d_ff,
+                               f->X[0]);
+        HIPCHK(c, hipGetLastError());
+        int r = launch_integrate(f);
+        if (r) return fail(c, r, std::string("fascia integrate: ") + f->err);
+        const float inv_dt2 = 1.0f / (f->sp.dt * f->sp.dt);
+        for (int it = 0; it < c->fascia_iters; ++it) {
+            if ((r = launch_iter(f, f->X[it & 1], f->X[(it + 1) & 1], MODE_FUSED, 0)))
+                return fail(c, r, std::string("fascia iteration: ") + f->err);
+            if (c->fascia_nglue > 0)
+                hipLaunchKernelGGL(k_fascia_glue, g(c->fascia_nglue), dim3(256), 0, c->stream, c->fascia_nglue,
+                                   (const int4 *)c->d_fglue, (const float4 *)f->X[(it + 1) & 1], Xm, inv_dt2,
+                                   c->fascia_omega);
+            HIPCHK(c, hipGetLastError());
+        }
+        if (c->fascia_iters & 1)
+            HIPCHK(c, hipMemcpyAsync(f->X[0], f->X[1], f->nv * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+        return XPBD_OK;
+    }
     if (c->fascia_nbound > 0) {
         hipLaunchKernelGGL(k_fascia_ride, dim3((unsigned)((c->fascia_nbound + 255) / 256)), dim3(256), 0, c->stream,
                            c->fascia_nbound, Xm, (const int32_t *)c->d_fb_vid, (const int4 *)c->d_fb_tri,
@@ -3655,6 +3743,7 @@ void xpbd_destroy(xpbd_ctx *c) {
     free_stage(c->st_gact);
     free_stage(c->st_mgpin);
     free_stage(c->bm.st);
+    free_stage(c->st_ff);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
 }
@@ -3702,6 +3791,16 @@ int32_t xpbd_get_rest(xpbd_ctx *c, float *rv, float *rm, float *mass) {
     if (rv) std::memcpy(rv, c->rest_volume.data(), c->rest_volume.size() * sizeof(float));
     if (rm) std::memcpy(rm, c->rest_matrix.data(), c->rest_matrix.size() * sizeof(float));
     if (mass) std::memcpy(mass, c->mass.data(), c->mass.size() * sizeof(float));
+    return XPBD_OK;
+}
+
+int32_t xpbd_scale_rest_shape(xpbd_ctx *c, float scale) {
+    if (!c || c->nt == 0) return fail(c, XPBD_EINVAL, "no mesh");
+    if (c->ready) return fail(c, XPBD_EINVAL, "scale the rest shape before xpbd_finalize");
+    if (!(scale > 0.0f) || !std::isfinite(scale)) return fail(c, XPBD_EINVAL, "rest scale must be > 0");
+    if (std::fabs((double)scale - 1.0) < 1.0e-9) return XPBD_OK;
+    const double k = 1.0 / (double)scale;
+    for (float &b : c->rest_matrix) b = (float)((double)b * k);
     return XPBD_OK;
 }
 
@@ -4743,6 +4842,68 @@ D3 tri_closest_d(D3 p, D3 a, D3 b, D3 c) {
     return d3axpy(d3axpy(a, vb / den, ab), vc / den, ac);
 }
 
+// squeeze: follow positions, glue pairs and anchors to the device (library orders)
+int set_fascia_squeeze(xpbd_ctx *c, xpbd_ctx *f, const xpbd_fascia_desc *d) {
+    const int nf = f->nv;
+    if (!d->follow0) return fail(c, XPBD_EINVAL, "squeeze needs follow0");
+    if (d->nglue < 0 || d->nanchor < 0 || (d->nglue > 0 && (!d->glue_fascia || !d->glue_muscle ||
+                                                             !d->glue_restlen || !d->glue_compliance)) ||
+        (d->nanchor > 0 && (!d->anchor || !d->anchor_offset)))
+        return fail(c, XPBD_EINVAL, "bad glue or anchor arrays");
+    std::vector<char> seen(c->nv, 0);
+    std::vector<int4> glue(d->nglue);
+    for (int k = 0; k < d->nglue; ++k) {
+        const int a = d->glue_fascia[k], m = d->glue_muscle[k];
+        if (a < 0 || a >= nf || m < 0 || m >= c->nv) return fail(c, XPBD_EINVAL, "glue vertex out of range");
+        if (seen[m]) return fail(c, XPBD_EINVAL, "a muscle vertex is in two glue pairs");
+        seen[m] = 1;
+        float rl = d->glue_restlen[k], cm = d->glue_compliance[k];
+        int rli, cmi;
+        std::memcpy(&rli, &rl, 4);
+        std::memcpy(&cmi, &cm, 4);
+        glue[k] = make_int4(f->L.iperm[a], c->L.iperm[m], rli, cmi);
+    }
+    std::vector<int32_t> anc(d->nanchor);
+    std::vector<float4> aoff(d->nanchor);
+    for (int k = 0; k < d->nanchor; ++k) {
+        if (d->anchor[k] < 0 || d->anchor[k] >= nf) return fail(c, XPBD_EINVAL, "anchor vertex out of range");
+        anc[k] = f->L.iperm[d->anchor[k]];
+        aoff[k] = make_float4(d->anchor_offset[3 * k], d->anchor_offset[3 * k + 1], d->anchor_offset[3 * k + 2], 0.f);
+    }
+    std::vector<float4> ff(nf);
+    for (int k = 0; k < nf; ++k) {
+        const int o = f->L.perm[k];
+        ff[k] = make_float4(d->follow0[3 * o], d->follow0[3 * o + 1], d->follow0[3 * o + 2], 0.f);
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(c, c->d_ff);
+    dfree(c, c->d_ff_prev);
+    dfree(c, c->d_fglue);
+    dfree(c, c->d_fanc);
+    dfree(c, c->d_fanc_off);
+    int r;
+    if ((r = dupload(c, &c->d_ff, ff.data(), ff.size())) || (r = dupload(c, &c->d_ff_prev, ff.data(), ff.size())) ||
+        (r = dupload(c, &c->d_fglue, glue.data(), glue.size())) || (r = dupload(c, &c->d_fanc, anc.data(), anc.size())) ||
+        (r = dupload(c, &c->d_fanc_off, aoff.data(), aoff.size())))
+        return r;
+    c->fascia = f;
+    c->fascia_gen = f->gen;
+    c->fascia_iters = d->iters;
+    c->fascia_mode = 1;
+    c->fascia_nbound = 0;
+    c->fascia_nglue = d->nglue;
+    c->fascia_nanchor = d->nanchor;
+    c->fascia_omega = d->omega;
+    c->fb_face.assign(nf, -1);
+    c->fb_ok.assign(nf, 0);
+    c->fb_w.assign(3 * (size_t)nf, 0.f);
+    c->fb_gap.assign(nf, 0.f);
+    f->stream = c->stream;
+    invalidate_graph(f);
+    invalidate_graph(c);
+    return XPBD_OK;
+}
+
 }  // namespace
 
 int32_t xpbd_set_fascia(xpbd_ctx *c, xpbd_ctx *f, const xpbd_fascia_desc *d) {
@@ -4759,8 +4920,11 @@ int32_t xpbd_set_fascia(xpbd_ctx *c, xpbd_ctx *f, const xpbd_fascia_desc *d) {
     if (f == c) return fail(c, XPBD_EINVAL, "a context cannot be its own fascia");
     if (!f->ready) return fail(c, XPBD_ENOTREADY, "finalize the fascia context first");
     if (f->device != c->device) return fail(c, XPBD_EINVAL, "the fascia context is on another device");
+    if (d->iters < 0) return fail(c, XPBD_EINVAL, "negative fascia iterations");
+    if (d->mode == 1) return set_fascia_squeeze(c, f, d);
+    if (d->mode != 0) return fail(c, XPBD_EINVAL, "fascia mode must be 0 (ride) or 1 (squeeze)");
     if (d->nsurf < 1 || !d->surf_tris) return fail(c, XPBD_EINVAL, "no surface triangles");
-    if (!(d->bind_max_dist > 0.0f) || d->iters < 0) return fail(c, XPBD_EINVAL, "bad bind_max_dist or iters");
+    if (!(d->bind_max_dist > 0.0f)) return fail(c, XPBD_EINVAL, "bad bind_max_dist");
     for (int64_t i = 0; i < 3 * (int64_t)d->nsurf; ++i)
         if (d->surf_tris[i] < 0 || d->surf_tris[i] >= c->nv)
             return fail(c, XPBD_EINVAL, "surface triangle vertex out of range");
@@ -4880,9 +5044,33 @@ int32_t xpbd_set_fascia(xpbd_ctx *c, xpbd_ctx *f, const xpbd_fascia_desc *d) {
     c->fascia_gen = f->gen;
     c->fascia_iters = d->iters;
     c->fascia_nbound = nb;
+    c->fascia_mode = 0;
     f->stream = c->stream;
     invalidate_graph(f);
     invalidate_graph(c);
+    return XPBD_OK;
+}
+
+int32_t xpbd_set_fascia_follow(xpbd_ctx *c, const float *xyz) {
+    if (!c || !c->fascia || c->fascia_mode != 1) return fail(c, XPBD_EINVAL, "no fascia in squeeze mode");
+    if (!xyz) return fail(c, XPBD_EINVAL, "null follow positions");
+    xpbd_ctx *f = c->fascia;
+    if (f->gen != c->fascia_gen || !f->ready)
+        return fail(c, XPBD_ENOTREADY, "the fascia context was finalized again; call xpbd_set_fascia again");
+    const int n = f->nv;
+    int slot;
+    void *h;
+    int r = stage_acquire(c, c->st_ff, (size_t)n * sizeof(float4), slot, h);
+    if (r) return r;
+    float4 *q = (float4 *)h;
+    for (int k = 0; k < n; ++k) {
+        const int o = f->L.perm[k];
+        q[k] = make_float4(xyz[3 * o], xyz[3 * o + 1], xyz[3 * o + 2], 0.f);
+    }
+    if ((r = stage_submit(c, c->st_ff, slot, c->d_ff, (size_t)n * sizeof(float4)))) return r;
+    hipLaunchKernelGGL(k_fascia_advect, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n,
+                       (const float4 *)c->d_ff, c->d_ff_prev, f->X[0]);
+    HIPCHK(c, hipGetLastError());
     return XPBD_OK;
 }
 

@@ -86,6 +86,8 @@ class XpbdEngine:
                 c.call("xpbd_set_sdf_groups", len(k["goff"]) - 1, ptr(k["goff"]), ptr(k["vgroup"]))
         if s.dcd is not None:
             self._set_dcd(s.dcd)
+        if s.rest_scale != 1.0:
+            c.call("xpbd_scale_rest_shape", C.c_float(s.rest_scale))
         if s.bone_mesh is not None:
             self.set_bone_mesh(s.bone_mesh)
         tun = capi.XpbdTuning()
@@ -159,21 +161,41 @@ class XpbdEngine:
         return depth, int(cnt.value), float(mx.value)
 
     # fascia shell, ride mode (Fascia, src/muscle/fascia/fascia.py:45-572) -----------
-    def set_fascia(self, fascia, surf_tris, bind_max_dist: float, iters: int) -> None:
-        """Bind the finalized fascia engine to this mesh's surface; None unlinks."""
+    def set_fascia(self, fascia, link) -> None:
+        """Link the finalized fascia engine (None unlinks). ``link``: dict with iters and
+        mode 0 (ride: surf_tris, bind_max_dist) or 1 (squeeze: follow0, glue_fascia,
+        glue_muscle, glue_restlen, glue_compliance, omega, anchor, anchor_offset)."""
         if fascia is None:
             self.ctx.call("xpbd_set_fascia", None, None)
             self._fascia = None
             return
         k = self._keep
-        k["fascia_tris"] = i32(surf_tris, (-1, 3))
         d = capi.XpbdFasciaDesc()
-        d.nsurf = k["fascia_tris"].shape[0]
-        d.surf_tris = ptr(k["fascia_tris"])
-        d.bind_max_dist = float(bind_max_dist)
-        d.iters = int(iters)
+        d.iters = int(link["iters"])
+        d.mode = int(link.get("mode", 0))
+        if d.mode == 0:
+            k["fascia_tris"] = i32(link["surf_tris"], (-1, 3))
+            d.nsurf = k["fascia_tris"].shape[0]
+            d.surf_tris = ptr(k["fascia_tris"])
+            d.bind_max_dist = float(link["bind_max_dist"])
+        else:
+            arr = dict(follow0=f32(link["follow0"], (-1, 3)), glue_fascia=i32(link["glue_fascia"]),
+                       glue_muscle=i32(link["glue_muscle"]), glue_restlen=f32(link["glue_restlen"]),
+                       glue_compliance=f32(link["glue_compliance"]), anchor=i32(link["anchor"]),
+                       anchor_offset=f32(link["anchor_offset"], (-1, 3)))
+            k["fascia_squeeze"] = arr
+            for name, a in arr.items():
+                setattr(d, name, ptr(a))
+            d.nglue = len(arr["glue_fascia"])
+            d.nanchor = len(arr["anchor"])
+            d.omega = float(link["omega"])
         self.ctx.call("xpbd_set_fascia", fascia.ctx.h, C.byref(d))
         self._fascia = fascia
+
+    def set_fascia_follow(self, xyz) -> None:
+        """squeeze: this frame's follow positions; advects the shell by their change"""
+        a = f32(xyz, (-1, 3))
+        self.ctx.call("xpbd_set_fascia_follow", ptr(a))
 
     def fascia_substep(self) -> None:
         self.ctx.call("xpbd_fascia_substep")

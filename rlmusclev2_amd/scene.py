@@ -87,6 +87,7 @@ class Scene:
     bone_mesh: BoneMeshCollider | None = None
     dcd: object | None = None                  # collision.DcdSpec
     attach_vids: np.ndarray | None = None      # vertex of each ATTACH constraint, table order
+    rest_scale: float = 1.0                    # shape constraints see the rest shape scaled by this
     meta: dict = field(default_factory=dict)
 
     @property

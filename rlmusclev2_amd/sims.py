@@ -50,13 +50,15 @@ def mg_constraint_configs(cfg: dict) -> list:
     return out
 
 
-def fascia_scene(cfg: dict, pos, tets, dt: float) -> Scene:
-    """The fascia shell in ride mode (Fascia.__init__, fascia.py:245-330): Stable
-    Neo-Hookean from the 'fascia_snh' entry, density from fascia_total_mass (else the
-    muscle's total_mass) over the shell's rest volume, quasi-static, no gravity."""
-    snh = find_constraint(cfg, "fascia_snh") or {}
-    ccfg = [dict(type="snh", name="fascia_snh", mu=float(snh.get("mu", 1.0e3)), lam=float(snh.get("lam", 1.0e8)),
-                 dampingratio=float(snh.get("dampingratio", 0.0)))]
+def fascia_scene(cfg: dict, pos, tets, dt: float, ccfg=None) -> Scene:
+    """The fascia shell (Fascia.__init__, fascia.py:245-330): in ride mode Stable
+    Neo-Hookean from the 'fascia_snh' entry (or the given constraint set), density from
+    fascia_total_mass (else the muscle's total_mass) over the shell's rest volume,
+    quasi-static, no gravity."""
+    if ccfg is None:
+        snh = find_constraint(cfg, "fascia_snh") or {}
+        ccfg = [dict(type="snh", name="fascia_snh", mu=float(snh.get("mu", 1.0e3)),
+                     lam=float(snh.get("lam", 1.0e8)), dampingratio=float(snh.get("dampingratio", 0.0)))]
     pos = np.asarray(pos, np.float32)
     tets = np.asarray(tets, np.int32)
     cons = builder.build_from_configs(pos, tets, ccfg)
@@ -68,29 +70,83 @@ def fascia_scene(cfg: dict, pos, tets, dt: float) -> Scene:
                  meta=dict(workload="fascia", constraint_configs=ccfg))
 
 
+def fascia_scene_squeeze(cfg: dict, pos, tets, dt: float) -> Scene:
+    """The fascia shell in squeeze mode (Fascia.__init__ and _apply_rest_shrink,
+    fascia.py:245-352): ARAP-SVD ('fascia_arap': stiffness, dampingratio, rest_scale)
+    plus volume ('fascia_volume'), the ARAP rest shape scaled by rest_scale."""
+    arap = find_constraint(cfg, "fascia_arap") or {}
+    vol = find_constraint(cfg, "fascia_volume") or {}
+    k = float(arap.get("stiffness", -1.0))
+    if k < 0:
+        raise ValueError("fascia_arap.stiffness < 0 asks for the fascia geometry's solidshapestiffness attribute, "
+                         "which a synthetic shell does not carry: give the stiffness")
+    ccfg = [dict(type="tetarapsvd", name="fascia_arap", stiffness=k, dampingratio=float(arap.get("dampingratio", 0.0))),
+            dict(type="volume", name="fascia_volume", stiffness=float(vol.get("stiffness", 1.0e10)),
+                 dampingratio=float(vol.get("dampingratio", 0.0)))]
+    sc = fascia_scene(cfg, pos, tets, dt, ccfg)
+    sc.rest_scale = float(arap.get("rest_scale", 0.95))
+    return sc
+
+
 class FasciaShell:
-    """Fascia sub-solver in ride mode (Fascia, fascia.py:217-572): its own engine on
-    its own mesh, bound to the muscle surface; its substep runs inside the muscle's."""
+    """Fascia sub-solver (Fascia, fascia.py:217-572): its own engine on its own mesh;
+    its substep runs inside the muscle's.  "ride": bound to the muscle surface, Stable
+    Neo-Hookean; "squeeze": follows the skeleton (advected per frame, anchors snapped per
+    substep), ARAP + volume with a shrunk rest shape, glued one way to the muscle surface."""
 
     def __init__(self, parent, engine_cls, device, tuning):
         cfg = parent.cfg
         self.mode = str(cfg.get("fascia_mode", "squeeze"))
-        if self.mode != "ride":
-            raise NotImplementedError(f"fascia_mode={self.mode!r}: only 'ride' (the humanV2 default) is on the "
-                                      "HIP path")
+        if self.mode not in ("ride", "squeeze"):
+            raise ValueError(f"fascia_mode={self.mode!r}: 'ride' or 'squeeze'")
         w = parent.workload
         if w is None or "fascia_mesh" not in w.info:
             raise ValueError("use_fascia needs a fascia mesh on the workload")
         fpos, ftets = w.info["fascia_mesh"]
-        self.scene = fascia_scene(cfg, fpos, ftets, parent.dt)
+        self.scene = (fascia_scene(cfg, fpos, ftets, parent.dt) if self.mode == "ride"
+                      else fascia_scene_squeeze(cfg, fpos, ftets, parent.dt))
         self.NV, self.NT = self.scene.nv, self.scene.nt
         self.engine = (engine_cls(self.scene, device=device, **tuning) if engine_cls is XpbdEngine
                        else engine_cls(self.scene))
         glue = find_constraint(cfg, "fascia_glue") or {}
         self.iters = int(glue.get("iterations", parent.fine_iters))
-        self.bind_max_dist = float(cfg.get("fascia_bind_max_dist", 0.05))
         self.surf_tris = collision.surface_tris(parent.scene.tets)
-        parent.engine.set_fascia(self.engine, self.surf_tris, self.bind_max_dist, self.iters)
+        self.bind_max_dist = float(cfg.get("fascia_bind_max_dist", 0.05))
+        if self.mode == "ride":
+            link = dict(mode=0, iters=self.iters, surf_tris=self.surf_tris, bind_max_dist=self.bind_max_dist)
+        else:
+            link = self._squeeze_link(parent, np.asarray(fpos, np.float32), glue)
+        parent.engine.set_fascia(self.engine, link)
+
+    def _squeeze_link(self, parent, fpos, glue):
+        """_build_coupling / _pair_surface (fascia.py:354-441): every muscle surface vertex
+        glued to its nearest fascia vertex at their rest distance; every fascia vertex
+        follows the bone-driven position of its nearest muscle vertex; anchors: a uniform
+        subsample of fascia_anchor.count vertices (no 'rigid' group on a synthetic shell)."""
+        from scipy.spatial import cKDTree
+        cfg = parent.cfg
+        rest_m = np.asarray(parent.scene.pos0, np.float32)
+        surf = np.unique(self.surf_tris).astype(np.int64)
+        d, floc = cKDTree(fpos).query(rest_m[surf], k=1)
+        if glue.get("stiffness") is None:
+            raise ValueError("fascia_glue.stiffness is needed: a synthetic shell has no tissuetomusclestiffness "
+                             "attribute")
+        stiff = np.full(len(surf), float(glue["stiffness"]) * float(glue.get("stiffness_scale", 0.001)), np.float32)
+        _, fm = cKDTree(rest_m).query(fpos, k=1)
+        self.follow_vids = fm.astype(np.int32)
+        follow0 = rest_m[self.follow_vids]
+        n_anc = max(1, int((find_constraint(cfg, "fascia_anchor") or {}).get("count", 256)))
+        rid = np.arange(0, len(fpos), max(1, len(fpos) // n_anc), dtype=np.int32)
+        self.anchor = rid
+        return dict(mode=1, iters=self.iters, follow0=follow0, glue_fascia=floc.astype(np.int32),
+                    glue_muscle=surf.astype(np.int32), glue_restlen=d.astype(np.float32),
+                    glue_compliance=(1.0 / np.maximum(stiff, 1e-30)).astype(np.float32),
+                    omega=float(cfg.get("omega", 1.0)), anchor=rid,
+                    anchor_offset=(fpos[rid] - follow0[rid]).astype(np.float32))
+
+    def follow(self, w, frame):
+        """squeeze: the bone-driven positions of the followed muscle vertices at frame"""
+        return workloads.follow_targets(w, frame, self.follow_vids)
 
     def positions_np(self):
         return self.engine.positions_np()
@@ -331,6 +387,9 @@ class MultiGridSim:
             self.engine.set_bone_mesh_points(workloads.bone_mesh_points(w, frame, **_mesh_res(coll)))
         if self.coarse is not None and self.coarse.pin_rule == 1 and w.bones:
             self.engine.set_coarse_pin_targets(multigrid.coarse_pin_targets(self.coarse, w, frame))
+        if self.fascia is not None and self.fascia.mode == "squeeze" and w.bones:
+            # Fascia.advect_bone once per frame, before the substeps (mg.py:1413-1416)
+            self.engine.set_fascia_follow(self.fascia.follow(w, frame))
         self._frame = frame
 
     def set_fiber_activation(self, acti) -> None:

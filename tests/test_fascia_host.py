@@ -100,8 +100,49 @@ def test_oracle_ride_is_one_way_and_moves_the_shell():
     assert np.isfinite(a.fascia.positions_np()).all()
 
 
-def test_squeeze_mode_is_not_on_the_hip_path():
-    w = _small_workload()
-    cfg = dict(_cfg_small(), fascia_mode="squeeze")
-    with pytest.raises(NotImplementedError):
-        sims.MultiGridSim(cfg, workload=w, engine_cls=oracle.OracleSim)
+def test_glue_step_closed_form():
+    """solve_glue_kernel with a zero multiplier: the muscle point moves omega*C*w/(w + alpha)
+    towards the fascia point; a kinematic (w = 0) or coincident pair stays."""
+    xf = np.array([[1.0, 0, 0], [0, 2.0, 0], [0.5, 0.5, 0.5]], np.float32)
+    xm = np.array([[0.0, 0, 0], [0, 0, 0], [0.5, 0.5, 0.5]], np.float32)
+    wj = np.array([2.0, 0.0, 3.0], np.float32)
+    alpha = np.float32(0.5)
+    out = oracle.glue_step(xf, xm, wj, np.float32(0.25), np.full(3, alpha, np.float32), 0.1)
+    np.testing.assert_allclose(out[0], [0.1 * 0.75 * 2.0 / 2.5, 0, 0], rtol=1e-6)
+    np.testing.assert_array_equal(out[1:], xm[1:])
+
+
+def test_oracle_squeeze_follows_the_bones_and_pulls_the_muscle():
+    w = _small_workload(2)
+    cfg = json.loads((CFG / "humanv2_fascia_squeeze.json").read_text())
+    for c in cfg["constraints"]:
+        if c["name"] == "fascia_glue":
+            c["iterations"] = 6
+    a = sims.MultiGridSim(cfg, workload=w, engine_cls=oracle.OracleSim)
+    b = sims.MultiGridSim(dict(cfg, use_fascia=False), workload=w, engine_cls=oracle.OracleSim)
+    assert a.fascia.mode == "squeeze" and len(a.fascia.anchor) > 0
+    fp = w.info["fascia_mesh"][0]
+    for f in (1, 2, 3):
+        a.step_frame(f)
+        b.step_frame(f)
+    # the glue moved the muscle surface; the shell moved with the bones (advect + anchors)
+    assert np.abs(a.positions_np() - b.positions_np()).max() > 1e-5
+    # the shell rides the skeleton: its mean displacement is its follow points' mean displacement
+    # (the shrunk rest shape contracts it about itself; anchors are snapped, then relaxed)
+    fv = a.fascia.follow(w, 3)
+    moved = (a.fascia.positions_np() - fp).mean(0)
+    np.testing.assert_allclose(moved, (fv - w.pos0[a.fascia.follow_vids]).mean(0), atol=3e-3)
+    assert np.abs(fv - w.pos0[a.fascia.follow_vids]).max() > 1e-2
+    assert np.isfinite(a.fascia.positions_np()).all()
+
+
+def test_rest_shape_scale_scales_the_rest_matrices():
+    w = _small_workload(3)
+    sc = sims.fascia_scene_squeeze(json.loads((CFG / "humanv2_fascia_squeeze.json").read_text()),
+                                   *w.info["fascia_mesh"], 0.004)
+    assert sc.rest_scale == pytest.approx(0.95)
+    o = oracle.OracleSim(sc)
+    import dataclasses
+    o1 = oracle.OracleSim(dataclasses.replace(sc, rest_scale=1.0))
+    np.testing.assert_allclose(o.rest_matrix, o1.rest_matrix / np.float32(0.95), rtol=2e-7)
+    np.testing.assert_array_equal(o.rest_volume, o1.rest_volume)

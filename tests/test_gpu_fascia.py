@@ -1,8 +1,9 @@
-"""Fascia ride on the GPU against the oracle, through the C ABI (xpbd_set_fascia).
+"""The fascia sub-solver on the GPU against the oracle, through the C ABI (xpbd_set_fascia).
 
-The fascia shell is its own context (SNH, quasi-static), bound to the muscle
-surface at rest and moved every muscle substep (Fascia, fascia.py:45-572,
-fascia_mode "ride", the humanV2 default).
+The fascia shell is its own context (quasi-static). Ride (the humanV2 default): SNH,
+bound to the muscle surface at rest and moved every muscle substep. Squeeze: ARAP +
+volume with a shrunk rest shape, advected with the bones per frame, anchors snapped
+per substep, a one-way glue pulling the muscle surface (Fascia, fascia.py:45-572).
 """
 import json
 from pathlib import Path
@@ -111,3 +112,52 @@ def test_full_size_bind_sample_and_frame():
     tgt = oracle.fascia_ride_targets(sim.positions_np(), sim.fascia.surf_tris, face[face >= 0], wts[face >= 0],
                                      gap[face >= 0])
     np.testing.assert_allclose(p[face >= 0], tgt, atol=2e-6)
+
+
+def _squeeze_cfg(iters=8):
+    cfg = json.loads((CFG / "humanv2_fascia_squeeze.json").read_text())
+    for c in cfg["constraints"]:
+        if c["name"] == "fascia_glue":
+            c["iterations"] = iters
+    cfg["synthetic"]["sdf_res"] = 32
+    return cfg
+
+
+@pytest.mark.parametrize("strategy", [1, 2])
+def test_squeeze_frames_match_oracle(strategy):
+    w = _small(4)
+    cfg = _squeeze_cfg()
+    sim = sims.MultiGridSim(cfg, workload=w, strategy=strategy, block_tets=256, block_threads=256)
+    ref = sims.MultiGridSim(cfg, workload=w, engine_cls=oracle.OracleSim)
+    for f in range(1, 6):
+        print(f"frame {f}", flush=True)
+        sim.step_frame(f)
+        ref.step_frame(f)
+    fp = w.info["fascia_mesh"][0]
+    p, q = sim.fascia.positions_np(), ref.fascia.positions_np()
+    err = float(np.abs(p - q).max())
+    print("squeeze: fascia max |dx|", err, "moved", float(np.abs(q - fp).max()))
+    assert err <= 1e-4 * _diag(fp)
+    pm, qm = sim.positions_np(), ref.positions_np()
+    errm = float(np.abs(pm - qm).max())
+    print("squeeze: muscle max |dx|", errm)
+    assert errm <= 1e-4 * _diag(w.pos0)
+    # the glue acted on the muscle: not the same as without the shell
+    off = sims.MultiGridSim(dict(cfg, use_fascia=False), workload=w, strategy=strategy, block_tets=256,
+                            block_threads=256)
+    for f in range(1, 6):
+        off.step_frame(f)
+    assert float(np.abs(off.positions_np() - pm).max()) > 10 * 1e-4 * _diag(w.pos0)
+
+
+def test_squeeze_graph_equals_stream():
+    w = _small(5)
+    cfg = _squeeze_cfg()
+    a = sims.MultiGridSim(cfg, workload=w, strategy=2, block_tets=256, block_threads=256)
+    b = sims.MultiGridSim(cfg, workload=w, strategy=2, block_tets=256, block_threads=256)
+    b.use_graph = False
+    for f in range(1, 4):
+        a.step_frame(f)
+        b.step_frame(f)
+    assert np.array_equal(a.positions_np(), b.positions_np())
+    assert np.array_equal(a.fascia.positions_np(), b.fascia.positions_np())
