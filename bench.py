@@ -20,7 +20,9 @@ plain @same residual".  Prints ONE JSON line.
   warm frames with the audited per-frame fallback.
 * "fascia": the humanV2-scale scene with its fascia tubes as the fascia
   sub-solver in ride mode (bound to the muscle surface, 20 Stable Neo-Hookean
-  shell iterations per substep), against the same muscles without it.
+  shell iterations per substep) and in squeeze mode (bone-driven shell, ARAP +
+  volume with a shrunk rest shape, glue on the muscle surface), against the
+  same muscles without it.
 * "bone_mesh": the humanV2-scale frame with the exact-mesh bone collider (a
   capsule skeleton of 97,536 triangles, BVH refit per frame, closest-point
   push in every substep) in place of the grid SDFs, with the push, refit and
@@ -67,6 +69,7 @@ WORKLOADS = {
     "c5_batch": ("batch64_c5.json", {}),
     "mesh_collider": ("humanv2_mesh_collider.json", {}),
     "fascia_ride": ("humanv2_fascia_ride.json", {}),
+    "fascia_squeeze": ("humanv2_fascia_squeeze.json", {}),
 }
 # the block shape the autotuner picks on the 512-instance batch (tuned once per mesh; re-tuning it in
 # every bench run would rebuild the 7 M-tet layout 13 times)
@@ -408,6 +411,11 @@ def fascia_leg(args, tuning):
     fs.engine.close()
     sim0, rec0 = run_frames(dict(cfg, use_fascia=False), w, tuning, args.steps, args.warmup, "fascia_ride_off")
     sim0.engine.close()
+    cfg_s = load_cfg("fascia_squeeze")
+    sim_s, rec_s = run_frames(cfg_s, w, tuning, args.steps, args.warmup, "fascia_squeeze")
+    fss = sim_s.fascia
+    sim_s.engine.close()
+    fss.engine.close()
     return {
         "workload": f"{w.name} muscles {w.nt} tets / {w.nv} verts; fascia shell {fs.NT} tets / {fs.NV} verts, "
                     f"{nb} bound to {len(fs.surf_tris)} muscle surface triangles (bind_max_dist "
@@ -417,6 +425,9 @@ def fascia_leg(args, tuning):
         "fascia_tet_iterations_per_s": round(fs.NT * fs.iters * sim.nsubsteps /
                                              max(1e-9, (rec["ms_per_frame"] - rec0["ms_per_frame"]) * 1e-3), 1),
         "bound": nb, "finite": rec["finite"],
+        "squeeze": {"ms_per_frame": rec_s["ms_per_frame"],
+                    "fascia_ms_per_frame": round(rec_s["ms_per_frame"] - rec0["ms_per_frame"], 4),
+                    "anchors": int(len(fss.anchor)), "iterations": fss.iters, "finite": rec_s["finite"]},
     }
 
 
