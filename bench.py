@@ -48,7 +48,9 @@ METRIC = "ms/frame @535k tets; batched env-substeps/s; warm-start vs plain @same
 
 # HBM bytes per projection iteration measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per
 # dispatch, scaled x2.0 and x1.0 by a plain-copy calibration), keyed by (tets, constraint types,
-# strategy, block tets, block threads); the runs are summarised under profiles/
+# strategy, block tets, block threads) for the per-iteration launches, with the blocks per part
+# appended for the resident launch (its bytes per dispatch divided by the iterations it runs); the
+# runs are summarised under profiles/
 TRAFFIC = {
     (534882, ("tetarapsvd", "volume"), 2, 512, 512): 73468826.0,                    # humanV2-scale, plain
     (534882, ("tetarapsvd", "volume"), 2, 256, 256): 81038234.0,                    # the same, smaller blocks
@@ -86,6 +88,9 @@ def parse():
     ap.add_argument("--strategy", type=int, default=-1, help="-1 = autotune, 0 = patch, 1 = flat, 2 = block")
     ap.add_argument("--block-tets", type=int, default=1024, help="block strategy: tets per workgroup")
     ap.add_argument("--block-threads", type=int, default=256, help="block strategy: threads per workgroup")
+    ap.add_argument("--resident-blocks", type=int, default=0,
+                    help="block strategy: 0 = one launch per iteration, R = resident mode with R blocks per "
+                         "part, -1 = resident with the smallest R that fits")
     ap.add_argument("--c4-frames", type=int, default=10, help="timed frames of the 512-instance batch")
     ap.add_argument("--ws-train-frames", type=int, default=0, help="warm-start legs: training frames (0 = config)")
     ap.add_argument("--ws-eval-frames", type=int, default=0, help="warm-start legs: evaluation frames (0 = config)")
@@ -169,9 +174,16 @@ def run_frames(cfg, w, tuning, steps, warmup, label):
     t_reps = 20
     avg_us = sim.engine.time_iterations(t_iters, t_reps)
     strat = tuning["strategy"]
+    res = int(tuning.get("resident_blocks", 0)) if strat == 2 else 0
     types = tuple(c["type"] for c in sim.scene.meta["constraint_configs"])
-    traffic = TRAFFIC.get((w.nt, types, strat, tuning.get("block_tets", 0) if strat == 2 else 0,
-                           tuning.get("block_threads", 0) if strat == 2 else 0))
+    key = (w.nt, types, strat, tuning.get("block_tets", 0) if strat == 2 else 0,
+           tuning.get("block_threads", 0) if strat == 2 else 0)
+    traffic = TRAFFIC.get(key + (st["resident_blocks"],) if res else key)
+    kernel = ("k_iterate (fused Jacobi projection)", "k_flat_tets + k_flat_verts (one Jacobi iteration)",
+              "k_block_tets + k_block_shared (one Jacobi iteration)")[strat]
+    if res:
+        kernel = (f"k_resident (every plain iteration of a substep in one launch, {st['resident_parts']} parts "
+                  f"of {st['resident_blocks']} blocks; per iteration)")
     alg = st["bytes_per_iter_alg"]
     achieved = alg / (avg_us * 1e-6) / 1e9
     rec = {
@@ -182,8 +194,7 @@ def run_frames(cfg, w, tuning, steps, warmup, label):
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": ("k_iterate (fused Jacobi projection)", "k_flat_tets + k_flat_verts (one Jacobi iteration)",
-                       "k_block_tets + k_block_shared (one Jacobi iteration)")[strat],
+            "kernel": kernel,
             "avg_iteration_us": round(avg_us, 3),
             "timing": "one HIP graph of the iteration kernels between two HIP events on the library stream",
             "iterations_timed": t_iters * t_reps, "alg_bytes_per_iteration": alg,
@@ -439,7 +450,8 @@ def one_workload(args, name):
     tune_table = None
     if args.patch_verts or args.strategy >= 0:
         tuning = dict(strategy=max(args.strategy, 0), patch_verts=args.patch_verts or 256,
-                      block_tets=args.block_tets, block_threads=args.block_threads)
+                      block_tets=args.block_tets, block_threads=args.block_threads,
+                      resident_blocks=args.resident_blocks)
     elif name == "batch":
         tuning = dict(BATCH_TUNING)
     else:
@@ -494,7 +506,8 @@ def main():
     w = sims.build_workload(cfg)
     if args.patch_verts or args.strategy >= 0:
         tuning = dict(strategy=max(args.strategy, 0), patch_verts=args.patch_verts or 256,
-                      block_tets=args.block_tets, block_threads=args.block_threads)
+                      block_tets=args.block_tets, block_threads=args.block_threads,
+                      resident_blocks=args.resident_blocks)
         tune_table = None
     else:
         tuning, tune_table = autotune.pick(cfg, w)

@@ -63,7 +63,19 @@ typedef struct {
                                      (tets in blocks reduced in LDS + shared-vertex pass)        */
     int32_t block_tets;           /* block strategy: tets per block (64..8192, default 1024)     */
     int32_t block_threads;        /* block strategy: threads per block (64..512, default 256)    */
-    int32_t reserved[4];
+    int32_t resident_blocks;      /* block strategy: 0 = one launch per iteration (plus the
+                                     shared-vertex pass); R in 1..4 = resident mode, runs of R
+                                     blocks form a part and one launch runs all plain iterations
+                                     of a substep with each part's vertices in LDS (needs
+                                     block_tets == block_threads <= 256 and every part resident
+                                     at once, else XPBD_ELAYOUT); -1 = resident with the smallest
+                                     R that fits.  Same positions as 0, bit for bit.  The parts
+                                     wait for each other, so no other long-running kernel may
+                                     hold the device during a resident launch; a wait that runs
+                                     out sets a status that xpbd_synchronize / xpbd_get_state
+                                     return as XPBD_EHIP.  No counterpart in the reference (its
+                                     Warp launches are one per constraint type and iteration). */
+    int32_t reserved[3];
 } xpbd_tuning;
 
 typedef struct {
@@ -74,6 +86,12 @@ typedef struct {
     int32_t feature_mask;         /* bit0 arap, bit1 volume, bit2 snh, bit3 aniso, bit4 damping, bit5 attach, bit6 pins */
     double  bytes_per_iter_alg;   /* algorithmic bytes of one Jacobi iteration   */
     double  bytes_per_iter_impl;  /* bytes the projection kernel actually moves  */
+    int32_t resident_blocks;      /* resident mode: blocks per part (0 = off)     */
+    int32_t resident_parts;       /* resident mode: parts = workgroups per launch  */
+    int32_t max_part_verts;       /* resident mode: largest part vertex table      */
+    int32_t max_part_neighbours;  /* resident mode: most parts one part waits for  */
+    int32_t cross_vertex_entries; /* resident mode: part-local entries of vertices other parts touch too */
+    int32_t resident_lds;         /* resident mode: LDS bytes per workgroup        */
 } xpbd_stats;
 
 /* ---- lifetime --------------------------------------------------------- */

@@ -7,7 +7,9 @@ strategy (one thread per tet, then one per vertex; two launches per
 iteration, no replication, every corner correction goes through memory); and
 the "block" strategy (consecutive tets per workgroup, corner sums in LDS,
 only vertices shared between blocks go through memory) at several block
-sizes and thread counts.  Each candidate layout is built for the actual mesh,
+sizes and thread counts, one launch per iteration or resident (every plain
+iteration of a substep in one launch, parts of 1-3 blocks keeping their
+vertices in LDS; the iteration unroll is the whole substep).  Each candidate layout is built for the actual mesh,
 100 iterations are captured in one HIP graph and launched between two HIP
 events (as they run inside a frame), and the fastest is kept (candidates
 within 2 % of it are ties, which go to the largest block).  Results are cached per mesh signature in
@@ -25,21 +27,26 @@ import numpy as np
 from .capi import XpbdError
 
 CACHE = Path(__file__).resolve().parent / "_tune" / "cache.json"
-# (strategy, patch_verts, block_tets, block_threads)
-CANDIDATES = ((1, 256, 0, 0), (0, 64, 0, 0), (0, 128, 0, 0), (0, 256, 0, 0),
-              (2, 256, 128, 128), (2, 256, 192, 192), (2, 256, 256, 256), (2, 256, 384, 384),
-              (2, 256, 512, 256), (2, 256, 512, 512), (2, 256, 768, 384), (2, 256, 1024, 512),
-              (2, 256, 2048, 512))
+# (strategy, patch_verts, block_tets, block_threads, resident blocks per part)
+CANDIDATES = ((1, 256, 0, 0, 0), (0, 64, 0, 0, 0), (0, 128, 0, 0, 0), (0, 256, 0, 0, 0),
+              (2, 256, 128, 128, 0), (2, 256, 192, 192, 0), (2, 256, 256, 256, 0), (2, 256, 384, 384, 0),
+              (2, 256, 512, 256, 0), (2, 256, 512, 512, 0), (2, 256, 768, 384, 0), (2, 256, 1024, 512, 0),
+              (2, 256, 2048, 512, 0),
+              (2, 256, 256, 256, 1), (2, 256, 256, 256, 2), (2, 256, 256, 256, 3),
+              (2, 256, 128, 128, 2), (2, 256, 192, 192, 2))
 
 
 def name(c) -> str:
-    strat, pv, tb, bs = c
-    return "flat" if strat == 1 else (f"patch{pv}" if strat == 0 else f"block{tb}x{bs}")
+    strat, pv, tb, bs, res = c
+    if strat != 2:
+        return "flat" if strat == 1 else f"patch{pv}"
+    return f"block{tb}x{bs}" + (f"_resident{res}" if res else "")
 
 
 def as_kwargs(c) -> dict:
-    strat, pv, tb, bs = c
-    return dict(strategy=strat, patch_verts=pv, block_tets=tb or 1024, block_threads=bs or 256)
+    strat, pv, tb, bs, res = c
+    return dict(strategy=strat, patch_verts=pv, block_tets=tb or 1024, block_threads=bs or 256,
+                resident_blocks=res)
 
 
 def signature(cfg: dict, w) -> str:

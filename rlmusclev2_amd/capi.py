@@ -39,7 +39,7 @@ class XpbdParams(C.Structure):
 
 class XpbdTuning(C.Structure):
     _fields_ = [("patch_verts", C.c_int32), ("strategy", C.c_int32), ("block_tets", C.c_int32),
-                ("block_threads", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("block_threads", C.c_int32), ("resident_blocks", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class XpbdCoarseDesc(C.Structure):
@@ -81,7 +81,10 @@ class XpbdStats(C.Structure):
     _fields_ = [("nv", C.c_int32), ("nt", C.c_int32), ("n_elems", C.c_int32),
                 ("n_patches", C.c_int32), ("n_patch_tets", C.c_int32), ("max_table", C.c_int32),
                 ("max_patch_tets", C.c_int32), ("feature_mask", C.c_int32),
-                ("bytes_per_iter_alg", C.c_double), ("bytes_per_iter_impl", C.c_double)]
+                ("bytes_per_iter_alg", C.c_double), ("bytes_per_iter_impl", C.c_double),
+                ("resident_blocks", C.c_int32), ("resident_parts", C.c_int32), ("max_part_verts", C.c_int32),
+                ("max_part_neighbours", C.c_int32), ("cross_vertex_entries", C.c_int32),
+                ("resident_lds", C.c_int32)]
 
 
 _P = C.c_void_p

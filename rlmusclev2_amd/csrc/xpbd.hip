@@ -303,14 +303,16 @@ __device__ __forceinline__ void load_elem(const IterArgs &a, int64_t g, ElemIn<F
 // across the iterations of a substep (a.lam read and updated in place, one
 // thread per element); 2: warm-start impulse, the step applies dlambda = lambda0
 // from a.lam along each constraint gradient and keeps lambda = lambda0.
+// cls: the material-class table (global, or a copy in LDS)
 template <int FEAT, int DAMP, int LAM = 0>
 __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const ElemIn<FEAT, LAM> &e,
-                                             const float4 *sx, const float4 *spp, const int lv[4], Corr &out) {
+                                             const float4 *sx, const float4 *spp, const int lv[4], Corr &out,
+                                             const DevClass *cls) {
     const float4 lam = e.lam;
     float4 nl = lam;
     const uint32_t meta = e.meta;
     const uint32_t mask = (meta >> 16) & 15u;
-    const DevClass dc = a.cls[meta & 0xffffu];
+    const DevClass dc = cls[meta & 0xffffu];
     const ClassParams &cp = dc.p;
     const float4 X0 = sx[lv[0]], X1 = sx[lv[1]], X2 = sx[lv[2]], X3 = sx[lv[3]];
     const V3 P[4] = {xyz(X0), xyz(X1), xyz(X2), xyz(X3)};
@@ -574,7 +576,7 @@ __global__ __launch_bounds__(BS) void k_iterate(IterArgs a, const float4 *__rest
             const int lv[4] = {li.x, li.y, li.z, li.w};
             ElemIn<FEAT, 0> e;
             load_elem(a, (int64_t)P.tbeg + t, e);
-            project_elem<FEAT, DAMP>(a, (int64_t)P.tbeg + t, e, sx, spp, lv, c);
+            project_elem<FEAT, DAMP>(a, (int64_t)P.tbeg + t, e, sx, spp, lv, c, a.cls);
 #pragma unroll
             for (int k = 0; k < 4; ++k) sc[tid * 4 + k] = make_float4(c.d[k].x, c.d[k].y, c.d[k].z, c.cnt);
         }
@@ -639,7 +641,7 @@ __global__ __launch_bounds__(256) void k_flat_tets(IterArgs a, const float4 *__r
     Corr c;
     ElemIn<FEAT, LAM> e;
     load_elem(a, g, e);
-    project_elem<FEAT, DAMP, LAM>(a, g, e, xin, pprev, lv, c);
+    project_elem<FEAT, DAMP, LAM>(a, g, e, xin, pprev, lv, c, a.cls);
     const int4 so = a.slot_of[g];
     corr[so.x] = make_float4(c.d[0].x, c.d[0].y, c.d[0].z, c.cnt);
     corr[so.y] = make_float4(c.d[1].x, c.d[1].y, c.d[1].z, c.cnt);
@@ -794,7 +796,7 @@ __global__ __launch_bounds__(512) void k_block_tets(IterArgs a, BlockArgs b, con
         if (has_t) {
             const int lv[4] = {li.x, li.y, li.z, li.w};
             Corr c;
-            project_elem<FEAT, DAMP, LAM>(a, g, e, sx, spp, lv, c);
+            project_elem<FEAT, DAMP, LAM>(a, g, e, sx, spp, lv, c, a.cls);
             sc[cp.x] = make_float4(c.d[0].x, c.d[0].y, c.d[0].z, c.cnt);
             sc[cp.y] = make_float4(c.d[1].x, c.d[1].y, c.d[1].z, c.cnt);
             sc[cp.z] = make_float4(c.d[2].x, c.d[2].y, c.d[2].z, c.cnt);
@@ -832,7 +834,7 @@ __global__ __launch_bounds__(512) void k_block_tets(IterArgs a, BlockArgs b, con
         Corr c;
         ElemIn<FEAT, LAM> e;
         load_elem(a, g, e);
-        project_elem<FEAT, DAMP, LAM>(a, g, e, sx, spp, lv, c);
+        project_elem<FEAT, DAMP, LAM>(a, g, e, sx, spp, lv, c, a.cls);
         const ushort4 cp = b.bcpos[g];
         sc[cp.x] = make_float4(c.d[0].x, c.d[0].y, c.d[0].z, c.cnt);
         sc[cp.y] = make_float4(c.d[1].x, c.d[1].y, c.d[1].z, c.cnt);
@@ -896,6 +898,249 @@ __global__ __launch_bounds__(256) void k_block_shared(IterArgs a, BlockArgs b, c
         aw += r.w;
     }
     finish_vertex<DAMP, MODE>(a, v, x, pp, ax, ay, az, aw, pv, xout);
+}
+
+// ---------------------------------------------------------------------------
+// "resident" mode of the block strategy: one launch runs all K plain Jacobi
+// iterations of a substep.  Runs of R consecutive blocks form a part and one
+// workgroup per part keeps the part's vertices (positions, previous positions)
+// in LDS and its elements in registers for the whole launch.  Per iteration a
+// part projects its blocks one after the other (one element per thread) and
+// sums every block-local vertex's corners as k_block_tets does; the sum of a
+// vertex that only this part touches goes into an LDS accumulator (ascending
+// block order), the block partial of a cross vertex (touched by blocks of other
+// parts too) goes to its slot of the block strategy's partial array.  After the
+// part has finished its own vertices it publishes its iteration count, waits
+// for the parts it shares vertices with, and adds the partials of each cross
+// vertex in slot order, exactly as k_block_shared does: every sum has the
+// block strategy's operands in its order, so the positions equal those of the
+// per-iteration launches bit for bit.
+//
+// Partials and iteration counts cross workgroups (and XCDs) through
+// agent-scope atomics; the partial array is double-buffered by iteration
+// parity, so a part that runs ahead never overwrites a partial that a slower
+// neighbour still has to read (it waits for that neighbour's next count
+// first).  Every part of a launch must be resident at once: the host checks
+// the occupancy, and a wait that runs out of polls sets a status word instead
+// of hanging (the iterations that follow skip their waits, later launches
+// return at once, and the host reports the error).
+// ---------------------------------------------------------------------------
+struct ResArgs {
+    const PartDesc *parts;
+    const int4 *prec;          // per part-local entry: vertex, svid index or -1, write-back flag, 0
+    const uint32_t *bmap;      // per block-local entry: part-local index | cross << 31
+    const ushort4 *plidx;      // per element: part-local corner ids
+    const int32_t *nbr;
+    float4 *xp;                // block partials of cross vertices: two arrays of np, by iteration parity
+    int np;
+    uint32_t *count;           // per part: iterations completed since the counts were reset
+    uint32_t *status;          // [0] != 0: a wait ran out
+    int max_nloc, spin_limit, nclass;
+};
+
+// 16-byte agent-scope stores and loads of the partials: buffer accesses with
+// the sc1 cache policy (the bits an agent-scope atomic access carries), one
+// memory transaction per partial instead of four dword atomics
+typedef float res_v4f __attribute__((ext_vector_type(4)));
+constexpr int RES_SC1 = 16;                     // cache policy: sc1
+constexpr int RES_RSRC_W3 = 0x00020000;         // buffer descriptor word 3 (32-bit data format)
+
+__device__ __forceinline__ void res_store(__amdgpu_buffer_rsrc_t r, int slot, float4 v) {
+    const res_v4f x = {v.x, v.y, v.z, v.w};
+    __builtin_amdgcn_raw_buffer_store_b128(x, r, slot * 16, 0, RES_SC1);
+}
+
+__device__ __forceinline__ float4 res_load(__amdgpu_buffer_rsrc_t r, int slot) {
+    const res_v4f x = __builtin_amdgcn_raw_buffer_load_b128(r, slot * 16, 0, RES_SC1);
+    return make_float4(x.x, x.y, x.z, x.w);
+}
+
+// one element per thread per block (block_tets == blockDim.x), at most VPT
+// part-local vertices per thread.  LDS: positions, accumulators, previous
+// positions (damping), the corner corrections of one block, the class table.
+template <int FEAT, int DAMP, int VPT>
+__global__ __launch_bounds__(256) void k_resident(IterArgs a, BlockArgs b, ResArgs rs, const float4 *__restrict__ xin,
+                                                  float4 *__restrict__ xout, const float4 *__restrict__ pprev,
+                                                  int iters) {
+    extern __shared__ float4 smem[];
+    __shared__ uint32_t s_base, s_fail;
+    const int M = rs.max_nloc, TB = b.block_tets;
+    float4 *sx = smem, *sacc = smem + M;
+    float4 *spp = smem + 2 * M;
+    float4 *sc = smem + (DAMP ? 3 : 2) * M;
+    DevClass *scls = (DevClass *)(sc + 4 * TB);
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int p = blockIdx.x;
+    const PartDesc P = rs.parts[p];
+    if (tid == 0) {
+        s_base = __hip_atomic_load(rs.count + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_fail = __hip_atomic_load(rs.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    for (int i = tid; i < rs.nclass; i += nth) scls[i] = a.cls[i];
+    // the elements of a round (one block) are loaded one round ahead
+    ElemIn<FEAT, 0> e;
+    ushort4 pl = make_ushort4(0, 0, 0, 0), cp = pl;
+    {
+        const int64_t g = (int64_t)P.b0 * TB + tid;
+        if (g < a.nptet) {
+            load_elem(a, g, e);
+            pl = rs.plidx[g];
+            cp = b.bcpos[g];
+        }
+    }
+    // the part's vertices: state into LDS; their table entries, partial
+    // ranges and pin records in registers
+    int4 pr[VPT];
+    int2 so[VPT];
+    float4 pv[VPT];
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+        const int j = tid + k * nth;
+        pr[k] = make_int4(0, -1, 0, 0);
+        so[k] = make_int2(0, 0);
+        pv[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (j < P.nloc) {
+            pr[k] = rs.prec[P.vbeg + j];
+            sx[j] = xin[pr[k].x];
+            if (DAMP) spp[j] = pprev[pr[k].x];
+            sacc[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+            pv[k] = pin_of(a, pr[k].x);
+            if (pr[k].y >= 0) so[k] = make_int2(b.soff[pr[k].y], b.soff[pr[k].y + 1]);
+        }
+    }
+    __syncthreads();
+    if (s_fail) return;
+    const uint32_t base = s_base;
+    for (int it = 0; it < iters; ++it) {
+        const __amdgpu_buffer_rsrc_t xp = __builtin_amdgcn_make_buffer_rsrc(
+            rs.xp + (((base + it) & 1u) ? rs.np : 0), 0, rs.np * 16, RES_RSRC_W3);
+#pragma unroll 1
+        for (int rr = 0; rr < P.nblk; ++rr) {
+            const BlockDesc D = b.blk[P.b0 + rr];
+            // this block's local entries, used after the projection
+            uint32_t run = 0, bm = 0;
+            int bps = -1;
+            if (tid < D.nloc) {
+                run = b.brun[D.vbeg + tid];
+                bm = rs.bmap[D.vbeg + tid];
+                bps = b.bps[D.vbeg + tid];
+            }
+            // the next round's elements (the first block's again for the next iteration)
+            const int rn = (rr + 1 < P.nblk) ? rr + 1 : 0;
+            const bool more = rr + 1 < P.nblk || it + 1 < iters;
+            const int64_t gn = (int64_t)(P.b0 + rn) * TB + tid;
+            ElemIn<FEAT, 0> en;
+            ushort4 pln = make_ushort4(0, 0, 0, 0), cpn = pln;
+            if (more && gn < a.nptet) {
+                load_elem(a, gn, en);
+                pln = rs.plidx[gn];
+                cpn = b.bcpos[gn];
+            }
+            if (tid < D.ntet) {
+                const int64_t g = (int64_t)D.tbeg + tid;
+                const int lv[4] = {pl.x, pl.y, pl.z, pl.w};
+                Corr c;
+                project_elem<FEAT, DAMP, 0>(a, g, e, sx, spp, lv, c, scls);
+                sc[cp.x] = make_float4(c.d[0].x, c.d[0].y, c.d[0].z, c.cnt);
+                sc[cp.y] = make_float4(c.d[1].x, c.d[1].y, c.d[1].z, c.cnt);
+                sc[cp.z] = make_float4(c.d[2].x, c.d[2].y, c.d[2].z, c.cnt);
+                sc[cp.w] = make_float4(c.d[3].x, c.d[3].y, c.d[3].z, c.cnt);
+            }
+            e = en;
+            pl = pln;
+            cp = cpn;
+            __syncthreads();
+            if (tid < D.nloc) {
+                const int s0 = (int)(run & 0xffffu), n = (int)(run >> 16);
+                float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
+                for (int j = 0; j < n; ++j) {
+                    const float4 q = sc[s0 + j];
+                    ax += q.x;
+                    ay += q.y;
+                    az += q.z;
+                    aw += q.w;
+                }
+                if (bm >> 31) {
+                    res_store(xp, bps, make_float4(ax, ay, az, aw));
+                } else {
+                    float4 &acc = sacc[bm & 0x7fffffffu];
+                    acc.x += ax;
+                    acc.y += ay;
+                    acc.z += az;
+                    acc.w += aw;
+                }
+            }
+            __syncthreads();
+        }
+        // the vertices only this part touches: every projection of this
+        // iteration has read its positions, so each is finished in place
+#pragma unroll
+        for (int k = 0; k < VPT; ++k) {
+            const int j = tid + k * nth;
+            if (j < P.nloc && pr[k].y < 0) {
+                const float4 acc = sacc[j];
+                sx[j] = finish_value(a, pr[k].x, sx[j], DAMP ? spp[j] : make_float4(0, 0, 0, 0), acc.x, acc.y,
+                                     acc.z, acc.w, pv[k]);
+                sacc[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+        if (P.nnb > 0) {
+            // publish: once every thread's partial stores of iteration it have
+            // completed (agent-scope stores: complete = at the agent's point of
+            // coherence), the part's count says so.  Nothing else crosses
+            // workgroups, so no cache write-back or invalidate is needed.
+            const uint32_t want = base + (uint32_t)it + 1u;
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            if (tid == 0) __hip_atomic_store(rs.count + p, want, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // wait for the neighbours' partials of iteration it: one neighbour
+            // per thread, a bounded number of polls
+            if (!s_fail) {
+                for (int q = tid; q < P.nnb; q += nth) {
+                    const int nb = rs.nbr[P.nbeg + q];
+                    int polls = 0;
+                    while ((int)(__hip_atomic_load(rs.count + nb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - want) < 0) {
+                        if (++polls > rs.spin_limit ||
+                            __hip_atomic_load(rs.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                            __hip_atomic_store(rs.status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            s_fail = 1;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(2);
+                    }
+                }
+            }
+            // the partial loads below are agent-scope loads issued after the
+            // counts were seen (the barrier waits for the polls' results)
+            __syncthreads();
+            // cross vertices: all block partials in slot order (k_block_shared)
+#pragma unroll
+            for (int k = 0; k < VPT; ++k) {
+                const int j = tid + k * nth;
+                if (j < P.nloc && pr[k].y >= 0) {
+                    float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
+                    for (int s = so[k].x; s < so[k].y; ++s) {
+                        const float4 q = res_load(xp, s);
+                        ax += q.x;
+                        ay += q.y;
+                        az += q.z;
+                        aw += q.w;
+                    }
+                    sx[j] = finish_value(a, pr[k].x, sx[j], DAMP ? spp[j] : make_float4(0, 0, 0, 0), ax, ay, az,
+                                         aw, pv[k]);
+                }
+            }
+        } else if (tid == 0) {
+            __hip_atomic_store(rs.count + p, base + (uint32_t)it + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+    }
+    // write back the vertices this part owns
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+        const int j = tid + k * nth;
+        if (j < P.nloc && pr[k].z) xout[pr[k].x] = sx[j];
+    }
 }
 
 
@@ -2717,6 +2962,17 @@ struct xpbd_ctx {
     ushort4 *d_blidx = nullptr;
     ushort4 *d_bcpos = nullptr;
     float4 *BPART = nullptr;
+    // resident mode of the block strategy (all plain iterations of a substep in one launch)
+    int res_R = 0, res_vpt = 0;
+    size_t res_lds = 0;
+    xpbd::Parts parts;
+    PartDesc *d_parts = nullptr;
+    int4 *d_prec = nullptr;
+    uint32_t *d_bmap = nullptr;
+    ushort4 *d_plidx = nullptr;
+    int32_t *d_nbr = nullptr;
+    float4 *RXP = nullptr;            // 2 x n_partial
+    uint32_t *d_rcount = nullptr, *d_rstatus = nullptr;
     // two-level correction: a finalized coarse context (not owned) and transfer tables
     xpbd_ctx *coarse = nullptr;
     // fascia shell in ride mode: a finalized context (not owned) whose bound vertices follow this
@@ -2892,6 +3148,8 @@ void free_device(xpbd_ctx *c) {
     c->SFEAT = nullptr;
     c->sfeat_ok = false;
     c->pin_cap = 0;
+    c->res_R = 0;
+    c->d_rcount = c->d_rstatus = nullptr;
 }
 
 void free_stage(HostStage &s) {
@@ -3291,6 +3549,28 @@ BlockVertKernel pick_block_shared(int feat, int mode) {
     return damp ? k_block_shared<1, MODE_FUSED> : k_block_shared<0, MODE_FUSED>;
 }
 
+typedef void (*ResKernel)(IterArgs, BlockArgs, ResArgs, const float4 *, float4 *, const float4 *, int);
+
+template <int FEAT, int DAMP>
+ResKernel pick_res_v(int vpt) {
+    return vpt <= 1 ? k_resident<FEAT, DAMP, 1> : (vpt == 2 ? k_resident<FEAT, DAMP, 2> : k_resident<FEAT, DAMP, 3>);
+}
+
+// part-local vertices per thread: 1 to 3
+ResKernel pick_resident(int feat, int vpt) {
+    constexpr int AV = F_ARAP | F_VOL, SA = F_SNH | F_ANISO, ALL = F_ARAP | F_VOL | F_SNH | F_ANISO;
+    const int f = feat & ALL;
+    const int damp = (feat & F_DAMP) ? 1 : 0;
+    if ((f & ~AV) == 0) return damp ? pick_res_v<AV, 1>(vpt) : pick_res_v<AV, 0>(vpt);
+    if ((f & ~SA) == 0) return damp ? pick_res_v<SA, 1>(vpt) : pick_res_v<SA, 0>(vpt);
+    return damp ? pick_res_v<ALL, 1>(vpt) : pick_res_v<ALL, 0>(vpt);
+}
+
+size_t resident_lds(int feature, int M, int TB, int nclass) {
+    const size_t planes = (feature & F_DAMP) ? 3 : 2;
+    return (planes * (size_t)M + 4 * (size_t)TB) * sizeof(float4) + (size_t)nclass * sizeof(DevClass);
+}
+
 size_t block_lds(const xpbd_ctx *c) {
     const size_t M = (size_t)c->L.max_nloc;
     const size_t damp = (c->feature & F_DAMP) ? 2 : 1;
@@ -3366,6 +3646,57 @@ int launch_iter(xpbd_ctx *c, const float4 *xin, float4 *xout, int mode, int lam 
         HIPCHK(c, hipEventRecord(ev.b, c->stream));
         c->evs.push_back(ev);
     }
+    return XPBD_OK;
+}
+
+// all `iters` plain iterations of a substep in one launch of the resident
+// kernel: positions from xin, the result in xout (xout may be xin: every part
+// has read its vertices before any neighbour can finish an iteration)
+int launch_resident(xpbd_ctx *c, const float4 *xin, float4 *xout, int iters) {
+    if (iters <= 0) return XPBD_OK;
+    IterArgs a = make_args(c, c->DPW);
+    BlockArgs b = make_block_args(c);
+    ResArgs rs;
+    rs.parts = c->d_parts;
+    rs.prec = c->d_prec;
+    rs.bmap = c->d_bmap;
+    rs.plidx = c->d_plidx;
+    rs.nbr = c->d_nbr;
+    const size_t np = std::max<size_t>((size_t)c->L.n_partial, 1);
+    rs.xp = c->RXP;
+    rs.np = (int)np;
+    rs.count = c->d_rcount;
+    rs.status = c->d_rstatus;
+    rs.max_nloc = c->parts.max_nloc;
+    rs.spin_limit = 1 << 20;
+    rs.nclass = (int)c->L.classes.size();
+    ResKernel k = pick_resident(c->feature, c->res_vpt);
+    const float4 *pp = c->PPv;
+    EvPair ev{};
+    const bool tm = c->timing;
+    if (tm) {
+        HIPCHK(c, hipEventCreate(&ev.a));
+        HIPCHK(c, hipEventCreate(&ev.b));
+        HIPCHK(c, hipEventRecord(ev.a, c->stream));
+    }
+    void *args[] = {(void *)&a, (void *)&b, (void *)&rs, (void *)&xin, (void *)&xout, (void *)&pp, (void *)&iters};
+    HIPCHK(c, hipLaunchKernel((const void *)k, dim3((unsigned)c->parts.parts.size()), dim3(c->block_bs), args,
+                              c->res_lds, c->stream));
+    if (tm) {
+        HIPCHK(c, hipEventRecord(ev.b, c->stream));
+        c->evs.push_back(ev);
+    }
+    return XPBD_OK;
+}
+
+// the resident kernel's status word: set when a wait for a neighbouring part ran out
+int check_resident(xpbd_ctx *c) {
+    if (!c->res_R || !c->d_rstatus) return XPBD_OK;
+    uint32_t st = 0;
+    HIPCHK(c, hipMemcpy(&st, c->d_rstatus, sizeof(st), hipMemcpyDeviceToHost));
+    if (st)
+        return fail(c, XPBD_EHIP, "resident iterations: a wait for a neighbouring part ran out (the parts were not "
+                                  "all resident at once); positions are invalid until xpbd_finalize runs again");
     return XPBD_OK;
 }
 
@@ -3678,9 +4009,13 @@ int enqueue_substep(xpbd_ctx *c, int iters, bool dcd_rebuild, int sub, bool inte
         c->eff_mode = c->lam_mode;
     }
     if (c->eff_mode && (r = launch_warm_start(c))) return r;
-    for (int it = 0; it < iters; ++it) {
-        r = launch_iter(c, c->X[it & 1], c->X[(it + 1) & 1], MODE_FUSED, lam_for_iteration(c));
-        if (r) return r;
+    if (c->res_R > 0 && c->eff_mode == 0) {
+        if ((r = launch_resident(c, c->X[0], c->X[iters & 1], iters))) return r;
+    } else {
+        for (int it = 0; it < iters; ++it) {
+            r = launch_iter(c, c->X[it & 1], c->X[(it + 1) & 1], MODE_FUSED, lam_for_iteration(c));
+            if (r) return r;
+        }
     }
     if ((r = finish_lambda(c))) return r;
     // post phase (mg.py:1362-1376): the fascia shell's substep, the self DCD passes, then the
@@ -3694,6 +4029,68 @@ int enqueue_substep(xpbd_ctx *c, int iters, bool dcd_rebuild, int sub, bool inte
                                 k + 1 < c->dcd_iters)))
                 return r;
     return launch_post(c, c->X[iters & 1], 1, 1, fuse_next, dcd);
+}
+
+// resident mode: parts of R consecutive blocks, the smallest R in the allowed
+// range whose parts all fit on the device at once (occupancy of the kernel at
+// its LDS size times the compute units), then the part tables on the device
+int setup_resident(xpbd_ctx *c, int want, int maxlds) {
+    c->res_R = 0;
+    if (want == 0) return XPBD_OK;
+    if (want < -1 || want > 4) return fail(c, XPBD_EINVAL, "resident_blocks must be -1, 0 or 1..4");
+    if (c->block != c->block_bs || c->block_bs > 256)
+        return fail(c, XPBD_ELAYOUT, "resident mode needs block_tets == block_threads <= 256");
+    if (c->L.max_nloc > c->block_bs)
+        return fail(c, XPBD_ELAYOUT, "resident mode: a block touches more vertices than it has threads");
+    if ((int64_t)c->L.n_partial > (1 << 26))
+        return fail(c, XPBD_ELAYOUT, "resident mode: more than 2^26 block partials");
+    int ncu = 0;
+    HIPCHK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+    const int r0 = want > 0 ? want : 1, r1 = want > 0 ? want : 4;
+    std::string why = "no part size fits";
+    for (int R = r0; R <= r1; ++R) {
+        xpbd::Parts P;
+        int code = 0;
+        std::string m = xpbd::build_parts(c->L, c->nv, R, P, &code);
+        if (code) { why = m; continue; }
+        const int vpt = (P.max_nloc + c->block_bs - 1) / c->block_bs;
+        if (vpt > 3) {
+            why = "a part of " + std::to_string(R) + " blocks has " + std::to_string(P.max_nloc) +
+                  " vertices, more than 3 per thread";
+            continue;
+        }
+        const size_t lds = resident_lds(c->feature, P.max_nloc, c->block, (int)c->L.classes.size());
+        if (lds > (size_t)maxlds) { why = "a part needs " + std::to_string(lds) + " B of LDS"; continue; }
+        ResKernel k = pick_resident(c->feature, std::max(vpt, 1));
+        HIPCHK(c, hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        int occ = 0;
+        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)k, c->block_bs, lds));
+        if ((int64_t)occ * ncu < (int64_t)P.parts.size()) {
+            why = std::to_string(P.parts.size()) + " parts of " + std::to_string(R) + " blocks, but only " +
+                  std::to_string(occ) + " x " + std::to_string(ncu) + " fit on the device at once";
+            continue;
+        }
+        c->parts = std::move(P);
+        c->res_R = R;
+        c->res_lds = lds;
+        c->res_vpt = std::max(vpt, 1);
+        break;
+    }
+    if (!c->res_R) return fail(c, XPBD_ELAYOUT, "resident mode: " + why);
+    const xpbd::Parts &P = c->parts;
+    int r;
+    if ((r = dupload(c, &c->d_parts, P.parts.data(), P.parts.size()))) return r;
+    if ((r = dupload(c, &c->d_prec, (const int4 *)P.prec.data(), P.prec.size() / 4))) return r;
+    if ((r = dupload(c, &c->d_bmap, P.bmap.data(), P.bmap.size()))) return r;
+    if ((r = dupload(c, &c->d_plidx, (const ushort4 *)P.plidx.data(), P.plidx.size() / 4))) return r;
+    if ((r = dupload(c, &c->d_nbr, P.nbr.data(), P.nbr.size()))) return r;
+    const size_t np = std::max<size_t>((size_t)c->L.n_partial, 1);
+    if ((r = dalloc(c, &c->RXP, 2 * np))) return r;
+    if ((r = dalloc(c, &c->d_rcount, P.parts.size()))) return r;
+    if ((r = dalloc(c, &c->d_rstatus, 4))) return r;
+    HIPCHK(c, hipMemset(c->d_rcount, 0, P.parts.size() * sizeof(uint32_t)));
+    HIPCHK(c, hipMemset(c->d_rstatus, 0, 4 * sizeof(uint32_t)));
+    return XPBD_OK;
 }
 
 int check_ready(xpbd_ctx *c) {
@@ -4349,6 +4746,7 @@ int32_t xpbd_finalize(xpbd_ctx *c, const xpbd_tuning *t) {
             UP(c->d_svid, L.svid);
             UP(c->d_soff, L.soff);
             if ((r = dalloc(c, &c->BPART, std::max<size_t>((size_t)L.n_partial, 1)))) return r;
+            if ((r = setup_resident(c, t ? t->resident_blocks : 0, maxlds))) return r;
         } else {
             if ((r = dalloc(c, &c->CORR, 4 * std::max<size_t>(nptet, 1)))) return r;
         }
@@ -4467,6 +4865,12 @@ int32_t xpbd_get_stats(xpbd_ctx *c, xpbd_stats *o) {
     o->feature_mask = c->feature;
     o->bytes_per_iter_alg = c->L.bytes_alg;
     o->bytes_per_iter_impl = c->L.bytes_impl;
+    o->resident_blocks = c->res_R;
+    o->resident_parts = c->res_R ? (int)c->parts.parts.size() : 0;
+    o->max_part_verts = c->res_R ? c->parts.max_nloc : 0;
+    o->max_part_neighbours = c->res_R ? c->parts.max_nnb : 0;
+    o->cross_vertex_entries = c->res_R ? c->parts.n_cross : 0;
+    o->resident_lds = (int)c->res_lds;
     return XPBD_OK;
 }
 
@@ -4479,6 +4883,7 @@ int32_t xpbd_set_state(xpbd_ctx *c, const float *pos, const float *pprev, const 
     HIPCHK(c, hipMemcpyAsync(pp.data(), c->PPv, nv * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(v.data(), c->VEL, nv * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((r = check_resident(c))) return r;
     for (int n = 0; n < nv; ++n) {
         const int o = c->L.perm[n];
         if (pos) { x[n].x = pos[3 * o]; x[n].y = pos[3 * o + 1]; x[n].z = pos[3 * o + 2]; }
@@ -4500,6 +4905,7 @@ int32_t xpbd_get_state(xpbd_ctx *c, float *pos, float *pprev, float *vel) {
     HIPCHK(c, hipMemcpyAsync(pp.data(), c->PPv, nv * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(v.data(), c->VEL, nv * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((r = check_resident(c))) return r;
     for (int n = 0; n < nv; ++n) {
         const int o = c->L.perm[n];
         if (pos) { pos[3 * o] = x[n].x; pos[3 * o + 1] = x[n].y; pos[3 * o + 2] = x[n].z; }
@@ -4550,8 +4956,12 @@ int32_t xpbd_iterate(xpbd_ctx *c, int32_t n) {
     if (r) return r;
     if (n < 0) return fail(c, XPBD_EINVAL, "negative iteration count");
     c->eff_mode = c->lam_mode == 4 ? (c->warm_net ? 2 : 3) : c->lam_mode;
-    for (int it = 0; it < n; ++it)
-        if ((r = launch_iter(c, c->X[it & 1], c->X[(it + 1) & 1], MODE_FUSED, lam_for_iteration(c)))) return r;
+    if (c->res_R > 0 && c->eff_mode == 0) {
+        if ((r = launch_resident(c, c->X[0], c->X[n & 1], n))) return r;
+    } else {
+        for (int it = 0; it < n; ++it)
+            if ((r = launch_iter(c, c->X[it & 1], c->X[(it + 1) & 1], MODE_FUSED, lam_for_iteration(c)))) return r;
+    }
     if (n & 1) {
         HIPCHK(c, hipMemcpyAsync(c->X[0], c->X[1], c->nv * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
     }
@@ -4622,7 +5032,7 @@ int32_t xpbd_step_frame(xpbd_ctx *c, int32_t nsub, int32_t iters, int32_t use_gr
 int32_t xpbd_synchronize(xpbd_ctx *c) {
     if (!c) return XPBD_EINVAL;
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    return XPBD_OK;
+    return check_resident(c);
 }
 
 int32_t xpbd_residuals(xpbd_ctx *c, double *out8) {
@@ -5452,8 +5862,13 @@ int32_t xpbd_time_iterations(xpbd_ctx *c, int32_t n, int32_t reps, double *us_pe
     // single graph launch and no host gaps between replays
     const int total = n * reps;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    for (int it = 0; it < total && r == XPBD_OK; ++it)
-        r = launch_iter(c, c->X[it & 1], c->X[(it + 1) & 1], MODE_FUSED, 0);
+    if (c->res_R > 0) {
+        // resident mode: one launch per n iterations (n is even, so each ends in X[0])
+        for (int k = 0; k < reps && r == XPBD_OK; ++k) r = launch_resident(c, c->X[0], c->X[0], n);
+    } else {
+        for (int it = 0; it < total && r == XPBD_OK; ++it)
+            r = launch_iter(c, c->X[it & 1], c->X[(it + 1) & 1], MODE_FUSED, 0);
+    }
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     c->timing = tm;
@@ -5478,6 +5893,7 @@ int32_t xpbd_time_iterations(xpbd_ctx *c, int32_t n, int32_t reps, double *us_pe
     if (he != hipSuccess) return fail(c, XPBD_EHIP, std::string("iteration timing: ") + hipGetErrorString(he));
     HIPCHK(c, hipMemcpyAsync(c->X[0], c->T_X, b, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((r = check_resident(c))) return r;
     *us_per_iter = 1e3 * (double)ms / (double)total;
     return XPBD_OK;
 }

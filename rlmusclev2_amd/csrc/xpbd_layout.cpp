@@ -273,6 +273,103 @@ std::string build_blocks(const HostInput &in, Layout &L, int feature, int *code)
     return std::string();
 }
 
+std::string build_parts(const Layout &L, int nv, int R, Parts &P, int *code) {
+    *code = 0;
+    P = Parts();
+    if (L.block_tets <= 0) { *code = 1; return "parts need the block strategy"; }
+    if (R < 1 || R > 64) { *code = 1; return "blocks per part must be in [1, 64]"; }
+    const int64_t nb = (int64_t)L.blocks.size();
+    const int64_t G = (nb + R - 1) / R;
+    P.R = R;
+    P.parts.assign((size_t)G, PartDesc{});
+    // the parts that touch each vertex, ascending (blocks are visited in order,
+    // so a vertex's parts come in ascending order): counts, then a CSR
+    std::vector<int32_t> npart(nv, 0), last(nv, -1);
+    for (int64_t b = 0; b < nb; ++b) {
+        const BlockDesc &d = L.blocks[(size_t)b];
+        const int32_t p = (int32_t)(b / R);
+        for (int i = 0; i < d.nloc; ++i) {
+            const int v = L.bvt[d.vbeg + i];
+            if (last[v] != p) { last[v] = p; ++npart[v]; }
+        }
+    }
+    std::vector<int64_t> poff(nv + 1, 0);
+    for (int v = 0; v < nv; ++v) poff[v + 1] = poff[v] + npart[v];
+    std::vector<int32_t> plist((size_t)poff[nv]);
+    {
+        std::vector<int64_t> fill(poff.begin(), poff.end() - 1);
+        std::fill(last.begin(), last.end(), -1);
+        for (int64_t b = 0; b < nb; ++b) {
+            const BlockDesc &d = L.blocks[(size_t)b];
+            const int32_t p = (int32_t)(b / R);
+            for (int i = 0; i < d.nloc; ++i) {
+                const int v = L.bvt[d.vbeg + i];
+                if (last[v] != p) { last[v] = p; plist[(size_t)fill[v]++] = p; }
+            }
+        }
+    }
+    std::vector<int32_t> sidx(nv, -1);
+    for (size_t s = 0; s < L.svid.size(); ++s) sidx[L.svid[s]] = (int32_t)s;
+    // vertices no element touches: finished every iteration like the others
+    // (attach, pins), each by one part
+    std::vector<std::vector<int32_t>> orphans((size_t)G);
+    if (G > 0)
+        for (int v = 0; v < nv; ++v)
+            if (npart[v] == 0) orphans[(size_t)std::min<int64_t>(G - 1, (int64_t)v * G / nv)].push_back(v);
+    P.bmap.assign(L.bvt.size(), 0u);
+    P.plidx.assign(L.blidx.size(), 0);
+    std::vector<int32_t> loc(nv, -1), verts, nb_list;
+    for (int64_t p = 0; p < G; ++p) {
+        PartDesc &pd = P.parts[(size_t)p];
+        pd.b0 = (int32_t)(p * R);
+        pd.nblk = (int32_t)std::min<int64_t>(R, nb - p * R);
+        verts.clear();
+        for (int rr = 0; rr < pd.nblk; ++rr) {
+            const BlockDesc &d = L.blocks[(size_t)(pd.b0 + rr)];
+            for (int i = 0; i < d.nloc; ++i) verts.push_back(L.bvt[d.vbeg + i]);
+        }
+        std::sort(verts.begin(), verts.end());
+        verts.erase(std::unique(verts.begin(), verts.end()), verts.end());
+        for (int v : orphans[(size_t)p]) verts.push_back(v);
+        if (verts.size() > 65535) { *code = 5; return "part vertex table exceeds 65535 entries"; }
+        pd.vbeg = (int32_t)(P.prec.size() / 4);
+        pd.nloc = (int32_t)verts.size();
+        P.max_nloc = std::max(P.max_nloc, pd.nloc);
+        nb_list.clear();
+        for (size_t j = 0; j < verts.size(); ++j) {
+            const int v = verts[j];
+            loc[v] = (int32_t)j;
+            const bool cross = npart[v] > 1;
+            P.prec.push_back(v);
+            P.prec.push_back(cross ? sidx[v] : -1);
+            P.prec.push_back(npart[v] == 0 || plist[(size_t)poff[v]] == (int32_t)p ? 1 : 0);
+            P.prec.push_back(0);
+            if (cross) {
+                ++P.n_cross;
+                for (int64_t q = poff[v]; q < poff[v + 1]; ++q)
+                    if (plist[(size_t)q] != (int32_t)p) nb_list.push_back(plist[(size_t)q]);
+            }
+        }
+        for (int rr = 0; rr < pd.nblk; ++rr) {
+            const BlockDesc &d = L.blocks[(size_t)(pd.b0 + rr)];
+            for (int i = 0; i < d.nloc; ++i) {
+                const int v = L.bvt[d.vbeg + i];
+                P.bmap[(size_t)d.vbeg + i] = (uint32_t)loc[v] | (npart[v] > 1 ? 0x80000000u : 0u);
+            }
+            for (int64_t g = d.tbeg; g < (int64_t)d.tbeg + d.ntet; ++g)
+                for (int k = 0; k < 4; ++k) P.plidx[(size_t)(4 * g + k)] = (uint16_t)loc[L.gidx[(size_t)(4 * g + k)]];
+        }
+        std::sort(nb_list.begin(), nb_list.end());
+        nb_list.erase(std::unique(nb_list.begin(), nb_list.end()), nb_list.end());
+        pd.nbeg = (int32_t)P.nbr.size();
+        pd.nnb = (int32_t)nb_list.size();
+        P.max_nnb = std::max(P.max_nnb, pd.nnb);
+        P.nbr.insert(P.nbr.end(), nb_list.begin(), nb_list.end());
+        for (int v : verts) loc[v] = -1;
+    }
+    return std::string();
+}
+
 std::string build_layout(const HostInput &in, Layout &L, int *code) {
     *code = 0;
     const int nv = in.nv;

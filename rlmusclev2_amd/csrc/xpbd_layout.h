@@ -60,6 +60,33 @@ struct BlockDesc {
     int32_t tbeg, ntet, vbeg, nloc;
 };
 
+// "resident" mode of the block strategy: runs of R consecutive blocks form a
+// part, and one workgroup per part runs every Jacobi iteration of a substep in
+// one launch.  The part's vertices stay in LDS across the iterations; its
+// elements stay in registers.  A vertex that blocks of other parts also touch
+// (a cross vertex) gets the same per-block partial slots as in the block
+// strategy; each part writes its blocks' partials, waits for the parts it shares
+// vertices with, then adds all partials of the vertex in ascending block order.
+// Every other vertex is summed in LDS, also in ascending block order, so a
+// substep gives bit for bit the positions of the block strategy's per-iteration
+// launches.
+struct PartDesc {
+    int32_t b0, nblk, vbeg, nloc, nbeg, nnb, pad0, pad1;
+};
+
+struct Parts {
+    int32_t R = 0;                   // blocks per part
+    std::vector<PartDesc> parts;
+    // per part-local entry: vertex id (new order), index into svid / soff for a
+    // cross vertex (else -1), write-back flag (the part of the vertex's first
+    // block; vertices no element touches go to one part), unused
+    std::vector<int32_t> prec;       // 4 per entry
+    std::vector<uint32_t> bmap;      // per block-local entry: part-local index | cross << 31
+    std::vector<uint16_t> plidx;     // 4 per element: part-local corner ids
+    std::vector<int32_t> nbr;        // neighbour parts, ascending, CSR by PartDesc::nbeg / nnb
+    int32_t max_nloc = 0, max_nnb = 0, n_cross = 0;
+};
+
 struct Layout {
     // vertex renumbering
     std::vector<int32_t> perm;   // new -> old
@@ -102,6 +129,9 @@ struct Layout {
 
 // returns empty string on success, else the error message; *code gets the status
 std::string build_layout(const HostInput &in, Layout &out, int *code);
+
+// parts of R consecutive blocks over a built block layout (L.blocks and its tables)
+std::string build_parts(const Layout &L, int nv, int R, Parts &out, int *code);
 
 // FP32 rest precompute in the reference's operation order (engine.py:199-224):
 // Dm columns p_k - p3, |det|/6, mass += V*density/4 (tet order), B = inv(Dm).

@@ -27,7 +27,7 @@ class XpbdEngine:
     """Device state + phase primitives for one mesh (or one packed batch)."""
 
     def __init__(self, scene: Scene, device: int = 0, patch_verts: int = 256, strategy: int = 0,
-                 block_tets: int = 1024, block_threads: int = 256):
+                 block_tets: int = 1024, block_threads: int = 256, resident_blocks: int = 0):
         self.scene = scene
         self.ctx = capi.Context(device)
         self.n_verts = scene.nv
@@ -42,6 +42,7 @@ class XpbdEngine:
         self.strategy = int(strategy)
         self.block_tets = int(block_tets)
         self.block_threads = int(block_threads)
+        self.resident_blocks = int(resident_blocks)
         self._upload(scene, patch_verts)
 
     # -- setup -----------------------------------------------------------
@@ -95,6 +96,7 @@ class XpbdEngine:
         tun.strategy = self.strategy
         tun.block_tets = self.block_tets
         tun.block_threads = self.block_threads
+        tun.resident_blocks = self.resident_blocks
         c.call("xpbd_finalize", C.byref(tun))
         if s.pin_vids is not None:
             tgt = f32(s.pos0, (-1, 3))[k["pin_vids"]]

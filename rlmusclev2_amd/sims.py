@@ -299,7 +299,7 @@ class MultiGridSim:
 
     def __init__(self, cfg: dict, workload: workloads.Workload | None = None, device: int = 0,
                  patch_verts: int = 256, engine_cls=XpbdEngine, strategy: int = 0, block_tets: int = 1024,
-                 block_threads: int = 256):
+                 block_threads: int = 256, resident_blocks: int = 0):
         if not cfg.get("use_jacobi", True):
             raise NotImplementedError("use_jacobi=false: Gauss-Seidel is not on the GPU path")
         self.cfg = cfg
@@ -310,7 +310,8 @@ class MultiGridSim:
         self.dt = self.scene.params.dt
         self.start_frame = int(cfg.get("start_frame", 1))
         self.engine = engine_cls(self.scene, device=device, patch_verts=patch_verts, strategy=strategy,
-                                 block_tets=block_tets, block_threads=block_threads) \
+                                 block_tets=block_tets, block_threads=block_threads,
+                                 resident_blocks=resident_blocks) \
             if engine_cls is XpbdEngine else engine_cls(self.scene)
         self.NV = self.scene.nv
         self.NT = self.scene.nt
@@ -321,7 +322,7 @@ class MultiGridSim:
         if cfg.get("use_fascia", False):
             self.fascia = FasciaShell(self, engine_cls, device,
                                       dict(strategy=strategy, patch_verts=patch_verts, block_tets=block_tets,
-                                           block_threads=block_threads))
+                                           block_threads=block_threads, resident_blocks=resident_blocks))
         self.use_graph = True
         self._native_collider = None
         self.collision_solvers = []

@@ -41,3 +41,25 @@ def test_attach_records_and_duplicate_rejection(tmp_path):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "bad 0" in r.stdout
+
+
+@pytest.fixture(scope="module")
+def parts_checker(tmp_path_factory):
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    exe = tmp_path_factory.mktemp("parts") / "check_parts_layout"
+    subprocess.run(["g++", "-O2", "-std=c++17", f"-I{ROOT / 'include'}", f"-I{ROOT / 'rlmusclev2_amd' / 'csrc'}",
+                    str(ROOT / "tests" / "cpp" / "check_parts_layout.cpp"),
+                    str(ROOT / "rlmusclev2_amd" / "csrc" / "xpbd_layout.cpp"), "-o", str(exe)], check=True)
+    return exe
+
+
+@pytest.mark.parametrize("n,tb,r", [(7, 64, 1), (12, 256, 3), (20, 256, 4), (20, 128, 2), (30, 256, 3),
+                                    (12, 64, 64)])
+def test_resident_parts_equal_block_sums(parts_checker, n, tb, r):
+    """parts of r blocks: cover, write-back once, symmetric neighbour lists,
+    corner ids, and FP32 sums bit-identical to the block strategy's"""
+    res = subprocess.run([str(parts_checker), str(n), str(tb), str(r)], capture_output=True, text=True)
+    print(res.stdout)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "bad 0" in res.stdout

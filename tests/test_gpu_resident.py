@@ -1,0 +1,129 @@
+"""Resident mode of the block strategy (all plain iterations of a substep in one
+launch, each part's vertices in LDS, partials of vertices shared between
+parts exchanged through agent-scope atomics): frames equal the per-iteration
+block launches bit for bit, on small meshes (every constraint family, pins,
+attach, damping, bone SDFs) and on the full humanV2-scale scene, and match
+the oracle."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from rlmusclev2_amd import sims, workloads
+
+pytestmark = pytest.mark.gpu
+CFG = Path(__file__).resolve().parents[1] / "rlmusclev2_amd" / "configs"
+
+
+def _cfg(name, **over):
+    cfg = json.loads((CFG / name).read_text())
+    cfg.update(over)
+    return cfg
+
+
+def _diag(p):
+    return float(np.linalg.norm(p.max(0) - p.min(0)))
+
+
+def _pair(cfg, w, resident):
+    kw = dict(strategy=2, block_tets=256, block_threads=256)
+    a = sims.MultiGridSim(cfg, workload=w, resident_blocks=resident, **kw)
+    b = sims.MultiGridSim(cfg, workload=w, resident_blocks=0, **kw)
+    return a, b
+
+
+def _step(f, *sims_):
+    """one frame per sim, each finished before the next starts: a resident
+    launch needs its parts on the device at once, so no other context's
+    kernels may run beside it"""
+    for s in sims_:
+        s.step_frame(f)
+        s.engine.synchronize()
+
+
+def _same(a, b, label):
+    p, q = a.positions_np(), b.positions_np()
+    nd = int(np.count_nonzero(p != q))
+    print(label, "differing coordinates", nd, "max |dx|", float(np.abs(p - q).max()), flush=True)
+    assert np.all(np.isfinite(p))
+    assert nd == 0
+
+
+@pytest.mark.parametrize("resident", [-1, 2])
+def test_resident_frames_equal_block_launches(resident):
+    """ARAP-SVD + volume with damping, tip pins, bone SDFs: 3 frames of 10 x 10
+    in the frame graph, resident (auto and two blocks per part) against the
+    per-iteration launches, and against the oracle"""
+    cfg = _cfg("humanv2_scale.json")
+    w = workloads.humanv2_scale(1, n_bones=2, per_bone=4, target_tets=12000, fascia_tets=3000, sdf_res=48)
+    a, b = _pair(cfg, w, resident)
+    ref = sims.MultiGridSim(cfg, workload=w, engine_cls=oracle.OracleSim)
+    print("parts", a.engine.stats(), flush=True)
+    for f in (1, 2, 3):
+        _step(f, a, b, ref)
+        _same(a, b, f"frame {f}")
+    err = float(np.abs(a.positions_np() - ref.positions_np()).max())
+    print("vs oracle", err, flush=True)
+    assert err <= 1e-4 * _diag(w.pos0)
+
+
+def test_resident_snh_aniso_activation_equals_block_launches():
+    """SNH + aniso_arap with an activation ramp (no damping path of ARAP)"""
+    cfg = _cfg("humanv2_snh_aniso.json", use_multigrid=False)
+    w = workloads.humanv2_scale(3, n_bones=1, per_bone=4, target_tets=8000, fascia_tets=1500, sdf_res=32)
+    a, b = _pair(cfg, w, -1)
+    for f in (1, 2, 3):
+        act = min(1.0, f / 3.0)
+        for s in (a, b):
+            s.set_fiber_activation(np.full(w.nt, act, np.float32))
+        _step(f, a, b)
+        _same(a, b, f"snh+aniso frame {f}")
+
+
+def test_resident_attach_equals_block_launches():
+    """belly attach on every non-pinned muscle vertex (humanV2.json's attach,
+    k = 1e6, damping 0.1)"""
+    cfg = _cfg("humanv2_scale.json")
+    cfg["constraints"] = cfg["constraints"] + [
+        {"type": "attach", "name": "belly_attach", "enable": True, "stiffness": 1e6, "dampingratio": 0.1}]
+    w = workloads.humanv2_scale(2, n_bones=2, per_bone=3, target_tets=6000, fascia_tets=1000, sdf_res=32)
+    a, b = _pair(cfg, w, -1)
+    assert a.engine.stats()["feature_mask"] & 32
+    for f in (1, 2, 3):
+        _step(f, a, b)
+        _same(a, b, f"attach frame {f}")
+
+
+def test_resident_iterate_and_substep_calls():
+    """the phase calls use the resident launch too: iterate(n) odd and even,
+    one substep, then a frame"""
+    cfg = _cfg("humanv2_scale.json")
+    w = workloads.humanv2_scale(1, n_bones=1, per_bone=4, target_tets=6000, fascia_tets=1000, sdf_res=32)
+    a, b = _pair(cfg, w, -1)
+    for s in (a, b):
+        s.engine.integrate()
+        s.engine.iterate(3)
+        s.engine.iterate(4)
+        s.engine.synchronize()
+    _same(a, b, "iterate 3 + 4")
+    _step(1, a, b)
+    _same(a, b, "frame after iterate")
+
+
+def test_resident_full_size_equals_block_launches():
+    """534,882 tets (the bench's scene): two frames bit for bit, and the
+    iteration timing runs on the resident launch"""
+    cfg = _cfg("humanv2_scale.json")
+    w = sims.build_workload(cfg)
+    a, b = _pair(cfg, w, -1)
+    st = a.engine.stats()
+    print("parts", st, flush=True)
+    for f in (1, 2):
+        _step(f, a, b)
+        _same(a, b, f"full size frame {f}")
+    us = a.engine.time_iterations(10, 4)
+    print("resident us per iteration", us, flush=True)
+    assert us > 0
+    _same(a, b, "after the iteration timing")
