@@ -8,10 +8,13 @@ iteration, no replication, every corner correction goes through memory); and
 the "block" strategy (consecutive tets per workgroup, corner sums in LDS,
 only vertices shared between blocks go through memory) at several block
 sizes and thread counts, one launch per iteration or resident (every plain
-iteration of a substep in one launch, parts of 1-3 blocks keeping their
-vertices in LDS; the iteration unroll is the whole substep).  Each candidate layout is built for the actual mesh,
+iteration of a substep in one launch, one block per part keeping its
+vertices in LDS; the iteration unroll is the whole substep).  Parts of
+several blocks fit larger meshes but lose to the per-iteration launches on
+them (fewer waves per SIMD), so only one block per part is weighed: it fits
+meshes up to about 250 k tets, where it is the faster launch.  Each candidate layout is built for the actual mesh,
 100 iterations are captured in one HIP graph and launched between two HIP
-events (as they run inside a frame), and the fastest is kept (candidates
+events (as they run inside a frame) from the same deformed state, and the fastest is kept (candidates
 within 2 % of it are ties, which go to the largest block).  Results are cached per mesh signature in
 ``rlmusclev2_amd/_tune/cache.json``.
 """
@@ -32,8 +35,7 @@ CANDIDATES = ((1, 256, 0, 0, 0), (0, 64, 0, 0, 0), (0, 128, 0, 0, 0), (0, 256, 0
               (2, 256, 128, 128, 0), (2, 256, 192, 192, 0), (2, 256, 256, 256, 0), (2, 256, 384, 384, 0),
               (2, 256, 512, 256, 0), (2, 256, 512, 512, 0), (2, 256, 768, 384, 0), (2, 256, 1024, 512, 0),
               (2, 256, 2048, 512, 0),
-              (2, 256, 256, 256, 1), (2, 256, 256, 256, 2), (2, 256, 256, 256, 3),
-              (2, 256, 128, 128, 2), (2, 256, 192, 192, 2))
+              (2, 256, 128, 128, 1), (2, 256, 192, 192, 1), (2, 256, 256, 256, 1))
 
 
 def name(c) -> str:
@@ -73,16 +75,35 @@ def _load():
         return {}
 
 
-def time_candidates(cfg: dict, w, candidates=CANDIDATES, iters: int = 20) -> dict:
+def deformed_state(w, seed: int = 5) -> np.ndarray:
+    """rest positions with every vertex moved by up to 5 % of the mean tet edge:
+    a state whose closest rotations take a few polar steps, as in a frame (at
+    rest every projection is cheap, which favours launch shapes with less
+    arithmetic parallelism)"""
+    p = np.asarray(w.pos0, np.float32).reshape(-1, 3)
+    t = np.asarray(w.tets, np.int64).reshape(-1, 4)
+    sample = t[:: max(1, len(t) // 20000)]
+    edge = float(np.mean(np.linalg.norm(p[sample[:, 0]] - p[sample[:, 1]], axis=1)))
+    rng = np.random.default_rng(seed)
+    return (p + rng.uniform(-0.05 * edge, 0.05 * edge, p.shape)).astype(np.float32)
+
+
+def time_candidates(cfg: dict, w, candidates=CANDIDATES, iters: int = 0) -> dict:
+    """us per iteration of each candidate from the same deformed state, in launches
+    of the config's iterations per substep (what a frame runs)"""
     from .sims import MultiGridSim
+    n = iters or int(cfg.get("fine_iterations", 10))
+    n = max(2, n + (n & 1))
+    start = deformed_state(w)
     out = {}
     for c in candidates:
         try:
             sim = MultiGridSim(cfg, workload=w, **as_kwargs(c))
         except XpbdError:      # e.g. the layout needs more LDS than a workgroup can have
             continue
+        sim.engine.set_state(pos=start, pprev=start)
         sim.engine.iterate(2)
-        out[c] = sim.engine.time_iterations(iters, 5)
+        out[c] = sim.engine.time_iterations(n, max(1, 100 // n))
         sim.engine.close()
         print(f"autotune {name(c)}: {out[c]:.2f} us/iteration", file=sys.stderr, flush=True)
     return out

@@ -14,8 +14,9 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import builder, collision, multigrid, workloads
+from . import builder, capi, collision, multigrid, workloads
 from .builder import ConsTable
+from .capi import XpbdError
 from .config import find_constraint
 from .engine import XpbdEngine
 from .scene import BoneMeshCollider, Scene, SdfColliders, SimParams
@@ -106,7 +107,7 @@ class FasciaShell:
         self.scene = (fascia_scene(cfg, fpos, ftets, parent.dt) if self.mode == "ride"
                       else fascia_scene_squeeze(cfg, fpos, ftets, parent.dt))
         self.NV, self.NT = self.scene.nv, self.scene.nt
-        self.engine = (engine_cls(self.scene, device=device, **tuning) if engine_cls is XpbdEngine
+        self.engine = (self._shell_engine(device, tuning) if engine_cls is XpbdEngine
                        else engine_cls(self.scene))
         glue = find_constraint(cfg, "fascia_glue") or {}
         self.iters = int(glue.get("iterations", parent.fine_iters))
@@ -117,6 +118,21 @@ class FasciaShell:
         else:
             link = self._squeeze_link(parent, np.asarray(fpos, np.float32), glue)
         parent.engine.set_fascia(self.engine, link)
+
+    def _shell_engine(self, device, tuning):
+        """The shell with the parent's launch shape; with 256-thread blocks its plain
+        iterations run resident (every iteration of a shell substep in one launch,
+        the smallest parts that fit) unless ``resident_blocks`` was given: a shell
+        is small, and small meshes gain most from one launch per substep."""
+        t = dict(tuning)
+        if (t.get("strategy") == 2 and t.get("block_tets") == t.get("block_threads") and
+                int(t.get("block_threads", 0)) <= 256 and not t.get("resident_blocks")):
+            try:
+                return XpbdEngine(self.scene, device=device, **dict(t, resident_blocks=-1))
+            except XpbdError as e:
+                if e.code != capi.XPBD_ELAYOUT:
+                    raise
+        return XpbdEngine(self.scene, device=device, **t)
 
     def _squeeze_link(self, parent, fpos, glue):
         """_build_coupling / _pair_surface (fascia.py:354-441): every muscle surface vertex
