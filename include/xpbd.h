@@ -161,6 +161,16 @@ int32_t xpbd_set_bone_transforms(xpbd_ctx *ctx, int32_t nbones, const float *xfo
 int32_t xpbd_finalize(xpbd_ctx *ctx, const xpbd_tuning *tuning);
 int32_t xpbd_get_stats(xpbd_ctx *ctx, xpbd_stats *out);
 
+/* Closest point of each query point on a triangle mesh among the faces nearer than max_dist, on the
+ * host (no context, no GPU): per point the face (-1: none), the barycentric weights (3 floats) of the
+ * closest point on (tri[0], tri[1], tri[2]) and the signed distance along the face normal
+ * (B - A) x (C - A); near-ties go to the lowest face index.  Replaces the bind queries of
+ * bone_bind_kernel (src/muscle/mg/kernel.py:288-312: a muscle vertex bound to its bone surface
+ * for target_mode "mesh") and fascia_bind_kernel (src/muscle/fascia/fascia.py:135-157), which
+ * call Warp's mesh_query_point. */
+int32_t xpbd_mesh_bind(int32_t nv, const float *verts, int32_t ntris, const int32_t *tris, int32_t npts,
+                       const float *pts, float max_dist, int32_t *face, float *bary, float *gap);
+
 /* Two-level hierarchical correction once per substep, after integrate and before the fine
  * iterations (MultiGridSim._hpbd_vcycle, src/muscle/mg/mg.py:1064-1079; coarse sweeps
  * _coarse_arap_sweep mg.py:939-952; restriction restrict_galerkin_scatter_kernel +

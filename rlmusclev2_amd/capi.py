@@ -115,6 +115,7 @@ SIGNATURES = {
     "xpbd_set_bone_transforms": [_P, _I, _P],
     "xpbd_finalize": [_P, C.POINTER(XpbdTuning)],
     "xpbd_get_stats": [_P, C.POINTER(XpbdStats)],
+    "xpbd_mesh_bind": [_I, _P, _I, _P, _I, _P, C.c_float, _P, _P, _P],
     "xpbd_set_state": [_P, _P, _P, _P],
     "xpbd_get_state": [_P, _P, _P, _P],
     "xpbd_integrate": [_P],

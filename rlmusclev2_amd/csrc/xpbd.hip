@@ -5314,7 +5314,113 @@ int set_fascia_squeeze(xpbd_ctx *c, xpbd_ctx *f, const xpbd_fascia_desc *d) {
     return XPBD_OK;
 }
 
+// closest point of each query point on a triangle mesh among the faces nearer than max_dist,
+// found with the bone-mesh tree on the host, then exact double distances over every face within
+// the float result plus slack; at near-ties (faces within 1e-6 relative of the nearest distance)
+// the lowest triangle index wins, so the choice does not depend on the tree.  Per point: the face
+// (-1: none within max_dist), the barycentric weights of the closest point on (tri[0], tri[1],
+// tri[2]) and the signed distance along the face normal (B - A) x (C - A).
+std::string mesh_bind_host(int nv, const float *pos, int ntri, const int32_t *tris, int npts, const float *pts,
+                           float max_dist, int32_t *face_out, float *w_out, float *gap_out) {
+    if (nv < 3 || ntri < 1 || !pos || !tris) return "no mesh";
+    if (!(max_dist > 0.0f)) return "bad max_dist";
+    for (int64_t i = 0; i < 3 * (int64_t)ntri; ++i)
+        if (tris[i] < 0 || tris[i] >= nv) return "triangle vertex out of range";
+    BmTables T;
+    const int32_t off[2] = {0, ntri};
+    std::string e = bm_build_tables(nv, pos, ntri, tris, 1, off, T);
+    if (!e.empty()) return e;
+    std::vector<bm_f4> bp(nv + 1), soup(3 * (size_t)ntri), fn(ntri), vn(nv), node(2 * (size_t)T.nnodes);
+    for (int i = 0; i < nv; ++i) bp[i] = bm_f4{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], 0.f};
+    const int32_t goff[2] = {0, 1}, vg0 = 0;
+    BoneMeshDev m{};
+    m.nbones = 1; m.ntris = ntri; m.nbv = nv;
+    m.bone = T.bone.data(); m.tv = T.tv.data(); m.adj = T.adj.data(); m.sub = T.sub.data();
+    m.vt_off = T.vt_off.data(); m.vt = T.vt.data(); m.pts = bp.data(); m.soup = soup.data(); m.fn = fn.data();
+    m.vn = vn.data(); m.node = node.data(); m.vgroup = &vg0; m.goff = goff;
+    m.max_dist = max_dist;
+    bm_refit_host(m);
+    auto P = [&](int v) { return D3{pos[3 * v], pos[3 * v + 1], pos[3 * v + 2]}; };
+    const double md2 = (double)max_dist * max_dist;
+    std::vector<int> stack, depth;
+    std::vector<std::pair<double, int>> cand;
+    for (int i = 0; i < npts; ++i) {
+        face_out[i] = -1;
+        w_out[3 * i] = w_out[3 * i + 1] = w_out[3 * i + 2] = 0.f;
+        gap_out[i] = 0.f;
+        const D3 p{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+        BmHit h;
+        float sg;
+        if (!bm_query(m, 0, bm_mk((float)p.x, (float)p.y, (float)p.z), h, sg)) continue;
+        // every face whose box is within the float result plus slack, then exact distances
+        const double r2 = std::min(md2, (double)h.d2 * (1.0 + 1e-3) + 1e-12);
+        const bm_i4 bn = T.bone[0];
+        const int D = bm_depth(bn.y);
+        double best = INFINITY;
+        cand.clear();
+        stack.assign(1, 0);
+        depth.assign(1, 0);
+        while (!stack.empty()) {
+            const int k = stack.back(), dk = depth.back();
+            stack.pop_back();
+            depth.pop_back();
+            if ((double)bm_box_d2(node.data() + 2 * (size_t)bn.x, k, bm_mk((float)p.x, (float)p.y, (float)p.z)) >
+                r2 * (1.0 + 1e-3) + 1e-12)
+                continue;
+            if (dk == D) {
+                const int leaf = k - (bn.y - 1);
+                for (int j = leaf * BM_LEAF; j < std::min(leaf * BM_LEAF + BM_LEAF, bn.w); ++j) {
+                    const int ft = T.tv[bn.z + j].w;
+                    const int32_t *tt = tris + 3 * (size_t)ft;
+                    const D3 q = tri_closest_d(p, P(tt[0]), P(tt[1]), P(tt[2]));
+                    const D3 dd = d3sub(p, q);
+                    const double e2 = d3dot(dd, dd);
+                    if (e2 < md2) {
+                        cand.push_back({e2, ft});
+                        best = std::min(best, e2);
+                    }
+                }
+            } else {
+                stack.push_back(2 * k + 1); depth.push_back(dk + 1);
+                stack.push_back(2 * k + 2); depth.push_back(dk + 1);
+            }
+        }
+        if (cand.empty()) continue;
+        int ft = -1;
+        for (auto &q : cand)
+            if (q.first <= best * (1.0 + 2e-6) + 1e-14 && (ft < 0 || q.second < ft)) ft = q.second;
+        const int32_t *tt = tris + 3 * (size_t)ft;
+        const D3 A = P(tt[0]), B = P(tt[1]), C = P(tt[2]);
+        const D3 q = tri_closest_d(p, A, B, C);
+        // barycentric weights of q (Cramer's rule), clamped onto the triangle
+        const D3 v0 = d3sub(B, A), v1 = d3sub(C, A), v2 = d3sub(q, A);
+        const double d00 = d3dot(v0, v0), d01 = d3dot(v0, v1), d11 = d3dot(v1, v1), d20 = d3dot(v2, v0),
+                     d21 = d3dot(v2, v1), den = d00 * d11 - d01 * d01;
+        double wb = 0, wc = 0;
+        if (den > 0) { wb = (d11 * d20 - d01 * d21) / den; wc = (d00 * d21 - d01 * d20) / den; }
+        wb = std::max(0.0, wb);
+        wc = std::max(0.0, wc);
+        if (wb + wc > 1.0) { const double s = wb + wc; wb /= s; wc /= s; }
+        const double wa = 1.0 - wb - wc;
+        D3 nrm = d3cross(v0, v1);
+        const double nl = std::sqrt(d3dot(nrm, nrm));
+        nrm = nl > 0 ? D3{nrm.x / nl, nrm.y / nl, nrm.z / nl} : D3{0, 0, 0};
+        face_out[i] = ft;
+        w_out[3 * i] = (float)wa; w_out[3 * i + 1] = (float)wb; w_out[3 * i + 2] = (float)wc;
+        gap_out[i] = (float)d3dot(d3sub(p, q), nrm);
+    }
+    return std::string();
+}
+
 }  // namespace
+
+int32_t xpbd_mesh_bind(int32_t nv, const float *verts, int32_t ntris, const int32_t *tris, int32_t npts,
+                       const float *pts, float max_dist, int32_t *face, float *bary, float *gap) {
+    if (nv < 3 || ntris < 1 || npts < 0 || !verts || !tris || (npts > 0 && (!pts || !face || !bary || !gap)))
+        return XPBD_EINVAL;
+    const std::string e = mesh_bind_host(nv, verts, ntris, tris, npts, pts, max_dist, face, bary, gap);
+    return e.empty() ? XPBD_OK : XPBD_EINVAL;
+}
 
 int32_t xpbd_set_fascia(xpbd_ctx *c, xpbd_ctx *f, const xpbd_fascia_desc *d) {
     if (!c) return XPBD_EINVAL;
@@ -5339,98 +5445,34 @@ int32_t xpbd_set_fascia(xpbd_ctx *c, xpbd_ctx *f, const xpbd_fascia_desc *d) {
         if (d->surf_tris[i] < 0 || d->surf_tris[i] >= c->nv)
             return fail(c, XPBD_EINVAL, "surface triangle vertex out of range");
     // the bind (fascia_bind_kernel, fascia.py:135-157): the closest point of the rest surface to
-    // each fascia rest vertex among the faces nearer than bind_max_dist, found with the bone-mesh
-    // tree on the host; at near-ties (two faces within 1e-6 relative of the nearest distance) the
-    // lowest triangle index wins, so that the choice does not depend on the tree
+    // each fascia rest vertex among the faces nearer than bind_max_dist
     const int ns = d->nsurf, nfv = f->nv;
-    BmTables T;
-    const int32_t off[2] = {0, ns};
-    std::string e = bm_build_tables(c->nv, c->pos0.data(), ns, d->surf_tris, 1, off, T);
-    if (!e.empty()) return fail(c, XPBD_EINVAL, e);
-    std::vector<bm_f4> pts(c->nv + 1), soup(3 * (size_t)ns), fn(ns), vn(c->nv), node(2 * (size_t)T.nnodes);
-    for (int i = 0; i < c->nv; ++i) pts[i] = bm_f4{c->pos0[3 * i], c->pos0[3 * i + 1], c->pos0[3 * i + 2], 0.f};
-    const int32_t goff[2] = {0, 1}, vg0 = 0;
-    BoneMeshDev m{};
-    m.nbones = 1; m.ntris = ns; m.nbv = c->nv;
-    m.bone = T.bone.data(); m.tv = T.tv.data(); m.adj = T.adj.data(); m.sub = T.sub.data();
-    m.vt_off = T.vt_off.data(); m.vt = T.vt.data(); m.pts = pts.data(); m.soup = soup.data(); m.fn = fn.data();
-    m.vn = vn.data(); m.node = node.data(); m.vgroup = &vg0; m.goff = goff;
-    m.max_dist = d->bind_max_dist;
-    bm_refit_host(m);
-    auto P = [&](int v) { return D3{c->pos0[3 * v], c->pos0[3 * v + 1], c->pos0[3 * v + 2]}; };
     c->fb_face.assign(nfv, -1);
     c->fb_ok.assign(nfv, 0);
     c->fb_w.assign(3 * (size_t)nfv, 0.f);
     c->fb_gap.assign(nfv, 0.f);
+    {
+        std::string e = mesh_bind_host(c->nv, c->pos0.data(), ns, d->surf_tris, nfv, f->pos0.data(),
+                                       d->bind_max_dist, c->fb_face.data(), c->fb_w.data(), c->fb_gap.data());
+        if (!e.empty()) return fail(c, XPBD_EINVAL, e);
+    }
+    auto P = [&](int v) { return D3{c->pos0[3 * v], c->pos0[3 * v + 1], c->pos0[3 * v + 2]}; };
     std::vector<int32_t> bvid;
     std::vector<int4> btri;
     std::vector<float4> bw;
     std::vector<float> tgt;
-    const double md2 = (double)d->bind_max_dist * d->bind_max_dist;
-    std::vector<int> stack;
     for (int i = 0; i < nfv; ++i) {
-        const D3 p{f->pos0[3 * i], f->pos0[3 * i + 1], f->pos0[3 * i + 2]};
-        BmHit h;
-        float sg;
-        if (!bm_query(m, 0, bm_mk((float)p.x, (float)p.y, (float)p.z), h, sg)) continue;
-        // every face whose box is within the float result plus slack, then exact distances
-        const double r2 = std::min(md2, (double)h.d2 * (1.0 + 1e-3) + 1e-12);
-        const bm_i4 bn = T.bone[0];
-        const int D = bm_depth(bn.y);
-        double best = INFINITY;
-        std::vector<std::pair<double, int>> cand;
-        stack.assign(1, 0);
-        std::vector<int> depth(1, 0);
-        while (!stack.empty()) {
-            const int k = stack.back(), dk = depth.back();
-            stack.pop_back();
-            depth.pop_back();
-            if ((double)bm_box_d2(node.data() + 2 * (size_t)bn.x, k, bm_mk((float)p.x, (float)p.y, (float)p.z)) >
-                r2 * (1.0 + 1e-3) + 1e-12)
-                continue;
-            if (dk == D) {
-                const int leaf = k - (bn.y - 1);
-                for (int j = leaf * BM_LEAF; j < std::min(leaf * BM_LEAF + BM_LEAF, bn.w); ++j) {
-                    const int ft = T.tv[bn.z + j].w;
-                    const int32_t *tt = d->surf_tris + 3 * (size_t)ft;
-                    const D3 q = tri_closest_d(p, P(tt[0]), P(tt[1]), P(tt[2]));
-                    const D3 dd = d3sub(p, q);
-                    const double e2 = d3dot(dd, dd);
-                    if (e2 < md2) {
-                        cand.push_back({e2, ft});
-                        best = std::min(best, e2);
-                    }
-                }
-            } else {
-                stack.push_back(2 * k + 1); depth.push_back(dk + 1);
-                stack.push_back(2 * k + 2); depth.push_back(dk + 1);
-            }
-        }
-        if (cand.empty()) continue;
-        int ft = -1;
-        for (auto &q : cand)
-            if (q.first <= best * (1.0 + 2e-6) + 1e-14 && (ft < 0 || q.second < ft)) ft = q.second;
+        const int ft = c->fb_face[i];
+        if (ft < 0) continue;
         const int32_t *tt = d->surf_tris + 3 * (size_t)ft;
+        const double wa = c->fb_w[3 * i], wb = c->fb_w[3 * i + 1], wc = c->fb_w[3 * i + 2];
+        const double gap = c->fb_gap[i];
         const D3 A = P(tt[0]), B = P(tt[1]), C = P(tt[2]);
-        const D3 q = tri_closest_d(p, A, B, C);
-        // barycentric weights of q (Cramer's rule), clamped onto the triangle
-        const D3 v0 = d3sub(B, A), v1 = d3sub(C, A), v2 = d3sub(q, A);
-        const double d00 = d3dot(v0, v0), d01 = d3dot(v0, v1), d11 = d3dot(v1, v1), d20 = d3dot(v2, v0),
-                     d21 = d3dot(v2, v1), den = d00 * d11 - d01 * d01;
-        double wb = 0, wc = 0;
-        if (den > 0) { wb = (d11 * d20 - d01 * d21) / den; wc = (d00 * d21 - d01 * d20) / den; }
-        wb = std::max(0.0, wb);
-        wc = std::max(0.0, wc);
-        if (wb + wc > 1.0) { const double s = wb + wc; wb /= s; wc /= s; }
-        const double wa = 1.0 - wb - wc;
-        D3 nrm = d3cross(v0, v1);
+        const D3 q{wa * A.x + wb * B.x + wc * C.x, wa * A.y + wb * B.y + wc * C.y, wa * A.z + wb * B.z + wc * C.z};
+        D3 nrm = d3cross(d3sub(B, A), d3sub(C, A));
         const double nl = std::sqrt(d3dot(nrm, nrm));
         nrm = nl > 0 ? D3{nrm.x / nl, nrm.y / nl, nrm.z / nl} : D3{0, 0, 0};
-        const double gap = d3dot(d3sub(p, q), nrm);
-        c->fb_face[i] = ft;
         c->fb_ok[i] = 1;
-        c->fb_w[3 * i] = (float)wa; c->fb_w[3 * i + 1] = (float)wb; c->fb_w[3 * i + 2] = (float)wc;
-        c->fb_gap[i] = (float)gap;
         bvid.push_back(i);
         btri.push_back(make_int4(c->L.iperm[tt[0]], c->L.iperm[tt[1]], c->L.iperm[tt[2]], 0));
         bw.push_back(make_float4((float)wa, (float)wb, (float)wc, (float)gap));

@@ -173,10 +173,16 @@ def multigrid_volume(scene) -> float:
     return float(np.abs(volumes[valid]).sum())
 
 
-def build_workload(cfg: dict) -> workloads.Workload:
+def build_workload(cfg: dict, base=None) -> workloads.Workload:
+    """A synthetic workload from the config's ``synthetic`` block, or the
+    reference's asset-driven inputs (``rest_path`` / ``geo_dir``, ``cons_dir``,
+    ``bone_dir``; ``assets.from_assets``, paths relative to ``base``)."""
     syn = cfg.get("synthetic")
+    if not syn and ("rest_path" in cfg or "geo_dir" in cfg):
+        from . import assets
+        return assets.from_assets(cfg, base)
     if not syn:
-        raise ValueError("config has no 'synthetic' block and no mesh arrays were given")
+        raise ValueError("config has no 'synthetic' block, no rest_path / geo_dir and no mesh arrays were given")
     name = syn["workload"]
     seed = int(syn.get("seed", 0))
     if name == "cantilever":
@@ -227,7 +233,14 @@ def scene_from_config(cfg: dict, w: workloads.Workload) -> Scene:
     # kinematic pins (muscleToBonePin); synthetic meshes carry the tendon-tip set
     pin_vids = pin_active = None
     pin = find_constraint(cfg, "muscleToBonePin")
-    if pin is not None and w.pin_vids is not None and w.stopped is None:
+    asset = w.info.get("assets")
+    if asset is not None:
+        # the constrained vertices of the asset's targets, hard-pinned by the region rule
+        # (mg.py:486-536); the others ride the belly attach
+        from . import assets
+        pin_vids = np.asarray(w.pin_vids, np.int32)
+        pin_active = assets.pin_active(pin, w)
+    elif pin is not None and w.pin_vids is not None and w.stopped is None:
         pin_vids = w.pin_vids
         region = str(pin.get("region", "tips")).lower()
         pin_active = np.ones(len(pin_vids), np.int32) if region != "none" else np.zeros(len(pin_vids), np.int32)
@@ -236,7 +249,14 @@ def scene_from_config(cfg: dict, w: workloads.Workload) -> Scene:
     # separate sub-solver in the reference and gets none)
     attach_vids = None
     belly = find_constraint(cfg, "belly_attach")
-    if belly is not None:
+    if belly is not None and asset is not None:
+        # the constrained vertices that are not hard-pinned, in constraint order
+        # (_build_fine_belly_attach, mg.py:700-734)
+        attach_vids = pin_vids[pin_active == 0].astype(np.int32)
+        att = builder.create_attach(attach_vids, pos0[attach_vids], float(belly["stiffness"]),
+                                    float(belly.get("dampingratio", 0.0)))
+        cons = builder.ConsTable.concat([cons, att]).sorted_by_type().contiguous()
+    elif belly is not None:
         free = np.ones(w.nv, bool)
         if w.is_fascia is not None:
             free &= ~np.asarray(w.is_fascia, bool)
@@ -255,7 +275,19 @@ def scene_from_config(cfg: dict, w: workloads.Workload) -> Scene:
     kind = str(coll.get("collider", "grid")).lower()
     if kind not in ("grid", "mesh"):
         raise ValueError(f"demo.collision.collider={kind!r}: 'grid' (bone SDF grids) or 'mesh' (bone triangle mesh)")
-    if w.bones and (coll.get("enabled", False) or kind == "mesh"):
+    bsrc = asset.get("bone_source") if asset is not None else None
+    if bsrc is not None and (coll.get("enabled", False) or kind == "mesh"):
+        # the asset's skeleton (.geo bone sequence) as the exact-mesh collider: Houdini winding,
+        # flipped to outward normals as AnimatedBoneMeshCollider does (humanV2.py:319-384)
+        m = np.ones(w.nv, np.float32)
+        bone_mesh = BoneMeshCollider(verts=np.asarray(bsrc.positions(asset["start_frame"]), np.float32),
+                                     tris=np.asarray(bsrc.tris, np.int32).reshape(-1, 3)[:, [0, 2, 1]],
+                                     tri_off=None, max_dist=float(coll.get("max_dist", 0.2)),
+                                     contact_offset=float(coll.get("contact_offset", 1e-5)),
+                                     max_push=float(coll.get("max_push", -1.0)), coll_w=m, meter_w=m.copy(),
+                                     vertex_group=None, group_bone_off=None,
+                                     resolve=bool(coll.get("enabled", False)), coherent=int(coll.get("coherent", 2)))
+    elif w.bones and (coll.get("enabled", False) or kind == "mesh"):
         m = np.ones(w.nv, np.float32)
         if w.is_fascia is not None:
             m[w.is_fascia] = 0.0   # fascia excluded from bone collision (humanV2.py:361-366)
@@ -393,6 +425,19 @@ class MultiGridSim:
         """Per-frame bone pose -> pin targets, belly-attach targets and SDF
         transforms (mg.py:1233-1249)."""
         w = self.workload
+        asset = w.info.get("assets")
+        if asset is not None:
+            # the asset's per-frame goals: cons_<frame>.geo target_pos + rest_offset, or the
+            # bone-surface binding riding bone_<frame>.geo (mg.py:1233-1249)
+            goals = asset["targets"].goals(frame)
+            if self.scene.pin_vids is not None:
+                self.engine.set_pin_targets(goals)
+            if self.scene.attach_vids is not None:
+                self.engine.set_attach_targets(goals[self.scene.pin_active == 0])
+            if self.scene.bone_mesh is not None:
+                self.engine.set_bone_mesh_points(asset["bone_source"].positions(frame))
+            self._frame = frame
+            return
         if self.scene.pin_vids is not None and w.bones:
             self.engine.set_pin_targets(workloads.pin_targets(w, frame))
         if self.scene.attach_vids is not None and w.bones:
