@@ -116,3 +116,28 @@ def test_scheduled_warm_frames_reach_plain_residual_with_fewer_iterations():
     refs = (sim.nsubsteps + 2) // 3
     iters = refs * 10 + (sim.nsubsteps - refs) * 4
     assert iters * 1.5 <= sim.nsubsteps * 10
+
+
+def test_network_beats_previous_multipliers_on_a_batch():
+    """Eight spindle instances (2k-8k tets) with per-instance activation walks,
+    ARAP + volume + aniso: trained on 150 plain frames, the network's lambda0
+    reaches the plain 10-iteration residual in fewer iterations than the
+    previous substep's multipliers (measured 5.3 against 6.25 here, 5.7 / 6.6 and
+    6.0 / 6.6 with two other seeds; the exact multipliers need 5.2)"""
+    cfg = json.loads((CFG / "batch64_c5.json").read_text())
+    cfg["demo"].pop("self_collision", None)
+    w = workloads.batch(0, n_inst=8, lo=2000, hi=8000, sdf_res=24)
+    sim = sims.MultiGridSim(cfg, workload=w, strategy=1)
+    ds = ws.collect(sim, n_frames=150, seed=0, samples_per_substep=2048, lags=(1, 2))
+    norm = ws.fit_normalizer(ds)
+    net, hist = ws.train(ds, norm, epochs=4, batch=4096, lr=2e-3, seed=0)
+    print("last epoch", hist[-1])
+    assert hist[-1]["val_loss"] < hist[-1]["prev_loss"]
+    sim.engine.close()
+    ev = sims.MultiGridSim(cfg, workload=w, strategy=1)
+    ws.install(ev.engine, net, norm)
+    out = ws.compare(ev, n_frames=4, seed=7, modes=(ws.MODE_PREV, ws.MODE_NET))
+    mean = {m: float(np.mean([11 if x is None else x for x in v])) for m, v in out.items()}
+    print("mean iterations to the plain residual", mean)
+    assert mean[ws.MODE_NET] <= mean[ws.MODE_PREV] - 0.3
+    assert mean[ws.MODE_NET] <= 10.0 / 1.5
