@@ -69,7 +69,9 @@ typedef struct {
                                      of a substep with each part's vertices in LDS (needs
                                      block_tets == block_threads <= 256 and every part resident
                                      at once, else XPBD_ELAYOUT); -1 = resident with the smallest
-                                     R that fits.  Same positions as 0, bit for bit.  The parts
+                                     R that fits; -2 = resident with as many parts as fit, the
+                                     blocks spread evenly over them.  Same positions as 0, bit
+                                     for bit.  The parts
                                      wait for each other, so no other long-running kernel may
                                      hold the device during a resident launch; a wait that runs
                                      out sets a status that xpbd_synchronize / xpbd_get_state

@@ -4037,7 +4037,7 @@ int enqueue_substep(xpbd_ctx *c, int iters, bool dcd_rebuild, int sub, bool inte
 int setup_resident(xpbd_ctx *c, int want, int maxlds) {
     c->res_R = 0;
     if (want == 0) return XPBD_OK;
-    if (want < -1 || want > 4) return fail(c, XPBD_EINVAL, "resident_blocks must be -1, 0 or 1..4");
+    if (want < -2 || want > 4) return fail(c, XPBD_EINVAL, "resident_blocks must be -2, -1, 0 or 1..4");
     if (c->block != c->block_bs || c->block_bs > 256)
         return fail(c, XPBD_ELAYOUT, "resident mode needs block_tets == block_threads <= 256");
     if (c->L.max_nloc > c->block_bs)
@@ -4048,10 +4048,15 @@ int setup_resident(xpbd_ctx *c, int want, int maxlds) {
     HIPCHK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
     const int r0 = want > 0 ? want : 1, r1 = want > 0 ? want : 4;
     std::string why = "no part size fits";
-    for (int R = r0; R <= r1; ++R) {
+    // -2: as many parts as fit at once (blocks spread evenly over them), found by
+    // shrinking the part count to the capacity its own part size allows
+    const int64_t nb = (int64_t)c->L.blocks.size();
+    int64_t target = want == -2 ? nb : 0;
+    const int tries = want == -2 ? 6 : r1 - r0 + 1;
+    for (int t = 0, R = r0; t < tries; ++t, ++R) {
         xpbd::Parts P;
         int code = 0;
-        std::string m = xpbd::build_parts(c->L, c->nv, R, P, &code);
+        std::string m = xpbd::build_parts(c->L, c->nv, want == -2 ? 0 : R, P, &code, (int)target);
         if (code) { why = m; continue; }
         const int vpt = (P.max_nloc + c->block_bs - 1) / c->block_bs;
         if (vpt > 3) {
@@ -4066,12 +4071,13 @@ int setup_resident(xpbd_ctx *c, int want, int maxlds) {
         int occ = 0;
         HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)k, c->block_bs, lds));
         if ((int64_t)occ * ncu < (int64_t)P.parts.size()) {
-            why = std::to_string(P.parts.size()) + " parts of " + std::to_string(R) + " blocks, but only " +
+            why = std::to_string(P.parts.size()) + " parts of up to " + std::to_string(P.R) + " blocks, but only " +
                   std::to_string(occ) + " x " + std::to_string(ncu) + " fit on the device at once";
+            if (want == -2) target = std::min<int64_t>((int64_t)P.parts.size() - 1, (int64_t)occ * ncu);
             continue;
         }
         c->parts = std::move(P);
-        c->res_R = R;
+        c->res_R = c->parts.R;
         c->res_lds = lds;
         c->res_vpt = std::max(vpt, 1);
         break;

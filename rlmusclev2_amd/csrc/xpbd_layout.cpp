@@ -273,13 +273,22 @@ std::string build_blocks(const HostInput &in, Layout &L, int feature, int *code)
     return std::string();
 }
 
-std::string build_parts(const Layout &L, int nv, int R, Parts &P, int *code) {
+std::string build_parts(const Layout &L, int nv, int R, Parts &P, int *code, int target_parts) {
     *code = 0;
     P = Parts();
     if (L.block_tets <= 0) { *code = 1; return "parts need the block strategy"; }
-    if (R < 1 || R > 64) { *code = 1; return "blocks per part must be in [1, 64]"; }
     const int64_t nb = (int64_t)L.blocks.size();
-    const int64_t G = (nb + R - 1) / R;
+    if (target_parts > 0) {
+        // as many parts as asked (at most one per block), the blocks spread evenly over them
+        R = (int)std::max<int64_t>(1, (nb + target_parts - 1) / target_parts);
+    }
+    if (R < 1 || R > 64) { *code = 1; return "blocks per part must be in [1, 64]"; }
+    const int64_t G = target_parts > 0 ? std::max<int64_t>(1, std::min<int64_t>(nb, target_parts)) : (nb + R - 1) / R;
+    // part p owns blocks [first(p), first(p + 1))
+    auto first = [&](int64_t p) { return target_parts > 0 ? p * nb / G : std::min<int64_t>(nb, p * R); };
+    std::vector<int32_t> part_of((size_t)nb);
+    for (int64_t p = 0; p < G; ++p)
+        for (int64_t b = first(p); b < first(p + 1); ++b) part_of[(size_t)b] = (int32_t)p;
     P.R = R;
     P.parts.assign((size_t)G, PartDesc{});
     // the parts that touch each vertex, ascending (blocks are visited in order,
@@ -287,7 +296,7 @@ std::string build_parts(const Layout &L, int nv, int R, Parts &P, int *code) {
     std::vector<int32_t> npart(nv, 0), last(nv, -1);
     for (int64_t b = 0; b < nb; ++b) {
         const BlockDesc &d = L.blocks[(size_t)b];
-        const int32_t p = (int32_t)(b / R);
+        const int32_t p = part_of[(size_t)b];
         for (int i = 0; i < d.nloc; ++i) {
             const int v = L.bvt[d.vbeg + i];
             if (last[v] != p) { last[v] = p; ++npart[v]; }
@@ -301,7 +310,7 @@ std::string build_parts(const Layout &L, int nv, int R, Parts &P, int *code) {
         std::fill(last.begin(), last.end(), -1);
         for (int64_t b = 0; b < nb; ++b) {
             const BlockDesc &d = L.blocks[(size_t)b];
-            const int32_t p = (int32_t)(b / R);
+            const int32_t p = part_of[(size_t)b];
             for (int i = 0; i < d.nloc; ++i) {
                 const int v = L.bvt[d.vbeg + i];
                 if (last[v] != p) { last[v] = p; plist[(size_t)fill[v]++] = p; }
@@ -321,8 +330,8 @@ std::string build_parts(const Layout &L, int nv, int R, Parts &P, int *code) {
     std::vector<int32_t> loc(nv, -1), verts, nb_list;
     for (int64_t p = 0; p < G; ++p) {
         PartDesc &pd = P.parts[(size_t)p];
-        pd.b0 = (int32_t)(p * R);
-        pd.nblk = (int32_t)std::min<int64_t>(R, nb - p * R);
+        pd.b0 = (int32_t)first(p);
+        pd.nblk = (int32_t)(first(p + 1) - first(p));
         verts.clear();
         for (int rr = 0; rr < pd.nblk; ++rr) {
             const BlockDesc &d = L.blocks[(size_t)(pd.b0 + rr)];

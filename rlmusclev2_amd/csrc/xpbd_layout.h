@@ -75,7 +75,7 @@ struct PartDesc {
 };
 
 struct Parts {
-    int32_t R = 0;                   // blocks per part
+    int32_t R = 0;                   // blocks per part (the most, when the parts were spread evenly)
     std::vector<PartDesc> parts;
     // per part-local entry: vertex id (new order), index into svid / soff for a
     // cross vertex (else -1), write-back flag (the part of the vertex's first
@@ -130,8 +130,9 @@ struct Layout {
 // returns empty string on success, else the error message; *code gets the status
 std::string build_layout(const HostInput &in, Layout &out, int *code);
 
-// parts of R consecutive blocks over a built block layout (L.blocks and its tables)
-std::string build_parts(const Layout &L, int nv, int R, Parts &out, int *code);
+// parts of R consecutive blocks over a built block layout (L.blocks and its tables); with
+// target_parts > 0, that many parts (at most one per block) of evenly spread block runs
+std::string build_parts(const Layout &L, int nv, int R, Parts &out, int *code, int target_parts = 0);
 
 // FP32 rest precompute in the reference's operation order (engine.py:199-224):
 // Dm columns p_k - p3, |det|/6, mass += V*density/4 (tet order), B = inv(Dm).

@@ -15,6 +15,7 @@
 using namespace xpbd;
 int main(int argc, char **argv) {
     int n = argc > 1 ? atoi(argv[1]) : 12, TB = argc > 2 ? atoi(argv[2]) : 256, R = argc > 3 ? atoi(argv[3]) : 3;
+    const int target = argc > 4 ? atoi(argv[4]) : 0;      // > 0: that many parts, blocks spread evenly
     int nx = n, ny = n / 2 + 1, nz = n / 3 + 1;
     auto vid = [&](int i, int j, int k) { return (i * (ny + 1) + j) * (nz + 1) + k; };
     std::vector<float> pos;
@@ -47,8 +48,12 @@ int main(int argc, char **argv) {
     std::string err = build_layout(in, L, &code);
     if (code) { printf("layout error %s\n", err.c_str()); return 1; }
     Parts P;
-    err = build_parts(L, nv, R, P, &code);
+    err = build_parts(L, nv, R, P, &code, target);
     if (code) { printf("parts error %s\n", err.c_str()); return 1; }
+    if (target > 0) {
+        R = P.R;
+        if ((int64_t)P.parts.size() != std::min<int64_t>(target, (int64_t)L.blocks.size())) { printf("part count\n"); return 1; }
+    }
     const size_t ne = L.meta.size();
     const int G = (int)P.parts.size();
     int bad = 0;

@@ -54,12 +54,15 @@ def parts_checker(tmp_path_factory):
     return exe
 
 
-@pytest.mark.parametrize("n,tb,r", [(7, 64, 1), (12, 256, 3), (20, 256, 4), (20, 128, 2), (30, 256, 3),
-                                    (12, 64, 64)])
-def test_resident_parts_equal_block_sums(parts_checker, n, tb, r):
-    """parts of r blocks: cover, write-back once, symmetric neighbour lists,
-    corner ids, and FP32 sums bit-identical to the block strategy's"""
-    res = subprocess.run([str(parts_checker), str(n), str(tb), str(r)], capture_output=True, text=True)
+@pytest.mark.parametrize("n,tb,r,target", [(7, 64, 1, 0), (12, 256, 3, 0), (20, 256, 4, 0), (20, 128, 2, 0),
+                                           (30, 256, 3, 0), (12, 64, 64, 0), (20, 128, 1, 7), (30, 256, 1, 23),
+                                           (12, 64, 1, 1000)])
+def test_resident_parts_equal_block_sums(parts_checker, n, tb, r, target):
+    """parts of r blocks (or a target count of parts, blocks spread evenly):
+    cover, write-back once, symmetric neighbour lists, corner ids, and FP32 sums
+    bit-identical to the block strategy's"""
+    res = subprocess.run([str(parts_checker), str(n), str(tb), str(r), str(target)], capture_output=True,
+                         text=True)
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr
     assert "bad 0" in res.stdout

@@ -51,11 +51,12 @@ def _same(a, b, label):
     assert nd == 0
 
 
-@pytest.mark.parametrize("resident", [-1, 2])
+@pytest.mark.parametrize("resident", [-1, -2, 2])
 def test_resident_frames_equal_block_launches(resident):
     """ARAP-SVD + volume with damping, tip pins, bone SDFs: 3 frames of 10 x 10
-    in the frame graph, resident (auto and two blocks per part) against the
-    per-iteration launches, and against the oracle"""
+    in the frame graph, resident (the smallest part size that fits, as many
+    parts as fit, two blocks per part) against the per-iteration launches, and
+    against the oracle"""
     cfg = _cfg("humanv2_scale.json")
     w = workloads.humanv2_scale(1, n_bones=2, per_bone=4, target_tets=12000, fascia_tets=3000, sdf_res=48)
     a, b = _pair(cfg, w, resident)
