@@ -60,6 +60,17 @@ TRAFFIC = {
     (7063998, ("tetarapsvd", "volume", "aniso_arap"), 2, 256, 256): 1005044531.0,  # 512 instances
 }
 
+# VALU lane-instructions of one projection iteration: SQ_INSTS_VALU per wave x 64 lanes x waves of the tet
+# kernel plus the shared-vertex kernel (rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU ..., 700 dispatches each;
+# profiles/projection_valu_sq_counters.md), keyed like TRAFFIC
+VALU_LANE_INSTS = {
+    (534882, ("tetarapsvd", "volume"), 2, 256, 256): 731.3 * 64 * 8360 + 124.7 * 64 * 1488,
+    (7063998, ("tetarapsvd", "volume", "aniso_arap"), 2, 256, 256): 856.3 * 64 * 110376 + 121.4 * 64 * 18556,
+}
+# one FP32 VALU instruction per SIMD every 4 cycles for a 64-lane wave (16 lanes): 256 CUs x 4 SIMDs x 16
+# lanes x 2.4 GHz (packed FP32 is kept out of the build)
+VALU_PEAK_LANE_INSTS = 256 * 4 * 16 * 2.4e9
+
 WORKLOADS = {
     "humanv2": ("humanv2_scale.json", {}),
     "humanv2_20": ("humanv2_scale.json", {"fine_iterations": 20}),
@@ -179,6 +190,7 @@ def run_frames(cfg, w, tuning, steps, warmup, label):
     key = (w.nt, types, strat, tuning.get("block_tets", 0) if strat == 2 else 0,
            tuning.get("block_threads", 0) if strat == 2 else 0)
     traffic = TRAFFIC.get(key + (st["resident_blocks"],) if res else key)
+    valu = VALU_LANE_INSTS.get(key) if not res else None
     kernel = ("k_iterate (fused Jacobi projection)", "k_flat_tets + k_flat_verts (one Jacobi iteration)",
               "k_block_tets + k_block_shared (one Jacobi iteration)")[strat]
     if res:
@@ -199,6 +211,11 @@ def run_frames(cfg, w, tuning, steps, warmup, label):
             "timing": "one HIP graph of the iteration kernels between two HIP events on the library stream",
             "iterations_timed": t_iters * t_reps, "alg_bytes_per_iteration": alg,
             "impl_bytes_per_iteration": st["bytes_per_iter_impl"],
+            # what bounds it: VALU issue (FP32 lane-instructions per iteration from SQ counters)
+            "valu": None if valu is None else {
+                "lane_instructions_per_iteration": valu, "achieved_tlane_per_s": round(valu / avg_us * 1e-6, 2),
+                "peak_tlane_per_s": round(VALU_PEAK_LANE_INSTS * 1e-12, 2),
+                "frac": round(valu / (avg_us * 1e-6) / VALU_PEAK_LANE_INSTS, 4)},
         },
         "residual_rms": sim.residuals(),
         "finite": bool(np.all(np.isfinite(sim.positions_np()))),
