@@ -576,6 +576,7 @@ class MuscleSim:
         self.dt = scene.params.dt
         self.step_cnt = 0
         self.post_smooth_iters = 0
+        self.program = None
 
     @classmethod
     def from_procedural(cls, vertices, tets, fiber_dirs_per_tet, bone_targets=None, *,
@@ -616,12 +617,59 @@ class MuscleSim:
         sim.bone_pos = bone
         return sim
 
-    def step(self):
-        """integrate_phase, solve_phase, finalize_phase per substep (musclesim.py:107-142)."""
+    # extension hooks (musclesim.py:107-142, engine.py:1039-1067): Python extras run in the
+    # solve phase, and a custom program replaces the phase list; either takes the phase-by-phase
+    # path (one C-ABI call per phase), the default program runs as one native substep
+    @property
+    def constraint_solvers(self):
+        return self.engine.constraint_solvers
+
+    @constraint_solvers.setter
+    def constraint_solvers(self, fns):
+        self.engine.constraint_solvers = list(fns)
+
+    def build_program(self):
+        """The default program as phase closures (build_program, musclesim.py:107-130)."""
+        e = self.engine
+
+        def integrate_phase():
+            e.integrate()
+            e.apply_pintotarget()
+            e.clear()
+
+        def solve_phase():
+            e.solve_constraints(run_extras=False)
+            e.apply_dP()
+            e.solve_extras()
+
+        def finalize_phase():
+            e.post_smooth(self.post_smooth_iters)
+            e.apply_pintotarget()
+            e.update_velocities()
+
+        self.program = [integrate_phase, solve_phase, finalize_phase]
+        self._default_program = list(self.program)
+        return self.program
+
+    def run_program(self):
         for _ in range(self.num_substeps):
-            self.engine.native_substep(1)
-            if self.post_smooth_iters:
-                self.engine.post_smooth(self.post_smooth_iters)
+            for phase in self.program:
+                phase()
+
+    def step(self):
+        """integrate_phase, solve_phase, finalize_phase per substep (musclesim.py:107-142):
+        one native substep each, or the phase program when extras or a custom program
+        are installed."""
+        custom = getattr(self, "program", None) is not None and self.program != getattr(self, "_default_program", None)
+        if custom or getattr(self.engine, "constraint_solvers", None):
+            if getattr(self, "program", None) is None:
+                self.build_program()
+            self.run_program()
+        else:
+            for _ in range(self.num_substeps):
+                self.engine.native_substep(1)
+                if self.post_smooth_iters:
+                    self.engine.post_smooth(self.post_smooth_iters)
         self.step_cnt += 1
 
     def reset(self):

@@ -41,3 +41,27 @@ def test_musclesim_steps_match_oracle(strategy):
     assert sim.step_cnt == 12
     sim.reset()
     np.testing.assert_array_equal(sim.positions_np(), sim.scene.pos0)
+
+
+def test_musclesim_extras_and_program_take_the_phase_path():
+    """A Python extra in constraint_solvers runs once per substep in the solve
+    phase (the phase-by-phase path), and with an extra that changes nothing the
+    frames equal the native substeps'; a custom program replaces the phase list."""
+    a = _make(lambda sc: XpbdEngine(sc, strategy=2, block_tets=128, block_threads=128))
+    b = _make(lambda sc: XpbdEngine(sc, strategy=2, block_tets=128, block_threads=128))
+    calls = []
+    b.constraint_solvers = [lambda: calls.append(1)]
+    for _ in range(4):
+        a.step()
+        b.step()
+    assert len(calls) == 4 * b.num_substeps
+    p, q = a.positions_np(), b.positions_np()
+    diag = float(np.linalg.norm(a.scene.pos0.max(0) - a.scene.pos0.min(0)))
+    print("native vs phase path max |dx|", float(np.abs(p - q).max()))
+    assert float(np.abs(p - q).max()) <= 1e-6 * diag
+    phases = []
+    b.constraint_solvers = []
+    b.build_program()
+    b.program = [lambda: phases.append("a")] + b.program
+    b.step()
+    assert phases == ["a"] * b.num_substeps
