@@ -904,7 +904,8 @@ __global__ __launch_bounds__(256) void k_block_shared(IterArgs a, BlockArgs b, c
 // "resident" mode of the block strategy: one launch runs all K plain Jacobi
 // iterations of a substep.  Runs of R consecutive blocks form a part and one
 // workgroup per part keeps the part's vertices (positions, previous positions)
-// in LDS and its elements in registers for the whole launch.  Per iteration a
+// and the class table in LDS for the whole launch, and loads each block's
+// elements one block ahead.  Per iteration a
 // part projects its blocks one after the other (one element per thread) and
 // sums every block-local vertex's corners as k_block_tets does; the sum of a
 // vertex that only this part touches goes into an LDS accumulator (ascending
@@ -916,8 +917,9 @@ __global__ __launch_bounds__(256) void k_block_shared(IterArgs a, BlockArgs b, c
 // block strategy's operands in its order, so the positions equal those of the
 // per-iteration launches bit for bit.
 //
-// Partials and iteration counts cross workgroups (and XCDs) through
-// agent-scope atomics; the partial array is double-buffered by iteration
+// Partials cross workgroups (and XCDs) as 16-byte agent-scope (sc1) stores and
+// loads, iteration counts as agent-scope atomics; a part publishes its count
+// only after its partial stores have completed.  The partial array is double-buffered by iteration
 // parity, so a part that runs ahead never overwrites a partial that a slower
 // neighbour still has to read (it waits for that neighbour's next count
 // first).  Every part of a launch must be resident at once: the host checks
