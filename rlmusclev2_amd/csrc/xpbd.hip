@@ -4046,6 +4046,7 @@ int setup_resident(xpbd_ctx *c, int want, int maxlds) {
         return fail(c, XPBD_ELAYOUT, "resident mode: a block touches more vertices than it has threads");
     if ((int64_t)c->L.n_partial > (1 << 26))
         return fail(c, XPBD_ELAYOUT, "resident mode: more than 2^26 block partials");
+    if (c->L.blocks.empty()) return fail(c, XPBD_ELAYOUT, "resident mode needs at least one element");
     int ncu = 0;
     HIPCHK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
     const int r0 = want > 0 ? want : 1, r1 = want > 0 ? want : 4;
