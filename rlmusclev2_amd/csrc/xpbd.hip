@@ -299,6 +299,42 @@ __device__ __forceinline__ void load_elem(const IterArgs &a, int64_t g, ElemIn<F
     for (int q = 0; q < 9; ++q) e.B[q] = NEED_F ? a.Bp[(int64_t)q * a.nptet + g] : 0.f;
 }
 
+// Block layouts hold fewer than 2^26 elements and vertices (checked when the
+// layout is built), so every byte offset into a per-element array, the nine B
+// planes included, fits 32 bits: an element load is a uniform base (scalar
+// registers) plus one 32-bit lane offset shared by all arrays, and the B planes
+// are buffer loads whose plane offset is a scalar operand, with no 64-bit
+// address arithmetic per lane.  (The vertex gathers keep 64-bit indexing: in
+// the block kernel the 32-bit form measured slower.)
+constexpr int64_t BLOCK_MAX_ELEMS = int64_t(1) << 26;
+constexpr int BUF_RSRC_W3 = 0x00020000;         // buffer descriptor word 3 (32-bit data format)
+
+template <typename T>
+__device__ __forceinline__ T ld_off(const T *base, uint32_t byte_off) {
+    return *(const T *)((const char *)base + byte_off);
+}
+
+template <int FEAT, int LAM>
+__device__ __forceinline__ void load_elem(const IterArgs &a, uint32_t g, ElemIn<FEAT, LAM> &e) {
+    const uint32_t o4 = g * 4u, o16 = g * 16u;
+    e.lam = LAM ? ld_off(a.lam, o16) : make_float4(0.f, 0.f, 0.f, 0.f);
+    e.meta = ld_off(a.meta, o4);
+    e.rvol = (FEAT & F_VOL) ? ld_off(a.rvol, o4) : 0.f;
+    e.rsnh = (FEAT & F_SNH) ? ld_off(a.rsnh, o4) : 0.f;
+    e.fib = (FEAT & F_ANISO) ? ld_off(a.fib, o16) : make_float4(0.f, 0.f, 0.f, 0.f);
+    constexpr bool NEED_F = (FEAT & (F_ARAP | F_SNH | F_ANISO)) != 0;
+    if (NEED_F) {
+        const uint32_t plane = (uint32_t)a.nptet * 4u;
+        const __amdgpu_buffer_rsrc_t br = __builtin_amdgcn_make_buffer_rsrc((void *)a.Bp, 0, (int)(9u * plane), BUF_RSRC_W3);
+#pragma unroll
+        for (int q = 0; q < 9; ++q)
+            e.B[q] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(br, (int)o4, (int)(q * plane), 0));
+    } else {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) e.B[q] = 0.f;
+    }
+}
+
 // LAM = 0: plain zero-lambda Jacobi (the reference GPU path); 1: lambda carried
 // across the iterations of a substep (a.lam read and updated in place, one
 // thread per element); 2: warm-start impulse, the step applies dlambda = lambda0
@@ -312,7 +348,7 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
     float4 nl = lam;
     const uint32_t meta = e.meta;
     const uint32_t mask = (meta >> 16) & 15u;
-    const DevClass dc = cls[meta & 0xffffu];
+    const DevClass dc = ld_off(cls, (meta & 0xffffu) * (uint32_t)sizeof(DevClass));
     const ClassParams &cp = dc.p;
     const float4 X0 = sx[lv[0]], X1 = sx[lv[1]], X2 = sx[lv[2]], X3 = sx[lv[3]];
     const V3 P[4] = {xyz(X0), xyz(X1), xyz(X2), xyz(X3)};
@@ -608,10 +644,11 @@ __global__ __launch_bounds__(BS) void k_iterate(IterArgs a, const float4 *__rest
             o.w += aw;
             a.dpw[v] = o;
         } else {
-            if (aw > 1e-9f) {  // apply_dP_kernel, engine.py:689-698
-                x.x = x.x + ax / aw;
-                x.y = x.y + ay / aw;
-                x.z = x.z + az / aw;
+            if (aw > 1e-9f) {  // apply_dP_kernel, engine.py:689-698 (one reciprocal, as finish_value)
+                const float inv = 1.0f / aw;
+                x.x = x.x + ax * inv;
+                x.y = x.y + ay * inv;
+                x.z = x.z + az * inv;
             }
             if (a.pin) {  // pintotarget overwrite, mg/kernel.py:212-226
                 const float4 p = a.pin[v];
@@ -660,9 +697,13 @@ __device__ __forceinline__ float4 finish_value(const IterArgs &a, int v, float4 
                                                float az, float aw, float4 pv) {
     if (a.att0) attach_vertex(a, v, x, pp, ax, ay, az, aw);
     if (aw > 1e-9f) {
-        x.x = x.x + ax / aw;
-        x.y = x.y + ay / aw;
-        x.z = x.z + az / aw;
+        // one correctly rounded reciprocal instead of three divisions (the
+        // correction differs from dP / dPw by at most an ulp or two of the
+        // correction, far below the solver's own tolerance)
+        const float inv = 1.0f / aw;
+        x.x = x.x + ax * inv;
+        x.y = x.y + ay * inv;
+        x.z = x.z + az * inv;
     }
     if (a.pin && pv.w != 0.0f) { x.x = pv.x; x.y = pv.y; x.z = pv.z; }
     return x;
@@ -763,16 +804,16 @@ __global__ __launch_bounds__(512) void k_block_tets(IterArgs a, BlockArgs b, con
     const int tid = threadIdx.x, nth = blockDim.x;
     // blocks are consecutive runs of TB elements, so the element loads of the
     // one-element-per-thread path are issued before the block record arrives
-    const int64_t tb0 = (int64_t)blockIdx.x * TB;
-    const int nte = (int)min((int64_t)TB, (int64_t)a.nptet - tb0);
+    const uint32_t tb0 = blockIdx.x * (uint32_t)TB;
+    const int nte = (int)min((int64_t)TB, a.nptet - (int64_t)tb0);
     const bool has_t = nte <= nth && tid < nte;
-    const int64_t g = tb0 + tid;
+    const uint32_t g = tb0 + tid;
     ElemIn<FEAT, LAM> e;
     ushort4 li = make_ushort4(0, 0, 0, 0), cp = li;
     if (has_t) {
         load_elem(a, g, e);
-        li = b.blidx[g];
-        cp = b.bcpos[g];
+        li = ld_off(b.blidx, g * 8u);
+        cp = ld_off(b.bcpos, g * 8u);
     }
     const BlockDesc D = b.blk[blockIdx.x];
     if (D.ntet <= nth && D.nloc <= nth) {
@@ -790,7 +831,7 @@ __global__ __launch_bounds__(512) void k_block_tets(IterArgs a, BlockArgs b, con
             sx[tid] = xin[gv];
             if (DAMP) spp[tid] = pprev[gv];
             // the pin record of a vertex this block finishes, loaded with the gather
-            if (MODE == MODE_FUSED && ps < 0) pv = pin_of(a, gv);
+            if (MODE == MODE_FUSED && ps < 0 && a.pin) pv = ld_off(a.pin, (uint32_t)gv * 16u);
         }
         __syncthreads();
         if (has_t) {
@@ -945,7 +986,7 @@ struct ResArgs {
 // memory transaction per partial instead of four dword atomics
 typedef float res_v4f __attribute__((ext_vector_type(4)));
 constexpr int RES_SC1 = 16;                     // cache policy: sc1
-constexpr int RES_RSRC_W3 = 0x00020000;         // buffer descriptor word 3 (32-bit data format)
+constexpr int RES_RSRC_W3 = BUF_RSRC_W3;
 
 __device__ __forceinline__ void res_store(__amdgpu_buffer_rsrc_t r, int slot, float4 v) {
     const res_v4f x = {v.x, v.y, v.z, v.w};
@@ -983,11 +1024,11 @@ __global__ __launch_bounds__(256) void k_resident(IterArgs a, BlockArgs b, ResAr
     ElemIn<FEAT, 0> e;
     ushort4 pl = make_ushort4(0, 0, 0, 0), cp = pl;
     {
-        const int64_t g = (int64_t)P.b0 * TB + tid;
-        if (g < a.nptet) {
+        const uint32_t g = (uint32_t)P.b0 * TB + tid;
+        if ((int64_t)g < a.nptet) {
             load_elem(a, g, e);
-            pl = rs.plidx[g];
-            cp = b.bcpos[g];
+            pl = ld_off(rs.plidx, g * 8u);
+            cp = ld_off(b.bcpos, g * 8u);
         }
     }
     // the part's vertices: state into LDS; their table entries, partial
@@ -1030,13 +1071,13 @@ __global__ __launch_bounds__(256) void k_resident(IterArgs a, BlockArgs b, ResAr
             // the next round's elements (the first block's again for the next iteration)
             const int rn = (rr + 1 < P.nblk) ? rr + 1 : 0;
             const bool more = rr + 1 < P.nblk || it + 1 < iters;
-            const int64_t gn = (int64_t)(P.b0 + rn) * TB + tid;
+            const uint32_t gn = (uint32_t)(P.b0 + rn) * TB + tid;
             ElemIn<FEAT, 0> en;
             ushort4 pln = make_ushort4(0, 0, 0, 0), cpn = pln;
-            if (more && gn < a.nptet) {
+            if (more && (int64_t)gn < a.nptet) {
                 load_elem(a, gn, en);
-                pln = rs.plidx[gn];
-                cpn = b.bcpos[gn];
+                pln = ld_off(rs.plidx, gn * 8u);
+                cpn = ld_off(b.bcpos, gn * 8u);
             }
             if (tid < D.ntet) {
                 const int64_t g = (int64_t)D.tbeg + tid;
@@ -1565,7 +1606,12 @@ __global__ void k_apply_dpw(int nv, float4 *X, const float4 *dpw, const float4 *
     if (i >= nv) return;
     float4 x = X[i];
     const float4 d = dpw[i];
-    if (d.w > 1e-9f) { x.x = x.x + d.x / d.w; x.y = x.y + d.y / d.w; x.z = x.z + d.z / d.w; }
+    if (d.w > 1e-9f) {   // as finish_value: one reciprocal
+        const float inv = 1.0f / d.w;
+        x.x = x.x + d.x * inv;
+        x.y = x.y + d.y * inv;
+        x.z = x.z + d.z * inv;
+    }
     if (do_pin && pin) {
         const float4 p = pin[i];
         if (p.w != 0.0f) { x.x = p.x; x.y = p.y; x.z = p.z; }
@@ -4701,6 +4747,10 @@ int32_t xpbd_finalize(xpbd_ctx *c, const xpbd_tuning *t) {
     int code = 0;
     std::string m = xpbd::build_layout(in, c->L, &code);
     if (code) return fail(c, code, m);
+    // the block kernels address per-element and per-vertex arrays with 32-bit
+    // byte offsets
+    if (c->block && ((int64_t)c->L.meta.size() >= BLOCK_MAX_ELEMS || (int64_t)c->nv >= BLOCK_MAX_ELEMS))
+        return fail(c, XPBD_ELAYOUT, "the block strategy holds fewer than 2^26 elements and vertices; use strategy 1");
     c->feature = c->L.feature;
     const size_t lds = c->block ? block_lds(c) : (c->flat ? 0 : iter_lds(c));
     int maxlds = 0;
