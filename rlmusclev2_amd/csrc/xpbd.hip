@@ -268,6 +268,32 @@ __device__ __forceinline__ float arap_grad(const float *F, const float *B, const
     return C;
 }
 
+// arap_grad without the early return: a satisfied constraint gives C = 0 and
+// zero gradients (wsum = 0), so the caller can select its update instead of
+// branching around it
+__device__ __forceinline__ float arap_grad_sel(const float *F, const float *B, const float w[4], V3 gr[4],
+                                               float &wsum) {
+    float R[9];
+    closest_rotation(F, R);
+    float G[9], c2s = 0.f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        G[i] = F[i] - R[i];
+        c2s += G[i] * G[i];
+    }
+    const bool sat = c2s < 1.0e-12f;
+    const float C = sat ? 0.0f : __builtin_amdgcn_sqrtf(c2s);
+    const float invC = sat ? 0.0f : __builtin_amdgcn_rcpf(C);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) G[i] *= invC;
+    gr[0] = grad_row(G, B, 0);
+    gr[1] = grad_row(G, B, 1);
+    gr[2] = grad_row(G, B, 2);
+    gr[3] = -gr[0] - gr[1] - gr[2];
+    wsum = w[0] * dot(gr[0], gr[0]) + w[1] * dot(gr[1], gr[1]) + w[2] * dot(gr[2], gr[2]) + w[3] * dot(gr[3], gr[3]);
+    return C;
+}
+
 // ---------------------------------------------------------------------------
 // per-element projection (all constraint types of one tet, Jacobi, lambda = 0)
 // ---------------------------------------------------------------------------
@@ -363,7 +389,13 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
     // pressure); the corrections of one element are summed per type, so the
     // order does not change the result
     // ---- volume: tet_volume_update_xpbd_fn, solve_kernels.py:187-281 ----
-    if ((FEAT & F_VOL) && (mask & M_VOL)) {
+    //
+    // Computed without branches: every quantity is finite for finite positions,
+    // and a constraint that the reference skips (wsum <= 1e-9, stiffness 0, type
+    // absent) gets dl = 0 by a select, so the corrections equal the branched
+    // form's (up to the sign of a zero) while the wave issues no exec-mask
+    // bookkeeping and keeps no zero copies of the accumulators live.
+    if (FEAT & F_VOL) {
         constexpr float SIXTH = 1.0f / 6.0f;
         V3 gr[4];
         gr[0] = SIXTH * cross(c1, c2);
@@ -372,36 +404,37 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
         gr[3] = -(gr[0] + gr[1] + gr[2]);
         const float wsum = w[0] * dot(gr[0], gr[0]) + w[1] * dot(gr[1], gr[1]) +
                            w[2] * dot(gr[2], gr[2]) + w[3] * dot(gr[3], gr[3]);
-        if (wsum > 1e-9f) {
-            const float vol = SIXTH * dot(cross(c0, c1), c2);
-            const float rest = e.rvol;
-            const bool comp = a.sp.has_compress && (vol < rest);
-            const float k = comp ? cp.kcomp_vol : cp.k_vol;
-            if (k != 0.0f) {
-                const float alpha = comp ? dc.a_comp : dc.a_vol;
-                const float C = vol - rest;
-                float dsum = 0.f, gamma = 1.f;
-                if (DAMP && cp.damp_vol > 0.0f) {
-                    gamma = comp ? dc.g_comp : dc.g_vol;
-                    for (int i = 0; i < 4; ++i) {
-                        const float4 pp = spp[lv[i]];
-                        dsum += dot(gr[i], P[i] - xyz(pp));
-                    }
-                    dsum = dsum * gamma;
-                    gamma = gamma + 1.0f;
-                }
-                const float dl = LAM == 2 ? lam.y : (-C - alpha * lam.y - dsum) * __builtin_amdgcn_rcpf(gamma * wsum + alpha);
-                if (LAM == 1) nl.y = lam.y + dl;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) out.d[i] = out.d[i] + (dl * w[i]) * gr[i];
-                out.cnt += 1.0f;
+        const float vol = SIXTH * dot(cross(c0, c1), c2);
+        const float rest = e.rvol;
+        const bool comp = a.sp.has_compress && (vol < rest);
+        const float k = comp ? cp.kcomp_vol : cp.k_vol;
+        const float alpha = comp ? dc.a_comp : dc.a_vol;
+        const float C = vol - rest;
+        float dsum = 0.f, gamma = 1.f;
+        if (DAMP && cp.damp_vol > 0.0f) {
+            gamma = comp ? dc.g_comp : dc.g_vol;
+            for (int i = 0; i < 4; ++i) {
+                const float4 pp = spp[lv[i]];
+                dsum += dot(gr[i], P[i] - xyz(pp));
             }
+            dsum = dsum * gamma;
+            gamma = gamma + 1.0f;
         }
+        const bool on = (mask & M_VOL) && wsum > 1e-9f && k != 0.0f;
+        float dl = LAM == 2 ? lam.y : (-C - alpha * lam.y - dsum) * __builtin_amdgcn_rcpf(gamma * wsum + alpha);
+        dl = on ? dl : 0.0f;
+        if (LAM == 1 && on) nl.y = lam.y + dl;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) out.d[i] = (dl * w[i]) * gr[i];
+        out.cnt = on ? 1.0f : 0.0f;
     }
 
+    // F is formed for every element when a deformation-gradient type is
+    // compiled in (finite for finite positions, which the selected fibre
+    // update relies on)
     float B[9], F[9];
     constexpr bool NEED_F = (FEAT & (F_ARAP | F_SNH | F_ANISO)) != 0;
-    if (NEED_F && (mask & (M_ARAP | M_SNH | M_ANISO))) {
+    if (NEED_F) {
 #pragma unroll
         for (int q = 0; q < 9; ++q) B[q] = e.B[q];
         const float Ds[9] = {c0.x, c1.x, c2.x, c0.y, c1.y, c2.y, c0.z, c1.z, c2.z};
@@ -413,19 +446,19 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
     }
 
     // ---- ARAP-SVD: solve_tetarap_svd_kernel, solve_kernels.py:1194-1307 ----
+    // (the polar iteration keeps its branches; the update is selected as for
+    // the volume constraint: arap_grad_sel gives zero gradients when C = 0)
     if ((FEAT & F_ARAP) && (mask & M_ARAP)) {
         V3 gr[4];
         float wsum;
-        const float C = arap_grad(F, B, w, gr, wsum);
-        if (C > 0.0f) {
-            if (wsum != 0.0f) {
-                const float dL = LAM == 2 ? lam.x : -(C + dc.at_arap * lam.x) * __builtin_amdgcn_rcpf(wsum + dc.at_arap);
-                if (LAM == 1) nl.x = lam.x + dL;
+        const float C = arap_grad_sel(F, B, w, gr, wsum);
+        const bool on = C > 0.0f && wsum != 0.0f;
+        float dL = LAM == 2 ? lam.x : -(C + dc.at_arap * lam.x) * __builtin_amdgcn_rcpf(wsum + dc.at_arap);
+        dL = on ? dL : 0.0f;
+        if (LAM == 1 && on) nl.x = lam.x + dL;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) out.d[i] = out.d[i] + (dL * w[i]) * gr[i];
-                out.cnt += 1.0f;
-            }
-        }
+        for (int i = 0; i < 4; ++i) out.d[i] = out.d[i] + (dL * w[i]) * gr[i];
+        out.cnt += on ? 1.0f : 0.0f;
     }
 
     // ---- SNH: tet_snh_update_xpbd_fn, solve_kernels.py:1429-1584 ----
@@ -503,38 +536,39 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
     }
 
     // ---- aniso_arap: solve_tet_aniso_arap_kernel, solve_kernels.py:1310-1426 ----
-    if ((FEAT & F_ANISO) && (mask & M_ANISO)) {
+    // (selected like the volume constraint: a skipped fibre gets gs = 0, so
+    // zero gradients, and dL = 0)
+    if (FEAT & F_ANISO) {
         const float4 fb = e.fib;  // f0 (normalised at build), fiberscale
         const V3 f0 = xyz(fb);
         const V3 d = V3{F[0] * f0.x + F[1] * f0.y + F[2] * f0.z, F[3] * f0.x + F[4] * f0.y + F[5] * f0.z,
                         F[6] * f0.x + F[7] * f0.y + F[8] * f0.z};
         const float s = sqrtf(dot(d, d));
         const float detF = det3(F);
-        if (!(s < 1.0e-9f) && !(detF <= 0.0f || s > 1.5f)) {
-            const float diff = s - fb.w;  // SI4 = +1 since det F > 0
-            const float C = 0.5f * diff * diff;
-            const float gs = diff / s;
-            float G[9];
-            const float dv[3] = {d.x, d.y, d.z}, fv[3] = {f0.x, f0.y, f0.z};
+        const bool act = (mask & M_ANISO) && !(s < 1.0e-9f) && !(detF <= 0.0f || s > 1.5f);
+        const float diff = s - fb.w;  // SI4 = +1 since det F > 0
+        const float C = 0.5f * diff * diff;
+        const float gs = act ? diff / s : 0.0f;
+        float G[9];
+        const float dv[3] = {d.x, d.y, d.z}, fv[3] = {f0.x, f0.y, f0.z};
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
+        for (int i = 0; i < 3; ++i)
 #pragma unroll
-                for (int j = 0; j < 3; ++j) G[3 * i + j] = gs * (dv[i] * fv[j]);
-            V3 gr[4];
-            gr[0] = grad_row(G, B, 0);
-            gr[1] = grad_row(G, B, 1);
-            gr[2] = grad_row(G, B, 2);
-            gr[3] = -gr[0] - gr[1] - gr[2];
-            const float wsum = w[0] * dot(gr[0], gr[0]) + w[1] * dot(gr[1], gr[1]) +
-                               w[2] * dot(gr[2], gr[2]) + w[3] * dot(gr[3], gr[3]);
-            if (!(wsum < 1.0e-20f)) {
-                const float dL = LAM == 2 ? lam.z : -(C + dc.at_aniso * lam.z) * __builtin_amdgcn_rcpf(wsum + dc.at_aniso);
-                if (LAM == 1) nl.z = lam.z + dL;
+            for (int j = 0; j < 3; ++j) G[3 * i + j] = gs * (dv[i] * fv[j]);
+        V3 gr[4];
+        gr[0] = grad_row(G, B, 0);
+        gr[1] = grad_row(G, B, 1);
+        gr[2] = grad_row(G, B, 2);
+        gr[3] = -gr[0] - gr[1] - gr[2];
+        const float wsum = w[0] * dot(gr[0], gr[0]) + w[1] * dot(gr[1], gr[1]) +
+                           w[2] * dot(gr[2], gr[2]) + w[3] * dot(gr[3], gr[3]);
+        const bool on = act && !(wsum < 1.0e-20f);
+        float dL = LAM == 2 ? lam.z : -(C + dc.at_aniso * lam.z) * __builtin_amdgcn_rcpf(wsum + dc.at_aniso);
+        dL = on ? dL : 0.0f;
+        if (LAM == 1 && on) nl.z = lam.z + dL;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) out.d[i] = out.d[i] + (dL * w[i]) * gr[i];
-                out.cnt += 1.0f;
-            }
-        }
+        for (int i = 0; i < 4; ++i) out.d[i] = out.d[i] + (dL * w[i]) * gr[i];
+        out.cnt += on ? 1.0f : 0.0f;
     }
     if (LAM == 1) a.lam[g] = nl;
 }
