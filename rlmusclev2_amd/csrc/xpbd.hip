@@ -821,6 +821,18 @@ struct BlockArgs {
     int max_nloc, block_tets;
 };
 
+// XCD-aware block order.  The dispatcher deals workgroups to the eight XCDs of
+// the chip in turn (workgroup i to XCD i mod 8), and each XCD has its own L2.
+// Workgroup i therefore takes the j-th block (j = i / 8) of XCD (i mod 8)'s
+// contiguous range of blocks: neighbouring blocks, which share vertices and
+// the cache lines of their positions, pins and partials, run on one XCD and
+// meet in its L2.  A bijection on [0, n), so the result does not depend on it.
+constexpr uint32_t N_XCD = 8;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t i, uint32_t n) {
+    const uint32_t x = i % N_XCD, j = i / N_XCD, q = n / N_XCD, r = n % N_XCD;
+    return x * q + min(x, r) + j;
+}
+
 // LDS: vertex table (and previous positions when damping), then the corner
 // corrections of all block elements in vertex-major order: each element stores
 // its four corners at their places in the block's local CSR (bcpos), so after
@@ -836,9 +848,10 @@ __global__ __launch_bounds__(512) void k_block_tets(IterArgs a, BlockArgs b, con
     float4 *spp = smem + M;
     float4 *sc = smem + (DAMP ? 2 : 1) * M;
     const int tid = threadIdx.x, nth = blockDim.x;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     // blocks are consecutive runs of TB elements, so the element loads of the
     // one-element-per-thread path are issued before the block record arrives
-    const uint32_t tb0 = blockIdx.x * (uint32_t)TB;
+    const uint32_t tb0 = blk * (uint32_t)TB;
     const int nte = (int)min((int64_t)TB, a.nptet - (int64_t)tb0);
     const bool has_t = nte <= nth && tid < nte;
     const uint32_t g = tb0 + tid;
@@ -849,7 +862,7 @@ __global__ __launch_bounds__(512) void k_block_tets(IterArgs a, BlockArgs b, con
         li = ld_off(b.blidx, g * 8u);
         cp = ld_off(b.bcpos, g * 8u);
     }
-    const BlockDesc D = b.blk[blockIdx.x];
+    const BlockDesc D = b.blk[blk];
     if (D.ntet <= nth && D.nloc <= nth) {
         // one element and at most one local vertex per thread: the element's
         // data and the vertex's tables are loaded before the staging barrier,
@@ -943,7 +956,7 @@ template <int DAMP, int MODE>
 __global__ __launch_bounds__(256) void k_block_shared(IterArgs a, BlockArgs b, const float4 *__restrict__ xin,
                                                       float4 *__restrict__ xout,
                                                       const float4 *__restrict__ pprev, int ns) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int i = (int)xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (i >= ns) return;
     const int v = b.svid[i];
     const int s0 = b.soff[i], e = b.soff[i + 1];
@@ -1047,7 +1060,7 @@ __global__ __launch_bounds__(256) void k_resident(IterArgs a, BlockArgs b, ResAr
     float4 *sc = smem + (DAMP ? 3 : 2) * M;
     DevClass *scls = (DevClass *)(sc + 4 * TB);
     const int tid = threadIdx.x, nth = blockDim.x;
-    const int p = blockIdx.x;
+    const int p = (int)xcd_block(blockIdx.x, gridDim.x);
     const PartDesc P = rs.parts[p];
     if (tid == 0) {
         s_base = __hip_atomic_load(rs.count + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1485,7 +1498,7 @@ __device__ __forceinline__ void integrate_vertex(int i, float4 &x, float4 &pp, f
 
 __global__ void k_integrate(int nv, float4 *X, float4 *PP, float4 *VEL, const float4 *FRC,
                             const uint32_t *flags, const float4 *pin, SimParams sp) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (int)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= nv) return;
     float4 x = X[i], pp = PP[i], v = VEL[i];
     integrate_vertex(i, x, pp, v, FRC, flags, pin, sp);
@@ -1554,7 +1567,7 @@ __global__ void k_post(int nv, const float4 *Xc, float4 *X, float4 *PP, float4 *
                        SdfParams sd, int32_t *cells, SimParams sp, int do_sdf, int do_vel, const float4 *FRC,
                        const uint32_t *flags, const float4 *pin, unsigned long long *dacc, uint32_t *dacc_n,
                        uint32_t *dnc) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (int)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= nv) return;
     float4 x = Xc[i];
     if (dacc) {

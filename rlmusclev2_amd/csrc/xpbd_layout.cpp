@@ -579,6 +579,39 @@ std::string build_layout(const HostInput &in, Layout &L, int *code) {
                          [&](int a, int b) { return ecode[a] < ecode[b]; });
     }
 
+    if (in.flat && in.block_tets >= 64) {
+        // block strategy: vertices are renumbered block by block in element
+        // order, each at the first block that touches it (Morton order among a
+        // block's new vertices; untouched vertices last).  A block's own
+        // vertices then form one contiguous range of its sorted vertex table,
+        // so their staging gathers and finished-position stores are coalesced.
+        // The sums do not depend on vertex numbers: every vertex still adds its
+        // corners in (element, corner) order and its partials in block order.
+        const size_t TB = (size_t)in.block_tets;
+        std::vector<int32_t> nperm;
+        nperm.reserve(nv);
+        std::vector<char> seen(nv, 0);
+        std::vector<int32_t> fresh;
+        for (size_t t0 = 0; t0 < eord.size(); t0 += TB) {
+            fresh.clear();
+            const size_t t1 = std::min(eord.size(), t0 + TB);
+            for (size_t g = t0; g < t1; ++g)
+                for (int k = 0; k < 4; ++k) {
+                    const int v = elems[eord[g]].p[k];
+                    if (!seen[v]) {
+                        seen[v] = 1;
+                        fresh.push_back(v);
+                    }
+                }
+            std::sort(fresh.begin(), fresh.end(), [&](int a, int b) { return L.iperm[a] < L.iperm[b]; });
+            nperm.insert(nperm.end(), fresh.begin(), fresh.end());
+        }
+        for (int i = 0; i < nv; ++i)
+            if (!seen[L.perm[i]]) nperm.push_back(L.perm[i]);
+        L.perm.swap(nperm);
+        for (int i = 0; i < nv; ++i) L.iperm[L.perm[i]] = i;
+    }
+
     if (in.flat) {
         renumber_vertex_records(L, attach_old, nv);
         std::string err = build_flat(in, L, elems, ecls, eord, feature, code);
