@@ -366,6 +366,15 @@ __device__ __forceinline__ void load_elem(const IterArgs &a, uint32_t g, ElemIn<
 // thread per element); 2: warm-start impulse, the step applies dlambda = lambda0
 // from a.lam along each constraint gradient and keeps lambda = lambda0.
 // cls: the material-class table (global, or a copy in LDS)
+// The corner positions (and the class parameters) are read where a constraint
+// uses them, again after the polar iteration, instead of being held in
+// registers across it: ARAP-SVD goes first with only F, B and the polar
+// iterate live, and the volume constraint, which needs the positions and the
+// previous positions, goes last.  A compiler fence before each re-read keeps
+// the compiler from reusing the first read.  (Fewer live registers: more
+// waves per SIMD to hide the vertex-table and LDS latencies.)
+__device__ __forceinline__ void reg_fence() { asm volatile("" ::: "memory"); }
+
 template <int FEAT, int DAMP, int LAM = 0>
 __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const ElemIn<FEAT, LAM> &e,
                                              const float4 *sx, const float4 *spp, const int lv[4], Corr &out,
@@ -374,60 +383,10 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
     float4 nl = lam;
     const uint32_t meta = e.meta;
     const uint32_t mask = (meta >> 16) & 15u;
-    const DevClass dc = ld_off(cls, (meta & 0xffffu) * (uint32_t)sizeof(DevClass));
-    const ClassParams &cp = dc.p;
-    const float4 X0 = sx[lv[0]], X1 = sx[lv[1]], X2 = sx[lv[2]], X3 = sx[lv[3]];
-    const V3 P[4] = {xyz(X0), xyz(X1), xyz(X2), xyz(X3)};
-    const float w[4] = {X0.w, X1.w, X2.w, X3.w};
-    const V3 c0 = P[0] - P[3], c1 = P[1] - P[3], c2 = P[2] - P[3];
+    const uint32_t cls_off = (meta & 0xffffu) * (uint32_t)sizeof(DevClass);
     const float dt = a.sp.dt;
     for (int i = 0; i < 4; ++i) out.d[i] = mk3(0.f, 0.f, 0.f);
     out.cnt = 0.f;
-
-    // the volume constraint goes first: it needs the positions and previous
-    // positions, which are then dead during the polar iteration (lower register
-    // pressure); the corrections of one element are summed per type, so the
-    // order does not change the result
-    // ---- volume: tet_volume_update_xpbd_fn, solve_kernels.py:187-281 ----
-    //
-    // Computed without branches: every quantity is finite for finite positions,
-    // and a constraint that the reference skips (wsum <= 1e-9, stiffness 0, type
-    // absent) gets dl = 0 by a select, so the corrections equal the branched
-    // form's (up to the sign of a zero) while the wave issues no exec-mask
-    // bookkeeping and keeps no zero copies of the accumulators live.
-    if (FEAT & F_VOL) {
-        constexpr float SIXTH = 1.0f / 6.0f;
-        V3 gr[4];
-        gr[0] = SIXTH * cross(c1, c2);
-        gr[1] = SIXTH * cross(c2, c0);
-        gr[2] = SIXTH * cross(c0, c1);
-        gr[3] = -(gr[0] + gr[1] + gr[2]);
-        const float wsum = w[0] * dot(gr[0], gr[0]) + w[1] * dot(gr[1], gr[1]) +
-                           w[2] * dot(gr[2], gr[2]) + w[3] * dot(gr[3], gr[3]);
-        const float vol = SIXTH * dot(cross(c0, c1), c2);
-        const float rest = e.rvol;
-        const bool comp = a.sp.has_compress && (vol < rest);
-        const float k = comp ? cp.kcomp_vol : cp.k_vol;
-        const float alpha = comp ? dc.a_comp : dc.a_vol;
-        const float C = vol - rest;
-        float dsum = 0.f, gamma = 1.f;
-        if (DAMP && cp.damp_vol > 0.0f) {
-            gamma = comp ? dc.g_comp : dc.g_vol;
-            for (int i = 0; i < 4; ++i) {
-                const float4 pp = spp[lv[i]];
-                dsum += dot(gr[i], P[i] - xyz(pp));
-            }
-            dsum = dsum * gamma;
-            gamma = gamma + 1.0f;
-        }
-        const bool on = (mask & M_VOL) && wsum > 1e-9f && k != 0.0f;
-        float dl = LAM == 2 ? lam.y : (-C - alpha * lam.y - dsum) * __builtin_amdgcn_rcpf(gamma * wsum + alpha);
-        dl = on ? dl : 0.0f;
-        if (LAM == 1 && on) nl.y = lam.y + dl;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) out.d[i] = (dl * w[i]) * gr[i];
-        out.cnt = on ? 1.0f : 0.0f;
-    }
 
     // F is formed for every element when a deformation-gradient type is
     // compiled in (finite for finite positions, which the selected fibre
@@ -435,6 +394,8 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
     float B[9], F[9];
     constexpr bool NEED_F = (FEAT & (F_ARAP | F_SNH | F_ANISO)) != 0;
     if (NEED_F) {
+        const float4 X0 = sx[lv[0]], X1 = sx[lv[1]], X2 = sx[lv[2]], X3 = sx[lv[3]];
+        const V3 c0 = xyz(X0) - xyz(X3), c1 = xyz(X1) - xyz(X3), c2 = xyz(X2) - xyz(X3);
 #pragma unroll
         for (int q = 0; q < 9; ++q) B[q] = e.B[q];
         const float Ds[9] = {c0.x, c1.x, c2.x, c0.y, c1.y, c2.y, c0.z, c1.z, c2.z};
@@ -446,23 +407,50 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
     }
 
     // ---- ARAP-SVD: solve_tetarap_svd_kernel, solve_kernels.py:1194-1307 ----
-    // (the polar iteration keeps its branches; the update is selected as for
-    // the volume constraint: arap_grad_sel gives zero gradients when C = 0)
+    // The polar iteration keeps its branches; the update is selected instead of
+    // branched (a satisfied constraint gets C = 0, zero gradients and dL = 0),
+    // so the wave issues no exec-mask bookkeeping around it.
     if ((FEAT & F_ARAP) && (mask & M_ARAP)) {
+        float R[9];
+        closest_rotation(F, R);
+        reg_fence();
+        const float w[4] = {sx[lv[0]].w, sx[lv[1]].w, sx[lv[2]].w, sx[lv[3]].w};
+        const float at_arap = ld_off(cls, cls_off).at_arap;
+        float G[9], c2s = 0.f;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            G[i] = F[i] - R[i];
+            c2s += G[i] * G[i];
+        }
+        const bool sat = c2s < 1.0e-12f;
+        const float C = sat ? 0.0f : __builtin_amdgcn_sqrtf(c2s);
+        const float invC = sat ? 0.0f : __builtin_amdgcn_rcpf(C);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) G[i] *= invC;
         V3 gr[4];
-        float wsum;
-        const float C = arap_grad_sel(F, B, w, gr, wsum);
+        gr[0] = grad_row(G, B, 0);
+        gr[1] = grad_row(G, B, 1);
+        gr[2] = grad_row(G, B, 2);
+        gr[3] = -gr[0] - gr[1] - gr[2];
+        const float wsum =
+            w[0] * dot(gr[0], gr[0]) + w[1] * dot(gr[1], gr[1]) + w[2] * dot(gr[2], gr[2]) + w[3] * dot(gr[3], gr[3]);
         const bool on = C > 0.0f && wsum != 0.0f;
-        float dL = LAM == 2 ? lam.x : -(C + dc.at_arap * lam.x) * __builtin_amdgcn_rcpf(wsum + dc.at_arap);
+        float dL = LAM == 2 ? lam.x : -(C + at_arap * lam.x) * __builtin_amdgcn_rcpf(wsum + at_arap);
         dL = on ? dL : 0.0f;
         if (LAM == 1 && on) nl.x = lam.x + dL;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) out.d[i] = out.d[i] + (dL * w[i]) * gr[i];
-        out.cnt += on ? 1.0f : 0.0f;
+        for (int i = 0; i < 4; ++i) out.d[i] = (dL * w[i]) * gr[i];
+        out.cnt = on ? 1.0f : 0.0f;
     }
 
     // ---- SNH: tet_snh_update_xpbd_fn, solve_kernels.py:1429-1584 ----
     if ((FEAT & F_SNH) && (mask & M_SNH)) {
+        reg_fence();
+        const float4 X0 = sx[lv[0]], X1 = sx[lv[1]], X2 = sx[lv[2]], X3 = sx[lv[3]];
+        const V3 P[4] = {xyz(X0), xyz(X1), xyz(X2), xyz(X3)};
+        const float w[4] = {X0.w, X1.w, X2.w, X3.w};
+        const V3 c0 = P[0] - P[3], c1 = P[1] - P[3], c2 = P[2] - P[3];
+        const ClassParams cp = ld_off(cls, cls_off).p;
         float Ic = 0.f;
 #pragma unroll
         for (int i = 0; i < 9; ++i) Ic += F[i] * F[i];
@@ -539,6 +527,9 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
     // (selected like the volume constraint: a skipped fibre gets gs = 0, so
     // zero gradients, and dL = 0)
     if (FEAT & F_ANISO) {
+        reg_fence();
+        const float w[4] = {sx[lv[0]].w, sx[lv[1]].w, sx[lv[2]].w, sx[lv[3]].w};
+        const float at_aniso = ld_off(cls, cls_off).at_aniso;
         const float4 fb = e.fib;  // f0 (normalised at build), fiberscale
         const V3 f0 = xyz(fb);
         const V3 d = V3{F[0] * f0.x + F[1] * f0.y + F[2] * f0.z, F[3] * f0.x + F[4] * f0.y + F[5] * f0.z,
@@ -563,13 +554,62 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
         const float wsum = w[0] * dot(gr[0], gr[0]) + w[1] * dot(gr[1], gr[1]) +
                            w[2] * dot(gr[2], gr[2]) + w[3] * dot(gr[3], gr[3]);
         const bool on = act && !(wsum < 1.0e-20f);
-        float dL = LAM == 2 ? lam.z : -(C + dc.at_aniso * lam.z) * __builtin_amdgcn_rcpf(wsum + dc.at_aniso);
+        float dL = LAM == 2 ? lam.z : -(C + at_aniso * lam.z) * __builtin_amdgcn_rcpf(wsum + at_aniso);
         dL = on ? dL : 0.0f;
         if (LAM == 1 && on) nl.z = lam.z + dL;
 #pragma unroll
         for (int i = 0; i < 4; ++i) out.d[i] = out.d[i] + (dL * w[i]) * gr[i];
         out.cnt += on ? 1.0f : 0.0f;
     }
+    // ---- volume: tet_volume_update_xpbd_fn, solve_kernels.py:187-281 ----
+    // (last: it needs the positions and the previous positions)
+    //
+    // Computed without branches: every quantity is finite for finite positions,
+    // and a constraint that the reference skips (wsum <= 1e-9, stiffness 0, type
+    // absent) gets dl = 0 by a select, so the corrections equal the branched
+    // form's (up to the sign of a zero) while the wave issues no exec-mask
+    // bookkeeping and keeps no zero copies of the accumulators live.
+    if (FEAT & F_VOL) {
+        reg_fence();
+        const float4 X0 = sx[lv[0]], X1 = sx[lv[1]], X2 = sx[lv[2]], X3 = sx[lv[3]];
+        const V3 P[4] = {xyz(X0), xyz(X1), xyz(X2), xyz(X3)};
+        const float w[4] = {X0.w, X1.w, X2.w, X3.w};
+        const V3 c0 = P[0] - P[3], c1 = P[1] - P[3], c2 = P[2] - P[3];
+        const DevClass dc = ld_off(cls, cls_off);
+        const ClassParams &cp = dc.p;
+        constexpr float SIXTH = 1.0f / 6.0f;
+        V3 gr[4];
+        gr[0] = SIXTH * cross(c1, c2);
+        gr[1] = SIXTH * cross(c2, c0);
+        gr[2] = SIXTH * cross(c0, c1);
+        gr[3] = -(gr[0] + gr[1] + gr[2]);
+        const float wsum = w[0] * dot(gr[0], gr[0]) + w[1] * dot(gr[1], gr[1]) +
+                           w[2] * dot(gr[2], gr[2]) + w[3] * dot(gr[3], gr[3]);
+        const float vol = SIXTH * dot(cross(c0, c1), c2);
+        const float rest = e.rvol;
+        const bool comp = a.sp.has_compress && (vol < rest);
+        const float k = comp ? cp.kcomp_vol : cp.k_vol;
+        const float alpha = comp ? dc.a_comp : dc.a_vol;
+        const float C = vol - rest;
+        float dsum = 0.f, gamma = 1.f;
+        if (DAMP && cp.damp_vol > 0.0f) {
+            gamma = comp ? dc.g_comp : dc.g_vol;
+            for (int i = 0; i < 4; ++i) {
+                const float4 pp = spp[lv[i]];
+                dsum += dot(gr[i], P[i] - xyz(pp));
+            }
+            dsum = dsum * gamma;
+            gamma = gamma + 1.0f;
+        }
+        const bool on = (mask & M_VOL) && wsum > 1e-9f && k != 0.0f;
+        float dl = LAM == 2 ? lam.y : (-C - alpha * lam.y - dsum) * __builtin_amdgcn_rcpf(gamma * wsum + alpha);
+        dl = on ? dl : 0.0f;
+        if (LAM == 1 && on) nl.y = lam.y + dl;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) out.d[i] = out.d[i] + (dl * w[i]) * gr[i];
+        out.cnt += on ? 1.0f : 0.0f;
+    }
+
     if (LAM == 1) a.lam[g] = nl;
 }
 
