@@ -1007,7 +1007,8 @@ __global__ __launch_bounds__(256) void k_block_shared(IterArgs a, BlockArgs b, c
 #pragma unroll
     for (int k = 0; k < 4; ++k) q[k] = k < n ? b.part[s0 + k] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 x = xin[v];
-    const float4 pp = DAMP ? pprev[v] : make_float4(0, 0, 0, 0);
+    // the previous position is read only by the attach update
+    const float4 pp = (DAMP && a.att0) ? pprev[v] : make_float4(0, 0, 0, 0);
     const float4 pv = MODE == MODE_FUSED ? pin_of(a, v) : make_float4(0, 0, 0, 0);
     float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
 #pragma unroll
