@@ -52,20 +52,21 @@ METRIC = "ms/frame @535k tets; batched env-substeps/s; warm-start vs plain @same
 # appended for the resident launch (its bytes per dispatch divided by the iterations it runs); the
 # runs are summarised under profiles/
 TRAFFIC = {
-    (534882, ("tetarapsvd", "volume"), 2, 512, 512): 73468826.0,                    # humanV2-scale, plain
-    (534882, ("tetarapsvd", "volume"), 2, 256, 256): 81038234.0,                    # the same, smaller blocks
-    (534882, ("tetarapsvd", "volume"), 2, 192, 192): 84720845.0,
-    (534882, ("snh", "aniso_arap"), 2, 256, 256): 78355866.0,                       # humanV2-scale, SNH + aniso
-    (534882, ("snh", "aniso_arap"), 2, 192, 192): 81385779.0,
-    (7063998, ("tetarapsvd", "volume", "aniso_arap"), 2, 256, 256): 1005044531.0,  # 512 instances
+    (534882, ("tetarapsvd", "volume"), 2, 256, 256): 57575112.0,                    # humanV2-scale, plain
+    (534882, ("tetarapsvd", "volume"), 2, 192, 192): 58638881.0,                    # the same, other shapes
+    (534882, ("tetarapsvd", "volume"), 2, 384, 384): 56246416.0,
+    (534882, ("tetarapsvd", "volume"), 2, 512, 512): 55199400.0,
+    (534882, ("snh", "aniso_arap"), 2, 256, 256): 63921155.0,                       # humanV2-scale, SNH + aniso
+    (7063998, ("tetarapsvd", "volume", "aniso_arap"), 2, 256, 256): 816899321.0,   # 512 instances
 }
 
 # VALU lane-instructions of one projection iteration: SQ_INSTS_VALU per wave x 64 lanes x waves of the tet
-# kernel plus the shared-vertex kernel (rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU ..., 700 dispatches each;
+# kernel plus the shared-vertex kernel (rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU ..., 800 dispatches each;
 # profiles/projection_valu_sq_counters.md), keyed like TRAFFIC
 VALU_LANE_INSTS = {
-    (534882, ("tetarapsvd", "volume"), 2, 256, 256): 731.3 * 64 * 8360 + 124.7 * 64 * 1488,
-    (7063998, ("tetarapsvd", "volume", "aniso_arap"), 2, 256, 256): 856.3 * 64 * 110376 + 121.4 * 64 * 18556,
+    (534882, ("tetarapsvd", "volume"), 2, 256, 256): 642.0 * 64 * 8360 + 106.1 * 64 * 1488,
+    (534882, ("tetarapsvd", "volume"), 2, 384, 384): 628.9 * 64 * 8358 + 103.4 * 64 * 1372,
+    (7063998, ("tetarapsvd", "volume", "aniso_arap"), 2, 256, 256): 760.1 * 64 * 110376 + 99.8 * 64 * 18556,
 }
 # one FP32 VALU instruction per SIMD every 4 cycles for a 64-lane wave (16 lanes): 256 CUs x 4 SIMDs x 16
 # lanes x 2.4 GHz (packed FP32 is kept out of the build)

@@ -15,7 +15,7 @@ them (fewer waves per SIMD), so only one block per part is weighed: it fits
 meshes up to about 250 k tets, where it is the faster launch.  Each candidate layout is built for the actual mesh,
 100 iterations are captured in one HIP graph and launched between two HIP
 events (as they run inside a frame) from the same deformed state, and the fastest is kept (candidates
-within 2 % of it are ties, which go to the largest block).  Results are cached per mesh signature in
+within 1 % of it are ties, which go to the largest block).  Results are cached per mesh signature in
 ``rlmusclev2_amd/_tune/cache.json``.
 """
 from __future__ import annotations
@@ -58,7 +58,7 @@ def signature(cfg: dict, w) -> str:
     return h.hexdigest()[:16]
 
 
-def choose(times: dict, tie: float = 0.02):
+def choose(times: dict, tie: float = 0.01):
     """Fastest candidate; those within ``tie`` of it count as equal (run-to-run
     noise of the timing is about 1 %) and the tie goes to the largest block
     (fewer vertices shared between workgroups, less memory traffic), then to
