@@ -66,6 +66,8 @@ TRAFFIC = {
 VALU_LANE_INSTS = {
     (534882, ("tetarapsvd", "volume"), 2, 256, 256): 642.0 * 64 * 8360 + 106.1 * 64 * 1488,
     (534882, ("tetarapsvd", "volume"), 2, 384, 384): 628.9 * 64 * 8358 + 103.4 * 64 * 1372,
+    (534882, ("tetarapsvd", "volume"), 2, 192, 192): 653.1 * 64 * 8358 + 108.6 * 64 * 1568,
+    (534882, ("tetarapsvd", "volume"), 2, 512, 512): 619.5 * 64 * 8360 + 101.0 * 64 * 1288,
     (7063998, ("tetarapsvd", "volume", "aniso_arap"), 2, 256, 256): 760.1 * 64 * 110376 + 99.8 * 64 * 18556,
 }
 # one FP32 VALU instruction per SIMD every 4 cycles for a 64-lane wave (16 lanes): 256 CUs x 4 SIMDs x 16
