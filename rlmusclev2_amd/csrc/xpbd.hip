@@ -2450,7 +2450,8 @@ __global__ __launch_bounds__(256) void k_dcd_search(DcdDev d) {
     __syncthreads();
     const int lane = (int)(threadIdx.x & (DCD_LANES - 1));
     const int grp = (int)(threadIdx.x / DCD_LANES);
-    const int64_t q0 = (int64_t)blockIdx.x * NG + grp;
+    // XCD-aware order: neighbouring triangles (bucket order) search on one XCD
+    const int64_t q0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * NG + grp;
     const bool live = q0 < (int64_t)d.bucket_off[(size_t)1 << d.tbits];
     bool stop = false;
     if (live) {
@@ -2849,7 +2850,16 @@ __global__ __launch_bounds__(256) void k_dcd_narrow(DcdDev d, const float4 *X) {
     if (threadIdx.x == 0) s_nc = 0u;
     const int64_t total = (int64_t)min(*d.npairs, (unsigned long long)d.max_pairs);
     const int64_t span = (int64_t)DCD_NR * 256;
-    for (int64_t base = (int64_t)blockIdx.x * span; base < total; base += (int64_t)gridDim.x * span) {
+    // XCD-aware order over the chunks of span pairs: XCD x (the workgroups
+    // i with i mod 8 = x) takes a contiguous range of chunks, so pairs that
+    // the search emitted together (neighbouring triangles) meet in one L2
+    const int64_t nchunk = (total + span - 1) / span;
+    const uint32_t xcd = blockIdx.x % N_XCD, jx = blockIdx.x / N_XCD;
+    const uint32_t gx = gridDim.x / N_XCD + (xcd < gridDim.x % N_XCD ? 1u : 0u);
+    const int64_t cq = nchunk / N_XCD, cr = nchunk % N_XCD;
+    const int64_t c0 = (int64_t)xcd * cq + min((int64_t)xcd, cr), cn = cq + ((int64_t)xcd < cr ? 1 : 0);
+    for (int64_t cc = jx; cc < cn; cc += gx) {
+        const int64_t base = (c0 + cc) * span;
         if (threadIdx.x == 0) s_nq = 0u;
         __syncthreads();
 #pragma unroll
@@ -2882,7 +2892,7 @@ __global__ __launch_bounds__(256) void k_dcd_narrow(DcdDev d, const float4 *X) {
 // apply_kernel (:802-815), averaged; clears the accumulators and the contact
 // counter for the next pass (the count is kept as the last pass's)
 __global__ void k_dcd_apply(DcdDev d, float4 *X, int nv) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (int)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i == 0) {
         d.ncontact[1] = d.ncontact[0];
         d.ncontact[0] = 0u;
