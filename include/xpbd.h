@@ -7,10 +7,11 @@
  * last failure from xpbd_last_error().  No exceptions cross the boundary.
  *
  * All host arrays are in the caller's vertex / tet / constraint order; the
- * library keeps its own device layout (vertices in Morton order, one fused
- * element per tet holding all of its constraint types, elements in Morton
- * order and cut into blocks of consecutive elements for the block strategy,
- * the autotuner's pick) and converts on upload / download.
+ * library keeps its own device layout (one fused element per tet holding all
+ * of its constraint types, elements in Morton order and cut into blocks of
+ * consecutive elements for the block strategy, the autotuner's pick;
+ * vertices in Morton order, renumbered block by block for the block
+ * strategy) and converts on upload / download.
  *
  * Each declaration names the reference interface it replaces.
  */
