@@ -444,6 +444,8 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
     }
 
     // ---- SNH: tet_snh_update_xpbd_fn, solve_kernels.py:1429-1584 ----
+    // (square roots and reciprocals in the hardware's single instructions,
+    // about 1 ulp, instead of the correctly rounded sequences, as in ARAP)
     if ((FEAT & F_SNH) && (mask & M_SNH)) {
         reg_fence();
         const float4 X0 = sx[lv[0]], X1 = sx[lv[1]], X2 = sx[lv[2]], X3 = sx[lv[3]];
@@ -454,10 +456,10 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
         float Ic = 0.f;
 #pragma unroll
         for (int i = 0; i < 9; ++i) Ic += F[i] * F[i];
-        const float r_s = sqrtf(fmaxf(Ic, 1.0e-12f));
+        const float r_s = __builtin_amdgcn_sqrtf(fmaxf(Ic, 1.0e-12f));
         if (!(r_s < 1.0e-6f)) {
             const float dC = r_s - 1.7320508075688772f;
-            const float inv_rs = 1.0f / r_s;
+            const float inv_rs = __builtin_amdgcn_rcpf(r_s);
             V3 dg[4];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
@@ -470,9 +472,9 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
                               w[2] * dot(dg[2], dg[2]) + w[3] * dot(dg[3], dg[3]);
             if (dsw != 0.0f) {
                 const float mu = cp.mu, lamS = cp.lam;
-                const float hC = det3(F) - (1.0f + mu / lamS);
+                const float hC = det3(F) - (1.0f + mu * __builtin_amdgcn_rcpf(lamS));
                 const float restv = e.rsnh;
-                const float inv_detDm = 1.0f / fmaxf(fabsf(restv) * 6.0f, 1.0e-20f);
+                const float inv_detDm = __builtin_amdgcn_rcpf(fmaxf(fabsf(restv) * 6.0f, 1.0e-20f));
                 V3 hg[4];
                 hg[0] = inv_detDm * cross(c1, c2);
                 hg[1] = inv_detDm * cross(c2, c0);
@@ -484,15 +486,15 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
                     const float dh = w[0] * dot(dg[0], hg[0]) + w[1] * dot(dg[1], hg[1]) +
                                      w[2] * dot(dg[2], hg[2]) + w[3] * dot(dg[3], hg[3]);
                     const float abs_vol = fmaxf(fabsf(restv), 1.0e-20f);
-                    const float dalpha = 1.0f / (mu * abs_vol * dt * dt);
-                    const float halpha = 1.0f / (lamS * abs_vol * dt * dt);
+                    const float dalpha = __builtin_amdgcn_rcpf(mu * abs_vol * dt * dt);
+                    const float halpha = __builtin_amdgcn_rcpf(lamS * abs_vol * dt * dt);
                     float ddamp = 0.f, dgam = 1.f, hdamp = 0.f, hgam = 1.f;
                     if (DAMP && cp.damp_snh > 0.0f) {
                         const float kd = cp.damp_snh;
                         const float dbeta = mu * abs_vol * kd * dt * dt;
                         const float hbeta = lamS * abs_vol * kd * dt * dt;
-                        dgam = dalpha * dbeta / dt;
-                        hgam = halpha * hbeta / dt;
+                        dgam = dalpha * dbeta * __builtin_amdgcn_rcpf(dt);
+                        hgam = halpha * hbeta * __builtin_amdgcn_rcpf(dt);
                         for (int i = 0; i < 4; ++i) {
                             const V3 dv = P[i] - xyz(spp[lv[i]]);
                             ddamp += dot(dg[i], dv);
@@ -507,7 +509,7 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
                     const float Ayy = hgam * hsw + halpha;
                     const float detA = Axx * Ayy - dh * dh;
                     if (!(detA < 1.0e-8f)) {
-                        const float inv_det = 1.0f / detA;
+                        const float inv_det = __builtin_amdgcn_rcpf(detA);
                         const float db = -dC - dalpha * lam.x - ddamp;
                         const float hb = -hC - halpha * lam.y - hdamp;
                         const float ddL = LAM == 2 ? lam.x : inv_det * (Ayy * db - dh * hb);
@@ -534,12 +536,12 @@ __device__ __forceinline__ void project_elem(const IterArgs &a, int64_t g, const
         const V3 f0 = xyz(fb);
         const V3 d = V3{F[0] * f0.x + F[1] * f0.y + F[2] * f0.z, F[3] * f0.x + F[4] * f0.y + F[5] * f0.z,
                         F[6] * f0.x + F[7] * f0.y + F[8] * f0.z};
-        const float s = sqrtf(dot(d, d));
+        const float s = __builtin_amdgcn_sqrtf(dot(d, d));
         const float detF = det3(F);
         const bool act = (mask & M_ANISO) && !(s < 1.0e-9f) && !(detF <= 0.0f || s > 1.5f);
         const float diff = s - fb.w;  // SI4 = +1 since det F > 0
         const float C = 0.5f * diff * diff;
-        const float gs = act ? diff / s : 0.0f;
+        const float gs = act ? diff * __builtin_amdgcn_rcpf(s) : 0.0f;
         float G[9];
         const float dv[3] = {d.x, d.y, d.z}, fv[3] = {f0.x, f0.y, f0.z};
 #pragma unroll
