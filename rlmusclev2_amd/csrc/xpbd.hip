@@ -268,32 +268,6 @@ __device__ __forceinline__ float arap_grad(const float *F, const float *B, const
     return C;
 }
 
-// arap_grad without the early return: a satisfied constraint gives C = 0 and
-// zero gradients (wsum = 0), so the caller can select its update instead of
-// branching around it
-__device__ __forceinline__ float arap_grad_sel(const float *F, const float *B, const float w[4], V3 gr[4],
-                                               float &wsum) {
-    float R[9];
-    closest_rotation(F, R);
-    float G[9], c2s = 0.f;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        G[i] = F[i] - R[i];
-        c2s += G[i] * G[i];
-    }
-    const bool sat = c2s < 1.0e-12f;
-    const float C = sat ? 0.0f : __builtin_amdgcn_sqrtf(c2s);
-    const float invC = sat ? 0.0f : __builtin_amdgcn_rcpf(C);
-#pragma unroll
-    for (int i = 0; i < 9; ++i) G[i] *= invC;
-    gr[0] = grad_row(G, B, 0);
-    gr[1] = grad_row(G, B, 1);
-    gr[2] = grad_row(G, B, 2);
-    gr[3] = -gr[0] - gr[1] - gr[2];
-    wsum = w[0] * dot(gr[0], gr[0]) + w[1] * dot(gr[1], gr[1]) + w[2] * dot(gr[2], gr[2]) + w[3] * dot(gr[3], gr[3]);
-    return C;
-}
-
 // ---------------------------------------------------------------------------
 // per-element projection (all constraint types of one tet, Jacobi, lambda = 0)
 // ---------------------------------------------------------------------------
